@@ -1,0 +1,525 @@
+// Bloom filters of FlexPushdownDB's predicate transfer on gfx950:
+//   * key hashing (Arrow 6 HashMultiColumn on fixed-width keys),
+//   * the blocked filter (one 64-bit block per key, 57-bit masks),
+//   * the legacy universal-hash filter,
+//   * order-preserving filter compaction (single pass, decoupled look-back),
+//     used plain (bitmap in) and fused with the bloom probe.
+//
+// Every kernel here streams its key columns once, coalesced: one wavefront
+// covers 64 consecutive rows and the per-row test result becomes one 64-bit
+// bitmap word by a wave ballot, which is exactly the LSB-first layout the
+// reference's FindImp writes (fpdb-tuple/src/arrow/exec/BloomFilter.cpp:138-170).
+#include "fpdb_internal.hpp"
+#include "compact.hpp"
+
+#include <algorithm>
+#include <random>
+#include <vector>
+
+namespace fpdb {
+
+// ---------------------------------------------------------------------------
+// key access modes: 0 = generic column set, 1 = one 8-byte column without
+// nulls, 2 = one 4-byte column without nulls.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ uint32_t key_hash(const ColSet& k, int64_t row) {
+  if constexpr (MODE == 1) {
+    return arrow6_hash_word(reinterpret_cast<const uint64_t*>(k.values[0])[row + k.offset[0]]);
+  } else if constexpr (MODE == 2) {
+    return arrow6_hash_word(reinterpret_cast<const uint32_t*>(k.values[0])[row + k.offset[0]]);
+  } else {
+    return hash_row(k, row);
+  }
+}
+
+static int key_mode(const ColSet& k) {
+  if (k.n == 1 && k.validity[0] == nullptr) {
+    if (k.width[0] == 8) return 1;
+    if (k.width[0] == 4) return 2;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t x, int n) {
+  return (x << n) | (x >> ((-n) & 63));
+}
+
+// BlockedBloomFilter::mask / block_id (BloomFilter.h:190-212) on a 32-bit hash.
+__device__ __forceinline__ uint64_t bb_mask(const uint64_t* s_masks, uint32_t h) {
+  return rotl64(s_masks[h & (kMaskNum - 1)], (h >> kMaskLogNum) & 63);
+}
+__device__ __forceinline__ uint64_t bb_block(uint32_t h, uint64_t nb_mask) {
+  return ((uint64_t)h >> (kMaskLogNum + 6)) & nb_mask;
+}
+
+__device__ __forceinline__ void load_masks_lds(uint64_t* s_masks) {
+  for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// hashing
+// ---------------------------------------------------------------------------
+__global__ void hash_keys_kernel(ColSet keys, int64_t n, uint32_t* __restrict__ out) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride)
+    out[r] = hash_row(keys, r);
+}
+
+// ---------------------------------------------------------------------------
+// blocked filter: insert
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void blocked_insert_kernel(ColSet keys, int64_t n,
+                                                             const uint8_t* __restrict__ select,
+                                                             uint64_t* __restrict__ blocks,
+                                                             uint64_t nb_mask) {
+  __shared__ uint64_t s_masks[kMaskNum];
+  load_masks_lds(s_masks);
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    if (select && !bitmap_bit(select, r)) continue;
+    uint32_t h = key_hash<MODE>(keys, r);
+    __hip_atomic_fetch_or(&blocks[bb_block(h, nb_mask)], bb_mask(s_masks, h), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// blocked filter: find -> packed bitmap
+// Each wave owns 64-row chunks; UNROLL chunks are in flight per wave.
+// ---------------------------------------------------------------------------
+constexpr int kFindUnroll = 4;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void blocked_find_kernel(const uint64_t* __restrict__ blocks,
+                                                           uint64_t nb_mask, ColSet keys, int64_t n,
+                                                           uint8_t* __restrict__ bitmap,
+                                                           int64_t* __restrict__ num_selected) {
+  __shared__ uint64_t s_masks[kMaskNum];
+  load_masks_lds(s_masks);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t nchunks = (n + 63) >> 6;
+  uint64_t count = 0;
+  for (int64_t c0 = wave * kFindUnroll; c0 < nchunks; c0 += nwaves * kFindUnroll) {
+    uint32_t h[kFindUnroll];
+    bool in[kFindUnroll];
+#pragma unroll
+    for (int u = 0; u < kFindUnroll; ++u) {
+      int64_t r = (c0 + u) * 64 + lane;
+      in[u] = r < n;
+      h[u] = in[u] ? key_hash<MODE>(keys, r) : 0;
+    }
+    uint64_t b[kFindUnroll];
+#pragma unroll
+    for (int u = 0; u < kFindUnroll; ++u) b[u] = in[u] ? blocks[bb_block(h[u], nb_mask)] : 0;
+#pragma unroll
+    for (int u = 0; u < kFindUnroll; ++u) {
+      uint64_t m = bb_mask(s_masks, h[u]);
+      uint64_t word = __ballot(in[u] && (b[u] & m) == m);
+      if (c0 + u < nchunks) {
+        write_bitmap_word(bitmap, c0 + u, word, n, lane);
+        count += __popcll(word);
+      }
+    }
+  }
+  if (num_selected && lane == 0 && count)
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(num_selected),
+                           (unsigned long long)count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// legacy filter
+// ---------------------------------------------------------------------------
+
+// Division of uint64 by an invariant divisor (Granlund & Montgomery 1994,
+// "Division by invariant integers using multiplication", Fig. 4.1):
+// q = SRL(t1 + SRL(n - t1, sh1), sh2), t1 = MULUH(mprime, n).
+struct UDiv {
+  uint64_t d;
+  uint64_t mprime;
+  int32_t sh1, sh2;
+};
+
+__host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(a, b);
+#else
+  return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+__host__ __device__ __forceinline__ uint64_t udiv_mod(uint64_t n, const UDiv& u) {
+  uint64_t t1 = mulhi64(u.mprime, n);
+  uint64_t q = (t1 + ((n - t1) >> u.sh1)) >> u.sh2;
+  return n - q * u.d;
+}
+
+static UDiv make_udiv(uint64_t d) {
+  UDiv u;
+  u.d = d;
+  int l = 0;
+  while (l < 64 && (d > (1ULL << l))) ++l;    // l = ceil(log2 d)
+  unsigned __int128 two_l = (unsigned __int128)1 << l;
+  u.mprime = (uint64_t)((((unsigned __int128)1 << 64) * (two_l - d)) / d + 1);
+  u.sh1 = l < 1 ? l : 1;
+  u.sh2 = l - 1 > 0 ? l - 1 : 0;
+  return u;
+}
+
+constexpr int kMaxLegacyK = 24;
+
+struct LegacyParams {
+  int64_t a[kMaxLegacyK];
+  int64_t b[kMaxLegacyK];
+  UDiv p[kMaxLegacyK];
+  UDiv m[kMaxLegacyK];
+  int32_t k;
+};
+
+// hashCombine over std::hash<intN> (identity) of each key column
+// (fpdb-tuple/src/ArrayHasher.cpp:12-19, fpdb-util/src/Util.cpp:106-113).
+// The legacy hasher reads the values buffer without looking at validity
+// (ArrayHasher.cpp:47-53), so nulls hash their stored value.
+__device__ __forceinline__ uint64_t legacy_key(const ColSet& k, int64_t row) {
+  uint64_t seed = (uint64_t)k.n;
+  for (int c = 0; c < k.n; ++c) {
+    uint64_t h = (uint64_t)load_sext(k.values[c], k.width[c], row + k.offset[c]);
+    seed ^= h + 0x9e3779b9ULL + (seed << 6) + (seed >> 2);
+  }
+  return seed;
+}
+
+// UniversalHashFunction::hash (UniversalHashFunction.cpp:51-57).
+__device__ __forceinline__ uint64_t legacy_h(const LegacyParams& P, int j, uint64_t x) {
+  uint64_t sum = (uint64_t)P.a[j] * x + (uint64_t)P.b[j];
+  return udiv_mod(udiv_mod(sum, P.p[j]), P.m[j]);
+}
+
+__global__ __launch_bounds__(256) void legacy_build_kernel(ColSet keys, int64_t n,
+                                                           const uint8_t* __restrict__ select,
+                                                           LegacyParams P,
+                                                           unsigned long long* __restrict__ words) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    if (select && !bitmap_bit(select, r)) continue;
+    uint64_t x = legacy_key(keys, r);
+    for (int j = 0; j < P.k; ++j) {
+      uint64_t h = legacy_h(P, j, x);
+      __hip_atomic_fetch_or(&words[h >> 6], 1ULL << (h & 63), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__device__ __forceinline__ bool legacy_contains(const LegacyParams& P, const uint64_t* words,
+                                                uint64_t x) {
+  for (int j = 0; j < P.k; ++j) {
+    uint64_t h = legacy_h(P, j, x);
+    if (!((words[h >> 6] >> (h & 63)) & 1)) return false;
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void legacy_find_kernel(LegacyParams P, int64_t capacity,
+                                                          const uint64_t* __restrict__ words,
+                                                          ColSet keys, int64_t n,
+                                                          uint8_t* __restrict__ bitmap,
+                                                          int64_t* __restrict__ num_selected) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t nchunks = (n + 63) >> 6;
+  uint64_t count = 0;
+  for (int64_t c = wave; c < nchunks; c += nwaves) {
+    int64_t r = c * 64 + lane;
+    bool sel = false;
+    if (r < n && capacity != 0) sel = legacy_contains(P, words, legacy_key(keys, r));
+    uint64_t word = __ballot(sel);
+    write_bitmap_word(bitmap, c, word, n, lane);
+    count += __popcll(word);
+  }
+  if (num_selected && lane == 0 && count)
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(num_selected),
+                           (unsigned long long)count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// blocked-filter predicate for the fused find + compaction
+template <int MODE>
+struct PredBlocked {
+  const uint64_t* blocks;
+  uint64_t nb_mask;
+  ColSet keys;
+  __device__ __forceinline__ bool operator()(int64_t r, const uint64_t* s_masks) const {
+    uint32_t h = key_hash<MODE>(keys, r);
+    uint64_t m = bb_mask(s_masks, h);
+    return (blocks[bb_block(h, nb_mask)] & m) == m;
+  }
+};
+
+static fpdb_status check_outputs_match(const ColSet& in, const OutColSet& out) {
+  FPDB_CHECK(in.n == out.n, FPDB_ERR_INVALID, "input and output column counts differ");
+  for (int i = 0; i < in.n; ++i)
+    FPDB_CHECK(in.width[i] == out.width[i], FPDB_ERR_INVALID,
+               "output column " + std::to_string(i) + " width differs from input");
+  return FPDB_OK;
+}
+
+}  // namespace fpdb
+
+using namespace fpdb;
+
+extern "C" {
+
+fpdb_status fpdb_bloom_masks(uint8_t out_masks[136]) {
+  FPDB_CHECK(out_masks, FPDB_ERR_INVALID, "NULL output");
+  std::memcpy(out_masks, kBloomMaskBytes, kMaskTotalBytes);
+  return FPDB_OK;
+}
+
+int fpdb_bloom_blocked_log_num_blocks(int64_t num_rows_to_insert) {
+  int64_t desired = std::max<int64_t>(512, num_rows_to_insert * 8);
+  int log_bits = 0;
+  while ((int64_t(1) << log_bits) < desired) ++log_bits;
+  return log_bits - 6;
+}
+
+fpdb_status fpdb_hash_keys(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                           int64_t num_rows, uint32_t* d_hashes) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  FPDB_CHECK(num_keys >= 1, FPDB_ERR_INVALID, "fpdb_hash_keys: no key columns");
+  Context* c = as_ctx(ctx);
+  ColSet k;
+  FPDB_TRY(make_colset(keys, num_keys, num_rows, &k, "fpdb_hash_keys"));
+  if (num_rows == 0) return FPDB_OK;
+  hipLaunchKernelGGL(hash_keys_kernel, dim3(grid_for(c, num_rows, 256)), dim3(256), 0, c->stream, k,
+                     num_rows, d_hashes);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+static fpdb_status blocked_insert_impl(Context* c, const fpdb_column* keys, int num_keys,
+                                       int64_t num_rows, const uint8_t* d_select,
+                                       int log_num_blocks, uint64_t* d_blocks, bool clear) {
+  FPDB_CHECK(num_keys >= 1, FPDB_ERR_INVALID, "bloom build: no key columns");
+  FPDB_CHECK(log_num_blocks >= 0 && log_num_blocks <= 40, FPDB_ERR_INVALID, "bad log_num_blocks");
+  FPDB_CHECK(d_blocks, FPDB_ERR_INVALID, "NULL blocks");
+  ColSet k;
+  FPDB_TRY(make_colset(keys, num_keys, num_rows, &k, "bloom build"));
+  uint64_t nb = 1ULL << log_num_blocks;
+  if (clear) FPDB_TRY(launch_fill(c, d_blocks, 0, nb * 8));
+  if (num_rows == 0) return FPDB_OK;
+  dim3 g(grid_for(c, num_rows, 256)), b(256);
+  ProfScope ps(c, "bloom_blocked_insert");
+  switch (key_mode(k)) {
+    case 1: hipLaunchKernelGGL(blocked_insert_kernel<1>, g, b, 0, c->stream, k, num_rows, d_select, d_blocks, nb - 1); break;
+    case 2: hipLaunchKernelGGL(blocked_insert_kernel<2>, g, b, 0, c->stream, k, num_rows, d_select, d_blocks, nb - 1); break;
+    default: hipLaunchKernelGGL(blocked_insert_kernel<0>, g, b, 0, c->stream, k, num_rows, d_select, d_blocks, nb - 1); break;
+  }
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_bloom_blocked_build(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                                     int64_t num_rows, const uint8_t* d_select,
+                                     int log_num_blocks, uint64_t* d_blocks) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  return blocked_insert_impl(as_ctx(ctx), keys, num_keys, num_rows, d_select, log_num_blocks, d_blocks, true);
+}
+
+fpdb_status fpdb_bloom_blocked_insert(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                                      int64_t num_rows, const uint8_t* d_select,
+                                      int log_num_blocks, uint64_t* d_blocks) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  return blocked_insert_impl(as_ctx(ctx), keys, num_keys, num_rows, d_select, log_num_blocks, d_blocks, false);
+}
+
+fpdb_status fpdb_bloom_blocked_find(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
+                                    const fpdb_column* keys, int num_keys, int64_t num_rows,
+                                    uint8_t* d_bitmap, int64_t* d_num_selected) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  FPDB_CHECK(num_keys >= 1, FPDB_ERR_INVALID, "bloom find: no key columns");
+  FPDB_CHECK(d_blocks && d_bitmap, FPDB_ERR_INVALID, "bloom find: NULL blocks or bitmap");
+  FPDB_CHECK(log_num_blocks >= 0 && log_num_blocks <= 40, FPDB_ERR_INVALID, "bad log_num_blocks");
+  Context* c = as_ctx(ctx);
+  ColSet k;
+  FPDB_TRY(make_colset(keys, num_keys, num_rows, &k, "bloom find"));
+  if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
+  if (num_rows == 0) return FPDB_OK;
+  uint64_t nbm = (1ULL << log_num_blocks) - 1;
+  int64_t nchunks = ceil_div(num_rows, 64);
+  dim3 g(grid_for(c, nchunks * 64, 256 * kFindUnroll, 16)), b(256);
+  ProfScope ps(c, "bloom_blocked_find");
+  switch (key_mode(k)) {
+    case 1: hipLaunchKernelGGL(blocked_find_kernel<1>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, d_num_selected); break;
+    case 2: hipLaunchKernelGGL(blocked_find_kernel<2>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, d_num_selected); break;
+    default: hipLaunchKernelGGL(blocked_find_kernel<0>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, d_num_selected); break;
+  }
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_bloom_legacy_params(int64_t capacity, double p, int64_t* k, int64_t* m) {
+  FPDB_CHECK(k && m, FPDB_ERR_INVALID, "NULL output");
+  FPDB_CHECK(p > 0.0 && p <= 1.0, FPDB_ERR_INVALID, "false positive rate must be in (0, 1]");
+  FPDB_CHECK(capacity >= 0, FPDB_ERR_INVALID, "negative capacity");
+  *k = (int64_t)std::ceil(std::log(1 / p) / std::log(2));
+  *m = (int64_t)std::ceil(((double)capacity) * std::abs(std::log(p)) / std::pow(std::log(2), 2));
+  return FPDB_OK;
+}
+
+static uint64_t mulmod_u64(uint64_t a, uint64_t b, uint64_t m) {
+  return (uint64_t)((unsigned __int128)a * b % m);
+}
+static bool is_prime_u64(uint64_t n) {
+  if (n < 2) return false;
+  static const uint64_t bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (uint64_t q : bases) if (n % q == 0) return n == q;
+  uint64_t d = n - 1;
+  int s = 0;
+  while (!(d & 1)) { d >>= 1; ++s; }
+  for (uint64_t a : bases) {
+    uint64_t x = 1, base = a % n, e = d;
+    while (e) { if (e & 1) x = mulmod_u64(x, base, n); base = mulmod_u64(base, base, n); e >>= 1; }
+    if (x == 1 || x == n - 1) continue;
+    bool composite = true;
+    for (int r = 1; r < s; ++r) {
+      x = mulmod_u64(x, x, n);
+      if (x == n - 1) { composite = false; break; }
+    }
+    if (composite) return false;
+  }
+  return true;
+}
+
+fpdb_status fpdb_bloom_legacy_make_hash_functions(uint32_t seed, int64_t num_bits, int k,
+                                                  fpdb_universal_hash* out) {
+  FPDB_CHECK(out && k >= 0 && k <= kMaxLegacyK, FPDB_ERR_INVALID, "bad arguments");
+  FPDB_CHECK(num_bits > 0, FPDB_ERR_INVALID, "num_bits must be positive");
+  std::mt19937 gen(seed);
+  for (int j = 0; j < k; ++j) {
+    int64_t i = std::uniform_int_distribution<int64_t>(num_bits, 2 * num_bits)(gen);
+    uint64_t p = (uint64_t)i;
+    while (!is_prime_u64(p)) ++p;
+    int64_t a = std::uniform_int_distribution<int64_t>(1, (int64_t)p - 1)(gen);
+    int64_t b = std::uniform_int_distribution<int64_t>(0, (int64_t)p - 1)(gen);
+    out[j] = fpdb_universal_hash{a, b, num_bits, (int64_t)p};
+  }
+  return FPDB_OK;
+}
+
+static fpdb_status make_legacy_params(const fpdb_universal_hash* hf, int k, int64_t num_bits,
+                                      LegacyParams* P) {
+  FPDB_CHECK(k >= 0 && k <= kMaxLegacyK, FPDB_ERR_INVALID,
+             "legacy bloom: k must be in [0, " + std::to_string(kMaxLegacyK) + "]");
+  FPDB_CHECK(k == 0 || hf, FPDB_ERR_INVALID, "legacy bloom: NULL hash functions");
+  std::memset(P, 0, sizeof(*P));
+  P->k = k;
+  for (int j = 0; j < k; ++j) {
+    FPDB_CHECK(hf[j].p > 0 && hf[j].m > 0, FPDB_ERR_INVALID, "legacy bloom: p and m must be positive");
+    FPDB_CHECK(num_bits < 0 || hf[j].m <= num_bits, FPDB_ERR_INVALID,
+               "legacy bloom: hash function m exceeds the bit array");
+    P->a[j] = hf[j].a;
+    P->b[j] = hf[j].b;
+    P->p[j] = make_udiv((uint64_t)hf[j].p);
+    P->m[j] = make_udiv((uint64_t)hf[j].m);
+  }
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_bloom_legacy_build(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                                    int64_t num_rows, const uint8_t* d_select,
+                                    const fpdb_universal_hash* hash_functions, int k,
+                                    int64_t num_bits, int64_t* d_words) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  FPDB_CHECK(num_keys >= 1, FPDB_ERR_INVALID, "legacy bloom: no key columns");
+  FPDB_CHECK(num_bits >= 0 && d_words, FPDB_ERR_INVALID, "legacy bloom: bad bit array");
+  Context* c = as_ctx(ctx);
+  ColSet kc;
+  FPDB_TRY(make_colset(keys, num_keys, num_rows, &kc, "legacy bloom build"));
+  for (int i = 0; i < kc.n; ++i)
+    FPDB_CHECK(type_is_signed_int(kc.type[i]) && (kc.width[i] == 4 || kc.width[i] == 8),
+               FPDB_ERR_INVALID, "legacy bloom: key columns must be int32 or int64");
+  LegacyParams P;
+  FPDB_TRY(make_legacy_params(hash_functions, k, num_bits, &P));
+  FPDB_TRY(launch_fill(c, d_words, 0, (uint64_t)(num_bits / 64 + 1) * 8));
+  if (num_rows == 0 || k == 0) return FPDB_OK;
+  ProfScope ps(c, "bloom_legacy_build");
+  hipLaunchKernelGGL(legacy_build_kernel, dim3(grid_for(c, num_rows, 256)), dim3(256), 0, c->stream, kc,
+                     num_rows, d_select, P, reinterpret_cast<unsigned long long*>(d_words));
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_bloom_legacy_find(fpdb_context* ctx, const fpdb_universal_hash* hash_functions,
+                                   int k, int64_t capacity, int64_t num_bits, const int64_t* d_words,
+                                   const fpdb_column* keys, int num_keys, int64_t num_rows,
+                                   uint8_t* d_bitmap, int64_t* d_num_selected) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  FPDB_CHECK(num_keys >= 1, FPDB_ERR_INVALID, "legacy bloom: no key columns");
+  FPDB_CHECK(d_words && d_bitmap, FPDB_ERR_INVALID, "legacy bloom find: NULL buffer");
+  Context* c = as_ctx(ctx);
+  ColSet kc;
+  FPDB_TRY(make_colset(keys, num_keys, num_rows, &kc, "legacy bloom find"));
+  for (int i = 0; i < kc.n; ++i)
+    FPDB_CHECK(type_is_signed_int(kc.type[i]) && (kc.width[i] == 4 || kc.width[i] == 8),
+               FPDB_ERR_INVALID, "legacy bloom: key columns must be int32 or int64");
+  FPDB_CHECK(num_bits >= 0, FPDB_ERR_INVALID, "legacy bloom find: negative num_bits");
+  LegacyParams P;
+  FPDB_TRY(make_legacy_params(hash_functions, k, num_bits, &P));
+  if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
+  if (num_rows == 0) return FPDB_OK;
+  ProfScope ps(c, "bloom_legacy_find");
+  hipLaunchKernelGGL(legacy_find_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, P,
+                     capacity, reinterpret_cast<const uint64_t*>(d_words), kc, num_rows, d_bitmap,
+                     d_num_selected);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int num_columns,
+                                int64_t num_rows, const uint8_t* d_bitmap,
+                                fpdb_column* out, int64_t* num_out) {
+  FPDB_CHECK(ctx && num_out, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(d_bitmap || num_rows == 0, FPDB_ERR_INVALID, "NULL bitmap");
+  Context* c = as_ctx(ctx);
+  ColSet ic;
+  OutColSet oc;
+  FPDB_TRY(make_colset(in, num_columns, num_rows, &ic, "filter compact"));
+  FPDB_TRY(make_outcolset(out, num_columns, &oc, "filter compact"));
+  FPDB_TRY(check_outputs_match(ic, oc));
+  return run_compact<PredBitmap, false>(c, PredBitmap{d_bitmap}, ic, oc, num_rows, nullptr, nullptr, num_out);
+}
+
+fpdb_status fpdb_bloom_blocked_filter(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
+                                      const fpdb_column* in, int num_columns,
+                                      const int* key_index, int num_keys, int64_t num_rows,
+                                      uint8_t* d_bitmap, fpdb_column* out, int64_t* num_out) {
+  FPDB_CHECK(ctx && num_out && d_blocks, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(num_keys >= 1 && key_index, FPDB_ERR_INVALID, "bloom filter: no key columns");
+  FPDB_CHECK(log_num_blocks >= 0 && log_num_blocks <= 40, FPDB_ERR_INVALID, "bad log_num_blocks");
+  Context* c = as_ctx(ctx);
+  ColSet ic;
+  OutColSet oc;
+  FPDB_TRY(make_colset(in, num_columns, num_rows, &ic, "bloom filter"));
+  FPDB_TRY(make_outcolset(out, num_columns, &oc, "bloom filter"));
+  FPDB_TRY(check_outputs_match(ic, oc));
+  std::vector<fpdb_column> kcols;
+  for (int j = 0; j < num_keys; ++j) {
+    FPDB_CHECK(key_index[j] >= 0 && key_index[j] < num_columns, FPDB_ERR_INVALID, "bad key index");
+    kcols.push_back(in[key_index[j]]);
+  }
+  ColSet kc;
+  FPDB_TRY(make_colset(kcols.data(), num_keys, num_rows, &kc, "bloom filter keys"));
+  uint64_t nbm = (1ULL << log_num_blocks) - 1;
+  switch (key_mode(kc)) {
+    case 1: return run_compact<PredBlocked<1>, true>(c, PredBlocked<1>{d_blocks, nbm, kc}, ic, oc, num_rows, d_bitmap, nullptr, num_out, nullptr, "bloom_blocked_filter");
+    case 2: return run_compact<PredBlocked<2>, true>(c, PredBlocked<2>{d_blocks, nbm, kc}, ic, oc, num_rows, d_bitmap, nullptr, num_out, nullptr, "bloom_blocked_filter");
+    default: return run_compact<PredBlocked<0>, true>(c, PredBlocked<0>{d_blocks, nbm, kc}, ic, oc, num_rows, d_bitmap, nullptr, num_out, nullptr, "bloom_blocked_filter");
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+// Order-preserving stream compaction shared by the bloom-filter use path, the
+// plain filter, and the build-side passes of outer/semi joins.
+//
+// A tile is 4 wavefronts x kCompactChunks chunks of 64 consecutive rows.  Each
+// wave evaluates its predicate for 64 rows at a time and turns the results into
+// one 64-bit word with a ballot (the LSB-first bitmap word of
+// fpdb-tuple/src/arrow/exec/BloomFilter.cpp:138-170).  Tiles are claimed by an
+// atomic ticket and chained by decoupled look-back, so each selected row is
+// written at its rank among all selected rows in one pass over the input.
+#pragma once
+
+#include "fpdb_internal.hpp"
+#include "bloom_masks.hpp"
+
+#include <algorithm>
+
+namespace fpdb {
+
+constexpr int kCompactWaves = 4;
+constexpr int kCompactChunks = 8;
+constexpr int64_t kCompactTileRows = kCompactWaves * kCompactChunks * 64;
+
+__device__ __forceinline__ bool bitmap_bit(const uint8_t* bm, int64_t i) {
+  return (bm[i >> 3] >> (i & 7)) & 1;
+}
+
+// Writes one 64-row bitmap word; a partial last word writes only
+// ceil((n % 64) / 8) bytes (BloomFilter.cpp:167-169).
+__device__ __forceinline__ void write_bitmap_word(uint8_t* bm, int64_t chunk, uint64_t word,
+                                                  int64_t n, int lane) {
+  int64_t row0 = chunk * 64;
+  if (row0 + 64 <= n) {
+    if (lane == 0) reinterpret_cast<uint64_t*>(bm)[chunk] = word;
+  } else if (row0 < n) {
+    int tail_bytes = (int)((n - row0 + 7) / 8);
+    if (lane < tail_bytes) bm[chunk * 8 + lane] = (uint8_t)(word >> (8 * lane));
+  }
+}
+
+// out.values[c][dst] = in.values[c][src]; validity bit OR-ed into a zeroed
+// output bitmap.
+__device__ __forceinline__ void copy_value(const ColSet& in, const OutColSet& out, int c,
+                                           int64_t src, int64_t dst) {
+  int64_t si = src + in.offset[c];
+  uint64_t v = load_zext(in.values[c], in.width[c], si);
+  int64_t di = dst + out.offset[c];
+  store_w(out.values[c], out.width[c], di, v);
+  if (out.validity[c] && valid_bit(in.validity[c], si))
+    __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[c]) + (di >> 5),
+                          1u << (di & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct PredBitmap {
+  const uint8_t* bm;
+  __device__ __forceinline__ bool operator()(int64_t r, const uint64_t*) const {
+    return bitmap_bit(bm, r);
+  }
+};
+
+struct PredByteFlag {
+  const uint8_t* flags;
+  uint8_t want;
+  __device__ __forceinline__ bool operator()(int64_t r, const uint64_t*) const {
+    return (flags[r] != 0) == (want != 0);
+  }
+};
+
+static __constant__ ExpandedMasks c_masks = expand_masks();
+
+// index_out (optional): the source row index of every selected row.
+template <class Pred, bool NEED_MASKS>
+__global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutColSet out, int64_t n,
+                                                      uint8_t* __restrict__ bitmap_out,
+                                                      int64_t* __restrict__ index_out,
+                                                      uint64_t* lb_status,
+                                                      unsigned long long* ticket,
+                                                      uint64_t ticket_base, uint32_t epoch,
+                                                      int64_t num_tiles, int64_t* total,
+                                                      int* err) {
+  __shared__ uint64_t s_masks[NEED_MASKS ? kMaskNum : 1];
+  __shared__ int64_t s_tile;
+  __shared__ uint32_t s_wave_cnt[kCompactWaves];
+  __shared__ uint64_t s_prefix;
+  if (NEED_MASKS) {
+    for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
+  }
+  if (threadIdx.x == 0)
+    s_tile = (int64_t)(__hip_atomic_fetch_add(ticket, 1ULL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) - ticket_base);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave_chunk0 = tile * (kCompactWaves * kCompactChunks) + wave * kCompactChunks;
+
+  uint64_t words[kCompactChunks];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int c = 0; c < kCompactChunks; ++c) {
+    int64_t r = (wave_chunk0 + c) * 64 + lane;
+    bool sel = (r < n) && pred(r, s_masks);
+    words[c] = __ballot(sel);
+    cnt += __popcll(words[c]);
+  }
+  if (bitmap_out) {
+#pragma unroll
+    for (int c = 0; c < kCompactChunks; ++c)
+      write_bitmap_word(bitmap_out, wave_chunk0 + c, words[c], n, lane);
+  }
+  if (lane == 0) s_wave_cnt[wave] = cnt;
+  __syncthreads();
+  if (wave == 0) {
+    uint64_t agg = 0;
+    for (int w = 0; w < kCompactWaves; ++w) agg += s_wave_cnt[w];
+    int e = 0;
+    uint64_t excl = lookback_exclusive(lb_status, tile, epoch, agg, &e);
+    if (lane == 0) {
+      s_prefix = excl;
+      if (e) *err = 1;
+      if (tile == num_tiles - 1) *total = (int64_t)(excl + agg);
+    }
+  }
+  __syncthreads();
+  uint64_t pos = s_prefix;
+  for (int w = 0; w < wave; ++w) pos += s_wave_cnt[w];
+  const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+#pragma unroll
+  for (int c = 0; c < kCompactChunks; ++c) {
+    uint64_t wc = words[c];
+    if ((wc >> lane) & 1) {
+      int64_t src = (wave_chunk0 + c) * 64 + lane;
+      int64_t dst = (int64_t)(pos + __popcll(wc & lt_mask));
+      for (int col = 0; col < in.n; ++col) copy_value(in, out, col, src, dst);
+      if (index_out) index_out[dst] = src;
+    }
+    pos += __popcll(wc);
+  }
+}
+
+inline int grid_for(Context* c, int64_t work_items, int per_block, int blocks_per_cu = 8) {
+  int64_t need = ceil_div(std::max<int64_t>(work_items, 1), per_block);
+  int64_t cap = (int64_t)std::max(c->num_cus, 1) * blocks_per_cu;
+  return (int)std::max<int64_t>(1, std::min(need, cap));
+}
+
+// Zeroes output validity bitmaps (ceil(n/32) words each) that the kernel ORs into.
+inline fpdb_status zero_out_validity(Context* c, const ColSet& in, const OutColSet& out, int64_t n) {
+  for (int i = 0; i < out.n; ++i) {
+    if (out.validity[i]) {
+      FPDB_CHECK((out.offset[i] & 31) == 0, FPDB_ERR_INVALID,
+                 "output validity offset must be a multiple of 32");
+      FPDB_TRY(launch_fill(c, out.validity[i] + out.offset[i] / 8, 0, (uint64_t)((n + 31) / 32) * 4));
+    }
+    if (i < in.n && !out.validity[i] && in.validity[i])
+      return fail(FPDB_ERR_INVALID, "input column " + std::to_string(i) +
+                                        " has a validity bitmap but its output has none");
+  }
+  return FPDB_OK;
+}
+
+// Runs the compaction.  With d_num_out == nullptr the selected count is read
+// back into *num_out (synchronizes); otherwise it is written to d_num_out on
+// the device and the call returns without waiting.
+template <class Pred, bool NEED_MASKS>
+fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const OutColSet& out,
+                        int64_t n, uint8_t* bitmap_out, int64_t* index_out, int64_t* num_out,
+                        int64_t* d_num_out = nullptr, const char* prof_name = "filter_compact") {
+  FPDB_TRY(zero_out_validity(c, in, out, n));
+  if (n == 0) {
+    if (d_num_out) return launch_fill(c, d_num_out, 0, 8);
+    *num_out = 0;
+    return FPDB_OK;
+  }
+  int64_t tiles = ceil_div(n, kCompactTileRows);
+  FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "compaction: too many rows");
+  uint32_t epoch;
+  uint64_t base;
+  FPDB_TRY(lookback_prepare(c, tiles, &epoch, &base));
+  {
+    ProfScope ps(c, prof_name);
+    hipLaunchKernelGGL((compact_kernel<Pred, NEED_MASKS>), dim3((unsigned)tiles), dim3(256), 0,
+                       c->stream, pred, in, out, n, bitmap_out, index_out, c->lb.status, c->lb.ticket,
+                       base, epoch, tiles, d_num_out ? d_num_out : c->d_counter, ctx_err_flag(c));
+  }
+  FPDB_HIP(hipGetLastError());
+  if (d_num_out) return FPDB_OK;
+  return read_count(c, num_out);
+}
+
+}  // namespace fpdb

@@ -1,0 +1,296 @@
+// Internal declarations shared by the HIP translation units of libfpdb_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fpdb_amd.h"
+
+namespace fpdb {
+
+// ---- errors -------------------------------------------------------------
+
+void set_error(const std::string& msg);
+fpdb_status fail(fpdb_status code, const std::string& msg);
+
+#define FPDB_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return ::fpdb::fail(FPDB_ERR_HIP, std::string(#call " failed: ") +        \
+                                             hipGetErrorString(e_));            \
+  } while (0)
+
+#define FPDB_CHECK(cond, code, msg)                                             \
+  do {                                                                          \
+    if (!(cond)) return ::fpdb::fail((code), (msg));                            \
+  } while (0)
+
+#define FPDB_TRY(call)                                                          \
+  do {                                                                          \
+    fpdb_status s_ = (call);                                                    \
+    if (s_ != FPDB_OK) return s_;                                               \
+  } while (0)
+
+// ---- context ---------------------------------------------------------------
+
+// Decoupled look-back scratch: one 64-bit status word per tile plus a ticket
+// counter.  Status words carry a launch epoch so they never need clearing
+// between launches: [epoch:20][flag:2][value:42].
+struct LookbackScratch {
+  uint64_t* status = nullptr;       // device, `tiles` words
+  unsigned long long* ticket = nullptr;  // device, 1 word, monotonically increasing
+  int64_t tiles = 0;
+  uint64_t ticket_base = 0;         // tickets handed out by earlier launches
+  uint32_t epoch = 0;               // epoch of the last launch (1..2^20-1)
+};
+
+struct ProfRecord {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint64_t budget = 0;              // 0 = unlimited
+  uint64_t used = 0;                // bytes of live allocations (rounded sizes)
+  std::unordered_map<void*, uint64_t> allocs;
+  // freed blocks kept for reuse, by rounded size
+  std::unordered_multimap<uint64_t, void*> cache;
+  uint64_t cached = 0;
+  std::mutex mu;
+  // kernel timing: events recorded around named launches while enabled
+  bool profiling = false;
+  std::vector<ProfRecord> prof;
+  std::vector<hipEvent_t> event_pool;
+  int num_cus = 0;
+  int num_xcds = 8;
+  uint64_t hbm_bytes = 0;
+  size_t max_lds_per_block = 64 * 1024;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  LookbackScratch lb;
+  int64_t* d_counter = nullptr;     // small device scratch for counts (8 words)
+  int64_t* h_counter = nullptr;     // pinned host mirror
+};
+
+fpdb_status ctx_alloc(Context* c, uint64_t bytes, void** out);
+fpdb_status ctx_free(Context* c, void* p);
+
+// Brackets one kernel launch with events on the context stream when the
+// context's profiling mode is on.
+hipEvent_t prof_event(Context* c);
+struct ProfScope {
+  Context* c;
+  const char* name;
+  hipEvent_t start = nullptr;
+  ProfScope(Context* ctx, const char* n) : c(ctx), name(n) {
+    if (c->profiling) {
+      start = prof_event(c);
+      if (start) (void)hipEventRecord(start, c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (c->profiling && start) {
+      hipEvent_t stop = prof_event(c);
+      if (stop) {
+        (void)hipEventRecord(stop, c->stream);
+        c->prof.push_back(ProfRecord{name, start, stop});
+      }
+    }
+  }
+};
+// Ensures look-back scratch for `tiles` tiles; returns the epoch and ticket base
+// for the next launch (and advances them).
+fpdb_status lookback_prepare(Context* c, int64_t tiles, uint32_t* epoch, uint64_t* ticket_base);
+fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes);
+// Kernel-side failure flag (look-back spin exhausted, table overflow); zero
+// unless a kernel reported a failure.
+inline int* ctx_err_flag(Context* c) { return reinterpret_cast<int*>(c->d_counter + 1); }
+// Synchronizes, reads the count a kernel left in d_counter[0] and checks (and
+// clears) the failure flag.
+fpdb_status read_count(Context* c, int64_t* count);
+
+inline Context* as_ctx(fpdb_context* c) { return reinterpret_cast<Context*>(c); }
+
+// ---- column helpers ----------------------------------------------------------
+
+int type_width(int32_t type);      // bytes, 0 if unknown
+bool type_is_float(int32_t type);
+bool type_is_signed_int(int32_t type);
+
+constexpr int kMaxCols = 24;
+
+// Kernel-argument view of a set of columns.
+struct ColSet {
+  const uint8_t* values[kMaxCols];
+  const uint8_t* validity[kMaxCols];
+  int64_t offset[kMaxCols];
+  int32_t width[kMaxCols];
+  int32_t type[kMaxCols];
+  int32_t n;
+};
+
+struct OutColSet {
+  uint8_t* values[kMaxCols];
+  uint8_t* validity[kMaxCols];
+  int64_t offset[kMaxCols];
+  int32_t width[kMaxCols];
+  int32_t n;
+};
+
+fpdb_status make_colset(const fpdb_column* cols, int n, int64_t num_rows, ColSet* out,
+                        const char* what);
+fpdb_status make_outcolset(const fpdb_column* cols, int n, OutColSet* out, const char* what);
+
+// ---- device helpers ------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t load_zext(const uint8_t* base, int width, int64_t i) {
+  switch (width) {
+    case 8: return reinterpret_cast<const uint64_t*>(base)[i];
+    case 4: return reinterpret_cast<const uint32_t*>(base)[i];
+    case 2: return reinterpret_cast<const uint16_t*>(base)[i];
+    default: return base[i];
+  }
+}
+
+// Sign-extending load used for join/group keys of signed integer types.
+__device__ __forceinline__ int64_t load_sext(const uint8_t* base, int width, int64_t i) {
+  switch (width) {
+    case 8: return reinterpret_cast<const int64_t*>(base)[i];
+    case 4: return reinterpret_cast<const int32_t*>(base)[i];
+    case 2: return reinterpret_cast<const int16_t*>(base)[i];
+    default: return reinterpret_cast<const int8_t*>(base)[i];
+  }
+}
+
+__device__ __forceinline__ void store_w(uint8_t* base, int width, int64_t i, uint64_t v) {
+  switch (width) {
+    case 8: reinterpret_cast<uint64_t*>(base)[i] = v; break;
+    case 4: reinterpret_cast<uint32_t*>(base)[i] = (uint32_t)v; break;
+    case 2: reinterpret_cast<uint16_t*>(base)[i] = (uint16_t)v; break;
+    default: base[i] = (uint8_t)v; break;
+  }
+}
+
+__device__ __forceinline__ bool valid_bit(const uint8_t* validity, int64_t i) {
+  return validity == nullptr || ((validity[i >> 3] >> (i & 7)) & 1);
+}
+
+// Arrow 6 Hashing::helper_8B for a key of <= 8 bytes, zero-extended:
+// (uint32) bswap64(x * 0xC2B2AE3D27D4EB4F) == bswap32(high 32 bits).
+__host__ __device__ __forceinline__ uint32_t arrow6_hash_word(uint64_t x) {
+  uint64_t p = x * 0xC2B2AE3D27D4EB4FULL;
+  uint32_t hi = (uint32_t)(p >> 32);
+  return (hi >> 24) | ((hi >> 8) & 0xff00u) | ((hi << 8) & 0xff0000u) | (hi << 24);
+}
+
+// Arrow 6 CombineHashesImp.
+__host__ __device__ __forceinline__ uint32_t arrow6_combine(uint32_t prev, uint32_t h) {
+  return prev ^ (h + 0x9e3779b9u + (prev << 6) + (prev >> 2));
+}
+
+__device__ __forceinline__ uint32_t hash_row(const ColSet& k, int64_t row) {
+  uint32_t h = 0;
+  for (int c = 0; c < k.n; ++c) {
+    int64_t i = row + k.offset[c];
+    uint32_t hc = arrow6_hash_word(load_zext(k.values[c], k.width[c], i));
+    if (!valid_bit(k.validity[c], i)) hc = 0;
+    h = (c == 0) ? hc : arrow6_combine(h, hc);
+  }
+  return h;
+}
+
+// 64-bit finaliser (Murmur3 fmix64) used for the join and group tables.
+__host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ULL;
+  k ^= k >> 33;
+  return k;
+}
+
+// Look-back status word packing.
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint32_t flag, uint64_t value) {
+  return ((uint64_t)epoch << 44) | ((uint64_t)flag << 42) | (value & ((1ULL << 42) - 1));
+}
+
+constexpr uint32_t kLbAggregate = 1;
+constexpr uint32_t kLbPrefix = 2;
+constexpr int64_t kLbSpinLimit = int64_t(1) << 28;
+
+// Decoupled look-back executed by one full wavefront (all 64 lanes active).
+// Publishes this tile's aggregate, walks predecessors 64 at a time and returns
+// the exclusive prefix of the tile (identical in every lane).  `err` is set
+// (and 0 returned) if the bounded spin runs out, which only happens if an
+// earlier tile never publishes.
+__device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t tile,
+                                                       uint32_t epoch, uint64_t aggregate,
+                                                       int* err) {
+  const int lane = threadIdx.x & 63;
+  if (tile == 0) {
+    if (lane == 0)
+      __hip_atomic_store(&status[0], lb_pack(epoch, kLbPrefix, aggregate), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&status[tile], lb_pack(epoch, kLbAggregate, aggregate), __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t exclusive = 0;
+  int64_t window_end = tile;   // predecessors [window_end-64, window_end)
+  int64_t spins = 0;
+  while (true) {
+    int64_t idx = window_end - 1 - lane;   // lane 0 = nearest predecessor
+    uint64_t w = 0;
+    uint32_t flag = 0;
+    if (idx >= 0) {
+      w = __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      flag = ((uint32_t)(w >> 44) == epoch) ? (uint32_t)((w >> 42) & 3) : 0;
+    } else {
+      flag = kLbPrefix;   // virtual prefix of zero before tile 0
+    }
+    // Lanes up to (and including) the first prefix lane must all be ready.
+    uint64_t prefix_mask = __ballot(flag == kLbPrefix);
+    uint64_t notready_mask = __ballot(flag == 0);
+    int first_prefix = prefix_mask ? __builtin_ctzll(prefix_mask) : 64;
+    uint64_t upto = (first_prefix >= 63) ? ~0ULL : ((2ULL << first_prefix) - 1);
+    if (notready_mask & upto) {
+      if (++spins > kLbSpinLimit) { *err = 1; return 0; }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint64_t v = (lane <= first_prefix && idx >= 0) ? (w & ((1ULL << 42) - 1)) : 0;
+    // wave reduction
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    exclusive += v;
+    if (first_prefix < 64) break;
+    window_end -= 64;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&status[tile], lb_pack(epoch, kLbPrefix, exclusive + aggregate),
+                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  return exclusive;
+}
+
+// Wave-wide inclusive prefix sum of a 32-bit value.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace fpdb

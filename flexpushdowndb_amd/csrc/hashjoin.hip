@@ -1,0 +1,672 @@
+// Hash join build/probe on gfx950.
+//
+// Table: open addressing with linear probing in HBM, 16-byte slots
+// {int64 key, uint32 build row, uint32 inline payload}, capacity a power of
+// two >= 2x the build rows (load factor <= 0.5).  A probe therefore reads one
+// 16-byte slot (one cache line) in the common case, and with an inline
+// payload the build side needs no second gather.
+//
+// Build: one thread per build row claims a slot with a 32-bit CAS on the row
+// field, then writes key and payload; keys are compared only by later probes.
+//
+// Probe: a tile of 1024 probe rows (256 threads x 4 rows, coalesced) counts
+// its matches, chains its output offset to earlier tiles by decoupled
+// look-back, then writes probe columns and gathered build columns at their
+// final positions -- one pass over the probe columns, output in probe order.
+#include "fpdb_internal.hpp"
+#include "compact.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace fpdb {
+
+struct alignas(16) Slot {
+  int64_t key;
+  uint32_t row;
+  uint32_t payload;
+};
+constexpr uint32_t kEmptyRow = 0xFFFFFFFFu;
+
+// Canonical join key of up to two integer columns.
+struct KeySpec {
+  const uint8_t* v[2];
+  const uint8_t* valid[2];
+  int64_t off[2];
+  int32_t w[2];
+  int32_t sgn[2];
+  int32_t n;
+};
+
+// Returns false for a null key.
+__device__ __forceinline__ bool canon_key(const KeySpec& k, int64_t row, int64_t* out) {
+  if (k.n == 1) {
+    int64_t i = row + k.off[0];
+    if (!valid_bit(k.valid[0], i)) return false;
+    *out = k.sgn[0] ? load_sext(k.v[0], k.w[0], i) : (int64_t)load_zext(k.v[0], k.w[0], i);
+    return true;
+  }
+  int64_t i0 = row + k.off[0], i1 = row + k.off[1];
+  if (!valid_bit(k.valid[0], i0) || !valid_bit(k.valid[1], i1)) return false;
+  uint32_t a = k.sgn[0] ? (uint32_t)load_sext(k.v[0], k.w[0], i0) : (uint32_t)load_zext(k.v[0], k.w[0], i0);
+  uint32_t b = k.sgn[1] ? (uint32_t)load_sext(k.v[1], k.w[1], i1) : (uint32_t)load_zext(k.v[1], k.w[1], i1);
+  *out = (int64_t)((uint64_t)a | ((uint64_t)b << 32));
+  return true;
+}
+
+// KM: 1 = one non-null int64 column, 2 = one non-null int32 column, 0 = generic.
+template <int KM>
+__device__ __forceinline__ bool get_key(const KeySpec& k, int64_t row, int64_t* out) {
+  if constexpr (KM == 1) {
+    *out = reinterpret_cast<const int64_t*>(k.v[0])[row + k.off[0]];
+    return true;
+  } else if constexpr (KM == 2) {
+    *out = reinterpret_cast<const int32_t*>(k.v[0])[row + k.off[0]];
+    return true;
+  } else {
+    return canon_key(k, row, out);
+  }
+}
+
+__device__ __forceinline__ uint64_t slot_of(int64_t key, uint64_t mask) {
+  return fmix64((uint64_t)key) & mask;
+}
+
+__device__ __forceinline__ Slot load_slot(const Slot* slots, uint64_t s) {
+  uint4 v = reinterpret_cast<const uint4*>(slots)[s];
+  Slot r;
+  r.key = (int64_t)((uint64_t)v.x | ((uint64_t)v.y << 32));
+  r.row = v.z;
+  r.payload = v.w;
+  return r;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void hj_build_kernel(Slot* __restrict__ slots, uint64_t mask, KeySpec k,
+                                                       int64_t n, const uint8_t* __restrict__ payload,
+                                                       int64_t payload_off, int* err,
+                                                       unsigned long long* inserted) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long mine = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    int64_t key;
+    if (!get_key<KM>(k, r, &key)) continue;
+    ++mine;
+    uint64_t s = slot_of(key, mask);
+    bool done = false;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+      uint32_t old = atomicCAS(&slots[s].row, kEmptyRow, (uint32_t)r);
+      if (old == kEmptyRow) {
+        slots[s].key = key;
+        slots[s].payload = payload ? reinterpret_cast<const uint32_t*>(payload)[r + payload_off] : 0u;
+        done = true;
+        break;
+      }
+      s = (s + 1) & mask;
+    }
+    if (!done) *err = 1;
+  }
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+  if ((threadIdx.x & 63) == 0 && mine)
+    __hip_atomic_fetch_add(inserted, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Flags a table whose inserted keys are not all distinct.
+__global__ void hj_unique_kernel(const Slot* __restrict__ slots, uint64_t mask, int* dup) {
+  uint64_t cap = mask + 1;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap;
+       s += (uint64_t)gridDim.x * blockDim.x) {
+    Slot me = load_slot(slots, s);
+    if (me.row == kEmptyRow) continue;
+    uint64_t h = slot_of(me.key, mask);
+    while (h != s) {
+      Slot o = load_slot(slots, h);
+      if (o.key == me.key) { *dup = 1; break; }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+// join kinds of the probe kernel
+constexpr int kJtInner = 0;      // pairs
+constexpr int kJtOuter = 1;      // pairs + unmatched probe rows
+constexpr int kJtSemi = 2;       // probe rows with a match
+constexpr int kJtMark = 3;       // no output, flag matched build rows
+
+constexpr int kProbeItems = 4;
+constexpr int kProbeThreads = 256;
+constexpr int kProbeWaves = kProbeThreads / 64;
+constexpr int64_t kProbeTileRows = kProbeItems * kProbeThreads;
+
+// Writes one joined row: probe columns copied from probe row r, build columns
+// gathered from build row brow (or the inline payload).
+__device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bcols, const OutColSet& out,
+                                          int64_t out_cap, int64_t r, int64_t dst, uint32_t brow,
+                                          uint32_t payload) {
+  if (dst >= out_cap) return;
+  for (int c = 0; c < pcols.n; ++c) copy_value(pcols, out, c, r, dst);
+  for (int c = 0; c < bcols.n; ++c) {
+    int oc = pcols.n + c;
+    int64_t di = dst + out.offset[oc];
+    bool valid;
+    if (bcols.values[c] == nullptr) {
+      store_w(out.values[oc], out.width[oc], di, payload);
+      valid = true;
+    } else {
+      int64_t si = (int64_t)brow + bcols.offset[c];
+      store_w(out.values[oc], out.width[oc], di, load_zext(bcols.values[c], bcols.width[c], si));
+      valid = valid_bit(bcols.validity[c], si);
+    }
+    if (out.validity[oc] && valid)
+      __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[oc]) + (di >> 5), 1u << (di & 31),
+                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <int KM, int JT, bool TRACK>
+__global__ __launch_bounds__(256) void hj_probe_kernel(
+    const Slot* __restrict__ slots, uint64_t mask, KeySpec pk, int64_t n, ColSet pcols, ColSet bcols,
+    OutColSet out, int64_t out_cap, uint8_t* __restrict__ matched, uint64_t* lb_status,
+    unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, int64_t num_tiles,
+    int64_t* total, int* err) {
+  __shared__ int64_t s_tile;
+  __shared__ uint32_t s_cnt[kProbeItems][kProbeWaves];
+  __shared__ uint64_t s_prefix;
+  if (threadIdx.x == 0)
+    s_tile = (int64_t)(__hip_atomic_fetch_add(ticket, 1ULL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT) - ticket_base);
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t row0 = tile * kProbeTileRows + threadIdx.x;
+
+  int64_t key[kProbeItems];
+  bool live[kProbeItems];
+  uint64_t home[kProbeItems];
+  Slot first[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    int64_t r = row0 + (int64_t)i * kProbeThreads;
+    live[i] = r < n && get_key<KM>(pk, r, &key[i]);
+    home[i] = live[i] ? slot_of(key[i], mask) : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) first[i] = live[i] ? load_slot(slots, home[i]) : Slot{0, kEmptyRow, 0};
+
+  uint32_t cnt[kProbeItems];       // matches
+  uint64_t fslot[kProbeItems];     // slot of the first match
+  uint32_t frow[kProbeItems], fpay[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    cnt[i] = 0;
+    fslot[i] = 0;
+    frow[i] = 0;
+    fpay[i] = 0;
+    if (!live[i]) continue;
+    Slot sl = first[i];
+    uint64_t s = home[i];
+    while (sl.row != kEmptyRow) {
+      if (sl.key == key[i]) {
+        if (cnt[i] == 0) { fslot[i] = s; frow[i] = sl.row; fpay[i] = sl.payload; }
+        ++cnt[i];
+        if (JT == kJtSemi) break;
+      }
+      s = (s + 1) & mask;
+      sl = load_slot(slots, s);
+    }
+  }
+
+  uint32_t outc[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    int64_t r = row0 + (int64_t)i * kProbeThreads;
+    if (JT == kJtOuter) outc[i] = (r < n) ? (cnt[i] ? cnt[i] : 1u) : 0u;
+    else if (JT == kJtSemi) outc[i] = cnt[i] ? 1u : 0u;
+    else if (JT == kJtMark) outc[i] = 0u;
+    else outc[i] = cnt[i];
+  }
+
+  // offsets within the tile, in row order (item-major, then thread)
+  uint32_t incl[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    incl[i] = wave_incl_scan(outc[i]);
+    if (lane == 63) s_cnt[i][wave] = incl[i];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    uint64_t agg = 0;
+    for (int i = 0; i < kProbeItems; ++i)
+      for (int w = 0; w < kProbeWaves; ++w) agg += s_cnt[i][w];
+    int e = 0;
+    uint64_t excl = lookback_exclusive(lb_status, tile, epoch, agg, &e);
+    if (lane == 0) {
+      s_prefix = excl;
+      if (e) *err = 1;
+      if (tile == num_tiles - 1) *total = (int64_t)(excl + agg);
+    }
+  }
+  __syncthreads();
+
+  uint64_t base = s_prefix;
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    uint64_t before = base;
+    for (int w = 0; w < wave; ++w) before += s_cnt[i][w];
+    uint64_t pos = before + incl[i] - outc[i];
+    for (int w = 0; w < kProbeWaves; ++w) base += s_cnt[i][w];
+    const int64_t r = row0 + (int64_t)i * kProbeThreads;
+
+    if (TRACK && cnt[i]) {
+      // flag every matched build row
+      matched[frow[i]] = 1;
+      uint32_t seen = 1;
+      uint64_t s = (fslot[i] + 1) & mask;
+      while (seen < cnt[i]) {
+        Slot sl = load_slot(slots, s);
+        if (sl.row == kEmptyRow) break;
+        if (sl.key == key[i]) { matched[sl.row] = 1; ++seen; }
+        s = (s + 1) & mask;
+      }
+    }
+    if (JT == kJtMark) continue;
+    if (outc[i] == 0) continue;
+
+    if (cnt[i] == 0 || JT == kJtSemi) {
+      // unmatched probe row (outer) or semi-join row: probe columns only
+      int64_t dst = (int64_t)pos;
+      if (dst < out_cap) {
+        for (int c = 0; c < pcols.n; ++c) copy_value(pcols, out, c, r, dst);
+        if (JT == kJtOuter) {
+          for (int c = 0; c < bcols.n; ++c) {
+            int oc = pcols.n + c;
+            store_w(out.values[oc], out.width[oc], dst + out.offset[oc], 0);
+          }
+        }
+      }
+      continue;
+    }
+    // matched rows: the first match is known, later ones continue the chain
+    int64_t dst = (int64_t)pos;
+    emit_pair(pcols, bcols, out, out_cap, r, dst, frow[i], fpay[i]);
+    uint32_t emitted = 1;
+    uint64_t s = (fslot[i] + 1) & mask;
+    while (emitted < cnt[i]) {
+      Slot sl = load_slot(slots, s);
+      if (sl.row == kEmptyRow) break;
+      if (sl.key == key[i]) {
+        emit_pair(pcols, bcols, out, out_cap, r, dst + emitted, sl.row, sl.payload);
+        ++emitted;
+      }
+      s = (s + 1) & mask;
+    }
+  }
+}
+
+__global__ void gather_kernel(ColSet in, const int64_t* __restrict__ rows, int64_t n, OutColSet out) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    int64_t src = rows[i];
+    for (int c = 0; c < in.n; ++c) {
+      int64_t di = i + out.offset[c];
+      if (src < 0) {
+        store_w(out.values[c], out.width[c], di, 0);
+        continue;
+      }
+      int64_t si = src + in.offset[c];
+      store_w(out.values[c], out.width[c], di, load_zext(in.values[c], in.width[c], si));
+      if (out.validity[c] && valid_bit(in.validity[c], si))
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[c]) + (di >> 5),
+                              1u << (di & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+}  // namespace fpdb
+
+struct fpdb_hash_table {
+  fpdb::Context* ctx = nullptr;
+  fpdb::Slot* slots = nullptr;
+  int64_t capacity = 0;
+  int64_t build_rows = 0;
+  int64_t inserted = 0;
+  int num_keys = 0;
+  int32_t key_types[2] = {0, 0};
+  bool has_payload = false;
+  int32_t payload_type = 0;
+  bool unique = true;
+  bool info_ready = false;
+  uint8_t* matched = nullptr;
+  int64_t* meta = nullptr;    // device: failure flag, duplicate flag, inserted rows
+};
+
+namespace fpdb {
+
+static fpdb_status make_keyspec(const fpdb_column* keys, int num_keys, int64_t num_rows, KeySpec* k,
+                                const char* what) {
+  FPDB_CHECK(num_keys == 1 || num_keys == 2, FPDB_ERR_INVALID,
+             std::string(what) + ": one or two key columns are supported");
+  FPDB_CHECK(keys, FPDB_ERR_INVALID, std::string(what) + ": NULL keys");
+  std::memset(k, 0, sizeof(*k));
+  k->n = num_keys;
+  for (int i = 0; i < num_keys; ++i) {
+    int w = type_width(keys[i].type);
+    FPDB_CHECK(w > 0 && !type_is_float(keys[i].type), FPDB_ERR_INVALID,
+               std::string(what) + ": key columns must be integers");
+    FPDB_CHECK(num_keys == 1 || w <= 4, FPDB_ERR_INVALID,
+               std::string(what) + ": two-column keys need columns of at most 4 bytes");
+    FPDB_CHECK(keys[i].length >= num_rows, FPDB_ERR_INVALID, std::string(what) + ": key column too short");
+    FPDB_CHECK(num_rows == 0 || keys[i].values, FPDB_ERR_INVALID, std::string(what) + ": NULL key values");
+    k->v[i] = static_cast<const uint8_t*>(keys[i].values);
+    k->valid[i] = keys[i].validity;
+    k->off[i] = keys[i].offset;
+    k->w[i] = w;
+    k->sgn[i] = type_is_signed_int(keys[i].type) ? 1 : 0;
+  }
+  return FPDB_OK;
+}
+
+static int key_mode_of(const KeySpec& k) {
+  if (k.n == 1 && k.valid[0] == nullptr && k.sgn[0]) {
+    if (k.w[0] == 8) return 1;
+    if (k.w[0] == 4) return 2;
+  }
+  return 0;
+}
+
+static fpdb_status check_key_compat(const fpdb_hash_table* t, const fpdb_column* keys, int num_keys) {
+  FPDB_CHECK(num_keys == t->num_keys, FPDB_ERR_INVALID, "probe key count differs from the build side");
+  for (int i = 0; i < num_keys; ++i) {
+    bool bs = type_is_signed_int(t->key_types[i]), ps = type_is_signed_int(keys[i].type);
+    FPDB_CHECK(bs == ps, FPDB_ERR_INVALID, "probe and build key signedness differ");
+  }
+  return FPDB_OK;
+}
+
+static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, const fpdb_column* probe_keys,
+                              int num_keys, int64_t n, const fpdb_column* probe_out, int num_probe_out,
+                              const fpdb_column* build_out, int num_build_out, fpdb_column* out,
+                              int64_t out_cap, int64_t* num_out, int64_t* d_num_out = nullptr) {
+  KeySpec pk;
+  FPDB_TRY(make_keyspec(probe_keys, num_keys, n, &pk, "hash join probe"));
+  FPDB_TRY(check_key_compat(t, probe_keys, num_keys));
+  int jt;
+  bool track = false;
+  switch (join_type) {
+    case FPDB_JOIN_INNER: jt = kJtInner; break;
+    case FPDB_JOIN_RIGHT: jt = kJtOuter; break;
+    case FPDB_JOIN_RIGHT_SEMI: jt = kJtSemi; break;
+    case FPDB_JOIN_LEFT: jt = kJtInner; track = true; break;
+    case FPDB_JOIN_FULL: jt = kJtOuter; track = true; break;
+    case FPDB_JOIN_LEFT_SEMI: jt = kJtMark; track = true; break;
+    default: return fail(FPDB_ERR_INVALID, "unknown join type " + std::to_string(join_type));
+  }
+  FPDB_CHECK(!track || t->matched, FPDB_ERR_INVALID,
+             "LEFT, FULL and LEFT_SEMI joins need a table built with track_build_matches");
+  if (jt == kJtSemi || jt == kJtMark) num_build_out = 0;
+  if (jt == kJtMark) num_probe_out = 0;
+  ColSet pc, bc;
+  OutColSet oc;
+  FPDB_TRY(make_colset(probe_out, num_probe_out, n, &pc, "hash join probe columns"));
+  FPDB_CHECK(num_build_out >= 0 && num_build_out + num_probe_out <= kMaxCols, FPDB_ERR_INVALID,
+             "hash join: too many output columns");
+  std::memset(&bc, 0, sizeof(bc));
+  bc.n = num_build_out;
+  for (int i = 0; i < num_build_out; ++i) {
+    int w = type_width(build_out[i].type);
+    FPDB_CHECK(w > 0, FPDB_ERR_INVALID, "hash join: unsupported build column type");
+    if (build_out[i].values == nullptr) {
+      FPDB_CHECK(t->has_payload && w == 4, FPDB_ERR_INVALID,
+                 "hash join: build column without values but the table has no 4-byte inline payload");
+    } else {
+      FPDB_CHECK(build_out[i].length >= t->build_rows, FPDB_ERR_INVALID, "hash join: build column too short");
+    }
+    bc.values[i] = static_cast<const uint8_t*>(build_out[i].values);
+    bc.validity[i] = build_out[i].validity;
+    bc.offset[i] = build_out[i].offset;
+    bc.width[i] = w;
+    bc.type[i] = build_out[i].type;
+  }
+  int nout = (jt == kJtMark) ? 0 : num_probe_out + num_build_out;
+  FPDB_TRY(make_outcolset(out, nout, &oc, "hash join output"));
+  for (int i = 0; i < num_probe_out; ++i)
+    FPDB_CHECK(oc.width[i] == pc.width[i], FPDB_ERR_INVALID, "hash join: output probe column width differs");
+  for (int i = 0; i < num_build_out; ++i) {
+    FPDB_CHECK(oc.width[num_probe_out + i] == bc.width[i], FPDB_ERR_INVALID,
+               "hash join: output build column width differs");
+    FPDB_CHECK(jt != kJtOuter || oc.validity[num_probe_out + i], FPDB_ERR_INVALID,
+               "hash join: outer join build columns need an output validity bitmap");
+    FPDB_CHECK(bc.validity[i] == nullptr || oc.validity[num_probe_out + i], FPDB_ERR_INVALID,
+               "hash join: nullable build column needs an output validity bitmap");
+  }
+  FPDB_CHECK(out_cap >= 0, FPDB_ERR_INVALID, "negative output capacity");
+  {
+    // zero output validity for the rows that may be written
+    ColSet none;
+    std::memset(&none, 0, sizeof(none));
+    for (int i = 0; i < oc.n; ++i) {
+      if (!oc.validity[i]) {
+        FPDB_CHECK(!(i < num_probe_out && pc.validity[i]), FPDB_ERR_INVALID,
+                   "hash join: nullable probe column needs an output validity bitmap");
+        continue;
+      }
+      FPDB_CHECK((oc.offset[i] & 31) == 0, FPDB_ERR_INVALID, "output validity offset must be a multiple of 32");
+      FPDB_TRY(launch_fill(c, oc.validity[i] + oc.offset[i] / 8, 0, (uint64_t)((out_cap + 31) / 32) * 4));
+    }
+  }
+  if (n == 0) {
+    if (d_num_out) return launch_fill(c, d_num_out, 0, 8);
+    *num_out = 0;
+    return FPDB_OK;
+  }
+  int64_t tiles = ceil_div(n, kProbeTileRows);
+  FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "hash join probe: too many rows");
+  uint32_t epoch;
+  uint64_t base;
+  FPDB_TRY(lookback_prepare(c, tiles, &epoch, &base));
+  int* d_err = ctx_err_flag(c);
+  int64_t* total = d_num_out ? d_num_out : c->d_counter;
+  uint64_t mask = (uint64_t)t->capacity - 1;
+  int km = key_mode_of(pk);
+  dim3 g((unsigned)tiles), b(kProbeThreads);
+  ProfScope ps(c, "hj_probe");
+#define FPDB_PROBE_LAUNCH(KM, JT, TR)                                                              \
+  hipLaunchKernelGGL((hj_probe_kernel<KM, JT, TR>), g, b, 0, c->stream, t->slots, mask, pk, n, pc, bc, \
+                     oc, out_cap, t->matched, c->lb.status, c->lb.ticket, base, epoch, tiles,      \
+                     total, d_err)
+#define FPDB_PROBE_JT(KM)                                                    \
+  switch (jt * 2 + (track ? 1 : 0)) {                                        \
+    case kJtInner * 2: FPDB_PROBE_LAUNCH(KM, kJtInner, false); break;        \
+    case kJtInner * 2 + 1: FPDB_PROBE_LAUNCH(KM, kJtInner, true); break;     \
+    case kJtOuter * 2: FPDB_PROBE_LAUNCH(KM, kJtOuter, false); break;        \
+    case kJtOuter * 2 + 1: FPDB_PROBE_LAUNCH(KM, kJtOuter, true); break;     \
+    case kJtSemi * 2: FPDB_PROBE_LAUNCH(KM, kJtSemi, false); break;          \
+    default: FPDB_PROBE_LAUNCH(KM, kJtMark, true); break;                    \
+  }
+  switch (km) {
+    case 1: FPDB_PROBE_JT(1); break;
+    case 2: FPDB_PROBE_JT(2); break;
+    default: FPDB_PROBE_JT(0); break;
+  }
+#undef FPDB_PROBE_JT
+#undef FPDB_PROBE_LAUNCH
+  FPDB_HIP(hipGetLastError());
+  if (d_num_out) return FPDB_OK;
+  FPDB_TRY(read_count(c, num_out));
+  if (*num_out > out_cap)
+    return fail(FPDB_ERR_CAPACITY, "hash join probe: " + std::to_string(*num_out) +
+                                       " output rows exceed the capacity of " + std::to_string(out_cap));
+  return FPDB_OK;
+}
+
+}  // namespace fpdb
+
+using namespace fpdb;
+
+extern "C" {
+
+fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_keys, int64_t num_rows,
+                          const fpdb_column* payload, int track_build_matches, fpdb_hash_table** out) {
+  FPDB_CHECK(ctx && out, FPDB_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  FPDB_CHECK(num_rows >= 0 && num_rows < (int64_t)kEmptyRow, FPDB_ERR_INVALID,
+             "hash join build: at most 2^32-2 build rows");
+  Context* c = as_ctx(ctx);
+  KeySpec k;
+  FPDB_TRY(make_keyspec(keys, num_keys, num_rows, &k, "hash join build"));
+  if (payload) {
+    FPDB_CHECK(type_width(payload->type) == 4, FPDB_ERR_INVALID, "inline payload must be a 4-byte column");
+    FPDB_CHECK(payload->validity == nullptr, FPDB_ERR_INVALID, "inline payload must not have nulls");
+    FPDB_CHECK(payload->length >= num_rows && (num_rows == 0 || payload->values), FPDB_ERR_INVALID,
+               "inline payload column too short");
+  }
+  int64_t cap = 64;
+  while (cap < 2 * num_rows) cap <<= 1;
+  auto* t = new fpdb_hash_table();
+  t->ctx = c;
+  t->capacity = cap;
+  t->build_rows = num_rows;
+  t->num_keys = num_keys;
+  for (int i = 0; i < num_keys; ++i) t->key_types[i] = keys[i].type;
+  t->has_payload = payload != nullptr;
+  t->payload_type = payload ? payload->type : 0;
+  fpdb_status s = ctx_alloc(c, (uint64_t)cap * sizeof(Slot), reinterpret_cast<void**>(&t->slots));
+  if (s != FPDB_OK) { delete t; return s; }
+  if (track_build_matches) {
+    s = ctx_alloc(c, (uint64_t)std::max<int64_t>(num_rows, 1), reinterpret_cast<void**>(&t->matched));
+    if (s == FPDB_OK) s = launch_fill(c, t->matched, 0, (uint64_t)std::max<int64_t>(num_rows, 1));
+    if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
+  }
+  s = ctx_alloc(c, 32, reinterpret_cast<void**>(&t->meta));
+  if (s == FPDB_OK) s = launch_fill(c, t->meta, 0, 32);
+  if (s == FPDB_OK) s = launch_fill(c, t->slots, 0xFF, (uint64_t)cap * sizeof(Slot));
+  if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
+  // meta: [0] insertion failure flag, [1] duplicate-key flag, [2] inserted rows
+  int* d_err = reinterpret_cast<int*>(t->meta);
+  auto* d_ins = reinterpret_cast<unsigned long long*>(t->meta + 2);
+  const uint8_t* pv = payload ? static_cast<const uint8_t*>(payload->values) : nullptr;
+  int64_t poff = payload ? payload->offset : 0;
+  if (num_rows > 0) {
+    dim3 g(grid_for(c, num_rows, 256)), b(256);
+    ProfScope ps(c, "hj_build");
+    switch (key_mode_of(k)) {
+      case 1: hipLaunchKernelGGL(hj_build_kernel<1>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
+      case 2: hipLaunchKernelGGL(hj_build_kernel<2>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
+      default: hipLaunchKernelGGL(hj_build_kernel<0>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { fpdb_hj_table_destroy(t); return fail(FPDB_ERR_HIP, hipGetErrorString(e)); }
+  }
+  *out = t;
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_hj_table_destroy(fpdb_hash_table* t) {
+  if (!t) return FPDB_OK;
+  if (t->slots) ctx_free(t->ctx, t->slots);
+  if (t->matched) ctx_free(t->ctx, t->matched);
+  if (t->meta) ctx_free(t->ctx, t->meta);
+  delete t;
+  return FPDB_OK;
+}
+
+// The build is asynchronous; the first info query runs the duplicate-key
+// check and reads the table's counters back.
+fpdb_status fpdb_hj_table_info(fpdb_hash_table* t, int64_t* build_rows, int64_t* inserted,
+                               int64_t* capacity, int* unique_keys) {
+  FPDB_CHECK(t, FPDB_ERR_INVALID, "NULL table");
+  if (!t->info_ready) {
+    Context* c = t->ctx;
+    if (t->build_rows > 0) {
+      hipLaunchKernelGGL(hj_unique_kernel, dim3(grid_for(c, t->capacity, 256)), dim3(256), 0, c->stream,
+                         t->slots, (uint64_t)t->capacity - 1, reinterpret_cast<int*>(t->meta + 1));
+      FPDB_HIP(hipGetLastError());
+    }
+    int64_t meta[3];
+    FPDB_HIP(hipMemcpyAsync(meta, t->meta, sizeof(meta), hipMemcpyDeviceToHost, c->stream));
+    FPDB_HIP(hipStreamSynchronize(c->stream));
+    FPDB_CHECK((int)meta[0] == 0, FPDB_ERR_INTERNAL, "hash join build: table full");
+    t->unique = (int)meta[1] == 0;
+    t->inserted = meta[2];
+    t->info_ready = true;
+  }
+  if (build_rows) *build_rows = t->build_rows;
+  if (inserted) *inserted = t->inserted;
+  if (capacity) *capacity = t->capacity;
+  if (unique_keys) *unique_keys = t->unique ? 1 : 0;
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_hj_probe_async(fpdb_context* ctx, fpdb_hash_table* table, int join_type,
+                                const fpdb_column* probe_keys, int num_keys, int64_t num_rows,
+                                const fpdb_column* probe_out, int num_probe_out,
+                                const fpdb_column* build_out, int num_build_out,
+                                fpdb_column* out, int64_t out_capacity, int64_t* d_num_out) {
+  FPDB_CHECK(ctx && table && d_num_out, FPDB_ERR_INVALID, "NULL argument");
+  return probe_impl(as_ctx(ctx), table, join_type, probe_keys, num_keys, num_rows, probe_out, num_probe_out,
+                    build_out, num_build_out, out, out_capacity, nullptr, d_num_out);
+}
+
+fpdb_status fpdb_hj_probe(fpdb_context* ctx, fpdb_hash_table* table, int join_type,
+                          const fpdb_column* probe_keys, int num_keys, int64_t num_rows,
+                          const fpdb_column* probe_out, int num_probe_out,
+                          const fpdb_column* build_out, int num_build_out,
+                          fpdb_column* out, int64_t out_capacity, int64_t* num_out) {
+  FPDB_CHECK(ctx && table && num_out, FPDB_ERR_INVALID, "NULL argument");
+  return probe_impl(as_ctx(ctx), table, join_type, probe_keys, num_keys, num_rows, probe_out,
+                    num_probe_out, build_out, num_build_out, out, out_capacity, num_out);
+}
+
+fpdb_status fpdb_hj_probe_count(fpdb_context* ctx, fpdb_hash_table* table, int join_type,
+                                const fpdb_column* probe_keys, int num_keys, int64_t num_rows,
+                                int64_t* count) {
+  FPDB_CHECK(ctx && table && count, FPDB_ERR_INVALID, "NULL argument");
+  if (join_type == FPDB_JOIN_LEFT) join_type = FPDB_JOIN_INNER;
+  if (join_type == FPDB_JOIN_FULL) join_type = FPDB_JOIN_RIGHT;
+  if (join_type == FPDB_JOIN_LEFT_SEMI) { *count = 0; return FPDB_OK; }
+  fpdb_status s = probe_impl(as_ctx(ctx), table, join_type, probe_keys, num_keys, num_rows, nullptr, 0,
+                             nullptr, 0, nullptr, 0, count);
+  if (s == FPDB_ERR_CAPACITY) return FPDB_OK;
+  return s;
+}
+
+fpdb_status fpdb_hj_build_side_rows(fpdb_context* ctx, fpdb_hash_table* t, int matched,
+                                    int64_t* d_rows, int64_t capacity, int64_t* num_out) {
+  FPDB_CHECK(ctx && t && num_out, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(t->matched, FPDB_ERR_INVALID, "table was built without track_build_matches");
+  FPDB_CHECK(capacity >= t->build_rows && (d_rows || t->build_rows == 0), FPDB_ERR_INVALID,
+             "build side rows: capacity must be at least the number of build rows");
+  Context* c = as_ctx(ctx);
+  ColSet none;
+  OutColSet onone;
+  std::memset(&none, 0, sizeof(none));
+  std::memset(&onone, 0, sizeof(onone));
+  return run_compact<PredByteFlag, false>(c, PredByteFlag{t->matched, (uint8_t)(matched ? 1 : 0)}, none,
+                                          onone, t->build_rows, nullptr, d_rows, num_out);
+}
+
+fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns, const int64_t* d_rows,
+                        int64_t num_rows, fpdb_column* out) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  FPDB_CHECK(d_rows || num_rows == 0, FPDB_ERR_INVALID, "NULL rows");
+  Context* c = as_ctx(ctx);
+  ColSet ic;
+  OutColSet oc;
+  FPDB_TRY(make_colset(in, num_columns, 0, &ic, "gather"));
+  FPDB_TRY(make_outcolset(out, num_columns, &oc, "gather"));
+  for (int i = 0; i < num_columns; ++i)
+    FPDB_CHECK(ic.width[i] == oc.width[i], FPDB_ERR_INVALID, "gather: output width differs");
+  for (int i = 0; i < oc.n; ++i) {
+    if (!oc.validity[i]) continue;
+    FPDB_CHECK((oc.offset[i] & 31) == 0, FPDB_ERR_INVALID, "output validity offset must be a multiple of 32");
+    FPDB_TRY(launch_fill(c, oc.validity[i] + oc.offset[i] / 8, 0, (uint64_t)((num_rows + 31) / 32) * 4));
+  }
+  if (num_rows == 0) return FPDB_OK;
+  hipLaunchKernelGGL(gather_kernel, dim3(grid_for(c, num_rows, 256)), dim3(256), 0, c->stream, ic, d_rows,
+                     num_rows, oc);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+}  // extern "C"

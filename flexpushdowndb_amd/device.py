@@ -1,0 +1,167 @@
+"""Host-side handles over the C ABI: a device context, device arrays and
+Arrow-layout device columns.  All device memory is owned by the library's
+context (with its HBM budget); numpy arrays are the host representation."""
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+
+
+class Context:
+    """One fpdb_context (device, stream, HBM budget)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self._lib = C.lib()
+        h = ctypes.c_void_p()
+        C.check(self._lib.fpdb_context_create(device, hbm_budget_bytes, None, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self._lib.fpdb_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- memory ---
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        C.check(self._lib.fpdb_device_alloc(self.handle, int(nbytes), ctypes.byref(p)))
+        return p.value
+
+    def free(self, ptr):
+        if ptr:
+            C.check(self._lib.fpdb_device_free(self.handle, ptr))
+
+    def memory(self):
+        used, budget = ctypes.c_uint64(), ctypes.c_uint64()
+        C.check(self._lib.fpdb_context_memory(self.handle, ctypes.byref(used), ctypes.byref(budget)))
+        return used.value, budget.value
+
+    def device_info(self):
+        cus, xcds, hbm = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        C.check(self._lib.fpdb_device_info(self.handle, ctypes.byref(cus), ctypes.byref(xcds), ctypes.byref(hbm)))
+        return {"num_cus": cus.value, "num_xcds": xcds.value, "hbm_bytes": hbm.value}
+
+    def synchronize(self):
+        C.check(self._lib.fpdb_context_synchronize(self.handle))
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host)
+        d = DeviceArray(self, max(1, host.nbytes), host.dtype, len(host) if host.ndim else 1)
+        if host.nbytes:
+            C.check(self._lib.fpdb_copy_to_device(self.handle, d.ptr, host.ctypes.data, host.nbytes))
+        return d
+
+    def empty(self, n, dtype):
+        dtype = np.dtype(dtype)
+        return DeviceArray(self, max(1, int(n) * dtype.itemsize), dtype, int(n))
+
+    def zeros(self, n, dtype):
+        d = self.empty(n, dtype)
+        self.fill(d.ptr, 0, d.nbytes)
+        return d
+
+    def fill(self, ptr, value, nbytes):
+        C.check(self._lib.fpdb_fill(self.handle, ptr, value, int(nbytes)))
+
+    def download(self, ptr, n, dtype):
+        dtype = np.dtype(dtype)
+        out = np.empty(int(n), dtype)
+        if out.nbytes:
+            C.check(self._lib.fpdb_copy_to_host(self.handle, out.ctypes.data, ptr, out.nbytes))
+        return out
+
+    # --- timing on the context stream ---
+    def timer_start(self):
+        C.check(self._lib.fpdb_timer_start(self.handle))
+
+    def timer_stop(self):
+        ms = ctypes.c_float()
+        C.check(self._lib.fpdb_timer_stop(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+
+class DeviceArray:
+    """A device buffer owned by a Context."""
+
+    def __init__(self, ctx, nbytes, dtype, length):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.length = int(length)
+        self.ptr = ctx.alloc(self.nbytes)
+
+    def numpy(self, n=None):
+        return self.ctx.download(self.ptr, self.length if n is None else n, self.dtype)
+
+    def free(self):
+        if self.ptr and self.ctx.handle:
+            self.ctx.free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceColumn:
+    """An Arrow-layout column on the device (values + optional validity)."""
+
+    def __init__(self, values, validity=None, type_id=None, length=None, offset=0):
+        self.values = values
+        self.validity = validity
+        self.type_id = type_id if type_id is not None else C.NP_TO_TYPE[values.dtype]
+        self.length = values.length if length is None else int(length)
+        self.offset = int(offset)
+
+    @classmethod
+    def from_numpy(cls, ctx, arr, valid=None, type_id=None):
+        """valid: optional boolean mask (True = valid), packed LSB-first."""
+        vals = ctx.upload(arr)
+        vb = None
+        if valid is not None:
+            bits = np.packbits(np.asarray(valid, dtype=bool), bitorder="little")
+            pad = np.zeros(((len(arr) + 31) // 32) * 4, np.uint8)
+            pad[:len(bits)] = bits
+            vb = ctx.upload(pad)
+        return cls(vals, vb, type_id, len(arr))
+
+    @classmethod
+    def empty(cls, ctx, n, dtype, nullable=False, type_id=None):
+        vals = ctx.empty(max(1, n), dtype)
+        vb = ctx.zeros(((max(1, n) + 31) // 32) * 4, np.uint8) if nullable else None
+        return cls(vals, vb, type_id, n)
+
+    @property
+    def dtype(self):
+        return C.TYPE_TO_NP[self.type_id]
+
+    def c(self, length=None):
+        return C.Column(self.type_id, 0, self.length if length is None else int(length), self.offset,
+                        self.values.ptr, self.validity.ptr if self.validity is not None else None)
+
+    def numpy(self, n=None):
+        n = self.length if n is None else int(n)
+        return self.values.ctx.download(self.values.ptr + self.offset * self.dtype.itemsize, n, self.dtype)
+
+    def valid_mask(self, n=None):
+        n = self.length if n is None else int(n)
+        if self.validity is None:
+            return np.ones(n, bool)
+        nbytes = (n + 7) // 8
+        bits = self.values.ctx.download(self.validity.ptr, nbytes, np.uint8)
+        return np.unpackbits(bits, bitorder="little")[:n].astype(bool)

@@ -1,0 +1,279 @@
+"""Kernel-level operators over the C ABI (device columns in, device columns out).
+
+These are the seams the reference's kernels expose
+(HashJoinArrowKernel, BloomFilterCreate*Kernel / BloomFilterUseKernel,
+GroupArrowKernel; fpdb-executor/src/physical/...), driven from Python for tests
+and benchmarks.  The C++ operator layer (flexpushdowndb_amd/pop) drives the same
+C ABI from the reference's PhysicalOp interface.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+from .device import DeviceColumn
+
+
+def _cols(cols, n=None):
+    return C.columns_array([c.c(n) for c in cols])
+
+
+def hash_keys(ctx, key_cols, n):
+    """Arrow 6 HashMultiColumn hashes (RecordBatchHasher.cpp:69-104)."""
+    out = ctx.empty(max(1, n), np.uint32)
+    C.check(C.lib().fpdb_hash_keys(ctx.handle, _cols(key_cols), len(key_cols), n, out.ptr))
+    return out
+
+
+def bloom_masks():
+    out = np.zeros(136, np.uint8)
+    C.check(C.lib().fpdb_bloom_masks(out.ctypes.data))
+    return out
+
+
+class BlockedBloomFilter:
+    """The default (Arrow) blocked bloom filter, device-resident
+    (fpdb-tuple/src/arrow/exec/BloomFilter.cpp, ArrowBloomFilter.cpp)."""
+
+    MAX_INPUT_SIZE = 20_000_000   # BLOOM_FILTER_MAX_INPUT_SIZE, physical/Globals.h:55
+
+    def __init__(self, ctx, capacity):
+        self.ctx = ctx
+        self.capacity = int(capacity)
+        self.valid = self.capacity <= self.MAX_INPUT_SIZE
+        self.log_num_blocks = int(C.lib().fpdb_bloom_blocked_log_num_blocks(self.capacity))
+        self.blocks = ctx.zeros(1 << self.log_num_blocks, np.uint64) if self.valid else None
+
+    @classmethod
+    def build(cls, ctx, key_cols, n, select=None, capacity=None):
+        f = cls(ctx, n if capacity is None else capacity)
+        if f.valid:
+            C.check(C.lib().fpdb_bloom_blocked_build(ctx.handle, _cols(key_cols), len(key_cols), n,
+                                                     select.ptr if select is not None else None,
+                                                     f.log_num_blocks, f.blocks.ptr))
+        return f
+
+    def insert(self, key_cols, n, select=None):
+        C.check(C.lib().fpdb_bloom_blocked_insert(self.ctx.handle, _cols(key_cols), len(key_cols), n,
+                                                  select.ptr if select is not None else None,
+                                                  self.log_num_blocks, self.blocks.ptr))
+
+    def find(self, key_cols, n):
+        """Selection bitmap (device, (n+7)/8 bytes) and selected count."""
+        bm = self.ctx.zeros(max(8, ((n + 63) // 64) * 8), np.uint8)
+        cnt = self.ctx.zeros(1, np.int64)
+        C.check(C.lib().fpdb_bloom_blocked_find(self.ctx.handle, self.blocks.ptr, self.log_num_blocks,
+                                                _cols(key_cols), len(key_cols), n, bm.ptr, cnt.ptr))
+        return bm, int(cnt.numpy()[0])
+
+    def filter(self, cols, key_index, n, want_bitmap=True):
+        """Fused Find + compaction of all columns (BloomFilterUseKernel.cpp:180-207)."""
+        out = [DeviceColumn.empty(self.ctx, n, c.dtype, c.validity is not None, c.type_id) for c in cols]
+        bm = self.ctx.zeros(max(8, ((n + 63) // 64) * 8), np.uint8) if want_bitmap else None
+        ki = (ctypes.c_int * len(key_index))(*key_index)
+        nout = ctypes.c_int64()
+        C.check(C.lib().fpdb_bloom_blocked_filter(self.ctx.handle, self.blocks.ptr, self.log_num_blocks,
+                                                  _cols(cols), len(cols), ki, len(key_index), n,
+                                                  bm.ptr if bm is not None else None,
+                                                  C.columns_array([o.c() for o in out]), ctypes.byref(nout)))
+        for o in out:
+            o.length = nout.value
+        return out, nout.value, bm
+
+    def blocks_numpy(self):
+        return self.blocks.numpy()
+
+
+class LegacyBloomFilter:
+    """The in-tree universal-hash bloom filter with a target false-positive
+    rate (fpdb-executor/src/physical/bloomfilter/BloomFilter.cpp)."""
+
+    MAX_INPUT_SIZE = 20_000_000
+
+    def __init__(self, ctx, capacity, fpr, hash_functions=None, seed=0):
+        self.ctx = ctx
+        self.capacity = int(capacity)
+        self.fpr = float(fpr)
+        self.valid = self.capacity <= self.MAX_INPUT_SIZE
+        k, m = ctypes.c_int64(), ctypes.c_int64()
+        C.check(C.lib().fpdb_bloom_legacy_params(self.capacity, self.fpr, ctypes.byref(k), ctypes.byref(m)))
+        self.num_bits = m.value
+        if self.capacity == 0:
+            self.k = 0      # BloomFilter::makeHashFunctions, BloomFilter.cpp:209-213
+        else:
+            self.k = k.value
+        if hash_functions is None:
+            hf = (C.UniversalHash * max(1, self.k))()
+            if self.k:
+                C.check(C.lib().fpdb_bloom_legacy_make_hash_functions(seed, self.num_bits, self.k, hf))
+            self.hash_functions = np.array([[h.a, h.b, h.m, h.p] for h in hf[:self.k]], np.int64).reshape(-1, 4)
+        else:
+            self.hash_functions = np.asarray(hash_functions, np.int64).reshape(-1, 4)
+            self.k = len(self.hash_functions)
+        self.words = ctx.zeros(self.num_bits // 64 + 1, np.int64)
+
+    def _hf(self):
+        hf = (C.UniversalHash * max(1, self.k))()
+        for i, (a, b, m, p) in enumerate(self.hash_functions):
+            hf[i] = C.UniversalHash(int(a), int(b), int(m), int(p))
+        return hf
+
+    def build(self, key_cols, n, select=None):
+        C.check(C.lib().fpdb_bloom_legacy_build(self.ctx.handle, _cols(key_cols), len(key_cols), n,
+                                                select.ptr if select is not None else None, self._hf(),
+                                                self.k, self.num_bits, self.words.ptr))
+        return self
+
+    def find(self, key_cols, n):
+        bm = self.ctx.zeros(max(8, ((n + 63) // 64) * 8), np.uint8)
+        cnt = self.ctx.zeros(1, np.int64)
+        C.check(C.lib().fpdb_bloom_legacy_find(self.ctx.handle, self._hf(), self.k, self.capacity,
+                                               self.num_bits, self.words.ptr, _cols(key_cols), len(key_cols), n, bm.ptr,
+                                               cnt.ptr))
+        return bm, int(cnt.numpy()[0])
+
+
+def filter_compact(ctx, cols, n, bitmap, capacity=None):
+    cap = n if capacity is None else capacity
+    out = [DeviceColumn.empty(ctx, max(cap, n) if c.validity is not None else cap, c.dtype,
+                              c.validity is not None, c.type_id) for c in cols]
+    nout = ctypes.c_int64()
+    C.check(C.lib().fpdb_filter_compact(ctx.handle, _cols(cols), len(cols), n, bitmap.ptr,
+                                        C.columns_array([o.c() for o in out]), ctypes.byref(nout)))
+    for o in out:
+        o.length = nout.value
+    return out, nout.value
+
+
+class HashJoinTable:
+    """Device hash table of the build side (HashJoinArrowKernel build phase)."""
+
+    def __init__(self, ctx, key_cols, n, payload=None, track_build_matches=False):
+        self.ctx = ctx
+        self.handle = ctypes.c_void_p()
+        C.check(C.lib().fpdb_hj_build(ctx.handle, _cols(key_cols), len(key_cols), n,
+                                      ctypes.byref(payload.c()) if payload is not None else None,
+                                      1 if track_build_matches else 0, ctypes.byref(self.handle)))
+        self.payload_type = payload.type_id if payload is not None else None
+
+    def info(self):
+        b, i, cap, u = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        C.check(C.lib().fpdb_hj_table_info(self.handle, ctypes.byref(b), ctypes.byref(i), ctypes.byref(cap),
+                                           ctypes.byref(u)))
+        return {"build_rows": b.value, "inserted": i.value, "capacity": cap.value, "unique": bool(u.value)}
+
+    def close(self):
+        if self.handle:
+            C.lib().fpdb_hj_table_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def probe_count(self, join_type, key_cols, n):
+        cnt = ctypes.c_int64()
+        C.check(C.lib().fpdb_hj_probe_count(self.ctx.handle, self.handle, join_type, _cols(key_cols),
+                                            len(key_cols), n, ctypes.byref(cnt)))
+        return cnt.value
+
+    def probe(self, join_type, key_cols, n, probe_out, build_out, capacity=None, out=None):
+        """build_out entries: DeviceColumn, or the string 'payload' for the inline payload."""
+        outer_build = join_type in (C.JOIN_RIGHT, C.JOIN_FULL)
+        if join_type in (C.JOIN_RIGHT_SEMI, C.JOIN_LEFT_SEMI):
+            build_out = []
+        if join_type == C.JOIN_LEFT_SEMI:
+            probe_out = []
+        if capacity is None:
+            capacity = self.probe_count(join_type, key_cols, n)
+        bcols = []
+        for b in build_out:
+            if isinstance(b, str):
+                bcols.append(C.Column(self.payload_type, 0, 0, 0, None, None))
+            else:
+                bcols.append(b.c())
+        if out is None:
+            out = []
+            for p in probe_out:
+                out.append(DeviceColumn.empty(self.ctx, capacity, p.dtype, p.validity is not None, p.type_id))
+            for b in build_out:
+                if isinstance(b, str):
+                    out.append(DeviceColumn.empty(self.ctx, capacity, C.TYPE_TO_NP[self.payload_type],
+                                                  outer_build, self.payload_type))
+                else:
+                    out.append(DeviceColumn.empty(self.ctx, capacity, b.dtype,
+                                                  outer_build or b.validity is not None, b.type_id))
+        nout = ctypes.c_int64()
+        C.check(C.lib().fpdb_hj_probe(self.ctx.handle, self.handle, join_type, _cols(key_cols), len(key_cols), n,
+                                      _cols(probe_out), len(probe_out), C.columns_array(bcols), len(bcols),
+                                      C.columns_array([o.c() for o in out]), capacity, ctypes.byref(nout)))
+        for o in out:
+            o.length = nout.value
+        return out, nout.value
+
+    def build_side_rows(self, matched):
+        b = self.info()["build_rows"]
+        rows = self.ctx.empty(max(1, b), np.int64)
+        nout = ctypes.c_int64()
+        C.check(C.lib().fpdb_hj_build_side_rows(self.ctx.handle, self.handle, 1 if matched else 0, rows.ptr,
+                                                max(1, b), ctypes.byref(nout)))
+        return rows, nout.value
+
+
+def gather(ctx, cols, rows, n):
+    out = [DeviceColumn.empty(ctx, n, c.dtype, True, c.type_id) for c in cols]
+    C.check(C.lib().fpdb_gather(ctx.handle, _cols(cols, 0), len(cols), rows.ptr, n,
+                                C.columns_array([o.c() for o in out])))
+    return out
+
+
+class GroupBy:
+    """Accumulating hash group-by (GroupArrowKernel's aggregate node)."""
+
+    def __init__(self, ctx, key_types, aggs, max_groups):
+        """aggs: list of (kind, input_type, nullable)."""
+        self.ctx = ctx
+        self.key_types = list(key_types)
+        self.specs = [C.AggSpec(k, t if t is not None else 0, 1 if nl else 0, 0) for k, t, nl in aggs]
+        kt = (ctypes.c_int32 * len(key_types))(*key_types)
+        sp = (C.AggSpec * max(1, len(aggs)))(*self.specs)
+        self.handle = ctypes.c_void_p()
+        C.check(C.lib().fpdb_groupby_create(ctx.handle, kt, len(key_types), sp, len(aggs), max_groups,
+                                            ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            C.lib().fpdb_groupby_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def consume(self, key_cols, value_cols, n):
+        vals = [v.c() if v is not None else C.Column(0, 0, 0, 0, None, None) for v in value_cols]
+        C.check(C.lib().fpdb_groupby_consume(self.ctx.handle, self.handle, _cols(key_cols),
+                                             C.columns_array(vals), n))
+
+    def num_groups(self):
+        n = ctypes.c_int64()
+        C.check(C.lib().fpdb_groupby_num_groups(self.ctx.handle, self.handle, ctypes.byref(n)))
+        return n.value
+
+    def finalize(self):
+        ng = self.num_groups()
+        keys = [DeviceColumn.empty(self.ctx, ng, C.TYPE_TO_NP[t], False, t) for t in self.key_types]
+        aggs = []
+        for s in self.specs:
+            ot = C.lib().fpdb_groupby_output_type(s)
+            aggs.append(DeviceColumn.empty(self.ctx, ng, C.TYPE_TO_NP[ot], True, ot))
+        n = ctypes.c_int64()
+        C.check(C.lib().fpdb_groupby_finalize(self.ctx.handle, self.handle, C.columns_array([k.c() for k in keys]),
+                                              C.columns_array([a.c() for a in aggs]), ctypes.byref(n)))
+        for c in keys + aggs:
+            c.length = n.value
+        return keys, aggs

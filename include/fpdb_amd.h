@@ -1,0 +1,342 @@
+/*
+ * fpdb_amd.h -- C ABI of the MI355X (gfx950) relational hot path of
+ * FlexPushdownDB: blocked and legacy bloom filters (predicate transfer),
+ * hash-join build/probe, hash group-by, and filter compaction.
+ *
+ * Conventions
+ *   - Every entry point returns an fpdb_status; FPDB_OK is 0.  On failure
+ *     fpdb_last_error() returns a message for the calling thread.  Nothing
+ *     throws across this boundary.
+ *   - Pointers named d_* are device (HBM) pointers; everything else is host.
+ *   - Inputs are borrowed for the duration of the call and never mutated.
+ *   - Work is enqueued on the context's stream; results are ready after
+ *     fpdb_context_synchronize() unless a function says it synchronizes.
+ *   - Columns use the Arrow layout: a values buffer, an optional validity
+ *     bitmap (LSB-first, 1 = valid) and an element offset.
+ *   - Calls on one context are serialised by the caller (one operator instance
+ *     per context, like one actor per physical operator); different contexts
+ *     may be used from different threads concurrently.
+ *
+ * Reference paths below are relative to the FlexPushdownDB source tree.
+ */
+#ifndef FPDB_AMD_H
+#define FPDB_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t fpdb_status;
+enum {
+  FPDB_OK = 0,
+  FPDB_ERR_INVALID = 1,     /* bad argument / unsupported type or layout   */
+  FPDB_ERR_HIP = 2,         /* HIP runtime error                           */
+  FPDB_ERR_BUDGET = 3,      /* allocation would exceed the HBM budget      */
+  FPDB_ERR_CAPACITY = 4,    /* an output or table capacity was exceeded    */
+  FPDB_ERR_INTERNAL = 5     /* kernel-side consistency check failed        */
+};
+
+/* Arrow type ids supported on the hot path. */
+typedef enum {
+  FPDB_TYPE_INT8 = 1, FPDB_TYPE_INT16 = 2, FPDB_TYPE_INT32 = 3, FPDB_TYPE_INT64 = 4,
+  FPDB_TYPE_UINT8 = 5, FPDB_TYPE_UINT16 = 6, FPDB_TYPE_UINT32 = 7, FPDB_TYPE_UINT64 = 8,
+  FPDB_TYPE_FLOAT32 = 9, FPDB_TYPE_FLOAT64 = 10, FPDB_TYPE_DATE32 = 11, FPDB_TYPE_DATE64 = 12
+} fpdb_type;
+
+/* One Arrow-layout column on the device.  For outputs, values/validity are
+ * caller-allocated buffers the library writes into (validity may be NULL when
+ * the output cannot hold nulls). */
+typedef struct fpdb_column {
+  int32_t type;             /* fpdb_type                                   */
+  int32_t reserved;
+  int64_t length;           /* rows                                        */
+  int64_t offset;           /* element offset into values and validity     */
+  void* values;             /* device pointer                              */
+  uint8_t* validity;        /* device pointer or NULL (all valid)          */
+} fpdb_column;
+
+typedef struct fpdb_context fpdb_context;
+
+/* ---- context, memory, timing ------------------------------------------- */
+
+/* Creates a context on `device` with an HBM budget (0 = unlimited) for the
+ * buffers this library allocates.  If `stream` is NULL the context creates its
+ * own default-priority stream.  Replaces the per-operator resources the
+ * reference binds in POpContext (fpdb-executor/src/physical/POpContext.cpp:25-58). */
+fpdb_status fpdb_context_create(int device, uint64_t hbm_budget_bytes, void* stream,
+                                fpdb_context** out);
+fpdb_status fpdb_context_destroy(fpdb_context* ctx);
+void* fpdb_context_stream(fpdb_context* ctx);
+fpdb_status fpdb_context_synchronize(fpdb_context* ctx);
+/* Bytes currently allocated through the context, and its budget. */
+fpdb_status fpdb_context_memory(fpdb_context* ctx, uint64_t* used, uint64_t* budget);
+/* Number of compute units and XCDs of the context's device. */
+fpdb_status fpdb_device_info(fpdb_context* ctx, int* num_cus, int* num_xcds, uint64_t* hbm_bytes);
+const char* fpdb_last_error(void);
+
+fpdb_status fpdb_device_alloc(fpdb_context* ctx, uint64_t bytes, void** d_out);
+fpdb_status fpdb_device_free(fpdb_context* ctx, void* d_ptr);
+/* Stream-ordered copies; fpdb_copy_to_host synchronizes before returning. */
+fpdb_status fpdb_copy_to_device(fpdb_context* ctx, void* d_dst, const void* src, uint64_t bytes);
+fpdb_status fpdb_copy_to_host(fpdb_context* ctx, void* dst, const void* d_src, uint64_t bytes);
+fpdb_status fpdb_copy_device(fpdb_context* ctx, void* d_dst, const void* d_src, uint64_t bytes);
+/* Fills with a byte value using a kernel (safe inside captured graphs). */
+fpdb_status fpdb_fill(fpdb_context* ctx, void* d_dst, uint8_t value, uint64_t bytes);
+
+/* HIP events on the context stream: elapsed milliseconds between the last
+ * fpdb_timer_start and fpdb_timer_stop (fpdb_timer_stop synchronizes). */
+fpdb_status fpdb_timer_start(fpdb_context* ctx);
+fpdb_status fpdb_timer_stop(fpdb_context* ctx, float* ms);
+
+/* Kernel timing (the reference's per-operator processing time,
+ * fpdb-executor/src/physical/POpActor.cpp:40-97, at kernel granularity): while
+ * enabled, every named kernel launch is bracketed by HIP events on the context
+ * stream.  fpdb_profile_kernel synchronizes and returns the summed duration
+ * and launch count of one kernel name ("hj_build", "hj_probe",
+ * "bloom_blocked_insert", "bloom_blocked_find", "bloom_blocked_filter",
+ * "bloom_legacy_build", "bloom_legacy_find", "filter_compact",
+ * "groupby_consume"). */
+fpdb_status fpdb_context_set_profiling(fpdb_context* ctx, int enabled);
+fpdb_status fpdb_profile_kernel(fpdb_context* ctx, const char* name, double* total_ms,
+                                int64_t* launches);
+fpdb_status fpdb_profile_reset(fpdb_context* ctx);
+
+/* ---- key hashing -------------------------------------------------------- */
+
+/* uint32 hash per row of fixed-width key columns, as the reference computes it
+ * with Arrow 6 Hashing::HashMultiColumn in RecordBatchHasher::hash
+ * (fpdb-tuple/src/RecordBatchHasher.cpp:69-104). */
+fpdb_status fpdb_hash_keys(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                           int64_t num_rows, uint32_t* d_hashes);
+
+/* ---- blocked bloom filter (default, Arrow BlockedBloomFilter) ----------- */
+
+/* The 136-byte mask bit vector (BloomFilterMasks(),
+ * fpdb-tuple/src/arrow/exec/BloomFilter.cpp:16-79). */
+fpdb_status fpdb_bloom_masks(uint8_t out_masks[136]);
+
+/* log2(number of 64-bit blocks) for a filter sized for `num_rows_to_insert`
+ * keys (BlockedBloomFilter::CreateEmpty, BloomFilter.cpp:87-107). */
+int fpdb_bloom_blocked_log_num_blocks(int64_t num_rows_to_insert);
+
+/* Zeroes the (1 << log_num_blocks) blocks and inserts every key row
+ * (ArrowBloomFilter::build, fpdb-executor/src/physical/bloomfilter/ArrowBloomFilter.cpp:29-75;
+ * BlockedBloomFilter::Insert, BloomFilter.cpp:109-136).  If d_select is not
+ * NULL only rows whose bit is set are inserted. */
+fpdb_status fpdb_bloom_blocked_build(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                                     int64_t num_rows, const uint8_t* d_select,
+                                     int log_num_blocks, uint64_t* d_blocks);
+
+/* Inserts into an existing filter without clearing it (a filter of several
+ * batches, or the OR-merge of partitioned builds). */
+fpdb_status fpdb_bloom_blocked_insert(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                                      int64_t num_rows, const uint8_t* d_select,
+                                      int log_num_blocks, uint64_t* d_blocks);
+
+/* Packed selection bitmap (bit i = Find(hash(row i)), LSB-first, (n+7)/8
+ * bytes written) and, if d_num_selected is not NULL, the number of selected
+ * rows (stored as int64 on the device)
+ * (BlockedBloomFilter::Find, BloomFilter.cpp:138-191). */
+fpdb_status fpdb_bloom_blocked_find(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
+                                    const fpdb_column* keys, int num_keys, int64_t num_rows,
+                                    uint8_t* d_bitmap, int64_t* d_num_selected);
+
+/* ---- legacy bloom filter (universal hashing, target FPR) ---------------- */
+
+/* One Carter-Wegman hash h(x) = ((a*x + b) mod p) mod m
+ * (UniversalHashFunction, fpdb-executor/src/physical/bloomfilter/UniversalHashFunction.cpp:51-57). */
+typedef struct fpdb_universal_hash { int64_t a, b, m, p; } fpdb_universal_hash;
+
+/* k = ceil(log2(1/p)), m = ceil(n |ln p| / ln^2 2)
+ * (BloomFilter::k_from_p / m_from_np, BloomFilter.cpp:249-257). */
+fpdb_status fpdb_bloom_legacy_params(int64_t capacity, double false_positive_rate,
+                                     int64_t* num_hash_functions, int64_t* num_bits);
+
+/* Universal-hash parameters drawn like UniversalHashFunction(m)
+ * (UniversalHashFunction.cpp:13-35) from an explicit seed instead of
+ * std::random_device, so the filter is reproducible. */
+fpdb_status fpdb_bloom_legacy_make_hash_functions(uint32_t seed, int64_t num_bits, int k,
+                                                  fpdb_universal_hash* out);
+
+/* Zeroes num_bits/64+1 words and adds every key row
+ * (BloomFilterCreateKernel::buildBloomFilter / addRecordBatchToBloomFilter,
+ * BloomFilterCreateKernel.cpp:24-99; BloomFilter::add, BloomFilter.cpp:62-73). */
+fpdb_status fpdb_bloom_legacy_build(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                                    int64_t num_rows, const uint8_t* d_select,
+                                    const fpdb_universal_hash* hash_functions, int k,
+                                    int64_t num_bits, int64_t* d_words);
+
+/* Selection bitmap of BloomFilter::contains (BloomFilter.cpp:75-91) per row,
+ * i.e. the rows the legacy filterRecordBatch keeps (BloomFilterUseKernel.cpp:140-178).
+ * capacity == 0 selects nothing. */
+fpdb_status fpdb_bloom_legacy_find(fpdb_context* ctx, const fpdb_universal_hash* hash_functions,
+                                   int k, int64_t capacity, int64_t num_bits, const int64_t* d_words,
+                                   const fpdb_column* keys, int num_keys, int64_t num_rows,
+                                   uint8_t* d_bitmap, int64_t* d_num_selected);
+
+/* ---- filter compaction --------------------------------------------------- */
+
+/* Keeps the rows whose bitmap bit is set, in order, for every column
+ * (arrow::compute::Filter at BloomFilterUseKernel.cpp:195; gandiva
+ * SelectionVector filtering at BloomFilterUseKernel.cpp:170-177).
+ * out[i] must have room for the selected rows; an output validity bitmap
+ * (required when the input column has one) must hold ceil(num_rows/32)*4
+ * bytes and its offset must be a multiple of 32.  *num_out (host) receives the
+ * number of rows written (synchronizes). */
+fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int num_columns,
+                                int64_t num_rows, const uint8_t* d_bitmap,
+                                fpdb_column* out, int64_t* num_out);
+
+/* Fused blocked-bloom use: Find + compaction in one pass over the columns
+ * (BloomFilterUseKernel::filterRecordBatch, BloomFilterUseKernel.cpp:180-207).
+ * key_index[j] names which of `in` are the key columns.  d_bitmap may be NULL.
+ * out[i] must have room for num_rows rows (the count is not known up front).
+ * *num_out (host) receives the selected row count (synchronizes). */
+fpdb_status fpdb_bloom_blocked_filter(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
+                                      const fpdb_column* in, int num_columns,
+                                      const int* key_index, int num_keys, int64_t num_rows,
+                                      uint8_t* d_bitmap, fpdb_column* out, int64_t* num_out);
+
+/* ---- hash join ------------------------------------------------------------ */
+
+/* Join types of fpdb::plan::prephysical::JoinType
+ * (fpdb-plan/include/fpdb/plan/prephysical/JoinType.h:13-20): LEFT = build
+ * side kept, RIGHT = probe side kept, *_SEMI likewise. */
+typedef enum {
+  FPDB_JOIN_INNER = 0, FPDB_JOIN_LEFT = 1, FPDB_JOIN_RIGHT = 2, FPDB_JOIN_FULL = 3,
+  FPDB_JOIN_LEFT_SEMI = 4, FPDB_JOIN_RIGHT_SEMI = 5
+} fpdb_join_type;
+
+typedef struct fpdb_hash_table fpdb_hash_table;
+
+/* Builds an open-addressing table in HBM over the build side's key columns
+ * (one integer column, or two integer columns of <= 4 bytes each).  `payload`
+ * (may be NULL) is one 4-byte build column stored inline in the table slots so
+ * the probe reads it together with the key.  Rows with a null key are never
+ * inserted.  track_build_matches != 0 keeps one flag per build row, set when a
+ * probe matches it (needed for LEFT, FULL and LEFT_SEMI joins).
+ * Replaces the build phase of the acero "hashjoin" node driven by
+ * HashJoinArrowKernel::joinBuildTupleSet/finalizeInput
+ * (fpdb-executor/src/physical/join/hashjoin/HashJoinArrowKernel.cpp:35-93,353-372)
+ * and TupleSetIndex::build (fpdb-tuple/src/TupleSetIndex.cpp:44-88). */
+fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                          int64_t num_rows, const fpdb_column* payload,
+                          int track_build_matches, fpdb_hash_table** out);
+fpdb_status fpdb_hj_table_destroy(fpdb_hash_table* table);
+/* Build rows seen, rows inserted (non-null keys), table slots, and whether all
+ * inserted keys are distinct (then a probe row has at most one match). */
+fpdb_status fpdb_hj_table_info(fpdb_hash_table* table, int64_t* build_rows, int64_t* inserted,
+                               int64_t* capacity, int* unique_keys);
+
+/* Probes and materialises in one pass.  Output rows are in probe-row order
+ * (the order among the matches of one probe row is unspecified).  Output
+ * columns are the probe columns listed in probe_out (row-copied) followed by
+ * the build columns listed in build_out (gathered by build row;
+ * build_out[j].values == NULL selects the inline payload).  By join_type:
+ *   INNER       matched pairs;
+ *   RIGHT       matched pairs, plus unmatched probe rows with null build columns;
+ *   RIGHT_SEMI  probe rows with at least one match (build_out ignored);
+ *   LEFT, FULL  as INNER, RIGHT, and matched build rows are flagged; after the
+ *               last probe, fpdb_hj_build_side_rows(matched = 0) lists the
+ *               unmatched build rows to append with null probe columns;
+ *   LEFT_SEMI   writes nothing, flags matched build rows; afterwards
+ *               fpdb_hj_build_side_rows(matched = 1) lists the result rows.
+ * out[] needs room for out_capacity rows; output validity bitmaps (required
+ * for nullable results) need ceil(out_capacity/32)*4 bytes.  If more rows
+ * would be produced the call fails with FPDB_ERR_CAPACITY and *num_out holds
+ * the required count.  *num_out (host) receives the row count (synchronizes).
+ * Replaces the probe + materialisation of the acero "hashjoin" node
+ * (HashJoinArrowKernel.cpp:126-206,305-351) and RecordBatchHashJoiner::join
+ * (fpdb-executor/src/physical/join/hashjoin/RecordBatchHashJoiner.cpp:38-131). */
+fpdb_status fpdb_hj_probe(fpdb_context* ctx, fpdb_hash_table* table, int join_type,
+                          const fpdb_column* probe_keys, int num_keys, int64_t num_rows,
+                          const fpdb_column* probe_out, int num_probe_out,
+                          const fpdb_column* build_out, int num_build_out,
+                          fpdb_column* out, int64_t out_capacity, int64_t* num_out);
+
+/* fpdb_hj_probe without waiting: the produced row count is written to
+ * d_num_out (device int64) by the kernel; rows past out_capacity are counted
+ * but not written.  The caller checks *d_num_out <= out_capacity once the
+ * stream has been synchronized. */
+fpdb_status fpdb_hj_probe_async(fpdb_context* ctx, fpdb_hash_table* table, int join_type,
+                                const fpdb_column* probe_keys, int num_keys, int64_t num_rows,
+                                const fpdb_column* probe_out, int num_probe_out,
+                                const fpdb_column* build_out, int num_build_out,
+                                fpdb_column* out, int64_t out_capacity, int64_t* d_num_out);
+
+/* The number of rows fpdb_hj_probe would produce (synchronizes). */
+fpdb_status fpdb_hj_probe_count(fpdb_context* ctx, fpdb_hash_table* table, int join_type,
+                                const fpdb_column* probe_keys, int num_keys, int64_t num_rows,
+                                int64_t* count);
+
+/* After all probes: build rows never matched (matched = 0) or matched at least
+ * once (matched = 1), as int64 row indices in build order (the acero
+ * RIGHT_OUTER / RIGHT_SEMI sides that LEFT / LEFT_SEMI map to,
+ * HashJoinArrowKernel.cpp:165-195).  *num_out (host) receives the count. */
+fpdb_status fpdb_hj_build_side_rows(fpdb_context* ctx, fpdb_hash_table* table, int matched,
+                                    int64_t* d_rows, int64_t capacity, int64_t* num_out);
+
+/* Gathers rows (int64 indices; -1 = null row) of columns into out (used to
+ * materialise build-side rows and the null probe side of outer joins). */
+fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns,
+                        const int64_t* d_rows, int64_t num_rows, fpdb_column* out);
+
+/* ---- hash group-by --------------------------------------------------------- */
+
+/* Aggregate kinds of fpdb::executor::physical::aggregate
+ * (Sum.cpp:63-74 hash_sum; Count.cpp:91-104 hash_count ALL / ONLY_VALID;
+ * MinMax.cpp:84 hash_min / hash_max). */
+typedef enum {
+  FPDB_AGG_COUNT_ALL = 0, FPDB_AGG_COUNT_VALID = 1, FPDB_AGG_SUM = 2,
+  FPDB_AGG_MIN = 3, FPDB_AGG_MAX = 4
+} fpdb_agg_kind;
+
+typedef struct fpdb_agg_spec {
+  int32_t kind;             /* fpdb_agg_kind                                  */
+  int32_t input_type;       /* fpdb_type of the input column (COUNT_ALL: any) */
+  int32_t input_nullable;   /* input batches may carry a validity bitmap      */
+  int32_t reserved;
+} fpdb_agg_spec;
+
+typedef struct fpdb_group_table fpdb_group_table;
+
+/* Creates an accumulating grouped aggregation over key columns of the given
+ * types (one of <= 8 bytes or two of <= 4 bytes, no nulls) for at most
+ * max_groups groups.  Replaces GroupArrowKernel's acero "aggregate" node
+ * (fpdb-executor/src/physical/group/GroupArrowKernel.cpp:16-68,127-154,245-291). */
+fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int num_keys,
+                                const fpdb_agg_spec* aggs, int num_aggs, int64_t max_groups,
+                                fpdb_group_table** out);
+fpdb_status fpdb_groupby_destroy(fpdb_group_table* table);
+
+/* Consumes one batch (GroupArrowKernel::group / doGroup, GroupArrowKernel.cpp:16-39,127-154).
+ * values[j] is the input of aggs[j] (ignored for COUNT_ALL). */
+fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* table,
+                                 const fpdb_column* keys, const fpdb_column* values,
+                                 int64_t num_rows);
+
+/* Number of groups so far (synchronizes). */
+fpdb_status fpdb_groupby_num_groups(fpdb_context* ctx, fpdb_group_table* table, int64_t* num_groups);
+
+/* Output type of an aggregate: COUNT -> INT64; SUM of signed integers ->
+ * INT64, of unsigned -> UINT64, of floats -> FLOAT64; MIN/MAX -> input type. */
+int32_t fpdb_groupby_output_type(fpdb_agg_spec spec);
+
+/* Writes the groups in first-appearance order (the serial grouper's group-id
+ * order): key columns then one column per aggregate (COUNT: int64; SUM of
+ * integers: int64; SUM of floats: float64; MIN/MAX: input type).  A SUM, MIN
+ * or MAX with no valid input in a group is null (ScalarAggregateOptions
+ * defaults, min_count = 1).  out_keys/out_aggs need room for num_groups rows
+ * (GroupArrowKernel::finalise, GroupArrowKernel.cpp:41-68). */
+fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* table,
+                                  fpdb_column* out_keys, fpdb_column* out_aggs,
+                                  int64_t* num_groups);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FPDB_AMD_H */

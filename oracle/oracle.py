@@ -1,0 +1,226 @@
+"""numpy/ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import
+this module.  It restates FlexPushdownDB's hot-path operators on the CPU so the
+HIP path can be checked against it; it is never the thing measured on the GPU
+and the product library never imports it.
+
+Each function names the reference file:line it restates (paths relative to the
+FlexPushdownDB source tree).  The C++ core is fpdb_oracle.cpp next to this file.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i64, i32, u32, dbl = ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_double
+        L.orc_bloom_masks.argtypes = [P]
+        L.orc_bloom_masks.restype = i32
+        L.orc_mt19937_raw.argtypes = [i32, P, i64]
+        L.orc_uniform_u64_draws.argtypes = [P, i64]
+        L.orc_blocked_log_num_blocks.argtypes = [i64]
+        L.orc_blocked_log_num_blocks.restype = i32
+        L.orc_blocked_insert.argtypes = [P, P, i32, P, i64]
+        L.orc_blocked_find.argtypes = [P, P, i32, P, i64, P]
+        L.orc_arrow6_hash_column.argtypes = [P, i32, P, i64, P, i32]
+        L.orc_legacy_k.argtypes = [dbl]
+        L.orc_legacy_k.restype = i64
+        L.orc_legacy_m.argtypes = [i64, dbl]
+        L.orc_legacy_m.restype = i64
+        L.orc_legacy_keys.argtypes = [P, P, i32, i64, P]
+        L.orc_legacy_add.argtypes = [P, i32, P, P, i64]
+        L.orc_legacy_find.argtypes = [P, i32, i64, P, P, i64, P]
+        L.orc_is_prime.argtypes = [ctypes.c_uint64]
+        L.orc_is_prime.restype = i32
+        L.orc_next_prime.argtypes = [ctypes.c_uint64]
+        L.orc_next_prime.restype = ctypes.c_uint64
+        L.orc_legacy_make_params.argtypes = [u32, i64, i32, P]
+        L.orc_join_pairs.argtypes = [P, P, i64, P, P, i64, P, P, P]
+        L.orc_join_pairs.restype = i64
+        L.orc_groupby.argtypes = [P, i64, P, i32, P, i32, i64, P, P, P, P, P]
+        L.orc_groupby.restype = i64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --- blocked bloom filter ---------------------------------------------------
+
+def bloom_masks():
+    """136-byte mask bit vector (fpdb-tuple/src/arrow/exec/BloomFilter.cpp:16-79)."""
+    out = np.zeros(136, np.uint8)
+    lib().orc_bloom_masks(_p(out))
+    return out
+
+
+def mask_table_u64(masks=None):
+    """The 1024 57-bit masks expanded to uint64 (BloomFilter.h:52-61)."""
+    masks = bloom_masks() if masks is None else masks
+    buf = np.zeros(144, np.uint8)
+    buf[:136] = masks
+    out = np.zeros(1024, np.uint64)
+    for i in range(1024):
+        w = int.from_bytes(bytes(buf[i // 8:i // 8 + 8]), "little")
+        out[i] = (w >> (i % 8)) & ((1 << 57) - 1)
+    return out
+
+
+def blocked_log_num_blocks(n):
+    return int(lib().orc_blocked_log_num_blocks(int(n)))
+
+
+def arrow6_hash(columns):
+    """uint32 key hash per row over fixed-width columns.
+
+    columns: list of (values ndarray, validity bitmap ndarray or None).
+    Restates Arrow 6 Hashing::HashMultiColumn as called at
+    fpdb-tuple/src/RecordBatchHasher.cpp:97-100 (parity unpinned: the Arrow 6
+    source is not in the reference tree).
+    """
+    n = len(columns[0][0])
+    out = np.zeros(n, np.uint32)
+    for i, (vals, valid) in enumerate(columns):
+        vals = np.ascontiguousarray(vals)
+        lib().orc_arrow6_hash_column(_p(vals), vals.dtype.itemsize, _p(valid), n, _p(out), 1 if i else 0)
+    return out
+
+
+def blocked_build(hashes, n_capacity=None):
+    """BlockedBloomFilter built by BloomFilterBuilder_SingleThreaded
+    (ArrowBloomFilter.cpp:29-75, BloomFilter.cpp:87-136)."""
+    hashes = np.ascontiguousarray(hashes, dtype=np.uint32)
+    cap = len(hashes) if n_capacity is None else n_capacity
+    lnb = blocked_log_num_blocks(cap)
+    blocks = np.zeros(1 << lnb, np.uint64)
+    masks = bloom_masks()
+    lib().orc_blocked_insert(_p(masks), _p(blocks), lnb, _p(hashes), len(hashes))
+    return lnb, blocks
+
+
+def blocked_find(lnb, blocks, hashes):
+    """Packed selection bitmap (BloomFilter.cpp:138-170)."""
+    hashes = np.ascontiguousarray(hashes, dtype=np.uint32)
+    n = len(hashes)
+    out = np.zeros((n + 7) // 8, np.uint8)
+    masks = bloom_masks()
+    lib().orc_blocked_find(_p(masks), _p(blocks), lnb, _p(hashes), n, _p(out))
+    return out
+
+
+# --- legacy bloom filter ----------------------------------------------------
+
+def legacy_k(p):
+    return int(lib().orc_legacy_k(p))
+
+
+def legacy_m(n, p):
+    return int(lib().orc_legacy_m(int(n), p))
+
+
+def legacy_keys(cols):
+    """hashCombine over std::hash of each column (ArrayHasher.cpp:12-19, Util.cpp:106-113)."""
+    cols = [np.ascontiguousarray(c) for c in cols]
+    ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    widths = (ctypes.c_int * len(cols))(*[c.dtype.itemsize for c in cols])
+    out = np.zeros(len(cols[0]), np.int64)
+    lib().orc_legacy_keys(ptrs, widths, len(cols), len(cols[0]), _p(out))
+    return out
+
+
+def legacy_params(seed, m, k):
+    out = np.zeros((k, 4), np.int64)
+    lib().orc_legacy_make_params(seed, m, k, _p(out))
+    return out
+
+
+def legacy_build(params, num_bits, keys):
+    words = np.zeros(num_bits // 64 + 1, np.int64)     # BloomFilter.cpp:234-237
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    params = np.ascontiguousarray(params, dtype=np.int64)
+    lib().orc_legacy_add(_p(params), len(params), _p(words), _p(keys), len(keys))
+    return words
+
+
+def legacy_find(params, capacity, words, keys):
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    params = np.ascontiguousarray(params, dtype=np.int64)
+    out = np.zeros((len(keys) + 7) // 8, np.uint8)
+    lib().orc_legacy_find(_p(params), len(params), int(capacity), _p(words), _p(keys), len(keys), _p(out))
+    return out
+
+
+def is_prime(n):
+    return bool(lib().orc_is_prime(int(n)))
+
+
+def next_prime(n):
+    return int(lib().orc_next_prime(int(n)))
+
+
+# --- hash join ----------------------------------------------------------------
+
+def join_pairs(bkeys, pkeys, bvalid=None, pvalid=None, want_matched=False):
+    """(probe_row, build_row) pairs of an inner equi-join in probe order
+    (HashJoinArrowKernel.cpp:126-206 semantics; TupleSetIndex.cpp:44-88)."""
+    bkeys = np.ascontiguousarray(bkeys, dtype=np.int64)
+    pkeys = np.ascontiguousarray(pkeys, dtype=np.int64)
+    L = lib()
+    matched = np.zeros(len(bkeys), np.uint8) if want_matched else None
+    cnt = L.orc_join_pairs(_p(bkeys), _p(bvalid), len(bkeys), _p(pkeys), _p(pvalid), len(pkeys),
+                           None, None, None)
+    op = np.zeros(cnt, np.int64)
+    ob = np.zeros(cnt, np.int64)
+    L.orc_join_pairs(_p(bkeys), _p(bvalid), len(bkeys), _p(pkeys), _p(pvalid), len(pkeys),
+                     _p(op), _p(ob), _p(matched))
+    if want_matched:
+        return op, ob, matched.astype(bool)
+    return op, ob
+
+
+# --- group-by -------------------------------------------------------------------
+
+def groupby(keys_u64, sums_i64=(), sums_f64=(), max_groups=None):
+    """Grouped COUNT(*), SUM(int64) and SUM(float64) in first-appearance group
+    order (GroupArrowKernel.cpp:202-243, Sum.cpp:63-74, Count.cpp:91-104)."""
+    keys_u64 = np.ascontiguousarray(keys_u64, dtype=np.uint64)
+    n = len(keys_u64)
+    mg = int(max_groups or max(1, n))
+    si = [np.ascontiguousarray(v, dtype=np.int64) for v in sums_i64]
+    sf = [np.ascontiguousarray(v, dtype=np.float64) for v in sums_f64]
+    pi = (ctypes.c_void_p * max(1, len(si)))(*[v.ctypes.data for v in si])
+    pf = (ctypes.c_void_p * max(1, len(sf)))(*[v.ctypes.data for v in sf])
+    ok = np.zeros(mg, np.uint64)
+    oc = np.zeros(mg, np.int64)
+    osi = np.zeros((max(1, len(si)), mg), np.int64)
+    osf = np.zeros((max(1, len(sf)), mg), np.float64)
+    ofr = np.zeros(mg, np.int64)
+    ng = lib().orc_groupby(_p(keys_u64), n, pi, len(si), pf, len(sf), mg,
+                           _p(ok), _p(oc), _p(osi), _p(osf), _p(ofr))
+    if ng < 0:
+        raise ValueError("more groups than max_groups")
+    return {
+        "keys": ok[:ng].copy(), "count": oc[:ng].copy(),
+        "sum_i64": [osi[j, :ng].copy() for j in range(len(si))],
+        "sum_f64": [osf[j, :ng].copy() for j in range(len(sf))],
+        "first_row": ofr[:ng].copy(),
+    }

@@ -1,0 +1,196 @@
+"""Bloom filters and filter compaction on the GPU, bit-exact against the oracle.
+
+Blocked filter: bit arrays and selection bitmaps are compared with the oracle's
+restatement of the vendored Arrow BlockedBloomFilter fed with the oracle's
+restatement of the Arrow 6 key hash (parity with the reference binary's hash is
+unpinned; the filter arithmetic given the hash is pinned).  Legacy filter:
+fully pinned, compared bit for bit.
+"""
+import numpy as np
+import pytest
+
+from flexpushdowndb_amd import _capi as C
+from flexpushdowndb_amd import datagen, ops
+from flexpushdowndb_amd.device import DeviceColumn
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(bm_dev, n):
+    return np.unpackbits(bm_dev.numpy((n + 7) // 8), bitorder="little")[:n].astype(bool)
+
+
+def _valid_bits(mask):
+    return np.packbits(mask, bitorder="little")
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.int32])
+def test_hash_keys_match_oracle(ctx, dtype):
+    rng = np.random.default_rng(1)
+    v = rng.integers(-2 ** 31, 2 ** 31 - 1, 10_001).astype(dtype)
+    valid = rng.random(10_001) > 0.1
+    col = DeviceColumn.from_numpy(ctx, v, valid)
+    got = ops.hash_keys(ctx, [col], len(v)).numpy()
+    assert np.array_equal(got, O.arrow6_hash([(v, _valid_bits(valid))]))
+    w = rng.integers(0, 1000, 10_001).astype(np.int32)
+    got2 = ops.hash_keys(ctx, [col, DeviceColumn.from_numpy(ctx, w)], len(v)).numpy()
+    assert np.array_equal(got2, O.arrow6_hash([(v, _valid_bits(valid)), (w, None)]))
+
+
+@pytest.mark.parametrize("n_build,n_probe", [(0, 100), (1, 1), (63, 64), (64, 65), (1000, 4097),
+                                             (70_000, 200_003)])
+def test_blocked_build_find_bit_exact(ctx, n_build, n_probe):
+    rng = np.random.default_rng(n_build + 7)
+    keys = rng.integers(0, 2 ** 40, n_build).astype(np.int64)
+    probe = np.concatenate([keys[: n_probe // 2], rng.integers(0, 2 ** 40, n_probe - min(n_probe // 2, n_build))])
+    probe = probe[:n_probe].astype(np.int64)
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, keys)], n_build)
+    lnb, ref_blocks = O.blocked_build(O.arrow6_hash([(keys, None)]) if n_build else np.zeros(0, np.uint32),
+                                      n_capacity=n_build)
+    assert f.log_num_blocks == lnb
+    assert np.array_equal(f.blocks_numpy(), ref_blocks)
+    bm, cnt = f.find([DeviceColumn.from_numpy(ctx, probe)], n_probe)
+    ref_bm = O.blocked_find(lnb, ref_blocks, O.arrow6_hash([(probe, None)]))
+    got_bytes = bm.numpy((n_probe + 7) // 8)
+    assert np.array_equal(got_bytes, ref_bm)
+    assert cnt == int(np.unpackbits(ref_bm, bitorder="little").sum())
+
+
+def test_blocked_more_than_2_16_blocks_quirk(ctx):
+    # 32-bit hashes address at most 2^16 blocks: a filter sized for > 524,288
+    # keys never sets blocks >= 2^16 (block_id uses hash bits 16..31)
+    n = 600_000
+    keys = np.arange(n, dtype=np.int64) * 7919
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, keys)], n)
+    blocks = f.blocks_numpy()
+    assert f.log_num_blocks == 17
+    assert not blocks[1 << 16:].any()
+    _, ref = O.blocked_build(O.arrow6_hash([(keys, None)]), n_capacity=n)
+    assert np.array_equal(blocks, ref)
+
+
+def test_blocked_with_nulls_and_select_and_int32(ctx):
+    rng = np.random.default_rng(4)
+    n = 50_000
+    k32 = rng.integers(-1000, 1000, n).astype(np.int32)
+    valid = rng.random(n) > 0.2
+    select = rng.random(n) > 0.5
+    kc = DeviceColumn.from_numpy(ctx, k32, valid)
+    sel = ctx.upload(np.packbits(select, bitorder="little"))
+    f = ops.BlockedBloomFilter.build(ctx, [kc], n, select=sel, capacity=int(select.sum()))
+    h = O.arrow6_hash([(k32, _valid_bits(valid))])
+    lnb, ref = O.blocked_build(h[select], n_capacity=int(select.sum()))
+    assert np.array_equal(f.blocks_numpy(), ref)
+    bm, _ = f.find([kc], n)
+    assert np.array_equal(bm.numpy((n + 7) // 8), O.blocked_find(lnb, ref, h))
+
+
+def test_blocked_insert_is_or_merge(ctx):
+    # partitioned builds OR together into the single-build bit array
+    rng = np.random.default_rng(9)
+    keys = rng.integers(0, 2 ** 40, 30_000).astype(np.int64)
+    whole = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, keys)], len(keys))
+    parts = ops.BlockedBloomFilter(ctx, len(keys))
+    for chunk in np.array_split(keys, 5):
+        parts.insert([DeviceColumn.from_numpy(ctx, chunk)], len(chunk))
+    assert np.array_equal(whole.blocks_numpy(), parts.blocks_numpy())
+
+
+@pytest.mark.parametrize("n", [0, 1, 100, 2048, 2049, 100_003])
+def test_fused_bloom_filter_compaction(ctx, n):
+    rng = np.random.default_rng(n + 3)
+    keys = rng.integers(0, 2 ** 20, 5000).astype(np.int64)
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, keys)], len(keys))
+    probe = rng.integers(0, 2 ** 21, n).astype(np.int64)
+    pay = rng.integers(-50, 50, n).astype(np.int32)
+    dbl = rng.random(n)
+    pvalid = rng.random(n) > 0.3
+    cols = [DeviceColumn.from_numpy(ctx, probe), DeviceColumn.from_numpy(ctx, pay, pvalid),
+            DeviceColumn.from_numpy(ctx, dbl)]
+    out, cnt, bm = f.filter(cols, [0], n)
+    ref_bm = O.blocked_find(f.log_num_blocks, f.blocks_numpy(), O.arrow6_hash([(probe, None)]))
+    sel = np.unpackbits(ref_bm, bitorder="little")[:n].astype(bool) if n else np.zeros(0, bool)
+    assert cnt == int(sel.sum())
+    assert np.array_equal(bm.numpy((n + 7) // 8), ref_bm)
+    assert np.array_equal(out[0].numpy(), probe[sel])
+    assert np.array_equal(out[2].numpy(), dbl[sel])
+    assert np.array_equal(out[1].valid_mask(), pvalid[sel])
+    assert np.array_equal(out[1].numpy()[pvalid[sel]], pay[sel][pvalid[sel]])
+    # no false negatives for inserted keys
+    if n:
+        _, c2, _ = f.filter([DeviceColumn.from_numpy(ctx, keys)], [0], len(keys))
+        assert c2 == len(keys)
+
+
+def test_filter_compact_plain(ctx):
+    rng = np.random.default_rng(12)
+    n = 77_777
+    sel = rng.random(n) > 0.6
+    a = rng.integers(0, 1 << 62, n).astype(np.int64)
+    b = rng.integers(0, 255, n).astype(np.uint8)
+    out, cnt = ops.filter_compact(ctx, [DeviceColumn.from_numpy(ctx, a), DeviceColumn.from_numpy(ctx, b)], n,
+                                  ctx.upload(np.packbits(sel, bitorder="little")))
+    assert cnt == sel.sum()
+    assert np.array_equal(out[0].numpy(), a[sel]) and np.array_equal(out[1].numpy(), b[sel])
+
+
+def test_legacy_build_find_bit_exact(ctx):
+    rng = np.random.default_rng(21)
+    n = 40_000
+    keys = rng.integers(-2 ** 40, 2 ** 40, n).astype(np.int64)
+    lf = ops.LegacyBloomFilter(ctx, n, 0.01, seed=1234)
+    assert lf.k == 7 and lf.num_bits == O.legacy_m(n, 0.01)
+    assert np.array_equal(lf.hash_functions, O.legacy_params(1234, lf.num_bits, 7))
+    lf.build([DeviceColumn.from_numpy(ctx, keys)], n)
+    okeys = O.legacy_keys([keys])
+    ref_words = O.legacy_build(lf.hash_functions, lf.num_bits, okeys)
+    assert np.array_equal(lf.words.numpy(), ref_words)
+    probe = np.concatenate([keys[:20_000], rng.integers(-2 ** 40, 2 ** 40, 30_001)]).astype(np.int64)
+    bm, cnt = lf.find([DeviceColumn.from_numpy(ctx, probe)], len(probe))
+    ref_bm = O.legacy_find(lf.hash_functions, n, ref_words, O.legacy_keys([probe]))
+    assert np.array_equal(bm.numpy(len(ref_bm)), ref_bm)
+    sel = np.unpackbits(ref_bm, bitorder="little")[:len(probe)]
+    assert cnt == sel.sum() and sel[:20_000].all()
+    # the false-positive rate lands near the 1% target
+    fpr = sel[20_000:].mean()
+    assert fpr < 0.02
+
+
+def test_legacy_two_columns_and_capacity_zero(ctx):
+    rng = np.random.default_rng(5)
+    a = rng.integers(-100, 100, 5000).astype(np.int32)
+    b = rng.integers(0, 10 ** 12, 5000).astype(np.int64)
+    lf = ops.LegacyBloomFilter(ctx, 5000, 0.001, seed=7)
+    lf.build([DeviceColumn.from_numpy(ctx, a), DeviceColumn.from_numpy(ctx, b)], 5000)
+    ref = O.legacy_build(lf.hash_functions, lf.num_bits, O.legacy_keys([a, b]))
+    assert np.array_equal(lf.words.numpy(), ref)
+    empty = ops.LegacyBloomFilter(ctx, 0, 0.01)
+    assert empty.k == 0
+    empty.build([DeviceColumn.from_numpy(ctx, b)], 0)
+    bm, cnt = empty.find([DeviceColumn.from_numpy(ctx, b)], 5000)
+    assert cnt == 0
+
+
+def test_bloom_workload_full_size_properties(ctx):
+    """Full single-edge predicate-transfer workload: 2M keys filtered to ~400k,
+    60M probe keys.  Bit array against the oracle; bitmap by no-false-negative
+    and popcount identities (size-independent checks)."""
+    w = datagen.bloom_workload()
+    kept = w["kept"]
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, kept)], len(kept))
+    lnb, ref = O.blocked_build(O.arrow6_hash([(kept, None)]), n_capacity=len(kept))
+    assert f.log_num_blocks == lnb == 16
+    assert np.array_equal(f.blocks_numpy(), ref)
+    probe = w["probe"]
+    pc = DeviceColumn.from_numpy(ctx, probe)
+    bm, cnt = f.find([pc], len(probe))
+    got = bm.numpy((len(probe) + 7) // 8)
+    # exact on a 2M-row slice, counting identity on all of it
+    ref_head = O.blocked_find(lnb, ref, O.arrow6_hash([(probe[:2_000_000], None)]))
+    assert np.array_equal(got[:250_000], ref_head)
+    assert cnt == int(np.unpackbits(got, bitorder="little")[:len(probe)].sum())
+    out, c2, bm2 = f.filter([pc], [0], len(probe))
+    assert c2 == cnt and np.array_equal(bm2.numpy(len(got)), got)
+    head_bits = np.unpackbits(got[:250_000], bitorder="little").astype(bool)
+    assert head_bits[np.isin(probe[:2_000_000], kept)].all()   # no false negatives

@@ -1,0 +1,139 @@
+"""Hash group-by on the GPU against the oracle: integer and fixed-point
+aggregates exact, group order = first appearance, FP64 sums within 1e-12
+relative of the sequential FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+from flexpushdowndb_amd import _capi as C
+from flexpushdowndb_amd import datagen, ops
+from flexpushdowndb_amd.device import DeviceColumn
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-12   # FP64 sums vs sequential FP64
+
+
+def _check(keys_np, res, okeys, count, sums_i, sums_f):
+    ref = O.groupby(keys_np.astype(np.uint64), sums_i64=sums_i, sums_f64=sums_f)
+    assert np.array_equal(okeys, ref["keys"].astype(okeys.dtype))
+    assert np.array_equal(count, ref["count"])
+    for j, s in enumerate(res.get("sum_i", [])):
+        assert np.array_equal(s, ref["sum_i64"][j])
+    for j, s in enumerate(res.get("sum_f", [])):
+        r = ref["sum_f64"][j]
+        err = np.abs(s - r) / np.maximum(np.abs(r), 1e-300)
+        assert err.max() <= REL_TOL, err.max()
+
+
+def _run(ctx, keys, qty, price, disc, batches=1, max_groups=200_000):
+    g = ops.GroupBy(ctx, [C.NP_TO_TYPE[keys.dtype]],
+                    [(C.AGG_COUNT_ALL, None, False), (C.AGG_SUM, C.TYPE_INT64, False),
+                     (C.AGG_SUM, C.TYPE_INT64, False), (C.AGG_SUM, C.TYPE_FLOAT64, False)], max_groups)
+    for part in np.array_split(np.arange(len(keys)), batches):
+        lo, hi = (part[0], part[-1] + 1) if len(part) else (0, 0)
+        kc = DeviceColumn.from_numpy(ctx, keys[lo:hi])
+        vs = [None, DeviceColumn.from_numpy(ctx, qty[lo:hi]), DeviceColumn.from_numpy(ctx, price[lo:hi]),
+              DeviceColumn.from_numpy(ctx, disc[lo:hi])]
+        g.consume([kc], vs, hi - lo)
+    k, a = g.finalize()
+    return k[0].numpy(), [x.numpy() for x in a]
+
+
+@pytest.mark.parametrize("n,batches", [(1, 1), (1000, 1), (300_001, 1), (300_001, 4)])
+def test_zipf_groupby_vs_oracle(ctx, n, batches):
+    w = datagen.groupby_workload(n=n, seed=11)
+    k, a = _run(ctx, w["key"], w["qty"], w["price"], w["disc"], batches)
+    _check(w["key"], {"sum_i": [a[1], a[2]], "sum_f": [a[3]]}, k, a[0], [w["qty"], w["price"]], [w["disc"]])
+
+
+def test_attic_group_vector(ctx):
+    # fpdb-executor/ATTIC/test/GroupTest.cpp:137-150: sum(ab) by aa -> 10: 27, 12: 15
+    g = ops.GroupBy(ctx, [C.TYPE_INT64], [(C.AGG_SUM, C.TYPE_INT64, False)], 16)
+    g.consume([DeviceColumn.from_numpy(ctx, np.array([10, 10, 12], np.int64))],
+              [DeviceColumn.from_numpy(ctx, np.array([13, 14, 15], np.int64))], 3)
+    k, a = g.finalize()
+    assert k[0].numpy().tolist() == [10, 12] and a[0].numpy().tolist() == [27, 15]
+
+
+def test_two_keys_sentinel_and_wrapping(ctx):
+    rng = np.random.default_rng(3)
+    n = 50_000
+    k1 = rng.integers(-3, 3, n).astype(np.int32)
+    k2 = rng.integers(-3, 3, n).astype(np.int32)
+    big = rng.integers(2 ** 62, 2 ** 63 - 1, n).astype(np.int64)     # sums wrap
+    g = ops.GroupBy(ctx, [C.TYPE_INT32, C.TYPE_INT32], [(C.AGG_SUM, C.TYPE_INT64, False),
+                                                        (C.AGG_COUNT_ALL, None, False)], 100)
+    g.consume([DeviceColumn.from_numpy(ctx, k1), DeviceColumn.from_numpy(ctx, k2)],
+              [DeviceColumn.from_numpy(ctx, big), None], n)
+    keys, aggs = g.finalize()
+    packed = (k1.astype(np.uint32).astype(np.uint64)) | (k2.astype(np.uint32).astype(np.uint64) << np.uint64(32))
+    ref = O.groupby(packed, sums_i64=[big])
+    # (-1, -1) packs to all-ones, the reserved value: it must still be a group
+    assert (np.uint64(0xFFFFFFFFFFFFFFFF) == packed).any()
+    got = keys[0].numpy().astype(np.uint32).astype(np.uint64) | (keys[1].numpy().astype(np.uint32).astype(np.uint64) << np.uint64(32))
+    assert np.array_equal(got, ref["keys"])
+    assert np.array_equal(aggs[0].numpy(), ref["sum_i64"][0])
+    assert np.array_equal(aggs[1].numpy(), ref["count"])
+
+
+def test_min_max_and_nullable_sums(ctx):
+    rng = np.random.default_rng(6)
+    n = 20_000
+    keys = rng.integers(0, 300, n).astype(np.int64)
+    v = rng.integers(-10 ** 9, 10 ** 9, n).astype(np.int64)
+    f = rng.standard_normal(n)
+    valid = rng.random(n) > 0.5
+    valid[keys == 7] = False      # group 7 has no valid inputs -> null sum
+    g = ops.GroupBy(ctx, [C.TYPE_INT64], [(C.AGG_MIN, C.TYPE_INT64, False), (C.AGG_MAX, C.TYPE_FLOAT64, False),
+                                          (C.AGG_SUM, C.TYPE_INT64, True), (C.AGG_COUNT_VALID, C.TYPE_INT64, True)],
+                    1000)
+    vn = DeviceColumn.from_numpy(ctx, v, valid)
+    g.consume([DeviceColumn.from_numpy(ctx, keys)],
+              [DeviceColumn.from_numpy(ctx, v), DeviceColumn.from_numpy(ctx, f), vn, vn], n)
+    k, a = g.finalize()
+    kk = k[0].numpy()
+    _, first = np.unique(keys, return_index=True)
+    assert np.array_equal(kk, keys[np.sort(first)])
+    for i, key in enumerate(kk):
+        m = keys == key
+        assert a[0].numpy()[i] == v[m].min()
+        assert a[1].numpy()[i] == f[m].max()
+        mv = m & valid
+        assert a[3].numpy()[i] == mv.sum()
+        assert a[2].valid_mask()[i] == (mv.sum() > 0)
+        if mv.sum():
+            assert a[2].numpy()[i] == v[mv].sum()
+
+
+def test_group_capacity_error(ctx):
+    g = ops.GroupBy(ctx, [C.TYPE_INT64], [(C.AGG_COUNT_ALL, None, False)], 10)
+    keys = np.arange(1000, dtype=np.int64)
+    g.consume([DeviceColumn.from_numpy(ctx, keys)], [None], 1000)
+    with pytest.raises(C.FpdbError) as e:
+        g.num_groups()
+    assert e.value.code == C.FPDB_ERR_CAPACITY
+
+
+def test_groupby_full_size_properties(ctx):
+    """60M-row Zipf(1.1) workload: count total, per-group integer sums against
+    dense bincounts (exact), FP64 sums of the 20 heaviest groups against the
+    sequential oracle within 1e-12."""
+    w = datagen.groupby_workload()
+    k, a = _run(ctx, w["key"], w["qty"], w["price"], w["disc"])
+    n = len(w["key"])
+    assert a[0].sum() == n and len(k) == len(np.unique(w["key"]))
+    key = w["key"].astype(np.int64)
+    cnt = np.bincount(key, minlength=100_000)
+    qty = np.bincount(key, weights=w["qty"], minlength=100_000)   # < 2^53, exact in float64
+    assert np.array_equal(a[0], cnt[k])
+    assert np.array_equal(a[1], qty[k].astype(np.int64))
+    heavy = np.argsort(-a[0])[:20]
+    for i in heavy[:5]:
+        m = key == k[i]
+        assert a[2][i] == int(w["price"][m].sum())
+        seq = O.groupby(np.zeros(int(m.sum()), np.uint64), sums_f64=[w["disc"][m]])["sum_f64"][0][0]
+        assert abs(a[3][i] - seq) <= REL_TOL * abs(seq)
+        assert abs(a[3][i] - math.fsum(w["disc"][m])) <= REL_TOL * abs(seq)

@@ -1,0 +1,173 @@
+"""Hash join on the GPU against the oracle: sorted join-output multisets must
+match exactly (the reference's join output order is unspecified)."""
+import numpy as np
+import pytest
+
+from flexpushdowndb_amd import _capi as C
+from flexpushdowndb_amd import datagen, ops
+from flexpushdowndb_amd.device import DeviceColumn
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(cols, n, masks=None):
+    arrs = [c.numpy(n) for c in cols]
+    if masks is not None:
+        arrs = [np.where(m, a, 0) if m is not None else a for a, m in zip(arrs, masks)]
+    return sorted(zip(*[a.tolist() for a in arrs]))
+
+
+def test_c1_shape_inner_join_closed_form(ctx):
+    w = datagen.join_workload(n_build=200_000, n_probe=1_200_000, seed=5)
+    bk = DeviceColumn.from_numpy(ctx, w["build_key"])
+    bp = DeviceColumn.from_numpy(ctx, w["build_pay"])
+    t = ops.HashJoinTable(ctx, [bk], len(w["build_key"]), payload=bp)
+    assert t.info()["unique"] and t.info()["inserted"] == 200_000
+    pf = DeviceColumn.from_numpy(ctx, w["probe_fk"])
+    pq = DeviceColumn.from_numpy(ctx, w["probe_qty"])
+    pp = DeviceColumn.from_numpy(ctx, w["probe_price"])
+    n = len(w["probe_fk"])
+    out, cnt = t.probe(C.JOIN_INNER, [pf], n, [pf, pq, pp], [bk, "payload"], capacity=n)
+    assert cnt == n
+    # output is in probe-row order and every probe row has exactly one match
+    inv = np.empty(200_000, np.int64)
+    inv[w["build_key"]] = np.arange(200_000)
+    assert np.array_equal(out[0].numpy(), w["probe_fk"])
+    assert np.array_equal(out[1].numpy(), w["probe_qty"])
+    assert np.array_equal(out[2].numpy(), w["probe_price"])
+    assert np.array_equal(out[3].numpy(), w["probe_fk"])
+    assert np.array_equal(out[4].numpy(), w["build_pay"][inv[w["probe_fk"]]])
+
+
+def test_inner_join_duplicates_misses_vs_oracle(ctx):
+    rng = np.random.default_rng(2)
+    bk = rng.integers(0, 5000, 20_000).astype(np.int64)
+    bv = rng.integers(-9, 9, 20_000).astype(np.int32)
+    pk = rng.integers(0, 8000, 30_001).astype(np.int64)
+    pv = rng.random(30_001)
+    dbk, dbv = DeviceColumn.from_numpy(ctx, bk), DeviceColumn.from_numpy(ctx, bv)
+    dpk, dpv = DeviceColumn.from_numpy(ctx, pk), DeviceColumn.from_numpy(ctx, pv)
+    t = ops.HashJoinTable(ctx, [dbk], len(bk))
+    assert not t.info()["unique"]
+    op, ob = O.join_pairs(bk, pk)
+    assert t.probe_count(C.JOIN_INNER, [dpk], len(pk)) == len(op)
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], len(pk), [dpk, dpv], [dbk, dbv])
+    assert cnt == len(op)
+    ref = sorted(zip(pk[op].tolist(), pv[op].tolist(), bk[ob].tolist(), bv[ob].tolist()))
+    assert _rows(out, cnt) == ref
+    # rows come out grouped in probe order
+    assert (np.diff(np.searchsorted(np.sort(op), op)) >= 0).all()
+    # capacity error reports the needed count
+    with pytest.raises(C.FpdbError) as e:
+        t.probe(C.JOIN_INNER, [dpk], len(pk), [dpk], [], capacity=10)
+    assert e.value.code == C.FPDB_ERR_CAPACITY
+
+
+def test_null_keys_never_match_and_int32_keys(ctx):
+    rng = np.random.default_rng(3)
+    bk = rng.integers(-100, 100, 3000).astype(np.int32)
+    bvalid = rng.random(3000) > 0.2
+    pk = rng.integers(-150, 150, 4000).astype(np.int64)
+    pvalid = rng.random(4000) > 0.2
+    dbk = DeviceColumn.from_numpy(ctx, bk, bvalid)
+    dpk = DeviceColumn.from_numpy(ctx, pk, pvalid)
+    t = ops.HashJoinTable(ctx, [dbk], len(bk))
+    assert t.info()["inserted"] == bvalid.sum()
+    op, ob = O.join_pairs(bk.astype(np.int64), pk, np.packbits(bvalid, bitorder="little"),
+                          np.packbits(pvalid, bitorder="little"))
+    rows = DeviceColumn.from_numpy(ctx, np.arange(4000, dtype=np.int64))
+    brows = DeviceColumn.from_numpy(ctx, np.arange(3000, dtype=np.int64))
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], len(pk), [rows], [brows])
+    assert sorted(zip(out[0].numpy(cnt).tolist(), out[1].numpy(cnt).tolist())) == sorted(zip(op.tolist(), ob.tolist()))
+
+
+def test_two_column_keys(ctx):
+    rng = np.random.default_rng(8)
+    b1 = rng.integers(0, 50, 4000).astype(np.int32)
+    b2 = rng.integers(0, 50, 4000).astype(np.int32)
+    p1 = rng.integers(0, 60, 5000).astype(np.int32)
+    p2 = rng.integers(0, 60, 5000).astype(np.int32)
+    t = ops.HashJoinTable(ctx, [DeviceColumn.from_numpy(ctx, b1), DeviceColumn.from_numpy(ctx, b2)], 4000)
+    bidx = DeviceColumn.from_numpy(ctx, np.arange(4000, dtype=np.int64))
+    pidx = DeviceColumn.from_numpy(ctx, np.arange(5000, dtype=np.int64))
+    out, cnt = t.probe(C.JOIN_INNER, [DeviceColumn.from_numpy(ctx, p1), DeviceColumn.from_numpy(ctx, p2)], 5000,
+                       [pidx], [bidx])
+    comb_b = b1.astype(np.int64) * 1000 + b2
+    comb_p = p1.astype(np.int64) * 1000 + p2
+    op, ob = O.join_pairs(comb_b, comb_p)
+    assert sorted(zip(out[0].numpy(cnt).tolist(), out[1].numpy(cnt).tolist())) == sorted(zip(op.tolist(), ob.tolist()))
+
+
+def test_outer_and_semi_joins(ctx):
+    rng = np.random.default_rng(4)
+    bk = rng.integers(0, 300, 500).astype(np.int64)
+    pk = rng.integers(100, 400, 700).astype(np.int64)
+    dbk, dpk = DeviceColumn.from_numpy(ctx, bk), DeviceColumn.from_numpy(ctx, pk)
+    bidx = DeviceColumn.from_numpy(ctx, np.arange(500, dtype=np.int64))
+    pidx = DeviceColumn.from_numpy(ctx, np.arange(700, dtype=np.int64))
+    op, ob, bmatched = O.join_pairs(bk, pk, want_matched=True)
+    pmatched = np.zeros(700, bool)
+    pmatched[op] = True
+    inner = sorted(zip(op.tolist(), ob.tolist()))
+
+    # RIGHT: probe side kept, unmatched probe rows with a null build side
+    t = ops.HashJoinTable(ctx, [dbk], 500)
+    out, cnt = t.probe(C.JOIN_RIGHT, [dpk], 700, [pidx], [bidx])
+    assert cnt == len(op) + (~pmatched).sum()
+    p_out, b_out, b_valid = out[0].numpy(cnt), out[1].numpy(cnt), out[1].valid_mask(cnt)
+    assert sorted(zip(p_out[b_valid].tolist(), b_out[b_valid].tolist())) == inner
+    assert sorted(p_out[~b_valid].tolist()) == np.flatnonzero(~pmatched).tolist()
+
+    # RIGHT_SEMI: probe rows with a match, in probe order
+    out, cnt = t.probe(C.JOIN_RIGHT_SEMI, [dpk], 700, [pidx], [])
+    assert out[0].numpy(cnt).tolist() == np.flatnonzero(pmatched).tolist()
+
+    # LEFT: inner pairs + unmatched build rows
+    t2 = ops.HashJoinTable(ctx, [dbk], 500, track_build_matches=True)
+    out, cnt = t2.probe(C.JOIN_LEFT, [dpk], 700, [pidx], [bidx])
+    assert _rows(out, cnt) == inner
+    rows, nr = t2.build_side_rows(matched=False)
+    assert rows.numpy(nr).tolist() == np.flatnonzero(~bmatched).tolist()
+
+    # FULL: both sides' unmatched rows
+    t3 = ops.HashJoinTable(ctx, [dbk], 500, track_build_matches=True)
+    out, cnt = t3.probe(C.JOIN_FULL, [dpk], 700, [pidx], [bidx])
+    assert cnt == len(op) + (~pmatched).sum()
+    rows, nr = t3.build_side_rows(matched=False)
+    assert rows.numpy(nr).tolist() == np.flatnonzero(~bmatched).tolist()
+    g = ops.gather(ctx, [bidx], rows, nr)
+    assert g[0].numpy(nr).tolist() == np.flatnonzero(~bmatched).tolist()
+
+    # LEFT_SEMI: build rows with a match, in build order
+    t4 = ops.HashJoinTable(ctx, [dbk], 500, track_build_matches=True)
+    _, cnt = t4.probe(C.JOIN_LEFT_SEMI, [dpk], 700, [], [])
+    assert cnt == 0
+    rows, nr = t4.build_side_rows(matched=True)
+    assert rows.numpy(nr).tolist() == np.flatnonzero(bmatched).tolist()
+
+
+def test_empty_sides(ctx):
+    dpk = DeviceColumn.from_numpy(ctx, np.arange(100, dtype=np.int64))
+    empty = DeviceColumn.from_numpy(ctx, np.zeros(0, np.int64))
+    t = ops.HashJoinTable(ctx, [empty], 0)
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], 100, [dpk], [])
+    assert cnt == 0
+    out, cnt = t.probe(C.JOIN_RIGHT, [dpk], 100, [dpk], [empty])
+    assert cnt == 100 and not out[1].valid_mask(100).any()
+    t2 = ops.HashJoinTable(ctx, [dpk], 100)
+    out, cnt = t2.probe(C.JOIN_INNER, [empty], 0, [empty], [dpk])
+    assert cnt == 0
+
+
+def test_attic_join_vector(ctx):
+    # fpdb-executor/ATTIC/test/JoinTest.cpp:82-123: a join b on aa = ba -> 4 rows
+    a = np.array([[10, 13, 16], [10, 14, 17], [12, 15, 18]], np.int64)
+    b = np.array([[10, 23, 26], [11, 24, 27], [10, 25, 28]], np.int64)
+    bc = [DeviceColumn.from_numpy(ctx, b[:, i].copy()) for i in range(3)]
+    ac = [DeviceColumn.from_numpy(ctx, a[:, i].copy()) for i in range(3)]
+    t = ops.HashJoinTable(ctx, [bc[0]], 3)
+    out, cnt = t.probe(C.JOIN_INNER, [ac[0]], 3, ac, bc)
+    assert cnt == 4
+    assert _rows(out, cnt) == sorted([(10, 13, 16, 10, 23, 26), (10, 13, 16, 10, 25, 28),
+                                      (10, 14, 17, 10, 23, 26), (10, 14, 17, 10, 25, 28)])

@@ -1,0 +1,287 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Each check is chosen so that a plausible restatement mistake (wrong seeding,
+wrong draw composition, an off-by-one in sizing, a transposed operand) fails:
+independent re-implementations (numpy's MT19937 engine fed by a Python
+restatement of std::seed_seq), standard known answers, closed forms,
+invariants and brute force on tiny inputs.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+M32 = 0xFFFFFFFF
+
+
+def seed_seq_generate(v, n):
+    """std::seed_seq::generate ([rand.util.seedseq]) restated."""
+    s = len(v)
+    b = [0x8B8B8B8B] * n
+    t = 11 if n >= 623 else 7 if n >= 68 else 5 if n >= 39 else 3 if n >= 7 else (n - 1) // 2
+    p = (n - t) // 2
+    q = p + t
+    m = max(s + 1, n)
+    T = lambda x: x ^ (x >> 27)
+    for k in range(m):
+        r1 = (1664525 * T(b[k % n] ^ b[(k + p) % n] ^ b[(k - 1) % n])) & M32
+        r2 = (r1 + (s if k == 0 else (k % n + v[k - 1] if k <= s else k % n))) & M32
+        b[(k + p) % n] = (b[(k + p) % n] + r1) & M32
+        b[(k + q) % n] = (b[(k + q) % n] + r2) & M32
+        b[k % n] = r2
+    for k in range(m, m + n):
+        r3 = (1566083941 * T((b[k % n] + b[(k + p) % n] + b[(k - 1) % n]) & M32)) & M32
+        r4 = (r3 - (k % n)) & M32
+        b[(k + p) % n] ^= r3
+        b[(k + q) % n] ^= r4
+        b[k % n] = r4
+    return b
+
+
+def numpy_engine():
+    key = np.array(seed_seq_generate([0] * 8, 624), dtype=np.uint32)
+    bg = np.random.MT19937(0)
+    bg.state = {"bit_generator": "MT19937", "state": {"key": key, "pos": 624}}
+    return bg
+
+
+def independent_masks():
+    """BloomFilterMasks() (fpdb-tuple/src/arrow/exec/BloomFilter.cpp:16-79)
+    driven by numpy's MT19937; a full-range uniform_int_distribution<uint64_t>
+    draw on a 32-bit engine is (first << 32) | second."""
+    bg = numpy_engine()
+
+    def u64():
+        a, b = (int(x) for x in bg.random_raw(2))
+        return (a << 32) | b
+
+    def rnd(lo, hi):
+        return lo + u64() % (hi - lo + 1)
+
+    bits = [0] * (1024 + 64)
+    nset = rnd(4, 5)
+    for _ in range(nset):
+        while True:
+            pos = rnd(0, 56)
+            if not bits[pos]:
+                bits[pos] = 1
+                break
+    for i in range(57, 1024 + 57 - 1):
+        leaving = bits[i - 57]
+        if leaving == 1 and nset == 4:
+            bits[i] = 1
+            continue
+        if leaving == 0 and nset == 5:
+            continue
+        if rnd(0, 57 * 2 - 1) < 9:
+            bits[i] = 1
+            if leaving == 0:
+                nset += 1
+        else:
+            if leaving == 1:
+                nset -= 1
+    return np.packbits(np.array(bits[:1088], np.uint8), bitorder="little")[:136]
+
+
+def test_mt19937_known_answer():
+    # [rand.predef]: the 10000th output of a default-constructed mt19937 is 4123659995
+    out = np.zeros(10000, np.uint32)
+    O.lib().orc_mt19937_raw(1, out.ctypes.data, 10000)
+    assert int(out[-1]) == 4123659995
+
+
+def test_seeded_engine_matches_independent_restatement():
+    out = np.zeros(2000, np.uint32)
+    O.lib().orc_mt19937_raw(0, out.ctypes.data, 2000)
+    assert np.array_equal(out, numpy_engine().random_raw(2000).astype(np.uint32))
+
+
+def test_uniform_u64_draw_composition():
+    out = np.zeros(100, np.uint64)
+    O.lib().orc_uniform_u64_draws(out.ctypes.data, 100)
+    raw = numpy_engine().random_raw(200).astype(np.uint64)
+    assert np.array_equal(out, (raw[0::2] << np.uint64(32)) | raw[1::2])
+
+
+def test_mask_table_matches_independent_generation_and_golden():
+    masks = O.bloom_masks()
+    assert np.array_equal(masks, independent_masks())
+    golden = bytes.fromhex(open(os.path.join(GOLDEN, "bloom_masks.hex")).read().strip())
+    assert masks.tobytes() == golden
+
+
+def test_mask_windows_have_4_or_5_bits():
+    # BloomFilter.h:74-75: every 57-bit window has between 4 and 5 bits set
+    table = O.mask_table_u64()
+    pop = np.array([bin(int(m)).count("1") for m in table])
+    assert pop.min() >= 4 and pop.max() <= 5
+    assert all(int(m) < (1 << 57) for m in table)
+
+
+@pytest.mark.parametrize("n,expected", [(0, 3), (1, 3), (64, 3), (65, 4), (400_000, 16),
+                                        (524_288, 16), (524_289, 17), (20_000_000, 22)])
+def test_blocked_sizing(n, expected):
+    # 2^ceil(log2(max(512, 8n))) bits = 2^(that - 6) blocks (BloomFilter.cpp:87-99)
+    assert O.blocked_log_num_blocks(n) == expected
+    bits = max(512, 8 * n)
+    assert expected + 6 == math.ceil(math.log2(bits))
+
+
+def _np_mask_block(h, lnb, table):
+    h = h.astype(np.uint64)
+    mid = (h & np.uint64(1023)).astype(np.int64)
+    rot = ((h >> np.uint64(10)) & np.uint64(63)).astype(np.uint64)
+    m = table[mid]
+    with np.errstate(over="ignore"):
+        rotated = (m << rot) | (m >> ((np.uint64(64) - rot) & np.uint64(63)))
+    rotated = np.where(rot == 0, m, rotated)
+    blk = ((h >> np.uint64(16)) & np.uint64((1 << lnb) - 1)).astype(np.int64)
+    return rotated, blk
+
+
+def test_blocked_insert_find_against_numpy_restatement():
+    rng = np.random.default_rng(7)
+    hashes = rng.integers(0, 2 ** 32, 5000, dtype=np.uint64).astype(np.uint32)
+    lnb, blocks = O.blocked_build(hashes)
+    table = O.mask_table_u64()
+    m, b = _np_mask_block(hashes, lnb, table)
+    ref = np.zeros(1 << lnb, np.uint64)
+    np.bitwise_or.at(ref, b, m)
+    assert np.array_equal(blocks, ref)
+    probes = rng.integers(0, 2 ** 32, 3001, dtype=np.uint64).astype(np.uint32)
+    got = np.unpackbits(O.blocked_find(lnb, blocks, probes), bitorder="little")[:3001].astype(bool)
+    pm, pb = _np_mask_block(probes, lnb, table)
+    assert np.array_equal(got, (blocks[pb] & pm) == pm)
+    # no false negatives
+    mine = np.unpackbits(O.blocked_find(lnb, blocks, hashes), bitorder="little")[:5000]
+    assert mine.all()
+
+
+def test_blocked_find_tail_bytes():
+    # FindImp writes ceil((n % 64) / 8) tail bytes; bits past n stay zero
+    hashes = np.arange(70, dtype=np.uint32)
+    lnb, blocks = O.blocked_build(hashes)
+    out = O.blocked_find(lnb, blocks, hashes)
+    assert len(out) == 9 and out[8] == 0x3F
+
+
+def test_arrow6_hash_properties():
+    # parity unpinned against the reference binary (Arrow 6 source absent);
+    # these only fix the restated formula's structure
+    vals = np.array([0, 1, 2, 0x1234, -1], np.int64)
+    h = O.arrow6_hash([(vals, None)])
+    assert h[0] == 0
+    c = 0xC2B2AE3D27D4EB4F
+    for v, hv in zip(vals, h):
+        p = (int(v) & (2 ** 64 - 1)) * c % 2 ** 64
+        assert int(hv) == int.from_bytes(p.to_bytes(8, "little")[::-1], "little") & M32
+    # int32 keys are zero-extended 4-byte keys
+    h32 = O.arrow6_hash([(np.array([-1], np.int32), None)])
+    assert h32[0] == O.arrow6_hash([(np.array([0xFFFFFFFF], np.int64), None)])[0]
+    # null -> 0, and the second column is folded in with the hash_combine step
+    valid = np.packbits(np.array([1, 0, 1, 1, 1], bool), bitorder="little")
+    hn = O.arrow6_hash([(vals, valid)])
+    assert hn[1] == 0 and hn[0] == h[0]
+    two = O.arrow6_hash([(vals, None), (vals[::-1].copy(), None)])
+    hb = O.arrow6_hash([(vals[::-1].copy(), None)])
+    exp = [(int(a) ^ ((int(b) + 0x9E3779B9 + (int(a) << 6) + (int(a) >> 2)) & M32)) & M32 for a, b in zip(h, hb)]
+    assert [int(x) for x in two] == exp
+
+
+def test_legacy_k_m_closed_forms():
+    assert O.legacy_k(0.01) == 7
+    assert O.legacy_m(400_000, 0.01) == math.ceil(400_000 * abs(math.log(0.01)) / math.log(2) ** 2)
+    assert O.legacy_m(400_000, 0.01) == 3_834_024
+    assert O.legacy_k(0.5) == 1 and O.legacy_m(0, 0.01) == 0
+
+
+def test_legacy_key_hash_combine_worked_example():
+    # hashCombine({v}) with seed 1: 1 ^ (v + 0x9e3779b9 + 64)
+    v = np.array([0, 5, -3], np.int64)
+    got = O.legacy_keys([v])
+    exp = [((1 ^ ((int(x) + 0x9E3779B9 + 64) & (2 ** 64 - 1))) + 2 ** 63) % 2 ** 64 - 2 ** 63 for x in v]
+    assert [int(x) for x in got] == exp
+    # int32 columns are sign-extended like std::hash<int> -> size_t
+    assert O.legacy_keys([v.astype(np.int32)])[2] == got[2]
+
+
+def test_primes_against_trial_division():
+    def isp(n):
+        return n >= 2 and all(n % d for d in range(2, int(n ** 0.5) + 1))
+    for n in range(0, 3000):
+        assert O.is_prime(n) == isp(n)
+    for n in [3_834_024, 7_668_048, 10 ** 9, 2 ** 31]:
+        p = O.next_prime(n)
+        assert isp(p) and all(not isp(k) for k in range(n, p))
+
+
+def test_legacy_filter_against_bigint_arithmetic():
+    params = O.legacy_params(123, 3_834_024, 7)
+    assert (params[:, 2] == 3_834_024).all()
+    assert all(O.is_prime(p) and 3_834_024 <= p for p in params[:, 3])
+    assert ((params[:, 0] >= 1) & (params[:, 0] < params[:, 3])).all()
+    keys = O.legacy_keys([np.arange(-50, 450, dtype=np.int64)])
+    words = O.legacy_build(params, 3_834_024, keys)
+    ref = np.zeros_like(words)
+    for x in keys:
+        for a, b, m, p in params:
+            s = (int(a) * int(x) + int(b)) % 2 ** 64
+            h = (s % int(p)) % int(m)
+            ref[h // 64] |= np.int64(np.uint64(1 << (h % 64)).view(np.int64))
+    assert np.array_equal(words, ref)
+    sel = np.unpackbits(O.legacy_find(params, 500, words, keys), bitorder="little")[:500]
+    assert sel.all()
+    assert not O.legacy_find(params, 0, words, keys).any()
+
+
+def test_join_closed_form_and_brute_force():
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(1000).astype(np.int64)
+    fk = rng.integers(0, 1000, 5000).astype(np.int64)
+    op, ob = O.join_pairs(perm, fk)
+    inv = np.empty(1000, np.int64)
+    inv[perm] = np.arange(1000)
+    assert np.array_equal(op, np.arange(5000)) and np.array_equal(ob, inv[fk])
+    # duplicates and misses, brute force
+    bk = rng.integers(0, 30, 200).astype(np.int64)
+    pk = rng.integers(0, 40, 300).astype(np.int64)
+    op, ob = O.join_pairs(bk, pk)
+    brute = sorted((p, b) for p in range(300) for b in range(200) if pk[p] == bk[b])
+    assert sorted(zip(op.tolist(), ob.tolist())) == brute
+
+
+def test_join_attic_vector():
+    # fpdb-executor/ATTIC/test/JoinTest.cpp:82-123 with data/join/{a,b}.csv:
+    # a(aa,ab,ac) join b(ba,bb,bc) on aa = ba -> 4 rows, all with aa = 10
+    a = np.array([[10, 13, 16], [10, 14, 17], [12, 15, 18]], np.int64)
+    b = np.array([[10, 23, 26], [11, 24, 27], [10, 25, 28]], np.int64)
+    op, ob = O.join_pairs(b[:, 0], a[:, 0])
+    rows = sorted(tuple(a[p]) + tuple(b[q]) for p, q in zip(op, ob))
+    assert len(rows) == 4
+    assert rows == sorted([(10, 13, 16, 10, 23, 26), (10, 13, 16, 10, 25, 28),
+                           (10, 14, 17, 10, 23, 26), (10, 14, 17, 10, 25, 28)])
+
+
+def test_groupby_closed_form_and_attic_vector():
+    # GroupTest.cpp:137-150: sum(ab) group by aa over data/group/a.csv -> 10:27, 12:15
+    r = O.groupby(np.array([10, 10, 12], np.uint64), sums_i64=[np.array([13, 14, 15])])
+    assert r["keys"].tolist() == [10, 12] and r["sum_i64"][0].tolist() == [27, 15]
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 500, 20000).astype(np.uint64)
+    vals = rng.integers(-1000, 1000, 20000)
+    fv = rng.random(20000)
+    r = O.groupby(keys, sums_i64=[vals], sums_f64=[fv])
+    # first-appearance order
+    _, first = np.unique(keys, return_index=True)
+    assert np.array_equal(r["keys"], keys[np.sort(first)])
+    dense_cnt = np.bincount(keys.astype(np.int64), minlength=500)
+    dense_sum = np.bincount(keys.astype(np.int64), weights=vals, minlength=500).astype(np.int64)
+    k = r["keys"].astype(np.int64)
+    assert np.array_equal(r["count"], dense_cnt[k]) and np.array_equal(r["sum_i64"][0], dense_sum[k])
+    for j, g in enumerate(k[:50]):
+        exact = math.fsum(fv[keys == g])
+        assert abs(r["sum_f64"][0][j] - exact) <= 1e-12 * abs(exact)
