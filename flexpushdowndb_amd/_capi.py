@@ -97,6 +97,7 @@ SIGNATURES = [
     ("fpdb_bloom_legacy_make_hash_functions", _I32, [ctypes.c_uint32, _I64, _I, ctypes.POINTER(UniversalHash)]),
     ("fpdb_bloom_legacy_build", _I32, [_P, _PCOL, _I, _I64, _P, ctypes.POINTER(UniversalHash), _I, _I64, _P]),
     ("fpdb_bloom_legacy_find", _I32, [_P, ctypes.POINTER(UniversalHash), _I, _I64, _I64, _P, _PCOL, _I, _I64, _P, _P]),
+    ("fpdb_compare_bitmap", _I32, [_P, _PCOL, _I, _I64, ctypes.c_double, _P, _I64, _P, _P]),
     ("fpdb_filter_compact", _I32, [_P, _PCOL, _I, _I64, _P, _PCOL, _PI64]),
     ("fpdb_bloom_blocked_filter", _I32, [_P, _P, _I, _PCOL, _I, ctypes.POINTER(_I), _I, _I64, _P, _PCOL, _PI64]),
     ("fpdb_hj_build", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I, ctypes.POINTER(_P)]),
@@ -107,6 +108,7 @@ SIGNATURES = [
     ("fpdb_hj_probe_count", _I32, [_P, _P, _I, _PCOL, _I, _I64, _PI64]),
     ("fpdb_hj_build_side_rows", _I32, [_P, _P, _I, _P, _I64, _PI64]),
     ("fpdb_gather", _I32, [_P, _PCOL, _I, _P, _I64, _PCOL]),
+    ("fpdb_copy_rows", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I64]),
     ("fpdb_groupby_create", _I32, [_P, ctypes.POINTER(_I32), _I, ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),
     ("fpdb_groupby_destroy", _I32, [_P]),
     ("fpdb_groupby_consume", _I32, [_P, _P, _PCOL, _PCOL, _I64]),

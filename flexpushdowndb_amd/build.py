@@ -78,10 +78,23 @@ def build_pop(jobs=8):
     return target
 
 
+def build_pop_tests():
+    """The operator-level C++ test program (run by tests/test_pop_gpu.py)."""
+    src = os.path.join(ROOT, "tests", "cpp", "pop_test.cpp")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "cpp", "pop_test")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT}/include", f"-I{POP}", src, "-o", out + ".tmp",
+          f"-L{LIB}", "-lfpdb_pop", "-lfpdb_amd", f"-Wl,-rpath,{LIB}", "-Wl,-rpath,$ORIGIN/../../flexpushdowndb_amd/lib"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     lib = build_library()
     pop = build_pop()
-    return lib, pop
+    test = build_pop_tests()
+    return lib, pop, test
 
 
 if __name__ == "__main__":

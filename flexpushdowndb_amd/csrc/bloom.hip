@@ -247,6 +247,57 @@ __global__ __launch_bounds__(256) void legacy_find_kernel(LegacyParams P, int64_
                            (unsigned long long)count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// column <op> constant -> selection bitmap (nulls never selected), optionally
+// AND-ed with an existing bitmap.
+__global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, int64_t ci, double cf,
+                                                      const uint8_t* __restrict__ and_bm, int64_t n,
+                                                      uint8_t* __restrict__ bitmap,
+                                                      int64_t* __restrict__ num_selected) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t nchunks = (n + 63) >> 6;
+  const bool is_f = col.type[0] == FPDB_TYPE_FLOAT64 || col.type[0] == FPDB_TYPE_FLOAT32;
+  const bool is_s = col.type[0] == FPDB_TYPE_INT8 || col.type[0] == FPDB_TYPE_INT16 ||
+                    col.type[0] == FPDB_TYPE_INT32 || col.type[0] == FPDB_TYPE_INT64 ||
+                    col.type[0] == FPDB_TYPE_DATE32 || col.type[0] == FPDB_TYPE_DATE64;
+  uint64_t count = 0;
+  for (int64_t c = wave; c < nchunks; c += nwaves) {
+    int64_t r = c * 64 + lane;
+    bool sel = false;
+    if (r < n) {
+      int64_t i = r + col.offset[0];
+      int cmp;
+      if (is_f) {
+        double v = col.type[0] == FPDB_TYPE_FLOAT64 ? reinterpret_cast<const double*>(col.values[0])[i]
+                                                   : (double)reinterpret_cast<const float*>(col.values[0])[i];
+        cmp = v < cf ? -1 : (v > cf ? 1 : (v == cf ? 0 : 2));
+      } else if (is_s) {
+        int64_t v = load_sext(col.values[0], col.width[0], i);
+        cmp = v < ci ? -1 : (v > ci ? 1 : 0);
+      } else {
+        uint64_t v = load_zext(col.values[0], col.width[0], i);
+        cmp = v < (uint64_t)ci ? -1 : (v > (uint64_t)ci ? 1 : 0);
+      }
+      switch (op) {
+        case FPDB_CMP_LT: sel = cmp == -1; break;
+        case FPDB_CMP_LE: sel = cmp == -1 || cmp == 0; break;
+        case FPDB_CMP_GT: sel = cmp == 1; break;
+        case FPDB_CMP_GE: sel = cmp == 1 || cmp == 0; break;
+        case FPDB_CMP_EQ: sel = cmp == 0; break;
+        default: sel = cmp != 0; break;
+      }
+      sel = sel && valid_bit(col.validity[0], i) && (and_bm == nullptr || bitmap_bit(and_bm, r));
+    }
+    uint64_t word = __ballot(sel);
+    write_bitmap_word(bitmap, c, word, n, lane);
+    count += __popcll(word);
+  }
+  if (num_selected && lane == 0 && count)
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(num_selected), (unsigned long long)count,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // blocked-filter predicate for the fused find + compaction
 template <int MODE>
 struct PredBlocked {
@@ -476,6 +527,23 @@ fpdb_status fpdb_bloom_legacy_find(fpdb_context* ctx, const fpdb_universal_hash*
   hipLaunchKernelGGL(legacy_find_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, P,
                      capacity, reinterpret_cast<const uint64_t*>(d_words), kc, num_rows, d_bitmap,
                      d_num_selected);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, int op, int64_t int_constant,
+                                double float_constant, const uint8_t* d_and_bitmap, int64_t num_rows,
+                                uint8_t* d_bitmap, int64_t* d_num_selected) {
+  FPDB_CHECK(ctx && column && d_bitmap, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(op >= FPDB_CMP_LT && op <= FPDB_CMP_NE, FPDB_ERR_INVALID, "unknown comparison");
+  Context* c = as_ctx(ctx);
+  ColSet cs;
+  FPDB_TRY(make_colset(column, 1, num_rows, &cs, "compare"));
+  if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
+  if (num_rows == 0) return FPDB_OK;
+  ProfScope ps(c, "compare");
+  hipLaunchKernelGGL(compare_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, cs, op,
+                     int_constant, float_constant, d_and_bitmap, num_rows, d_bitmap, d_num_selected);
   FPDB_HIP(hipGetLastError());
   return FPDB_OK;
 }

@@ -328,6 +328,7 @@ struct GOutSpec {
   int32_t agg_w[kMaxAggs];
   int32_t agg_count_acc[kMaxAggs];   // hidden valid-count accumulator, -1 if none
   int32_t agg_is_f32[kMaxAggs];      // MIN/MAX of float32: narrow back
+  int32_t agg_mean[kMaxAggs];        // 0: not a mean, 1: integer sum / count, 2: float sum / count
   int32_t num_keys, num_aggs;
 };
 
@@ -353,10 +354,15 @@ __global__ void groupby_emit_kernel(GlobalTable g, AccSpec A, const unsigned lon
         float fv = (float)__longlong_as_double((long long)v);
         v = (uint64_t)__float_as_uint(fv);
       }
+      uint64_t cnt = o.agg_count_acc[a] >= 0 ? g.acc[(size_t)o.agg_count_acc[a] * g.stride + s] : 1;
+      if (o.agg_mean[a] && cnt) {
+        double mean = o.agg_mean[a] == 1 ? (double)((long long)v / (long long)cnt)
+                                         : __longlong_as_double((long long)v) / (double)cnt;
+        v = (uint64_t)__double_as_longlong(mean);
+      }
       int64_t di = rank + o.agg_off[a];
       store_w(o.agg_values[a], o.agg_w[a], di, v);
-      bool valid = true;
-      if (o.agg_count_acc[a] >= 0) valid = g.acc[(size_t)o.agg_count_acc[a] * g.stride + s] != 0;
+      bool valid = cnt != 0;
       if (o.agg_validity[a] && valid)
         atomicOr(reinterpret_cast<unsigned int*>(o.agg_validity[a]) + (di >> 5), 1u << (di & 31));
     }
@@ -401,7 +407,8 @@ static int32_t acc_op_for(const fpdb_agg_spec& s) {
   switch (s.kind) {
     case FPDB_AGG_COUNT_ALL: return kAccCountAll;
     case FPDB_AGG_COUNT_VALID: return kAccCountValid;
-    case FPDB_AGG_SUM: return f ? kAccSumF : kAccSumI;
+    case FPDB_AGG_SUM:
+    case FPDB_AGG_MEAN: return f ? kAccSumF : kAccSumI;
     case FPDB_AGG_MIN: return f ? kAccMinF : (sg ? kAccMinI : kAccMinU);
     case FPDB_AGG_MAX: return f ? kAccMaxF : (sg ? kAccMaxI : kAccMaxU);
     default: return -1;
@@ -423,6 +430,7 @@ int32_t fpdb_groupby_output_type(fpdb_agg_spec s) {
       return type_is_signed_int(s.input_type) ? FPDB_TYPE_INT64 : FPDB_TYPE_UINT64;
     case FPDB_AGG_MIN:
     case FPDB_AGG_MAX: return s.input_type;
+    case FPDB_AGG_MEAN: return FPDB_TYPE_FLOAT64;
     default: return 0;
   }
 }
@@ -462,8 +470,9 @@ fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int
     t->hidden_of[a] = -1;
   }
   for (int a = 0; a < num_aggs; ++a) {
-    bool needs = aggs[a].input_nullable && (aggs[a].kind == FPDB_AGG_SUM || aggs[a].kind == FPDB_AGG_MIN ||
-                                            aggs[a].kind == FPDB_AGG_MAX);
+    bool needs = aggs[a].kind == FPDB_AGG_MEAN ||
+                 (aggs[a].input_nullable && (aggs[a].kind == FPDB_AGG_SUM || aggs[a].kind == FPDB_AGG_MIN ||
+                                             aggs[a].kind == FPDB_AGG_MAX));
     if (!needs) continue;
     if (nacc >= kMaxAggs) {
       delete t;
@@ -630,6 +639,7 @@ fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* t, fpdb_c
     o.agg_count_acc[a] = t->hidden_of[a];
     o.agg_is_f32[a] = (t->aggs[a].input_type == FPDB_TYPE_FLOAT32 &&
                        (t->aggs[a].kind == FPDB_AGG_MIN || t->aggs[a].kind == FPDB_AGG_MAX)) ? 1 : 0;
+    o.agg_mean[a] = t->aggs[a].kind != FPDB_AGG_MEAN ? 0 : (type_is_float(t->aggs[a].input_type) ? 2 : 1);
     if (out_aggs[a].validity) {
       FPDB_CHECK((out_aggs[a].offset & 31) == 0, FPDB_ERR_INVALID, "output validity offset must be a multiple of 32");
       FPDB_TRY(launch_fill(c, out_aggs[a].validity + out_aggs[a].offset / 8, 0, (uint64_t)((ng + 31) / 32) * 4));

@@ -323,6 +323,27 @@ __global__ void gather_kernel(ColSet in, const int64_t* __restrict__ rows, int64
   }
 }
 
+// Copies n rows of `in` into `out` starting at output row `dst0`; validity bits
+// are set or cleared individually, so any bit offset works.
+__global__ void copy_rows_kernel(ColSet in, int64_t n, OutColSet out, int64_t dst0) {
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (int c = 0; c < in.n; ++c) {
+      int64_t si = i + in.offset[c];
+      int64_t di = dst0 + i + out.offset[c];
+      store_w(out.values[c], out.width[c], di, load_zext(in.values[c], in.width[c], si));
+      if (out.validity[c]) {
+        uint32_t* w = reinterpret_cast<uint32_t*>(out.validity[c]) + (di >> 5);
+        uint32_t bit = 1u << (di & 31);
+        if (valid_bit(in.validity[c], si))
+          __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+          __hip_atomic_fetch_and(w, ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+}
+
 }  // namespace fpdb
 
 struct fpdb_hash_table {
@@ -644,6 +665,28 @@ fpdb_status fpdb_hj_build_side_rows(fpdb_context* ctx, fpdb_hash_table* t, int m
   std::memset(&onone, 0, sizeof(onone));
   return run_compact<PredByteFlag, false>(c, PredByteFlag{t->matched, (uint8_t)(matched ? 1 : 0)}, none,
                                           onone, t->build_rows, nullptr, d_rows, num_out);
+}
+
+fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_columns, int64_t num_rows,
+                           fpdb_column* out, int64_t out_row) {
+  FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
+  FPDB_CHECK(out_row >= 0, FPDB_ERR_INVALID, "copy rows: negative output row");
+  Context* c = as_ctx(ctx);
+  ColSet ic;
+  OutColSet oc;
+  FPDB_TRY(make_colset(in, num_columns, num_rows, &ic, "copy rows"));
+  FPDB_TRY(make_outcolset(out, num_columns, &oc, "copy rows"));
+  for (int i = 0; i < num_columns; ++i) {
+    FPDB_CHECK(ic.width[i] == oc.width[i], FPDB_ERR_INVALID, "copy rows: output width differs");
+    FPDB_CHECK(out[i].length >= out_row + num_rows, FPDB_ERR_INVALID, "copy rows: output column too short");
+    FPDB_CHECK(oc.validity[i] || !ic.validity[i], FPDB_ERR_INVALID,
+               "copy rows: nullable input needs an output validity bitmap");
+  }
+  if (num_rows == 0) return FPDB_OK;
+  hipLaunchKernelGGL(copy_rows_kernel, dim3(grid_for(c, num_rows, 256)), dim3(256), 0, c->stream, ic, num_rows,
+                     oc, out_row);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
 }
 
 fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns, const int64_t* d_rows,

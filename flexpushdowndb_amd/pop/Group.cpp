@@ -1,0 +1,161 @@
+#include "Group.h"
+
+namespace fpdb::amd {
+
+GroupArrowKernel::GroupArrowKernel(std::vector<std::string> groupColumnNames,
+                                   std::vector<std::shared_ptr<AggregateFunction>> aggregateFunctions)
+    : groupColumnNames_(std::move(groupColumnNames)), aggregateFunctions_(std::move(aggregateFunctions)) {}
+
+GroupArrowKernel::~GroupArrowKernel() { clear(); }
+
+void GroupArrowKernel::clear() {
+  if (table_) fpdb_groupby_destroy(table_);
+  table_ = nullptr;
+  inputs_.clear();
+}
+
+Expected<void> GroupArrowKernel::makeOutputSchema(const Schema& in) {
+  if (outputSchema_.has_value()) return {};
+  Schema out;
+  keyTypes_.clear();
+  specs_.clear();
+  for (auto& g : groupColumnNames_) {
+    int i = in.getFieldIndex(g);
+    if (i < 0) return make_unexpected("Group field '" + g + "' not found in input schema");
+    out.fields.push_back(in.fields[i]);
+    keyTypes_.push_back(in.fields[i].type);
+  }
+  for (auto& f : aggregateFunctions_) {
+    fpdb_agg_spec s{};
+    s.kind = f->kind();
+    if (!f->getInputColumnName().empty()) {
+      int i = in.getFieldIndex(f->getInputColumnName());
+      if (i < 0) return make_unexpected("Aggregate input '" + f->getInputColumnName() + "' not found");
+      s.input_type = in.fields[i].type;
+      s.input_nullable = in.fields[i].nullable ? 1 : 0;
+    } else {
+      s.input_type = FPDB_TYPE_INT64;
+    }
+    specs_.push_back(s);
+    Field of;
+    of.name = f->getOutputColumnName();
+    of.type = (fpdb_type)fpdb_groupby_output_type(s);
+    of.nullable = s.kind != FPDB_AGG_COUNT_ALL && s.kind != FPDB_AGG_COUNT_VALID;
+    out.fields.push_back(of);
+  }
+  outputSchema_ = out;
+  return {};
+}
+
+Expected<void> GroupArrowKernel::recreate(int64_t maxGroups) {
+  if (table_) fpdb_groupby_destroy(table_);
+  table_ = nullptr;
+  maxGroups_ = maxGroups;
+  if (fpdb_groupby_create(defaultContext(), keyTypes_.data(), (int)keyTypes_.size(), specs_.data(),
+                          (int)specs_.size(), maxGroups_, &table_) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  return {};
+}
+
+Expected<void> GroupArrowKernel::consume(const std::shared_ptr<TupleSet>& ts) {
+  std::vector<fpdb_column> keys, vals;
+  for (auto& g : groupColumnNames_) keys.push_back(ts->column(ts->schema().getFieldIndex(g)).c());
+  for (auto& f : aggregateFunctions_) {
+    if (f->getInputColumnName().empty()) {
+      vals.push_back(fpdb_column{});
+    } else {
+      vals.push_back(ts->column(ts->schema().getFieldIndex(f->getInputColumnName())).c());
+    }
+  }
+  if (fpdb_groupby_consume(defaultContext(), table_, keys.data(), vals.data(), ts->numRows()) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  return {};
+}
+
+// GroupArrowKernel::group (GroupArrowKernel.cpp:16-39).  Batches are consumed as
+// they arrive; if the table turns out too small, it is recreated larger and
+// the batches seen so far are consumed again in arrival order.
+Expected<void> GroupArrowKernel::group(const std::shared_ptr<TupleSet>& tupleSet) {
+  auto r = makeOutputSchema(tupleSet->schema());
+  if (!r) return r;
+  if (tupleSet->numRows() == 0) return {};
+  if (!table_) {
+    r = recreate(std::max<int64_t>(maxGroups_, 1));
+    if (!r) return r;
+  }
+  inputs_.push_back(tupleSet);
+  r = consume(tupleSet);
+  if (!r) return r;
+  int64_t groups = 0;
+  fpdb_status s = fpdb_groupby_num_groups(defaultContext(), table_, &groups);
+  while (s == FPDB_ERR_CAPACITY) {
+    r = recreate(maxGroups_ * 8);
+    if (!r) return r;
+    for (auto& in : inputs_) {
+      r = consume(in);
+      if (!r) return r;
+    }
+    s = fpdb_groupby_num_groups(defaultContext(), table_, &groups);
+  }
+  if (s != FPDB_OK) return make_unexpected(fpdb_last_error());
+  return {};
+}
+
+Expected<std::shared_ptr<TupleSet>> GroupArrowKernel::finalise() {
+  if (!outputSchema_.has_value()) return make_unexpected("Output schema not made yet");
+  const Schema& os = *outputSchema_;
+  if (!table_) return TupleSet::make(os);   // no input rows
+  fpdb_context* ctx = defaultContext();
+  int64_t ng = 0;
+  if (fpdb_groupby_num_groups(ctx, table_, &ng) != FPDB_OK) return make_unexpected(fpdb_last_error());
+  std::vector<Column> cols;
+  std::vector<fpdb_column> outKeys, outAggs;
+  size_t nk = groupColumnNames_.size();
+  for (size_t i = 0; i < os.fields.size(); ++i) {
+    cols.push_back(Column::allocate(ctx, os.fields[i].type, ng, os.fields[i].nullable));
+    (i < nk ? outKeys : outAggs).push_back(cols.back().c());
+  }
+  int64_t written = 0;
+  if (fpdb_groupby_finalize(ctx, table_, outKeys.data(), outAggs.data(), &written) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  return TupleSet::make(os, cols);
+}
+
+GroupPOp::GroupPOp(const std::string& name, const std::vector<std::string>& projectColumnNames, int nodeId,
+                   const std::vector<std::string>& groupColumnNames,
+                   const std::vector<std::shared_ptr<AggregateFunction>>& aggregateFunctions)
+    : PhysicalOp(name, POpType::GROUP, projectColumnNames, nodeId), kernel_(groupColumnNames, aggregateFunctions) {}
+
+void GroupPOp::onReceive(const Envelope& msg) {
+  switch (msg.message().type()) {
+    case MessageType::START: break;
+    case MessageType::TUPLESET: onTupleSet(static_cast<const TupleSetMessage&>(msg.message())); break;
+    case MessageType::COMPLETE: onComplete(static_cast<const CompleteMessage&>(msg.message())); break;
+    default: ctx()->notifyError("Unrecognized message type " + msg.message().getTypeString());
+  }
+}
+
+void GroupPOp::onTupleSet(const TupleSetMessage& message) {
+  auto r = kernel_.group(message.tuples());
+  if (!r) ctx()->notifyError(r.error());
+}
+
+// GroupPOp.cpp:67-85: finalise, project, send, complete
+void GroupPOp::onComplete(const CompleteMessage&) {
+  if (!ctx()->isComplete() && ctx()->operatorMap().allComplete(POpRelationshipType::Producer)) {
+    auto grouped = kernel_.finalise();
+    if (!grouped) {
+      ctx()->notifyError(grouped.error());
+      return;
+    }
+    auto projected = (*grouped)->projectExist(getProjectColumnNames());
+    if (!projected) {
+      ctx()->notifyError(projected.error());
+      return;
+    }
+    ctx()->tell(std::make_shared<TupleSetMessage>(*projected, name_));
+    ctx()->notifyComplete();
+  }
+}
+
+}  // namespace fpdb::amd

@@ -1,0 +1,103 @@
+// Hash group-by operator: GroupPOp + GroupArrowKernel
+// (fpdb-executor/include/fpdb/executor/physical/group/GroupPOp.h:36-40,
+// GroupArrowKernel.h) and the aggregate functions it evaluates
+// (fpdb-executor/src/physical/aggregate/function/{Sum,Count,MinMax,Avg}.cpp).
+#pragma once
+
+#include <optional>
+
+#include "PhysicalOp.h"
+
+namespace fpdb::amd {
+
+enum AggregateFunctionType { SUM, COUNT, MIN_MAX, AVG };
+
+// An aggregate over one input column (the reference evaluates a gandiva
+// expression; here the expression is a column reference, empty for count(*)).
+class AggregateFunction {
+ public:
+  AggregateFunction(AggregateFunctionType type, std::string outputColumnName, std::string inputColumnName)
+      : type_(type), outputColumnName_(std::move(outputColumnName)), inputColumnName_(std::move(inputColumnName)) {}
+  virtual ~AggregateFunction() = default;
+  AggregateFunctionType getType() const { return type_; }
+  const std::string& getOutputColumnName() const { return outputColumnName_; }
+  const std::string& getInputColumnName() const { return inputColumnName_; }
+  // AGG_INPUT_<output> naming of AggregateFunction.cpp:199-201
+  std::string getAggregateInputColumnName() const { return "AGG_INPUT_" + outputColumnName_; }
+  virtual fpdb_agg_kind kind() const = 0;
+
+ protected:
+  AggregateFunctionType type_;
+  std::string outputColumnName_;
+  std::string inputColumnName_;
+};
+
+struct Sum : AggregateFunction {
+  Sum(std::string out, std::string in) : AggregateFunction(SUM, std::move(out), std::move(in)) {}
+  fpdb_agg_kind kind() const override { return FPDB_AGG_SUM; }
+};
+
+// count(*) when inputColumnName is empty (hash_count ALL), else ONLY_VALID
+// (Count.cpp:91-104).
+struct Count : AggregateFunction {
+  Count(std::string out, std::string in = "") : AggregateFunction(COUNT, std::move(out), std::move(in)) {}
+  fpdb_agg_kind kind() const override { return inputColumnName_.empty() ? FPDB_AGG_COUNT_ALL : FPDB_AGG_COUNT_VALID; }
+};
+
+struct MinMax : AggregateFunction {
+  MinMax(bool isMin, std::string out, std::string in)
+      : AggregateFunction(MIN_MAX, std::move(out), std::move(in)), isMin_(isMin) {}
+  fpdb_agg_kind kind() const override { return isMin_ ? FPDB_AGG_MIN : FPDB_AGG_MAX; }
+  bool isMin_;
+};
+
+struct Avg : AggregateFunction {
+  Avg(std::string out, std::string in) : AggregateFunction(AVG, std::move(out), std::move(in)) {}
+  fpdb_agg_kind kind() const override { return FPDB_AGG_MEAN; }
+};
+
+class GroupArrowKernel {
+ public:
+  GroupArrowKernel(std::vector<std::string> groupColumnNames,
+                   std::vector<std::shared_ptr<AggregateFunction>> aggregateFunctions);
+  ~GroupArrowKernel();
+  GroupArrowKernel(const GroupArrowKernel&) = delete;
+  GroupArrowKernel& operator=(const GroupArrowKernel&) = delete;
+
+  Expected<void> group(const std::shared_ptr<TupleSet>& tupleSet);
+  // [group columns..., aggregates in function order] (GroupArrowKernel::finalizeAvg,
+  // GroupArrowKernel.cpp:156-200), groups in first-appearance order.
+  Expected<std::shared_ptr<TupleSet>> finalise();
+  void clear();
+
+ private:
+  Expected<void> makeOutputSchema(const Schema& in);
+  Expected<void> consume(const std::shared_ptr<TupleSet>& ts);
+  Expected<void> recreate(int64_t maxGroups);
+
+  std::vector<std::string> groupColumnNames_;
+  std::vector<std::shared_ptr<AggregateFunction>> aggregateFunctions_;
+  std::optional<Schema> outputSchema_;
+  std::vector<int32_t> keyTypes_;
+  std::vector<fpdb_agg_spec> specs_;
+  std::vector<std::shared_ptr<TupleSet>> inputs_;   // kept to rebuild a table that overflowed
+  fpdb_group_table* table_ = nullptr;
+  int64_t maxGroups_ = 1 << 16;
+};
+
+class GroupPOp : public PhysicalOp {
+ public:
+  GroupPOp(const std::string& name, const std::vector<std::string>& projectColumnNames, int nodeId,
+           const std::vector<std::string>& groupColumnNames,
+           const std::vector<std::shared_ptr<AggregateFunction>>& aggregateFunctions);
+  std::string getTypeString() const override { return "GroupPOp"; }
+  void onReceive(const Envelope& msg) override;
+  void clear() override { kernel_.clear(); }
+
+ private:
+  void onTupleSet(const TupleSetMessage& msg);
+  void onComplete(const CompleteMessage& msg);
+  GroupArrowKernel kernel_;
+};
+
+}  // namespace fpdb::amd

@@ -1,0 +1,261 @@
+#include "TupleSet.h"
+
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+
+namespace fpdb::amd {
+
+void checkStatus(fpdb_status s, const char* what) {
+  if (s != FPDB_OK) throw std::runtime_error(std::string(what) + ": " + fpdb_last_error());
+}
+
+fpdb_context* defaultContext() {
+  static std::once_flag once;
+  static fpdb_context* ctx = nullptr;
+  std::call_once(once, [] { checkStatus(fpdb_context_create(0, 0, nullptr, &ctx), "fpdb_context_create"); });
+  return ctx;
+}
+
+DeviceBuffer::DeviceBuffer(fpdb_context* ctx, uint64_t bytes) : ctx_(ctx), bytes_(bytes) {
+  checkStatus(fpdb_device_alloc(ctx, bytes ? bytes : 1, &ptr_), "fpdb_device_alloc");
+}
+
+DeviceBuffer::~DeviceBuffer() {
+  if (ptr_) fpdb_device_free(ctx_, ptr_);
+}
+
+int typeWidth(fpdb_type t) {
+  switch (t) {
+    case FPDB_TYPE_INT8: case FPDB_TYPE_UINT8: return 1;
+    case FPDB_TYPE_INT16: case FPDB_TYPE_UINT16: return 2;
+    case FPDB_TYPE_INT32: case FPDB_TYPE_UINT32: case FPDB_TYPE_FLOAT32: case FPDB_TYPE_DATE32: return 4;
+    default: return 8;
+  }
+}
+
+std::string typeName(fpdb_type t) {
+  switch (t) {
+    case FPDB_TYPE_INT8: return "int8";
+    case FPDB_TYPE_INT16: return "int16";
+    case FPDB_TYPE_INT32: return "int32";
+    case FPDB_TYPE_INT64: return "int64";
+    case FPDB_TYPE_UINT8: return "uint8";
+    case FPDB_TYPE_UINT16: return "uint16";
+    case FPDB_TYPE_UINT32: return "uint32";
+    case FPDB_TYPE_UINT64: return "uint64";
+    case FPDB_TYPE_FLOAT32: return "float";
+    case FPDB_TYPE_FLOAT64: return "double";
+    case FPDB_TYPE_DATE32: return "date32";
+    case FPDB_TYPE_DATE64: return "date64";
+  }
+  return "?";
+}
+
+int Schema::getFieldIndex(const std::string& name) const {
+  for (size_t i = 0; i < fields.size(); ++i)
+    if (fields[i].name == name) return (int)i;
+  return -1;
+}
+
+std::string Schema::toString() const {
+  std::ostringstream o;
+  for (size_t i = 0; i < fields.size(); ++i)
+    o << (i ? ", " : "") << fields[i].name << ": " << typeName(fields[i].type);
+  return o.str();
+}
+
+static uint64_t validityBytes(int64_t n) { return (uint64_t)((n + 31) / 32) * 4; }
+
+fpdb_column Column::c() const {
+  fpdb_column col;
+  col.type = type;
+  col.reserved = 0;
+  col.length = length;
+  col.offset = 0;
+  col.values = values ? values->data() : nullptr;
+  col.validity = validity ? static_cast<uint8_t*>(validity->data()) : nullptr;
+  return col;
+}
+
+Column Column::allocate(fpdb_context* ctx, fpdb_type type, int64_t length, bool nullable) {
+  Column c;
+  c.type = type;
+  c.length = length;
+  c.values = std::make_shared<DeviceBuffer>(ctx, (uint64_t)length * typeWidth(type));
+  if (nullable) {
+    c.validity = std::make_shared<DeviceBuffer>(ctx, validityBytes(length));
+    checkStatus(fpdb_fill(ctx, c.validity->data(), 0, c.validity->size()), "fpdb_fill");
+  }
+  return c;
+}
+
+Column Column::allNull(fpdb_context* ctx, fpdb_type type, int64_t length) {
+  Column c = allocate(ctx, type, length, true);
+  checkStatus(fpdb_fill(ctx, c.values->data(), 0, c.values->size()), "fpdb_fill");
+  return c;
+}
+
+int64_t HostColumn::int64At(int64_t i) const {
+  switch (type) {
+    case FPDB_TYPE_INT8: return (int8_t)bytes[i];
+    case FPDB_TYPE_UINT8: return bytes[i];
+    case FPDB_TYPE_INT16: { int16_t v; std::memcpy(&v, &bytes[i * 2], 2); return v; }
+    case FPDB_TYPE_UINT16: { uint16_t v; std::memcpy(&v, &bytes[i * 2], 2); return v; }
+    case FPDB_TYPE_INT32: case FPDB_TYPE_DATE32: { int32_t v; std::memcpy(&v, &bytes[i * 4], 4); return v; }
+    case FPDB_TYPE_UINT32: { uint32_t v; std::memcpy(&v, &bytes[i * 4], 4); return v; }
+    case FPDB_TYPE_FLOAT32: { float v; std::memcpy(&v, &bytes[i * 4], 4); return (int64_t)v; }
+    case FPDB_TYPE_FLOAT64: { double v; std::memcpy(&v, &bytes[i * 8], 8); return (int64_t)v; }
+    default: { int64_t v; std::memcpy(&v, &bytes[i * 8], 8); return v; }
+  }
+}
+
+double HostColumn::doubleAt(int64_t i) const {
+  if (type == FPDB_TYPE_FLOAT64) { double v; std::memcpy(&v, &bytes[i * 8], 8); return v; }
+  if (type == FPDB_TYPE_FLOAT32) { float v; std::memcpy(&v, &bytes[i * 4], 4); return v; }
+  return (double)int64At(i);
+}
+
+TupleSet::TupleSet(Schema schema, std::vector<Column> columns)
+    : schema_(std::move(schema)), columns_(std::move(columns)) {
+  numRows_ = columns_.empty() ? 0 : columns_[0].length;
+}
+
+std::shared_ptr<TupleSet> TupleSet::make(const Schema& schema) {
+  std::vector<Column> cols;
+  for (auto& f : schema.fields) {
+    Column c;
+    c.type = f.type;
+    c.length = 0;
+    cols.push_back(c);
+  }
+  return std::make_shared<TupleSet>(schema, cols);
+}
+
+std::shared_ptr<TupleSet> TupleSet::make(const Schema& schema, std::vector<Column> columns) {
+  if (columns.size() != schema.fields.size())
+    throw std::runtime_error("TupleSet::make: column count differs from the schema");
+  return std::make_shared<TupleSet>(schema, std::move(columns));
+}
+
+std::shared_ptr<TupleSet> TupleSet::fromHost(fpdb_context* ctx, const Schema& schema,
+                                             const std::vector<HostColumn>& hcols) {
+  std::vector<Column> cols;
+  for (size_t i = 0; i < hcols.size(); ++i) {
+    const HostColumn& h = hcols[i];
+    bool nullable = !h.valid.empty();
+    Column c = Column::allocate(ctx, schema.fields[i].type, h.length, nullable);
+    checkStatus(fpdb_copy_to_device(ctx, c.values->data(), h.bytes.data(), h.bytes.size()), "copy");
+    if (nullable) {
+      std::vector<uint8_t> bits(validityBytes(h.length), 0);
+      for (int64_t r = 0; r < h.length; ++r)
+        if (h.valid[r]) bits[r >> 3] |= (uint8_t)(1u << (r & 7));
+      checkStatus(fpdb_copy_to_device(ctx, c.validity->data(), bits.data(), bits.size()), "copy");
+    }
+    cols.push_back(c);
+  }
+  return make(schema, cols);
+}
+
+Expected<Column> TupleSet::getColumnByName(const std::string& name) const {
+  int i = schema_.getFieldIndex(name);
+  if (i < 0) return make_unexpected("Column '" + name + "' does not exist");
+  return columns_[i];
+}
+
+// Device concatenation of two columns (values and validity bits).
+static Column concatColumns(fpdb_context* ctx, const Column& a, const Column& b) {
+  int64_t n = a.length + b.length;
+  bool nullable = a.validity || b.validity;
+  Column out = Column::allocate(ctx, a.type, n, nullable);
+  fpdb_column o = out.c();
+  fpdb_column ia = a.c(), ib = b.c();
+  checkStatus(fpdb_copy_rows(ctx, &ia, 1, a.length, &o, 0), "fpdb_copy_rows");
+  checkStatus(fpdb_copy_rows(ctx, &ib, 1, b.length, &o, a.length), "fpdb_copy_rows");
+  return out;
+}
+
+Expected<void> TupleSet::append(const std::shared_ptr<TupleSet>& other) {
+  if (other->numColumns() != numColumns())
+    return make_unexpected("TupleSet::append: column count differs");
+  if (other->numRows() == 0) return {};
+  if (numRows_ == 0) {
+    columns_ = other->columns_;
+    numRows_ = other->numRows_;
+    return {};
+  }
+  fpdb_context* ctx = defaultContext();
+  for (int i = 0; i < numColumns(); ++i) {
+    if (columns_[i].type != other->columns_[i].type)
+      return make_unexpected("TupleSet::append: column type differs for '" + schema_.fields[i].name + "'");
+    try {
+      columns_[i] = concatColumns(ctx, columns_[i], other->columns_[i]);
+    } catch (const std::exception& e) {
+      return make_unexpected(e.what());
+    }
+  }
+  numRows_ += other->numRows_;
+  return {};
+}
+
+Expected<std::shared_ptr<TupleSet>> TupleSet::concatenate(const std::vector<std::shared_ptr<TupleSet>>& sets) {
+  if (sets.empty()) return make_unexpected("TupleSet::concatenate: no inputs");
+  auto out = std::make_shared<TupleSet>(sets[0]->schema(), sets[0]->columns());
+  for (size_t i = 1; i < sets.size(); ++i) {
+    auto r = out->append(sets[i]);
+    if (!r) return make_unexpected(r.error());
+  }
+  return out;
+}
+
+Expected<std::shared_ptr<TupleSet>> TupleSet::projectExist(const std::vector<std::string>& names) const {
+  Schema s;
+  std::vector<Column> cols;
+  for (auto& n : names) {
+    int i = schema_.getFieldIndex(n);
+    if (i < 0) continue;
+    s.fields.push_back(schema_.fields[i]);
+    cols.push_back(columns_[i]);
+  }
+  return std::make_shared<TupleSet>(s, cols);
+}
+
+std::vector<HostColumn> TupleSet::toHost() const {
+  fpdb_context* ctx = defaultContext();
+  std::vector<HostColumn> out;
+  for (auto& c : columns_) {
+    HostColumn h;
+    h.type = c.type;
+    h.length = c.length;
+    h.bytes.resize((size_t)c.length * typeWidth(c.type));
+    if (c.length)
+      checkStatus(fpdb_copy_to_host(ctx, h.bytes.data(), c.values->data(), h.bytes.size()), "copy");
+    if (c.validity && c.length) {
+      std::vector<uint8_t> bits((c.length + 7) / 8);
+      checkStatus(fpdb_copy_to_host(ctx, bits.data(), c.validity->data(), bits.size()), "copy");
+      h.valid.resize(c.length);
+      for (int64_t r = 0; r < c.length; ++r) h.valid[r] = (bits[r >> 3] >> (r & 7)) & 1;
+    }
+    out.push_back(std::move(h));
+  }
+  return out;
+}
+
+std::string TupleSet::showString(int64_t maxRows) const {
+  std::ostringstream o;
+  o << schema_.toString() << "\n";
+  auto host = toHost();
+  for (int64_t r = 0; r < std::min<int64_t>(numRows_, maxRows); ++r) {
+    for (size_t c = 0; c < host.size(); ++c) {
+      o << (c ? " | " : "");
+      if (!host[c].isValid(r)) o << "null";
+      else if (host[c].type == FPDB_TYPE_FLOAT64 || host[c].type == FPDB_TYPE_FLOAT32) o << host[c].doubleAt(r);
+      else o << host[c].int64At(r);
+    }
+    o << "\n";
+  }
+  return o.str();
+}
+
+}  // namespace fpdb::amd

@@ -1,0 +1,108 @@
+// Device-resident TupleSet: the in/out contract of the reference's physical
+// operators (fpdb-tuple/include/fpdb/tuple/TupleSet.h:31-192), with Arrow
+// column layout (values + LSB-first validity bitmap) kept in HBM.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "Expected.h"
+#include "fpdb_amd.h"
+
+namespace fpdb::amd {
+
+// Owns one device allocation made through an fpdb_context.
+class DeviceBuffer {
+ public:
+  DeviceBuffer(fpdb_context* ctx, uint64_t bytes);
+  ~DeviceBuffer();
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  void* data() const { return ptr_; }
+  uint64_t size() const { return bytes_; }
+  fpdb_context* ctx() const { return ctx_; }
+
+ private:
+  fpdb_context* ctx_;
+  void* ptr_ = nullptr;
+  uint64_t bytes_;
+};
+
+// Device context shared by the operators of one executor.
+fpdb_context* defaultContext();
+// Throws std::runtime_error with the library's message on a failed status.
+void checkStatus(fpdb_status s, const char* what);
+
+int typeWidth(fpdb_type t);
+std::string typeName(fpdb_type t);
+
+struct Field {
+  std::string name;
+  fpdb_type type;
+  bool nullable = false;
+};
+
+struct Schema {
+  std::vector<Field> fields;
+  int getFieldIndex(const std::string& name) const;
+  std::string toString() const;
+};
+
+// One column of a TupleSet: a single contiguous device chunk.
+struct Column {
+  fpdb_type type = FPDB_TYPE_INT64;
+  int64_t length = 0;
+  std::shared_ptr<DeviceBuffer> values;
+  std::shared_ptr<DeviceBuffer> validity;   // null: no nulls
+  fpdb_column c() const;
+
+  static Column allocate(fpdb_context* ctx, fpdb_type type, int64_t length, bool nullable);
+  static Column allNull(fpdb_context* ctx, fpdb_type type, int64_t length);
+};
+
+// Host copy of a column, for tests and display.
+struct HostColumn {
+  fpdb_type type;
+  std::vector<uint8_t> bytes;   // length * width
+  std::vector<bool> valid;      // empty: all valid
+  int64_t length = 0;
+  int64_t int64At(int64_t i) const;
+  double doubleAt(int64_t i) const;
+  bool isValid(int64_t i) const { return valid.empty() || valid[i]; }
+};
+
+class TupleSet {
+ public:
+  TupleSet(Schema schema, std::vector<Column> columns);
+
+  static std::shared_ptr<TupleSet> make(const Schema& schema);
+  static std::shared_ptr<TupleSet> make(const Schema& schema, std::vector<Column> columns);
+  // Uploads host columns (values as raw little-endian bytes).
+  static std::shared_ptr<TupleSet> fromHost(fpdb_context* ctx, const Schema& schema,
+                                            const std::vector<HostColumn>& columns);
+  static Expected<std::shared_ptr<TupleSet>> concatenate(const std::vector<std::shared_ptr<TupleSet>>& sets);
+
+  int64_t numRows() const { return numRows_; }
+  int numColumns() const { return (int)columns_.size(); }
+  const Schema& schema() const { return schema_; }
+  const std::vector<Column>& columns() const { return columns_; }
+  const Column& column(int i) const { return columns_[i]; }
+  Expected<Column> getColumnByName(const std::string& name) const;
+
+  // Appends another TupleSet with the same schema (a device copy).
+  Expected<void> append(const std::shared_ptr<TupleSet>& other);
+  // Keeps the named columns that exist, in the given order.
+  Expected<std::shared_ptr<TupleSet>> projectExist(const std::vector<std::string>& names) const;
+
+  std::vector<HostColumn> toHost() const;
+  std::string showString(int64_t maxRows = 20) const;
+
+ private:
+  Schema schema_;
+  std::vector<Column> columns_;
+  int64_t numRows_ = 0;
+};
+
+}  // namespace fpdb::amd

@@ -179,6 +179,18 @@ fpdb_status fpdb_bloom_legacy_find(fpdb_context* ctx, const fpdb_universal_hash*
 
 /* ---- filter compaction --------------------------------------------------- */
 
+/* Comparison of one column with a constant into a selection bitmap (nulls are
+ * not selected), AND-ed with d_and_bitmap when given.  Integer columns compare
+ * with int_constant, float columns with float_constant.  Stands in for the
+ * simple gandiva filters (column op literal) that feed the join and bloom
+ * operators (fpdb-executor/src/physical/filter/FilterPOp.cpp). */
+typedef enum {
+  FPDB_CMP_LT = 0, FPDB_CMP_LE = 1, FPDB_CMP_GT = 2, FPDB_CMP_GE = 3, FPDB_CMP_EQ = 4, FPDB_CMP_NE = 5
+} fpdb_cmp_op;
+fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, int op, int64_t int_constant,
+                                double float_constant, const uint8_t* d_and_bitmap, int64_t num_rows,
+                                uint8_t* d_bitmap, int64_t* d_num_selected);
+
 /* Keeps the rows whose bitmap bit is set, in order, for every column
  * (arrow::compute::Filter at BloomFilterUseKernel.cpp:195; gandiva
  * SelectionVector filtering at BloomFilterUseKernel.cpp:170-177).
@@ -279,6 +291,12 @@ fpdb_status fpdb_hj_probe_count(fpdb_context* ctx, fpdb_hash_table* table, int j
 fpdb_status fpdb_hj_build_side_rows(fpdb_context* ctx, fpdb_hash_table* table, int matched,
                                     int64_t* d_rows, int64_t capacity, int64_t* num_out);
 
+/* Copies num_rows rows of each input column into the output columns starting
+ * at output row out_row, validity bits included at any bit offset (the device
+ * side of TupleSet::append / concatenate, fpdb-tuple/src/TupleSet.cpp). */
+fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_columns, int64_t num_rows,
+                           fpdb_column* out, int64_t out_row);
+
 /* Gathers rows (int64 indices; -1 = null row) of columns into out (used to
  * materialise build-side rows and the null probe side of outer joins). */
 fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns,
@@ -291,7 +309,11 @@ fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_column
  * MinMax.cpp:84 hash_min / hash_max). */
 typedef enum {
   FPDB_AGG_COUNT_ALL = 0, FPDB_AGG_COUNT_VALID = 1, FPDB_AGG_SUM = 2,
-  FPDB_AGG_MIN = 3, FPDB_AGG_MAX = 4
+  FPDB_AGG_MIN = 3, FPDB_AGG_MAX = 4,
+  /* hash_sum / hash_count then "divide" and a cast to float64 (AvgBase::finalize,
+   * fpdb-executor/src/physical/aggregate/function/AvgBase.cpp:79-96): integer
+   * sums divide by the count with truncation, float sums divide exactly. */
+  FPDB_AGG_MEAN = 5
 } fpdb_agg_kind;
 
 typedef struct fpdb_agg_spec {
@@ -322,7 +344,8 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* table,
 fpdb_status fpdb_groupby_num_groups(fpdb_context* ctx, fpdb_group_table* table, int64_t* num_groups);
 
 /* Output type of an aggregate: COUNT -> INT64; SUM of signed integers ->
- * INT64, of unsigned -> UINT64, of floats -> FLOAT64; MIN/MAX -> input type. */
+ * INT64, of unsigned -> UINT64, of floats -> FLOAT64; MIN/MAX -> input type;
+ * MEAN -> FLOAT64. */
 int32_t fpdb_groupby_output_type(fpdb_agg_spec spec);
 
 /* Writes the groups in first-appearance order (the serial grouper's group-id

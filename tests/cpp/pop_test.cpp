@@ -1,0 +1,398 @@
+// Operator-level tests of the drop-in physical operators, in the shape of the
+// reference's operator tests (fpdb-executor/ATTIC/test/{JoinTest,GroupTest,
+// BloomJoinTest}.cpp): build a plan of POps, execute it, check the output.
+//
+//   pop_test unit                 built-in vectors and error paths
+//   pop_test starjoin N OUT.csv   predicate-transfer star join + group-by over
+//                                 N seeded fact rows; result rows to OUT.csv
+//                                 (checked against numpy by tests/test_pop_gpu.py)
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "BloomFilterPOps.h"
+#include "Globals.h"
+#include "Group.h"
+#include "HashJoin.h"
+#include "Plumbing.h"
+
+using namespace fpdb::amd;
+
+static int g_failures = 0;
+#define CHECK(cond)                                                                 \
+  do {                                                                              \
+    if (!(cond)) {                                                                  \
+      std::printf("  CHECK failed at line %d: %s\n", __LINE__, #cond);               \
+      ++g_failures;                                                                 \
+    }                                                                               \
+  } while (0)
+
+static HostColumn hostInt64(const std::vector<int64_t>& v) {
+  HostColumn h;
+  h.type = FPDB_TYPE_INT64;
+  h.length = (int64_t)v.size();
+  h.bytes.resize(v.size() * 8);
+  std::memcpy(h.bytes.data(), v.data(), h.bytes.size());
+  return h;
+}
+
+static HostColumn hostInt32(const std::vector<int32_t>& v) {
+  HostColumn h;
+  h.type = FPDB_TYPE_INT32;
+  h.length = (int64_t)v.size();
+  h.bytes.resize(v.size() * 4);
+  std::memcpy(h.bytes.data(), v.data(), h.bytes.size());
+  return h;
+}
+
+static std::shared_ptr<TupleSet> table3(const std::string& prefix, const std::vector<std::vector<int64_t>>& rows) {
+  Schema s;
+  std::vector<std::vector<int64_t>> cols(3);
+  for (auto& r : rows)
+    for (int c = 0; c < 3; ++c) cols[c].push_back(r[c]);
+  std::vector<HostColumn> hc;
+  for (int c = 0; c < 3; ++c) {
+    s.fields.push_back(Field{prefix + (char)('a' + c), FPDB_TYPE_INT64, false});
+    hc.push_back(hostInt64(cols[c]));
+  }
+  return TupleSet::fromHost(defaultContext(), s, hc);
+}
+
+static std::vector<std::vector<int64_t>> rowsOf(const std::shared_ptr<TupleSet>& ts) {
+  auto h = ts->toHost();
+  std::vector<std::vector<int64_t>> rows(ts->numRows());
+  for (int64_t r = 0; r < ts->numRows(); ++r)
+    for (auto& c : h) rows[r].push_back(c.isValid(r) ? c.int64At(r) : INT64_MIN);
+  return rows;
+}
+
+// ATTIC data/join/{a,b}.csv and data/group/a.csv
+static const std::vector<std::vector<int64_t>> kA = {{10, 13, 16}, {10, 14, 17}, {12, 15, 18}};
+static const std::vector<std::vector<int64_t>> kB = {{10, 23, 26}, {11, 24, 27}, {10, 25, 28}};
+
+static std::shared_ptr<TupleSet> runJoin(JoinType jt, bool bloomFirst) {
+  Executor ex;
+  auto a = std::make_shared<SourcePOp>("scan_a", std::vector<std::shared_ptr<TupleSet>>{table3("a", kA)});
+  auto b = std::make_shared<SourcePOp>("scan_b", std::vector<std::shared_ptr<TupleSet>>{table3("b", kB)});
+  auto join = std::make_shared<HashJoinArrowPOp>("join", std::vector<std::string>{"aa", "ab", "ac", "ba", "bb", "bc"}, 0,
+                                                 HashJoinPredicate{{"ba"}, {"aa"}}, jt);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  ex.put(a);
+  ex.put(b);
+  ex.put(join);
+  ex.put(collate);
+  if (bloomFirst) {
+    // BloomJoinTest: a filter on b.ba applied to a.aa before the join
+    auto bfc = std::make_shared<BloomFilterCreatePOp>("bf_create", std::vector<std::string>{}, 0,
+                                                      std::vector<std::string>{"ba"});
+    auto bfu = std::make_shared<BloomFilterUsePOp>("bf_use", std::vector<std::string>{}, 0,
+                                                   std::vector<std::string>{"aa"});
+    ex.put(bfc);
+    ex.put(bfu);
+    connect(b, bfc);
+    bfc->addBloomFilterUsePOp(bfu);
+    bfu->consume(bfc);
+    connect(a, bfu);
+    bfc->produce(join);
+    join->addBuildProducer(bfc);
+    bfu->produce(join);
+    join->addProbeProducer(bfu);
+  } else {
+    b->produce(join);
+    join->addBuildProducer(b);
+    a->produce(join);
+    join->addProbeProducer(a);
+  }
+  connect(join, collate);
+  auto r = ex.execute();
+  if (!r) {
+    std::printf("  execute failed: %s\n", r.error().c_str());
+    ++g_failures;
+    return nullptr;
+  }
+  return collate->tupleSet().value();
+}
+
+static void testJoin() {
+  // JoinTest.cpp:82-123: 4 rows, probe columns (a) first
+  auto out = runJoin(INNER, false);
+  if (!out) return;
+  CHECK(out->numRows() == 4 && out->numColumns() == 6);
+  CHECK(out->schema().fields[0].name == "aa" && out->schema().fields[3].name == "ba");
+  auto rows = rowsOf(out);
+  std::sort(rows.begin(), rows.end());
+  std::vector<std::vector<int64_t>> want = {{10, 13, 16, 10, 23, 26}, {10, 13, 16, 10, 25, 28},
+                                            {10, 14, 17, 10, 23, 26}, {10, 14, 17, 10, 25, 28}};
+  CHECK(rows == want);
+}
+
+static void testOuterSemiJoins() {
+  auto right = runJoin(RIGHT, false);    // probe side (a) kept: a row aa=12 with null b
+  if (right) {
+    auto rows = rowsOf(right);
+    CHECK(right->numRows() == 5);
+    int nulls = 0;
+    for (auto& r : rows) nulls += r[3] == INT64_MIN;
+    CHECK(nulls == 1);
+  }
+  auto left = runJoin(LEFT, false);      // build side (b) kept: b row ba=11 with null a
+  if (left) {
+    auto rows = rowsOf(left);
+    CHECK(left->numRows() == 5);
+    int found = 0;
+    for (auto& r : rows) found += (r[0] == INT64_MIN && r[3] == 11);
+    CHECK(found == 1);
+  }
+  auto full = runJoin(FULL, false);
+  if (full) CHECK(full->numRows() == 6);
+  auto lsemi = runJoin(LEFT_SEMI, false);   // build rows with a match: ba = 10 twice
+  if (lsemi) {
+    CHECK(lsemi->numRows() == 2 && lsemi->numColumns() == 3);
+    for (auto& r : rowsOf(lsemi)) CHECK(r[0] == 10);
+  }
+  auto rsemi = runJoin(RIGHT_SEMI, false);  // probe rows with a match: aa = 10 twice
+  if (rsemi) {
+    CHECK(rsemi->numRows() == 2 && rsemi->schema().fields[0].name == "aa");
+  }
+}
+
+static void testBloomJoin(bool arrowImpl) {
+  USE_ARROW_BLOOM_FILTER_IMPL = arrowImpl;
+  auto out = runJoin(INNER, true);
+  USE_ARROW_BLOOM_FILTER_IMPL = true;
+  if (!out) return;
+  CHECK(out->numRows() == 4);   // no false negatives: the join result is unchanged
+}
+
+static void testGroup() {
+  // GroupTest.cpp:137-150: sum(ab) group by aa -> 10: 27, 12: 15
+  Executor ex;
+  auto a = std::make_shared<SourcePOp>("scan", std::vector<std::shared_ptr<TupleSet>>{table3("a", kA)});
+  std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum", "ab"),
+                                                        std::make_shared<Count>("cnt"),
+                                                        std::make_shared<Avg>("avg", "ac")};
+  auto group = std::make_shared<GroupPOp>("group", std::vector<std::string>{"aa", "sum", "cnt", "avg"}, 0,
+                                          std::vector<std::string>{"aa"}, fns);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  ex.put(a);
+  ex.put(group);
+  ex.put(collate);
+  connect(a, group);
+  connect(group, collate);
+  auto r = ex.execute();
+  CHECK(r.has_value());
+  if (!r) return;
+  auto out = collate->tupleSet().value();
+  CHECK(out->numRows() == 2 && out->numColumns() == 4);
+  auto h = out->toHost();
+  CHECK(h[0].int64At(0) == 10 && h[0].int64At(1) == 12);
+  CHECK(h[1].int64At(0) == 27 && h[1].int64At(1) == 15);
+  CHECK(h[2].int64At(0) == 2 && h[2].int64At(1) == 1);
+  CHECK(h[3].doubleAt(0) == 16.0 && h[3].doubleAt(1) == 18.0);   // (16 + 17) / 2 truncates for ints
+}
+
+static void testEmptyAndErrors() {
+  // an empty build side still yields an (empty) result with the output schema
+  Executor ex;
+  Schema sb;
+  sb.fields = {Field{"ba", FPDB_TYPE_INT64, false}};
+  auto emptyB = TupleSet::make(sb);
+  auto a = std::make_shared<SourcePOp>("scan_a", std::vector<std::shared_ptr<TupleSet>>{table3("a", kA)});
+  auto b = std::make_shared<SourcePOp>("scan_b", std::vector<std::shared_ptr<TupleSet>>{emptyB});
+  auto join = std::make_shared<HashJoinArrowPOp>("join", std::vector<std::string>{"aa", "ba"}, 0,
+                                                 HashJoinPredicate{{"ba"}, {"aa"}}, INNER);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  ex.put(a);
+  ex.put(b);
+  ex.put(join);
+  ex.put(collate);
+  b->produce(join);
+  join->addBuildProducer(b);
+  a->produce(join);
+  join->addProbeProducer(a);
+  connect(join, collate);
+  auto r = ex.execute();
+  CHECK(r.has_value());
+  if (r) {
+    auto out = collate->tupleSet().value();
+    CHECK(out->numRows() == 0 && out->numColumns() == 2);
+  }
+  // a bad join column name is reported as an operator error, not a crash
+  Executor ex2;
+  auto a2 = std::make_shared<SourcePOp>("scan_a", std::vector<std::shared_ptr<TupleSet>>{table3("a", kA)});
+  auto b2 = std::make_shared<SourcePOp>("scan_b", std::vector<std::shared_ptr<TupleSet>>{table3("b", kB)});
+  auto j2 = std::make_shared<HashJoinArrowPOp>("join", std::vector<std::string>{"aa"}, 0,
+                                               HashJoinPredicate{{"nope"}, {"aa"}}, INNER);
+  auto c2 = std::make_shared<CollatePOp>("collate");
+  ex2.put(a2);
+  ex2.put(b2);
+  ex2.put(j2);
+  ex2.put(c2);
+  b2->produce(j2);
+  j2->addBuildProducer(b2);
+  a2->produce(j2);
+  j2->addProbeProducer(a2);
+  connect(j2, c2);
+  auto r2 = ex2.execute();
+  CHECK(!r2.has_value());
+  if (!r2) CHECK(r2.error().find("nope") != std::string::npos);
+}
+
+// ---- star join ---------------------------------------------------------------
+
+// SplitMix64 over seed ^ column*phi ^ row, as flexpushdowndb_amd/datagen.py
+static uint64_t splitmix(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+static int64_t uni(uint64_t seed, uint64_t col, int64_t row, int64_t lo, int64_t hi) {
+  uint64_t u = splitmix(seed ^ (col * 0x9E3779B97F4A7C15ULL) ^ (uint64_t)row);
+  return lo + (int64_t)(u % (uint64_t)(hi - lo + 1));
+}
+
+// fact(fk1, fk2, m) x dim1(k1, a1, g1) x dim2(k2, a2, g2):
+//   dims filtered by a1 < 30, a2 < 10; forward bloom filters dim -> fact,
+//   backward filter fact.fk1 -> dim1; joins; group by (g1, g2): sum(m), count(*).
+static int starJoin(int64_t n, const char* outPath) {
+  const int64_t d1 = 1000, d2 = 500;
+  const uint64_t seed = 77;
+  fpdb_context* ctx = defaultContext();
+  std::vector<int64_t> fk1(n), fk2(n), m(n), k1(d1), k2(d2);
+  std::vector<int32_t> a1(d1), g1(d1), a2(d2), g2(d2);
+  for (int64_t i = 0; i < n; ++i) {
+    fk1[i] = uni(seed, 1, i, 0, d1 - 1);
+    fk2[i] = uni(seed, 2, i, 0, d2 - 1);
+    m[i] = uni(seed, 3, i, 1, 100);
+  }
+  for (int64_t i = 0; i < d1; ++i) {
+    k1[i] = i;
+    a1[i] = (int32_t)uni(seed, 4, i, 0, 99);
+    g1[i] = (int32_t)uni(seed, 5, i, 0, 9);
+  }
+  for (int64_t i = 0; i < d2; ++i) {
+    k2[i] = i;
+    a2[i] = (int32_t)uni(seed, 6, i, 0, 99);
+    g2[i] = (int32_t)uni(seed, 7, i, 0, 4);
+  }
+  Schema sf{{Field{"fk1", FPDB_TYPE_INT64}, Field{"fk2", FPDB_TYPE_INT64}, Field{"m", FPDB_TYPE_INT64}}};
+  Schema s1{{Field{"k1", FPDB_TYPE_INT64}, Field{"a1", FPDB_TYPE_INT32}, Field{"g1", FPDB_TYPE_INT32}}};
+  Schema s2{{Field{"k2", FPDB_TYPE_INT64}, Field{"a2", FPDB_TYPE_INT32}, Field{"g2", FPDB_TYPE_INT32}}};
+  // fact arrives in batches of DefaultChunkSize-like size
+  std::vector<std::shared_ptr<TupleSet>> factBatches;
+  const int64_t chunk = 100000;
+  for (int64_t s = 0; s < n; s += chunk) {
+    int64_t e = std::min(n, s + chunk);
+    factBatches.push_back(TupleSet::fromHost(
+        ctx, sf,
+        {hostInt64({fk1.begin() + s, fk1.begin() + e}), hostInt64({fk2.begin() + s, fk2.begin() + e}),
+         hostInt64({m.begin() + s, m.begin() + e})}));
+  }
+  auto dim1 = TupleSet::fromHost(ctx, s1, {hostInt64(k1), hostInt32(a1), hostInt32(g1)});
+  auto dim2 = TupleSet::fromHost(ctx, s2, {hostInt64(k2), hostInt32(a2), hostInt32(g2)});
+
+  Executor ex;
+  auto scanF = std::make_shared<SourcePOp>("scan_fact", factBatches);
+  auto scan1 = std::make_shared<SourcePOp>("scan_dim1", std::vector<std::shared_ptr<TupleSet>>{dim1});
+  auto scan2 = std::make_shared<SourcePOp>("scan_dim2", std::vector<std::shared_ptr<TupleSet>>{dim2});
+  auto filt1 = std::make_shared<FilterPOp>("filter_dim1", std::vector<std::string>{}, 0, "a1", FPDB_CMP_LT, 30);
+  auto filt2 = std::make_shared<FilterPOp>("filter_dim2", std::vector<std::string>{}, 0, "a2", FPDB_CMP_LT, 10);
+  // forward: dim1 -> fact.fk1, dim2 -> fact.fk2 (chained)
+  auto bfc1 = std::make_shared<BloomFilterCreatePOp>("bfc_dim1", std::vector<std::string>{}, 0, std::vector<std::string>{"k1"});
+  auto bfu1 = std::make_shared<BloomFilterUsePOp>("bfu_fact_fk1", std::vector<std::string>{}, 0, std::vector<std::string>{"fk1"});
+  auto bfc2 = std::make_shared<BloomFilterCreatePOp>("bfc_dim2", std::vector<std::string>{}, 0, std::vector<std::string>{"k2"});
+  auto bfu2 = std::make_shared<BloomFilterUsePOp>("bfu_fact_fk2", std::vector<std::string>{}, 0, std::vector<std::string>{"fk2"});
+  // backward: filtered fact.fk1 -> dim1
+  auto bfcB = std::make_shared<BloomFilterCreatePOp>("bfc_fact_fk1", std::vector<std::string>{}, 0, std::vector<std::string>{"fk1"});
+  auto bfuB = std::make_shared<BloomFilterUsePOp>("bfu_dim1", std::vector<std::string>{}, 0, std::vector<std::string>{"k1"});
+  auto join1 = std::make_shared<HashJoinArrowPOp>("join_dim1", std::vector<std::string>{"fk2", "m", "g1"}, 0,
+                                                  HashJoinPredicate{{"k1"}, {"fk1"}}, INNER);
+  auto join2 = std::make_shared<HashJoinArrowPOp>("join_dim2", std::vector<std::string>{"m", "g1", "g2"}, 0,
+                                                  HashJoinPredicate{{"k2"}, {"fk2"}}, INNER);
+  std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum_m", "m"),
+                                                        std::make_shared<Count>("cnt")};
+  auto group = std::make_shared<GroupPOp>("group", std::vector<std::string>{"g1", "g2", "sum_m", "cnt"}, 0,
+                                          std::vector<std::string>{"g1", "g2"}, fns);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{scanF, scan1, scan2, filt1, filt2, bfc1, bfu1, bfc2, bfu2,
+                                                          bfcB, bfuB, join1, join2, group, collate})
+    ex.put(op);
+  connect(scan1, filt1);
+  connect(scan2, filt2);
+  connect(filt1, bfc1);
+  bfc1->addBloomFilterUsePOp(bfu1);
+  bfu1->consume(bfc1);
+  connect(scanF, bfu1);
+  connect(filt2, bfc2);
+  bfc2->addBloomFilterUsePOp(bfu2);
+  bfu2->consume(bfc2);
+  connect(bfu1, bfu2);
+  connect(bfu2, bfcB);           // fact passes through the backward create
+  bfcB->addBloomFilterUsePOp(bfuB);
+  bfuB->consume(bfcB);
+  connect(bfc1, bfuB);           // dim1 passes through its forward create into the backward use
+  bfuB->produce(join1);
+  join1->addBuildProducer(bfuB);
+  bfcB->produce(join1);
+  join1->addProbeProducer(bfcB);
+  bfc2->produce(join2);
+  join2->addBuildProducer(bfc2);
+  join1->produce(join2);
+  join2->addProbeProducer(join1);
+  connect(join2, group);
+  connect(group, collate);
+  auto r = ex.execute();
+  if (!r) {
+    std::printf("starjoin failed: %s\n", r.error().c_str());
+    return 1;
+  }
+  auto out = collate->tupleSet().value();
+  FILE* f = std::fopen(outPath, "w");
+  if (!f) return 1;
+  std::fprintf(f, "g1,g2,sum_m,cnt\n");
+  auto h = out->toHost();
+  for (int64_t i = 0; i < out->numRows(); ++i)
+    std::fprintf(f, "%lld,%lld,%lld,%lld\n", (long long)h[0].int64At(i), (long long)h[1].int64At(i),
+                 (long long)h[2].int64At(i), (long long)h[3].int64At(i));
+  std::fclose(f);
+  auto bfuFact = std::static_pointer_cast<BloomFilterUsePOp>(ex.get("bfu_fact_fk2"));
+  std::printf("starjoin rows=%lld groups=%lld fact_after_bloom=%lld\n", (long long)n, (long long)out->numRows(),
+              (long long)bfuFact->numRowsOutput());
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  std::string mode = argc > 1 ? argv[1] : "unit";
+  if (mode == "starjoin") {
+    if (argc < 4) return 2;
+    return starJoin(std::atoll(argv[2]), argv[3]);
+  }
+  std::vector<std::pair<std::string, std::function<void()>>> tests = {
+      {"join", testJoin},
+      {"outer_semi_joins", testOuterSemiJoins},
+      {"bloom_join_blocked", [] { testBloomJoin(true); }},
+      {"bloom_join_legacy", [] { testBloomJoin(false); }},
+      {"group", testGroup},
+      {"empty_and_errors", testEmptyAndErrors},
+  };
+  for (auto& t : tests) {
+    int before = g_failures;
+    std::printf("RUN %s\n", t.first.c_str());
+    std::fflush(stdout);
+    try {
+      t.second();
+    } catch (const std::exception& e) {
+      std::printf("  exception: %s\n", e.what());
+      ++g_failures;
+    }
+    std::printf("%s %s\n", g_failures == before ? "OK" : "FAIL", t.first.c_str());
+    std::fflush(stdout);
+  }
+  return g_failures ? 1 : 0;
+}
