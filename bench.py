@@ -218,6 +218,26 @@ def bench_groupby(ctx, steps):
             "traffic_bytes": load_traffic("groupby_consume")}
 
 
+def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
+    """A plain copy of n int64 values (8n bytes read, 8n written) with
+    copy_rows_kernel: the reference point for the FETCH_SIZE / WRITE_SIZE
+    counters of the profiled kernels."""
+    from flexpushdowndb_amd import _capi as C
+    from flexpushdowndb_amd.device import DeviceColumn
+    L = C.lib()
+    src = DeviceColumn.empty(ctx, n, np.int64)
+    dst = DeviceColumn.empty(ctx, n, np.int64)
+    ctx.fill(src.values.ptr, 1, src.values.nbytes)
+    a_in, a_out = C.columns_array([src.c()]), C.columns_array([dst.c()])
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(steps):
+        C.check(L.fpdb_copy_rows(ctx.handle, a_in, 1, n, a_out, 0))
+    ms = ctx.timer_stop() / steps
+    return {"workload": f"copy_rows_kernel over {n} int64 values", "bytes_read": 8 * n, "bytes_written": 8 * n,
+            "ms": ms, "gbs": 16 * n / (ms * 1e-3) / 1e9}
+
+
 def cpu_baseline_join(w, runs=3):
     """The oracle's hash join (C++ unordered_map build + probe, pairs only) on
     the whole join workload, single-threaded on the host."""
@@ -240,7 +260,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", choices=["join", "bloom", "groupby"], default=None,
+    ap.add_argument("--only", choices=["join", "bloom", "groupby", "calib"], default=None,
                     help="run one workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -253,6 +273,9 @@ def main():
     info = ctx.device_info()
     log(f"device: {info}")
     res = {}
+    if args.only == "calib":
+        print(json.dumps(bench_copy_calibration(ctx, args.steps)))
+        return
     if args.only in (None, "join"):
         res["join"] = bench_join(ctx, args.steps, args.warmup)
         log(f"join: {res['join']['ms_per_step']:.4f} ms/step, probe kernel {res['join']['probe_ms']:.4f} ms")

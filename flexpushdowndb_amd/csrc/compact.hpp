@@ -72,23 +72,18 @@ template <class Pred, bool NEED_MASKS>
 __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutColSet out, int64_t n,
                                                       uint8_t* __restrict__ bitmap_out,
                                                       int64_t* __restrict__ index_out,
-                                                      uint64_t* lb_status,
-                                                      unsigned long long* ticket,
-                                                      uint64_t ticket_base, uint32_t epoch,
-                                                      int64_t num_tiles, int64_t* total,
-                                                      int* err) {
+                                                      LbArgs lb, int64_t* total, int* err) {
   __shared__ uint64_t s_masks[NEED_MASKS ? kMaskNum : 1];
   __shared__ int64_t s_tile;
+  __shared__ uint32_t s_epoch;
   __shared__ uint32_t s_wave_cnt[kCompactWaves];
   __shared__ uint64_t s_prefix;
   if (NEED_MASKS) {
     for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
   }
-  if (threadIdx.x == 0)
-    s_tile = (int64_t)(__hip_atomic_fetch_add(ticket, 1ULL, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT) - ticket_base);
-  __syncthreads();
+  lb_begin(lb, &s_tile, &s_epoch);
   const int64_t tile = s_tile;
+  const int64_t num_tiles = lb.num_tiles;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t wave_chunk0 = tile * (kCompactWaves * kCompactChunks) + wave * kCompactChunks;
@@ -113,7 +108,7 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
     uint64_t agg = 0;
     for (int w = 0; w < kCompactWaves; ++w) agg += s_wave_cnt[w];
     int e = 0;
-    uint64_t excl = lookback_exclusive(lb_status, tile, epoch, agg, &e);
+    uint64_t excl = lookback_exclusive(lb.status, tile, s_epoch, agg, &e);
     if (lane == 0) {
       s_prefix = excl;
       if (e) *err = 1;
@@ -135,6 +130,7 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
     }
     pos += __popcll(wc);
   }
+  lb_end(lb);
 }
 
 inline int grid_for(Context* c, int64_t work_items, int per_block, int blocks_per_cu = 8) {
@@ -173,14 +169,13 @@ fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const Ou
   }
   int64_t tiles = ceil_div(n, kCompactTileRows);
   FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "compaction: too many rows");
-  uint32_t epoch;
-  uint64_t base;
-  FPDB_TRY(lookback_prepare(c, tiles, &epoch, &base));
+  LbArgs lb;
+  FPDB_TRY(lookback_prepare(c, tiles, &lb));
   {
     ProfScope ps(c, prof_name);
     hipLaunchKernelGGL((compact_kernel<Pred, NEED_MASKS>), dim3((unsigned)tiles), dim3(256), 0,
-                       c->stream, pred, in, out, n, bitmap_out, index_out, c->lb.status, c->lb.ticket,
-                       base, epoch, tiles, d_num_out ? d_num_out : c->d_counter, ctx_err_flag(c));
+                       c->stream, pred, in, out, n, bitmap_out, index_out, lb,
+                       d_num_out ? d_num_out : c->d_counter, ctx_err_flag(c));
   }
   FPDB_HIP(hipGetLastError());
   if (d_num_out) return FPDB_OK;

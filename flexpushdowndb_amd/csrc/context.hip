@@ -184,30 +184,34 @@ fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes) {
   return FPDB_OK;
 }
 
-fpdb_status lookback_prepare(Context* c, int64_t tiles, uint32_t* epoch, uint64_t* ticket_base) {
+__global__ void lb_reset_kernel(LbState* st) {
+  st->ticket = 0;
+  st->done = 0;
+  st->epoch = 1;
+}
+
+fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args) {
   LookbackScratch& lb = c->lb;
+  if (!lb.state) {
+    FPDB_TRY(ctx_alloc(c, sizeof(LbState), reinterpret_cast<void**>(&lb.state)));
+    hipLaunchKernelGGL(lb_reset_kernel, dim3(1), dim3(1), 0, c->stream, lb.state);
+    FPDB_HIP(hipGetLastError());
+  }
   if (tiles > lb.tiles) {
+    // grow: a fresh zeroed status array; the epoch restarts at 1
     int64_t want = std::max<int64_t>(tiles, 4096);
     if (lb.status) FPDB_TRY(ctx_free(c, lb.status));
     lb.status = nullptr;
     FPDB_TRY(ctx_alloc(c, (uint64_t)want * 8, reinterpret_cast<void**>(&lb.status)));
     FPDB_TRY(launch_fill(c, lb.status, 0, (uint64_t)want * 8));
+    hipLaunchKernelGGL(lb_reset_kernel, dim3(1), dim3(1), 0, c->stream, lb.state);
+    FPDB_HIP(hipGetLastError());
     lb.tiles = want;
-    lb.epoch = 0;
   }
-  if (!lb.ticket) {
-    FPDB_TRY(ctx_alloc(c, 8, reinterpret_cast<void**>(&lb.ticket)));
-    FPDB_TRY(launch_fill(c, lb.ticket, 0, 8));
-    lb.ticket_base = 0;
-  }
-  lb.epoch += 1;
-  if (lb.epoch >= (1u << 20)) {   // epoch field exhausted: clear and restart
-    FPDB_TRY(launch_fill(c, lb.status, 0, (uint64_t)lb.tiles * 8));
-    lb.epoch = 1;
-  }
-  *epoch = lb.epoch;
-  *ticket_base = lb.ticket_base;
-  lb.ticket_base += (uint64_t)tiles;
+  args->state = lb.state;
+  args->status = lb.status;
+  args->status_cap = lb.tiles;
+  args->num_tiles = tiles;
   return FPDB_OK;
 }
 

@@ -40,15 +40,29 @@ fpdb_status fail(fpdb_status code, const std::string& msg);
 
 // ---- context ---------------------------------------------------------------
 
-// Decoupled look-back scratch: one 64-bit status word per tile plus a ticket
-// counter.  Status words carry a launch epoch so they never need clearing
-// between launches: [epoch:20][flag:2][value:42].
+// Decoupled look-back state, kept entirely on the device so a launch can be
+// replayed from a captured graph: tiles take tickets from `ticket`; the last
+// workgroup to finish (counted by `done`) resets both and advances `epoch`.
+// Status words carry the epoch, [epoch:20][flag:2][value:42], so they never
+// need clearing between launches (only when the 20-bit epoch wraps).
+struct LbState {
+  unsigned long long ticket;
+  unsigned int done;
+  unsigned int epoch;
+};
+
 struct LookbackScratch {
   uint64_t* status = nullptr;       // device, `tiles` words
-  unsigned long long* ticket = nullptr;  // device, 1 word, monotonically increasing
+  LbState* state = nullptr;         // device
   int64_t tiles = 0;
-  uint64_t ticket_base = 0;         // tickets handed out by earlier launches
-  uint32_t epoch = 0;               // epoch of the last launch (1..2^20-1)
+};
+
+// Kernel-argument view of the look-back scratch for one launch.
+struct LbArgs {
+  LbState* state;
+  uint64_t* status;
+  int64_t status_cap;
+  int64_t num_tiles;
 };
 
 struct ProfRecord {
@@ -107,9 +121,8 @@ struct ProfScope {
     }
   }
 };
-// Ensures look-back scratch for `tiles` tiles; returns the epoch and ticket base
-// for the next launch (and advances them).
-fpdb_status lookback_prepare(Context* c, int64_t tiles, uint32_t* epoch, uint64_t* ticket_base);
+// Ensures look-back scratch for `tiles` tiles and returns the launch arguments.
+fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args);
 fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes);
 // Kernel-side failure flag (look-back spin exhausted, table overflow); zero
 // unless a kernel reported a failure.
@@ -279,6 +292,42 @@ __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t
     __hip_atomic_store(&status[tile], lb_pack(epoch, kLbPrefix, exclusive + aggregate),
                        __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   return exclusive;
+}
+
+// Start of a look-back launch: thread 0 reads the epoch and takes a tile ticket.
+__device__ __forceinline__ void lb_begin(const LbArgs& lb, int64_t* s_tile, uint32_t* s_epoch) {
+  if (threadIdx.x == 0) {
+    *s_epoch = __hip_atomic_load(&lb.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *s_tile = (int64_t)__hip_atomic_fetch_add(&lb.state->ticket, 1ULL, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+}
+
+// End of a look-back launch; every workgroup must reach it.  The last one to
+// finish resets the ticket and advances the epoch for the next launch (every
+// other workgroup read the epoch before it counted itself done).
+__device__ __forceinline__ void lb_end(const LbArgs& lb) {
+  __shared__ int s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = __hip_atomic_fetch_add(&lb.state->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = prev == (unsigned)(lb.num_tiles - 1);
+  }
+  __syncthreads();
+  if (s_last) {
+    unsigned e = __hip_atomic_load(&lb.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (e >= (1u << 20)) {
+      for (int64_t i = threadIdx.x; i < lb.status_cap; i += blockDim.x) lb.status[i] = 0;
+      e = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&lb.state->ticket, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&lb.state->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&lb.state->epoch, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // Wave-wide inclusive prefix sum of a 32-bit value.

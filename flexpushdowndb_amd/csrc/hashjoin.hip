@@ -166,17 +166,14 @@ __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bco
 template <int KM, int JT, bool TRACK>
 __global__ __launch_bounds__(256) void hj_probe_kernel(
     const Slot* __restrict__ slots, uint64_t mask, KeySpec pk, int64_t n, ColSet pcols, ColSet bcols,
-    OutColSet out, int64_t out_cap, uint8_t* __restrict__ matched, uint64_t* lb_status,
-    unsigned long long* ticket, uint64_t ticket_base, uint32_t epoch, int64_t num_tiles,
-    int64_t* total, int* err) {
+    OutColSet out, int64_t out_cap, uint8_t* __restrict__ matched, LbArgs lb, int64_t* total, int* err) {
   __shared__ int64_t s_tile;
+  __shared__ uint32_t s_epoch;
   __shared__ uint32_t s_cnt[kProbeItems][kProbeWaves];
   __shared__ uint64_t s_prefix;
-  if (threadIdx.x == 0)
-    s_tile = (int64_t)(__hip_atomic_fetch_add(ticket, 1ULL, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT) - ticket_base);
-  __syncthreads();
+  lb_begin(lb, &s_tile, &s_epoch);
   const int64_t tile = s_tile;
+  const int64_t num_tiles = lb.num_tiles;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t row0 = tile * kProbeTileRows + threadIdx.x;
@@ -240,7 +237,7 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
     for (int i = 0; i < kProbeItems; ++i)
       for (int w = 0; w < kProbeWaves; ++w) agg += s_cnt[i][w];
     int e = 0;
-    uint64_t excl = lookback_exclusive(lb_status, tile, epoch, agg, &e);
+    uint64_t excl = lookback_exclusive(lb.status, tile, s_epoch, agg, &e);
     if (lane == 0) {
       s_prefix = excl;
       if (e) *err = 1;
@@ -302,6 +299,7 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
       s = (s + 1) & mask;
     }
   }
+  lb_end(lb);
 }
 
 __global__ void gather_kernel(ColSet in, const int64_t* __restrict__ rows, int64_t n, OutColSet out) {
@@ -483,9 +481,8 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
   }
   int64_t tiles = ceil_div(n, kProbeTileRows);
   FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "hash join probe: too many rows");
-  uint32_t epoch;
-  uint64_t base;
-  FPDB_TRY(lookback_prepare(c, tiles, &epoch, &base));
+  LbArgs lb;
+  FPDB_TRY(lookback_prepare(c, tiles, &lb));
   int* d_err = ctx_err_flag(c);
   int64_t* total = d_num_out ? d_num_out : c->d_counter;
   uint64_t mask = (uint64_t)t->capacity - 1;
@@ -494,8 +491,7 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
   ProfScope ps(c, "hj_probe");
 #define FPDB_PROBE_LAUNCH(KM, JT, TR)                                                              \
   hipLaunchKernelGGL((hj_probe_kernel<KM, JT, TR>), g, b, 0, c->stream, t->slots, mask, pk, n, pc, bc, \
-                     oc, out_cap, t->matched, c->lb.status, c->lb.ticket, base, epoch, tiles,      \
-                     total, d_err)
+                     oc, out_cap, t->matched, lb, total, d_err)
 #define FPDB_PROBE_JT(KM)                                                    \
   switch (jt * 2 + (track ? 1 : 0)) {                                        \
     case kJtInner * 2: FPDB_PROBE_LAUNCH(KM, kJtInner, false); break;        \

@@ -122,14 +122,11 @@ TupleSet::TupleSet(Schema schema, std::vector<Column> columns)
   numRows_ = columns_.empty() ? 0 : columns_[0].length;
 }
 
+// Zero-row columns still own (tiny) device buffers, so every column handed to
+// the C ABI has a values pointer.
 std::shared_ptr<TupleSet> TupleSet::make(const Schema& schema) {
   std::vector<Column> cols;
-  for (auto& f : schema.fields) {
-    Column c;
-    c.type = f.type;
-    c.length = 0;
-    cols.push_back(c);
-  }
+  for (auto& f : schema.fields) cols.push_back(Column::allocate(defaultContext(), f.type, 0, f.nullable));
   return std::make_shared<TupleSet>(schema, cols);
 }
 
