@@ -201,8 +201,11 @@ def bench_groupby(ctx, steps):
         g.consume([kc], vals, n)
         ctx.synchronize()
     cons_ms, _ = kernel_ms(ctx, "groupby_consume")
-    L.fpdb_context_set_profiling(ctx.handle, 0)
+    t0 = time.perf_counter()
     keys, out = g.finalize()
+    finalize_ms = (time.perf_counter() - t0) * 1e3
+    merge_ms, _ = kernel_ms(ctx, "groupby_merge")
+    L.fpdb_context_set_profiling(ctx.handle, 0)
     k = keys[0].numpy()
     c = out[0].numpy()
     key = w["key"].astype(np.int64)
@@ -214,8 +217,49 @@ def bench_groupby(ctx, steps):
     return {"workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
                         "COUNT, SUM int64 x2, SUM float64", "groups": int(len(k)),
             "consume_kernel_ms": cons_ms, "grows_per_s": n / (cons_ms * 1e-3) / 1e9,
+            "replica_merge_kernel_ms": merge_ms, "finalize_ms": finalize_ms,
             "hbm_frac": alg / (cons_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "traffic_bytes": load_traffic("groupby_consume")}
+
+
+def bench_star(ctx, steps, device_name):
+    """Batched (persistent, graph-replayed, autotuned) vs plain per-partition
+    execution of the 64-partition star join with full predicate transfer."""
+    from flexpushdowndb_amd import datagen
+    from flexpushdowndb_amd.starjoin import StarJoin, autotune
+    log("star: generating inputs (60M fact rows, 64 partitions)")
+    w = datagen.star_workload()
+    exp = datagen.star_expected(w)
+    sj = StarJoin(ctx, w)
+    best, entry = autotune(sj, device_name)
+    log(f"star: autotuned {best} ({entry['ms']:.3f} ms)")
+    sj.capture()
+    sj.replay()
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(steps):
+        sj.replay()
+    batched_ms = ctx.timer_stop() / steps
+    got = sj.result()
+    ok = all(np.array_equal(got[k], exp[k]) for k in ("g_a", "g_b", "count", "sum"))
+    plain = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        sj.run_plain()
+        ctx.synchronize()
+        plain.append((time.perf_counter() - t0) * 1e3)
+    got2 = sj.result()
+    ok = ok and all(np.array_equal(got2[k], exp[k]) for k in ("g_a", "g_b", "count", "sum"))
+    if not ok:
+        raise SystemExit("star join result does not match the expected join + group-by")
+    plain_ms = statistics.median(plain)
+    fact_bytes = len(w["measure"]) * 8 * len(w["dims"])
+    return {"workload": "star join, 60,000,000 fact rows in 64 lognormal(0,1) partitions, 4 dimensions "
+                        "(30,000/200,000/2,000,000/2,556 rows kept at 20%/4%/1%/14%), forward + backward "
+                        "blocked bloom filters, 4 joins, group-by on 2 dimension attributes",
+            "batched_ms": batched_ms, "plain_ms": plain_ms, "speedup": plain_ms / batched_ms,
+            "groups": int(len(got["count"])), "survivors": exp["survivors"], "tuning": list(best),
+            "fact_fk_hbm_frac": fact_bytes / (batched_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
 def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
@@ -260,7 +304,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", choices=["join", "bloom", "groupby", "calib"], default=None,
+    ap.add_argument("--only", choices=["join", "bloom", "groupby", "star", "calib"], default=None,
                     help="run one workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -285,6 +329,9 @@ def main():
     if args.only in (None, "groupby"):
         res["groupby"] = bench_groupby(ctx, max(2, args.steps // 5))
         log(f"groupby: consume {res['groupby']['consume_kernel_ms']:.4f} ms")
+    if args.only in (None, "star"):
+        res["star"] = bench_star(ctx, max(3, args.steps // 2), f"gfx950-{info['num_cus']}cu")
+        log(f"star: batched {res['star']['batched_ms']:.3f} ms, plain {res['star']['plain_ms']:.3f} ms")
     if args.only not in (None, "join"):
         print(json.dumps({k: v for k, v in res[args.only].items() if k != "inputs"}))
         return
@@ -319,6 +366,8 @@ def main():
         line["bloom_use"] = res["bloom"]
     if "groupby" in res:
         line["groupby"] = res["groupby"]
+    if "star" in res:
+        line["batched_star_join"] = res["star"]
     line["device"] = info
     print(json.dumps(line), flush=True)
 

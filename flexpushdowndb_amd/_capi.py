@@ -54,6 +54,28 @@ class AggSpec(ctypes.Structure):
                 ("input_nullable", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class StarDimension(ctypes.Structure):
+    _fields_ = [("key", Column), ("filter_attr", Column), ("filter_op", ctypes.c_int32),
+                ("has_group_attr", ctypes.c_int32), ("filter_constant", ctypes.c_int64),
+                ("group_attr", Column), ("fact_fk", Column)]
+
+
+class StarQuery(ctypes.Structure):
+    _fields_ = [("num_dims", ctypes.c_int32), ("group_dims", ctypes.c_int32 * 2),
+                ("filter_order", ctypes.c_int32 * 8), ("dims", StarDimension * 8),
+                ("fact_measure", Column), ("fact_rows", ctypes.c_int64), ("num_partitions", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("partition_offsets", ctypes.POINTER(ctypes.c_int64)),
+                ("max_groups", ctypes.c_int64)]
+
+
+class StarTuning(ctypes.Structure):
+    _fields_ = [("blocks_per_cu", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("ht_load_shift", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
+
+
+CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
+
+
 class FpdbError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"fpdb error {code}: {msg}")
@@ -115,6 +137,14 @@ SIGNATURES = [
     ("fpdb_groupby_num_groups", _I32, [_P, _P, _PI64]),
     ("fpdb_groupby_output_type", _I32, [AggSpec]),
     ("fpdb_groupby_finalize", _I32, [_P, _P, _PCOL, _PCOL, _PI64]),
+    ("fpdb_star_prepare", _I32, [_P, ctypes.POINTER(StarQuery), ctypes.POINTER(StarTuning), ctypes.POINTER(_P)]),
+    ("fpdb_star_destroy", _I32, [_P]),
+    ("fpdb_star_run_plain", _I32, [_P, _P]),
+    ("fpdb_star_run_batched", _I32, [_P, _P]),
+    ("fpdb_star_graph_capture", _I32, [_P, _P]),
+    ("fpdb_star_graph_launch", _I32, [_P, _P]),
+    ("fpdb_star_result", _I32, [_P, _P, _P, _P, _P, _P, _I64, _PI64]),
+    ("fpdb_star_filter", _I32, [_P, _P, _I, _I, _P, _I64, ctypes.POINTER(_I), _PI64]),
 ]
 
 _lib = None

@@ -245,18 +245,21 @@ constexpr int64_t kLbSpinLimit = int64_t(1) << 28;
 // the exclusive prefix of the tile (identical in every lane).  `err` is set
 // (and 0 returned) if the bounded spin runs out, which only happens if an
 // earlier tile never publishes.
+// The status words carry their own payload, so relaxed device-scope atomics
+// suffice: acquire/release at device scope would write back or invalidate the
+// whole L2 of the XCD on every publish and poll.
 __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t tile,
                                                        uint32_t epoch, uint64_t aggregate,
                                                        int* err) {
   const int lane = threadIdx.x & 63;
   if (tile == 0) {
     if (lane == 0)
-      __hip_atomic_store(&status[0], lb_pack(epoch, kLbPrefix, aggregate), __ATOMIC_RELEASE,
+      __hip_atomic_store(&status[0], lb_pack(epoch, kLbPrefix, aggregate), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
   if (lane == 0)
-    __hip_atomic_store(&status[tile], lb_pack(epoch, kLbAggregate, aggregate), __ATOMIC_RELEASE,
+    __hip_atomic_store(&status[tile], lb_pack(epoch, kLbAggregate, aggregate), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   uint64_t exclusive = 0;
   int64_t window_end = tile;   // predecessors [window_end-64, window_end)
@@ -266,7 +269,7 @@ __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t
     uint64_t w = 0;
     uint32_t flag = 0;
     if (idx >= 0) {
-      w = __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       flag = ((uint32_t)(w >> 44) == epoch) ? (uint32_t)((w >> 42) & 3) : 0;
     } else {
       flag = kLbPrefix;   // virtual prefix of zero before tile 0
@@ -290,7 +293,7 @@ __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t
   }
   if (lane == 0)
     __hip_atomic_store(&status[tile], lb_pack(epoch, kLbPrefix, exclusive + aggregate),
-                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return exclusive;
 }
 
@@ -311,7 +314,7 @@ __device__ __forceinline__ void lb_end(const LbArgs& lb) {
   __shared__ int s_last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned prev = __hip_atomic_fetch_add(&lb.state->done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned prev = __hip_atomic_fetch_add(&lb.state->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = prev == (unsigned)(lb.num_tiles - 1);
   }
   __syncthreads();
@@ -325,7 +328,7 @@ __device__ __forceinline__ void lb_end(const LbArgs& lb) {
     if (threadIdx.x == 0) {
       __hip_atomic_store(&lb.state->ticket, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&lb.state->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&lb.state->epoch, e, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&lb.state->epoch, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

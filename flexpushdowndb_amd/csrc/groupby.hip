@@ -140,27 +140,53 @@ __device__ __forceinline__ void acc_apply(uint64_t* slot, int32_t op, uint64_t v
   }
 }
 
+// The global table has one replica per XCD: a workgroup aggregates into the
+// replica of the XCD it runs on, so the cache lines its atomics touch stay in
+// that XCD's L2 instead of bouncing between L2s.  Replicas 1.. are merged into
+// replica 0 before groups are counted or emitted.  Any workgroup may use any
+// replica (all atomics are device-coherent); the XCD choice is only locality.
+constexpr int kReplicas = 8;
+
 struct GlobalTable {
-  uint64_t* keys;        // cap
-  uint64_t* acc;         // nacc x (cap + 1); slot cap holds the key equal to kEmptyKey
-  int64_t* first;        // cap + 1
+  uint64_t* keys;        // nrep x cap
+  uint64_t* acc;         // nrep x nacc x (cap + 1); slot cap holds the key equal to kEmptyKey
+  int64_t* first;        // nrep x (cap + 1)
   uint64_t mask;         // cap - 1
   int64_t stride;        // cap + 1
-  unsigned long long* groups;  // number of claimed slots
+  int32_t nacc;
+  int32_t nrep;
+  unsigned long long* groups;  // number of claimed slots of replica 0
   int* overflow;
 };
 
-// Finds or claims the global slot of key k (cap for the reserved key value).
-__device__ __forceinline__ int64_t global_slot(const GlobalTable& g, uint64_t k) {
+__device__ __forceinline__ uint64_t* gkey(const GlobalTable& g, int rep, int64_t s) {
+  return g.keys + (size_t)rep * (g.mask + 1) + s;
+}
+__device__ __forceinline__ uint64_t* gacc(const GlobalTable& g, int rep, int a, int64_t s) {
+  return g.acc + ((size_t)rep * g.nacc + a) * g.stride + s;
+}
+__device__ __forceinline__ int64_t* gfirst(const GlobalTable& g, int rep, int64_t s) {
+  return g.first + (size_t)rep * g.stride + s;
+}
+
+__device__ __forceinline__ int xcc_id() {
+  unsigned v;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+  return (int)(v & 0xF);
+}
+
+// Finds or claims the slot of key k in a replica (cap for the reserved key value).
+__device__ __forceinline__ int64_t global_slot(const GlobalTable& g, int rep, uint64_t k) {
   if (k == kEmptyKey) return (int64_t)g.mask + 1;
   uint64_t s = fmix64(k) & g.mask;
   for (uint64_t p = 0; p <= g.mask; ++p) {
-    uint64_t cur = __hip_atomic_load(&g.keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* kp = gkey(g, rep, s);
+    uint64_t cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == k) return (int64_t)s;
     if (cur == kEmptyKey) {
-      unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&g.keys[s]), kEmptyKey, k);
+      unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(kp), kEmptyKey, k);
       if (prev == kEmptyKey) {
-        atomicAdd(g.groups, 1ULL);
+        if (rep == 0) atomicAdd(g.groups, 1ULL);
         return (int64_t)s;
       }
       if (prev == k) return (int64_t)s;
@@ -187,6 +213,7 @@ __global__ __launch_bounds__(kGroupThreads) void groupby_consume_kernel(GKeySpec
     for (int a = 0; a < nacc; ++a) lacc[(size_t)a * lds_slots + s] = acc_identity(A.op[a]);
   }
   __syncthreads();
+  const int rep = xcc_id() % g.nrep;
 
   const int64_t per_block = ceil_div_d(n, gridDim.x);
   const int64_t begin = (int64_t)blockIdx.x * per_block;
@@ -220,13 +247,13 @@ __global__ __launch_bounds__(kGroupThreads) void groupby_consume_kernel(GKeySpec
       }
       if (grow < lfirst[lslot]) atomicMin(reinterpret_cast<long long*>(&lfirst[lslot]), (long long)grow);
     } else {
-      int64_t gs = global_slot(g, k);
+      int64_t gs = global_slot(g, rep, k);
       if (gs >= 0) {
 #pragma unroll
         for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-          if (a < nacc && ok[a]) acc_apply(&g.acc[(size_t)a * g.stride + gs], A.op[a], v[a]);
+          if (a < nacc && ok[a]) acc_apply(gacc(g, rep, a, gs), A.op[a], v[a]);
         }
-        atomicMin(reinterpret_cast<long long*>(&g.first[gs]), (long long)grow);
+        atomicMin(reinterpret_cast<long long*>(gfirst(g, rep, gs)), (long long)grow);
       }
     }
   }
@@ -235,23 +262,49 @@ __global__ __launch_bounds__(kGroupThreads) void groupby_consume_kernel(GKeySpec
   for (int s = threadIdx.x; s < lds_slots; s += blockDim.x) {
     uint64_t k = lkeys[s];
     if (k == kEmptyKey) continue;
-    int64_t gs = global_slot(g, k);
+    int64_t gs = global_slot(g, rep, k);
     if (gs < 0) continue;
     for (int a = 0; a < nacc; ++a) {
       uint64_t v = lacc[(size_t)a * lds_slots + s];
-      if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(&g.acc[(size_t)a * g.stride + gs], A.op[a], v);
+      if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(gacc(g, rep, a, gs), A.op[a], v);
     }
-    atomicMin(reinterpret_cast<long long*>(&g.first[gs]), (long long)lfirst[s]);
+    atomicMin(reinterpret_cast<long long*>(gfirst(g, rep, gs)), (long long)lfirst[s]);
   }
 }
 
 __global__ void groupby_init_kernel(GlobalTable g, AccSpec A) {
-  int64_t total = g.stride;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < total;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    if (s < total - 1) g.keys[s] = kEmptyKey;
-    g.first[s] = LLONG_MAX;
-    for (int a = 0; a < A.n; ++a) g.acc[(size_t)a * g.stride + s] = acc_identity(A.op[a]);
+  int64_t total = g.stride * g.nrep;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int rep = (int)(i / g.stride);
+    int64_t s = i % g.stride;
+    if (s < g.stride - 1) *gkey(g, rep, s) = kEmptyKey;
+    *gfirst(g, rep, s) = LLONG_MAX;
+    for (int a = 0; a < A.n; ++a) *gacc(g, rep, a, s) = acc_identity(A.op[a]);
+  }
+}
+
+// Folds replicas 1.. into replica 0 and resets them to empty.
+__global__ void groupby_merge_kernel(GlobalTable g, AccSpec A) {
+  int64_t total = g.stride * (g.nrep - 1);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int rep = 1 + (int)(i / g.stride);
+    int64_t s = i % g.stride;
+    int64_t f = *gfirst(g, rep, s);
+    if (f == LLONG_MAX) continue;    // slot never used
+    uint64_t k = (s == g.stride - 1) ? kEmptyKey : *gkey(g, rep, s);
+    int64_t d = global_slot(g, 0, k);
+    if (d >= 0) {
+      for (int a = 0; a < A.n; ++a) {
+        uint64_t v = *gacc(g, rep, a, s);
+        if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(gacc(g, 0, a, d), A.op[a], v);
+        *gacc(g, rep, a, s) = acc_identity(A.op[a]);
+      }
+      atomicMin(reinterpret_cast<long long*>(gfirst(g, 0, d)), (long long)f);
+    }
+    if (s < g.stride - 1) *gkey(g, rep, s) = kEmptyKey;
+    *gfirst(g, rep, s) = LLONG_MAX;
   }
 }
 
@@ -396,6 +449,8 @@ static GlobalTable global_view(fpdb_group_table* t) {
   g.first = t->first;
   g.mask = (uint64_t)t->capacity - 1;
   g.stride = t->capacity + 1;
+  g.nacc = t->acc.n;
+  g.nrep = kReplicas;
   g.groups = t->d_groups;
   g.overflow = reinterpret_cast<int*>(t->d_groups + 1);
   return g;
@@ -491,9 +546,10 @@ fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int
   int64_t cap = 64;
   while (cap < 2 * max_groups) cap <<= 1;
   t->capacity = cap;
-  fpdb_status s = ctx_alloc(c, (uint64_t)cap * 8, reinterpret_cast<void**>(&t->keys));
-  if (s == FPDB_OK) s = ctx_alloc(c, (uint64_t)(cap + 1) * 8 * std::max(nacc, 1), reinterpret_cast<void**>(&t->accs));
-  if (s == FPDB_OK) s = ctx_alloc(c, (uint64_t)(cap + 1) * 8, reinterpret_cast<void**>(&t->first));
+  fpdb_status s = ctx_alloc(c, (uint64_t)cap * 8 * kReplicas, reinterpret_cast<void**>(&t->keys));
+  if (s == FPDB_OK)
+    s = ctx_alloc(c, (uint64_t)(cap + 1) * 8 * std::max(nacc, 1) * kReplicas, reinterpret_cast<void**>(&t->accs));
+  if (s == FPDB_OK) s = ctx_alloc(c, (uint64_t)(cap + 1) * 8 * kReplicas, reinterpret_cast<void**>(&t->first));
   if (s == FPDB_OK) s = ctx_alloc(c, 16, reinterpret_cast<void**>(&t->d_groups));
   if (s == FPDB_OK) s = launch_fill(c, t->d_groups, 0, 16);
   if (s != FPDB_OK) {
@@ -501,7 +557,8 @@ fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int
     return s;
   }
   GlobalTable g = global_view(t);
-  hipLaunchKernelGGL(groupby_init_kernel, dim3(grid_for(c, cap + 1, 256)), dim3(256), 0, c->stream, g, t->acc);
+  hipLaunchKernelGGL(groupby_init_kernel, dim3(grid_for(c, (cap + 1) * kReplicas, 256)), dim3(256), 0, c->stream, g,
+                     t->acc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     fpdb_groupby_destroy(t);
@@ -596,6 +653,13 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* t, const f
 fpdb_status fpdb_groupby_num_groups(fpdb_context* ctx, fpdb_group_table* t, int64_t* num_groups) {
   FPDB_CHECK(ctx && t && num_groups, FPDB_ERR_INVALID, "NULL argument");
   Context* c = as_ctx(ctx);
+  if (t->rows > 0) {
+    GlobalTable g = global_view(t);
+    ProfScope ps(c, "groupby_merge");
+    hipLaunchKernelGGL(groupby_merge_kernel, dim3(grid_for(c, g.stride * (kReplicas - 1), 256)), dim3(256), 0,
+                       c->stream, g, t->acc);
+    FPDB_HIP(hipGetLastError());
+  }
   FPDB_HIP(hipMemcpyAsync(c->h_counter, t->d_groups, 16, hipMemcpyDeviceToHost, c->stream));
   int64_t sentinel_first = 0;
   FPDB_HIP(hipMemcpyAsync(&sentinel_first, t->first + t->capacity, 8, hipMemcpyDeviceToHost, c->stream));

@@ -1,0 +1,857 @@
+// Star join with full predicate transfer over a partitioned fact table
+// (SmallToLargePredTransOrder, fpdb-executor/src/physical/transform/pred-trans/
+// SmallToLargePredTransOrder.cpp:38-214, with the partitioned semantics the
+// reference lacks):
+//   1. each dimension is filtered (attr <op> constant) and a forward blocked
+//      bloom filter is built over the kept keys;
+//   2. every fact partition is filtered by all forward filters;
+//   3. backward filters are built over the union of the surviving fact rows'
+//      foreign keys (the OR of per-partition filters) and applied to the kept
+//      dimension rows;
+//   4. the survivors are joined with the reduced dimensions (hash tables with
+//      the group attribute inline) and grouped on two dimension attributes
+//      (COUNT(*), SUM(measure)); groups are numbered by first appearance in
+//      fact order, so per-partition partials merge in partition order.
+//
+// Two executions of the same plan:
+//   plain    per-partition launches of the operator entry points (bloom filter
+//            use, hash-join probe, group-by), as the drop-in operators run it;
+//   batched  one persistent launch: workgroups stay resident and pass grid
+//            barriers between the phases, so the whole predicate-transfer
+//            chain, the joins and the aggregation are a single kernel; it can
+//            be captured in a hipGraph and replayed.
+#include "fpdb_internal.hpp"
+#include "compact.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <vector>
+
+namespace fpdb {
+
+constexpr int kStarMaxDims = 8;
+constexpr uint32_t kStarEmptyRow = 0xFFFFFFFFu;
+constexpr uint64_t kStarEmptyKey = ~0ULL;
+
+struct alignas(16) StarSlot {
+  int64_t key;
+  uint32_t row;
+  uint32_t payload;
+};
+
+// Device view of one dimension.
+struct StarDimDev {
+  const int64_t* key;
+  const int32_t* attr;
+  const int32_t* gattr;      // group attribute, or nullptr
+  int64_t rows;
+  int64_t cst;
+  int32_t op;
+  int32_t pad;
+  uint8_t* kept;             // rows bytes
+  uint64_t* fbf;             // forward filter blocks (capacity for `rows` keys)
+  uint64_t* bbf;             // backward filter blocks (capacity for all fact rows)
+  StarSlot* ht;              // hash table (capacity >= 2 x rows)
+  uint64_t ht_mask;
+  const int64_t* fk;         // fact foreign key column
+};
+
+struct StarDev {
+  StarDimDev dim[kStarMaxDims];
+  int32_t ndims;
+  int32_t order[kStarMaxDims];     // forward filter order over the fact columns
+  int32_t gdim[2];                 // dimensions whose group attribute forms the key
+  const int64_t* measure;
+  int64_t fact_rows;
+  // survivors of the forward pass
+  int64_t* s_idx;                  // original fact row
+  int64_t* s_fk;                   // ndims x fact_rows
+  int64_t* s_meas;
+  // state words: [0] survivor count, [1..ndims] kept count per dim,
+  // [16..31] barrier, [32] error flag
+  unsigned long long* state;
+  // group table
+  uint64_t* g_keys;
+  unsigned long long* g_cnt;
+  unsigned long long* g_sum;
+  long long* g_first;
+  uint64_t g_mask;
+  int32_t tile_rows;
+};
+
+__device__ __forceinline__ int log_blocks_for(int64_t n) {
+  int64_t desired = n * 8 < 512 ? 512 : n * 8;
+  int lb = 0;
+  while ((int64_t(1) << lb) < desired) ++lb;
+  return lb - 6;
+}
+
+__device__ __forceinline__ bool star_cmp(int32_t v, int32_t op, int64_t c) {
+  switch (op) {
+    case FPDB_CMP_LT: return v < c;
+    case FPDB_CMP_LE: return v <= c;
+    case FPDB_CMP_GT: return v > c;
+    case FPDB_CMP_GE: return v >= c;
+    case FPDB_CMP_EQ: return v == c;
+    default: return v != c;
+  }
+}
+
+__device__ __forceinline__ uint64_t rotl64s(uint64_t x, int n) { return (x << n) | (x >> ((-n) & 63)); }
+
+__device__ __forceinline__ void bf_insert(uint64_t* blocks, int lnb, const uint64_t* s_masks, int64_t key) {
+  uint32_t h = arrow6_hash_word((uint64_t)key);
+  uint64_t m = rotl64s(s_masks[h & (kMaskNum - 1)], (h >> kMaskLogNum) & 63);
+  uint64_t b = ((uint64_t)h >> (kMaskLogNum + 6)) & ((1ULL << lnb) - 1);
+  __hip_atomic_fetch_or(&blocks[b], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const uint64_t* s_masks, int64_t key) {
+  uint32_t h = arrow6_hash_word((uint64_t)key);
+  uint64_t m = rotl64s(s_masks[h & (kMaskNum - 1)], (h >> kMaskLogNum) & 63);
+  uint64_t b = ((uint64_t)h >> (kMaskLogNum + 6)) & ((1ULL << lnb) - 1);
+  return (blocks[b] & m) == m;
+}
+
+// Grid barrier for a launch whose workgroups are all resident (the grid is
+// sized from the occupancy calculator).  Sense-free: a generation counter
+// advances when the last workgroup arrives.  Bounded spin: a workgroup that
+// waits too long sets the error flag and continues, so the kernel always ends.
+__device__ __forceinline__ void star_barrier(unsigned long long* st) {
+  unsigned long long* count = st + 16;
+  unsigned long long* gen = st + 17;
+  unsigned long long* err = st + 32;
+  __threadfence();    // every thread's writes of this phase become visible device-wide
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long arrived =
+        __hip_atomic_fetch_add(count, 1ULL, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (arrived == gridDim.x) {
+      __hip_atomic_store(count, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(gen, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int64_t spins = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (int64_t(1) << 26)) {
+          __hip_atomic_store(err, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  __threadfence();    // and every thread observes the other workgroups' writes
+}
+
+__device__ __forceinline__ void ht_insert(StarSlot* ht, uint64_t mask, int64_t key, uint32_t row, uint32_t payload) {
+  uint64_t s = fmix64((uint64_t)key) & mask;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    uint32_t old = atomicCAS(&ht[s].row, kStarEmptyRow, row);
+    if (old == kStarEmptyRow) {
+      ht[s].key = key;
+      ht[s].payload = payload;
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+// Returns true and the payload if key is present (keys are unique per dimension).
+__device__ __forceinline__ bool ht_find(const StarSlot* ht, uint64_t mask, int64_t key, uint32_t* payload) {
+  uint64_t s = fmix64((uint64_t)key) & mask;
+  while (true) {
+    uint4 v = reinterpret_cast<const uint4*>(ht)[s];
+    if (v.z == kStarEmptyRow) return false;
+    int64_t k = (int64_t)((uint64_t)v.x | ((uint64_t)v.y << 32));
+    if (k == key) {
+      *payload = v.w;
+      return true;
+    }
+    s = (s + 1) & mask;
+  }
+}
+
+// Group slots 0..g_mask hold keys; slot g_mask + 1 is reserved for the one
+// key equal to the empty marker.
+__device__ __forceinline__ void group_add(const StarDev& P, uint64_t gkey, int64_t meas, int64_t first) {
+  if (gkey == kStarEmptyKey) {
+    uint64_t s = P.g_mask + 1;
+    atomicAdd(&P.g_cnt[s], 1ULL);
+    atomicAdd(&P.g_sum[s], (unsigned long long)meas);
+    atomicMin(&P.g_first[s], (long long)first);
+    return;
+  }
+  uint64_t s = fmix64(gkey) & P.g_mask;
+  for (uint64_t p = 0; p <= P.g_mask; ++p) {
+    uint64_t cur = __hip_atomic_load(&P.g_keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == kStarEmptyKey) {
+      unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&P.g_keys[s]), kStarEmptyKey, gkey);
+      if (prev == kStarEmptyKey || prev == gkey) cur = gkey;
+      else cur = prev;
+    }
+    if (cur == gkey) {
+      atomicAdd(&P.g_cnt[s], 1ULL);
+      atomicAdd(&P.g_sum[s], (unsigned long long)meas);
+      atomicMin(&P.g_first[s], (long long)first);
+      return;
+    }
+    s = (s + 1) & P.g_mask;
+  }
+  __hip_atomic_store(P.state + 32, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- phases shared by the persistent kernel and the plain path ---------------
+
+// Forward-pass survivors are appended with one atomic per wavefront.
+__device__ __forceinline__ void append_survivor(const StarDev& P, bool pass, int64_t r) {
+  uint64_t mask = __ballot(pass);
+  if (!mask) return;
+  const int lane = threadIdx.x & 63;
+  int leader = __builtin_ctzll(mask);
+  unsigned long long base = 0;
+  if (lane == leader)
+    base = __hip_atomic_fetch_add(P.state + 0, (unsigned long long)__popcll(mask), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+  base = __shfl(base, leader);
+  if (pass) {
+    uint64_t below = lane ? (mask & (~0ULL >> (64 - lane))) : 0ULL;
+    int64_t pos = (int64_t)(base + __popcll(below));
+    P.s_idx[pos] = r;
+    for (int d = 0; d < P.ndims; ++d) P.s_fk[(int64_t)d * P.fact_rows + pos] = P.dim[d].fk[r];
+    P.s_meas[pos] = P.measure[r];
+  }
+}
+
+__global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
+  __shared__ uint64_t s_masks[kMaskNum];
+  for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
+  __syncthreads();
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gthreads = (int64_t)gridDim.x * blockDim.x;
+
+  // A1: filter the dimensions, count kept rows, prepare the hash tables
+  for (int d = 0; d < P.ndims; ++d) {
+    const StarDimDev& D = P.dim[d];
+    unsigned long long kept = 0;
+    for (int64_t r = gtid; r < D.rows; r += gthreads) {
+      bool k = star_cmp(D.attr[r], D.op, D.cst);
+      D.kept[r] = k ? 1 : 0;
+      kept += k ? 1 : 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
+    if ((threadIdx.x & 63) == 0 && kept)
+      __hip_atomic_fetch_add(P.state + 1 + d, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    for (int64_t s = gtid; s <= (int64_t)D.ht_mask; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
+  }
+  star_barrier(P.state);
+  // A2: zero the forward filters at their sizes
+  for (int d = 0; d < P.ndims; ++d) {
+    const StarDimDev& D = P.dim[d];
+    int lnb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int64_t b = gtid; b < (int64_t(1) << lnb); b += gthreads) D.fbf[b] = 0;
+  }
+  star_barrier(P.state);
+  // A3: insert the kept keys into the forward filters
+  for (int d = 0; d < P.ndims; ++d) {
+    const StarDimDev& D = P.dim[d];
+    int lnb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int64_t r = gtid; r < D.rows; r += gthreads)
+      if (D.kept[r]) bf_insert(D.fbf, lnb, s_masks, D.key[r]);
+  }
+  star_barrier(P.state);
+  // B: forward pass over every fact row (all partitions), filters in `order`
+  {
+    int lnb[kStarMaxDims];
+    for (int d = 0; d < P.ndims; ++d)
+      lnb[d] = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int64_t rounded = (P.fact_rows + 63) & ~int64_t(63);
+    for (int64_t r = gtid; r < rounded; r += gthreads) {     // whole waves iterate together
+      bool pass = r < P.fact_rows;
+      for (int i = 0; i < P.ndims && pass; ++i) {
+        int d = P.order[i];
+        pass = bf_find(P.dim[d].fbf, lnb[d], s_masks, P.dim[d].fk[r]);
+      }
+      append_survivor(P, pass, r);
+    }
+  }
+  star_barrier(P.state);
+  // B2: backward filters over the survivors
+  const int64_t nsurv = (int64_t)__hip_atomic_load(P.state + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int blnb = log_blocks_for(nsurv);
+  for (int d = 0; d < P.ndims; ++d)
+    for (int64_t b = gtid; b < (int64_t(1) << blnb); b += gthreads) P.dim[d].bbf[b] = 0;
+  star_barrier(P.state);
+  for (int d = 0; d < P.ndims; ++d)
+    for (int64_t i = gtid; i < nsurv; i += gthreads)
+      bf_insert(P.dim[d].bbf, blnb, s_masks, P.s_fk[(int64_t)d * P.fact_rows + i]);
+  star_barrier(P.state);
+  // C: kept dimension rows that pass the backward filter go into the hash tables
+  for (int d = 0; d < P.ndims; ++d) {
+    const StarDimDev& D = P.dim[d];
+    for (int64_t r = gtid; r < D.rows; r += gthreads) {
+      if (!D.kept[r]) continue;
+      int64_t k = D.key[r];
+      if (!bf_find(D.bbf, blnb, s_masks, k)) continue;
+      ht_insert(D.ht, D.ht_mask, k, (uint32_t)r, D.gattr ? (uint32_t)D.gattr[r] : 0u);
+    }
+  }
+  star_barrier(P.state);
+  // D: join the survivors with every dimension and aggregate
+  for (int64_t i = gtid; i < nsurv; i += gthreads) {
+    uint32_t g[2] = {0, 0};
+    bool ok = true;
+    for (int j = 0; j < P.ndims && ok; ++j) {
+      int d = P.order[j];
+      uint32_t pay = 0;
+      ok = ht_find(P.dim[d].ht, P.dim[d].ht_mask, P.s_fk[(int64_t)d * P.fact_rows + i], &pay);
+      if (d == P.gdim[0]) g[0] = pay;
+      if (d == P.gdim[1]) g[1] = pay;
+    }
+    if (ok) group_add(P, (uint64_t)g[0] | ((uint64_t)g[1] << 32), P.s_meas[i], P.s_idx[i]);
+  }
+}
+
+__global__ void star_init_kernel(StarDev P) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gthreads = (int64_t)gridDim.x * blockDim.x;
+  if (gtid < 64) P.state[gtid] = 0;
+  for (int64_t s = gtid; s <= (int64_t)P.g_mask + 1; s += gthreads) {
+    P.g_keys[s] = kStarEmptyKey;
+    P.g_cnt[s] = 0;
+    P.g_sum[s] = 0;
+    P.g_first[s] = LLONG_MAX;
+  }
+}
+
+// Ranks the groups by first appearance and writes the result columns.
+__global__ void star_emit_kernel(StarDev P, int32_t* out_g0, int32_t* out_g1, int64_t* out_cnt,
+                                 int64_t* out_sum, unsigned long long* out_n) {
+  const int64_t cap = (int64_t)P.g_mask + 2;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += (int64_t)gridDim.x * blockDim.x) {
+    long long f = P.g_first[s];
+    if (f == LLONG_MAX) continue;
+    int64_t rank = 0;
+    for (int64_t t = 0; t < cap; ++t) {
+      long long o = P.g_first[t];
+      rank += (o < f) ? 1 : 0;
+    }
+    uint64_t k = P.g_keys[s];
+    out_g0[rank] = (int32_t)(uint32_t)(k & 0xFFFFFFFFULL);
+    out_g1[rank] = (int32_t)(uint32_t)(k >> 32);
+    out_cnt[rank] = (int64_t)P.g_cnt[s];
+    out_sum[rank] = (int64_t)P.g_sum[s];
+    atomicAdd(out_n, 1ULL);
+  }
+}
+
+}  // namespace fpdb
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct fpdb_star {
+  fpdb::Context* ctx = nullptr;
+  fpdb_star_query q;
+  std::vector<int64_t> parts;          // partition offsets (num_partitions + 1)
+  fpdb_star_tuning tune;
+  fpdb::StarDev dev;
+  std::vector<void*> owned;
+  int grid = 0;
+  // result (device)
+  int32_t* out_g0 = nullptr;
+  int32_t* out_g1 = nullptr;
+  int64_t* out_cnt = nullptr;
+  int64_t* out_sum = nullptr;
+  unsigned long long* out_n = nullptr;
+  int64_t max_groups = 0;
+  // sizes of the last plain run (a batched run keeps them in the state words)
+  int last_mode = 0;                       // 1 plain, 2 batched
+  int64_t kept[fpdb::kStarMaxDims] = {0};
+  int fwd_lnb[fpdb::kStarMaxDims] = {0};
+  int bwd_lnb = 0;
+  int64_t survivors = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace fpdb {
+
+static fpdb_status star_alloc(fpdb_star* s, uint64_t bytes, void** p) {
+  FPDB_TRY(ctx_alloc(s->ctx, bytes, p));
+  s->owned.push_back(*p);
+  return FPDB_OK;
+}
+
+static int host_log_blocks(int64_t n) { return fpdb_bloom_blocked_log_num_blocks(n); }
+
+static fpdb_column col_i64(const int64_t* p, int64_t n) {
+  fpdb_column c{};
+  c.type = FPDB_TYPE_INT64;
+  c.length = n;
+  c.values = const_cast<int64_t*>(p);
+  return c;
+}
+static fpdb_column col_i32(const int32_t* p, int64_t n) {
+  fpdb_column c{};
+  c.type = FPDB_TYPE_INT32;
+  c.length = n;
+  c.values = const_cast<int32_t*>(p);
+  return c;
+}
+
+// Per-partition launches of the operator entry points (the drop-in path).
+static fpdb_status star_plain(fpdb_star* s) {
+  Context* c = s->ctx;
+  fpdb_context* ctx = reinterpret_cast<fpdb_context*>(c);
+  const fpdb_star_query& q = s->q;
+  const int nd = q.num_dims;
+  const int64_t N = q.fact_rows;
+  StarDev& P = s->dev;
+  // dimensions: filter, forward filters, kept rows
+  std::vector<int64_t> kept_n(nd);
+  std::vector<int64_t*> kept_key(nd);
+  std::vector<int32_t*> kept_g(nd);
+  uint8_t* bm = nullptr;
+  int64_t max_dim = 0;
+  for (int d = 0; d < nd; ++d) max_dim = std::max(max_dim, q.dims[d].key.length);
+  std::vector<void*> tmp;
+  auto talloc = [&](uint64_t bytes, void** p) -> fpdb_status {
+    FPDB_TRY(ctx_alloc(c, bytes, p));
+    tmp.push_back(*p);
+    return FPDB_OK;
+  };
+  auto cleanup = [&]() {
+    for (void* p : tmp) ctx_free(c, p);
+    tmp.clear();
+  };
+  fpdb_status st = FPDB_OK;
+#define STAR_TRY(x)               \
+  do {                            \
+    st = (x);                     \
+    if (st != FPDB_OK) {          \
+      cleanup();                  \
+      return st;                  \
+    }                             \
+  } while (0)
+  STAR_TRY(talloc((uint64_t)((std::max<int64_t>(max_dim, N) + 63) / 64) * 8, reinterpret_cast<void**>(&bm)));
+  int64_t* d_cnt = nullptr;
+  STAR_TRY(talloc(64, reinterpret_cast<void**>(&d_cnt)));
+  for (int d = 0; d < nd; ++d) {
+    const fpdb_star_dimension& D = q.dims[d];
+    int64_t rows = D.key.length;
+    STAR_TRY(fpdb_compare_bitmap(ctx, &D.filter_attr, D.filter_op, D.filter_constant, 0.0, nullptr, rows, bm, d_cnt));
+    int64_t k = 0;
+    FPDB_HIP(hipMemcpyAsync(&k, d_cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    FPDB_HIP(hipStreamSynchronize(c->stream));
+    kept_n[d] = k;
+    s->kept[d] = k;
+    s->fwd_lnb[d] = host_log_blocks(k);
+    STAR_TRY(fpdb_bloom_blocked_build(ctx, &D.key, 1, rows, bm, s->fwd_lnb[d], P.dim[d].fbf));
+    STAR_TRY(talloc((uint64_t)std::max<int64_t>(rows, 1) * 8, reinterpret_cast<void**>(&kept_key[d])));
+    STAR_TRY(talloc((uint64_t)std::max<int64_t>(rows, 1) * 4, reinterpret_cast<void**>(&kept_g[d])));
+    fpdb_column in[2] = {D.key, D.has_group_attr ? D.group_attr : D.filter_attr};
+    fpdb_column out[2] = {col_i64(kept_key[d], rows), col_i32(kept_g[d], rows)};
+    int64_t kk = 0;
+    STAR_TRY(fpdb_filter_compact(ctx, in, 2, rows, bm, out, &kk));
+  }
+  // forward pass, one partition at a time, filters in order
+  const int np = (int)s->parts.size() - 1;
+  std::vector<std::vector<int64_t*>> surv(np);   // per partition: nd fk columns + measure
+  std::vector<int64_t> surv_n(np, 0);
+  int64_t total = 0;
+  for (int p = 0; p < np; ++p) {
+    int64_t lo = s->parts[p], n = s->parts[p + 1] - lo;
+    std::vector<fpdb_column> cur;
+    for (int d = 0; d < nd; ++d) cur.push_back(col_i64(static_cast<const int64_t*>(q.dims[d].fact_fk.values) + lo, n));
+    cur.push_back(col_i64(static_cast<const int64_t*>(q.fact_measure.values) + lo, n));
+    for (int i = 0; i < nd && n > 0; ++i) {
+      int d = q.filter_order[i];
+      std::vector<int64_t*> outp(nd + 1);
+      std::vector<fpdb_column> out;
+      for (int j = 0; j <= nd; ++j) {
+        STAR_TRY(talloc((uint64_t)std::max<int64_t>(n, 1) * 8, reinterpret_cast<void**>(&outp[j])));
+        out.push_back(col_i64(outp[j], n));
+      }
+      int key = d;
+      int64_t kept = 0;
+      STAR_TRY(fpdb_bloom_blocked_filter(ctx, P.dim[d].fbf, s->fwd_lnb[d], cur.data(), nd + 1, &key, 1, n, nullptr,
+                                         out.data(), &kept));
+      n = kept;
+      cur.clear();
+      for (int j = 0; j <= nd; ++j) cur.push_back(col_i64(outp[j], n));
+      surv[p] = outp;
+    }
+    if (surv[p].empty()) {
+      // zero-row partition or no filters: keep the (empty) input columns
+      for (int j = 0; j <= nd; ++j) surv[p].push_back(static_cast<int64_t*>(cur[j].values));
+    }
+    surv_n[p] = n;
+    total += n;
+  }
+  // backward filters over the union of the survivors
+  s->bwd_lnb = host_log_blocks(total);
+  s->survivors = total;
+  for (int d = 0; d < nd; ++d) {
+    STAR_TRY(launch_fill(c, P.dim[d].bbf, 0, (uint64_t)8 << s->bwd_lnb));
+    for (int p = 0; p < np; ++p) {
+      if (!surv_n[p]) continue;
+      fpdb_column k = col_i64(surv[p][d], surv_n[p]);
+      STAR_TRY(fpdb_bloom_blocked_insert(ctx, &k, 1, surv_n[p], nullptr, s->bwd_lnb, P.dim[d].bbf));
+    }
+  }
+  // reduce the dimensions with the backward filters, build the hash tables
+  std::vector<fpdb_hash_table*> tables(nd, nullptr);
+  auto drop_tables = [&]() {
+    for (auto t : tables)
+      if (t) fpdb_hj_table_destroy(t);
+  };
+  for (int d = 0; d < nd; ++d) {
+    int64_t rows = kept_n[d];
+    int64_t* rk = nullptr;
+    int32_t* rg = nullptr;
+    STAR_TRY(talloc((uint64_t)std::max<int64_t>(rows, 1) * 8, reinterpret_cast<void**>(&rk)));
+    STAR_TRY(talloc((uint64_t)std::max<int64_t>(rows, 1) * 4, reinterpret_cast<void**>(&rg)));
+    fpdb_column in[2] = {col_i64(kept_key[d], rows), col_i32(kept_g[d], rows)};
+    fpdb_column out[2] = {col_i64(rk, rows), col_i32(rg, rows)};
+    int key = 0;
+    int64_t kept = 0;
+    st = fpdb_bloom_blocked_filter(ctx, P.dim[d].bbf, s->bwd_lnb, in, 2, &key, 1, rows, nullptr, out, &kept);
+    if (st == FPDB_OK) {
+      fpdb_column kc = col_i64(rk, kept), pc = col_i32(rg, kept);
+      st = fpdb_hj_build(ctx, &kc, 1, kept, &pc, 0, &tables[d]);
+    }
+    if (st != FPDB_OK) {
+      drop_tables();
+      cleanup();
+      return st;
+    }
+  }
+  // joins and group-by, one partition at a time
+  int32_t ktypes[2] = {FPDB_TYPE_INT32, FPDB_TYPE_INT32};
+  fpdb_agg_spec aggs[2] = {{FPDB_AGG_COUNT_ALL, FPDB_TYPE_INT64, 0, 0}, {FPDB_AGG_SUM, FPDB_TYPE_INT64, 0, 0}};
+  fpdb_group_table* gt = nullptr;
+  st = fpdb_groupby_create(ctx, ktypes, 2, aggs, 2, s->max_groups, &gt);
+  if (st != FPDB_OK) {
+    drop_tables();
+    cleanup();
+    return st;
+  }
+  for (int p = 0; p < np && st == FPDB_OK; ++p) {
+    int64_t n = surv_n[p];
+    if (!n) continue;
+    // working columns: the fact columns still needed, then gathered group attributes
+    std::vector<fpdb_column> cur;
+    std::vector<int> what;   // >= 0: fk of dim, -1: measure, -2 - g: group attribute g
+    for (int d = 0; d < nd; ++d) {
+      cur.push_back(col_i64(surv[p][d], n));
+      what.push_back(d);
+    }
+    cur.push_back(col_i64(surv[p][nd], n));
+    what.push_back(-1);
+    for (int j = 0; j < nd && st == FPDB_OK && n > 0; ++j) {
+      int d = q.filter_order[j];
+      int keyi = 0;
+      while (what[keyi] != d) ++keyi;
+      std::vector<fpdb_column> probe_out;
+      std::vector<int> nwhat;
+      for (size_t i = 0; i < cur.size(); ++i) {
+        if ((int)i == keyi) continue;
+        probe_out.push_back(cur[i]);
+        nwhat.push_back(what[i]);
+      }
+      std::vector<fpdb_column> build_out;
+      int g = (d == q.group_dims[0]) ? 0 : (d == q.group_dims[1]) ? 1 : -1;
+      if (g >= 0) {
+        fpdb_column pc{};
+        pc.type = FPDB_TYPE_INT32;
+        build_out.push_back(pc);   // inline payload = group attribute
+        nwhat.push_back(-2 - g);
+      }
+      std::vector<fpdb_column> out;
+      for (size_t i = 0; i < probe_out.size() + build_out.size(); ++i) {
+        void* pbuf = nullptr;
+        st = talloc((uint64_t)n * 8, &pbuf);
+        if (st != FPDB_OK) break;
+        fpdb_column oc = i < probe_out.size() ? probe_out[i] : build_out[i - probe_out.size()];
+        oc.values = pbuf;
+        oc.length = n;
+        oc.offset = 0;
+        oc.validity = nullptr;
+        out.push_back(oc);
+      }
+      if (st != FPDB_OK) break;
+      int64_t produced = 0;
+      st = fpdb_hj_probe(ctx, tables[d], FPDB_JOIN_INNER, &cur[keyi], 1, n, probe_out.data(), (int)probe_out.size(),
+                         build_out.data(), (int)build_out.size(), out.data(), n, &produced);
+      n = produced;
+      for (auto& oc : out) oc.length = n;
+      cur = out;
+      what = nwhat;
+    }
+    if (st != FPDB_OK || n == 0) continue;
+    fpdb_column keys[2], vals[2];
+    for (size_t i = 0; i < cur.size(); ++i) {
+      if (what[i] == -2) keys[0] = cur[i];
+      if (what[i] == -3) keys[1] = cur[i];
+      if (what[i] == -1) vals[1] = cur[i];
+    }
+    vals[0] = fpdb_column{};
+    st = fpdb_groupby_consume(ctx, gt, keys, vals, n);
+  }
+  if (st == FPDB_OK) {
+    int64_t ng = 0;
+    st = fpdb_groupby_num_groups(ctx, gt, &ng);
+    if (st == FPDB_OK && ng > s->max_groups) st = fail(FPDB_ERR_CAPACITY, "star join: too many groups");
+    if (st == FPDB_OK) {
+      fpdb_column ok[2] = {col_i32(s->out_g0, ng), col_i32(s->out_g1, ng)};
+      int64_t* cnt_tmp = s->out_cnt;
+      fpdb_column oa[2] = {col_i64(cnt_tmp, ng), col_i64(s->out_sum, ng)};
+      int64_t written = 0;
+      st = fpdb_groupby_finalize(ctx, gt, ok, oa, &written);
+      unsigned long long hn = (unsigned long long)written;
+      if (st == FPDB_OK) {
+        hipError_t e = hipMemcpyAsync(s->out_n, &hn, 8, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) st = fail(FPDB_ERR_HIP, hipGetErrorString(e));
+      }
+    }
+  }
+  fpdb_groupby_destroy(gt);
+  drop_tables();
+  cleanup();
+#undef STAR_TRY
+  return st;
+}
+
+static fpdb_status star_batched_enqueue(fpdb_star* s) {
+  Context* c = s->ctx;
+  hipLaunchKernelGGL(star_init_kernel, dim3(64), dim3(256), 0, c->stream, s->dev);
+  FPDB_HIP(hipGetLastError());
+  {
+    ProfScope ps(c, "star_persistent");
+    hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(256), 0, c->stream, s->dev);
+  }
+  FPDB_HIP(hipGetLastError());
+  FPDB_TRY(launch_fill(c, s->out_n, 0, 8));
+  hipLaunchKernelGGL(star_emit_kernel, dim3(std::max<int64_t>(1, (s->dev.g_mask + 1 + 255) / 256)), dim3(256), 0,
+                     c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt, s->out_sum, s->out_n);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
+}  // namespace fpdb
+
+using namespace fpdb;
+
+extern "C" {
+
+fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const fpdb_star_tuning* tuning,
+                              fpdb_star** out) {
+  FPDB_CHECK(ctx && q && out, FPDB_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  FPDB_CHECK(q->num_dims >= 1 && q->num_dims <= kStarMaxDims, FPDB_ERR_INVALID, "star join: 1..8 dimensions");
+  FPDB_CHECK(q->num_partitions >= 1 && q->partition_offsets, FPDB_ERR_INVALID, "star join: partitions needed");
+  FPDB_CHECK(q->partition_offsets[0] == 0 && q->partition_offsets[q->num_partitions] == q->fact_rows,
+             FPDB_ERR_INVALID, "star join: partition offsets must cover the fact table");
+  for (int p = 0; p < q->num_partitions; ++p)
+    FPDB_CHECK(q->partition_offsets[p] <= q->partition_offsets[p + 1], FPDB_ERR_INVALID,
+               "star join: partition offsets must not decrease");
+  FPDB_CHECK(q->fact_measure.type == FPDB_TYPE_INT64 && q->fact_measure.length >= q->fact_rows && q->fact_measure.values,
+             FPDB_ERR_INVALID, "star join: int64 measure column needed");
+  FPDB_CHECK(q->group_dims[0] >= 0 && q->group_dims[0] < q->num_dims && q->group_dims[1] >= 0 &&
+                 q->group_dims[1] < q->num_dims && q->group_dims[0] != q->group_dims[1],
+             FPDB_ERR_INVALID, "star join: two distinct group dimensions");
+  std::vector<int> seen(q->num_dims, 0);
+  for (int i = 0; i < q->num_dims; ++i) {
+    int d = q->filter_order[i];
+    FPDB_CHECK(d >= 0 && d < q->num_dims && !seen[d], FPDB_ERR_INVALID, "star join: filter_order is not a permutation");
+    seen[d] = 1;
+    const fpdb_star_dimension& D = q->dims[d];
+    FPDB_CHECK(D.key.type == FPDB_TYPE_INT64 && D.key.values && D.key.offset == 0 && !D.key.validity, FPDB_ERR_INVALID,
+               "star join: dimension keys are non-null int64 columns");
+    FPDB_CHECK(D.filter_attr.type == FPDB_TYPE_INT32 && D.filter_attr.length >= D.key.length && D.filter_attr.values &&
+                   D.filter_attr.offset == 0,
+               FPDB_ERR_INVALID, "star join: int32 filter attribute needed");
+    FPDB_CHECK(D.filter_op >= FPDB_CMP_LT && D.filter_op <= FPDB_CMP_NE, FPDB_ERR_INVALID, "star join: bad filter op");
+    FPDB_CHECK(D.fact_fk.type == FPDB_TYPE_INT64 && D.fact_fk.length >= q->fact_rows && D.fact_fk.values &&
+                   D.fact_fk.offset == 0 && !D.fact_fk.validity,
+               FPDB_ERR_INVALID, "star join: non-null int64 fact foreign keys needed");
+    FPDB_CHECK(D.key.length < (int64_t)kStarEmptyRow, FPDB_ERR_INVALID, "star join: dimension too large");
+  }
+  for (int g = 0; g < 2; ++g) {
+    const fpdb_star_dimension& D = q->dims[q->group_dims[g]];
+    FPDB_CHECK(D.has_group_attr && D.group_attr.type == FPDB_TYPE_INT32 && D.group_attr.values &&
+                   D.group_attr.length >= D.key.length && D.group_attr.offset == 0,
+               FPDB_ERR_INVALID, "star join: group dimensions need an int32 group attribute");
+  }
+  FPDB_CHECK(q->max_groups >= 1 && q->max_groups <= (1 << 20), FPDB_ERR_INVALID, "star join: max_groups in [1, 2^20]");
+  Context* c = as_ctx(ctx);
+  auto* s = new fpdb_star();
+  s->ctx = c;
+  s->q = *q;
+  s->parts.assign(q->partition_offsets, q->partition_offsets + q->num_partitions + 1);
+  s->q.partition_offsets = nullptr;
+  s->tune = tuning ? *tuning : fpdb_star_tuning{0, 0, 1, 0};
+  if (s->tune.ht_load_shift < 1 || s->tune.ht_load_shift > 3) s->tune.ht_load_shift = 1;
+  s->max_groups = q->max_groups;
+  StarDev& P = s->dev;
+  std::memset(&P, 0, sizeof(P));
+  P.ndims = q->num_dims;
+  for (int i = 0; i < q->num_dims; ++i) P.order[i] = q->filter_order[i];
+  P.gdim[0] = q->group_dims[0];
+  P.gdim[1] = q->group_dims[1];
+  P.measure = static_cast<const int64_t*>(q->fact_measure.values);
+  P.fact_rows = q->fact_rows;
+  fpdb_status st = FPDB_OK;
+  int blnb = host_log_blocks(q->fact_rows);
+  for (int d = 0; d < q->num_dims && st == FPDB_OK; ++d) {
+    const fpdb_star_dimension& D = q->dims[d];
+    StarDimDev& X = P.dim[d];
+    X.key = static_cast<const int64_t*>(D.key.values);
+    X.attr = static_cast<const int32_t*>(D.filter_attr.values);
+    X.gattr = D.has_group_attr ? static_cast<const int32_t*>(D.group_attr.values) : nullptr;
+    X.rows = D.key.length;
+    X.cst = D.filter_constant;
+    X.op = D.filter_op;
+    X.fk = static_cast<const int64_t*>(D.fact_fk.values);
+    int64_t cap = 64;
+    while (cap < (X.rows << s->tune.ht_load_shift)) cap <<= 1;
+    X.ht_mask = (uint64_t)cap - 1;
+    st = star_alloc(s, (uint64_t)std::max<int64_t>(X.rows, 1), reinterpret_cast<void**>(&X.kept));
+    if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << host_log_blocks(X.rows), reinterpret_cast<void**>(&X.fbf));
+    if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << blnb, reinterpret_cast<void**>(&X.bbf));
+    if (st == FPDB_OK) st = star_alloc(s, (uint64_t)cap * sizeof(StarSlot), reinterpret_cast<void**>(&X.ht));
+  }
+  int64_t N = std::max<int64_t>(q->fact_rows, 1);
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8, reinterpret_cast<void**>(&P.s_idx));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8 * q->num_dims, reinterpret_cast<void**>(&P.s_fk));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8, reinterpret_cast<void**>(&P.s_meas));
+  if (st == FPDB_OK) st = star_alloc(s, 64 * 8, reinterpret_cast<void**>(&P.state));
+  int64_t gcap = 64;
+  while (gcap < 2 * q->max_groups) gcap <<= 1;
+  P.g_mask = (uint64_t)gcap - 1;
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)(gcap + 1) * 8, reinterpret_cast<void**>(&P.g_keys));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)(gcap + 1) * 8, reinterpret_cast<void**>(&P.g_cnt));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)(gcap + 1) * 8, reinterpret_cast<void**>(&P.g_sum));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)(gcap + 1) * 8, reinterpret_cast<void**>(&P.g_first));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 4, reinterpret_cast<void**>(&s->out_g0));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 4, reinterpret_cast<void**>(&s->out_g1));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 8, reinterpret_cast<void**>(&s->out_cnt));
+  if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 8, reinterpret_cast<void**>(&s->out_sum));
+  if (st == FPDB_OK) st = star_alloc(s, 8, reinterpret_cast<void**>(&s->out_n));
+  if (st != FPDB_OK) {
+    fpdb_star_destroy(s);
+    return st;
+  }
+  // persistent grid: every workgroup resident at once
+  int per_cu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, star_persistent_kernel, 256, 0);
+  if (e != hipSuccess || per_cu < 1) {
+    fpdb_star_destroy(s);
+    return fail(FPDB_ERR_HIP, "star join: occupancy query failed");
+  }
+  int want = s->tune.blocks_per_cu > 0 ? s->tune.blocks_per_cu : 4;
+  s->grid = std::max(1, c->num_cus) * std::min(want, per_cu);
+  *out = s;
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_star_destroy(fpdb_star* s) {
+  if (!s) return FPDB_OK;
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (void* p : s->owned) ctx_free(s->ctx, p);
+  delete s;
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_star_run_plain(fpdb_context* ctx, fpdb_star* s) {
+  FPDB_CHECK(ctx && s && as_ctx(ctx) == s->ctx, FPDB_ERR_INVALID, "star join: bad context");
+  s->last_mode = 1;
+  return star_plain(s);
+}
+
+fpdb_status fpdb_star_run_batched(fpdb_context* ctx, fpdb_star* s) {
+  FPDB_CHECK(ctx && s && as_ctx(ctx) == s->ctx, FPDB_ERR_INVALID, "star join: bad context");
+  s->last_mode = 2;
+  return star_batched_enqueue(s);
+}
+
+fpdb_status fpdb_star_graph_capture(fpdb_context* ctx, fpdb_star* s) {
+  FPDB_CHECK(ctx && s && as_ctx(ctx) == s->ctx, FPDB_ERR_INVALID, "star join: bad context");
+  Context* c = s->ctx;
+  FPDB_CHECK(c->own_stream, FPDB_ERR_INVALID, "star join: graph capture needs the context's own stream");
+  if (s->graph_exec) return FPDB_OK;
+  bool prof = c->profiling;
+  c->profiling = false;
+  FPDB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  fpdb_status st = star_batched_enqueue(s);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->profiling = prof;
+  FPDB_TRY(st);
+  if (e != hipSuccess) return fail(FPDB_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+  s->graph = g;
+  FPDB_HIP(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_star_graph_launch(fpdb_context* ctx, fpdb_star* s) {
+  FPDB_CHECK(ctx && s && s->graph_exec, FPDB_ERR_INVALID, "star join: no captured graph");
+  ProfScope ps(s->ctx, "star_graph");
+  s->last_mode = 2;
+  FPDB_HIP(hipGraphLaunch(s->graph_exec, s->ctx->stream));
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_star_result(fpdb_context* ctx, fpdb_star* s, int32_t* g0, int32_t* g1, int64_t* count,
+                             int64_t* sum, int64_t capacity, int64_t* num_groups) {
+  FPDB_CHECK(ctx && s && num_groups, FPDB_ERR_INVALID, "NULL argument");
+  Context* c = s->ctx;
+  unsigned long long n = 0;
+  unsigned long long err = 0;
+  FPDB_HIP(hipMemcpyAsync(&n, s->out_n, 8, hipMemcpyDeviceToHost, c->stream));
+  FPDB_HIP(hipMemcpyAsync(&err, s->dev.state + 32, 8, hipMemcpyDeviceToHost, c->stream));
+  FPDB_HIP(hipStreamSynchronize(c->stream));
+  FPDB_CHECK(err == 0, FPDB_ERR_INTERNAL, "star join: a grid barrier timed out or the group table overflowed");
+  *num_groups = (int64_t)n;
+  FPDB_CHECK((int64_t)n <= capacity, FPDB_ERR_CAPACITY, "star join: result capacity too small");
+  if (n) {
+    if (g0) FPDB_HIP(hipMemcpy(g0, s->out_g0, n * 4, hipMemcpyDeviceToHost));
+    if (g1) FPDB_HIP(hipMemcpy(g1, s->out_g1, n * 4, hipMemcpyDeviceToHost));
+    if (count) FPDB_HIP(hipMemcpy(count, s->out_cnt, n * 8, hipMemcpyDeviceToHost));
+    if (sum) FPDB_HIP(hipMemcpy(sum, s->out_sum, n * 8, hipMemcpyDeviceToHost));
+  }
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_star_filter(fpdb_context* ctx, fpdb_star* s, int dim, int backward, uint64_t* blocks,
+                             int64_t capacity, int* log_num_blocks, int64_t* filter_input_rows) {
+  FPDB_CHECK(ctx && s && log_num_blocks, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(dim >= 0 && dim < s->q.num_dims, FPDB_ERR_INVALID, "star join: bad dimension");
+  Context* c = s->ctx;
+  FPDB_CHECK(s->last_mode != 0, FPDB_ERR_INVALID, "star join: not run yet");
+  int64_t rows;
+  if (s->last_mode == 2) {
+    unsigned long long st[16];
+    FPDB_HIP(hipMemcpyAsync(st, s->dev.state, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    FPDB_HIP(hipStreamSynchronize(c->stream));
+    rows = backward ? (int64_t)st[0] : (int64_t)st[1 + dim];
+  } else {
+    rows = backward ? s->survivors : s->kept[dim];
+  }
+  int lnb = fpdb_bloom_blocked_log_num_blocks(rows);
+  *log_num_blocks = lnb;
+  if (filter_input_rows) *filter_input_rows = rows;
+  FPDB_CHECK(capacity >= (int64_t(1) << lnb) || !blocks, FPDB_ERR_CAPACITY, "star join: filter buffer too small");
+  if (blocks)
+    FPDB_HIP(hipMemcpy(blocks, backward ? s->dev.dim[dim].bbf : s->dev.dim[dim].fbf, (size_t)8 << lnb,
+                       hipMemcpyDeviceToHost));
+  return FPDB_OK;
+}
+
+}  // extern "C"

@@ -92,6 +92,61 @@ def zipf_keys(seed, column, n, n_values=100_000, s=1.1):
     return perm[r].astype(np.int32)
 
 
+def lognormal_partitions(n_rows, n_parts, seed, column=50):
+    """Partition sizes proportional to lognormal(0, 1) draws (Box-Muller on
+    the counter RNG), normalised to n_rows by largest remainder.  Returns the
+    n_parts + 1 row offsets."""
+    u1 = np.maximum(uniform_unit(seed, column, n_parts), 1e-300)
+    u2 = uniform_unit(seed, column + 1, n_parts)
+    z = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    w = np.exp(z)
+    exact = w / w.sum() * n_rows
+    sizes = np.floor(exact).astype(np.int64)
+    rest = n_rows - int(sizes.sum())
+    order = np.argsort(-(exact - sizes), kind="stable")
+    sizes[order[:rest]] += 1
+    return np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+
+
+# star join: four dimensions of 30,000 / 200,000 / 2,000,000 / 2,556 rows kept
+# at 20% / 4% / 1% / 14%, group attributes on dimensions 0 and 3
+STAR_DIMS = (30_000, 200_000, 2_000_000, 2_556)
+STAR_KEEP_BELOW = (20, 4, 1, 14)
+STAR_GROUP_DOMAIN = {0: 40, 3: 12}
+
+
+def star_workload(n_fact=60_000_000, n_parts=64, seed=4, dims=STAR_DIMS, keep_below=STAR_KEEP_BELOW):
+    out = {"dims": [], "fact_fk": [], "offsets": lognormal_partitions(n_fact, n_parts, seed)}
+    for d, rows in enumerate(dims):
+        dim = {"key": np.arange(rows, dtype=np.int64),
+               "attr": uniform_int(seed, 10 + d, rows, 0, 99).astype(np.int32),
+               "keep_below": keep_below[d]}
+        if d in STAR_GROUP_DOMAIN:
+            dim["group"] = uniform_int(seed, 20 + d, rows, 0, STAR_GROUP_DOMAIN[d] - 1).astype(np.int32)
+        out["dims"].append(dim)
+        out["fact_fk"].append(uniform_int(seed, 30 + d, n_fact, 0, rows - 1))
+    out["measure"] = uniform_int(seed, 40, n_fact, 1, 100)
+    return out
+
+
+def star_expected(w, group_dims=(0, 3)):
+    """SELECT g_a, g_b, COUNT(*), SUM(measure) ... in first-appearance order,
+    by direct indexing of the dense dimension keys."""
+    keep = np.ones(len(w["measure"]), bool)
+    for d, dim in enumerate(w["dims"]):
+        keep &= (dim["attr"] < dim["keep_below"])[w["fact_fk"][d]]
+    ga = w["dims"][group_dims[0]]["group"][w["fact_fk"][group_dims[0]][keep]].astype(np.int64)
+    gb = w["dims"][group_dims[1]]["group"][w["fact_fk"][group_dims[1]][keep]].astype(np.int64)
+    meas = w["measure"][keep]
+    key = ga * (1 << 32) + gb
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    cnt = np.bincount(inv, minlength=len(uniq))
+    sm = np.bincount(inv, weights=meas, minlength=len(uniq)).astype(np.int64)
+    order = np.argsort(first, kind="stable")
+    return {"g_a": ga[first[order]], "g_b": gb[first[order]], "count": cnt[order], "sum": sm[order],
+            "survivors": int(keep.sum())}
+
+
 def groupby_workload(n=60_000_000, seed=3, n_values=100_000):
     return {"key": zipf_keys(seed, 1, n, n_values),
             "qty": uniform_int(seed, 2, n, 1, 50),

@@ -358,6 +358,80 @@ fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* table,
                                   fpdb_column* out_keys, fpdb_column* out_aggs,
                                   int64_t* num_groups);
 
+/* ---- batched star join with predicate transfer ------------------------------ */
+
+/* One dimension of a star join: dimension keys (non-null int64), a filter
+ * `filter_attr <filter_op> filter_constant` on an int32 attribute, an optional
+ * int32 group attribute, and the fact table's foreign-key column for it. */
+typedef struct fpdb_star_dimension {
+  fpdb_column key;
+  fpdb_column filter_attr;
+  int32_t filter_op;            /* fpdb_cmp_op */
+  int32_t has_group_attr;
+  int64_t filter_constant;
+  fpdb_column group_attr;
+  fpdb_column fact_fk;
+} fpdb_star_dimension;
+
+/* SELECT g_a, g_b, COUNT(*), SUM(measure) FROM fact JOIN dims ... GROUP BY g_a, g_b
+ * with full predicate transfer: forward blocked bloom filters from every
+ * filtered dimension into the fact table, backward filters from the surviving
+ * fact rows into every dimension, then the joins and the group-by; the plan
+ * shape SmallToLargePredTransOrder builds
+ * (fpdb-executor/src/physical/transform/pred-trans/SmallToLargePredTransOrder.cpp:38-214),
+ * over a fact table split into partitions (the reference supports one
+ * partition, :45-48): forward filters cover all of a dimension's kept rows,
+ * backward filters the union of all partitions' survivors, and groups are
+ * numbered by first appearance in fact order. */
+typedef struct fpdb_star_query {
+  int32_t num_dims;                   /* 1..8 */
+  int32_t group_dims[2];              /* dimensions whose group_attr forms the key */
+  int32_t filter_order[8];            /* order of the forward filters on the fact rows */
+  fpdb_star_dimension dims[8];
+  fpdb_column fact_measure;           /* non-null int64 */
+  int64_t fact_rows;
+  int32_t num_partitions;
+  int32_t reserved;
+  const int64_t* partition_offsets;   /* host, num_partitions + 1 row offsets */
+  int64_t max_groups;
+} fpdb_star_query;
+
+/* Launch shape of the batched form (0 = default): workgroups per CU of the
+ * persistent kernel, and hash-table capacity as a power-of-two multiple
+ * (2^ht_load_shift) of the dimension rows.  Chosen by the autotuner. */
+typedef struct fpdb_star_tuning {
+  int32_t blocks_per_cu;
+  int32_t reserved0;
+  int32_t ht_load_shift;
+  int32_t reserved1;
+} fpdb_star_tuning;
+
+typedef struct fpdb_star fpdb_star;
+
+/* Allocates every buffer of both executions up front (no allocation while
+ * running, so the batched form can be captured in a graph).  No counterpart in
+ * the reference, which plans the same chain as separate operators. */
+fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* query, const fpdb_star_tuning* tuning,
+                              fpdb_star** out);
+fpdb_status fpdb_star_destroy(fpdb_star* star);
+/* The plain form: per-partition launches of the operator entry points
+ * (fpdb_bloom_blocked_filter, fpdb_bloom_blocked_insert, fpdb_hj_build,
+ * fpdb_hj_probe, fpdb_groupby_*), synchronizing as the operators do. */
+fpdb_status fpdb_star_run_plain(fpdb_context* ctx, fpdb_star* star);
+/* The batched form: one persistent launch over all partitions (asynchronous). */
+fpdb_status fpdb_star_run_batched(fpdb_context* ctx, fpdb_star* star);
+/* Captures the batched form in a hipGraph once; launches replay it. */
+fpdb_status fpdb_star_graph_capture(fpdb_context* ctx, fpdb_star* star);
+fpdb_status fpdb_star_graph_launch(fpdb_context* ctx, fpdb_star* star);
+/* Result of the last run (host arrays with room for `capacity` groups). */
+fpdb_status fpdb_star_result(fpdb_context* ctx, fpdb_star* star, int32_t* g_a, int32_t* g_b, int64_t* count,
+                             int64_t* sum, int64_t capacity, int64_t* num_groups);
+/* A forward (backward = 0) or backward filter of the last run, for parity
+ * checks: its blocks (if `blocks` is not NULL), log2 block count and the
+ * number of input rows it was sized for. */
+fpdb_status fpdb_star_filter(fpdb_context* ctx, fpdb_star* star, int dim, int backward, uint64_t* blocks,
+                             int64_t capacity, int* log_num_blocks, int64_t* filter_input_rows);
+
 #ifdef __cplusplus
 }
 #endif
