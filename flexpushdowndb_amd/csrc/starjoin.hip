@@ -328,7 +328,7 @@ __global__ void star_init_kernel(StarDev P) {
 
 // Ranks the groups by first appearance and writes the result columns.
 __global__ void star_emit_kernel(StarDev P, int32_t* out_g0, int32_t* out_g1, int64_t* out_cnt,
-                                 int64_t* out_sum, unsigned long long* out_n) {
+                                 int64_t* out_sum, unsigned long long* out_n, int64_t max_groups) {
   const int64_t cap = (int64_t)P.g_mask + 2;
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += (int64_t)gridDim.x * blockDim.x) {
     long long f = P.g_first[s];
@@ -338,7 +338,11 @@ __global__ void star_emit_kernel(StarDev P, int32_t* out_g0, int32_t* out_g1, in
       long long o = P.g_first[t];
       rank += (o < f) ? 1 : 0;
     }
-    uint64_t k = P.g_keys[s];
+    if (rank >= max_groups) {   // more groups than the result buffers hold
+      __hip_atomic_store(P.state + 32, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      continue;
+    }
+    uint64_t k = (s == cap - 1) ? kStarEmptyKey : P.g_keys[s];
     out_g0[rank] = (int32_t)(uint32_t)(k & 0xFFFFFFFFULL);
     out_g1[rank] = (int32_t)(uint32_t)(k >> 32);
     out_cnt[rank] = (int64_t)P.g_cnt[s];
@@ -638,7 +642,7 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
   FPDB_HIP(hipGetLastError());
   FPDB_TRY(launch_fill(c, s->out_n, 0, 8));
   hipLaunchKernelGGL(star_emit_kernel, dim3(std::max<int64_t>(1, (s->dev.g_mask + 1 + 255) / 256)), dim3(256), 0,
-                     c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt, s->out_sum, s->out_n);
+                     c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt, s->out_sum, s->out_n, s->max_groups);
   FPDB_HIP(hipGetLastError());
   return FPDB_OK;
 }
