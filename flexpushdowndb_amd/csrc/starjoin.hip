@@ -747,6 +747,7 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 8, reinterpret_cast<void**>(&s->out_cnt));
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 8, reinterpret_cast<void**>(&s->out_sum));
   if (st == FPDB_OK) st = star_alloc(s, 8, reinterpret_cast<void**>(&s->out_n));
+  if (st == FPDB_OK) st = launch_fill(c, P.state, 0, 64 * 8);   // no error recorded before the first run
   if (st != FPDB_OK) {
     fpdb_star_destroy(s);
     return st;
@@ -819,7 +820,8 @@ fpdb_status fpdb_star_result(fpdb_context* ctx, fpdb_star* s, int32_t* g0, int32
   unsigned long long n = 0;
   unsigned long long err = 0;
   FPDB_HIP(hipMemcpyAsync(&n, s->out_n, 8, hipMemcpyDeviceToHost, c->stream));
-  FPDB_HIP(hipMemcpyAsync(&err, s->dev.state + 32, 8, hipMemcpyDeviceToHost, c->stream));
+  // the error word belongs to the batched form; the plain form reports errors from its own calls
+  if (s->last_mode == 2) FPDB_HIP(hipMemcpyAsync(&err, s->dev.state + 32, 8, hipMemcpyDeviceToHost, c->stream));
   FPDB_HIP(hipStreamSynchronize(c->stream));
   FPDB_CHECK(err == 0, FPDB_ERR_INTERNAL, "star join: a grid barrier timed out or the group table overflowed");
   *num_groups = (int64_t)n;
