@@ -74,6 +74,7 @@ class StarTuning(ctypes.Structure):
 
 
 CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
+CMP_OR = 0x10
 
 
 class FpdbError(RuntimeError):
@@ -132,7 +133,10 @@ SIGNATURES = [
     ("fpdb_gather", _I32, [_P, _PCOL, _I, _P, _I64, _PCOL]),
     ("fpdb_copy_rows", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I64]),
     ("fpdb_groupby_create", _I32, [_P, ctypes.POINTER(_I32), _I, ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),
+    ("fpdb_groupby_create_packed", _I32, [_P, ctypes.POINTER(_I32), _I, _PI64, ctypes.POINTER(_I32),
+                                          ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),
     ("fpdb_groupby_destroy", _I32, [_P]),
+    ("fpdb_column_min_max", _I32, [_P, _PCOL, _I64, _PI64, _PI64, _PI64]),
     ("fpdb_groupby_consume", _I32, [_P, _P, _PCOL, _PCOL, _I64]),
     ("fpdb_groupby_num_groups", _I32, [_P, _P, _PI64]),
     ("fpdb_groupby_output_type", _I32, [AggSpec]),

@@ -248,10 +248,10 @@ __global__ __launch_bounds__(256) void legacy_find_kernel(LegacyParams P, int64_
 }
 
 // column <op> constant -> selection bitmap (nulls never selected), optionally
-// AND-ed with an existing bitmap.
-__global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, int64_t ci, double cf,
-                                                      const uint8_t* __restrict__ and_bm, int64_t n,
-                                                      uint8_t* __restrict__ bitmap,
+// AND-ed (or OR-ed, for_or) with an existing bitmap.  Each wave reads and
+// writes only its own 64-row words, so the existing bitmap may be the output.
+__global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, bool for_or, int64_t ci, double cf,
+                                                      const uint8_t* and_bm, int64_t n, uint8_t* bitmap,
                                                       int64_t* __restrict__ num_selected) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -287,7 +287,8 @@ __global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, in
         case FPDB_CMP_EQ: sel = cmp == 0; break;
         default: sel = cmp != 0; break;
       }
-      sel = sel && valid_bit(col.validity[0], i) && (and_bm == nullptr || bitmap_bit(and_bm, r));
+      sel = sel && valid_bit(col.validity[0], i);
+      if (and_bm) sel = for_or ? (sel || bitmap_bit(and_bm, r)) : (sel && bitmap_bit(and_bm, r));
     }
     uint64_t word = __ballot(sel);
     write_bitmap_word(bitmap, c, word, n, lane);
@@ -535,14 +536,17 @@ fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, in
                                 double float_constant, const uint8_t* d_and_bitmap, int64_t num_rows,
                                 uint8_t* d_bitmap, int64_t* d_num_selected) {
   FPDB_CHECK(ctx && column && d_bitmap, FPDB_ERR_INVALID, "NULL argument");
+  const bool for_or = (op & FPDB_CMP_OR) != 0;
+  op &= ~FPDB_CMP_OR;
   FPDB_CHECK(op >= FPDB_CMP_LT && op <= FPDB_CMP_NE, FPDB_ERR_INVALID, "unknown comparison");
+  FPDB_CHECK(!for_or || d_and_bitmap, FPDB_ERR_INVALID, "FPDB_CMP_OR needs a bitmap to combine with");
   Context* c = as_ctx(ctx);
   ColSet cs;
   FPDB_TRY(make_colset(column, 1, num_rows, &cs, "compare"));
   if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
   if (num_rows == 0) return FPDB_OK;
   ProfScope ps(c, "compare");
-  hipLaunchKernelGGL(compare_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, cs, op,
+  hipLaunchKernelGGL(compare_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, cs, op, for_or,
                      int_constant, float_constant, d_and_bitmap, num_rows, d_bitmap, d_num_selected);
   FPDB_HIP(hipGetLastError());
   return FPDB_OK;

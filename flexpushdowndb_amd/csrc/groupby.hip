@@ -50,23 +50,42 @@ struct AccSpec {
   int32_t n;
 };
 
+// The group key of a row is one 64-bit word: key column j contributes
+// (value - bias[j]) in bits[j] bits at bit position shift[j].  One column is
+// the value itself (bias 0, 64 bits); two columns of at most 4 bytes are the
+// low 32 bits of each (bias 0, 32 bits); packed keys take caller-given ranges
+// and flag values outside them (check).  No key column: every row is group 0.
+constexpr int kMaxGroupKeys = 4;
+
 struct GKeySpec {
-  const uint8_t* v[2];
-  int64_t off[2];
-  int32_t w[2];
-  int32_t sgn[2];
+  const uint8_t* v[kMaxGroupKeys];
+  int64_t off[kMaxGroupKeys];
+  int64_t bias[kMaxGroupKeys];
+  int32_t w[kMaxGroupKeys];
+  int32_t sgn[kMaxGroupKeys];
+  int32_t bits[kMaxGroupKeys];
+  int32_t shift[kMaxGroupKeys];
   int32_t n;
+  int32_t check;
 };
 
-__device__ __forceinline__ uint64_t group_key(const GKeySpec& k, int64_t r) {
-  if (k.n == 1) {
-    int64_t i = r + k.off[0];
-    return k.sgn[0] ? (uint64_t)load_sext(k.v[0], k.w[0], i) : load_zext(k.v[0], k.w[0], i);
+__host__ __device__ __forceinline__ uint64_t key_field_mask(int bits) {
+  return bits >= 64 ? ~0ULL : ((1ULL << bits) - 1);
+}
+
+__device__ __forceinline__ uint64_t group_key(const GKeySpec& k, int64_t r, bool* out_of_range) {
+  uint64_t key = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxGroupKeys; ++j) {
+    if (j >= k.n) break;
+    int64_t i = r + k.off[j];
+    int64_t v = k.sgn[j] ? load_sext(k.v[j], k.w[j], i) : (int64_t)load_zext(k.v[j], k.w[j], i);
+    uint64_t d = (uint64_t)v - (uint64_t)k.bias[j];
+    uint64_t m = key_field_mask(k.bits[j]);
+    if (k.check && d > m) *out_of_range = true;
+    key |= (d & m) << k.shift[j];
   }
-  int64_t i0 = r + k.off[0], i1 = r + k.off[1];
-  uint32_t a = k.sgn[0] ? (uint32_t)load_sext(k.v[0], k.w[0], i0) : (uint32_t)load_zext(k.v[0], k.w[0], i0);
-  uint32_t b = k.sgn[1] ? (uint32_t)load_sext(k.v[1], k.w[1], i1) : (uint32_t)load_zext(k.v[1], k.w[1], i1);
-  return (uint64_t)a | ((uint64_t)b << 32);
+  return key;
 }
 
 __host__ __device__ __forceinline__ uint64_t acc_identity(int32_t op) {
@@ -157,6 +176,7 @@ struct GlobalTable {
   int32_t nrep;
   unsigned long long* groups;  // number of claimed slots of replica 0
   int* overflow;
+  int* key_range;              // set when a packed key value lies outside its range
 };
 
 __device__ __forceinline__ uint64_t* gkey(const GlobalTable& g, int rep, int64_t s) {
@@ -218,8 +238,9 @@ __global__ __launch_bounds__(kGroupThreads) void groupby_consume_kernel(GKeySpec
   const int64_t per_block = ceil_div_d(n, gridDim.x);
   const int64_t begin = (int64_t)blockIdx.x * per_block;
   const int64_t end = begin + per_block < n ? begin + per_block : n;
+  bool out_of_range = false;
   for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
-    uint64_t k = group_key(keys, r);
+    uint64_t k = group_key(keys, r, &out_of_range);
     uint64_t v[kMaxAggs];
     bool ok[kMaxAggs];
 #pragma unroll
@@ -257,6 +278,7 @@ __global__ __launch_bounds__(kGroupThreads) void groupby_consume_kernel(GKeySpec
       }
     }
   }
+  if (out_of_range) *g.key_range = 1;
   __syncthreads();
   // merge the LDS table into the global table
   for (int s = threadIdx.x; s < lds_slots; s += blockDim.x) {
@@ -372,9 +394,12 @@ __global__ __launch_bounds__(1024) void word_prefix_kernel(const unsigned long l
 }
 
 struct GOutSpec {
-  uint8_t* key_values[2];
-  int32_t key_w[2];
-  int64_t key_off[2];
+  uint8_t* key_values[kMaxGroupKeys];
+  int32_t key_w[kMaxGroupKeys];
+  int64_t key_off[kMaxGroupKeys];
+  int64_t key_bias[kMaxGroupKeys];
+  int32_t key_bits[kMaxGroupKeys];
+  int32_t key_shift[kMaxGroupKeys];
   uint8_t* agg_values[kMaxAggs];
   uint8_t* agg_validity[kMaxAggs];
   int64_t agg_off[kMaxAggs];
@@ -395,11 +420,9 @@ __global__ void groupby_emit_kernel(GlobalTable g, AccSpec A, const unsigned lon
     uint64_t below = (f & 63) ? (w & (~0ULL >> (64 - (f & 63)))) : 0ULL;
     int64_t rank = (int64_t)word_prefix[f >> 6] + __popcll(below);
     uint64_t key = (s == g.stride - 1) ? kEmptyKey : g.keys[s];
-    if (o.num_keys == 1) {
-      store_w(o.key_values[0], o.key_w[0], rank + o.key_off[0], key);
-    } else {
-      store_w(o.key_values[0], o.key_w[0], rank + o.key_off[0], key & 0xFFFFFFFFULL);
-      store_w(o.key_values[1], o.key_w[1], rank + o.key_off[1], key >> 32);
+    for (int j = 0; j < o.num_keys; ++j) {
+      uint64_t v = ((key >> o.key_shift[j]) & key_field_mask(o.key_bits[j])) + (uint64_t)o.key_bias[j];
+      store_w(o.key_values[j], o.key_w[j], rank + o.key_off[j], v);
     }
     for (int a = 0; a < o.num_aggs; ++a) {
       uint64_t v = g.acc[(size_t)a * g.stride + s];
@@ -422,12 +445,52 @@ __global__ void groupby_emit_kernel(GlobalTable g, AccSpec A, const unsigned lon
   }
 }
 
+// Minimum and maximum of the valid values of an integer column, in an
+// order-preserving unsigned encoding (signed values with the sign bit flipped)
+// so one unsigned atomic min/max serves both.  out: [min, max, valid count].
+__global__ void range_init_kernel(unsigned long long* out) {
+  if (threadIdx.x == 0) {
+    out[0] = ~0ULL;
+    out[1] = 0ULL;
+    out[2] = 0ULL;
+  }
+}
+
+__global__ __launch_bounds__(256) void range_kernel(ColSet col, int64_t n, unsigned long long* out) {
+  const bool sgn = type_is_signed_int_d(col.type[0]);
+  unsigned long long lo = ~0ULL, hi = 0ULL, cnt = 0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = r + col.offset[0];
+    if (!valid_bit(col.validity[0], i)) continue;
+    unsigned long long u = sgn ? ((unsigned long long)load_sext(col.values[0], col.width[0], i) ^ (1ULL << 63))
+                               : (unsigned long long)load_zext(col.values[0], col.width[0], i);
+    lo = u < lo ? u : lo;
+    hi = u > hi ? u : hi;
+    ++cnt;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    unsigned long long olo = __shfl_xor(lo, d), ohi = __shfl_xor(hi, d), oc = __shfl_xor(cnt, d);
+    lo = olo < lo ? olo : lo;
+    hi = ohi > hi ? ohi : hi;
+    cnt += oc;
+  }
+  if ((threadIdx.x & 63) == 0 && cnt) {
+    atomicMin(out + 0, lo);
+    atomicMax(out + 1, hi);
+    atomicAdd(out + 2, cnt);
+  }
+}
+
 }  // namespace fpdb
 
 struct fpdb_group_table {
   fpdb::Context* ctx = nullptr;
   int num_keys = 0;
-  int32_t key_types[2] = {0, 0};
+  int32_t key_types[fpdb::kMaxGroupKeys] = {0, 0, 0, 0};
+  int64_t key_bias[fpdb::kMaxGroupKeys] = {0, 0, 0, 0};
+  int32_t key_bits[fpdb::kMaxGroupKeys] = {0, 0, 0, 0};
+  int32_t key_shift[fpdb::kMaxGroupKeys] = {0, 0, 0, 0};
+  int key_check = 0;                 // packed keys: values outside their ranges are an error
   int num_aggs = 0;
   fpdb_agg_spec aggs[fpdb::kMaxAggs];
   fpdb::AccSpec acc;                 // visible aggregates first, then hidden valid counts
@@ -437,7 +500,7 @@ struct fpdb_group_table {
   uint64_t* accs = nullptr;
   int64_t* first = nullptr;
   int64_t rows = 0;
-  unsigned long long* d_groups = nullptr;   // [0] groups, [1] overflow flag
+  unsigned long long* d_groups = nullptr;   // [0] groups, [1] overflow flag (int) + key range flag (int)
 };
 
 namespace fpdb {
@@ -453,6 +516,7 @@ static GlobalTable global_view(fpdb_group_table* t) {
   g.nrep = kReplicas;
   g.groups = t->d_groups;
   g.overflow = reinterpret_cast<int*>(t->d_groups + 1);
+  g.key_range = reinterpret_cast<int*>(t->d_groups + 1) + 1;
   return g;
 }
 
@@ -490,26 +554,33 @@ int32_t fpdb_groupby_output_type(fpdb_agg_spec s) {
   }
 }
 
-fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int num_keys,
-                                const fpdb_agg_spec* aggs, int num_aggs, int64_t max_groups,
-                                fpdb_group_table** out) {
-  FPDB_CHECK(ctx && out, FPDB_ERR_INVALID, "NULL argument");
-  *out = nullptr;
-  FPDB_CHECK(num_keys == 1 || num_keys == 2, FPDB_ERR_INVALID, "group-by: one or two key columns");
-  FPDB_CHECK(key_types, FPDB_ERR_INVALID, "group-by: NULL key types");
-  for (int i = 0; i < num_keys; ++i) {
-    int w = type_width(key_types[i]);
-    FPDB_CHECK(w > 0 && !type_is_float(key_types[i]), FPDB_ERR_INVALID, "group-by: keys must be integers");
-    FPDB_CHECK(num_keys == 1 || w <= 4, FPDB_ERR_INVALID,
-               "group-by: two-column keys need columns of at most 4 bytes");
-  }
+}  // extern "C"
+
+namespace fpdb {
+
+static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_types, int num_keys,
+                                      const int64_t* key_min, const int32_t* key_bits, const fpdb_agg_spec* aggs,
+                                      int num_aggs, int64_t max_groups, fpdb_group_table** out) {
   FPDB_CHECK(num_aggs >= 0 && (num_aggs == 0 || aggs), FPDB_ERR_INVALID, "group-by: bad aggregates");
   FPDB_CHECK(max_groups >= 1, FPDB_ERR_INVALID, "group-by: max_groups must be positive");
   Context* c = as_ctx(ctx);
   auto* t = new fpdb_group_table();
   t->ctx = c;
   t->num_keys = num_keys;
-  for (int i = 0; i < num_keys; ++i) t->key_types[i] = key_types[i];
+  int shift = 0;
+  for (int i = 0; i < num_keys; ++i) {
+    t->key_types[i] = key_types[i];
+    if (key_bits) {               // packed: caller-given ranges, checked
+      t->key_bias[i] = key_min[i];
+      t->key_bits[i] = key_bits[i];
+      t->key_check = 1;
+    } else {                      // one column as is, or the low 32 bits of each of two
+      t->key_bias[i] = 0;
+      t->key_bits[i] = num_keys == 1 ? 64 : 32;
+    }
+    t->key_shift[i] = shift;
+    shift += t->key_bits[i];
+  }
   t->num_aggs = num_aggs;
   std::memset(&t->acc, 0, sizeof(t->acc));
   int nacc = num_aggs;
@@ -568,6 +639,76 @@ fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int
   return FPDB_OK;
 }
 
+}  // namespace fpdb
+
+extern "C" {
+
+fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int num_keys,
+                                const fpdb_agg_spec* aggs, int num_aggs, int64_t max_groups,
+                                fpdb_group_table** out) {
+  FPDB_CHECK(ctx && out, FPDB_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  FPDB_CHECK(num_keys >= 0 && num_keys <= 2, FPDB_ERR_INVALID,
+             "group-by: zero, one or two key columns (more through fpdb_groupby_create_packed)");
+  FPDB_CHECK(key_types || num_keys == 0, FPDB_ERR_INVALID, "group-by: NULL key types");
+  for (int i = 0; i < num_keys; ++i) {
+    int w = type_width(key_types[i]);
+    FPDB_CHECK(w > 0 && !type_is_float(key_types[i]), FPDB_ERR_INVALID, "group-by: keys must be integers");
+    FPDB_CHECK(num_keys == 1 || w <= 4, FPDB_ERR_INVALID,
+               "group-by: two-column keys need columns of at most 4 bytes");
+  }
+  return groupby_create_impl(ctx, key_types, num_keys, nullptr, nullptr, aggs, num_aggs, max_groups, out);
+}
+
+fpdb_status fpdb_groupby_create_packed(fpdb_context* ctx, const int32_t* key_types, int num_keys,
+                                       const int64_t* key_min, const int32_t* key_bits, const fpdb_agg_spec* aggs,
+                                       int num_aggs, int64_t max_groups, fpdb_group_table** out) {
+  FPDB_CHECK(ctx && out, FPDB_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  FPDB_CHECK(num_keys >= 1 && num_keys <= kMaxGroupKeys, FPDB_ERR_INVALID, "group-by: one to four packed key columns");
+  FPDB_CHECK(key_types && key_min && key_bits, FPDB_ERR_INVALID, "group-by: NULL key description");
+  int total = 0;
+  for (int i = 0; i < num_keys; ++i) {
+    int w = type_width(key_types[i]);
+    FPDB_CHECK(w > 0 && !type_is_float(key_types[i]), FPDB_ERR_INVALID, "group-by: keys must be integers");
+    FPDB_CHECK(key_bits[i] >= 1 && key_bits[i] <= 64, FPDB_ERR_INVALID, "group-by: key bits must be in [1, 64]");
+    total += key_bits[i];
+  }
+  FPDB_CHECK(total <= 64, FPDB_ERR_INVALID, "group-by: packed key wider than 64 bits");
+  return groupby_create_impl(ctx, key_types, num_keys, key_min, key_bits, aggs, num_aggs, max_groups, out);
+}
+
+fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, int64_t num_rows, int64_t* min,
+                                int64_t* max, int64_t* num_valid) {
+  FPDB_CHECK(ctx && column && min && max, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(!type_is_float(column->type) && type_width(column->type) > 0, FPDB_ERR_INVALID,
+             "min/max: integer column needed");
+  Context* c = as_ctx(ctx);
+  ColSet cs;
+  FPDB_TRY(make_colset(column, 1, num_rows, &cs, "min/max"));
+  unsigned long long* d = nullptr;
+  FPDB_TRY(ctx_alloc(c, 24, reinterpret_cast<void**>(&d)));
+  unsigned long long h[3] = {~0ULL, 0ULL, 0ULL};
+  hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, c->stream, d);
+  if (num_rows > 0)
+    hipLaunchKernelGGL(range_kernel, dim3(grid_for(c, num_rows, 256, 4)), dim3(256), 0, c->stream, cs, num_rows, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, 24, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  ctx_free(c, d);
+  FPDB_CHECK(e == hipSuccess, FPDB_ERR_HIP, hipGetErrorString(e));
+  const bool sgn = type_is_signed_int(column->type);
+  if (num_valid) *num_valid = (int64_t)h[2];
+  if (h[2] == 0) {
+    *min = 0;
+    *max = 0;
+  } else {
+    *min = (int64_t)(sgn ? (h[0] ^ (1ULL << 63)) : h[0]);
+    *max = (int64_t)(sgn ? (h[1] ^ (1ULL << 63)) : h[1]);
+  }
+  return FPDB_OK;
+}
+
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
   if (t->keys) ctx_free(t->ctx, t->keys);
@@ -586,6 +727,7 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* t, const f
   GKeySpec k;
   std::memset(&k, 0, sizeof(k));
   k.n = t->num_keys;
+  k.check = t->key_check;
   for (int i = 0; i < t->num_keys; ++i) {
     FPDB_CHECK(keys && keys[i].length >= n && (n == 0 || keys[i].values), FPDB_ERR_INVALID,
                "group-by: key column too short");
@@ -597,6 +739,9 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* t, const f
     k.off[i] = keys[i].offset;
     k.w[i] = type_width(keys[i].type);
     k.sgn[i] = type_is_signed_int(keys[i].type) ? 1 : 0;
+    k.bias[i] = t->key_bias[i];
+    k.bits[i] = t->key_bits[i];
+    k.shift[i] = t->key_shift[i];
   }
   ColSet vals;
   std::memset(&vals, 0, sizeof(vals));
@@ -664,6 +809,8 @@ fpdb_status fpdb_groupby_num_groups(fpdb_context* ctx, fpdb_group_table* t, int6
   int64_t sentinel_first = 0;
   FPDB_HIP(hipMemcpyAsync(&sentinel_first, t->first + t->capacity, 8, hipMemcpyDeviceToHost, c->stream));
   FPDB_HIP(hipStreamSynchronize(c->stream));
+  FPDB_CHECK(reinterpret_cast<int*>(c->h_counter + 1)[1] == 0, FPDB_ERR_INVALID,
+             "group-by: a key value lies outside the range the packed key was created for");
   FPDB_CHECK(reinterpret_cast<int*>(c->h_counter + 1)[0] == 0, FPDB_ERR_CAPACITY,
              "group-by: more groups than max_groups");
   *num_groups = c->h_counter[0] + (sentinel_first != LLONG_MAX ? 1 : 0);
@@ -688,6 +835,9 @@ fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* t, fpdb_c
     o.key_values[i] = static_cast<uint8_t*>(out_keys[i].values);
     o.key_w[i] = type_width(out_keys[i].type);
     o.key_off[i] = out_keys[i].offset;
+    o.key_bias[i] = t->key_bias[i];
+    o.key_bits[i] = t->key_bits[i];
+    o.key_shift[i] = t->key_shift[i];
   }
   for (int a = 0; a < t->num_aggs; ++a) {
     int32_t ot = fpdb_groupby_output_type(t->aggs[a]);

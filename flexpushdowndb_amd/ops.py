@@ -133,6 +133,25 @@ class LegacyBloomFilter:
         return bm, int(cnt.numpy()[0])
 
 
+_CMP_OPS = {"<": C.CMP_LT, "<=": C.CMP_LE, ">": C.CMP_GT, ">=": C.CMP_GE, "==": C.CMP_EQ, "!=": C.CMP_NE}
+
+
+def compare_bitmap(ctx, col, op, constant, n, combine=None, combine_or=False, out=None):
+    """Selection bitmap of `col op constant` (nulls unselected), AND-ed with the
+    bitmap `combine` (OR-ed when combine_or); `out` may be `combine` itself.
+    Returns (bitmap, selected count)."""
+    code = _CMP_OPS[op] if isinstance(op, str) else int(op)
+    if combine_or:
+        code |= C.CMP_OR
+    bm = out if out is not None else ctx.empty(max(8, (n + 63) // 64 * 8), np.uint8)
+    cnt = ctx.zeros(1, np.int64)
+    is_f = np.dtype(col.dtype).kind == "f"
+    C.check(C.lib().fpdb_compare_bitmap(ctx.handle, ctypes.byref(col.c()), code, 0 if is_f else int(constant),
+                                        float(constant) if is_f else 0.0, combine.ptr if combine is not None else None,
+                                        n, bm.ptr, cnt.ptr))
+    return bm, int(cnt.numpy()[0])
+
+
 def filter_compact(ctx, cols, n, bitmap, capacity=None):
     cap = n if capacity is None else capacity
     out = [DeviceColumn.empty(ctx, max(cap, n) if c.validity is not None else cap, c.dtype,
@@ -222,6 +241,24 @@ class HashJoinTable:
         return rows, nout.value
 
 
+def column_min_max(ctx, col, n):
+    """(min, max, valid count) of an integer device column."""
+    lo, hi, cnt = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    C.check(C.lib().fpdb_column_min_max(ctx.handle, ctypes.byref(col.c()), n, ctypes.byref(lo), ctypes.byref(hi),
+                                        ctypes.byref(cnt)))
+    return lo.value, hi.value, cnt.value
+
+
+def packing_for(ranges):
+    """(key_min, key_bits) per key from (min, max) ranges, or None when the
+    packed key would need more than 64 bits."""
+    out = []
+    for lo, hi in ranges:
+        span = int(hi) - int(lo)
+        out.append((int(lo), max(1, span.bit_length())))
+    return out if sum(b for _, b in out) <= 64 else None
+
+
 def gather(ctx, cols, rows, n):
     out = [DeviceColumn.empty(ctx, n, c.dtype, True, c.type_id) for c in cols]
     C.check(C.lib().fpdb_gather(ctx.handle, _cols(cols, 0), len(cols), rows.ptr, n,
@@ -232,16 +269,23 @@ def gather(ctx, cols, rows, n):
 class GroupBy:
     """Accumulating hash group-by (GroupArrowKernel's aggregate node)."""
 
-    def __init__(self, ctx, key_types, aggs, max_groups):
-        """aggs: list of (kind, input_type, nullable)."""
+    def __init__(self, ctx, key_types, aggs, max_groups, packed=None):
+        """aggs: list of (kind, input_type, nullable).  packed: (key_min, key_bits)
+        per key column, for up to four keys packed into one 64-bit key."""
         self.ctx = ctx
         self.key_types = list(key_types)
         self.specs = [C.AggSpec(k, t if t is not None else 0, 1 if nl else 0, 0) for k, t, nl in aggs]
-        kt = (ctypes.c_int32 * len(key_types))(*key_types)
+        kt = (ctypes.c_int32 * max(1, len(key_types)))(*key_types)
         sp = (C.AggSpec * max(1, len(aggs)))(*self.specs)
         self.handle = ctypes.c_void_p()
-        C.check(C.lib().fpdb_groupby_create(ctx.handle, kt, len(key_types), sp, len(aggs), max_groups,
-                                            ctypes.byref(self.handle)))
+        if packed is None:
+            C.check(C.lib().fpdb_groupby_create(ctx.handle, kt, len(key_types), sp, len(aggs), max_groups,
+                                                ctypes.byref(self.handle)))
+        else:
+            mins = (ctypes.c_int64 * len(key_types))(*[int(m) for m, _ in packed])
+            bits = (ctypes.c_int32 * len(key_types))(*[int(b) for _, b in packed])
+            C.check(C.lib().fpdb_groupby_create_packed(ctx.handle, kt, len(key_types), mins, bits, sp, len(aggs),
+                                                       max_groups, ctypes.byref(self.handle)))
 
     def close(self):
         if self.handle:

@@ -12,6 +12,11 @@ void GroupArrowKernel::clear() {
   if (table_) fpdb_groupby_destroy(table_);
   table_ = nullptr;
   inputs_.clear();
+  haveRange_ = false;
+  keyMin_.clear();
+  keyMax_.clear();
+  packMin_.clear();
+  packBits_.clear();
 }
 
 Expected<void> GroupArrowKernel::makeOutputSchema(const Schema& in) {
@@ -25,6 +30,9 @@ Expected<void> GroupArrowKernel::makeOutputSchema(const Schema& in) {
     out.fields.push_back(in.fields[i]);
     keyTypes_.push_back(in.fields[i].type);
   }
+  packed_ = keyTypes_.size() > 2;
+  if (keyTypes_.size() == 2)
+    for (auto t : keyTypes_) packed_ = packed_ || typeWidth((fpdb_type)t) > 4;
   for (auto& f : aggregateFunctions_) {
     fpdb_agg_spec s{};
     s.kind = f->kind();
@@ -51,10 +59,61 @@ Expected<void> GroupArrowKernel::recreate(int64_t maxGroups) {
   if (table_) fpdb_groupby_destroy(table_);
   table_ = nullptr;
   maxGroups_ = maxGroups;
-  if (fpdb_groupby_create(defaultContext(), keyTypes_.data(), (int)keyTypes_.size(), specs_.data(),
-                          (int)specs_.size(), maxGroups_, &table_) != FPDB_OK)
-    return make_unexpected(fpdb_last_error());
+  fpdb_status s = packed_ ? fpdb_groupby_create_packed(defaultContext(), keyTypes_.data(), (int)keyTypes_.size(),
+                                                       packMin_.data(), packBits_.data(), specs_.data(),
+                                                       (int)specs_.size(), maxGroups_, &table_)
+                          : fpdb_groupby_create(defaultContext(), keyTypes_.data(), (int)keyTypes_.size(),
+                                                specs_.data(), (int)specs_.size(), maxGroups_, &table_);
+  if (s != FPDB_OK) return make_unexpected(fpdb_last_error());
   return {};
+}
+
+static int bitLength(uint64_t v) {
+  int b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+// Widens the key value ranges by one batch; true when the current table's
+// packing no longer covers them (a new packing is then chosen: each key gets
+// the bits of its span, plus one spare bit each while the total stays <= 64,
+// so a slowly growing key does not force a rebuild per batch).
+Expected<bool> GroupArrowKernel::widenKeyRanges(const TupleSet& ts) {
+  size_t nk = groupColumnNames_.size();
+  for (size_t j = 0; j < nk; ++j) {
+    fpdb_column col = ts.column(ts.schema().getFieldIndex(groupColumnNames_[j])).c();
+    int64_t lo = 0, hi = 0;
+    if (fpdb_column_min_max(defaultContext(), &col, ts.numRows(), &lo, &hi, nullptr) != FPDB_OK)
+      return make_unexpected(fpdb_last_error());
+    if (!haveRange_) {
+      keyMin_.push_back(lo);
+      keyMax_.push_back(hi);
+    } else {
+      keyMin_[j] = std::min(keyMin_[j], lo);
+      keyMax_[j] = std::max(keyMax_[j], hi);
+    }
+  }
+  haveRange_ = true;
+  bool fits = table_ != nullptr && packMin_.size() == nk;
+  for (size_t j = 0; fits && j < nk; ++j) {
+    uint64_t span = (uint64_t)keyMax_[j] - (uint64_t)packMin_[j];
+    fits = keyMin_[j] >= packMin_[j] && (packBits_[j] >= 64 || span < (1ULL << packBits_[j]));
+  }
+  if (fits) return false;
+  std::vector<int32_t> bits(nk);
+  int total = 0;
+  for (size_t j = 0; j < nk; ++j) {
+    bits[j] = std::max(1, bitLength((uint64_t)keyMax_[j] - (uint64_t)keyMin_[j]));
+    total += bits[j];
+  }
+  if (total > 64) return make_unexpected("Group keys need more than 64 bits when packed");
+  for (size_t j = 0; j < nk && total < 64; ++j, ++total) bits[j] += 1;
+  packMin_ = keyMin_;
+  packBits_ = bits;
+  return true;
 }
 
 Expected<void> GroupArrowKernel::consume(const std::shared_ptr<TupleSet>& ts) {
@@ -79,13 +138,24 @@ Expected<void> GroupArrowKernel::group(const std::shared_ptr<TupleSet>& tupleSet
   auto r = makeOutputSchema(tupleSet->schema());
   if (!r) return r;
   if (tupleSet->numRows() == 0) return {};
-  if (!table_) {
-    r = recreate(std::max<int64_t>(maxGroups_, 1));
-    if (!r) return r;
+  bool rebuild = !table_;
+  if (packed_) {
+    auto w = widenKeyRanges(*tupleSet);
+    if (!w) return make_unexpected(w.error());
+    rebuild = rebuild || *w;
   }
   inputs_.push_back(tupleSet);
-  r = consume(tupleSet);
-  if (!r) return r;
+  if (rebuild) {
+    r = recreate(std::max<int64_t>(maxGroups_, 1));
+    if (!r) return r;
+    for (auto& in : inputs_) {
+      r = consume(in);
+      if (!r) return r;
+    }
+  } else {
+    r = consume(tupleSet);
+    if (!r) return r;
+  }
   int64_t groups = 0;
   fpdb_status s = fpdb_groupby_num_groups(defaultContext(), table_, &groups);
   while (s == FPDB_ERR_CAPACITY) {

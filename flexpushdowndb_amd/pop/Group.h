@@ -74,6 +74,7 @@ class GroupArrowKernel {
   Expected<void> makeOutputSchema(const Schema& in);
   Expected<void> consume(const std::shared_ptr<TupleSet>& ts);
   Expected<void> recreate(int64_t maxGroups);
+  Expected<bool> widenKeyRanges(const TupleSet& ts);
 
   std::vector<std::string> groupColumnNames_;
   std::vector<std::shared_ptr<AggregateFunction>> aggregateFunctions_;
@@ -83,6 +84,14 @@ class GroupArrowKernel {
   std::vector<std::shared_ptr<TupleSet>> inputs_;   // kept to rebuild a table that overflowed
   fpdb_group_table* table_ = nullptr;
   int64_t maxGroups_ = 1 << 16;
+  // Keys that do not fit the plain 64-bit key (more than two columns, or two
+  // with one wider than 4 bytes) are packed from their value ranges, which
+  // grow as batches arrive.
+  bool packed_ = false;
+  bool haveRange_ = false;
+  std::vector<int64_t> keyMin_, keyMax_;     // value range seen so far
+  std::vector<int64_t> packMin_;             // packing of the current table
+  std::vector<int32_t> packBits_;
 };
 
 class GroupPOp : public PhysicalOp {

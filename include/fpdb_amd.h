@@ -180,12 +180,15 @@ fpdb_status fpdb_bloom_legacy_find(fpdb_context* ctx, const fpdb_universal_hash*
 /* ---- filter compaction --------------------------------------------------- */
 
 /* Comparison of one column with a constant into a selection bitmap (nulls are
- * not selected), AND-ed with d_and_bitmap when given.  Integer columns compare
- * with int_constant, float columns with float_constant.  Stands in for the
- * simple gandiva filters (column op literal) that feed the join and bloom
- * operators (fpdb-executor/src/physical/filter/FilterPOp.cpp). */
+ * not selected), AND-ed with d_and_bitmap when given, or OR-ed with it when op
+ * carries FPDB_CMP_OR (conjunctions and disjunctions of such comparisons chain
+ * through the bitmap; d_and_bitmap may be d_bitmap itself).  Integer columns
+ * compare with int_constant, float columns with float_constant.  Stands in for
+ * the simple gandiva filters (column op literal, AND/OR of them) that feed the
+ * join and bloom operators (fpdb-executor/src/physical/filter/FilterPOp.cpp). */
 typedef enum {
-  FPDB_CMP_LT = 0, FPDB_CMP_LE = 1, FPDB_CMP_GT = 2, FPDB_CMP_GE = 3, FPDB_CMP_EQ = 4, FPDB_CMP_NE = 5
+  FPDB_CMP_LT = 0, FPDB_CMP_LE = 1, FPDB_CMP_GT = 2, FPDB_CMP_GE = 3, FPDB_CMP_EQ = 4, FPDB_CMP_NE = 5,
+  FPDB_CMP_OR = 0x10
 } fpdb_cmp_op;
 fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, int op, int64_t int_constant,
                                 double float_constant, const uint8_t* d_and_bitmap, int64_t num_rows,
@@ -326,13 +329,30 @@ typedef struct fpdb_agg_spec {
 typedef struct fpdb_group_table fpdb_group_table;
 
 /* Creates an accumulating grouped aggregation over key columns of the given
- * types (one of <= 8 bytes or two of <= 4 bytes, no nulls) for at most
- * max_groups groups.  Replaces GroupArrowKernel's acero "aggregate" node
+ * types (none: one group for all rows; one of <= 8 bytes; or two of <= 4
+ * bytes; no nulls) for at most max_groups groups.  Replaces GroupArrowKernel's
+ * acero "aggregate" node
  * (fpdb-executor/src/physical/group/GroupArrowKernel.cpp:16-68,127-154,245-291). */
 fpdb_status fpdb_groupby_create(fpdb_context* ctx, const int32_t* key_types, int num_keys,
                                 const fpdb_agg_spec* aggs, int num_aggs, int64_t max_groups,
                                 fpdb_group_table** out);
+
+/* Same, for one to four integer key columns packed into one 64-bit key: key j
+ * holds values in [key_min[j], key_min[j] + 2^key_bits[j]), the bits summing
+ * to at most 64.  A value outside its range makes fpdb_groupby_num_groups and
+ * fpdb_groupby_finalize fail with FPDB_ERR_INVALID.  The grouping of the
+ * reference's multi-column group keys (GroupArrowKernel.cpp:245-291, any
+ * number of keys) for key ranges that fit; fpdb_column_min_max gives them. */
+fpdb_status fpdb_groupby_create_packed(fpdb_context* ctx, const int32_t* key_types, int num_keys,
+                                       const int64_t* key_min, const int32_t* key_bits,
+                                       const fpdb_agg_spec* aggs, int num_aggs, int64_t max_groups,
+                                       fpdb_group_table** out);
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* table);
+
+/* Smallest and largest valid value of an integer column (0 and 0 when it has
+ * none) and the number of valid values (may be NULL); synchronizes. */
+fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, int64_t num_rows,
+                                int64_t* min, int64_t* max, int64_t* num_valid);
 
 /* Consumes one batch (GroupArrowKernel::group / doGroup, GroupArrowKernel.cpp:16-39,127-154).
  * values[j] is the input of aggs[j] (ignored for COUNT_ALL). */
