@@ -17,7 +17,10 @@
 namespace fpdb {
 
 constexpr int kCompactWaves = 4;
-constexpr int kCompactChunks = 8;
+#ifndef FPDB_COMPACT_CHUNKS
+#define FPDB_COMPACT_CHUNKS 32   // 64-row chunks per wave (tile = 4 waves x this x 64 rows)
+#endif
+constexpr int kCompactChunks = FPDB_COMPACT_CHUNKS;
 constexpr int64_t kCompactTileRows = kCompactWaves * kCompactChunks * 64;
 
 __device__ __forceinline__ bool bitmap_bit(const uint8_t* bm, int64_t i) {

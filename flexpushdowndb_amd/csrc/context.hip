@@ -185,9 +185,24 @@ fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes) {
 }
 
 __global__ void lb_reset_kernel(LbState* st) {
-  st->ticket = 0;
-  st->done = 0;
-  st->epoch = 1;
+  st->reserved = 0;
+  st->reserved2 = 0;
+  st->epoch = 0;
+}
+
+// Advances the epoch before a look-back launch (one workgroup); on wrap-around
+// the status words are cleared so no old word can carry the new epoch.
+__global__ void lb_advance_kernel(LbState* st, uint64_t* status, int64_t status_cap) {
+  __shared__ unsigned s_e;
+  if (threadIdx.x == 0) s_e = st->epoch + 1;
+  __syncthreads();
+  unsigned e = s_e;
+  if (e >= (1u << 20)) {
+    for (int64_t i = threadIdx.x; i < status_cap; i += blockDim.x) status[i] = 0;
+    e = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) st->epoch = e;
 }
 
 fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args) {
@@ -198,7 +213,7 @@ fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args) {
     FPDB_HIP(hipGetLastError());
   }
   if (tiles > lb.tiles) {
-    // grow: a fresh zeroed status array; the epoch restarts at 1
+    // grow: a fresh zeroed status array; the epoch restarts
     int64_t want = std::max<int64_t>(tiles, 4096);
     if (lb.status) FPDB_TRY(ctx_free(c, lb.status));
     lb.status = nullptr;
@@ -208,6 +223,8 @@ fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args) {
     FPDB_HIP(hipGetLastError());
     lb.tiles = want;
   }
+  hipLaunchKernelGGL(lb_advance_kernel, dim3(1), dim3(256), 0, c->stream, lb.state, lb.status, lb.tiles);
+  FPDB_HIP(hipGetLastError());
   args->state = lb.state;
   args->status = lb.status;
   args->status_cap = lb.tiles;

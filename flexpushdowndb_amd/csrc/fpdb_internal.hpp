@@ -41,13 +41,17 @@ fpdb_status fail(fpdb_status code, const std::string& msg);
 // ---- context ---------------------------------------------------------------
 
 // Decoupled look-back state, kept entirely on the device so a launch can be
-// replayed from a captured graph: tiles take tickets from `ticket`; the last
-// workgroup to finish (counted by `done`) resets both and advances `epoch`.
-// Status words carry the epoch, [epoch:20][flag:2][value:42], so they never
-// need clearing between launches (only when the 20-bit epoch wraps).
+// replayed from a captured graph.  Workgroup b handles tile b: each XCD
+// dispatches its workgroups in index order, so the lowest-numbered unfinished
+// tile always runs and every look-back wait ends (a bounded spin still turns a
+// stuck wait into an error).  A one-workgroup kernel before each launch
+// advances `epoch`; status words carry it, [epoch:20][flag:2][value:42], so
+// they never need clearing between launches (only when the 20-bit epoch
+// wraps).  No per-tile atomic on a shared word: one same-address device
+// atomic per workgroup serializes at about 12 ns each on gfx950.
 struct LbState {
-  unsigned long long ticket;
-  unsigned int done;
+  unsigned long long reserved;
+  unsigned int reserved2;
   unsigned int epoch;
 };
 
@@ -297,41 +301,19 @@ __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t
   return exclusive;
 }
 
-// Start of a look-back launch: thread 0 reads the epoch and takes a tile ticket.
+// Start of a look-back launch: the tile is the workgroup index; thread 0 reads
+// the epoch the advance kernel wrote before this launch.
 __device__ __forceinline__ void lb_begin(const LbArgs& lb, int64_t* s_tile, uint32_t* s_epoch) {
   if (threadIdx.x == 0) {
     *s_epoch = __hip_atomic_load(&lb.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *s_tile = (int64_t)__hip_atomic_fetch_add(&lb.state->ticket, 1ULL, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+    *s_tile = (int64_t)blockIdx.x;
   }
   __syncthreads();
 }
 
-// End of a look-back launch; every workgroup must reach it.  The last one to
-// finish resets the ticket and advances the epoch for the next launch (every
-// other workgroup read the epoch before it counted itself done).
-__device__ __forceinline__ void lb_end(const LbArgs& lb) {
-  __shared__ int s_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = __hip_atomic_fetch_add(&lb.state->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = prev == (unsigned)(lb.num_tiles - 1);
-  }
-  __syncthreads();
-  if (s_last) {
-    unsigned e = __hip_atomic_load(&lb.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
-    if (e >= (1u << 20)) {
-      for (int64_t i = threadIdx.x; i < lb.status_cap; i += blockDim.x) lb.status[i] = 0;
-      e = 1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __hip_atomic_store(&lb.state->ticket, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&lb.state->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&lb.state->epoch, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
+// End of a look-back launch (nothing to do: the next launch's advance kernel
+// moves the epoch on).
+__device__ __forceinline__ void lb_end(const LbArgs&) {}
 
 // Wave-wide inclusive prefix sum of a 32-bit value.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {

@@ -133,7 +133,10 @@ constexpr int kJtOuter = 1;      // pairs + unmatched probe rows
 constexpr int kJtSemi = 2;       // probe rows with a match
 constexpr int kJtMark = 3;       // no output, flag matched build rows
 
-constexpr int kProbeItems = 4;
+#ifndef FPDB_PROBE_ITEMS
+#define FPDB_PROBE_ITEMS 4   // probe rows per thread (tile = 256 x this)
+#endif
+constexpr int kProbeItems = FPDB_PROBE_ITEMS;
 constexpr int kProbeThreads = 256;
 constexpr int kProbeWaves = kProbeThreads / 64;
 constexpr int64_t kProbeTileRows = kProbeItems * kProbeThreads;
