@@ -106,7 +106,8 @@ def bench_join(ctx, steps, warmup):
     inv[w["build_key"]] = np.arange(n_build)
     ok = (cnt == n_probe and np.array_equal(outs[0].numpy(), w["probe_fk"])
           and np.array_equal(outs[4].numpy(), w["build_pay"][inv[w["probe_fk"]]])
-          and np.array_equal(outs[2].numpy(), w["probe_price"]))
+          and np.array_equal(outs[2].numpy(), w["probe_price"])
+          and np.array_equal(outs[3].numpy(), w["probe_fk"]))
     if not ok:
         raise SystemExit("join output does not match the closed form")
     ms_step = ms_total / steps
@@ -189,22 +190,29 @@ def bench_groupby(ctx, steps):
             DeviceColumn.from_numpy(ctx, w["disc"])]
     aggs = [(C.AGG_COUNT_ALL, None, False), (C.AGG_SUM, C.TYPE_INT64, False), (C.AGG_SUM, C.TYPE_INT64, False),
             (C.AGG_SUM, C.TYPE_FLOAT64, False)]
-    ms = []
     g = None
+    wall = []
     for i in range(steps + 1):
         if g is not None:
             g.close()
         g = ops.GroupBy(ctx, [C.TYPE_INT32], aggs, 200_000)
+        ctx.synchronize()
         if i == 1:
             L.fpdb_profile_reset(ctx.handle)
             L.fpdb_context_set_profiling(ctx.handle, 1)
+        t0 = time.perf_counter()
         g.consume([kc], vals, n)
         ctx.synchronize()
-    cons_ms, _ = kernel_ms(ctx, "groupby_consume")
+        if i >= 1:
+            wall.append((time.perf_counter() - t0) * 1e3)
+    # consume = local pass + scan, scatter pass, compact pass (one host sync between)
+    local_ms, _ = kernel_ms(ctx, "groupby_local")
+    scatter_ms, _ = kernel_ms(ctx, "groupby_scatter")
+    compact_ms, _ = kernel_ms(ctx, "groupby_compact")
+    cons_ms = local_ms + scatter_ms + compact_ms
     t0 = time.perf_counter()
     keys, out = g.finalize()
     finalize_ms = (time.perf_counter() - t0) * 1e3
-    merge_ms, _ = kernel_ms(ctx, "groupby_merge")
     L.fpdb_context_set_profiling(ctx.handle, 0)
     k = keys[0].numpy()
     c = out[0].numpy()
@@ -216,10 +224,11 @@ def bench_groupby(ctx, steps):
     alg = n * (4 + 8 + 8 + 8)
     return {"workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
                         "COUNT, SUM int64 x2, SUM float64", "groups": int(len(k)),
-            "consume_kernel_ms": cons_ms, "grows_per_s": n / (cons_ms * 1e-3) / 1e9,
-            "replica_merge_kernel_ms": merge_ms, "finalize_ms": finalize_ms,
+            "consume_kernels_ms": cons_ms, "local_ms": local_ms, "scatter_ms": scatter_ms,
+            "compact_ms": compact_ms, "consume_wall_ms": statistics.median(wall),
+            "grows_per_s": n / (cons_ms * 1e-3) / 1e9, "finalize_ms": finalize_ms,
             "hbm_frac": alg / (cons_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic_bytes": load_traffic("groupby_consume")}
+            "traffic_bytes": load_traffic("gb_local_kernel")}
 
 
 def bench_star(ctx, steps, device_name):
@@ -328,7 +337,7 @@ def main():
         log(f"bloom: find {res['bloom']['find_kernel_ms']:.4f} ms")
     if args.only in (None, "groupby"):
         res["groupby"] = bench_groupby(ctx, max(2, args.steps // 5))
-        log(f"groupby: consume {res['groupby']['consume_kernel_ms']:.4f} ms")
+        log(f"groupby: consume {res['groupby']['consume_kernels_ms']:.4f} ms (local {res['groupby']['local_ms']:.3f}, scatter {res['groupby']['scatter_ms']:.3f}, compact {res['groupby']['compact_ms']:.3f}; wall {res['groupby']['consume_wall_ms']:.3f})")
     if args.only in (None, "star"):
         res["star"] = bench_star(ctx, max(3, args.steps // 2), f"gfx950-{info['num_cus']}cu")
         log(f"star: batched {res['star']['batched_ms']:.3f} ms, plain {res['star']['plain_ms']:.3f} ms")

@@ -1,15 +1,30 @@
-// Hash group-by aggregation on gfx950.
+// Hash group-by aggregation on gfx950, with no global atomics on the row path.
 //
-// Consume: every workgroup owns a contiguous range of rows and a hash table in
-// LDS (keys, one 8-byte accumulator per aggregate, first row).  Rows whose key
-// finds or claims an LDS slot within a few probes are aggregated with LDS
-// atomics, which absorbs the heavy head of skewed (Zipf) keys; the remaining
-// rows go straight to the global open-addressing table in HBM.  At the end the
-// workgroup merges its LDS table into the global table.
+// Every global 64-bit atomic on this part is executed memory-side at a fixed
+// rate of about 24 G/s for the whole chip, whatever the table size or scope,
+// so a row-at-a-time global hash table cannot stream 60M rows at HBM speed.
+// Instead:
 //
-// Integer aggregates are exact (wrapping int64 arithmetic is associative), so
-// they are independent of the order in which rows arrive.  FP64 sums are
-// accumulated with hardware FP64 atomics.
+//   local    (one workgroup per contiguous row range, 1024 threads): a hash
+//            table in LDS aggregates every row whose key finds or claims a
+//            slot within a few probes (this absorbs the heavy head of skewed
+//            keys).  The rest ("tail" rows) are only counted per bucket, a
+//            bucket being the top bits of the key hash.  At the end the table
+//            is written out slot by slot as partial records.
+//   scan     bucket-major exclusive scan of the per-workgroup counts.
+//   scatter  (same workgroup / row range): the LDS table is rebuilt at the same
+//            slots from the partial records, so a row is a tail row exactly
+//            when the local pass found no slot for it; tail rows and partial
+//            records are written as records to their bucket's range.
+//   compact  (one workgroup per bucket): the bucket's records, old and new,
+//            are aggregated in an LDS table and written back as one record per
+//            group.
+//
+// A record is (key, first row, accumulators...), all 8-byte words; raw rows
+// and partial aggregates have the same form, so every pass combines them with
+// the same operations.  Integer aggregates are exact (wrapping int64
+// arithmetic is associative); FP64 sums depend on the order in which records
+// are combined.
 //
 // Finalize: groups are emitted in order of first appearance (the order in
 // which a serial grouper numbers them), ranked by a bitmap over the first-row
@@ -27,13 +42,14 @@ constexpr uint64_t kEmptyKey = ~0ULL;
 constexpr int kMaxAggs = 16;          // visible + hidden accumulators
 constexpr int kLdsProbes = 8;
 constexpr int kGroupThreads = 1024;
+constexpr int kMinBuckets = 256;
+constexpr int kMaxBuckets = 1 << 14;
+constexpr int64_t kRowsPerWorkgroup = 16384;   // fewer rows per workgroup: fewer workgroups
 
 __device__ __forceinline__ bool type_is_signed_int_d(int32_t t) {
   return t == FPDB_TYPE_INT8 || t == FPDB_TYPE_INT16 || t == FPDB_TYPE_INT32 || t == FPDB_TYPE_INT64 ||
          t == FPDB_TYPE_DATE32 || t == FPDB_TYPE_DATE64;
 }
-
-__device__ __forceinline__ int64_t ceil_div_d(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // accumulator operations
 enum AccOp : int32_t {
@@ -138,8 +154,7 @@ __device__ __forceinline__ void atomic_max_f64(double* p, double v) {
   }
 }
 
-// Applies one operand; works on LDS and global addresses alike (the compiler
-// picks ds_* or global_* atomics from the address space).
+// Applies one operand to an LDS accumulator.
 __device__ __forceinline__ void acc_apply(uint64_t* slot, int32_t op, uint64_t v) {
   switch (op) {
     case kAccCountAll:
@@ -159,183 +174,349 @@ __device__ __forceinline__ void acc_apply(uint64_t* slot, int32_t op, uint64_t v
   }
 }
 
-// The global table has one replica per XCD: a workgroup aggregates into the
-// replica of the XCD it runs on, so the cache lines its atomics touch stay in
-// that XCD's L2 instead of bouncing between L2s.  Replicas 1.. are merged into
-// replica 0 before groups are counted or emitted.  Any workgroup may use any
-// replica (all atomics are device-coherent); the XCD choice is only locality.
-constexpr int kReplicas = 8;
+// ---- LDS hash table --------------------------------------------------------
+//
+// nslots ordinary slots (any count; the home slot is a multiply-shift of the
+// low hash bits) plus one reserved slot, index nslots, for the key equal to
+// the empty marker.  Layout: keys[stride], first[stride], acc[nacc][stride].
 
-struct GlobalTable {
-  uint64_t* keys;        // nrep x cap
-  uint64_t* acc;         // nrep x nacc x (cap + 1); slot cap holds the key equal to kEmptyKey
-  int64_t* first;        // nrep x (cap + 1)
-  uint64_t mask;         // cap - 1
-  int64_t stride;        // cap + 1
-  int32_t nacc;
-  int32_t nrep;
-  unsigned long long* groups;  // number of claimed slots of replica 0
-  int* overflow;
-  int* key_range;              // set when a packed key value lies outside its range
+struct LdsTable {
+  uint64_t* keys;
+  long long* first;
+  uint64_t* acc;
+  uint32_t nslots;
+  uint32_t stride;
 };
 
-__device__ __forceinline__ uint64_t* gkey(const GlobalTable& g, int rep, int64_t s) {
-  return g.keys + (size_t)rep * (g.mask + 1) + s;
-}
-__device__ __forceinline__ uint64_t* gacc(const GlobalTable& g, int rep, int a, int64_t s) {
-  return g.acc + ((size_t)rep * g.nacc + a) * g.stride + s;
-}
-__device__ __forceinline__ int64_t* gfirst(const GlobalTable& g, int rep, int64_t s) {
-  return g.first + (size_t)rep * g.stride + s;
-}
-
-__device__ __forceinline__ int xcc_id() {
-  unsigned v;
-  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
-  return (int)(v & 0xF);
+__device__ __forceinline__ LdsTable lds_table(uint64_t* base, uint32_t nslots) {
+  LdsTable T;
+  T.nslots = nslots;
+  T.stride = nslots + 1;
+  T.keys = base;
+  T.first = reinterpret_cast<long long*>(base + T.stride);
+  T.acc = base + 2 * (size_t)T.stride;
+  return T;
 }
 
-// Finds or claims the slot of key k in a replica (cap for the reserved key value).
-__device__ __forceinline__ int64_t global_slot(const GlobalTable& g, int rep, uint64_t k) {
-  if (k == kEmptyKey) return (int64_t)g.mask + 1;
-  uint64_t s = fmix64(k) & g.mask;
-  for (uint64_t p = 0; p <= g.mask; ++p) {
-    uint64_t* kp = gkey(g, rep, s);
-    uint64_t cur = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == k) return (int64_t)s;
-    if (cur == kEmptyKey) {
-      unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(kp), kEmptyKey, k);
-      if (prev == kEmptyKey) {
-        if (rep == 0) atomicAdd(g.groups, 1ULL);
-        return (int64_t)s;
-      }
-      if (prev == k) return (int64_t)s;
-    }
-    s = (s + 1) & g.mask;
+__device__ __forceinline__ size_t lds_table_words(uint32_t nslots, int nacc) {
+  return (size_t)(nslots + 1) * (2 + nacc);
+}
+
+__device__ __forceinline__ uint32_t slot_home(uint64_t h, uint32_t nslots) {
+  return (uint32_t)(((h & 0xFFFFFFFFULL) * (uint64_t)nslots) >> 32);
+}
+
+__device__ __forceinline__ uint32_t bucket_of(uint64_t h, int bucket_shift) {
+  return bucket_shift >= 64 ? 0u : (uint32_t)(h >> bucket_shift);
+}
+
+__device__ __forceinline__ void lds_table_init(const LdsTable& T, const AccSpec& A, int nacc) {
+  for (uint32_t s = threadIdx.x; s < T.stride; s += blockDim.x) {
+    T.keys[s] = kEmptyKey;
+    T.first[s] = LLONG_MAX;
+    for (int a = 0; a < nacc; ++a) T.acc[(size_t)a * T.stride + s] = acc_identity(A.op[a]);
   }
-  *g.overflow = 1;
+}
+
+// Slot of key k, claiming an empty one within max_probes of home; -1 if none.
+__device__ __forceinline__ int lds_find_or_insert(const LdsTable& T, uint64_t k, uint64_t h, uint32_t max_probes) {
+  if (k == kEmptyKey) return (int)T.nslots;
+  uint32_t s = slot_home(h, T.nslots);
+  for (uint32_t p = 0; p < max_probes; ++p) {
+    uint64_t cur = T.keys[s];
+    if (cur == k) return (int)s;
+    if (cur == kEmptyKey) {
+      unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&T.keys[s]), kEmptyKey, k);
+      if (prev == kEmptyKey || prev == k) return (int)s;
+    }
+    s = (s + 1 == T.nslots) ? 0 : s + 1;
+  }
   return -1;
 }
 
+// Lookup only (the table is no longer changing).
+__device__ __forceinline__ int lds_find(const LdsTable& T, uint64_t k, uint64_t h, uint32_t max_probes) {
+  if (k == kEmptyKey) return (int)T.nslots;
+  uint32_t s = slot_home(h, T.nslots);
+  for (uint32_t p = 0; p < max_probes; ++p) {
+    uint64_t cur = T.keys[s];
+    if (cur == k) return (int)s;
+    if (cur == kEmptyKey) return -1;
+    s = (s + 1 == T.nslots) ? 0 : s + 1;
+  }
+  return -1;
+}
+
+// ---- kernels ---------------------------------------------------------------
+
+struct GbGeom {
+  int64_t n;            // rows in this batch
+  int64_t row_base;     // global row index of the batch's first row
+  int64_t per_block;    // rows per workgroup
+  uint32_t nslots;      // LDS slots of the local / scatter passes
+  int32_t nbuckets;
+  int32_t bucket_shift; // 64 - log2(nbuckets)
+  int32_t rec_words;    // 2 + nacc
+};
+
+// Local pass: LDS aggregation of the workgroup's rows; tail rows counted per
+// bucket; the table written out slot by slot (partial[b][slot]).
 template <int NACC>
-__global__ __launch_bounds__(kGroupThreads) void groupby_consume_kernel(GKeySpec keys, ColSet vals, AccSpec A,
-                                                                        int64_t n, int64_t row_base,
-                                                                        GlobalTable g, int lds_slots) {
+__global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, ColSet vals, AccSpec A, GbGeom G,
+                                                                 uint32_t* __restrict__ hist,
+                                                                 uint64_t* __restrict__ partial, int* key_range) {
   extern __shared__ uint64_t lds[];
   const int nacc = NACC > 0 ? NACC : A.n;
-  uint64_t* lkeys = lds;
-  uint64_t* lacc = lds + lds_slots;                       // nacc x lds_slots
-  int64_t* lfirst = reinterpret_cast<int64_t*>(lds + (size_t)lds_slots * (1 + nacc));
-  const uint32_t lmask = (uint32_t)lds_slots - 1;
-  for (int s = threadIdx.x; s < lds_slots; s += blockDim.x) {
-    lkeys[s] = kEmptyKey;
-    lfirst[s] = LLONG_MAX;
-    for (int a = 0; a < nacc; ++a) lacc[(size_t)a * lds_slots + s] = acc_identity(A.op[a]);
+  LdsTable T = lds_table(lds, G.nslots);
+  uint32_t* lhist = reinterpret_cast<uint32_t*>(lds + lds_table_words(G.nslots, nacc));
+  lds_table_init(T, A, nacc);
+  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x) lhist[p] = 0;
+  __syncthreads();
+
+  const int64_t begin = (int64_t)blockIdx.x * G.per_block;
+  const int64_t end = begin + G.per_block < G.n ? begin + G.per_block : G.n;
+  bool out_of_range = false;
+  // ITEMS rows per thread per step, their loads issued before any LDS work
+  constexpr int ITEMS = NACC > 0 ? 4 : 1;
+  constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
+  for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)ITEMS * blockDim.x) {
+    uint64_t kk[ITEMS];
+    uint64_t vv[ITEMS][NA];
+    bool okk[ITEMS][NA];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      int64_t r = r0 + (int64_t)i * blockDim.x;
+      if (r >= end) continue;
+      kk[i] = group_key(keys, r, &out_of_range);
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+        if (a < nacc) vv[i][a] = acc_operand(vals, A.op[a], A.input[a], r, &okk[i][a]);
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      int64_t r = r0 + (int64_t)i * blockDim.x;
+      if (r >= end) continue;
+      uint64_t h = fmix64(kk[i]);
+      int s = lds_find_or_insert(T, kk[i], h, kLdsProbes);
+      if (s < 0) {
+        atomicAdd(&lhist[bucket_of(h, G.bucket_shift)], 1u);
+        continue;
+      }
+#pragma unroll
+      for (int a = 0; a < NA; ++a)
+        if (a < nacc && okk[i][a]) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], vv[i][a]);
+      long long grow = (long long)(G.row_base + r);
+      if (grow < T.first[s]) atomicMin(&T.first[s], grow);
+    }
+  }
+  if (out_of_range) *key_range = 1;
+  __syncthreads();
+  // write the table out slot by slot; used slots count towards their bucket
+  uint64_t* out = partial + (size_t)blockIdx.x * T.stride * G.rec_words;
+  for (uint32_t s = threadIdx.x; s < T.stride; s += blockDim.x) {
+    long long f = T.first[s];
+    uint64_t k = s == T.nslots ? kEmptyKey : T.keys[s];
+    uint64_t* rec = out + (size_t)s * G.rec_words;
+    rec[0] = k;
+    rec[1] = (uint64_t)f;
+    for (int a = 0; a < nacc; ++a) rec[2 + a] = T.acc[(size_t)a * T.stride + s];
+    if (f != LLONG_MAX) atomicAdd(&lhist[bucket_of(fmix64(k), G.bucket_shift)], 1u);
   }
   __syncthreads();
-  const int rep = xcc_id() % g.nrep;
+  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x)
+    hist[(size_t)blockIdx.x * G.nbuckets + p] = lhist[p];
+}
 
-  const int64_t per_block = ceil_div_d(n, gridDim.x);
-  const int64_t begin = (int64_t)blockIdx.x * per_block;
-  const int64_t end = begin + per_block < n ? begin + per_block : n;
-  bool out_of_range = false;
-  for (int64_t r = begin + threadIdx.x; r < end; r += blockDim.x) {
-    uint64_t k = group_key(keys, r, &out_of_range);
-    uint64_t v[kMaxAggs];
-    bool ok[kMaxAggs];
+// Bucket-major exclusive scan: base[p][b] = where workgroup b writes its
+// bucket-p records in the new pool, after the bucket's old records.  One
+// workgroup; also resets the group counter the compact pass adds to.
+__global__ __launch_bounds__(1024) void gb_scan_kernel(const uint32_t* __restrict__ hist, int blocks, int nbuckets,
+                                                       const int64_t* __restrict__ ocount,
+                                                       uint64_t* __restrict__ base, int64_t* __restrict__ nstart,
+                                                       int64_t* __restrict__ newcount, unsigned long long* totals) {
+  __shared__ int64_t s_w[16];
+  __shared__ int64_t s_carry;
+  // per-bucket totals
+  for (int p = threadIdx.x; p < nbuckets; p += blockDim.x) {
+    int64_t t = 0;
+    for (int b = 0; b < blocks; ++b) t += hist[(size_t)b * nbuckets + p];
+    newcount[p] = t;
+  }
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  // exclusive scan over buckets of (old + new) sizes
+  for (int p0 = 0; p0 < nbuckets; p0 += 1024) {
+    int p = p0 + threadIdx.x;
+    int64_t v = p < nbuckets ? ocount[p] + newcount[p] : 0;
+    int64_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      int64_t t = __shfl_up(incl, off);
+      if ((threadIdx.x & 63) >= off) incl += t;
+    }
+    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int64_t wprefix = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wprefix += s_w[w];
+    int64_t carry = s_carry;
+    if (p < nbuckets) nstart[p] = carry + wprefix + incl - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) s_carry = carry + wprefix + incl;
+    __syncthreads();
+  }
+  for (int p = threadIdx.x; p < nbuckets; p += blockDim.x) {
+    uint64_t at = (uint64_t)(nstart[p] + ocount[p]);
+    for (int b = 0; b < blocks; ++b) {
+      base[(size_t)p * blocks + b] = at;
+      at += hist[(size_t)b * nbuckets + p];
+    }
+  }
+  if (threadIdx.x == 0) {
+    totals[0] = (unsigned long long)s_carry;   // records in the new pool
+    totals[1] = 0;                             // groups (the compact pass adds them up)
+  }
+}
+
+// Scatter pass: the workgroup's LDS table is rebuilt at its old slots (keys
+// only), so lookups within the same probe window classify every row exactly as
+// the local pass did; tail rows and the partial records go to their buckets.
+template <int NACC>
+__global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys, ColSet vals, AccSpec A, GbGeom G,
+                                                                   const uint64_t* __restrict__ partial,
+                                                                   const uint64_t* __restrict__ base,
+                                                                   uint64_t* __restrict__ pool) {
+  extern __shared__ uint64_t lds[];
+  const int nacc = NACC > 0 ? NACC : A.n;
+  const uint32_t stride = G.nslots + 1;
+  uint64_t* skeys = lds;
+  uint32_t* lcnt = reinterpret_cast<uint32_t*>(lds + stride);
+  const uint64_t* part = partial + (size_t)blockIdx.x * stride * G.rec_words;
+  for (uint32_t s = threadIdx.x; s < stride; s += blockDim.x) skeys[s] = part[(size_t)s * G.rec_words];
+  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x) lcnt[p] = 0;
+  __syncthreads();
+  LdsTable T;
+  T.keys = skeys;
+  T.first = nullptr;
+  T.acc = nullptr;
+  T.nslots = G.nslots;
+  T.stride = stride;
+  const int64_t begin = (int64_t)blockIdx.x * G.per_block;
+  const int64_t end = begin + G.per_block < G.n ? begin + G.per_block : G.n;
+  bool dummy = false;
+  constexpr int ITEMS = NACC > 0 ? 8 : 1;
+  for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)ITEMS * blockDim.x) {
+    uint64_t kk[ITEMS];
 #pragma unroll
-    for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-      if (a < nacc) v[a] = acc_operand(vals, A.op[a], A.input[a], r, &ok[a]);
+    for (int i = 0; i < ITEMS; ++i) {
+      int64_t r = r0 + (int64_t)i * blockDim.x;
+      kk[i] = r < end ? group_key(keys, r, &dummy) : 0;
     }
-    const int64_t grow = row_base + r;
-    int lslot = -1;
-    if (k != kEmptyKey) {
-      uint32_t s = (uint32_t)fmix64(k) & lmask;
-      for (int p = 0; p < kLdsProbes; ++p) {
-        uint64_t cur = lkeys[s];
-        if (cur == k) { lslot = (int)s; break; }
-        if (cur == kEmptyKey) {
-          unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&lkeys[s]), kEmptyKey, k);
-          if (prev == kEmptyKey || prev == k) { lslot = (int)s; break; }
-        }
-        s = (s + 1) & lmask;
-      }
-    }
-    if (lslot >= 0) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      int64_t r = r0 + (int64_t)i * blockDim.x;
+      if (r >= end) continue;
+      uint64_t k = kk[i];
+      uint64_t h = fmix64(k);
+      if (lds_find(T, k, h, kLdsProbes) >= 0) continue;
+      uint32_t p = bucket_of(h, G.bucket_shift);
+      uint64_t at = base[(size_t)p * gridDim.x + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
+      uint64_t* rec = pool + at * G.rec_words;
+      rec[0] = k;
+      rec[1] = (uint64_t)(G.row_base + r);
 #pragma unroll
       for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-        if (a < nacc && ok[a]) acc_apply(&lacc[(size_t)a * lds_slots + lslot], A.op[a], v[a]);
-      }
-      if (grow < lfirst[lslot]) atomicMin(reinterpret_cast<long long*>(&lfirst[lslot]), (long long)grow);
-    } else {
-      int64_t gs = global_slot(g, rep, k);
-      if (gs >= 0) {
-#pragma unroll
-        for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-          if (a < nacc && ok[a]) acc_apply(gacc(g, rep, a, gs), A.op[a], v[a]);
+        if (a < nacc) {
+          bool ok;
+          uint64_t v = acc_operand(vals, A.op[a], A.input[a], r, &ok);
+          rec[2 + a] = ok ? v : acc_identity(A.op[a]);
         }
-        atomicMin(reinterpret_cast<long long*>(gfirst(g, rep, gs)), (long long)grow);
       }
     }
   }
-  if (out_of_range) *g.key_range = 1;
+  for (uint32_t s = threadIdx.x; s < stride; s += blockDim.x) {
+    const uint64_t* src = part + (size_t)s * G.rec_words;
+    if ((long long)src[1] == LLONG_MAX) continue;
+    uint32_t p = bucket_of(fmix64(src[0]), G.bucket_shift);
+    uint64_t at = base[(size_t)p * gridDim.x + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
+    uint64_t* rec = pool + at * G.rec_words;
+    for (int w = 0; w < G.rec_words; ++w) rec[w] = src[w];
+  }
+}
+
+// Compact pass, one workgroup per bucket: old records (previous batches, one
+// per group) and the new ones are aggregated in LDS and written back, one per
+// group, at the start of the bucket's range in the new pool.
+template <int NACC>
+__global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, uint32_t nslots, int rec_words,
+                                                                   const uint64_t* __restrict__ oldpool,
+                                                                   int64_t* __restrict__ ostart,
+                                                                   int64_t* __restrict__ ocount,
+                                                                   uint64_t* pool, const int64_t* __restrict__ nstart,
+                                                                   const int64_t* __restrict__ newcount,
+                                                                   unsigned long long* groups, int* overflow) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t s_n;
+  __shared__ int s_over;
+  const int nacc = NACC > 0 ? NACC : A.n;
+  const int p = blockIdx.x;
+  LdsTable T = lds_table(lds, nslots);
+  lds_table_init(T, A, nacc);
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_over = 0;
+  }
   __syncthreads();
-  // merge the LDS table into the global table
-  for (int s = threadIdx.x; s < lds_slots; s += blockDim.x) {
-    uint64_t k = lkeys[s];
-    if (k == kEmptyKey) continue;
-    int64_t gs = global_slot(g, rep, k);
-    if (gs < 0) continue;
-    for (int a = 0; a < nacc; ++a) {
-      uint64_t v = lacc[(size_t)a * lds_slots + s];
-      if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(gacc(g, rep, a, gs), A.op[a], v);
+  const int64_t nold = ocount[p], nnew = newcount[p];
+  const int64_t start = nstart[p];
+  const uint64_t* news = pool + (size_t)(start + nold) * rec_words;
+  for (int64_t i = threadIdx.x; i < nold + nnew; i += blockDim.x) {
+    const uint64_t* rec = i < nold ? oldpool + (size_t)(ostart[p] + i) * rec_words
+                                   : news + (size_t)(i - nold) * rec_words;
+    uint64_t k = rec[0];
+    int s = lds_find_or_insert(T, k, fmix64(k), nslots);
+    if (s < 0) {
+      s_over = 1;
+      continue;
     }
-    atomicMin(reinterpret_cast<long long*>(gfirst(g, rep, gs)), (long long)lfirst[s]);
-  }
-}
-
-__global__ void groupby_init_kernel(GlobalTable g, AccSpec A) {
-  int64_t total = g.stride * g.nrep;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int rep = (int)(i / g.stride);
-    int64_t s = i % g.stride;
-    if (s < g.stride - 1) *gkey(g, rep, s) = kEmptyKey;
-    *gfirst(g, rep, s) = LLONG_MAX;
-    for (int a = 0; a < A.n; ++a) *gacc(g, rep, a, s) = acc_identity(A.op[a]);
-  }
-}
-
-// Folds replicas 1.. into replica 0 and resets them to empty.
-__global__ void groupby_merge_kernel(GlobalTable g, AccSpec A) {
-  int64_t total = g.stride * (g.nrep - 1);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int rep = 1 + (int)(i / g.stride);
-    int64_t s = i % g.stride;
-    int64_t f = *gfirst(g, rep, s);
-    if (f == LLONG_MAX) continue;    // slot never used
-    uint64_t k = (s == g.stride - 1) ? kEmptyKey : *gkey(g, rep, s);
-    int64_t d = global_slot(g, 0, k);
-    if (d >= 0) {
-      for (int a = 0; a < A.n; ++a) {
-        uint64_t v = *gacc(g, rep, a, s);
-        if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(gacc(g, 0, a, d), A.op[a], v);
-        *gacc(g, rep, a, s) = acc_identity(A.op[a]);
+#pragma unroll
+    for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
+      if (a < nacc) {
+        uint64_t v = rec[2 + a];
+        if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
       }
-      atomicMin(reinterpret_cast<long long*>(gfirst(g, 0, d)), (long long)f);
     }
-    if (s < g.stride - 1) *gkey(g, rep, s) = kEmptyKey;
-    *gfirst(g, rep, s) = LLONG_MAX;
+    long long f = (long long)rec[1];
+    if (f < T.first[s]) atomicMin(&T.first[s], f);
+  }
+  __syncthreads();   // every record of the bucket is read before any is overwritten
+  uint64_t* out = pool + (size_t)start * rec_words;
+  for (uint32_t s = threadIdx.x; s < T.stride; s += blockDim.x) {
+    long long f = T.first[s];
+    if (f == LLONG_MAX) continue;
+    uint32_t idx = atomicAdd(&s_n, 1u);
+    uint64_t* rec = out + (size_t)idx * rec_words;
+    rec[0] = s == T.nslots ? kEmptyKey : T.keys[s];
+    rec[1] = (uint64_t)f;
+    for (int a = 0; a < nacc; ++a) rec[2 + a] = T.acc[(size_t)a * T.stride + s];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ostart[p] = start;
+    ocount[p] = s_n;
+    if (s_n) atomicAdd(groups, (unsigned long long)s_n);
+    if (s_over) *overflow = 1;
   }
 }
 
-// Marks the first row of every group in a bitmap over consumed rows.
-__global__ void groupby_mark_kernel(GlobalTable g, unsigned long long* __restrict__ bits) {
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < g.stride;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    int64_t f = g.first[s];
-    if (f == LLONG_MAX) continue;
+// ---- finalize ----------------------------------------------------------------
+
+// Marks the first row of every group.
+__global__ void gb_mark_kernel(const uint64_t* __restrict__ pool, const int64_t* __restrict__ ostart,
+                               const int64_t* __restrict__ ocount, int rec_words, unsigned long long* bits) {
+  const int p = blockIdx.x;
+  const int64_t n = ocount[p];
+  const uint64_t* recs = pool + (size_t)ostart[p] * rec_words;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    int64_t f = (int64_t)recs[(size_t)i * rec_words + 1];
     atomicOr(&bits[f >> 6], 1ULL << (f & 63));
   }
 }
@@ -410,27 +591,32 @@ struct GOutSpec {
   int32_t num_keys, num_aggs;
 };
 
-__global__ void groupby_emit_kernel(GlobalTable g, AccSpec A, const unsigned long long* __restrict__ bits,
-                                    const uint32_t* __restrict__ word_prefix, GOutSpec o) {
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < g.stride;
-       s += (int64_t)gridDim.x * blockDim.x) {
-    int64_t f = g.first[s];
-    if (f == LLONG_MAX) continue;
+// One workgroup per bucket: every group record to its first-appearance rank.
+__global__ void gb_emit_kernel(const uint64_t* __restrict__ pool, const int64_t* __restrict__ ostart,
+                               const int64_t* __restrict__ ocount, int rec_words,
+                               const unsigned long long* __restrict__ bits,
+                               const uint32_t* __restrict__ word_prefix, GOutSpec o) {
+  const int p = blockIdx.x;
+  const int64_t n = ocount[p];
+  const uint64_t* recs = pool + (size_t)ostart[p] * rec_words;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const uint64_t* rec = recs + (size_t)i * rec_words;
+    int64_t f = (int64_t)rec[1];
     uint64_t w = bits[f >> 6];
     uint64_t below = (f & 63) ? (w & (~0ULL >> (64 - (f & 63)))) : 0ULL;
     int64_t rank = (int64_t)word_prefix[f >> 6] + __popcll(below);
-    uint64_t key = (s == g.stride - 1) ? kEmptyKey : g.keys[s];
+    uint64_t key = rec[0];
     for (int j = 0; j < o.num_keys; ++j) {
       uint64_t v = ((key >> o.key_shift[j]) & key_field_mask(o.key_bits[j])) + (uint64_t)o.key_bias[j];
       store_w(o.key_values[j], o.key_w[j], rank + o.key_off[j], v);
     }
     for (int a = 0; a < o.num_aggs; ++a) {
-      uint64_t v = g.acc[(size_t)a * g.stride + s];
+      uint64_t v = rec[2 + a];
       if (o.agg_is_f32[a]) {
         float fv = (float)__longlong_as_double((long long)v);
         v = (uint64_t)__float_as_uint(fv);
       }
-      uint64_t cnt = o.agg_count_acc[a] >= 0 ? g.acc[(size_t)o.agg_count_acc[a] * g.stride + s] : 1;
+      uint64_t cnt = o.agg_count_acc[a] >= 0 ? rec[2 + o.agg_count_acc[a]] : 1;
       if (o.agg_mean[a] && cnt) {
         double mean = o.agg_mean[a] == 1 ? (double)((long long)v / (long long)cnt)
                                          : __longlong_as_double((long long)v) / (double)cnt;
@@ -495,30 +681,31 @@ struct fpdb_group_table {
   fpdb_agg_spec aggs[fpdb::kMaxAggs];
   fpdb::AccSpec acc;                 // visible aggregates first, then hidden valid counts
   int32_t hidden_of[fpdb::kMaxAggs];  // accumulator index of agg a's valid count, -1 if none
-  int64_t capacity = 0;
-  uint64_t* keys = nullptr;
-  uint64_t* accs = nullptr;
-  int64_t* first = nullptr;
+  int64_t max_groups = 0;
+  int rec_words = 0;                 // 2 + accumulators
+  int nbuckets = 1, bucket_shift = 64;
+  uint32_t local_slots = 0;          // LDS slots of the local and scatter passes
+  uint32_t compact_slots = 0;        // LDS slots of the compact pass
+  size_t local_lds = 0, scatter_lds = 0, compact_lds = 0;
+  int max_blocks = 0;
+  // device buffers
+  uint64_t* pool = nullptr;          // one record per group so far (bucket ranges ostart/ocount)
+  int64_t pool_cap = 0;              // records
+  uint64_t* spare = nullptr;         // the next pool
+  int64_t spare_cap = 0;
+  int64_t* ostart = nullptr;         // nbuckets
+  int64_t* ocount = nullptr;         // nbuckets
+  int64_t* nstart = nullptr;         // nbuckets
+  int64_t* newcount = nullptr;       // nbuckets
+  uint64_t* base = nullptr;          // nbuckets x max_blocks
+  uint32_t* hist = nullptr;          // max_blocks x nbuckets
+  uint64_t* partial = nullptr;       // max_blocks x (local_slots + 1) records
+  unsigned long long* totals = nullptr;   // [0] records of the new pool, [1] groups
+  int* flags = nullptr;              // [0] bucket overflow, [1] key outside its packed range
   int64_t rows = 0;
-  unsigned long long* d_groups = nullptr;   // [0] groups, [1] overflow flag (int) + key range flag (int)
 };
 
 namespace fpdb {
-
-static GlobalTable global_view(fpdb_group_table* t) {
-  GlobalTable g;
-  g.keys = t->keys;
-  g.acc = t->accs;
-  g.first = t->first;
-  g.mask = (uint64_t)t->capacity - 1;
-  g.stride = t->capacity + 1;
-  g.nacc = t->acc.n;
-  g.nrep = kReplicas;
-  g.groups = t->d_groups;
-  g.overflow = reinterpret_cast<int*>(t->d_groups + 1);
-  g.key_range = reinterpret_cast<int*>(t->d_groups + 1) + 1;
-  return g;
-}
 
 static int32_t acc_op_for(const fpdb_agg_spec& s) {
   bool f = type_is_float(s.input_type);
@@ -532,6 +719,10 @@ static int32_t acc_op_for(const fpdb_agg_spec& s) {
     case FPDB_AGG_MAX: return f ? kAccMaxF : (sg ? kAccMaxI : kAccMaxU);
     default: return -1;
   }
+}
+
+static fpdb_status gb_alloc(fpdb_group_table* t, uint64_t bytes, void** p) {
+  return ctx_alloc(t->ctx, std::max<uint64_t>(bytes, 8), p);
 }
 
 }  // namespace fpdb
@@ -559,14 +750,17 @@ int32_t fpdb_groupby_output_type(fpdb_agg_spec s) {
 namespace fpdb {
 
 static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_types, int num_keys,
-                                      const int64_t* key_min, const int32_t* key_bits, const fpdb_agg_spec* aggs,
-                                      int num_aggs, int64_t max_groups, fpdb_group_table** out) {
+                                       const int64_t* key_min, const int32_t* key_bits, const fpdb_agg_spec* aggs,
+                                       int num_aggs, int64_t max_groups, fpdb_group_table** out) {
   FPDB_CHECK(num_aggs >= 0 && (num_aggs == 0 || aggs), FPDB_ERR_INVALID, "group-by: bad aggregates");
-  FPDB_CHECK(max_groups >= 1, FPDB_ERR_INVALID, "group-by: max_groups must be positive");
+  FPDB_CHECK(num_aggs <= kMaxAggs, FPDB_ERR_INVALID, "group-by: too many aggregates");
+  FPDB_CHECK(max_groups >= 1 && max_groups < (int64_t(1) << 32), FPDB_ERR_INVALID,
+             "group-by: max_groups must be in [1, 2^32)");
   Context* c = as_ctx(ctx);
   auto* t = new fpdb_group_table();
   t->ctx = c;
   t->num_keys = num_keys;
+  t->max_groups = max_groups;
   int shift = 0;
   for (int i = 0; i < num_keys; ++i) {
     t->key_types[i] = key_types[i];
@@ -609,35 +803,86 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
     t->hidden_of[a] = nacc;
     ++nacc;
   }
-  if (nacc > kMaxAggs) {
-    delete t;
-    return fail(FPDB_ERR_INVALID, "group-by: too many aggregates");
-  }
   t->acc.n = nacc;
-  int64_t cap = 64;
-  while (cap < 2 * max_groups) cap <<= 1;
-  t->capacity = cap;
-  fpdb_status s = ctx_alloc(c, (uint64_t)cap * 8 * kReplicas, reinterpret_cast<void**>(&t->keys));
+  t->rec_words = 2 + nacc;
+  // LDS budget per workgroup (one workgroup of 1024 threads per CU)
+  size_t budget = std::min<size_t>(c->max_lds_per_block, 160 * 1024) - 256;   // room for static LDS
+  size_t per_slot = 8 * (size_t)(2 + nacc);
+  // buckets: a compact workgroup sees about max_groups / nbuckets groups; keep
+  // that within a quarter of its slots
+  uint32_t cslots = (uint32_t)(budget / per_slot) - 1;
+  int64_t nb = kMinBuckets;
+  while (nb < kMaxBuckets && nb * (int64_t)(cslots / 4) < max_groups) nb <<= 1;
+  t->nbuckets = (int)nb;
+  int lg = 0;
+  while ((int64_t(1) << lg) < nb) ++lg;
+  t->bucket_shift = 64 - lg;
+  t->compact_slots = cslots;
+  t->compact_lds = (size_t)(cslots + 1) * per_slot;
+  size_t hist_bytes = (size_t)nb * 4;
+  if (budget < hist_bytes + 65 * per_slot) {
+    delete t;
+    return fail(FPDB_ERR_INVALID, "group-by: too many aggregates for the LDS table");
+  }
+  t->local_slots = (uint32_t)((budget - hist_bytes) / per_slot) - 1;
+  t->local_lds = (size_t)(t->local_slots + 1) * per_slot + hist_bytes;
+  t->scatter_lds = (size_t)(t->local_slots + 1) * 8 + hist_bytes;
+  t->max_blocks = std::max(1, c->num_cus);
+  if (t->local_slots < 64 || cslots < 64) {
+    delete t;
+    return fail(FPDB_ERR_INVALID, "group-by: too many aggregates for the LDS table");
+  }
+  fpdb_status s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->ostart));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->ocount));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->nstart));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->newcount));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 8, reinterpret_cast<void**>(&t->base));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 4, reinterpret_cast<void**>(&t->hist));
   if (s == FPDB_OK)
-    s = ctx_alloc(c, (uint64_t)(cap + 1) * 8 * std::max(nacc, 1) * kReplicas, reinterpret_cast<void**>(&t->accs));
-  if (s == FPDB_OK) s = ctx_alloc(c, (uint64_t)(cap + 1) * 8 * kReplicas, reinterpret_cast<void**>(&t->first));
-  if (s == FPDB_OK) s = ctx_alloc(c, 16, reinterpret_cast<void**>(&t->d_groups));
-  if (s == FPDB_OK) s = launch_fill(c, t->d_groups, 0, 16);
+    s = gb_alloc(t, (uint64_t)t->max_blocks * (t->local_slots + 1) * t->rec_words * 8,
+                 reinterpret_cast<void**>(&t->partial));
+  if (s == FPDB_OK) s = gb_alloc(t, 16, reinterpret_cast<void**>(&t->totals));
+  if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->flags));
+  if (s == FPDB_OK) s = launch_fill(c, t->ostart, 0, (uint64_t)nb * 8);
+  if (s == FPDB_OK) s = launch_fill(c, t->ocount, 0, (uint64_t)nb * 8);
+  if (s == FPDB_OK) s = launch_fill(c, t->totals, 0, 16);
+  if (s == FPDB_OK) s = launch_fill(c, t->flags, 0, 8);
   if (s != FPDB_OK) {
     fpdb_groupby_destroy(t);
     return s;
   }
-  GlobalTable g = global_view(t);
-  hipLaunchKernelGGL(groupby_init_kernel, dim3(grid_for(c, (cap + 1) * kReplicas, 256)), dim3(256), 0, c->stream, g,
-                     t->acc);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    fpdb_groupby_destroy(t);
-    return fail(FPDB_ERR_HIP, hipGetErrorString(e));
-  }
   *out = t;
   return FPDB_OK;
 }
+
+template <int NACC>
+static void launch_local(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
+                         const GbGeom& G) {
+  hipLaunchKernelGGL(gb_local_kernel<NACC>, dim3(blocks), dim3(kGroupThreads), t->local_lds, c->stream, k, vals,
+                     t->acc, G, t->hist, t->partial, t->flags + 1);
+}
+template <int NACC>
+static void launch_scatter(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
+                           const GbGeom& G) {
+  hipLaunchKernelGGL(gb_scatter_kernel<NACC>, dim3(blocks), dim3(kGroupThreads), t->scatter_lds, c->stream, k, vals,
+                     t->acc, G, t->partial, t->base, t->spare);
+}
+template <int NACC>
+static void launch_compact(Context* c, fpdb_group_table* t) {
+  hipLaunchKernelGGL(gb_compact_kernel<NACC>, dim3(t->nbuckets), dim3(kGroupThreads), t->compact_lds, c->stream,
+                     t->acc, t->compact_slots, t->rec_words, t->pool, t->ostart, t->ocount, t->spare, t->nstart,
+                     t->newcount, t->totals + 1, t->flags);
+}
+
+#define FPDB_GB_DISPATCH(FN, ...)                 \
+  switch (t->acc.n) {                             \
+    case 1: FN<1>(__VA_ARGS__); break;            \
+    case 2: FN<2>(__VA_ARGS__); break;            \
+    case 3: FN<3>(__VA_ARGS__); break;            \
+    case 4: FN<4>(__VA_ARGS__); break;            \
+    case 5: FN<5>(__VA_ARGS__); break;            \
+    default: FN<0>(__VA_ARGS__); break;           \
+  }
 
 }  // namespace fpdb
 
@@ -711,10 +956,10 @@ fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, in
 
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
-  if (t->keys) ctx_free(t->ctx, t->keys);
-  if (t->accs) ctx_free(t->ctx, t->accs);
-  if (t->first) ctx_free(t->ctx, t->first);
-  if (t->d_groups) ctx_free(t->ctx, t->d_groups);
+  void* bufs[] = {t->pool, t->spare, t->ostart, t->ocount, t->nstart, t->newcount,
+                  t->base, t->hist, t->partial, t->totals, t->flags};
+  for (void* b : bufs)
+    if (b) ctx_free(t->ctx, b);
   delete t;
   return FPDB_OK;
 }
@@ -761,36 +1006,48 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* t, const f
     vals.type[a] = values[a].type;
   }
   if (n == 0) return FPDB_OK;
-  int nacc = t->acc.n;
-  // LDS table size: the largest power of two that fits the per-block budget
-  size_t per_slot = 8 * (size_t)(2 + nacc);
-  size_t budget = std::min<size_t>(c->max_lds_per_block, 128 * 1024);
-  int slots = 256;
-  while ((size_t)slots * 2 * per_slot <= budget) slots *= 2;
-  size_t shmem = (size_t)slots * per_slot;
-  int blocks = std::max(1, c->num_cus);
-  int64_t max_blocks = ceil_div(n, 4096);
-  blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, max_blocks));
-  GlobalTable g = global_view(t);
-  ProfScope ps(c, "groupby_consume");
-  switch (nacc) {
-#define FPDB_GB_CASE(N)                                                                                     \
-  case N:                                                                                                   \
-    hipLaunchKernelGGL(groupby_consume_kernel<N>, dim3(blocks), dim3(kGroupThreads), shmem, c->stream, k, vals, \
-                       t->acc, n, t->rows, g, slots);                                                       \
-    break;
-    FPDB_GB_CASE(1)
-    FPDB_GB_CASE(2)
-    FPDB_GB_CASE(3)
-    FPDB_GB_CASE(4)
-    FPDB_GB_CASE(5)
-#undef FPDB_GB_CASE
-    default:
-      hipLaunchKernelGGL(groupby_consume_kernel<0>, dim3(blocks), dim3(kGroupThreads), shmem, c->stream, k, vals,
-                         t->acc, n, t->rows, g, slots);
-      break;
+  FPDB_CHECK(t->rows + n < (int64_t(1) << 62), FPDB_ERR_INVALID, "group-by: too many rows");
+  int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(t->max_blocks, ceil_div(n, kRowsPerWorkgroup)));
+  GbGeom G;
+  G.n = n;
+  G.row_base = t->rows;
+  G.per_block = ceil_div(n, blocks);
+  G.nslots = t->local_slots;
+  G.nbuckets = t->nbuckets;
+  G.bucket_shift = t->bucket_shift;
+  G.rec_words = t->rec_words;
+  {
+    ProfScope ps(c, "groupby_local");
+    FPDB_GB_DISPATCH(launch_local, c, t, blocks, k, vals, G);
+    FPDB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(gb_scan_kernel, dim3(1), dim3(1024), 0, c->stream, t->hist, blocks, t->nbuckets, t->ocount,
+                       t->base, t->nstart, t->newcount, t->totals);
+    FPDB_HIP(hipGetLastError());
   }
-  FPDB_HIP(hipGetLastError());
+  // the new pool's size decides its allocation
+  unsigned long long total = 0;
+  FPDB_HIP(hipMemcpyAsync(&total, t->totals, 8, hipMemcpyDeviceToHost, c->stream));
+  FPDB_HIP(hipStreamSynchronize(c->stream));
+  if ((int64_t)total > t->spare_cap) {
+    if (t->spare) FPDB_TRY(ctx_free(c, t->spare));
+    t->spare = nullptr;
+    t->spare_cap = 0;
+    int64_t want = std::max<int64_t>((int64_t)total, 1024);
+    FPDB_TRY(gb_alloc(t, (uint64_t)want * t->rec_words * 8, reinterpret_cast<void**>(&t->spare)));
+    t->spare_cap = want;
+  }
+  {
+    ProfScope ps(c, "groupby_scatter");
+    FPDB_GB_DISPATCH(launch_scatter, c, t, blocks, k, vals, G);
+    FPDB_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(c, "groupby_compact");
+    FPDB_GB_DISPATCH(launch_compact, c, t);
+    FPDB_HIP(hipGetLastError());
+  }
+  std::swap(t->pool, t->spare);
+  std::swap(t->pool_cap, t->spare_cap);
   t->rows += n;
   return FPDB_OK;
 }
@@ -798,22 +1055,16 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* t, const f
 fpdb_status fpdb_groupby_num_groups(fpdb_context* ctx, fpdb_group_table* t, int64_t* num_groups) {
   FPDB_CHECK(ctx && t && num_groups, FPDB_ERR_INVALID, "NULL argument");
   Context* c = as_ctx(ctx);
-  if (t->rows > 0) {
-    GlobalTable g = global_view(t);
-    ProfScope ps(c, "groupby_merge");
-    hipLaunchKernelGGL(groupby_merge_kernel, dim3(grid_for(c, g.stride * (kReplicas - 1), 256)), dim3(256), 0,
-                       c->stream, g, t->acc);
-    FPDB_HIP(hipGetLastError());
-  }
-  FPDB_HIP(hipMemcpyAsync(c->h_counter, t->d_groups, 16, hipMemcpyDeviceToHost, c->stream));
-  int64_t sentinel_first = 0;
-  FPDB_HIP(hipMemcpyAsync(&sentinel_first, t->first + t->capacity, 8, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long groups[2] = {0, 0};
+  int flags[2] = {0, 0};
+  FPDB_HIP(hipMemcpyAsync(groups, t->totals, 16, hipMemcpyDeviceToHost, c->stream));
+  FPDB_HIP(hipMemcpyAsync(flags, t->flags, 8, hipMemcpyDeviceToHost, c->stream));
   FPDB_HIP(hipStreamSynchronize(c->stream));
-  FPDB_CHECK(reinterpret_cast<int*>(c->h_counter + 1)[1] == 0, FPDB_ERR_INVALID,
+  FPDB_CHECK(flags[1] == 0, FPDB_ERR_INVALID,
              "group-by: a key value lies outside the range the packed key was created for");
-  FPDB_CHECK(reinterpret_cast<int*>(c->h_counter + 1)[0] == 0, FPDB_ERR_CAPACITY,
+  FPDB_CHECK(flags[0] == 0 && (int64_t)groups[1] <= t->max_groups, FPDB_ERR_CAPACITY,
              "group-by: more groups than max_groups");
-  *num_groups = c->h_counter[0] + (sentinel_first != LLONG_MAX ? 1 : 0);
+  *num_groups = (int64_t)groups[1];
   return FPDB_OK;
 }
 
@@ -870,15 +1121,15 @@ fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* t, fpdb_c
   if (s == FPDB_OK) s = ctx_alloc(c, (uint64_t)words * 4, reinterpret_cast<void**>(&word_prefix));
   if (s == FPDB_OK) s = launch_fill(c, bits, 0, (uint64_t)words * 8);
   if (s == FPDB_OK) {
-    GlobalTable g = global_view(t);
-    hipLaunchKernelGGL(groupby_mark_kernel, dim3(grid_for(c, g.stride, 256)), dim3(256), 0, c->stream, g, bits);
+    hipLaunchKernelGGL(gb_mark_kernel, dim3(t->nbuckets), dim3(256), 0, c->stream, t->pool, t->ostart, t->ocount,
+                       t->rec_words, bits);
     hipLaunchKernelGGL(popcount_chunk_kernel, dim3((unsigned)chunks), dim3(1024), 0, c->stream, bits, words,
                        chunk_sums);
     hipLaunchKernelGGL(chunk_scan_kernel, dim3(1), dim3(1024), 0, c->stream, chunk_sums, chunks);
     hipLaunchKernelGGL(word_prefix_kernel, dim3((unsigned)chunks), dim3(1024), 0, c->stream, bits, words,
                        chunk_sums, word_prefix);
-    hipLaunchKernelGGL(groupby_emit_kernel, dim3(grid_for(c, g.stride, 256)), dim3(256), 0, c->stream, g, t->acc,
-                       bits, word_prefix, o);
+    hipLaunchKernelGGL(gb_emit_kernel, dim3(t->nbuckets), dim3(256), 0, c->stream, t->pool, t->ostart, t->ocount,
+                       t->rec_words, bits, word_prefix, o);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) s = fail(FPDB_ERR_HIP, hipGetErrorString(e));
     if (s == FPDB_OK) {

@@ -142,17 +142,21 @@ constexpr int kProbeWaves = kProbeThreads / 64;
 constexpr int64_t kProbeTileRows = kProbeItems * kProbeThreads;
 
 // Writes one joined row: probe columns copied from probe row r, build columns
-// gathered from build row brow (or the inline payload).
+// gathered from build row brow, or the inline payload, or (bit c of keymask:
+// the column is the build key itself) the matching probe key.
 __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bcols, const OutColSet& out,
                                           int64_t out_cap, int64_t r, int64_t dst, uint32_t brow,
-                                          uint32_t payload) {
+                                          uint32_t payload, uint32_t keymask, int64_t key) {
   if (dst >= out_cap) return;
   for (int c = 0; c < pcols.n; ++c) copy_value(pcols, out, c, r, dst);
   for (int c = 0; c < bcols.n; ++c) {
     int oc = pcols.n + c;
     int64_t di = dst + out.offset[oc];
     bool valid;
-    if (bcols.values[c] == nullptr) {
+    if (keymask & (1u << c)) {
+      store_w(out.values[oc], out.width[oc], di, (uint64_t)key);
+      valid = true;
+    } else if (bcols.values[c] == nullptr) {
       store_w(out.values[oc], out.width[oc], di, payload);
       valid = true;
     } else {
@@ -169,7 +173,8 @@ __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bco
 template <int KM, int JT, bool TRACK>
 __global__ __launch_bounds__(256) void hj_probe_kernel(
     const Slot* __restrict__ slots, uint64_t mask, KeySpec pk, int64_t n, ColSet pcols, ColSet bcols,
-    OutColSet out, int64_t out_cap, uint8_t* __restrict__ matched, LbArgs lb, int64_t* total, int* err) {
+    OutColSet out, int64_t out_cap, uint8_t* __restrict__ matched, uint32_t keymask, const int* dup_flag,
+    LbArgs lb, int64_t* total, int* err) {
   __shared__ int64_t s_tile;
   __shared__ uint32_t s_epoch;
   __shared__ uint32_t s_cnt[kProbeItems][kProbeWaves];
@@ -180,19 +185,27 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t row0 = tile * kProbeTileRows + threadIdx.x;
+  // distinct build keys (checked when the table was built): a probe stops at
+  // its first match
+  const bool unique = *dup_flag == 0;
 
   int64_t key[kProbeItems];
   bool live[kProbeItems];
   uint64_t home[kProbeItems];
-  Slot first[kProbeItems];
+  Slot first[kProbeItems], second[kProbeItems];
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
     int64_t r = row0 + (int64_t)i * kProbeThreads;
     live[i] = r < n && get_key<KM>(pk, r, &key[i]);
     home[i] = live[i] ? slot_of(key[i], mask) : 0;
   }
+  // the home slot and the next one are loaded together (one round trip for
+  // most probes)
 #pragma unroll
-  for (int i = 0; i < kProbeItems; ++i) first[i] = live[i] ? load_slot(slots, home[i]) : Slot{0, kEmptyRow, 0};
+  for (int i = 0; i < kProbeItems; ++i) {
+    first[i] = live[i] ? load_slot(slots, home[i]) : Slot{0, kEmptyRow, 0};
+    second[i] = live[i] ? load_slot(slots, (home[i] + 1) & mask) : Slot{0, kEmptyRow, 0};
+  }
 
   uint32_t cnt[kProbeItems];       // matches
   uint64_t fslot[kProbeItems];     // slot of the first match
@@ -206,14 +219,15 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
     if (!live[i]) continue;
     Slot sl = first[i];
     uint64_t s = home[i];
+    int step = 0;
     while (sl.row != kEmptyRow) {
       if (sl.key == key[i]) {
         if (cnt[i] == 0) { fslot[i] = s; frow[i] = sl.row; fpay[i] = sl.payload; }
         ++cnt[i];
-        if (JT == kJtSemi) break;
+        if (JT == kJtSemi || unique) break;
       }
       s = (s + 1) & mask;
-      sl = load_slot(slots, s);
+      sl = (++step == 1) ? second[i] : load_slot(slots, s);
     }
   }
 
@@ -289,14 +303,14 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
     }
     // matched rows: the first match is known, later ones continue the chain
     int64_t dst = (int64_t)pos;
-    emit_pair(pcols, bcols, out, out_cap, r, dst, frow[i], fpay[i]);
+    emit_pair(pcols, bcols, out, out_cap, r, dst, frow[i], fpay[i], keymask, key[i]);
     uint32_t emitted = 1;
     uint64_t s = (fslot[i] + 1) & mask;
     while (emitted < cnt[i]) {
       Slot sl = load_slot(slots, s);
       if (sl.row == kEmptyRow) break;
       if (sl.key == key[i]) {
-        emit_pair(pcols, bcols, out, out_cap, r, dst + emitted, sl.row, sl.payload);
+        emit_pair(pcols, bcols, out, out_cap, r, dst + emitted, sl.row, sl.payload, keymask, key[i]);
         ++emitted;
       }
       s = (s + 1) & mask;
@@ -355,6 +369,8 @@ struct fpdb_hash_table {
   int64_t inserted = 0;
   int num_keys = 0;
   int32_t key_types[2] = {0, 0};
+  const void* key_values = nullptr;   // the build key column (compared, never read)
+  int64_t key_offset = 0;
   bool has_payload = false;
   int32_t payload_type = 0;
   bool unique = true;
@@ -435,9 +451,14 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
              "hash join: too many output columns");
   std::memset(&bc, 0, sizeof(bc));
   bc.n = num_build_out;
+  uint32_t keymask = 0;
   for (int i = 0; i < num_build_out; ++i) {
     int w = type_width(build_out[i].type);
     FPDB_CHECK(w > 0, FPDB_ERR_INVALID, "hash join: unsupported build column type");
+    // the single-column build key itself: a matching row's value is the probe key
+    if (t->num_keys == 1 && build_out[i].values && build_out[i].values == t->key_values &&
+        build_out[i].offset == t->key_offset && build_out[i].type == t->key_types[0])
+      keymask |= 1u << i;
     if (build_out[i].values == nullptr) {
       FPDB_CHECK(t->has_payload && w == 4, FPDB_ERR_INVALID,
                  "hash join: build column without values but the table has no 4-byte inline payload");
@@ -494,7 +515,7 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
   ProfScope ps(c, "hj_probe");
 #define FPDB_PROBE_LAUNCH(KM, JT, TR)                                                              \
   hipLaunchKernelGGL((hj_probe_kernel<KM, JT, TR>), g, b, 0, c->stream, t->slots, mask, pk, n, pc, bc, \
-                     oc, out_cap, t->matched, lb, total, d_err)
+                     oc, out_cap, t->matched, keymask, reinterpret_cast<const int*>(t->meta + 1), lb, total, d_err)
 #define FPDB_PROBE_JT(KM)                                                    \
   switch (jt * 2 + (track ? 1 : 0)) {                                        \
     case kJtInner * 2: FPDB_PROBE_LAUNCH(KM, kJtInner, false); break;        \
@@ -549,6 +570,8 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
   t->build_rows = num_rows;
   t->num_keys = num_keys;
   for (int i = 0; i < num_keys; ++i) t->key_types[i] = keys[i].type;
+  t->key_values = keys[0].values;
+  t->key_offset = keys[0].offset;
   t->has_payload = payload != nullptr;
   t->payload_type = payload ? payload->type : 0;
   fpdb_status s = ctx_alloc(c, (uint64_t)cap * sizeof(Slot), reinterpret_cast<void**>(&t->slots));
@@ -575,6 +598,10 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
       case 2: hipLaunchKernelGGL(hj_build_kernel<2>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
       default: hipLaunchKernelGGL(hj_build_kernel<0>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
     }
+    // the duplicate-key flag, read by every probe (a table of distinct keys
+    // lets a probe stop at its first match)
+    hipLaunchKernelGGL(hj_unique_kernel, dim3(grid_for(c, cap, 256)), dim3(256), 0, c->stream, t->slots,
+                       (uint64_t)cap - 1, reinterpret_cast<int*>(t->meta + 1));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { fpdb_hj_table_destroy(t); return fail(FPDB_ERR_HIP, hipGetErrorString(e)); }
   }
@@ -591,18 +618,13 @@ fpdb_status fpdb_hj_table_destroy(fpdb_hash_table* t) {
   return FPDB_OK;
 }
 
-// The build is asynchronous; the first info query runs the duplicate-key
-// check and reads the table's counters back.
+// The build is asynchronous (its duplicate-key check included); the first
+// info query reads the table's counters back.
 fpdb_status fpdb_hj_table_info(fpdb_hash_table* t, int64_t* build_rows, int64_t* inserted,
                                int64_t* capacity, int* unique_keys) {
   FPDB_CHECK(t, FPDB_ERR_INVALID, "NULL table");
   if (!t->info_ready) {
     Context* c = t->ctx;
-    if (t->build_rows > 0) {
-      hipLaunchKernelGGL(hj_unique_kernel, dim3(grid_for(c, t->capacity, 256)), dim3(256), 0, c->stream,
-                         t->slots, (uint64_t)t->capacity - 1, reinterpret_cast<int*>(t->meta + 1));
-      FPDB_HIP(hipGetLastError());
-    }
     int64_t meta[3];
     FPDB_HIP(hipMemcpyAsync(meta, t->meta, sizeof(meta), hipMemcpyDeviceToHost, c->stream));
     FPDB_HIP(hipStreamSynchronize(c->stream));

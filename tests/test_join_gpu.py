@@ -78,8 +78,12 @@ def test_null_keys_never_match_and_int32_keys(ctx):
                           np.packbits(pvalid, bitorder="little"))
     rows = DeviceColumn.from_numpy(ctx, np.arange(4000, dtype=np.int64))
     brows = DeviceColumn.from_numpy(ctx, np.arange(3000, dtype=np.int64))
-    out, cnt = t.probe(C.JOIN_INNER, [dpk], len(pk), [rows], [brows])
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], len(pk), [rows], [brows, dbk])
     assert sorted(zip(out[0].numpy(cnt).tolist(), out[1].numpy(cnt).tolist())) == sorted(zip(op.tolist(), ob.tolist()))
+    # the (nullable, int32) build key as an output column: the matched key, always valid
+    got = out[1].numpy(cnt)
+    assert np.array_equal(out[2].numpy(cnt), bk[got])
+    assert out[2].valid_mask()[:cnt].all()
 
 
 def test_two_column_keys(ctx):
