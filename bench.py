@@ -228,7 +228,7 @@ def bench_groupby(ctx, steps):
             "compact_ms": compact_ms, "consume_wall_ms": statistics.median(wall),
             "grows_per_s": n / (cons_ms * 1e-3) / 1e9, "finalize_ms": finalize_ms,
             "hbm_frac": alg / (cons_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic_bytes": load_traffic("gb_local_kernel")}
+            "traffic_bytes": load_traffic("groupby_local")}
 
 
 def bench_star(ctx, steps, device_name):
@@ -269,6 +269,44 @@ def bench_star(ctx, steps, device_name):
             "batched_ms": batched_ms, "plain_ms": plain_ms, "speedup": plain_ms / batched_ms,
             "groups": int(len(got["count"])), "survivors": exp["survivors"], "tuning": list(best),
             "fact_fk_hbm_frac": fact_bytes / (batched_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+SERVE_RATES = ((50, 150), (200, 400), (800, 800))   # (queries/s, queries) per arrival rate
+
+
+def bench_serve(ctx, rates=SERVE_RATES):
+    """The serving loop over the resident 60M-row star schema: Poisson
+    arrivals at each rate, replayed in real time by the native loop; p50/p99
+    latency (arrival to completion) and throughput per rate."""
+    from flexpushdowndb_amd import datagen
+    from flexpushdowndb_amd.serve import Server, poisson_arrivals
+    log("serve: generating the resident schema (60M fact rows)")
+    w = datagen.serve_workload()
+    srv = Server(ctx, w, max_active=8)
+    # warm-up, and two queries checked against direct indexing
+    warm = datagen.serve_queries(8, seed=99)
+    ids = [srv.submit(q) for q in warm]
+    while srv.status(ids[-1]) != 2:
+        srv.step()
+    for q, i in list(zip(warm, ids))[:2]:
+        got, exp = srv.result(i), datagen.serve_expected(w, q)
+        if not all(np.array_equal(got[k], exp[k]) for k in ("g0", "g1", "count", "sum")):
+            raise SystemExit("serving result does not match the expected join + group-by")
+    out = {"workload": "serving: star schema resident in HBM (60,000,000 fact rows, 4 dimensions of "
+                       "30,000/200,000/2,000,000/2,556 rows); each query filters 1-4 dimensions at log-uniform "
+                       "selectivity in [0.1%, 50%] and groups by 1-2 dimension attributes; up to 8 queries "
+                       "share one scan", "rates": []}
+    for k, (rate, n) in enumerate(rates):
+        qs = datagen.serve_queries(n, seed=100 + k)
+        arr = poisson_arrivals(n, rate, seed=200 + k)
+        ids, lat, makespan = srv.run_trace(qs, arr)
+        lat_ms = lat / 1e6
+        out["rates"].append({"arrival_qps": rate, "queries": n, "p50_ms": float(np.percentile(lat_ms, 50)),
+                             "p99_ms": float(np.percentile(lat_ms, 99)), "mean_ms": float(lat_ms.mean()),
+                             "throughput_qps": n / (makespan / 1e9)})
+        log(f"serve: {rate} q/s: p50 {out['rates'][-1]['p50_ms']:.3f} ms, p99 {out['rates'][-1]['p99_ms']:.3f} ms")
+    srv.close()
+    return out
 
 
 def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
@@ -313,7 +351,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", choices=["join", "bloom", "groupby", "star", "calib"], default=None,
+    ap.add_argument("--only", choices=["join", "bloom", "groupby", "star", "serve", "calib"], default=None,
                     help="run one workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -341,6 +379,8 @@ def main():
     if args.only in (None, "star"):
         res["star"] = bench_star(ctx, max(3, args.steps // 2), f"gfx950-{info['num_cus']}cu")
         log(f"star: batched {res['star']['batched_ms']:.3f} ms, plain {res['star']['plain_ms']:.3f} ms")
+    if args.only in (None, "serve"):
+        res["serve"] = bench_serve(ctx)
     if args.only not in (None, "join"):
         print(json.dumps({k: v for k, v in res[args.only].items() if k != "inputs"}))
         return
@@ -377,6 +417,8 @@ def main():
         line["groupby"] = res["groupby"]
     if "star" in res:
         line["batched_star_join"] = res["star"]
+    if "serve" in res:
+        line["serving"] = res["serve"]
     line["device"] = info
     print(json.dumps(line), flush=True)
 

@@ -73,6 +73,26 @@ class StarTuning(ctypes.Structure):
                 ("ht_load_shift", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
 
 
+class ServeDimension(ctypes.Structure):
+    _fields_ = [("key", Column), ("filter_attr", Column), ("group_attr", Column), ("group_domain", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("fact_fk", Column)]
+
+
+class ServeSchema(ctypes.Structure):
+    _fields_ = [("num_dims", ctypes.c_int32), ("reserved", ctypes.c_int32), ("dims", ServeDimension * 8),
+                ("fact_measure", Column), ("fact_rows", ctypes.c_int64)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("max_active", ctypes.c_int32), ("reserved", ctypes.c_int32), ("workspace_bytes", ctypes.c_uint64)]
+
+
+class ServeQuery(ctypes.Structure):
+    _fields_ = [("num_filters", ctypes.c_int32), ("num_group", ctypes.c_int32), ("filter_dim", ctypes.c_int32 * 4),
+                ("filter_op", ctypes.c_int32 * 4), ("filter_constant", ctypes.c_int64 * 4),
+                ("group_dim", ctypes.c_int32 * 2)]
+
+
 CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
 CMP_OR = 0x10
 
@@ -149,6 +169,13 @@ SIGNATURES = [
     ("fpdb_star_graph_launch", _I32, [_P, _P]),
     ("fpdb_star_result", _I32, [_P, _P, _P, _P, _P, _P, _I64, _PI64]),
     ("fpdb_star_filter", _I32, [_P, _P, _I, _I, _P, _I64, ctypes.POINTER(_I), _PI64]),
+    ("fpdb_serve_create", _I32, [_P, ctypes.POINTER(ServeSchema), ctypes.POINTER(ServeConfig), ctypes.POINTER(_P)]),
+    ("fpdb_serve_destroy", _I32, [_P]),
+    ("fpdb_serve_submit", _I32, [_P, ctypes.POINTER(ServeQuery), _PI64]),
+    ("fpdb_serve_step", _I32, [_P, ctypes.POINTER(_I)]),
+    ("fpdb_serve_status", _I32, [_P, _I64, ctypes.POINTER(_I)]),
+    ("fpdb_serve_result", _I32, [_P, _I64, _P, _P, _P, _P, _I64, _PI64]),
+    ("fpdb_serve_run_trace", _I32, [_P, ctypes.POINTER(ServeQuery), _PI64, _I64, _PI64, _PI64, _PI64]),
 ]
 
 _lib = None

@@ -152,3 +152,60 @@ def groupby_workload(n=60_000_000, seed=3, n_values=100_000):
             "qty": uniform_int(seed, 2, n, 1, 50),
             "price": uniform_int(seed, 3, n, 90_000, 10_500_000),
             "disc": uniform_unit(seed, 4, n) * 0.10}
+
+
+# ---- serving workload: the star schema resident in HBM, random queries ------
+
+SERVE_GROUP_DOMAIN = (40, 25, 50, 12)
+SERVE_ATTR_DOMAIN = 100_000
+
+
+def serve_workload(n_fact=60_000_000, seed=6, dims=STAR_DIMS, group_domain=SERVE_GROUP_DOMAIN):
+    """Star schema for the serving loop: dense dimension keys, a filter
+    attribute uniform in [0, 100000) and a group attribute per dimension."""
+    out = {"dims": [], "fact_fk": []}
+    for d, rows in enumerate(dims):
+        out["dims"].append({
+            "key": np.arange(rows, dtype=np.int64),
+            "attr": uniform_int(seed, 10 + d, rows, 0, SERVE_ATTR_DOMAIN - 1).astype(np.int32),
+            "group": uniform_int(seed, 20 + d, rows, 0, group_domain[d] - 1).astype(np.int32),
+            "domain": group_domain[d]})
+        out["fact_fk"].append(uniform_int(seed, 30 + d, n_fact, 0, rows - 1))
+    out["measure"] = uniform_int(seed, 40, n_fact, 1, 100)
+    return out
+
+
+def serve_queries(n, seed=7, n_dims=4):
+    """Queries of the serving workload: 1-4 filtered dimensions, each kept
+    with selectivity log-uniform in [0.1 %, 50 %] (attr < c), grouped by 1-2
+    dimensions' group attributes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        k = int(rng.integers(1, n_dims + 1))
+        fdims = sorted(rng.choice(n_dims, size=k, replace=False).tolist())
+        sel = 10 ** rng.uniform(np.log10(0.001), np.log10(0.5), size=k)
+        consts = [max(1, int(round(s * SERVE_ATTR_DOMAIN))) for s in sel]
+        g = int(rng.integers(1, 3))
+        gdims = rng.choice(n_dims, size=g, replace=False).tolist()
+        out.append({"filters": list(zip(fdims, consts)), "group": gdims})
+    return out
+
+
+def serve_expected(w, q):
+    """Result of one serving query by direct indexing of the dense keys:
+    groups (g0, g1) with COUNT(*) and SUM(measure) in first-appearance order."""
+    keep = np.ones(len(w["measure"]), bool)
+    for d, c in q["filters"]:
+        keep &= (w["dims"][d]["attr"] < c)[w["fact_fk"][d]]
+    rows = np.flatnonzero(keep)
+    gd = q["group"]
+    g0 = w["dims"][gd[0]]["group"][w["fact_fk"][gd[0]][rows]].astype(np.int64)
+    g1 = (w["dims"][gd[1]]["group"][w["fact_fk"][gd[1]][rows]].astype(np.int64) if len(gd) == 2
+          else np.zeros(len(rows), np.int64))
+    key = g0 * (1 << 32) + g1
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    cnt = np.bincount(inv, minlength=len(uniq))
+    sm = np.bincount(inv, weights=w["measure"][rows], minlength=len(uniq)).astype(np.int64)
+    order = np.argsort(first, kind="stable")
+    return {"g0": g0[first[order]], "g1": g1[first[order]], "count": cnt[order], "sum": sm[order]}

@@ -452,6 +452,75 @@ fpdb_status fpdb_star_result(fpdb_context* ctx, fpdb_star* star, int32_t* g_a, i
 fpdb_status fpdb_star_filter(fpdb_context* ctx, fpdb_star* star, int dim, int backward, uint64_t* blocks,
                              int64_t capacity, int* log_num_blocks, int64_t* filter_input_rows);
 
+/* ---- serving loop: continuous batching over a resident star schema ------- */
+
+/* A dimension of the resident schema: dense keys (key[i] == i), an int32
+ * filter attribute, an optional int32 group attribute with values in
+ * [0, group_domain), and the fact table's foreign keys into it. */
+typedef struct fpdb_serve_dimension {
+  fpdb_column key;
+  fpdb_column filter_attr;
+  fpdb_column group_attr;     /* values NULL: no group attribute */
+  int32_t group_domain;
+  int32_t reserved;
+  fpdb_column fact_fk;
+} fpdb_serve_dimension;
+
+typedef struct fpdb_serve_schema {
+  int32_t num_dims;           /* 1..8 */
+  int32_t reserved;
+  fpdb_serve_dimension dims[8];
+  fpdb_column fact_measure;   /* int64, non-null */
+  int64_t fact_rows;
+} fpdb_serve_schema;
+
+typedef struct fpdb_serve_config {
+  int32_t max_active;         /* queries sharing one scan, 1..8 */
+  int32_t reserved;
+  uint64_t workspace_bytes;   /* HBM budget for the queries in a scan (0: none) */
+} fpdb_serve_config;
+
+/* SELECT g0, g1, COUNT(*), SUM(measure) FROM fact JOIN dims WHERE <filters>
+ * GROUP BY <group attributes> - the star join + group-by the predicate-transfer
+ * plans of SmallToLargePredTransOrder.cpp:38-214 produce, for a serving
+ * workload (no counterpart in the reference, which runs one query plan at a
+ * time).  Filters on one dimension are AND-ed. */
+typedef struct fpdb_serve_query {
+  int32_t num_filters;        /* 0..4 */
+  int32_t num_group;          /* 0..2 (0: one group over all passing rows) */
+  int32_t filter_dim[4];
+  int32_t filter_op[4];       /* fpdb_cmp_op */
+  int64_t filter_constant[4];
+  int32_t group_dim[2];
+} fpdb_serve_query;
+
+typedef struct fpdb_serve fpdb_serve;
+
+/* Registers the resident schema (borrowed: the columns must outlive the
+ * server) and checks that dimension keys are dense. */
+fpdb_status fpdb_serve_create(fpdb_context* ctx, const fpdb_serve_schema* schema,
+                              const fpdb_serve_config* config, fpdb_serve** out);
+fpdb_status fpdb_serve_destroy(fpdb_serve* server);
+/* Queues a query (thread-safe); *query_id identifies it from then on. */
+fpdb_status fpdb_serve_submit(fpdb_serve* server, const fpdb_serve_query* query, int64_t* query_id);
+/* Admits queued queries (FIFO, within max_active, 8192 groups and the
+ * workspace budget), runs one shared scan for them and finishes them;
+ * *completed = queries finished (synchronizes). */
+fpdb_status fpdb_serve_step(fpdb_serve* server, int* completed);
+/* 0 queued, 1 in a scan, 2 finished. */
+fpdb_status fpdb_serve_status(fpdb_serve* server, int64_t query_id, int* state);
+/* A finished query's groups in first-appearance order (host arrays; g1 is 0
+ * for one group dimension, g0 and g1 are 0 for none). */
+fpdb_status fpdb_serve_result(fpdb_serve* server, int64_t query_id, int32_t* g0, int32_t* g1, int64_t* count,
+                              int64_t* sum, int64_t capacity, int64_t* num_groups);
+/* Replays an arrival trace in real time (arrival_ns non-decreasing, relative
+ * to the call): submits each query at its arrival, steps while work is queued,
+ * and reports each query's id and latency (completion minus arrival) and the
+ * whole trace's makespan. */
+fpdb_status fpdb_serve_run_trace(fpdb_serve* server, const fpdb_serve_query* queries, const int64_t* arrival_ns,
+                                 int64_t num_queries, int64_t* query_ids, int64_t* latency_ns,
+                                 int64_t* makespan_ns);
+
 #ifdef __cplusplus
 }
 #endif
