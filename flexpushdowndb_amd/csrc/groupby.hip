@@ -42,7 +42,7 @@ constexpr uint64_t kEmptyKey = ~0ULL;
 constexpr int kMaxAggs = 16;          // visible + hidden accumulators
 constexpr int kLdsProbes = 8;
 constexpr int kGroupThreads = 1024;
-constexpr int kMinBuckets = 256;
+constexpr int kMinBuckets = 64;
 constexpr int kMaxBuckets = 1 << 14;
 constexpr int64_t kRowsPerWorkgroup = 16384;   // fewer rows per workgroup: fewer workgroups
 
@@ -323,56 +323,67 @@ __global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, 
     if (f != LLONG_MAX) atomicAdd(&lhist[bucket_of(fmix64(k), G.bucket_shift)], 1u);
   }
   __syncthreads();
+  // bucket-major: entry 0 of each bucket's row is left for its old record count
   for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x)
-    hist[(size_t)blockIdx.x * G.nbuckets + p] = lhist[p];
+    hist[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x] = lhist[p];
 }
 
-// Bucket-major exclusive scan: base[p][b] = where workgroup b writes its
-// bucket-p records in the new pool, after the bucket's old records.  One
-// workgroup; also resets the group counter the compact pass adds to.
+// Bucket-major exclusive scan over rows of (old record count, per-workgroup
+// counts) of every bucket: base[p][0] = nstart[p], the start of bucket p in the
+// new pool; base[p][1 + b] = where workgroup b writes its bucket-p records,
+// after the bucket's old records.  One workgroup, each thread scanning a
+// contiguous run; also resets the group counter the compact pass adds to.
 __global__ __launch_bounds__(1024) void gb_scan_kernel(const uint32_t* __restrict__ hist, int blocks, int nbuckets,
                                                        const int64_t* __restrict__ ocount,
                                                        uint64_t* __restrict__ base, int64_t* __restrict__ nstart,
                                                        int64_t* __restrict__ newcount, unsigned long long* totals) {
-  __shared__ int64_t s_w[16];
-  __shared__ int64_t s_carry;
-  // per-bucket totals
-  for (int p = threadIdx.x; p < nbuckets; p += blockDim.x) {
-    int64_t t = 0;
-    for (int b = 0; b < blocks; ++b) t += hist[(size_t)b * nbuckets + p];
-    newcount[p] = t;
+  __shared__ uint64_t s_w[16];
+  const int64_t row = blocks + 1;
+  const int64_t len = (int64_t)nbuckets * row;
+  const int64_t per = (len + blockDim.x - 1) / blockDim.x;
+  const int64_t lo = (int64_t)threadIdx.x * per;
+  const int64_t hi = lo + per < len ? lo + per : len;
+  // walk (bucket, column) along with the flat index: one division per thread
+  const int64_t p_lo = lo / row, b_lo = lo - p_lo * row;
+  uint64_t mine = 0;
+  {
+    int64_t p = p_lo, b = b_lo;
+    for (int64_t i = lo; i < hi; ++i) {
+      mine += b == 0 ? (uint64_t)ocount[p] : (uint64_t)hist[i];
+      if (++b == row) { b = 0; ++p; }
+    }
   }
-  if (threadIdx.x == 0) s_carry = 0;
+  // exclusive scan of the per-thread sums
+  uint64_t incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    uint64_t t = __shfl_up(incl, off);
+    if ((threadIdx.x & 63) >= off) incl += t;
+  }
+  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
   __syncthreads();
-  // exclusive scan over buckets of (old + new) sizes
-  for (int p0 = 0; p0 < nbuckets; p0 += 1024) {
-    int p = p0 + threadIdx.x;
-    int64_t v = p < nbuckets ? ocount[p] + newcount[p] : 0;
-    int64_t incl = v;
-    for (int off = 1; off < 64; off <<= 1) {
-      int64_t t = __shfl_up(incl, off);
-      if ((threadIdx.x & 63) >= off) incl += t;
+  uint64_t run = incl - mine;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += s_w[w];
+  {
+    int64_t p = p_lo, b = b_lo;
+    for (int64_t i = lo; i < hi; ++i) {
+      base[i] = run;
+      if (b == 0) nstart[p] = (int64_t)run;
+      run += b == 0 ? (uint64_t)ocount[p] : (uint64_t)hist[i];
+      if (++b == row) { b = 0; ++p; }
     }
-    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
-    __syncthreads();
-    int64_t wprefix = 0;
-    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wprefix += s_w[w];
-    int64_t carry = s_carry;
-    if (p < nbuckets) nstart[p] = carry + wprefix + incl - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) s_carry = carry + wprefix + incl;
-    __syncthreads();
   }
+  if (threadIdx.x == blockDim.x - 1) {
+    totals[0] = (unsigned long long)run;   // records in the new pool
+    totals[1] = 0;                         // groups (the compact pass adds them up)
+  }
+  __syncthreads();
   for (int p = threadIdx.x; p < nbuckets; p += blockDim.x) {
-    uint64_t at = (uint64_t)(nstart[p] + ocount[p]);
-    for (int b = 0; b < blocks; ++b) {
-      base[(size_t)p * blocks + b] = at;
-      at += hist[(size_t)b * nbuckets + p];
+    uint64_t end = p + 1 < nbuckets ? base[(int64_t)(p + 1) * row] : 0;
+    if (p + 1 == nbuckets) {
+      end = base[(int64_t)p * row] + (uint64_t)ocount[p];
+      for (int b = 1; b <= blocks; ++b) end += hist[(int64_t)p * row + b];
     }
-  }
-  if (threadIdx.x == 0) {
-    totals[0] = (unsigned long long)s_carry;   // records in the new pool
-    totals[1] = 0;                             // groups (the compact pass adds them up)
+    newcount[p] = (int64_t)(end - base[(int64_t)p * row]) - ocount[p];
   }
 }
 
@@ -418,7 +429,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
       uint64_t h = fmix64(k);
       if (lds_find(T, k, h, kLdsProbes) >= 0) continue;
       uint32_t p = bucket_of(h, G.bucket_shift);
-      uint64_t at = base[(size_t)p * gridDim.x + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
+      uint64_t at = base[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
       uint64_t* rec = pool + at * G.rec_words;
       rec[0] = k;
       rec[1] = (uint64_t)(G.row_base + r);
@@ -436,7 +447,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
     const uint64_t* src = part + (size_t)s * G.rec_words;
     if ((long long)src[1] == LLONG_MAX) continue;
     uint32_t p = bucket_of(fmix64(src[0]), G.bucket_shift);
-    uint64_t at = base[(size_t)p * gridDim.x + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
+    uint64_t at = base[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
     uint64_t* rec = pool + at * G.rec_words;
     for (int w = 0; w < G.rec_words; ++w) rec[w] = src[w];
   }
@@ -697,8 +708,8 @@ struct fpdb_group_table {
   int64_t* ocount = nullptr;         // nbuckets
   int64_t* nstart = nullptr;         // nbuckets
   int64_t* newcount = nullptr;       // nbuckets
-  uint64_t* base = nullptr;          // nbuckets x max_blocks
-  uint32_t* hist = nullptr;          // max_blocks x nbuckets
+  uint64_t* base = nullptr;          // nbuckets x (max_blocks + 1), bucket-major
+  uint32_t* hist = nullptr;          // nbuckets x (max_blocks + 1), bucket-major
   uint64_t* partial = nullptr;       // max_blocks x (local_slots + 1) records
   unsigned long long* totals = nullptr;   // [0] records of the new pool, [1] groups
   int* flags = nullptr;              // [0] bucket overflow, [1] key outside its packed range
@@ -836,8 +847,8 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->ocount));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->nstart));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->newcount));
-  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 8, reinterpret_cast<void**>(&t->base));
-  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 4, reinterpret_cast<void**>(&t->hist));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * (t->max_blocks + 1) * 8, reinterpret_cast<void**>(&t->base));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * (t->max_blocks + 1) * 4, reinterpret_cast<void**>(&t->hist));
   if (s == FPDB_OK)
     s = gb_alloc(t, (uint64_t)t->max_blocks * (t->local_slots + 1) * t->rec_words * 8,
                  reinterpret_cast<void**>(&t->partial));

@@ -326,29 +326,75 @@ __global__ void star_init_kernel(StarDev P) {
   }
 }
 
-// Ranks the groups by first appearance and writes the result columns.
-__global__ void star_emit_kernel(StarDev P, int32_t* out_g0, int32_t* out_g1, int64_t* out_cnt,
-                                 int64_t* out_sum, unsigned long long* out_n, int64_t max_groups) {
+// Ranks the groups by first appearance and writes the result columns.  One
+// workgroup: the used slots are collected in LDS and sorted by (first row,
+// slot); beyond kStarSortGroups groups each group counts the groups that came
+// first instead.
+constexpr int kStarSortGroups = 8192;
+
+__device__ __forceinline__ void star_write_group(const StarDev& P, int64_t s, int64_t cap, int64_t rank,
+                                                 int32_t* out_g0, int32_t* out_g1, int64_t* out_cnt,
+                                                 int64_t* out_sum, int64_t max_groups) {
+  if (rank >= max_groups) {   // more groups than the result buffers hold
+    __hip_atomic_store(P.state + 32, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  uint64_t k = (s == cap - 1) ? kStarEmptyKey : P.g_keys[s];
+  out_g0[rank] = (int32_t)(uint32_t)(k & 0xFFFFFFFFULL);
+  out_g1[rank] = (int32_t)(uint32_t)(k >> 32);
+  out_cnt[rank] = (int64_t)P.g_cnt[s];
+  out_sum[rank] = (int64_t)P.g_sum[s];
+}
+
+__global__ __launch_bounds__(1024) void star_emit_kernel(StarDev P, int32_t* out_g0, int32_t* out_g1,
+                                                         int64_t* out_cnt, int64_t* out_sum,
+                                                         unsigned long long* out_n, int64_t max_groups) {
+  __shared__ unsigned long long skey[kStarSortGroups];
+  __shared__ unsigned s_n;
   const int64_t cap = (int64_t)P.g_mask + 2;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < cap; s += (int64_t)gridDim.x * blockDim.x) {
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int64_t s = threadIdx.x; s < cap; s += blockDim.x) {
     long long f = P.g_first[s];
     if (f == LLONG_MAX) continue;
-    int64_t rank = 0;
-    for (int64_t t = 0; t < cap; ++t) {
-      long long o = P.g_first[t];
-      rank += (o < f) ? 1 : 0;
-    }
-    if (rank >= max_groups) {   // more groups than the result buffers hold
-      __hip_atomic_store(P.state + 32, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      continue;
-    }
-    uint64_t k = (s == cap - 1) ? kStarEmptyKey : P.g_keys[s];
-    out_g0[rank] = (int32_t)(uint32_t)(k & 0xFFFFFFFFULL);
-    out_g1[rank] = (int32_t)(uint32_t)(k >> 32);
-    out_cnt[rank] = (int64_t)P.g_cnt[s];
-    out_sum[rank] = (int64_t)P.g_sum[s];
-    atomicAdd(out_n, 1ULL);
+    unsigned i = atomicAdd(&s_n, 1u);
+    if (i < kStarSortGroups) skey[i] = ((unsigned long long)f << 22) | (unsigned long long)s;
   }
+  __syncthreads();
+  const unsigned n = s_n;
+  if (n > kStarSortGroups) {
+    for (int64_t s = threadIdx.x; s < cap; s += blockDim.x) {
+      long long f = P.g_first[s];
+      if (f == LLONG_MAX) continue;
+      int64_t rank = 0;
+      for (int64_t t = 0; t < cap; ++t) rank += (P.g_first[t] < f) ? 1 : 0;
+      star_write_group(P, s, cap, rank, out_g0, out_g1, out_cnt, out_sum, max_groups);
+    }
+  } else {
+    unsigned np = 1;
+    while (np < n) np <<= 1;
+    for (unsigned i = n + threadIdx.x; i < np; i += blockDim.x) skey[i] = ~0ULL;
+    __syncthreads();
+    for (unsigned k = 2; k <= np; k <<= 1) {
+      for (unsigned j = k >> 1; j > 0; j >>= 1) {
+        for (unsigned i = threadIdx.x; i < np; i += blockDim.x) {
+          unsigned l = i ^ j;
+          if (l > i) {
+            unsigned long long a = skey[i], b = skey[l];
+            if ((a > b) == ((i & k) == 0)) {
+              skey[i] = b;
+              skey[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x)
+      star_write_group(P, (int64_t)(skey[i] & ((1ULL << 22) - 1)), cap, i, out_g0, out_g1, out_cnt, out_sum,
+                       max_groups);
+  }
+  if (threadIdx.x == 0) *out_n = n < (unsigned long long)max_groups ? n : (unsigned long long)max_groups;
 }
 
 }  // namespace fpdb
@@ -640,9 +686,8 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
     hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(256), 0, c->stream, s->dev);
   }
   FPDB_HIP(hipGetLastError());
-  FPDB_TRY(launch_fill(c, s->out_n, 0, 8));
-  hipLaunchKernelGGL(star_emit_kernel, dim3(std::max<int64_t>(1, (s->dev.g_mask + 1 + 255) / 256)), dim3(256), 0,
-                     c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt, s->out_sum, s->out_n, s->max_groups);
+  hipLaunchKernelGGL(star_emit_kernel, dim3(1), dim3(1024), 0, c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt,
+                     s->out_sum, s->out_n, s->max_groups);
   FPDB_HIP(hipGetLastError());
   return FPDB_OK;
 }
