@@ -96,7 +96,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void blocked_find_kernel(const uint64_t* __restrict__ blocks,
                                                            uint64_t nb_mask, ColSet keys, int64_t n,
                                                            uint8_t* __restrict__ bitmap,
-                                                           int64_t* __restrict__ num_selected) {
+                                                           unsigned long long* partials) {
   __shared__ uint64_t s_masks[kMaskNum];
   load_masks_lds(s_masks);
   const int lane = threadIdx.x & 63;
@@ -126,9 +126,7 @@ __global__ __launch_bounds__(256) void blocked_find_kernel(const uint64_t* __res
       }
     }
   }
-  if (num_selected && lane == 0 && count)
-    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(num_selected),
-                           (unsigned long long)count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (partials) block_count_store(partials, lane == 0 ? count : 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -228,7 +226,7 @@ __global__ __launch_bounds__(256) void legacy_find_kernel(LegacyParams P, int64_
                                                           const uint64_t* __restrict__ words,
                                                           ColSet keys, int64_t n,
                                                           uint8_t* __restrict__ bitmap,
-                                                          int64_t* __restrict__ num_selected) {
+                                                          unsigned long long* partials) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -242,9 +240,7 @@ __global__ __launch_bounds__(256) void legacy_find_kernel(LegacyParams P, int64_
     write_bitmap_word(bitmap, c, word, n, lane);
     count += __popcll(word);
   }
-  if (num_selected && lane == 0 && count)
-    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(num_selected),
-                           (unsigned long long)count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (partials) block_count_store(partials, lane == 0 ? count : 0);
 }
 
 // column <op> constant -> selection bitmap (nulls never selected), optionally
@@ -252,7 +248,7 @@ __global__ __launch_bounds__(256) void legacy_find_kernel(LegacyParams P, int64_
 // writes only its own 64-row words, so the existing bitmap may be the output.
 __global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, bool for_or, int64_t ci, double cf,
                                                       const uint8_t* and_bm, int64_t n, uint8_t* bitmap,
-                                                      int64_t* __restrict__ num_selected) {
+                                                      unsigned long long* partials) {
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -294,9 +290,7 @@ __global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, bo
     write_bitmap_word(bitmap, c, word, n, lane);
     count += __popcll(word);
   }
-  if (num_selected && lane == 0 && count)
-    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(num_selected), (unsigned long long)count,
-                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (partials) block_count_store(partials, lane == 0 ? count : 0);
 }
 
 // blocked-filter predicate for the fused find + compaction
@@ -399,18 +393,19 @@ fpdb_status fpdb_bloom_blocked_find(fpdb_context* ctx, const uint64_t* d_blocks,
   Context* c = as_ctx(ctx);
   ColSet k;
   FPDB_TRY(make_colset(keys, num_keys, num_rows, &k, "bloom find"));
-  if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
-  if (num_rows == 0) return FPDB_OK;
+  if (num_rows == 0) return d_num_selected ? launch_fill(c, d_num_selected, 0, 8) : FPDB_OK;
   uint64_t nbm = (1ULL << log_num_blocks) - 1;
   int64_t nchunks = ceil_div(num_rows, 64);
   dim3 g(grid_for(c, nchunks * 64, 256 * kFindUnroll, 16)), b(256);
+  unsigned long long* parts = d_num_selected ? c->d_partials : nullptr;
   ProfScope ps(c, "bloom_blocked_find");
   switch (key_mode(k)) {
-    case 1: hipLaunchKernelGGL(blocked_find_kernel<1>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, d_num_selected); break;
-    case 2: hipLaunchKernelGGL(blocked_find_kernel<2>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, d_num_selected); break;
-    default: hipLaunchKernelGGL(blocked_find_kernel<0>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, d_num_selected); break;
+    case 1: hipLaunchKernelGGL(blocked_find_kernel<1>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, parts); break;
+    case 2: hipLaunchKernelGGL(blocked_find_kernel<2>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, parts); break;
+    default: hipLaunchKernelGGL(blocked_find_kernel<0>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, parts); break;
   }
   FPDB_HIP(hipGetLastError());
+  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, (int)g.x, d_num_selected));
   return FPDB_OK;
 }
 
@@ -522,13 +517,14 @@ fpdb_status fpdb_bloom_legacy_find(fpdb_context* ctx, const fpdb_universal_hash*
   FPDB_CHECK(num_bits >= 0, FPDB_ERR_INVALID, "legacy bloom find: negative num_bits");
   LegacyParams P;
   FPDB_TRY(make_legacy_params(hash_functions, k, num_bits, &P));
-  if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
-  if (num_rows == 0) return FPDB_OK;
+  if (num_rows == 0) return d_num_selected ? launch_fill(c, d_num_selected, 0, 8) : FPDB_OK;
   ProfScope ps(c, "bloom_legacy_find");
-  hipLaunchKernelGGL(legacy_find_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, P,
-                     capacity, reinterpret_cast<const uint64_t*>(d_words), kc, num_rows, d_bitmap,
-                     d_num_selected);
+  const int grid = grid_for(c, num_rows, 256, 16);
+  hipLaunchKernelGGL(legacy_find_kernel, dim3(grid), dim3(256), 0, c->stream, P, capacity,
+                     reinterpret_cast<const uint64_t*>(d_words), kc, num_rows, d_bitmap,
+                     d_num_selected ? c->d_partials : nullptr);
   FPDB_HIP(hipGetLastError());
+  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, grid, d_num_selected));
   return FPDB_OK;
 }
 
@@ -543,12 +539,13 @@ fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, in
   Context* c = as_ctx(ctx);
   ColSet cs;
   FPDB_TRY(make_colset(column, 1, num_rows, &cs, "compare"));
-  if (d_num_selected) FPDB_TRY(launch_fill(c, d_num_selected, 0, 8));
-  if (num_rows == 0) return FPDB_OK;
+  if (num_rows == 0) return d_num_selected ? launch_fill(c, d_num_selected, 0, 8) : FPDB_OK;
   ProfScope ps(c, "compare");
-  hipLaunchKernelGGL(compare_kernel, dim3(grid_for(c, num_rows, 256, 16)), dim3(256), 0, c->stream, cs, op, for_or,
-                     int_constant, float_constant, d_and_bitmap, num_rows, d_bitmap, d_num_selected);
+  const int grid = grid_for(c, num_rows, 256, 16);
+  hipLaunchKernelGGL(compare_kernel, dim3(grid), dim3(256), 0, c->stream, cs, op, for_or, int_constant,
+                     float_constant, d_and_bitmap, num_rows, d_bitmap, d_num_selected ? c->d_partials : nullptr);
   FPDB_HIP(hipGetLastError());
+  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, grid, d_num_selected));
   return FPDB_OK;
 }
 

@@ -205,6 +205,28 @@ __global__ void lb_advance_kernel(LbState* st, uint64_t* status, int64_t status_
   if (threadIdx.x == 0) st->epoch = e;
 }
 
+__global__ __launch_bounds__(1024) void sum_partials_kernel(const unsigned long long* __restrict__ partials,
+                                                             int n, int64_t* dst) {
+  __shared__ unsigned long long s_w[16];
+  unsigned long long v = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w];
+    *dst = (int64_t)t;
+  }
+}
+
+fpdb_status launch_sum_partials(Context* c, int blocks, int64_t* dst) {
+  FPDB_CHECK(blocks >= 1 && blocks <= kMaxPartials, FPDB_ERR_INTERNAL, "too many workgroups for the partial counts");
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_partials, blocks, dst);
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
 fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args) {
   LookbackScratch& lb = c->lb;
   if (!lb.state) {
@@ -272,6 +294,7 @@ fpdb_status fpdb_context_create(int device, uint64_t hbm_budget_bytes, void* str
   FPDB_HIP(hipEventCreate(&c->ev_start));
   FPDB_HIP(hipEventCreate(&c->ev_stop));
   fpdb_status s = ctx_alloc(c, 64, reinterpret_cast<void**>(&c->d_counter));
+  if (s == FPDB_OK) s = ctx_alloc(c, (uint64_t)kMaxPartials * 8, reinterpret_cast<void**>(&c->d_partials));
   if (s == FPDB_OK) s = launch_fill(c, c->d_counter, 0, 64);
   if (s != FPDB_OK) { delete c; return s; }
   FPDB_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->h_counter), 64, hipHostMallocDefault));

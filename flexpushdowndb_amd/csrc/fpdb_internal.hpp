@@ -96,6 +96,7 @@ struct Context {
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
   LookbackScratch lb;
   int64_t* d_counter = nullptr;     // small device scratch for counts (8 words)
+  unsigned long long* d_partials = nullptr;   // per-workgroup partial counts (kMaxPartials)
   int64_t* h_counter = nullptr;     // pinned host mirror
 };
 
@@ -127,6 +128,13 @@ struct ProfScope {
 };
 // Ensures look-back scratch for `tiles` tiles and returns the launch arguments.
 fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args);
+
+// Per-workgroup partial counts.  A same-address device atomic serializes at
+// about 12 ns each on gfx950, so kernels that count (selected rows, inserted
+// keys) store one partial per workgroup and a one-workgroup kernel adds the
+// partials into the destination (stored, not added).
+constexpr int kMaxPartials = 1 << 14;
+fpdb_status launch_sum_partials(Context* c, int blocks, int64_t* dst);
 fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes);
 // Kernel-side failure flag (look-back spin exhausted, table overflow); zero
 // unless a kernel reported a failure.
@@ -314,6 +322,18 @@ __device__ __forceinline__ void lb_begin(const LbArgs& lb, int64_t* s_tile, uint
 // End of a look-back launch (nothing to do: the next launch's advance kernel
 // moves the epoch on).
 __device__ __forceinline__ void lb_end(const LbArgs&) {}
+
+// Stores this workgroup's count (the sum of every thread's `mine`) into
+// partials[blockIdx.x]; every thread of the workgroup must call it.
+__device__ __forceinline__ void block_count_store(unsigned long long* partials, unsigned long long mine) {
+  __shared__ unsigned long long s_block_count;
+  if (threadIdx.x == 0) s_block_count = 0;
+  __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(&s_block_count, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = s_block_count;
+}
 
 // Wave-wide inclusive prefix sum of a 32-bit value.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {

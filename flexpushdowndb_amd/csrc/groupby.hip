@@ -644,16 +644,45 @@ __global__ void gb_emit_kernel(const uint64_t* __restrict__ pool, const int64_t*
 
 // Minimum and maximum of the valid values of an integer column, in an
 // order-preserving unsigned encoding (signed values with the sign bit flipped)
-// so one unsigned atomic min/max serves both.  out: [min, max, valid count].
-__global__ void range_init_kernel(unsigned long long* out) {
+// so one unsigned min/max serves both.  Each workgroup stores its partial
+// (min, max, count) at part[b], part[B + b], part[2B + b]; one workgroup then
+// folds them into out = [min, max, valid count].
+__global__ __launch_bounds__(1024) void range_fold_kernel(const unsigned long long* __restrict__ part, int blocks,
+                                                          unsigned long long* out) {
+  __shared__ unsigned long long s_lo[16], s_hi[16], s_c[16];
+  unsigned long long lo = ~0ULL, hi = 0ULL, cnt = 0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    if (!part[2 * blocks + b]) continue;
+    lo = part[b] < lo ? part[b] : lo;
+    hi = part[blocks + b] > hi ? part[blocks + b] : hi;
+    cnt += part[2 * blocks + b];
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    unsigned long long olo = __shfl_xor(lo, d), ohi = __shfl_xor(hi, d), oc = __shfl_xor(cnt, d);
+    lo = olo < lo ? olo : lo;
+    hi = ohi > hi ? ohi : hi;
+    cnt += oc;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+    s_c[threadIdx.x >> 6] = cnt;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    out[0] = ~0ULL;
-    out[1] = 0ULL;
-    out[2] = 0ULL;
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = s_lo[w] < lo ? s_lo[w] : lo;
+      hi = s_hi[w] > hi ? s_hi[w] : hi;
+      cnt += s_c[w];
+    }
+    out[0] = lo;
+    out[1] = hi;
+    out[2] = cnt;
   }
 }
 
-__global__ __launch_bounds__(256) void range_kernel(ColSet col, int64_t n, unsigned long long* out) {
+__global__ __launch_bounds__(256) void range_kernel(ColSet col, int64_t n, unsigned long long* part) {
+  __shared__ unsigned long long s_lo[4], s_hi[4], s_c[4];
   const bool sgn = type_is_signed_int_d(col.type[0]);
   unsigned long long lo = ~0ULL, hi = 0ULL, cnt = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
@@ -671,10 +700,21 @@ __global__ __launch_bounds__(256) void range_kernel(ColSet col, int64_t n, unsig
     hi = ohi > hi ? ohi : hi;
     cnt += oc;
   }
-  if ((threadIdx.x & 63) == 0 && cnt) {
-    atomicMin(out + 0, lo);
-    atomicMax(out + 1, hi);
-    atomicAdd(out + 2, cnt);
+  if ((threadIdx.x & 63) == 0) {
+    s_lo[threadIdx.x >> 6] = lo;
+    s_hi[threadIdx.x >> 6] = hi;
+    s_c[threadIdx.x >> 6] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      lo = s_lo[w] < lo ? s_lo[w] : lo;
+      hi = s_hi[w] > hi ? s_hi[w] : hi;
+      cnt += s_c[w];
+    }
+    part[blockIdx.x] = lo;
+    part[gridDim.x + blockIdx.x] = hi;
+    part[2 * gridDim.x + blockIdx.x] = cnt;
   }
 }
 
@@ -945,11 +985,14 @@ fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, in
   unsigned long long* d = nullptr;
   FPDB_TRY(ctx_alloc(c, 24, reinterpret_cast<void**>(&d)));
   unsigned long long h[3] = {~0ULL, 0ULL, 0ULL};
-  hipLaunchKernelGGL(range_init_kernel, dim3(1), dim3(64), 0, c->stream, d);
-  if (num_rows > 0)
-    hipLaunchKernelGGL(range_kernel, dim3(grid_for(c, num_rows, 256, 4)), dim3(256), 0, c->stream, cs, num_rows, d);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(h, d, 24, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipSuccess;
+  if (num_rows > 0) {
+    const int grid = std::min(grid_for(c, num_rows, 256, 4), kMaxPartials / 3);
+    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, c->stream, cs, num_rows, c->d_partials);
+    hipLaunchKernelGGL(range_fold_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_partials, grid, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && num_rows > 0) e = hipMemcpyAsync(h, d, 24, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   ctx_free(c, d);
   FPDB_CHECK(e == hipSuccess, FPDB_ERR_HIP, hipGetErrorString(e));

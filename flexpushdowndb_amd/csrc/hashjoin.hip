@@ -85,7 +85,7 @@ template <int KM>
 __global__ __launch_bounds__(256) void hj_build_kernel(Slot* __restrict__ slots, uint64_t mask, KeySpec k,
                                                        int64_t n, const uint8_t* __restrict__ payload,
                                                        int64_t payload_off, int* err,
-                                                       unsigned long long* inserted) {
+                                                       unsigned long long* partials) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   unsigned long long mine = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
@@ -106,9 +106,7 @@ __global__ __launch_bounds__(256) void hj_build_kernel(Slot* __restrict__ slots,
     }
     if (!done) *err = 1;
   }
-  for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
-  if ((threadIdx.x & 63) == 0 && mine)
-    __hip_atomic_fetch_add(inserted, mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_count_store(partials, mine);   // inserted (non-null) rows
 }
 
 // Flags a table whose inserted keys are not all distinct.
@@ -587,7 +585,7 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
   if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
   // meta: [0] insertion failure flag, [1] duplicate-key flag, [2] inserted rows
   int* d_err = reinterpret_cast<int*>(t->meta);
-  auto* d_ins = reinterpret_cast<unsigned long long*>(t->meta + 2);
+  auto* d_ins = c->d_partials;
   const uint8_t* pv = payload ? static_cast<const uint8_t*>(payload->values) : nullptr;
   int64_t poff = payload ? payload->offset : 0;
   if (num_rows > 0) {
@@ -598,6 +596,10 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
       case 2: hipLaunchKernelGGL(hj_build_kernel<2>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
       default: hipLaunchKernelGGL(hj_build_kernel<0>, g, b, 0, c->stream, t->slots, (uint64_t)cap - 1, k, num_rows, pv, poff, d_err, d_ins); break;
     }
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) { fpdb_hj_table_destroy(t); return fail(FPDB_ERR_HIP, hipGetErrorString(le)); }
+    s = launch_sum_partials(c, (int)g.x, t->meta + 2);
+    if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
     // the duplicate-key flag, read by every probe (a table of distinct keys
     // lets a probe stop at its first match)
     hipLaunchKernelGGL(hj_unique_kernel, dim3(grid_for(c, cap, 256)), dim3(256), 0, c->stream, t->slots,

@@ -240,9 +240,16 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
       D.kept[r] = k ? 1 : 0;
       kept += k ? 1 : 0;
     }
+    // one device atomic per workgroup (same-address atomics serialize)
+    __shared__ unsigned long long s_kept;
+    if (threadIdx.x == 0) s_kept = 0;
+    __syncthreads();
     for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
-    if ((threadIdx.x & 63) == 0 && kept)
-      __hip_atomic_fetch_add(P.state + 1 + d, kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_kept, kept);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_kept)
+      __hip_atomic_fetch_add(P.state + 1 + d, s_kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
     uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
     for (int64_t s = gtid; s <= (int64_t)D.ht_mask; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
   }
