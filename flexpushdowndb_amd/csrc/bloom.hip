@@ -87,46 +87,115 @@ __global__ __launch_bounds__(256) void blocked_insert_kernel(ColSet keys, int64_
 }
 
 // ---------------------------------------------------------------------------
-// blocked filter: find -> packed bitmap
-// Each wave owns 64-row chunks; UNROLL chunks are in flight per wave.
-// ---------------------------------------------------------------------------
-constexpr int kFindUnroll = 4;
+// blocked filter: find with (part of) the filter in LDS
+//
+// A key's block index is bits 16..31 of its hash, so at most 2^16 blocks
+// (512 KiB) are ever read.  Random 8-byte reads from the L2 bound a find whose
+// filter lives there (≈ 220 G requests/s on this part), so each workgroup
+// copies one slice of up to 2^14 blocks (128 KiB) into LDS: keys whose block
+// lies in that slice are tested in LDS, the others in L2.  A filter of up to
+// 2^14 blocks is read from LDS only; with S slices, workgroup b holds slice
+// b % S, so 1/S of the random reads leave the L2.
+constexpr int kSliceLog = 14;
+constexpr int kSliceThreads = 1024;
+constexpr int kSliceUnroll = 8;
+
+inline int reachable_log_blocks(int log_num_blocks) { return log_num_blocks < 16 ? log_num_blocks : 16; }
+inline int num_slices(int log_num_blocks) {
+  int r = reachable_log_blocks(log_num_blocks);
+  return r <= kSliceLog ? 1 : 1 << (r - kSliceLog);
+}
 
 template <int MODE>
-__global__ __launch_bounds__(256) void blocked_find_kernel(const uint64_t* __restrict__ blocks,
-                                                           uint64_t nb_mask, ColSet keys, int64_t n,
-                                                           uint8_t* __restrict__ bitmap,
-                                                           unsigned long long* partials) {
-  __shared__ uint64_t s_masks[kMaskNum];
-  load_masks_lds(s_masks);
+__global__ __launch_bounds__(kSliceThreads) void blocked_find_lds_kernel(
+    const uint64_t* __restrict__ blocks, uint64_t nb_mask, int slices, ColSet keys, int64_t n,
+    int64_t words_per_block, uint8_t* __restrict__ bitmap, unsigned long long* partials) {
+  extern __shared__ uint64_t lds[];
+  uint64_t* s_masks = lds;
+  uint64_t* s_blk = lds + kMaskNum;
+  const int q = (int)(blockIdx.x % slices);
+  const uint64_t reach = (nb_mask + 1) < (1ULL << 16) ? (nb_mask + 1) : (1ULL << 16);
+  const uint64_t lo = (uint64_t)q << kSliceLog;
+  const uint64_t cnt = reach - lo < (1ULL << kSliceLog) ? reach - lo : (1ULL << kSliceLog);
+  for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
+  for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) s_blk[i] = blocks[lo + i];
+  __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const int64_t nchunks = (n + 63) >> 6;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int64_t nwords = (n + 63) >> 6;
+  const int64_t w0 = (int64_t)blockIdx.x * words_per_block;
+  const int64_t w1 = w0 + words_per_block < nwords ? w0 + words_per_block : nwords;
   uint64_t count = 0;
-  for (int64_t c0 = wave * kFindUnroll; c0 < nchunks; c0 += nwaves * kFindUnroll) {
-    uint32_t h[kFindUnroll];
-    bool in[kFindUnroll];
+  // Loads are issued unconditionally (rows past the end re-read the last row,
+  // blocks held in LDS read one shared dummy block) so the compiler does not
+  // wrap each in a branch that waits for it: all of a step's loads are in
+  // flight together.
+  const int64_t last_row = n - 1;
+  for (int64_t wb = w0 + (int64_t)wave * kSliceUnroll; wb < w1; wb += (int64_t)nwaves * kSliceUnroll) {
+    uint32_t h[kSliceUnroll];
+    bool in[kSliceUnroll];
 #pragma unroll
-    for (int u = 0; u < kFindUnroll; ++u) {
-      int64_t r = (c0 + u) * 64 + lane;
-      in[u] = r < n;
-      h[u] = in[u] ? key_hash<MODE>(keys, r) : 0;
+    for (int u = 0; u < kSliceUnroll; ++u) {
+      int64_t r = (wb + u) * 64 + lane;
+      in[u] = wb + u < w1 && r < n;
+      h[u] = key_hash<MODE>(keys, in[u] ? r : last_row);
     }
-    uint64_t b[kFindUnroll];
+    uint64_t blk[kSliceUnroll];
+    if (slices == 1) {
 #pragma unroll
-    for (int u = 0; u < kFindUnroll; ++u) b[u] = in[u] ? blocks[bb_block(h[u], nb_mask)] : 0;
+      for (int u = 0; u < kSliceUnroll; ++u) {
+        uint64_t b = bb_block(h[u], nb_mask) - lo;
+        blk[u] = s_blk[b < cnt ? b : 0];
+      }
+    } else {
 #pragma unroll
-    for (int u = 0; u < kFindUnroll; ++u) {
+      for (int u = 0; u < kSliceUnroll; ++u) {
+        uint64_t b = bb_block(h[u], nb_mask);
+        bool local = b - lo < cnt;
+        uint64_t g = blocks[local ? lo : b];
+        uint64_t l = s_blk[local ? b - lo : 0];
+        blk[u] = local ? l : g;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSliceUnroll; ++u) {
       uint64_t m = bb_mask(s_masks, h[u]);
-      uint64_t word = __ballot(in[u] && (b[u] & m) == m);
-      if (c0 + u < nchunks) {
-        write_bitmap_word(bitmap, c0 + u, word, n, lane);
+      uint64_t word = __ballot(in[u] && (blk[u] & m) == m);
+      if (wb + u < w1) {
+        write_bitmap_word(bitmap, wb + u, word, n, lane);
         count += __popcll(word);
       }
     }
   }
   if (partials) block_count_store(partials, lane == 0 ? count : 0);
+}
+
+// Find of n rows into d_bitmap (and the selected count into d_num_selected
+// when given).
+static fpdb_status lds_find(Context* c, const uint64_t* d_blocks, int log_num_blocks, const ColSet& k, int64_t n,
+                            uint8_t* d_bitmap, int64_t* d_num_selected) {
+  const int S = num_slices(log_num_blocks);
+  const int64_t nwords = ceil_div(n, 64);
+  const size_t lds = (size_t)(kMaskNum + (1 << std::min(kSliceLog, reachable_log_blocks(log_num_blocks)))) * 8;
+  // two workgroups per CU when their LDS allows, contiguous word ranges
+  const int per_cu = lds * 2 <= (size_t)c->max_lds_per_block ? 2 : 1;
+  int64_t grid = std::max<int64_t>(1, (int64_t)std::max(c->num_cus, 1) * per_cu);
+  int64_t words_per_block = std::max<int64_t>(kSliceUnroll, ceil_div(nwords, grid));
+  grid = ceil_div(nwords, words_per_block);
+  FPDB_CHECK(grid <= kMaxPartials, FPDB_ERR_INTERNAL, "bloom find: too many workgroups");
+  const uint64_t nbm = (1ULL << log_num_blocks) - 1;
+  unsigned long long* parts = d_num_selected ? c->d_partials : nullptr;
+  ProfScope ps(c, "bloom_blocked_find");
+  dim3 g((unsigned)grid), b(kSliceThreads);
+  switch (key_mode(k)) {
+    case 1: hipLaunchKernelGGL(blocked_find_lds_kernel<1>, g, b, lds, c->stream, d_blocks, nbm, S, k, n, words_per_block, d_bitmap, parts); break;
+    case 2: hipLaunchKernelGGL(blocked_find_lds_kernel<2>, g, b, lds, c->stream, d_blocks, nbm, S, k, n, words_per_block, d_bitmap, parts); break;
+    default: hipLaunchKernelGGL(blocked_find_lds_kernel<0>, g, b, lds, c->stream, d_blocks, nbm, S, k, n, words_per_block, d_bitmap, parts); break;
+  }
+  FPDB_HIP(hipGetLastError());
+  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, (int)grid, d_num_selected));
+  return FPDB_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -294,17 +363,6 @@ __global__ __launch_bounds__(256) void compare_kernel(ColSet col, int32_t op, bo
 }
 
 // blocked-filter predicate for the fused find + compaction
-template <int MODE>
-struct PredBlocked {
-  const uint64_t* blocks;
-  uint64_t nb_mask;
-  ColSet keys;
-  __device__ __forceinline__ bool operator()(int64_t r, const uint64_t* s_masks) const {
-    uint32_t h = key_hash<MODE>(keys, r);
-    uint64_t m = bb_mask(s_masks, h);
-    return (blocks[bb_block(h, nb_mask)] & m) == m;
-  }
-};
 
 static fpdb_status check_outputs_match(const ColSet& in, const OutColSet& out) {
   FPDB_CHECK(in.n == out.n, FPDB_ERR_INVALID, "input and output column counts differ");
@@ -394,19 +452,7 @@ fpdb_status fpdb_bloom_blocked_find(fpdb_context* ctx, const uint64_t* d_blocks,
   ColSet k;
   FPDB_TRY(make_colset(keys, num_keys, num_rows, &k, "bloom find"));
   if (num_rows == 0) return d_num_selected ? launch_fill(c, d_num_selected, 0, 8) : FPDB_OK;
-  uint64_t nbm = (1ULL << log_num_blocks) - 1;
-  int64_t nchunks = ceil_div(num_rows, 64);
-  dim3 g(grid_for(c, nchunks * 64, 256 * kFindUnroll, 16)), b(256);
-  unsigned long long* parts = d_num_selected ? c->d_partials : nullptr;
-  ProfScope ps(c, "bloom_blocked_find");
-  switch (key_mode(k)) {
-    case 1: hipLaunchKernelGGL(blocked_find_kernel<1>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, parts); break;
-    case 2: hipLaunchKernelGGL(blocked_find_kernel<2>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, parts); break;
-    default: hipLaunchKernelGGL(blocked_find_kernel<0>, g, b, 0, c->stream, d_blocks, nbm, k, num_rows, d_bitmap, parts); break;
-  }
-  FPDB_HIP(hipGetLastError());
-  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, (int)g.x, d_num_selected));
-  return FPDB_OK;
+  return lds_find(c, d_blocks, log_num_blocks, k, num_rows, d_bitmap, d_num_selected);
 }
 
 fpdb_status fpdb_bloom_legacy_params(int64_t capacity, double p, int64_t* k, int64_t* m) {
@@ -583,12 +629,16 @@ fpdb_status fpdb_bloom_blocked_filter(fpdb_context* ctx, const uint64_t* d_block
   }
   ColSet kc;
   FPDB_TRY(make_colset(kcols.data(), num_keys, num_rows, &kc, "bloom filter keys"));
-  uint64_t nbm = (1ULL << log_num_blocks) - 1;
-  switch (key_mode(kc)) {
-    case 1: return run_compact<PredBlocked<1>, true>(c, PredBlocked<1>{d_blocks, nbm, kc}, ic, oc, num_rows, d_bitmap, nullptr, num_out, nullptr, "bloom_blocked_filter");
-    case 2: return run_compact<PredBlocked<2>, true>(c, PredBlocked<2>{d_blocks, nbm, kc}, ic, oc, num_rows, d_bitmap, nullptr, num_out, nullptr, "bloom_blocked_filter");
-    default: return run_compact<PredBlocked<0>, true>(c, PredBlocked<0>{d_blocks, nbm, kc}, ic, oc, num_rows, d_bitmap, nullptr, num_out, nullptr, "bloom_blocked_filter");
-  }
+  if (num_rows == 0) return run_compact<PredBitmap, false>(c, PredBitmap{d_bitmap}, ic, oc, 0, nullptr, nullptr, num_out);
+  // find with the filter in LDS, then an order-preserving compaction by the bitmap
+  uint8_t* bm = d_bitmap;
+  if (!bm) FPDB_TRY(ctx_alloc(c, (uint64_t)ceil_div(num_rows, 64) * 8, reinterpret_cast<void**>(&bm)));
+  fpdb_status st = lds_find(c, d_blocks, log_num_blocks, kc, num_rows, bm, nullptr);
+  if (st == FPDB_OK)
+    st = run_compact<PredBitmap, false>(c, PredBitmap{bm}, ic, oc, num_rows, nullptr, nullptr, num_out, nullptr,
+                                        "bloom_blocked_filter");
+  if (!d_bitmap) ctx_free(c, bm);
+  return st;
 }
 
 }  // extern "C"

@@ -53,18 +53,30 @@ __device__ __forceinline__ void copy_value(const ColSet& in, const OutColSet& ou
                           1u << (di & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Selection predicates of the compaction: the 64-bit selection word of rows
+// [64 * chunk, 64 * chunk + 64), zero past row n (every lane gets the word).
 struct PredBitmap {
-  const uint8_t* bm;
-  __device__ __forceinline__ bool operator()(int64_t r, const uint64_t*) const {
-    return bitmap_bit(bm, r);
+  const uint8_t* bm;   // LSB-first, ceil(n / 8) bytes
+  __device__ __forceinline__ uint64_t word(int64_t chunk, int lane, int64_t n) const {
+    int64_t row0 = chunk * 64;
+    if (row0 + 64 <= n && (reinterpret_cast<uintptr_t>(bm) & 7) == 0)
+      return reinterpret_cast<const uint64_t*>(bm)[chunk];   // one broadcast load
+    if (row0 >= n) return 0;
+    int nb = row0 + 64 <= n ? 8 : (int)((n - row0 + 7) / 8);
+    uint64_t w = 0;
+    for (int i = 0; i < nb; ++i) w |= (uint64_t)bm[chunk * 8 + i] << (8 * i);
+    int rem = (int)(n - row0);
+    return w & (rem >= 64 ? ~0ULL : ((1ULL << rem) - 1));
   }
 };
 
 struct PredByteFlag {
   const uint8_t* flags;
   uint8_t want;
-  __device__ __forceinline__ bool operator()(int64_t r, const uint64_t*) const {
-    return (flags[r] != 0) == (want != 0);
+  __device__ __forceinline__ uint64_t word(int64_t chunk, int lane, int64_t n) const {
+    int64_t r = chunk * 64 + lane;
+    uint8_t f = flags[r < n ? r : n - 1];
+    return __ballot(r < n && ((f != 0) == (want != 0)));
   }
 };
 
@@ -94,12 +106,9 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
   uint64_t words[kCompactChunks];
   uint32_t cnt = 0;
 #pragma unroll
-  for (int c = 0; c < kCompactChunks; ++c) {
-    int64_t r = (wave_chunk0 + c) * 64 + lane;
-    bool sel = (r < n) && pred(r, s_masks);
-    words[c] = __ballot(sel);
-    cnt += __popcll(words[c]);
-  }
+  for (int c = 0; c < kCompactChunks; ++c) words[c] = pred.word(wave_chunk0 + c, lane, n);
+#pragma unroll
+  for (int c = 0; c < kCompactChunks; ++c) cnt += __popcll(words[c]);
   if (bitmap_out) {
 #pragma unroll
     for (int c = 0; c < kCompactChunks; ++c)
@@ -119,19 +128,49 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
     }
   }
   __syncthreads();
-  uint64_t pos = s_prefix;
-  for (int w = 0; w < wave; ++w) pos += s_wave_cnt[w];
+  uint64_t pos0 = s_prefix;
+  for (int w = 0; w < wave; ++w) pos0 += s_wave_cnt[w];
   const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  // Column by column, 8 chunks at a time: the 8 source values are loaded
+  // together (rows past the end read the last row) before the selected lanes
+  // store theirs, so no load waits in its own branch.
+  constexpr int G = 8;
+  static_assert(kCompactChunks % G == 0, "chunk groups");
+  const int64_t last = n - 1;
+  for (int col = 0; col < in.n; ++col) {
+    uint64_t pos = pos0;
 #pragma unroll
-  for (int c = 0; c < kCompactChunks; ++c) {
-    uint64_t wc = words[c];
-    if ((wc >> lane) & 1) {
-      int64_t src = (wave_chunk0 + c) * 64 + lane;
-      int64_t dst = (int64_t)(pos + __popcll(wc & lt_mask));
-      for (int col = 0; col < in.n; ++col) copy_value(in, out, col, src, dst);
-      if (index_out) index_out[dst] = src;
+    for (int c0 = 0; c0 < kCompactChunks; c0 += G) {
+      int64_t si[G];
+      uint64_t v[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        int64_t src = (wave_chunk0 + c0 + u) * 64 + lane;
+        si[u] = (src < n ? src : last) + in.offset[col];
+      }
+      load_zext_n<G>(in.values[col], in.width[col], si, v);
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        uint64_t wc = words[c0 + u];
+        if ((wc >> lane) & 1) {
+          int64_t di = (int64_t)(pos + __popcll(wc & lt_mask)) + out.offset[col];
+          store_w(out.values[col], out.width[col], di, v[u]);
+          if (out.validity[col] && valid_bit(in.validity[col], si[u]))
+            __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[col]) + (di >> 5), 1u << (di & 31),
+                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        pos += __popcll(wc);
+      }
     }
-    pos += __popcll(wc);
+  }
+  if (index_out) {
+    uint64_t pos = pos0;
+#pragma unroll
+    for (int c = 0; c < kCompactChunks; ++c) {
+      uint64_t wc = words[c];
+      if ((wc >> lane) & 1) index_out[pos + __popcll(wc & lt_mask)] = (wave_chunk0 + c) * 64 + lane;
+      pos += __popcll(wc);
+    }
   }
   lb_end(lb);
 }

@@ -205,6 +205,39 @@ __device__ __forceinline__ void store_w(uint8_t* base, int width, int64_t i, uin
   }
 }
 
+// Loads N values of one column at once (indices already clamped to valid
+// rows): the width switch sits outside the per-value loop, so the N loads are
+// issued back to back instead of each in a branch that waits for it.
+template <int N>
+__device__ __forceinline__ void load_zext_n(const uint8_t* base, int width, const int64_t (&idx)[N],
+                                            uint64_t (&out)[N]) {
+  switch (width) {
+    case 8: {
+      const uint64_t* p = reinterpret_cast<const uint64_t*>(base);
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] = p[idx[i]];
+      break;
+    }
+    case 4: {
+      const uint32_t* p = reinterpret_cast<const uint32_t*>(base);
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] = p[idx[i]];
+      break;
+    }
+    case 2: {
+      const uint16_t* p = reinterpret_cast<const uint16_t*>(base);
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] = p[idx[i]];
+      break;
+    }
+    default: {
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] = base[idx[i]];
+      break;
+    }
+  }
+}
+
 __device__ __forceinline__ bool valid_bit(const uint8_t* validity, int64_t i) {
   return validity == nullptr || ((validity[i >> 3] >> (i & 7)) & 1);
 }

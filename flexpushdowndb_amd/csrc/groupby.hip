@@ -135,6 +135,81 @@ __device__ __forceinline__ uint64_t acc_operand(const ColSet& vals, int32_t op, 
   return load_zext(vals.values[in], w, i);
 }
 
+__device__ __forceinline__ int64_t sext_w(uint64_t raw, int width) {
+  const int sh = 64 - 8 * width;
+  return (int64_t)(raw << sh) >> sh;
+}
+
+// Group keys of N rows (indices clamped to valid rows by the caller), every
+// key column's N values loaded together.
+template <int N>
+__device__ __forceinline__ void group_keys_n(const GKeySpec& k, const int64_t (&rc)[N], uint64_t (&key)[N],
+                                             bool* out_of_range) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) key[i] = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxGroupKeys; ++j) {
+    if (j >= k.n) break;
+    int64_t idx[N];
+    uint64_t raw[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) idx[i] = rc[i] + k.off[j];
+    load_zext_n<N>(k.v[j], k.w[j], idx, raw);
+    const uint64_t m = key_field_mask(k.bits[j]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      int64_t v = k.sgn[j] ? sext_w(raw[i], k.w[j]) : (int64_t)raw[i];
+      uint64_t d = (uint64_t)v - (uint64_t)k.bias[j];
+      if (k.check && d > m) *out_of_range = true;
+      key[i] |= (d & m) << k.shift[j];
+    }
+  }
+}
+
+// Accumulator operands of N rows (as acc_operand), each input column's N
+// values (and validity bytes) loaded together.
+template <int N, int NA>
+__device__ __forceinline__ void acc_operands_n(const ColSet& vals, const AccSpec& A, int nacc,
+                                               const int64_t (&rc)[N], uint64_t (&v)[NA][N], bool (&ok)[NA][N]) {
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    if (a >= nacc) break;
+    const int32_t op = A.op[a];
+    if (op == kAccCountAll) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) { v[a][i] = 1; ok[a][i] = true; }
+      continue;
+    }
+    const int in = A.input[a];
+    int64_t idx[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) idx[i] = rc[i] + vals.offset[in];
+    if (vals.validity[in]) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) ok[a][i] = valid_bit(vals.validity[in], idx[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < N; ++i) ok[a][i] = true;
+    }
+    if (op == kAccCountValid) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[a][i] = ok[a][i] ? 1 : 0;
+      continue;
+    }
+    uint64_t raw[N];
+    load_zext_n<N>(vals.values[in], vals.width[in], idx, raw);
+    const int32_t t = vals.type[in];
+    const int w = vals.width[in];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (t == FPDB_TYPE_FLOAT64) v[a][i] = raw[i];
+      else if (t == FPDB_TYPE_FLOAT32) v[a][i] = (uint64_t)__double_as_longlong((double)__uint_as_float((uint32_t)raw[i]));
+      else if (type_is_signed_int_d(t)) v[a][i] = (uint64_t)sext_w(raw[i], w);
+      else v[a][i] = raw[i];
+    }
+  }
+}
+
 __device__ __forceinline__ void atomic_min_f64(double* p, double v) {
   unsigned long long* a = reinterpret_cast<unsigned long long*>(p);
   unsigned long long old = *a;
@@ -280,18 +355,18 @@ __global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, 
   constexpr int ITEMS = NACC > 0 ? 4 : 1;
   constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
   for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)ITEMS * blockDim.x) {
-    uint64_t kk[ITEMS];
-    uint64_t vv[ITEMS][NA];
-    bool okk[ITEMS][NA];
+    // rows past the range re-read its last row (r0 < end), so every load is unconditional
+    int64_t rc[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       int64_t r = r0 + (int64_t)i * blockDim.x;
-      if (r >= end) continue;
-      kk[i] = group_key(keys, r, &out_of_range);
-#pragma unroll
-      for (int a = 0; a < NA; ++a)
-        if (a < nacc) vv[i][a] = acc_operand(vals, A.op[a], A.input[a], r, &okk[i][a]);
+      rc[i] = r < end ? r : end - 1;
     }
+    uint64_t kk[ITEMS];
+    uint64_t vv[NA][ITEMS];
+    bool okk[NA][ITEMS];
+    group_keys_n<ITEMS>(keys, rc, kk, &out_of_range);
+    acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       int64_t r = r0 + (int64_t)i * blockDim.x;
@@ -304,7 +379,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, 
       }
 #pragma unroll
       for (int a = 0; a < NA; ++a)
-        if (a < nacc && okk[i][a]) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], vv[i][a]);
+        if (a < nacc && okk[a][i]) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], vv[a][i]);
       long long grow = (long long)(G.row_base + r);
       if (grow < T.first[s]) atomicMin(&T.first[s], grow);
     }
@@ -413,14 +488,22 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
   const int64_t begin = (int64_t)blockIdx.x * G.per_block;
   const int64_t end = begin + G.per_block < G.n ? begin + G.per_block : G.n;
   bool dummy = false;
-  constexpr int ITEMS = NACC > 0 ? 8 : 1;
+  constexpr int ITEMS = NACC > 0 ? 4 : 1;
+  constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
   for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)ITEMS * blockDim.x) {
-    uint64_t kk[ITEMS];
+    // unconditional loads of keys and operands (tail rows touch nearly every
+    // line of the value columns anyway)
+    int64_t rc[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       int64_t r = r0 + (int64_t)i * blockDim.x;
-      kk[i] = r < end ? group_key(keys, r, &dummy) : 0;
+      rc[i] = r < end ? r : end - 1;
     }
+    uint64_t kk[ITEMS];
+    uint64_t vv[NA][ITEMS];
+    bool okk[NA][ITEMS];
+    group_keys_n<ITEMS>(keys, rc, kk, &dummy);
+    acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       int64_t r = r0 + (int64_t)i * blockDim.x;
@@ -434,13 +517,8 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
       rec[0] = k;
       rec[1] = (uint64_t)(G.row_base + r);
 #pragma unroll
-      for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-        if (a < nacc) {
-          bool ok;
-          uint64_t v = acc_operand(vals, A.op[a], A.input[a], r, &ok);
-          rec[2 + a] = ok ? v : acc_identity(A.op[a]);
-        }
-      }
+      for (int a = 0; a < NA; ++a)
+        if (a < nacc) rec[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
     }
   }
   for (uint32_t s = threadIdx.x; s < stride; s += blockDim.x) {

@@ -187,23 +187,48 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
   // its first match
   const bool unique = *dup_flag == 0;
 
+  // Rows past the end read the last row (n >= 1 here), and every slot load is
+  // unconditional: no load sits in a branch that waits for it, so all of a
+  // phase's loads are in flight together.
   int64_t key[kProbeItems];
   bool live[kProbeItems];
+  int64_t rc[kProbeItems];          // the row, clamped
   uint64_t home[kProbeItems];
   Slot first[kProbeItems], second[kProbeItems];
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
     int64_t r = row0 + (int64_t)i * kProbeThreads;
-    live[i] = r < n && get_key<KM>(pk, r, &key[i]);
-    home[i] = live[i] ? slot_of(key[i], mask) : 0;
+    rc[i] = r < n ? r : n - 1;
   }
-  // the home slot and the next one are loaded together (one round trip for
-  // most probes)
+  if (KM == 1 || KM == 2) {
+    int64_t ki[kProbeItems];
+    uint64_t kv[kProbeItems];
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) ki[i] = rc[i] + pk.off[0];
+    load_zext_n<kProbeItems>(pk.v[0], KM == 1 ? 8 : 4, ki, kv);
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) {
+      key[i] = KM == 1 ? (int64_t)kv[i] : (int64_t)(int32_t)(uint32_t)kv[i];
+      live[i] = row0 + (int64_t)i * kProbeThreads < n;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) {
+      key[i] = 0;
+      live[i] = row0 + (int64_t)i * kProbeThreads < n && canon_key(pk, rc[i], &key[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) home[i] = slot_of(key[i], mask);
+  // the home slot and the next one, loaded together (one round trip for most probes)
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
-    first[i] = live[i] ? load_slot(slots, home[i]) : Slot{0, kEmptyRow, 0};
-    second[i] = live[i] ? load_slot(slots, (home[i] + 1) & mask) : Slot{0, kEmptyRow, 0};
+    first[i] = load_slot(slots, home[i]);
+    second[i] = load_slot(slots, (home[i] + 1) & mask);
   }
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i)
+    if (!live[i]) first[i].row = kEmptyRow;
 
   uint32_t cnt[kProbeItems];       // matches
   uint64_t fslot[kProbeItems];     // slot of the first match
@@ -261,16 +286,23 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
   }
   __syncthreads();
 
+  // output positions of every item's first output row
   uint64_t base = s_prefix;
+  int64_t dst[kProbeItems];
+  bool emit[kProbeItems];
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
     uint64_t before = base;
     for (int w = 0; w < wave; ++w) before += s_cnt[i][w];
-    uint64_t pos = before + incl[i] - outc[i];
+    dst[i] = (int64_t)(before + incl[i] - outc[i]);
     for (int w = 0; w < kProbeWaves; ++w) base += s_cnt[i][w];
-    const int64_t r = row0 + (int64_t)i * kProbeThreads;
+    emit[i] = JT != kJtMark && outc[i] > 0 && dst[i] < out_cap;
+  }
 
-    if (TRACK && cnt[i]) {
+  if (TRACK) {
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) {
+      if (!cnt[i]) continue;
       // flag every matched build row
       matched[frow[i]] = 1;
       uint32_t seen = 1;
@@ -282,36 +314,76 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
         s = (s + 1) & mask;
       }
     }
-    if (JT == kJtMark) continue;
-    if (outc[i] == 0) continue;
+  }
+  if (JT == kJtMark) {
+    lb_end(lb);
+    return;
+  }
 
-    if (cnt[i] == 0 || JT == kJtSemi) {
-      // unmatched probe row (outer) or semi-join row: probe columns only
-      int64_t dst = (int64_t)pos;
-      if (dst < out_cap) {
-        for (int c = 0; c < pcols.n; ++c) copy_value(pcols, out, c, r, dst);
-        if (JT == kJtOuter) {
-          for (int c = 0; c < bcols.n; ++c) {
-            int oc = pcols.n + c;
-            store_w(out.values[oc], out.width[oc], dst + out.offset[oc], 0);
-          }
-        }
-      }
-      continue;
+  // probe columns of the first output row: one column at a time, the values
+  // of all items loaded together
+  for (int c = 0; c < pcols.n; ++c) {
+    int64_t si[kProbeItems];
+    uint64_t v[kProbeItems];
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) si[i] = rc[i] + pcols.offset[c];
+    load_zext_n<kProbeItems>(pcols.values[c], pcols.width[c], si, v);
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) {
+      if (!emit[i]) continue;
+      int64_t di = dst[i] + out.offset[c];
+      store_w(out.values[c], out.width[c], di, v[i]);
+      if (out.validity[c] && valid_bit(pcols.validity[c], si[i]))
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[c]) + (di >> 5), 1u << (di & 31),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    // matched rows: the first match is known, later ones continue the chain
-    int64_t dst = (int64_t)pos;
-    emit_pair(pcols, bcols, out, out_cap, r, dst, frow[i], fpay[i], keymask, key[i]);
-    uint32_t emitted = 1;
-    uint64_t s = (fslot[i] + 1) & mask;
-    while (emitted < cnt[i]) {
-      Slot sl = load_slot(slots, s);
-      if (sl.row == kEmptyRow) break;
-      if (sl.key == key[i]) {
-        emit_pair(pcols, bcols, out, out_cap, r, dst + emitted, sl.row, sl.payload, keymask, key[i]);
-        ++emitted;
+  }
+  // build columns of the first output row: the first match, or null for an
+  // unmatched outer row (semi joins have none)
+  if (JT != kJtSemi) {
+    for (int c = 0; c < bcols.n; ++c) {
+      const int oc = pcols.n + c;
+      uint64_t v[kProbeItems];
+      int64_t bi[kProbeItems];
+#pragma unroll
+      for (int i = 0; i < kProbeItems; ++i) bi[i] = (int64_t)(cnt[i] ? frow[i] : 0u) + bcols.offset[c];
+      const bool from_key = (keymask >> c) & 1u;
+      if (from_key) {
+#pragma unroll
+        for (int i = 0; i < kProbeItems; ++i) v[i] = (uint64_t)key[i];
+      } else if (bcols.values[c] == nullptr) {
+#pragma unroll
+        for (int i = 0; i < kProbeItems; ++i) v[i] = fpay[i];
+      } else {
+        load_zext_n<kProbeItems>(bcols.values[c], bcols.width[c], bi, v);
       }
-      s = (s + 1) & mask;
+#pragma unroll
+      for (int i = 0; i < kProbeItems; ++i) {
+        if (!emit[i]) continue;
+        int64_t di = dst[i] + out.offset[oc];
+        bool m = cnt[i] != 0;
+        store_w(out.values[oc], out.width[oc], di, m ? v[i] : 0);
+        bool valid = m && (from_key || bcols.values[c] == nullptr || valid_bit(bcols.validity[c], bi[i]));
+        if (out.validity[oc] && valid)
+          __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[oc]) + (di >> 5), 1u << (di & 31),
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    // later matches of a key (duplicate build keys) continue the chain
+#pragma unroll
+    for (int i = 0; i < kProbeItems; ++i) {
+      if (cnt[i] < 2) continue;
+      uint32_t emitted = 1;
+      uint64_t s = (fslot[i] + 1) & mask;
+      while (emitted < cnt[i]) {
+        Slot sl = load_slot(slots, s);
+        if (sl.row == kEmptyRow) break;
+        if (sl.key == key[i]) {
+          emit_pair(pcols, bcols, out, out_cap, rc[i], dst[i] + emitted, sl.row, sl.payload, keymask, key[i]);
+          ++emitted;
+        }
+        s = (s + 1) & mask;
+      }
     }
   }
   lb_end(lb);
@@ -500,6 +572,12 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
     if (d_num_out) return launch_fill(c, d_num_out, 0, 8);
     *num_out = 0;
     return FPDB_OK;
+  }
+  if (t->build_rows == 0) {
+    // nothing can match: the kernel reads no build column (an empty one may
+    // have no storage); unmatched rows get nulls
+    for (int i = 0; i < bc.n; ++i) bc.values[i] = nullptr;
+    keymask = 0;
   }
   int64_t tiles = ceil_div(n, kProbeTileRows);
   FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "hash join probe: too many rows");

@@ -57,6 +57,28 @@ def test_blocked_build_find_bit_exact(ctx, n_build, n_probe):
     assert cnt == int(np.unpackbits(ref_bm, bitorder="little").sum())
 
 
+@pytest.mark.parametrize("n_build,lnb", [(100_000, 14), (200_000, 15), (400_000, 16), (600_000, 17)])
+def test_blocked_find_filter_slices(ctx, n_build, lnb):
+    """Filters of 1, 2 and 4 LDS slices (and the 2^16-reachable-block cap):
+    find bitmap, count and fused compaction match the oracle."""
+    rng = np.random.default_rng(lnb)
+    keys = rng.integers(0, 2 ** 40, n_build).astype(np.int64)
+    n = 1_000_003
+    probe = np.concatenate([rng.choice(keys, n // 2), rng.integers(0, 2 ** 40, n - n // 2)]).astype(np.int64)
+    rng.shuffle(probe)
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, keys)], n_build)
+    assert f.log_num_blocks == lnb
+    ref_bm = O.blocked_find(lnb, f.blocks_numpy(), O.arrow6_hash([(probe, None)]))
+    dp = DeviceColumn.from_numpy(ctx, probe)
+    bm, cnt = f.find([dp], n)
+    assert np.array_equal(bm.numpy((n + 7) // 8), ref_bm)
+    sel = np.unpackbits(ref_bm, bitorder="little")[:n].astype(bool)
+    assert cnt == int(sel.sum())
+    out, c2, bm2 = f.filter([dp], [0], n)
+    assert c2 == cnt and np.array_equal(out[0].numpy(), probe[sel])
+    assert np.array_equal(bm2.numpy((n + 7) // 8), ref_bm)
+
+
 def test_blocked_more_than_2_16_blocks_quirk(ctx):
     # 32-bit hashes address at most 2^16 blocks: a filter sized for > 524,288
     # keys never sets blocks >= 2^16 (block_id uses hash bits 16..31)
