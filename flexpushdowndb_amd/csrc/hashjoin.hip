@@ -168,33 +168,20 @@ __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bco
   }
 }
 
-template <int KM, int JT, bool TRACK>
-__global__ __launch_bounds__(256) void hj_probe_kernel(
-    const Slot* __restrict__ slots, uint64_t mask, KeySpec pk, int64_t n, ColSet pcols, ColSet bcols,
-    OutColSet out, int64_t out_cap, uint8_t* __restrict__ matched, uint32_t keymask, const int* dup_flag,
-    LbArgs lb, int64_t* total, int* err) {
-  __shared__ int64_t s_tile;
-  __shared__ uint32_t s_epoch;
-  __shared__ uint32_t s_cnt[kProbeItems][kProbeWaves];
-  __shared__ uint64_t s_prefix;
-  lb_begin(lb, &s_tile, &s_epoch);
-  const int64_t tile = s_tile;
-  const int64_t num_tiles = lb.num_tiles;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int64_t row0 = tile * kProbeTileRows + threadIdx.x;
-  // distinct build keys (checked when the table was built): a probe stops at
-  // its first match
-  const bool unique = *dup_flag == 0;
+// The probe runs as two passes so the random table reads are not interleaved
+// with a look-back wait:
+//   match  tiles of 1024 rows (256 threads x 4): keys, table lookups (the home
+//          slot and the next one loaded together), per row the match count and
+//          first match (build row, inline payload), per tile the output count;
+//   scan   exclusive scan of the tile counts (output offsets, total);
+//   emit   the same tiles again: output positions from the tile offset and an
+//          in-tile scan, then the output columns, written in probe-row order.
+// Every phase issues its loads unconditionally (rows past the end read the
+// last row) so none waits in its own branch.
 
-  // Rows past the end read the last row (n >= 1 here), and every slot load is
-  // unconditional: no load sits in a branch that waits for it, so all of a
-  // phase's loads are in flight together.
-  int64_t key[kProbeItems];
-  bool live[kProbeItems];
-  int64_t rc[kProbeItems];          // the row, clamped
-  uint64_t home[kProbeItems];
-  Slot first[kProbeItems], second[kProbeItems];
+template <int KM>
+__device__ __forceinline__ void probe_keys(const KeySpec& pk, int64_t n, int64_t row0, int64_t (&rc)[kProbeItems],
+                                           int64_t (&key)[kProbeItems], bool (&live)[kProbeItems]) {
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
     int64_t r = row0 + (int64_t)i * kProbeThreads;
@@ -218,27 +205,47 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
       live[i] = row0 + (int64_t)i * kProbeThreads < n && canon_key(pk, rc[i], &key[i]);
     }
   }
+}
+
+template <int JT>
+__device__ __forceinline__ uint32_t probe_outputs(bool in_range, uint32_t cnt) {
+  if (JT == kJtOuter) return in_range ? (cnt ? cnt : 1u) : 0u;
+  if (JT == kJtSemi) return cnt ? 1u : 0u;
+  if (JT == kJtMark) return 0u;
+  return cnt;
+}
+
+template <int KM, int JT, bool TRACK>
+__global__ __launch_bounds__(256) void hj_match_kernel(const Slot* __restrict__ slots, uint64_t mask, KeySpec pk,
+                                                       int64_t n, uint8_t* __restrict__ matched, const int* dup_flag,
+                                                       uint64_t* __restrict__ mrow, uint32_t* __restrict__ mcnt,
+                                                       uint32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t s_w[kProbeWaves];
+  const int64_t tile = blockIdx.x;
+  const int64_t row0 = tile * kProbeTileRows + threadIdx.x;
+  // distinct build keys (checked when the table was built): a probe stops at
+  // its first match
+  const bool unique = *dup_flag == 0;
+  int64_t key[kProbeItems], rc[kProbeItems];
+  bool live[kProbeItems];
+  probe_keys<KM>(pk, n, row0, rc, key, live);
+  uint64_t home[kProbeItems];
+  Slot first[kProbeItems], second[kProbeItems];
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) home[i] = slot_of(key[i], mask);
-  // the home slot and the next one, loaded together (one round trip for most probes)
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
     first[i] = load_slot(slots, home[i]);
     second[i] = load_slot(slots, (home[i] + 1) & mask);
   }
-#pragma unroll
-  for (int i = 0; i < kProbeItems; ++i)
-    if (!live[i]) first[i].row = kEmptyRow;
-
-  uint32_t cnt[kProbeItems];       // matches
-  uint64_t fslot[kProbeItems];     // slot of the first match
-  uint32_t frow[kProbeItems], fpay[kProbeItems];
+  uint32_t cnt[kProbeItems], frow[kProbeItems], fpay[kProbeItems];
+  uint64_t fslot[kProbeItems];
 #pragma unroll
   for (int i = 0; i < kProbeItems; ++i) {
     cnt[i] = 0;
-    fslot[i] = 0;
     frow[i] = 0;
     fpay[i] = 0;
+    fslot[i] = 0;
     if (!live[i]) continue;
     Slot sl = first[i];
     uint64_t s = home[i];
@@ -253,58 +260,11 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
       sl = (++step == 1) ? second[i] : load_slot(slots, s);
     }
   }
-
-  uint32_t outc[kProbeItems];
-#pragma unroll
-  for (int i = 0; i < kProbeItems; ++i) {
-    int64_t r = row0 + (int64_t)i * kProbeThreads;
-    if (JT == kJtOuter) outc[i] = (r < n) ? (cnt[i] ? cnt[i] : 1u) : 0u;
-    else if (JT == kJtSemi) outc[i] = cnt[i] ? 1u : 0u;
-    else if (JT == kJtMark) outc[i] = 0u;
-    else outc[i] = cnt[i];
-  }
-
-  // offsets within the tile, in row order (item-major, then thread)
-  uint32_t incl[kProbeItems];
-#pragma unroll
-  for (int i = 0; i < kProbeItems; ++i) {
-    incl[i] = wave_incl_scan(outc[i]);
-    if (lane == 63) s_cnt[i][wave] = incl[i];
-  }
-  __syncthreads();
-  if (wave == 0) {
-    uint64_t agg = 0;
-    for (int i = 0; i < kProbeItems; ++i)
-      for (int w = 0; w < kProbeWaves; ++w) agg += s_cnt[i][w];
-    int e = 0;
-    uint64_t excl = lookback_exclusive(lb.status, tile, s_epoch, agg, &e);
-    if (lane == 0) {
-      s_prefix = excl;
-      if (e) *err = 1;
-      if (tile == num_tiles - 1) *total = (int64_t)(excl + agg);
-    }
-  }
-  __syncthreads();
-
-  // output positions of every item's first output row
-  uint64_t base = s_prefix;
-  int64_t dst[kProbeItems];
-  bool emit[kProbeItems];
-#pragma unroll
-  for (int i = 0; i < kProbeItems; ++i) {
-    uint64_t before = base;
-    for (int w = 0; w < wave; ++w) before += s_cnt[i][w];
-    dst[i] = (int64_t)(before + incl[i] - outc[i]);
-    for (int w = 0; w < kProbeWaves; ++w) base += s_cnt[i][w];
-    emit[i] = JT != kJtMark && outc[i] > 0 && dst[i] < out_cap;
-  }
-
   if (TRACK) {
 #pragma unroll
     for (int i = 0; i < kProbeItems; ++i) {
       if (!cnt[i]) continue;
-      // flag every matched build row
-      matched[frow[i]] = 1;
+      matched[frow[i]] = 1;   // flag every matched build row
       uint32_t seen = 1;
       uint64_t s = (fslot[i] + 1) & mask;
       while (seen < cnt[i]) {
@@ -315,11 +275,102 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
       }
     }
   }
-  if (JT == kJtMark) {
-    lb_end(lb);
-    return;
+  uint32_t total = 0;
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    int64_t r = row0 + (int64_t)i * kProbeThreads;
+    if (r < n) {
+      mcnt[r] = cnt[i];
+      mrow[r] = ((uint64_t)frow[i] << 32) | fpay[i];
+    }
+    total += probe_outputs<JT>(r < n, cnt[i]);
   }
+  for (int off = 32; off > 0; off >>= 1) total += __shfl_xor(total, off);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kProbeWaves; ++w) t += s_w[w];
+    tile_cnt[tile] = t;
+  }
+}
 
+// Exclusive scan of the tile counts (one workgroup); *total = the output rows.
+__global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ cnt, int64_t tiles,
+                                                         uint64_t* __restrict__ base, int64_t* total) {
+  __shared__ uint64_t s_w[16];
+  __shared__ uint64_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t t0 = 0; t0 < tiles; t0 += blockDim.x) {
+    int64_t t = t0 + threadIdx.x;
+    uint64_t v = t < tiles ? cnt[t] : 0;
+    uint64_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      uint64_t u = __shfl_up(incl, off);
+      if ((threadIdx.x & 63) >= off) incl += u;
+    }
+    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint64_t wprefix = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wprefix += s_w[w];
+    uint64_t carry = s_carry;
+    if (t < tiles) base[t] = carry + wprefix + incl - v;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) s_carry = carry + wprefix + incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = (int64_t)s_carry;
+}
+
+template <int KM, int JT>
+__global__ __launch_bounds__(256) void hj_emit_kernel(const Slot* __restrict__ slots, uint64_t mask, KeySpec pk,
+                                                      int64_t n, ColSet pcols, ColSet bcols, OutColSet out,
+                                                      int64_t out_cap, uint32_t keymask,
+                                                      const uint64_t* __restrict__ mrow,
+                                                      const uint32_t* __restrict__ mcnt,
+                                                      const uint64_t* __restrict__ tile_base) {
+  __shared__ uint32_t s_cnt[kProbeItems][kProbeWaves];
+  const int64_t tile = blockIdx.x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t row0 = tile * kProbeTileRows + threadIdx.x;
+  int64_t key[kProbeItems], rc[kProbeItems];
+  bool live[kProbeItems];
+  probe_keys<KM>(pk, n, row0, rc, key, live);
+  uint32_t cnt[kProbeItems], frow[kProbeItems], fpay[kProbeItems], outc[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    cnt[i] = mcnt[rc[i]];
+    uint64_t m = mrow[rc[i]];
+    frow[i] = (uint32_t)(m >> 32);
+    fpay[i] = (uint32_t)m;
+  }
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    bool in_range = row0 + (int64_t)i * kProbeThreads < n;
+    if (!in_range) cnt[i] = 0;
+    outc[i] = probe_outputs<JT>(in_range, cnt[i]);
+  }
+  // positions within the tile, in row order (item-major, then thread)
+  uint32_t incl[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    incl[i] = wave_incl_scan(outc[i]);
+    if (lane == 63) s_cnt[i][wave] = incl[i];
+  }
+  __syncthreads();
+  uint64_t base = tile_base[tile];
+  int64_t dst[kProbeItems];
+  bool emit[kProbeItems];
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    uint64_t before = base;
+    for (int w = 0; w < wave; ++w) before += s_cnt[i][w];
+    dst[i] = (int64_t)(before + incl[i] - outc[i]);
+    for (int w = 0; w < kProbeWaves; ++w) base += s_cnt[i][w];
+    emit[i] = outc[i] > 0 && dst[i] < out_cap;
+  }
   // probe columns of the first output row: one column at a time, the values
   // of all items loaded together
   for (int c = 0; c < pcols.n; ++c) {
@@ -338,55 +389,55 @@ __global__ __launch_bounds__(256) void hj_probe_kernel(
                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+  if (JT == kJtSemi) return;
   // build columns of the first output row: the first match, or null for an
-  // unmatched outer row (semi joins have none)
-  if (JT != kJtSemi) {
-    for (int c = 0; c < bcols.n; ++c) {
-      const int oc = pcols.n + c;
-      uint64_t v[kProbeItems];
-      int64_t bi[kProbeItems];
+  // unmatched outer row
+  for (int c = 0; c < bcols.n; ++c) {
+    const int oc = pcols.n + c;
+    uint64_t v[kProbeItems];
+    int64_t bi[kProbeItems];
 #pragma unroll
-      for (int i = 0; i < kProbeItems; ++i) bi[i] = (int64_t)(cnt[i] ? frow[i] : 0u) + bcols.offset[c];
-      const bool from_key = (keymask >> c) & 1u;
-      if (from_key) {
+    for (int i = 0; i < kProbeItems; ++i) bi[i] = (int64_t)(cnt[i] ? frow[i] : 0u) + bcols.offset[c];
+    const bool from_key = (keymask >> c) & 1u;
+    if (from_key) {
 #pragma unroll
-        for (int i = 0; i < kProbeItems; ++i) v[i] = (uint64_t)key[i];
-      } else if (bcols.values[c] == nullptr) {
+      for (int i = 0; i < kProbeItems; ++i) v[i] = (uint64_t)key[i];
+    } else if (bcols.values[c] == nullptr) {
 #pragma unroll
-        for (int i = 0; i < kProbeItems; ++i) v[i] = fpay[i];
-      } else {
-        load_zext_n<kProbeItems>(bcols.values[c], bcols.width[c], bi, v);
-      }
-#pragma unroll
-      for (int i = 0; i < kProbeItems; ++i) {
-        if (!emit[i]) continue;
-        int64_t di = dst[i] + out.offset[oc];
-        bool m = cnt[i] != 0;
-        store_w(out.values[oc], out.width[oc], di, m ? v[i] : 0);
-        bool valid = m && (from_key || bcols.values[c] == nullptr || valid_bit(bcols.validity[c], bi[i]));
-        if (out.validity[oc] && valid)
-          __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[oc]) + (di >> 5), 1u << (di & 31),
-                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      for (int i = 0; i < kProbeItems; ++i) v[i] = fpay[i];
+    } else {
+      load_zext_n<kProbeItems>(bcols.values[c], bcols.width[c], bi, v);
     }
-    // later matches of a key (duplicate build keys) continue the chain
 #pragma unroll
     for (int i = 0; i < kProbeItems; ++i) {
-      if (cnt[i] < 2) continue;
-      uint32_t emitted = 1;
-      uint64_t s = (fslot[i] + 1) & mask;
-      while (emitted < cnt[i]) {
-        Slot sl = load_slot(slots, s);
-        if (sl.row == kEmptyRow) break;
-        if (sl.key == key[i]) {
-          emit_pair(pcols, bcols, out, out_cap, rc[i], dst[i] + emitted, sl.row, sl.payload, keymask, key[i]);
-          ++emitted;
-        }
-        s = (s + 1) & mask;
-      }
+      if (!emit[i]) continue;
+      int64_t di = dst[i] + out.offset[oc];
+      bool m = cnt[i] != 0;
+      store_w(out.values[oc], out.width[oc], di, m ? v[i] : 0);
+      bool valid = m && (from_key || bcols.values[c] == nullptr || valid_bit(bcols.validity[c], bi[i]));
+      if (out.validity[oc] && valid)
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[oc]) + (di >> 5), 1u << (di & 31),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
-  lb_end(lb);
+  // later matches of a key (duplicate build keys): walk the chain again from
+  // the home slot, skipping the first match
+#pragma unroll
+  for (int i = 0; i < kProbeItems; ++i) {
+    if (cnt[i] < 2) continue;
+    uint32_t seen = 0;
+    uint64_t s = slot_of(key[i], mask);
+    while (seen < cnt[i]) {
+      Slot sl = load_slot(slots, s);
+      if (sl.row == kEmptyRow) break;
+      if (sl.key == key[i]) {
+        if (seen > 0)
+          emit_pair(pcols, bcols, out, out_cap, rc[i], dst[i] + seen, sl.row, sl.payload, keymask, key[i]);
+        ++seen;
+      }
+      s = (s + 1) & mask;
+    }
+  }
 }
 
 __global__ void gather_kernel(ColSet in, const int64_t* __restrict__ rows, int64_t n, OutColSet out) {
@@ -581,33 +632,71 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
   }
   int64_t tiles = ceil_div(n, kProbeTileRows);
   FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "hash join probe: too many rows");
-  LbArgs lb;
-  FPDB_TRY(lookback_prepare(c, tiles, &lb));
-  int* d_err = ctx_err_flag(c);
   int64_t* total = d_num_out ? d_num_out : c->d_counter;
   uint64_t mask = (uint64_t)t->capacity - 1;
   int km = key_mode_of(pk);
+  uint64_t* mrow = nullptr;
+  uint32_t* mcnt = nullptr;
+  uint32_t* tcnt = nullptr;
+  uint64_t* tbase = nullptr;
+  fpdb_status st = ctx_alloc(c, (uint64_t)n * 8, reinterpret_cast<void**>(&mrow));
+  if (st == FPDB_OK) st = ctx_alloc(c, (uint64_t)n * 4, reinterpret_cast<void**>(&mcnt));
+  if (st == FPDB_OK) st = ctx_alloc(c, (uint64_t)tiles * 4, reinterpret_cast<void**>(&tcnt));
+  if (st == FPDB_OK) st = ctx_alloc(c, (uint64_t)tiles * 8, reinterpret_cast<void**>(&tbase));
+  auto release = [&]() {
+    if (mrow) ctx_free(c, mrow);
+    if (mcnt) ctx_free(c, mcnt);
+    if (tcnt) ctx_free(c, tcnt);
+    if (tbase) ctx_free(c, tbase);
+  };
+  if (st != FPDB_OK) {
+    release();
+    return st;
+  }
+  const int* dup = reinterpret_cast<const int*>(t->meta + 1);
   dim3 g((unsigned)tiles), b(kProbeThreads);
-  ProfScope ps(c, "hj_probe");
-#define FPDB_PROBE_LAUNCH(KM, JT, TR)                                                              \
-  hipLaunchKernelGGL((hj_probe_kernel<KM, JT, TR>), g, b, 0, c->stream, t->slots, mask, pk, n, pc, bc, \
-                     oc, out_cap, t->matched, keymask, reinterpret_cast<const int*>(t->meta + 1), lb, total, d_err)
-#define FPDB_PROBE_JT(KM)                                                    \
+  {
+    ProfScope ps(c, "hj_probe");
+#define FPDB_MATCH_LAUNCH(KM, JT, TR)                                                                 \
+  hipLaunchKernelGGL((hj_match_kernel<KM, JT, TR>), g, b, 0, c->stream, t->slots, mask, pk, n, t->matched, dup, \
+                     mrow, mcnt, tcnt)
+#define FPDB_MATCH_JT(KM)                                                    \
   switch (jt * 2 + (track ? 1 : 0)) {                                        \
-    case kJtInner * 2: FPDB_PROBE_LAUNCH(KM, kJtInner, false); break;        \
-    case kJtInner * 2 + 1: FPDB_PROBE_LAUNCH(KM, kJtInner, true); break;     \
-    case kJtOuter * 2: FPDB_PROBE_LAUNCH(KM, kJtOuter, false); break;        \
-    case kJtOuter * 2 + 1: FPDB_PROBE_LAUNCH(KM, kJtOuter, true); break;     \
-    case kJtSemi * 2: FPDB_PROBE_LAUNCH(KM, kJtSemi, false); break;          \
-    default: FPDB_PROBE_LAUNCH(KM, kJtMark, true); break;                    \
+    case kJtInner * 2: FPDB_MATCH_LAUNCH(KM, kJtInner, false); break;        \
+    case kJtInner * 2 + 1: FPDB_MATCH_LAUNCH(KM, kJtInner, true); break;     \
+    case kJtOuter * 2: FPDB_MATCH_LAUNCH(KM, kJtOuter, false); break;        \
+    case kJtOuter * 2 + 1: FPDB_MATCH_LAUNCH(KM, kJtOuter, true); break;     \
+    case kJtSemi * 2: FPDB_MATCH_LAUNCH(KM, kJtSemi, false); break;          \
+    default: FPDB_MATCH_LAUNCH(KM, kJtMark, true); break;                    \
   }
-  switch (km) {
-    case 1: FPDB_PROBE_JT(1); break;
-    case 2: FPDB_PROBE_JT(2); break;
-    default: FPDB_PROBE_JT(0); break;
+    switch (km) {
+      case 1: FPDB_MATCH_JT(1); break;
+      case 2: FPDB_MATCH_JT(2); break;
+      default: FPDB_MATCH_JT(0); break;
+    }
+#undef FPDB_MATCH_JT
+#undef FPDB_MATCH_LAUNCH
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tcnt, tiles, tbase, total);
+    if (jt != kJtMark && oc.n > 0 && out_cap > 0) {
+#define FPDB_EMIT_LAUNCH(KM, JT)                                                                      \
+  hipLaunchKernelGGL((hj_emit_kernel<KM, JT>), g, b, 0, c->stream, t->slots, mask, pk, n, pc, bc, oc, out_cap, \
+                     keymask, mrow, mcnt, tbase)
+#define FPDB_EMIT_JT(KM)                                           \
+  switch (jt) {                                                    \
+    case kJtInner: FPDB_EMIT_LAUNCH(KM, kJtInner); break;          \
+    case kJtOuter: FPDB_EMIT_LAUNCH(KM, kJtOuter); break;          \
+    default: FPDB_EMIT_LAUNCH(KM, kJtSemi); break;                 \
   }
-#undef FPDB_PROBE_JT
-#undef FPDB_PROBE_LAUNCH
+      switch (km) {
+        case 1: FPDB_EMIT_JT(1); break;
+        case 2: FPDB_EMIT_JT(2); break;
+        default: FPDB_EMIT_JT(0); break;
+      }
+#undef FPDB_EMIT_JT
+#undef FPDB_EMIT_LAUNCH
+    }
+  }
+  release();   // stream-ordered reuse: later work on the stream runs after these kernels
   FPDB_HIP(hipGetLastError());
   if (d_num_out) return FPDB_OK;
   FPDB_TRY(read_count(c, num_out));
