@@ -106,6 +106,12 @@ __device__ __forceinline__ void bf_insert(uint64_t* blocks, int lnb, const uint6
   __hip_atomic_fetch_or(&blocks[b], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// LDS for the forward filters of the fact pass (filters that do not fit are
+// read from HBM / L2), and fact rows per thread per step.
+constexpr int kStarFilterLdsWords = 8192;   // 64 KiB
+constexpr size_t kStarDynamicLds = kStarFilterLdsWords * 8;
+constexpr int kStarItems = 8;
+
 __device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const uint64_t* s_masks, int64_t key) {
   uint32_t h = arrow6_hash_word((uint64_t)key);
   uint64_t m = rotl64s(s_masks[h & (kMaskNum - 1)], (h >> kMaskLogNum) & 63);
@@ -269,19 +275,59 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
       if (D.kept[r]) bf_insert(D.fbf, lnb, s_masks, D.key[r]);
   }
   star_barrier(P.state);
-  // B: forward pass over every fact row (all partitions), filters in `order`
+  // B: forward pass over every fact row (all partitions), filters in `order`.
+  // The filters that fit (in that order) are copied into LDS; each thread
+  // takes kStarItems rows per step with the first filter's keys loaded
+  // together, and only rows still passing read the next filter's keys.
   {
+    extern __shared__ uint64_t s_filt[];
     int lnb[kStarMaxDims];
-    for (int d = 0; d < P.ndims; ++d)
+    int64_t loff[kStarMaxDims];
+    int64_t used = 0;
+    for (int i = 0; i < P.ndims; ++i) {
+      int d = P.order[i];
       lnb[d] = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const int64_t rounded = (P.fact_rows + 63) & ~int64_t(63);
-    for (int64_t r = gtid; r < rounded; r += gthreads) {     // whole waves iterate together
-      bool pass = r < P.fact_rows;
-      for (int i = 0; i < P.ndims && pass; ++i) {
-        int d = P.order[i];
-        pass = bf_find(P.dim[d].fbf, lnb[d], s_masks, P.dim[d].fk[r]);
+      int64_t words = int64_t(1) << lnb[d];
+      if (used + words <= kStarFilterLdsWords) {
+        loff[d] = used;
+        for (int64_t b = threadIdx.x; b < words; b += blockDim.x) s_filt[used + b] = P.dim[d].fbf[b];
+        used += words;
+      } else {
+        loff[d] = -1;
       }
-      append_survivor(P, pass, r);
+    }
+    __syncthreads();
+    const int d0 = P.order[0];
+    const int64_t step = gthreads * kStarItems;
+    const int64_t rounded = (P.fact_rows + step - 1) / step * step;   // whole waves iterate together
+    const int64_t last = P.fact_rows - 1;
+    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * kStarItems + threadIdx.x; r0 < rounded; r0 += step) {
+      int64_t fk[kStarItems];
+#pragma unroll
+      for (int j = 0; j < kStarItems; ++j) {
+        int64_t r = r0 + (int64_t)j * blockDim.x;
+        fk[j] = P.dim[d0].fk[r <= last ? r : last];
+      }
+      bool pass[kStarItems];
+#pragma unroll
+      for (int j = 0; j < kStarItems; ++j) {
+        int64_t r = r0 + (int64_t)j * blockDim.x;
+        bool hit = loff[d0] >= 0 ? bf_find(s_filt + loff[d0], lnb[d0], s_masks, fk[j])
+                                 : bf_find(P.dim[d0].fbf, lnb[d0], s_masks, fk[j]);
+        pass[j] = r <= last && hit;
+      }
+      for (int i = 1; i < P.ndims; ++i) {
+        const int d = P.order[i];
+#pragma unroll
+        for (int j = 0; j < kStarItems; ++j) {
+          if (!pass[j]) continue;
+          int64_t key = P.dim[d].fk[r0 + (int64_t)j * blockDim.x];
+          pass[j] = loff[d] >= 0 ? bf_find(s_filt + loff[d], lnb[d], s_masks, key)
+                                 : bf_find(P.dim[d].fbf, lnb[d], s_masks, key);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kStarItems; ++j) append_survivor(P, pass[j], r0 + (int64_t)j * blockDim.x);
     }
   }
   star_barrier(P.state);
@@ -690,7 +736,7 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
   FPDB_HIP(hipGetLastError());
   {
     ProfScope ps(c, "star_persistent");
-    hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(256), 0, c->stream, s->dev);
+    hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(256), kStarDynamicLds, c->stream, s->dev);
   }
   FPDB_HIP(hipGetLastError());
   hipLaunchKernelGGL(star_emit_kernel, dim3(1), dim3(1024), 0, c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt,
@@ -806,7 +852,7 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   }
   // persistent grid: every workgroup resident at once
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, star_persistent_kernel, 256, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, star_persistent_kernel, 256, kStarDynamicLds);
   if (e != hipSuccess || per_cu < 1) {
     fpdb_star_destroy(s);
     return fail(FPDB_ERR_HIP, "star join: occupancy query failed");
