@@ -13,6 +13,7 @@
 #include "compact.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <random>
 #include <vector>
 
@@ -87,18 +88,40 @@ __global__ __launch_bounds__(256) void blocked_insert_kernel(ColSet keys, int64_
 }
 
 // ---------------------------------------------------------------------------
-// blocked filter: find with (part of) the filter in LDS
+// blocked filter: find with the filter in LDS
 //
 // A key's block index is bits 16..31 of its hash, so at most 2^16 blocks
-// (512 KiB) are ever read.  Random 8-byte reads from the L2 bound a find whose
-// filter lives there (≈ 220 G requests/s on this part), so each workgroup
-// copies one slice of up to 2^14 blocks (128 KiB) into LDS: keys whose block
-// lies in that slice are tested in LDS, the others in L2.  A filter of up to
-// 2^14 blocks is read from LDS only; with S slices, workgroup b holds slice
-// b % S, so 1/S of the random reads leave the L2.
+// (512 KiB) are ever read.  Random 8-byte reads from the L2 run at about
+// 220 G requests/s on this part, a quarter of what the key stream needs, so
+// block reads go to LDS wherever the filter fits:
+//   * up to 2^14 reachable blocks (128 KiB): every workgroup holds the whole
+//     filter and tests its own range of keys;
+//   * larger filters are cut into slices of 2^14 blocks.  Workgroups form
+//     groups of G that read the same range of keys; member q holds slice q
+//     and tests the keys whose block lies in it, plus -- when the G slices do
+//     not cover the filter -- the keys of the remaining blocks b with
+//     b % G == q, read from L2.  Each member writes partial bitmap words (bit i
+//     set iff member q tested row i and it passed); a combine kernel ORs the G
+//     partial words.  Workgroup b runs on XCD b % 8 (round-robin dispatch)
+//     and the members of a group are b, b + 8, ..., b + 8(G-1): they share one
+//     XCD's L2, so a key line one member fetched from HBM is an L2 hit for the
+//     others.
+// Every member loads and hashes every key of its range, so G trades VALU and
+// texture-address work (G loads and hashes per key) against L2 gathers (keys
+// of blocks outside the slices; a 64-lane gather of distinct lines costs the
+// texture-address unit about 4x a coalesced load).  Measured on a 2^16-block
+// filter with 60M keys: G = 1, 2 and 4 all take 0.28-0.30 ms, G = 1 bound by
+// the gathers, G = 4 by VALU (33 instructions per 64 keys per member); G = 4
+// is kept since it makes no L2 gathers at all.
 constexpr int kSliceLog = 14;
 constexpr int kSliceThreads = 1024;
-constexpr int kSliceUnroll = 8;
+#ifndef FPDB_SLICE_UNROLL
+#define FPDB_SLICE_UNROLL 8   // bitmap words per wave per step (loads in flight together)
+#endif
+constexpr int kSliceUnroll = FPDB_SLICE_UNROLL;
+#ifndef FPDB_FIND_GROUP
+#define FPDB_FIND_GROUP 4     // most workgroups sharing one range of keys
+#endif
 
 inline int reachable_log_blocks(int log_num_blocks) { return log_num_blocks < 16 ? log_num_blocks : 16; }
 inline int num_slices(int log_num_blocks) {
@@ -106,33 +129,94 @@ inline int num_slices(int log_num_blocks) {
   return r <= kSliceLog ? 1 : 1 << (r - kSliceLog);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kSliceThreads) void blocked_find_lds_kernel(
-    const uint64_t* __restrict__ blocks, uint64_t nb_mask, int slices, ColSet keys, int64_t n,
-    int64_t words_per_block, uint8_t* __restrict__ bitmap, unsigned long long* partials) {
+// One key's test for member q of a group of G: true iff the member owns the
+// key's block and the block holds every bit of the key's mask.
+template <bool HAS_L2>
+__device__ __forceinline__ bool find_test(uint32_t h, uint32_t bmask, uint32_t G, uint32_t q,
+                                          const uint64_t* s_blk, const uint64_t* s_masks,
+                                          const uint64_t* __restrict__ blocks) {
+  const uint32_t b = (h >> (kMaskLogNum + 6)) & bmask;
+  const uint32_t sl = b >> kSliceLog;
+  bool mine = sl == q;
+  uint64_t blk = s_blk[b & ((1u << kSliceLog) - 1)];
+  if (HAS_L2) {
+    const bool l2 = sl >= G && (b & (G - 1)) == q;
+    const uint64_t g = blocks[l2 ? b : 0];
+    blk = l2 ? g : blk;
+    mine |= l2;
+  }
+  const uint64_t m = rotl64(s_masks[h & (kMaskNum - 1)], (h >> kMaskLogNum) & 63);
+  return mine & ((~blk & m) == 0);
+}
+
+// PARTIAL: member q writes partial words to out_words[q * nwords + w];
+// otherwise (G == 1) the bitmap is written directly (FindImp's tail-byte rule)
+// and per-workgroup counts go to partials.
+template <int MODE, bool HAS_L2, bool PARTIAL>
+__global__ __launch_bounds__(kSliceThreads) void blocked_find_kernel(
+    const uint64_t* __restrict__ blocks, uint64_t nb_mask, int group_size, ColSet keys, int64_t n,
+    int64_t words_per_group, uint64_t* __restrict__ out_words, uint8_t* __restrict__ bitmap,
+    unsigned long long* partials) {
   extern __shared__ uint64_t lds[];
   uint64_t* s_masks = lds;
   uint64_t* s_blk = lds + kMaskNum;
-  const int q = (int)(blockIdx.x % slices);
-  const uint64_t reach = (nb_mask + 1) < (1ULL << 16) ? (nb_mask + 1) : (1ULL << 16);
+  const uint32_t G = (uint32_t)group_size;
+  uint32_t q;
+  int64_t group;
+  if (PARTIAL) {
+    const uint32_t j = blockIdx.x >> 3;
+    q = j % G;
+    group = (int64_t)(j / G) * 8 + (blockIdx.x & 7);
+  } else {
+    q = 0;
+    group = blockIdx.x;
+  }
+  const uint64_t reach = nb_mask + 1 < (1ULL << 16) ? nb_mask + 1 : (1ULL << 16);
   const uint64_t lo = (uint64_t)q << kSliceLog;
   const uint64_t cnt = reach - lo < (1ULL << kSliceLog) ? reach - lo : (1ULL << kSliceLog);
   for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
   for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) s_blk[i] = blocks[lo + i];
   __syncthreads();
+  const uint32_t bmask = (uint32_t)(reach - 1);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
   const int64_t nwords = (n + 63) >> 6;
-  const int64_t w0 = (int64_t)blockIdx.x * words_per_block;
-  const int64_t w1 = w0 + words_per_block < nwords ? w0 + words_per_block : nwords;
+  const int64_t w0 = group * words_per_group;
+  const int64_t w1 = w0 + words_per_group < nwords ? w0 + words_per_group : nwords;
+  const int64_t wfull = w1 < (n >> 6) ? w1 : (n >> 6);   // words whose 64 rows all exist
+  uint64_t* __restrict__ mine = PARTIAL ? out_words + (int64_t)q * nwords : nullptr;
   uint64_t count = 0;
-  // Loads are issued unconditionally (rows past the end re-read the last row,
-  // blocks held in LDS read one shared dummy block) so the compiler does not
-  // wrap each in a branch that waits for it: all of a step's loads are in
-  // flight together.
+  auto emit = [&](int64_t w, uint64_t word) {
+    if (PARTIAL) {
+      if (lane == 0) mine[w] = word;
+    } else {
+      write_bitmap_word(bitmap, w, word, n, lane);
+      count += __popcll(word);
+    }
+  };
+  int64_t wb = w0 + (int64_t)wave * kSliceUnroll;
+  if (MODE == 1 || MODE == 2) {
+    // whole steps of full words: no clamping, one address for the step's loads
+    using T = typename std::conditional<MODE == 1, uint64_t, uint32_t>::type;
+    const T* __restrict__ kp = reinterpret_cast<const T*>(keys.values[0]) + keys.offset[0] + lane;
+    for (; wb + kSliceUnroll <= wfull; wb += (int64_t)nwaves * kSliceUnroll) {
+      const T* p = kp + wb * 64;
+      T x[kSliceUnroll];
+#pragma unroll
+      for (int u = 0; u < kSliceUnroll; ++u) x[u] = p[u * 64];
+      uint64_t word[kSliceUnroll];
+#pragma unroll
+      for (int u = 0; u < kSliceUnroll; ++u)
+        word[u] = __ballot(find_test<HAS_L2>(arrow6_hash_word(x[u]), bmask, G, q, s_blk, s_masks, blocks));
+#pragma unroll
+      for (int u = 0; u < kSliceUnroll; ++u) emit(wb + u, word[u]);
+    }
+  }
+  // the rest (and every step of the generic key path): rows past the end
+  // re-read the last row, so all of a step's loads are issued together
   const int64_t last_row = n - 1;
-  for (int64_t wb = w0 + (int64_t)wave * kSliceUnroll; wb < w1; wb += (int64_t)nwaves * kSliceUnroll) {
+  for (; wb < w1; wb += (int64_t)nwaves * kSliceUnroll) {
     uint32_t h[kSliceUnroll];
     bool in[kSliceUnroll];
 #pragma unroll
@@ -141,34 +225,49 @@ __global__ __launch_bounds__(kSliceThreads) void blocked_find_lds_kernel(
       in[u] = wb + u < w1 && r < n;
       h[u] = key_hash<MODE>(keys, in[u] ? r : last_row);
     }
-    uint64_t blk[kSliceUnroll];
-    if (slices == 1) {
+    uint64_t word[kSliceUnroll];
 #pragma unroll
-      for (int u = 0; u < kSliceUnroll; ++u) {
-        uint64_t b = bb_block(h[u], nb_mask) - lo;
-        blk[u] = s_blk[b < cnt ? b : 0];
-      }
+    for (int u = 0; u < kSliceUnroll; ++u)
+      word[u] = __ballot(in[u] & find_test<HAS_L2>(h[u], bmask, G, q, s_blk, s_masks, blocks));
+#pragma unroll
+    for (int u = 0; u < kSliceUnroll; ++u)
+      if (wb + u < w1) emit(wb + u, word[u]);
+  }
+  if (!PARTIAL && partials) block_count_store(partials, lane == 0 ? count : 0);
+}
+
+// bitmap word w = OR of the G partial words; FindImp's tail-byte rule for the
+// last word; per-workgroup selected counts.
+__global__ __launch_bounds__(256) void find_combine_kernel(const uint64_t* __restrict__ part, int slices,
+                                                           int64_t n, uint8_t* __restrict__ bitmap,
+                                                           unsigned long long* partials) {
+  const int64_t nwords = (n + 63) >> 6;
+  unsigned long long count = 0;
+  for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t word = 0;
+    for (int q = 0; q < slices; ++q) word |= part[(int64_t)q * nwords + w];
+    count += __popcll(word);
+    if (w * 64 + 64 <= n) {
+      reinterpret_cast<uint64_t*>(bitmap)[w] = word;
     } else {
-#pragma unroll
-      for (int u = 0; u < kSliceUnroll; ++u) {
-        uint64_t b = bb_block(h[u], nb_mask);
-        bool local = b - lo < cnt;
-        uint64_t g = blocks[local ? lo : b];
-        uint64_t l = s_blk[local ? b - lo : 0];
-        blk[u] = local ? l : g;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kSliceUnroll; ++u) {
-      uint64_t m = bb_mask(s_masks, h[u]);
-      uint64_t word = __ballot(in[u] && (blk[u] & m) == m);
-      if (wb + u < w1) {
-        write_bitmap_word(bitmap, wb + u, word, n, lane);
-        count += __popcll(word);
-      }
+      const int tail_bytes = (int)((n - w * 64 + 7) / 8);
+      for (int i = 0; i < tail_bytes; ++i) bitmap[w * 8 + i] = (uint8_t)(word >> (8 * i));
     }
   }
-  if (partials) block_count_store(partials, lane == 0 ? count : 0);
+  if (partials) block_count_store(partials, count);
+}
+
+template <bool HAS_L2, bool PARTIAL>
+static void launch_find(int mode, dim3 g, size_t lds, hipStream_t s, const uint64_t* blocks, uint64_t nbm,
+                        int G, const ColSet& k, int64_t n, int64_t wpg, uint64_t* part, uint8_t* bitmap,
+                        unsigned long long* parts) {
+  dim3 b(kSliceThreads);
+  switch (mode) {
+    case 1: hipLaunchKernelGGL((blocked_find_kernel<1, HAS_L2, PARTIAL>), g, b, lds, s, blocks, nbm, G, k, n, wpg, part, bitmap, parts); break;
+    case 2: hipLaunchKernelGGL((blocked_find_kernel<2, HAS_L2, PARTIAL>), g, b, lds, s, blocks, nbm, G, k, n, wpg, part, bitmap, parts); break;
+    default: hipLaunchKernelGGL((blocked_find_kernel<0, HAS_L2, PARTIAL>), g, b, lds, s, blocks, nbm, G, k, n, wpg, part, bitmap, parts); break;
+  }
 }
 
 // Find of n rows into d_bitmap (and the selected count into d_num_selected
@@ -176,25 +275,58 @@ __global__ __launch_bounds__(kSliceThreads) void blocked_find_lds_kernel(
 static fpdb_status lds_find(Context* c, const uint64_t* d_blocks, int log_num_blocks, const ColSet& k, int64_t n,
                             uint8_t* d_bitmap, int64_t* d_num_selected) {
   const int S = num_slices(log_num_blocks);
+  const int G = std::min(S, FPDB_FIND_GROUP);
+  const bool has_l2 = G < S;
   const int64_t nwords = ceil_div(n, 64);
-  const size_t lds = (size_t)(kMaskNum + (1 << std::min(kSliceLog, reachable_log_blocks(log_num_blocks)))) * 8;
-  // two workgroups per CU when their LDS allows, contiguous word ranges
-  const int per_cu = lds * 2 <= (size_t)c->max_lds_per_block ? 2 : 1;
-  int64_t grid = std::max<int64_t>(1, (int64_t)std::max(c->num_cus, 1) * per_cu);
-  int64_t words_per_block = std::max<int64_t>(kSliceUnroll, ceil_div(nwords, grid));
-  grid = ceil_div(nwords, words_per_block);
-  FPDB_CHECK(grid <= kMaxPartials, FPDB_ERR_INTERNAL, "bloom find: too many workgroups");
   const uint64_t nbm = (1ULL << log_num_blocks) - 1;
-  unsigned long long* parts = d_num_selected ? c->d_partials : nullptr;
-  ProfScope ps(c, "bloom_blocked_find");
-  dim3 g((unsigned)grid), b(kSliceThreads);
-  switch (key_mode(k)) {
-    case 1: hipLaunchKernelGGL(blocked_find_lds_kernel<1>, g, b, lds, c->stream, d_blocks, nbm, S, k, n, words_per_block, d_bitmap, parts); break;
-    case 2: hipLaunchKernelGGL(blocked_find_lds_kernel<2>, g, b, lds, c->stream, d_blocks, nbm, S, k, n, words_per_block, d_bitmap, parts); break;
-    default: hipLaunchKernelGGL(blocked_find_lds_kernel<0>, g, b, lds, c->stream, d_blocks, nbm, S, k, n, words_per_block, d_bitmap, parts); break;
+  const size_t lds = (size_t)(kMaskNum + (1 << std::min(kSliceLog, reachable_log_blocks(log_num_blocks)))) * 8;
+  FPDB_CHECK(lds <= (size_t)c->max_lds_per_block, FPDB_ERR_INTERNAL, "bloom find: filter slice exceeds LDS");
+  const int mode = key_mode(k);
+  if (G == 1) {
+    // two workgroups per CU when their LDS allows, contiguous word ranges
+    const int per_cu = lds * 2 <= (size_t)c->max_lds_per_block ? 2 : 1;
+    int64_t grid = std::max<int64_t>(1, (int64_t)std::max(c->num_cus, 1) * per_cu);
+    int64_t words_per_block = std::max<int64_t>(kSliceUnroll, ceil_div(nwords, grid));
+    grid = ceil_div(nwords, words_per_block);
+    FPDB_CHECK(grid <= kMaxPartials, FPDB_ERR_INTERNAL, "bloom find: too many workgroups");
+    unsigned long long* parts = d_num_selected ? c->d_partials : nullptr;
+    {
+      ProfScope ps(c, "bloom_blocked_find");
+      if (has_l2)
+        launch_find<true, false>(mode, dim3((unsigned)grid), lds, c->stream, d_blocks, nbm, 1, k, n,
+                                 words_per_block, nullptr, d_bitmap, parts);
+      else
+        launch_find<false, false>(mode, dim3((unsigned)grid), lds, c->stream, d_blocks, nbm, 1, k, n,
+                                  words_per_block, nullptr, d_bitmap, parts);
+    }
+    FPDB_HIP(hipGetLastError());
+    if (d_num_selected) FPDB_TRY(launch_sum_partials(c, (int)grid, d_num_selected));
+    return FPDB_OK;
   }
+  // one workgroup per CU (a slice fills most of the LDS); groups of G
+  // workgroups, 8 groups (one per XCD) per 8 * G workgroups
+  const int64_t per_round = 8LL * G;
+  int64_t rounds = std::max<int64_t>(1, std::max(c->num_cus, 1) / per_round);
+  int64_t words_per_group = std::max<int64_t>(kSliceUnroll, ceil_div(nwords, rounds * 8));
+  rounds = std::max<int64_t>(1, ceil_div(ceil_div(nwords, words_per_group), 8));
+  const int64_t grid = rounds * per_round;
+  uint64_t* part = nullptr;
+  FPDB_TRY(ctx_alloc(c, (uint64_t)G * nwords * 8, reinterpret_cast<void**>(&part)));
+  const int cgrid = grid_for(c, nwords, 256, 4);
+  {
+    ProfScope ps(c, "bloom_blocked_find");
+    if (has_l2)
+      launch_find<true, true>(mode, dim3((unsigned)grid), lds, c->stream, d_blocks, nbm, G, k, n, words_per_group,
+                              part, nullptr, nullptr);
+    else
+      launch_find<false, true>(mode, dim3((unsigned)grid), lds, c->stream, d_blocks, nbm, G, k, n, words_per_group,
+                               part, nullptr, nullptr);
+    hipLaunchKernelGGL(find_combine_kernel, dim3(cgrid), dim3(256), 0, c->stream, part, G, n, d_bitmap,
+                       d_num_selected ? c->d_partials : nullptr);
+  }
+  ctx_free(c, part);   // stream-ordered reuse: later work on the stream runs after these kernels
   FPDB_HIP(hipGetLastError());
-  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, (int)grid, d_num_selected));
+  if (d_num_selected) FPDB_TRY(launch_sum_partials(c, cgrid, d_num_selected));
   return FPDB_OK;
 }
 

@@ -79,6 +79,23 @@ def test_blocked_find_filter_slices(ctx, n_build, lnb):
     assert np.array_equal(bm2.numpy((n + 7) // 8), ref_bm)
 
 
+@pytest.mark.parametrize("n_build", [9_000, 200_000, 400_000])
+def test_blocked_find_slices_nullable_int32(ctx, n_build):
+    """Generic key path (int32 with nulls) through 1, 2 and 4 slices."""
+    rng = np.random.default_rng(n_build)
+    keys = rng.integers(-2 ** 31, 2 ** 31 - 1, n_build).astype(np.int32)
+    n = 300_017
+    probe = np.concatenate([rng.choice(keys, n // 2), rng.integers(-2 ** 31, 2 ** 31 - 1, n - n // 2)])
+    probe = probe.astype(np.int32)
+    valid = rng.random(n) > 0.05
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, keys)], n_build)
+    h = O.arrow6_hash([(probe, _valid_bits(valid))])
+    ref_bm = O.blocked_find(f.log_num_blocks, f.blocks_numpy(), h)
+    bm, cnt = f.find([DeviceColumn.from_numpy(ctx, probe, valid)], n)
+    assert np.array_equal(bm.numpy((n + 7) // 8), ref_bm)
+    assert cnt == int(np.unpackbits(ref_bm, bitorder="little")[:n].sum())
+
+
 def test_blocked_more_than_2_16_blocks_quirk(ctx):
     # 32-bit hashes address at most 2^16 blocks: a filter sized for > 524,288
     # keys never sets blocks >= 2^16 (block_id uses hash bits 16..31)
