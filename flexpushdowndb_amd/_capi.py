@@ -146,6 +146,7 @@ SIGNATURES = [
     ("fpdb_hj_build", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I, ctypes.POINTER(_P)]),
     ("fpdb_hj_table_destroy", _I32, [_P]),
     ("fpdb_hj_table_info", _I32, [_P, _PI64, _PI64, _PI64, ctypes.POINTER(_I)]),
+    ("fpdb_hj_table_form", _I32, [_P, ctypes.POINTER(_I)]),
     ("fpdb_hj_probe", _I32, [_P, _P, _I, _PCOL, _I, _I64, _PCOL, _I, _PCOL, _I, _PCOL, _I64, _PI64]),
     ("fpdb_hj_probe_async", _I32, [_P, _P, _I, _PCOL, _I, _I64, _PCOL, _I, _PCOL, _I, _PCOL, _I64, _P]),
     ("fpdb_hj_probe_count", _I32, [_P, _P, _I, _PCOL, _I, _I64, _PI64]),

@@ -179,32 +179,40 @@ __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bco
 // Every phase issues its loads unconditionally (rows past the end read the
 // last row) so none waits in its own branch.
 
-template <int KM>
-__device__ __forceinline__ void probe_keys(const KeySpec& pk, int64_t n, int64_t row0, int64_t (&rc)[kProbeItems],
-                                           int64_t (&key)[kProbeItems], bool (&live)[kProbeItems]) {
+// Keys of N rows row0 + i * stride (rows past the end read row n - 1 and are
+// not live; a null key is not live).
+template <int KM, int N, int STRIDE>
+__device__ __forceinline__ void probe_keys_n(const KeySpec& pk, int64_t n, int64_t row0, int64_t (&rc)[N],
+                                             int64_t (&key)[N], bool (&live)[N]) {
 #pragma unroll
-  for (int i = 0; i < kProbeItems; ++i) {
-    int64_t r = row0 + (int64_t)i * kProbeThreads;
+  for (int i = 0; i < N; ++i) {
+    int64_t r = row0 + (int64_t)i * STRIDE;
     rc[i] = r < n ? r : n - 1;
   }
   if (KM == 1 || KM == 2) {
-    int64_t ki[kProbeItems];
-    uint64_t kv[kProbeItems];
+    int64_t ki[N];
+    uint64_t kv[N];
 #pragma unroll
-    for (int i = 0; i < kProbeItems; ++i) ki[i] = rc[i] + pk.off[0];
-    load_zext_n<kProbeItems>(pk.v[0], KM == 1 ? 8 : 4, ki, kv);
+    for (int i = 0; i < N; ++i) ki[i] = rc[i] + pk.off[0];
+    load_zext_n<N>(pk.v[0], KM == 1 ? 8 : 4, ki, kv);
 #pragma unroll
-    for (int i = 0; i < kProbeItems; ++i) {
+    for (int i = 0; i < N; ++i) {
       key[i] = KM == 1 ? (int64_t)kv[i] : (int64_t)(int32_t)(uint32_t)kv[i];
-      live[i] = row0 + (int64_t)i * kProbeThreads < n;
+      live[i] = row0 + (int64_t)i * STRIDE < n;
     }
   } else {
 #pragma unroll
-    for (int i = 0; i < kProbeItems; ++i) {
+    for (int i = 0; i < N; ++i) {
       key[i] = 0;
-      live[i] = row0 + (int64_t)i * kProbeThreads < n && canon_key(pk, rc[i], &key[i]);
+      live[i] = row0 + (int64_t)i * STRIDE < n && canon_key(pk, rc[i], &key[i]);
     }
   }
+}
+
+template <int KM>
+__device__ __forceinline__ void probe_keys(const KeySpec& pk, int64_t n, int64_t row0, int64_t (&rc)[kProbeItems],
+                                           int64_t (&key)[kProbeItems], bool (&live)[kProbeItems]) {
+  probe_keys_n<KM, kProbeItems, kProbeThreads>(pk, n, row0, rc, key, live);
 }
 
 template <int JT>
@@ -296,31 +304,46 @@ __global__ __launch_bounds__(256) void hj_match_kernel(const Slot* __restrict__ 
 }
 
 // Exclusive scan of the tile counts (one workgroup); *total = the output rows.
+// Each thread owns kScanPerThread consecutive counts, all loaded before any
+// is used, so a scan of up to 1024 * kScanPerThread tiles waits for memory
+// once; longer inputs are taken in chunks of that size.
+constexpr int kScanPerThread = 16;
 __global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ cnt, int64_t tiles,
                                                          uint64_t* __restrict__ base, int64_t* total) {
   __shared__ uint64_t s_w[16];
-  __shared__ uint64_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (int64_t t0 = 0; t0 < tiles; t0 += blockDim.x) {
-    int64_t t = t0 + threadIdx.x;
-    uint64_t v = t < tiles ? cnt[t] : 0;
-    uint64_t incl = v;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  uint64_t carry = 0;
+  for (int64_t c0 = 0; c0 < tiles; c0 += (int64_t)blockDim.x * kScanPerThread) {
+    const int64_t t0 = c0 + (int64_t)threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread];
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) v[j] = t0 + j < tiles ? cnt[t0 + j] : 0u;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) mine += v[j];
+    uint64_t incl = mine;
     for (int off = 1; off < 64; off <<= 1) {
       uint64_t u = __shfl_up(incl, off);
-      if ((threadIdx.x & 63) >= off) incl += u;
+      if (lane >= off) incl += u;
     }
-    if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+    if (lane == 63) s_w[wave] = incl;
     __syncthreads();
-    uint64_t wprefix = 0;
-    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) wprefix += s_w[w];
-    uint64_t carry = s_carry;
-    if (t < tiles) base[t] = carry + wprefix + incl - v;
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) s_carry = carry + wprefix + incl;
+    uint64_t before = carry, chunk = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      if (w < wave) before += s_w[w];
+      chunk += s_w[w];
+    }
+    before += incl - mine;
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) {
+      if (t0 + j < tiles) base[t0 + j] = before;
+      before += v[j];
+    }
+    carry += chunk;
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = (int64_t)s_carry;
+  if (threadIdx.x == 0) *total = (int64_t)carry;
 }
 
 template <int KM, int JT>
@@ -440,6 +463,350 @@ __global__ __launch_bounds__(256) void hj_emit_kernel(const Slot* __restrict__ s
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dense (direct-index) tables
+//
+// A single integer key column whose non-null keys are distinct and span a
+// range [kmin, kmax] of at most twice as many values as there are keys
+// (surrogate keys, permutations) gets a collision-free table indexed by
+// slot = key - kmin: build row (kEmptyRow where no key), inline payload, and a
+// presence bitmap of one bit per slot.  No hashing, no chains.  The build runs
+// without atomics: min/max by per-workgroup partials, a plain scatter, then a
+// verify pass that flags a duplicate key (two rows scattered to one slot: one
+// of them does not find itself there), then the bitmap.
+struct DenseMeta {
+  int64_t kmin, kmax, count, dup;
+};
+
+template <int KM>
+__global__ __launch_bounds__(256) void hj_minmax_kernel(KeySpec k, int64_t n, int64_t* __restrict__ partials) {
+  int64_t lo = INT64_MAX, hi = INT64_MIN, cnt = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    int64_t key;
+    if (!get_key<KM>(k, r, &key)) continue;
+    lo = key < lo ? key : lo;
+    hi = key > hi ? key : hi;
+    ++cnt;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    int64_t l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+    cnt += __shfl_xor(cnt, off);
+  }
+  __shared__ int64_t s[3][4];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s[0][wave] = lo; s[1][wave] = hi; s[2][wave] = cnt; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = s[0][w] < lo ? s[0][w] : lo;
+      hi = s[1][w] > hi ? s[1][w] : hi;
+      cnt += s[2][w];
+    }
+    partials[3 * blockIdx.x] = lo;
+    partials[3 * blockIdx.x + 1] = hi;
+    partials[3 * blockIdx.x + 2] = cnt;
+  }
+}
+
+__global__ __launch_bounds__(256) void hj_minmax_reduce_kernel(const int64_t* __restrict__ partials, int blocks,
+                                                               DenseMeta* meta) {
+  int64_t lo = INT64_MAX, hi = INT64_MIN, cnt = 0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    lo = partials[3 * b] < lo ? partials[3 * b] : lo;
+    hi = partials[3 * b + 1] > hi ? partials[3 * b + 1] : hi;
+    cnt += partials[3 * b + 2];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    int64_t l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+    cnt += __shfl_xor(cnt, off);
+  }
+  __shared__ int64_t s[3][4];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s[0][wave] = lo; s[1][wave] = hi; s[2][wave] = cnt; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = s[0][w] < lo ? s[0][w] : lo;
+      hi = s[1][w] > hi ? s[1][w] : hi;
+      cnt += s[2][w];
+    }
+    meta->kmin = lo;
+    meta->kmax = hi;
+    meta->count = cnt;
+    meta->dup = 0;
+  }
+}
+
+// Number of slots the keys span (0 when they do not fit `cap` slots).
+__device__ __forceinline__ uint64_t dense_span(const DenseMeta* meta, uint64_t cap) {
+  if (meta->count == 0) return 0;
+  uint64_t span = (uint64_t)meta->kmax - (uint64_t)meta->kmin + 1;
+  return (span != 0 && span <= cap) ? span : 0;
+}
+
+// Clears the slots of the span: build row kEmptyRow (in the row array, or in
+// the row half of the {row, payload} pairs).
+__global__ __launch_bounds__(256) void hj_dense_fill_kernel(uint32_t* __restrict__ rows, uint2* __restrict__ pairs,
+                                                            uint64_t cap, const DenseMeta* meta) {
+  const uint64_t span = dense_span(meta, cap);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < span;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (pairs)
+      pairs[i] = make_uint2(kEmptyRow, 0u);
+    else
+      rows[i] = kEmptyRow;
+  }
+}
+
+// With an inline payload the scatter writes {row, payload} pairs into one
+// 8-byte array (one random store per row instead of two) and
+// hj_dense_split_kernel then separates the two halves.
+template <int KM>
+__global__ __launch_bounds__(256) void hj_dense_scatter_kernel(uint32_t* __restrict__ rows, uint2* __restrict__ pairs,
+                                                               uint64_t cap, const DenseMeta* meta, KeySpec k,
+                                                               int64_t n, const uint8_t* __restrict__ payload,
+                                                               int64_t payload_off) {
+  if (dense_span(meta, cap) == 0) return;
+  const uint64_t kmin = (uint64_t)meta->kmin;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    int64_t key;
+    if (!get_key<KM>(k, r, &key)) continue;
+    const uint64_t slot = (uint64_t)key - kmin;
+    if (pairs)
+      pairs[slot] = make_uint2((uint32_t)r, reinterpret_cast<const uint32_t*>(payload)[r + payload_off]);
+    else
+      rows[slot] = (uint32_t)r;
+  }
+}
+
+__global__ __launch_bounds__(256) void hj_dense_split_kernel(const uint2* __restrict__ pairs, uint64_t cap,
+                                                             const DenseMeta* meta, uint32_t* __restrict__ rows,
+                                                             uint32_t* __restrict__ pay) {
+  const uint64_t span = dense_span(meta, cap);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < span;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 e = pairs[i];
+    rows[i] = e.x;
+    pay[i] = e.y;
+  }
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void hj_dense_verify_kernel(const uint32_t* __restrict__ rows, uint64_t cap,
+                                                              DenseMeta* meta, KeySpec k, int64_t n) {
+  if (dense_span(meta, cap) == 0) return;
+  const uint64_t kmin = (uint64_t)meta->kmin;
+  bool dup = false;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+    int64_t key;
+    if (!get_key<KM>(k, r, &key)) continue;
+    dup |= rows[(uint64_t)key - kmin] != (uint32_t)r;
+  }
+  if (__ballot(dup) && (threadIdx.x & 63) == 0) meta->dup = 1;   // every writer stores 1
+}
+
+// Presence bitmap: bit s of word s / 64 set iff slot s holds a key.
+__global__ __launch_bounds__(256) void hj_dense_bits_kernel(const uint32_t* __restrict__ rows, uint64_t cap,
+                                                            const DenseMeta* meta, uint64_t* __restrict__ bits) {
+  const uint64_t span = dense_span(meta, cap);
+  const uint64_t words = (span + 63) >> 6;
+  const int lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t w = wave; w < words; w += nwaves) {
+    const uint64_t s = w * 64 + lane;
+    const uint64_t word = __ballot(s < span && rows[s < span ? s : 0] != kEmptyRow);
+    if (lane == 0) bits[w] = word;
+  }
+}
+
+// Dense probe, pass 1: per probe row whether its key is present (the hit
+// bitmap, 64 rows per word, LSB first) and per tile of kDenseTileRows rows its
+// output count.  The presence bitmap is staged in LDS when it fits
+// (BITS_IN_LDS), else read from L2.  Workgroups loop over tiles; one wave
+// counts one tile at a time.
+constexpr int kDenseItems = 4;
+constexpr int64_t kDenseTileRows = (int64_t)kDenseItems * kProbeThreads;
+constexpr int kDensePrefetch = 4;   // probe columns loaded before the in-tile scan
+constexpr int kDenseTileWords = (int)(kDenseTileRows / 64);
+constexpr int kDenseCountThreads = 1024;
+
+template <int KM, int JT, bool BITS_IN_LDS>
+__global__ __launch_bounds__(kDenseCountThreads) void hj_dense_count_kernel(
+    const uint64_t* __restrict__ bits, int64_t kmin, uint64_t span, KeySpec pk, int64_t n, int64_t tiles,
+    uint64_t* __restrict__ hitmap, uint32_t* __restrict__ tile_cnt) {
+  extern __shared__ uint64_t s_bits[];
+  const uint64_t nbw = (span + 63) >> 6;
+  if (BITS_IN_LDS) {
+    for (uint64_t i = threadIdx.x; i < nbw; i += blockDim.x) s_bits[i] = bits[i];
+    __syncthreads();
+  }
+  const uint64_t* __restrict__ bw = BITS_IN_LDS ? s_bits : bits;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * (kDenseCountThreads / 64) + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * (kDenseCountThreads / 64);
+  constexpr int U = kDenseTileWords < 8 ? kDenseTileWords : 8;   // words whose keys are loaded together
+  for (int64_t t = wave; t < tiles; t += nwaves) {
+    uint32_t cnt = 0;
+    for (int w0 = 0; w0 < kDenseTileWords; w0 += U) {
+      const int64_t row0 = (t * kDenseTileWords + w0) * 64 + lane;
+      int64_t key[U], rc[U];
+      bool live[U];
+      probe_keys_n<KM, U, 64>(pk, n, row0, rc, key, live);
+      uint64_t hw[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint64_t slot = (uint64_t)key[u] - (uint64_t)kmin;
+        const bool in = live[u] && slot < span;
+        const uint64_t s = in ? slot : 0;
+        hw[u] = __ballot(in && ((bw[s >> 6] >> (s & 63)) & 1));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t w = t * kDenseTileWords + w0 + u;
+        const int64_t r0 = w * 64;
+        if (lane == 0 && r0 < n) hitmap[w] = hw[u];
+        const int64_t rows_here = r0 >= n ? 0 : (n - r0 < 64 ? n - r0 : 64);
+        if (JT == kJtOuter) cnt += (uint32_t)rows_here;
+        else if (JT != kJtMark) cnt += (uint32_t)__popcll(hw[u]);
+      }
+    }
+    if (lane == 0) tile_cnt[t] = cnt;
+  }
+}
+
+// Dense probe, pass 2: a tile of kDenseTileRows probe rows (256 threads x 4,
+// coalesced) writes its output rows at the scanned tile offset, in probe-row
+// order.  The reads that do not depend on the scan -- keys, hit words, the
+// slot's build row or payload, and the first kDensePrefetch probe columns --
+// are all issued before the in-tile scan, so a wave waits for memory about
+// twice instead of once per column.
+// NEED_ROW: build columns gathered by build row, or build matches tracked.
+template <int KM, int JT, bool TRACK, bool NEED_ROW>
+__global__ __launch_bounds__(256) void hj_dense_emit_kernel(
+    const uint32_t* __restrict__ drows, const uint32_t* __restrict__ dpay, int64_t kmin, uint64_t span, KeySpec pk,
+    int64_t n, const uint64_t* __restrict__ hitmap, const uint64_t* __restrict__ tile_base, ColSet pcols,
+    ColSet bcols, OutColSet out, int64_t out_cap, uint32_t keymask, uint8_t* __restrict__ matched) {
+  __shared__ uint32_t s_cnt[kDenseItems][kProbeWaves];
+  const int64_t tile = blockIdx.x;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t row0 = tile * kDenseTileRows + threadIdx.x;
+  int64_t key[kDenseItems], rc[kDenseItems];
+  bool live[kDenseItems];
+  probe_keys_n<KM, kDenseItems, kProbeThreads>(pk, n, row0, rc, key, live);
+  uint64_t hw[kDenseItems];
+#pragma unroll
+  for (int i = 0; i < kDenseItems; ++i) hw[i] = hitmap[rc[i] >> 6];
+  // probe columns that are written whatever the scan says
+  const bool emit_cols = !(JT == kJtMark || out.n == 0);
+  uint64_t pre[kDensePrefetch][kDenseItems];
+#pragma unroll
+  for (int c = 0; c < kDensePrefetch; ++c) {
+    if (emit_cols && c < pcols.n) {
+      int64_t si[kDenseItems];
+#pragma unroll
+      for (int i = 0; i < kDenseItems; ++i) si[i] = rc[i] + pcols.offset[c];
+      load_zext_n<kDenseItems>(pcols.values[c], pcols.width[c], si, pre[c]);
+    }
+  }
+  uint32_t brow[kDenseItems], bpay[kDenseItems];
+#pragma unroll
+  for (int i = 0; i < kDenseItems; ++i) {
+    const uint64_t slot = (uint64_t)key[i] - (uint64_t)kmin;
+    const uint64_t s = slot < span ? slot : 0;
+    brow[i] = NEED_ROW ? drows[s] : 0u;
+    bpay[i] = dpay ? dpay[s] : 0u;
+  }
+  bool hit[kDenseItems];
+  uint32_t outc[kDenseItems], incl[kDenseItems];
+#pragma unroll
+  for (int i = 0; i < kDenseItems; ++i) {
+    hit[i] = live[i] && ((hw[i] >> (rc[i] & 63)) & 1);
+    outc[i] = probe_outputs<JT>(row0 + (int64_t)i * kProbeThreads < n, hit[i] ? 1u : 0u);
+    incl[i] = wave_incl_scan(outc[i]);
+    if (lane == 63) s_cnt[i][wave] = incl[i];
+  }
+  if (TRACK) {
+#pragma unroll
+    for (int i = 0; i < kDenseItems; ++i)
+      if (hit[i]) matched[brow[i]] = 1;
+  }
+  if (!emit_cols) return;
+  __syncthreads();
+  uint64_t base = tile_base[tile];
+  int64_t dst[kDenseItems];
+  bool emit[kDenseItems];
+#pragma unroll
+  for (int i = 0; i < kDenseItems; ++i) {
+    uint64_t before = base;
+    for (int w = 0; w < wave; ++w) before += s_cnt[i][w];
+    dst[i] = (int64_t)(before + incl[i] - outc[i]);
+    for (int w = 0; w < kProbeWaves; ++w) base += s_cnt[i][w];
+    emit[i] = outc[i] > 0 && dst[i] < out_cap;
+  }
+  for (int c = 0; c < pcols.n; ++c) {
+    int64_t si[kDenseItems];
+    uint64_t v[kDenseItems];
+#pragma unroll
+    for (int i = 0; i < kDenseItems; ++i) si[i] = rc[i] + pcols.offset[c];
+    if (c < kDensePrefetch) {
+#pragma unroll
+      for (int cc = 0; cc < kDensePrefetch; ++cc)
+        if (cc == c) {
+#pragma unroll
+          for (int i = 0; i < kDenseItems; ++i) v[i] = pre[cc][i];
+        }
+    } else {
+      load_zext_n<kDenseItems>(pcols.values[c], pcols.width[c], si, v);
+    }
+#pragma unroll
+    for (int i = 0; i < kDenseItems; ++i) {
+      if (!emit[i]) continue;
+      int64_t di = dst[i] + out.offset[c];
+      store_w(out.values[c], out.width[c], di, v[i]);
+      if (out.validity[c] && valid_bit(pcols.validity[c], si[i]))
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[c]) + (di >> 5), 1u << (di & 31),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (JT == kJtSemi) return;
+  for (int c = 0; c < bcols.n; ++c) {
+    const int oc = pcols.n + c;
+    const bool from_key = (keymask >> c) & 1u;
+    uint64_t v[kDenseItems];
+    int64_t bi[kDenseItems];
+#pragma unroll
+    for (int i = 0; i < kDenseItems; ++i) bi[i] = (int64_t)(hit[i] ? brow[i] : 0u) + bcols.offset[c];
+    if (from_key) {
+#pragma unroll
+      for (int i = 0; i < kDenseItems; ++i) v[i] = (uint64_t)key[i];
+    } else if (bcols.values[c] == nullptr) {
+#pragma unroll
+      for (int i = 0; i < kDenseItems; ++i) v[i] = bpay[i];
+    } else {
+      load_zext_n<kDenseItems>(bcols.values[c], bcols.width[c], bi, v);
+    }
+#pragma unroll
+    for (int i = 0; i < kDenseItems; ++i) {
+      if (!emit[i]) continue;
+      int64_t di = dst[i] + out.offset[oc];
+      store_w(out.values[oc], out.width[oc], di, hit[i] ? v[i] : 0);
+      bool valid = hit[i] && (from_key || bcols.values[c] == nullptr || valid_bit(bcols.validity[c], bi[i]));
+      if (out.validity[oc] && valid)
+        __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[oc]) + (di >> 5), 1u << (di & 31),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 __global__ void gather_kernel(ColSet in, const int64_t* __restrict__ rows, int64_t n, OutColSet out) {
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -498,6 +865,14 @@ struct fpdb_hash_table {
   bool info_ready = false;
   uint8_t* matched = nullptr;
   int64_t* meta = nullptr;    // device: failure flag, duplicate flag, inserted rows
+  // dense (direct-index) form, slot = key - dense_kmin < dense_span: build
+  // row per slot (kEmptyRow where no key), inline payload per slot, presence
+  // bitmap; slots == nullptr then
+  uint32_t* dense = nullptr;        // rows, then payloads (when has_payload)
+  uint32_t* dense_pay = nullptr;
+  uint64_t* dense_bits = nullptr;
+  int64_t dense_kmin = 0;
+  uint64_t dense_span = 0;
 };
 
 namespace fpdb {
@@ -540,6 +915,105 @@ static fpdb_status check_key_compat(const fpdb_hash_table* t, const fpdb_column*
     bool bs = type_is_signed_int(t->key_types[i]), ps = type_is_signed_int(keys[i].type);
     FPDB_CHECK(bs == ps, FPDB_ERR_INVALID, "probe and build key signedness differ");
   }
+  return FPDB_OK;
+}
+
+// Dense probe: count pass (hit bitmap + tile counts), scan, emit pass.
+static fpdb_status dense_probe(Context* c, fpdb_hash_table* t, int jt, bool track, const KeySpec& pk, int64_t n,
+                               const ColSet& pc, const ColSet& bc, const OutColSet& oc, int64_t out_cap,
+                               uint32_t keymask, int64_t* num_out, int64_t* d_num_out) {
+  const int64_t tiles = ceil_div(n, kDenseTileRows);
+  FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "hash join probe: too many rows");
+  const int64_t hwords = tiles * kDenseTileWords;
+  uint64_t* hitmap = nullptr;
+  uint32_t* tcnt = nullptr;
+  uint64_t* tbase = nullptr;
+  fpdb_status st = ctx_alloc(c, (uint64_t)hwords * 8, reinterpret_cast<void**>(&hitmap));
+  if (st == FPDB_OK) st = ctx_alloc(c, (uint64_t)tiles * 4, reinterpret_cast<void**>(&tcnt));
+  if (st == FPDB_OK) st = ctx_alloc(c, (uint64_t)tiles * 8, reinterpret_cast<void**>(&tbase));
+  auto release = [&]() {
+    if (hitmap) ctx_free(c, hitmap);
+    if (tcnt) ctx_free(c, tcnt);
+    if (tbase) ctx_free(c, tbase);
+  };
+  if (st != FPDB_OK) {
+    release();
+    return st;
+  }
+  int64_t* total = d_num_out ? d_num_out : c->d_counter;
+  const int km = key_mode_of(pk);
+  const uint64_t span = t->dense_span;
+  const size_t bits_bytes = ((span + 63) / 64) * 8;
+  const bool in_lds = bits_bytes <= std::min<size_t>(c->max_lds_per_block, 160 * 1024);
+  const int cgrid = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(c->num_cus, 1),
+                                                                 ceil_div(tiles, kDenseCountThreads / 64)));
+  const bool need_row = track || [&]() {
+    for (int i = 0; i < bc.n; ++i)
+      if (!((keymask >> i) & 1u) && bc.values[i] != nullptr) return true;
+    return false;
+  }();
+  const uint32_t* dpay = t->dense_pay;
+  {
+    ProfScope ps(c, "hj_probe");
+    dim3 cg(cgrid), cb(kDenseCountThreads);
+    const size_t lds = in_lds ? bits_bytes : 0;
+#define FPDB_DCOUNT(KM, JT)                                                                                    \
+  do {                                                                                                         \
+    if (in_lds)                                                                                                \
+      hipLaunchKernelGGL((hj_dense_count_kernel<KM, JT, true>), cg, cb, lds, c->stream, t->dense_bits,         \
+                         t->dense_kmin, span, pk, n, tiles, hitmap, tcnt);                                     \
+    else                                                                                                       \
+      hipLaunchKernelGGL((hj_dense_count_kernel<KM, JT, false>), cg, cb, 0, c->stream, t->dense_bits,          \
+                         t->dense_kmin, span, pk, n, tiles, hitmap, tcnt);                                     \
+  } while (0)
+#define FPDB_DCOUNT_JT(KM)                                 \
+  switch (jt) {                                            \
+    case kJtInner: FPDB_DCOUNT(KM, kJtInner); break;       \
+    case kJtOuter: FPDB_DCOUNT(KM, kJtOuter); break;       \
+    case kJtSemi: FPDB_DCOUNT(KM, kJtSemi); break;         \
+    default: FPDB_DCOUNT(KM, kJtMark); break;              \
+  }
+    switch (km) {
+      case 1: FPDB_DCOUNT_JT(1); break;
+      case 2: FPDB_DCOUNT_JT(2); break;
+      default: FPDB_DCOUNT_JT(0); break;
+    }
+#undef FPDB_DCOUNT_JT
+#undef FPDB_DCOUNT
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tcnt, tiles, tbase, total);
+    if ((jt != kJtMark && oc.n > 0 && out_cap > 0) || track) {
+      dim3 g((unsigned)tiles), b(kProbeThreads);
+#define FPDB_DEMIT(KM, JT, TR, NR)                                                                             \
+  hipLaunchKernelGGL((hj_dense_emit_kernel<KM, JT, TR, NR>), g, b, 0, c->stream, t->dense, dpay, t->dense_kmin, \
+                     span, pk, n, hitmap, tbase, pc, bc, oc, out_cap, keymask, t->matched)
+#define FPDB_DEMIT_JT(KM)                                                         \
+  switch (jt * 4 + (track ? 2 : 0) + (need_row ? 1 : 0)) {                       \
+    case kJtInner * 4: FPDB_DEMIT(KM, kJtInner, false, false); break;             \
+    case kJtInner * 4 + 1: FPDB_DEMIT(KM, kJtInner, false, true); break;          \
+    case kJtInner * 4 + 3: FPDB_DEMIT(KM, kJtInner, true, true); break;           \
+    case kJtOuter * 4: FPDB_DEMIT(KM, kJtOuter, false, false); break;             \
+    case kJtOuter * 4 + 1: FPDB_DEMIT(KM, kJtOuter, false, true); break;          \
+    case kJtOuter * 4 + 3: FPDB_DEMIT(KM, kJtOuter, true, true); break;           \
+    case kJtSemi * 4: FPDB_DEMIT(KM, kJtSemi, false, false); break;               \
+    case kJtSemi * 4 + 1: FPDB_DEMIT(KM, kJtSemi, false, true); break;            \
+    default: FPDB_DEMIT(KM, kJtMark, true, true); break;                          \
+  }
+      switch (km) {
+        case 1: FPDB_DEMIT_JT(1); break;
+        case 2: FPDB_DEMIT_JT(2); break;
+        default: FPDB_DEMIT_JT(0); break;
+      }
+#undef FPDB_DEMIT_JT
+#undef FPDB_DEMIT
+    }
+  }
+  release();   // stream-ordered reuse: later work on the stream runs after these kernels
+  FPDB_HIP(hipGetLastError());
+  if (d_num_out) return FPDB_OK;
+  FPDB_TRY(read_count(c, num_out));
+  if (*num_out > out_cap)
+    return fail(FPDB_ERR_CAPACITY, "hash join probe: " + std::to_string(*num_out) +
+                                       " output rows exceed the capacity of " + std::to_string(out_cap));
   return FPDB_OK;
 }
 
@@ -630,6 +1104,7 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
     for (int i = 0; i < bc.n; ++i) bc.values[i] = nullptr;
     keymask = 0;
   }
+  if (t->dense) return dense_probe(c, t, jt, track, pk, n, pc, bc, oc, out_cap, keymask, num_out, d_num_out);
   int64_t tiles = ceil_div(n, kProbeTileRows);
   FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "hash join probe: too many rows");
   int64_t* total = d_num_out ? d_num_out : c->d_counter;
@@ -706,6 +1181,84 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
   return FPDB_OK;
 }
 
+// Tries the dense form of a single-column table: min/max, fill, scatter, the
+// duplicate check and the presence bitmap all run on the device into arrays
+// sized for a span of up to twice the build rows; one read-back of
+// {kmin, kmax, count, dup} decides.  Leaves t->dense null (nothing allocated)
+// when the keys are absent, not distinct or too spread out.
+static fpdb_status try_dense_build(Context* c, fpdb_hash_table* t, const KeySpec& k, int64_t n,
+                                   const uint8_t* pv, int64_t poff) {
+  const uint64_t cap = (uint64_t)std::max<int64_t>(2 * n, 1024);
+  uint32_t* arr = nullptr;      // rows [cap], then payloads [cap]
+  uint2* pairs = nullptr;       // {row, payload} scatter target (with a payload)
+  uint64_t* bits = nullptr;
+  DenseMeta* meta = nullptr;
+  FPDB_TRY(ctx_alloc(c, cap * (pv ? 8 : 4), reinterpret_cast<void**>(&arr)));
+  fpdb_status st = ctx_alloc(c, ((cap + 63) / 64) * 8, reinterpret_cast<void**>(&bits));
+  if (st == FPDB_OK) st = ctx_alloc(c, sizeof(DenseMeta), reinterpret_cast<void**>(&meta));
+  if (st == FPDB_OK && pv) st = ctx_alloc(c, cap * 8, reinterpret_cast<void**>(&pairs));
+  auto drop = [&]() {
+    if (arr) ctx_free(c, arr);
+    if (pairs) ctx_free(c, pairs);
+    if (bits) ctx_free(c, bits);
+    if (meta) ctx_free(c, meta);
+  };
+  if (st != FPDB_OK) {
+    drop();
+    return st;
+  }
+  uint32_t* pay = pv ? arr + cap : nullptr;
+  const int g = grid_for(c, n, 256);
+  const int gf = grid_for(c, (int64_t)cap, 256);
+  int64_t* parts = reinterpret_cast<int64_t*>(c->d_partials);   // 3 words per workgroup
+  {
+    ProfScope ps(c, "hj_build");
+    switch (key_mode_of(k)) {
+#define FPDB_DENSE_BUILD(KM)                                                                                \
+  hipLaunchKernelGGL(hj_minmax_kernel<KM>, dim3(g), dim3(256), 0, c->stream, k, n, parts);                  \
+  hipLaunchKernelGGL(hj_minmax_reduce_kernel, dim3(1), dim3(256), 0, c->stream, parts, g, meta);            \
+  hipLaunchKernelGGL(hj_dense_fill_kernel, dim3(gf), dim3(256), 0, c->stream, arr, pairs, cap, meta);       \
+  hipLaunchKernelGGL(hj_dense_scatter_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, pairs, cap, meta, k,  \
+                     n, pv, poff);                                                                          \
+  if (pairs) hipLaunchKernelGGL(hj_dense_split_kernel, dim3(gf), dim3(256), 0, c->stream, pairs, cap, meta, \
+                                arr, pay);                                                                  \
+  hipLaunchKernelGGL(hj_dense_verify_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, cap, meta, k, n);   \
+  hipLaunchKernelGGL(hj_dense_bits_kernel, dim3(gf), dim3(256), 0, c->stream, arr, cap, meta, bits)
+      case 1: FPDB_DENSE_BUILD(1); break;
+      case 2: FPDB_DENSE_BUILD(2); break;
+      default: FPDB_DENSE_BUILD(0); break;
+#undef FPDB_DENSE_BUILD
+    }
+  }
+  DenseMeta h;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, meta, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  ctx_free(c, meta);
+  meta = nullptr;
+  if (pairs) ctx_free(c, pairs);   // stream-ordered: the split kernel has run
+  pairs = nullptr;
+  if (e != hipSuccess) {
+    drop();
+    return fail(FPDB_ERR_HIP, std::string("hash join dense build: ") + hipGetErrorString(e));
+  }
+  const uint64_t span = h.count > 0 ? (uint64_t)h.kmax - (uint64_t)h.kmin + 1 : 0;
+  if (h.count == 0 || span == 0 || span > cap || h.dup) {
+    drop();
+    return FPDB_OK;
+  }
+  t->dense = arr;
+  t->dense_pay = pay;
+  t->dense_bits = bits;
+  t->dense_kmin = h.kmin;
+  t->dense_span = span;
+  t->capacity = (int64_t)span;
+  t->inserted = h.count;
+  t->unique = true;
+  t->info_ready = true;
+  return FPDB_OK;
+}
+
 }  // namespace fpdb
 
 using namespace fpdb;
@@ -739,13 +1292,24 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
   t->key_offset = keys[0].offset;
   t->has_payload = payload != nullptr;
   t->payload_type = payload ? payload->type : 0;
-  fpdb_status s = ctx_alloc(c, (uint64_t)cap * sizeof(Slot), reinterpret_cast<void**>(&t->slots));
-  if (s != FPDB_OK) { delete t; return s; }
+  fpdb_status s = FPDB_OK;
   if (track_build_matches) {
     s = ctx_alloc(c, (uint64_t)std::max<int64_t>(num_rows, 1), reinterpret_cast<void**>(&t->matched));
     if (s == FPDB_OK) s = launch_fill(c, t->matched, 0, (uint64_t)std::max<int64_t>(num_rows, 1));
     if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
   }
+  const uint8_t* pv = payload ? static_cast<const uint8_t*>(payload->values) : nullptr;
+  int64_t poff = payload ? payload->offset : 0;
+  if (num_keys == 1 && num_rows > 0) {
+    s = try_dense_build(c, t, k, num_rows, pv, poff);
+    if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
+    if (t->dense) {
+      *out = t;
+      return FPDB_OK;
+    }
+  }
+  s = ctx_alloc(c, (uint64_t)cap * sizeof(Slot), reinterpret_cast<void**>(&t->slots));
+  if (s != FPDB_OK) { fpdb_hj_table_destroy(t); return s; }
   s = ctx_alloc(c, 32, reinterpret_cast<void**>(&t->meta));
   if (s == FPDB_OK) s = launch_fill(c, t->meta, 0, 32);
   if (s == FPDB_OK) s = launch_fill(c, t->slots, 0xFF, (uint64_t)cap * sizeof(Slot));
@@ -753,8 +1317,6 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
   // meta: [0] insertion failure flag, [1] duplicate-key flag, [2] inserted rows
   int* d_err = reinterpret_cast<int*>(t->meta);
   auto* d_ins = c->d_partials;
-  const uint8_t* pv = payload ? static_cast<const uint8_t*>(payload->values) : nullptr;
-  int64_t poff = payload ? payload->offset : 0;
   if (num_rows > 0) {
     dim3 g(grid_for(c, num_rows, 256)), b(256);
     ProfScope ps(c, "hj_build");
@@ -781,6 +1343,8 @@ fpdb_status fpdb_hj_build(fpdb_context* ctx, const fpdb_column* keys, int num_ke
 fpdb_status fpdb_hj_table_destroy(fpdb_hash_table* t) {
   if (!t) return FPDB_OK;
   if (t->slots) ctx_free(t->ctx, t->slots);
+  if (t->dense) ctx_free(t->ctx, t->dense);
+  if (t->dense_bits) ctx_free(t->ctx, t->dense_bits);
   if (t->matched) ctx_free(t->ctx, t->matched);
   if (t->meta) ctx_free(t->ctx, t->meta);
   delete t;
@@ -806,6 +1370,12 @@ fpdb_status fpdb_hj_table_info(fpdb_hash_table* t, int64_t* build_rows, int64_t*
   if (inserted) *inserted = t->inserted;
   if (capacity) *capacity = t->capacity;
   if (unique_keys) *unique_keys = t->unique ? 1 : 0;
+  return FPDB_OK;
+}
+
+fpdb_status fpdb_hj_table_form(fpdb_hash_table* t, int* form) {
+  FPDB_CHECK(t && form, FPDB_ERR_INVALID, "NULL argument");
+  *form = t->dense ? 1 : 0;
   return FPDB_OK;
 }
 

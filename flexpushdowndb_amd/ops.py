@@ -181,6 +181,12 @@ class HashJoinTable:
                                            ctypes.byref(u)))
         return {"build_rows": b.value, "inserted": i.value, "capacity": cap.value, "unique": bool(u.value)}
 
+    def form(self):
+        """'dense' (direct-index table) or 'hash' (open addressing)."""
+        f = ctypes.c_int()
+        C.check(C.lib().fpdb_hj_table_form(self.handle, ctypes.byref(f)))
+        return "dense" if f.value == 1 else "hash"
+
     def close(self):
         if self.handle:
             C.lib().fpdb_hj_table_destroy(self.handle)

@@ -227,8 +227,13 @@ typedef enum {
 
 typedef struct fpdb_hash_table fpdb_hash_table;
 
-/* Builds an open-addressing table in HBM over the build side's key columns
- * (one integer column, or two integer columns of <= 4 bytes each).  `payload`
+/* Builds a table in HBM over the build side's key columns (one integer
+ * column, or two integer columns of <= 4 bytes each).  A single key column
+ * whose non-null keys are distinct and span at most twice as many values as
+ * there are keys (surrogate keys) gets a dense table indexed by key - min key
+ * (no hashing, one read per probe row); any other build side gets an
+ * open-addressing hash table.  Deciding reads back one 32-byte record, so the
+ * call synchronizes the context stream once for a single-column build.  `payload`
  * (may be NULL) is one 4-byte build column stored inline in the table slots so
  * the probe reads it together with the key.  Rows with a null key are never
  * inserted.  track_build_matches != 0 keeps one flag per build row, set when a
@@ -245,6 +250,10 @@ fpdb_status fpdb_hj_table_destroy(fpdb_hash_table* table);
  * inserted keys are distinct (then a probe row has at most one match). */
 fpdb_status fpdb_hj_table_info(fpdb_hash_table* table, int64_t* build_rows, int64_t* inserted,
                                int64_t* capacity, int* unique_keys);
+/* The table's form: 0 = open-addressing hash table, 1 = dense (direct-index)
+ * table.  No counterpart in the reference (Arrow's SwissTable has one form);
+ * for tests and diagnostics. */
+fpdb_status fpdb_hj_table_form(fpdb_hash_table* table, int* form);
 
 /* Probes and materialises in one pass.  Output rows are in probe-row order
  * (the order among the matches of one probe row is unspecified).  Output

@@ -175,3 +175,111 @@ def test_attic_join_vector(ctx):
     assert cnt == 4
     assert _rows(out, cnt) == sorted([(10, 13, 16, 10, 23, 26), (10, 13, 16, 10, 25, 28),
                                       (10, 14, 17, 10, 23, 26), (10, 14, 17, 10, 25, 28)])
+
+
+def _bits(mask):
+    return np.packbits(mask, bitorder="little")
+
+
+def _check_all_join_types(ctx, bk, pk, bvalid, pvalid, want_form):
+    """Every join type of a build/probe pair against the oracle's hash join."""
+    nb, npr = len(bk), len(pk)
+    dbk = DeviceColumn.from_numpy(ctx, bk, bvalid)
+    dpk = DeviceColumn.from_numpy(ctx, pk, pvalid)
+    bidx = DeviceColumn.from_numpy(ctx, np.arange(nb, dtype=np.int64))
+    pidx = DeviceColumn.from_numpy(ctx, np.arange(npr, dtype=np.int64))
+    op, ob, bmatched = O.join_pairs(bk.astype(np.int64), pk.astype(np.int64),
+                                    None if bvalid is None else _bits(bvalid),
+                                    None if pvalid is None else _bits(pvalid), want_matched=True)
+    pmatched = np.zeros(npr, bool)
+    pmatched[op] = True
+    inner = sorted(zip(op.tolist(), ob.tolist()))
+    t = ops.HashJoinTable(ctx, [dbk], nb)
+    assert t.form() == want_form
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], npr, [pidx], [bidx, dbk])
+    assert sorted(zip(out[0].numpy(cnt).tolist(), out[1].numpy(cnt).tolist())) == inner
+    assert np.array_equal(out[2].numpy(cnt), bk[out[1].numpy(cnt)])   # build key column = matched key
+    # probe order: output rows follow the probe rows
+    assert (np.diff(out[0].numpy(cnt)) >= 0).all()
+    out, cnt = t.probe(C.JOIN_RIGHT, [dpk], npr, [pidx], [bidx])
+    assert cnt == len(op) + (~pmatched).sum()
+    p_out, b_out, b_valid = out[0].numpy(cnt), out[1].numpy(cnt), out[1].valid_mask(cnt)
+    assert sorted(zip(p_out[b_valid].tolist(), b_out[b_valid].tolist())) == inner
+    assert sorted(p_out[~b_valid].tolist()) == np.flatnonzero(~pmatched).tolist()
+    out, cnt = t.probe(C.JOIN_RIGHT_SEMI, [dpk], npr, [pidx], [])
+    assert out[0].numpy(cnt).tolist() == np.flatnonzero(pmatched).tolist()
+    for jt in (C.JOIN_LEFT, C.JOIN_FULL, C.JOIN_LEFT_SEMI):
+        tt = ops.HashJoinTable(ctx, [dbk], nb, track_build_matches=True)
+        assert tt.form() == want_form
+        out, cnt = tt.probe(jt, [dpk], npr, [pidx], [bidx])
+        if jt == C.JOIN_LEFT:
+            assert _rows(out, cnt) == inner
+        elif jt == C.JOIN_FULL:
+            assert cnt == len(op) + (~pmatched).sum()
+        else:
+            assert cnt == 0
+        rows, nr = tt.build_side_rows(matched=False)
+        assert rows.numpy(nr).tolist() == np.flatnonzero(~bmatched).tolist()
+        tt.close()
+
+
+@pytest.mark.parametrize("dtype", [np.int64, np.int32])
+def test_dense_table_all_join_types(ctx, dtype):
+    """Distinct keys with gaps over a range < 2x the rows, negative keys, nulls
+    on both sides, probe keys inside and outside the range: the dense table."""
+    rng = np.random.default_rng(21)
+    nb = 30_000
+    bk = rng.choice(np.arange(-25_000, 25_000), nb, replace=False).astype(dtype)
+    pk = rng.integers(-30_000, 30_000, 70_001).astype(dtype)
+    _check_all_join_types(ctx, bk, pk, rng.random(nb) > 0.1, rng.random(len(pk)) > 0.1, "dense")
+    _check_all_join_types(ctx, bk, pk, None, None, "dense")
+
+
+def test_dense_table_extreme_keys(ctx):
+    """Keys at the int64 limits: slot arithmetic must not wrap into the range."""
+    lo = np.iinfo(np.int64).min
+    hi = np.iinfo(np.int64).max
+    bk = np.concatenate([np.arange(hi - 999, hi + 1, dtype=np.int64)])
+    pk = np.array([hi, hi - 1, hi - 1000, lo, lo + 1, 0, -1, hi - 500] * 100, np.int64)
+    _check_all_join_types(ctx, bk, pk, None, None, "dense")
+    bk2 = np.arange(lo, lo + 1000, dtype=np.int64)
+    pk2 = np.array([lo, lo + 999, lo + 1000, hi, 0, -1, lo + 3] * 100, np.int64)
+    _check_all_join_types(ctx, bk2, pk2, None, None, "dense")
+
+
+def test_dense_falls_back_to_hash(ctx):
+    """A duplicate key or a range wider than twice the rows: the hash table,
+    with the same results."""
+    rng = np.random.default_rng(22)
+    bk = rng.permutation(20_000).astype(np.int64)
+    bk[7] = bk[19_000]                       # one duplicate
+    pk = rng.integers(-10, 20_010, 50_000).astype(np.int64)
+    _check_all_join_types(ctx, bk, pk, None, None, "hash")
+    sparse = rng.choice(100_000, 20_000, replace=False).astype(np.int64)   # span 5x the rows
+    _check_all_join_types(ctx, sparse, pk * 5, None, None, "hash")
+    # all keys null: nothing inserted, hash form
+    _check_all_join_types(ctx, bk, pk, np.zeros(len(bk), bool), None, "hash")
+
+
+def test_dense_and_hash_same_output(ctx):
+    """The same join through both forms (a duplicate forces the hash table;
+    its extra match is removed from the comparison): identical pairs."""
+    rng = np.random.default_rng(23)
+    n = 100_000
+    bk = rng.permutation(n).astype(np.int64)
+    pay = rng.integers(0, 2 ** 31, n).astype(np.int32)
+    pk = rng.integers(0, n, 300_000).astype(np.int64)
+    dbk, dpay, dpk = (DeviceColumn.from_numpy(ctx, a) for a in (bk, pay, pk))
+    td = ops.HashJoinTable(ctx, [dbk], n, payload=dpay)
+    assert td.form() == "dense"
+    out_d, cnt_d = td.probe(C.JOIN_INNER, [dpk], len(pk), [dpk], [dbk, "payload"], capacity=len(pk))
+    bk2 = np.concatenate([bk, [n + 5]])      # range 2x + a key outside: still distinct, but spread
+    bk2[-1] = 10 * n
+    pay2 = np.concatenate([pay, [7]]).astype(np.int32)
+    th = ops.HashJoinTable(ctx, [DeviceColumn.from_numpy(ctx, bk2)], n + 1, payload=DeviceColumn.from_numpy(ctx, pay2))
+    assert th.form() == "hash"
+    out_h, cnt_h = th.probe(C.JOIN_INNER, [dpk], len(pk), [dpk], [DeviceColumn.from_numpy(ctx, bk2), "payload"],
+                            capacity=len(pk))
+    assert cnt_d == cnt_h == len(pk)
+    for a, b in zip(out_d, out_h):
+        assert np.array_equal(a.numpy(cnt_d), b.numpy(cnt_h))
