@@ -54,14 +54,23 @@ def load_traffic(name):
         return json.load(f).get("bytes_per_launch")
 
 
-def bench_join(ctx, steps, warmup):
+def bench_join(ctx, steps, warmup, spread=1, w=None):
+    """spread = 1: the workload as specified (build keys a permutation of
+    0..n-1, so the table takes its dense form); spread = 3: every key k
+    becomes 3k + 1 (same rows, same matches), so the keys span 3x the build
+    rows and the table takes its open-addressing hash form."""
     from flexpushdowndb_amd import _capi as C
     from flexpushdowndb_amd import datagen
     from flexpushdowndb_amd.device import DeviceColumn
     L = C.lib()
     n_build, n_probe = 1_000_000, 6_000_000
-    log("join: generating inputs")
-    w = datagen.join_workload(n_build, n_probe)
+    if w is None:
+        log("join: generating inputs")
+        w = datagen.join_workload(n_build, n_probe)
+    if spread != 1:
+        w = dict(w)
+        w["build_key"] = w["build_key"] * spread + 1
+        w["probe_fk"] = w["probe_fk"] * spread + 1
     bk = DeviceColumn.from_numpy(ctx, w["build_key"])
     bp = DeviceColumn.from_numpy(ctx, w["build_pay"])
     pf = DeviceColumn.from_numpy(ctx, w["probe_fk"])
@@ -80,10 +89,13 @@ def bench_join(ctx, steps, warmup):
     a_out = C.columns_array([o.c() for o in outs])
     table = ctypes.c_void_p()
 
+    form = ctypes.c_int(-1)
+
     def step():
         C.check(L.fpdb_hj_build(ctx.handle, a_bkeys, 1, n_build, ctypes.byref(a_pay), 0, ctypes.byref(table)))
         C.check(L.fpdb_hj_probe_async(ctx.handle, table, C.JOIN_INNER, a_pkeys, 1, n_probe, a_pout, 3,
                                       a_bout, 2, a_out, n_probe, d_count.ptr))
+        C.check(L.fpdb_hj_table_form(table, ctypes.byref(form)))
         L.fpdb_hj_table_destroy(table)
 
     for _ in range(warmup):
@@ -103,9 +115,10 @@ def bench_join(ctx, steps, warmup):
     # validate the last step's output: closed form of the permutation build side
     cnt = int(d_count.numpy()[0])
     inv = np.empty(n_build, np.int64)
-    inv[w["build_key"]] = np.arange(n_build)
+    inv[(w["build_key"] - 1) // spread if spread != 1 else w["build_key"]] = np.arange(n_build)
+    pidx = (w["probe_fk"] - 1) // spread if spread != 1 else w["probe_fk"]
     ok = (cnt == n_probe and np.array_equal(outs[0].numpy(), w["probe_fk"])
-          and np.array_equal(outs[4].numpy(), w["build_pay"][inv[w["probe_fk"]]])
+          and np.array_equal(outs[4].numpy(), w["build_pay"][inv[pidx]])
           and np.array_equal(outs[2].numpy(), w["probe_price"])
           and np.array_equal(outs[3].numpy(), w["probe_fk"]))
     if not ok:
@@ -116,7 +129,7 @@ def bench_join(ctx, steps, warmup):
         "ms_per_step": ms_step,
         "value": n_probe / (ms_step * 1e-3) / 1e9,
         "probe_ms": probe_ms, "build_ms": build_ms, "probe_launches": probe_n,
-        "alg_bytes": alg_bytes, "inputs": w,
+        "alg_bytes": alg_bytes, "inputs": w, "table_form": "dense" if form.value == 1 else "hash",
     }
 
 
@@ -166,6 +179,22 @@ def bench_bloom(ctx, steps):
         raise SystemExit("bloom filter output does not match the oracle")
     find_bytes = n * 8 + (n + 7) // 8
     filt_bytes = n * 8 + nout.value * 8
+    # the same probe column against a filter of 100,000 of the kept keys
+    # (2^14 blocks, 128 KiB): every workgroup holds the whole filter in LDS
+    small = kept[:100_000]
+    fs = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, small)], len(small))
+    ctx.synchronize()
+    L.fpdb_profile_reset(ctx.handle)
+    L.fpdb_context_set_profiling(ctx.handle, 1)
+    for _ in range(steps):
+        C.check(L.fpdb_bloom_blocked_find(ctx.handle, fs.blocks.ptr, fs.log_num_blocks, a_keys, 1, n, bm.ptr, cnt.ptr))
+    ctx.synchronize()
+    small_ms, _ = kernel_ms(ctx, "bloom_blocked_find")
+    L.fpdb_context_set_profiling(ctx.handle, 0)
+    _, ref_s = O.blocked_build(O.arrow6_hash([(small, None)]), n_capacity=len(small))
+    if not (np.array_equal(fs.blocks_numpy(), ref_s) and np.array_equal(
+            bm.numpy(125_000), O.blocked_find(fs.log_num_blocks, ref_s, O.arrow6_hash([(w["probe"][:1_000_000], None)])))):
+        raise SystemExit("bloom filter (LDS-resident) output does not match the oracle")
     return {
         "workload": f"blocked bloom use: filter from {len(kept)} keys (2^{f.log_num_blocks} blocks), "
                     f"{n} int64 probe keys",
@@ -174,6 +203,9 @@ def bench_bloom(ctx, steps):
         "find_traffic_bytes": load_traffic("bloom_blocked_find"),
         "filter_kernel_ms": filt_ms, "filter_hbm_frac": filt_bytes / (filt_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "selected": nout.value, "insert_kernel_ms": ins_ms,
+        "lds_filter": {"workload": f"same {n} probe keys, filter from 100000 keys (2^{fs.log_num_blocks} blocks, "
+                                   "held whole in every workgroup's LDS)",
+                       "find_kernel_ms": small_ms, "find_hbm_frac": find_bytes / (small_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
 
 
@@ -369,7 +401,12 @@ def main():
         return
     if args.only in (None, "join"):
         res["join"] = bench_join(ctx, args.steps, args.warmup)
-        log(f"join: {res['join']['ms_per_step']:.4f} ms/step, probe kernel {res['join']['probe_ms']:.4f} ms")
+        log(f"join: {res['join']['ms_per_step']:.4f} ms/step, probe kernel {res['join']['probe_ms']:.4f} ms "
+            f"({res['join']['table_form']} table)")
+        hj = bench_join(ctx, args.steps, args.warmup, spread=3, w=res["join"]["inputs"])
+        res["join_hash"] = {k: v for k, v in hj.items() if k != "inputs"}
+        log(f"join (keys spread 3x): {hj['ms_per_step']:.4f} ms/step, probe kernel {hj['probe_ms']:.4f} ms "
+            f"({hj['table_form']} table)")
     if args.only in (None, "bloom"):
         res["bloom"] = bench_bloom(ctx, args.steps)
         log(f"bloom: find {res['bloom']['find_kernel_ms']:.4f} ms")
@@ -407,7 +444,17 @@ def main():
                      "kernel": "hj_probe", "kernel_ms": round(j["probe_ms"], 5),
                      "alg_bytes_per_launch": j["alg_bytes"]},
         "build_kernel_ms": round(j["build_ms"], 5),
+        "table_form": j["table_form"],
     }
+    if "join_hash" in res:
+        h = res["join_hash"]
+        line["join_hash_table"] = {
+            "workload": "the same join with every key k replaced by 3k + 1 (keys span 3x the build rows, so the "
+                        "build takes the open-addressing hash form)",
+            "value_grows_per_s": round(h["value"], 4), "ms_per_step": round(h["ms_per_step"], 5),
+            "probe_kernel_ms": round(h["probe_ms"], 5), "build_kernel_ms": round(h["build_ms"], 5),
+            "probe_hbm_frac": round(h["alg_bytes"] / (h["probe_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "table_form": h["table_form"]}
     if not args.no_cpu_baseline:
         log("cpu baseline: oracle join on the host")
         line["cpu_baseline"] = cpu_baseline_join(j["inputs"])
