@@ -4,9 +4,12 @@
 // A tile is 4 wavefronts x kCompactChunks chunks of 64 consecutive rows.  Each
 // wave evaluates its predicate for 64 rows at a time and turns the results into
 // one 64-bit word with a ballot (the LSB-first bitmap word of
-// fpdb-tuple/src/arrow/exec/BloomFilter.cpp:138-170).  Tiles are claimed by an
-// atomic ticket and chained by decoupled look-back, so each selected row is
-// written at its rank among all selected rows in one pass over the input.
+// fpdb-tuple/src/arrow/exec/BloomFilter.cpp:138-170).  The predicates here are
+// cheap to evaluate twice (a bitmap word, 64 flag bytes), so the compaction
+// runs as a count pass (selected rows per tile), a scan of the tile counts and
+// a copy pass that writes each selected row at its rank: the copy pass knows
+// its offsets before it loads anything, so its loads are never held up by a
+// chain of earlier tiles.
 #pragma once
 
 #include "fpdb_internal.hpp"
@@ -18,7 +21,7 @@ namespace fpdb {
 
 constexpr int kCompactWaves = 4;
 #ifndef FPDB_COMPACT_CHUNKS
-#define FPDB_COMPACT_CHUNKS 32   // 64-row chunks per wave (tile = 4 waves x this x 64 rows)
+#define FPDB_COMPACT_CHUNKS 16   // 64-row chunks per wave (tile = 4 waves x this x 64 rows)
 #endif
 constexpr int kCompactChunks = FPDB_COMPACT_CHUNKS;
 constexpr int64_t kCompactTileRows = kCompactWaves * kCompactChunks * 64;
@@ -55,18 +58,24 @@ __device__ __forceinline__ void copy_value(const ColSet& in, const OutColSet& ou
 
 // Selection predicates of the compaction: the 64-bit selection word of rows
 // [64 * chunk, 64 * chunk + 64), zero past row n (every lane gets the word).
+// Bytes of a bitmap word that is partial (past row n) or unaligned, kept out
+// of line so the unrolled callers stay small.
+__device__ __noinline__ uint64_t bitmap_word_slow(const uint8_t* bm, int64_t chunk, int64_t n) {
+  int64_t row0 = chunk * 64;
+  if (row0 >= n) return 0;
+  int nb = row0 + 64 <= n ? 8 : (int)((n - row0 + 7) / 8);
+  uint64_t w = 0;
+  for (int i = 0; i < nb; ++i) w |= (uint64_t)bm[chunk * 8 + i] << (8 * i);
+  int rem = (int)(n - row0);
+  return w & (rem >= 64 ? ~0ULL : ((1ULL << rem) - 1));
+}
+
 struct PredBitmap {
   const uint8_t* bm;   // LSB-first, ceil(n / 8) bytes
   __device__ __forceinline__ uint64_t word(int64_t chunk, int lane, int64_t n) const {
-    int64_t row0 = chunk * 64;
-    if (row0 + 64 <= n && (reinterpret_cast<uintptr_t>(bm) & 7) == 0)
+    if (chunk * 64 + 64 <= n && (reinterpret_cast<uintptr_t>(bm) & 7) == 0)
       return reinterpret_cast<const uint64_t*>(bm)[chunk];   // one broadcast load
-    if (row0 >= n) return 0;
-    int nb = row0 + 64 <= n ? 8 : (int)((n - row0 + 7) / 8);
-    uint64_t w = 0;
-    for (int i = 0; i < nb; ++i) w |= (uint64_t)bm[chunk * 8 + i] << (8 * i);
-    int rem = (int)(n - row0);
-    return w & (rem >= 64 ? ~0ULL : ((1ULL << rem) - 1));
+    return bitmap_word_slow(bm, chunk, n);
   }
 };
 
@@ -82,31 +91,102 @@ struct PredByteFlag {
 
 static __constant__ ExpandedMasks c_masks = expand_masks();
 
-// index_out (optional): the source row index of every selected row.
-template <class Pred, bool NEED_MASKS>
-__global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutColSet out, int64_t n,
-                                                      uint8_t* __restrict__ bitmap_out,
-                                                      int64_t* __restrict__ index_out,
-                                                      LbArgs lb, int64_t* total, int* err) {
-  __shared__ uint64_t s_masks[NEED_MASKS ? kMaskNum : 1];
-  __shared__ int64_t s_tile;
-  __shared__ uint32_t s_epoch;
-  __shared__ uint32_t s_wave_cnt[kCompactWaves];
-  __shared__ uint64_t s_prefix;
-  if (NEED_MASKS) {
-    for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
+// Exclusive scan of the tile counts (one workgroup); *total = the output rows.
+// Each thread owns kScanPerThread consecutive counts, all loaded before any
+// is used, so a scan of up to 1024 * kScanPerThread tiles waits for memory
+// once; longer inputs are taken in chunks of that size.
+constexpr int kScanPerThread = 16;
+static __global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ cnt, int64_t tiles,
+                                                         uint64_t* __restrict__ base, int64_t* total) {
+  __shared__ uint64_t s_w[16];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  uint64_t carry = 0;
+  for (int64_t c0 = 0; c0 < tiles; c0 += (int64_t)blockDim.x * kScanPerThread) {
+    const int64_t t0 = c0 + (int64_t)threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread];
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) v[j] = t0 + j < tiles ? cnt[t0 + j] : 0u;
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) mine += v[j];
+    uint64_t incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+      uint64_t u = __shfl_up(incl, off);
+      if (lane >= off) incl += u;
+    }
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    uint64_t before = carry, chunk = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      if (w < wave) before += s_w[w];
+      chunk += s_w[w];
+    }
+    before += incl - mine;
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) {
+      if (t0 + j < tiles) base[t0 + j] = before;
+      before += v[j];
+    }
+    carry += chunk;
+    __syncthreads();
   }
-  lb_begin(lb, &s_tile, &s_epoch);
-  const int64_t tile = s_tile;
-  const int64_t num_tiles = lb.num_tiles;
+  if (threadIdx.x == 0) *total = (int64_t)carry;
+}
+
+// Count pass: selected rows of each tile.
+template <class Pred>
+__global__ __launch_bounds__(256) void compact_count_kernel(Pred pred, int64_t n, uint32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t s_w[kCompactWaves];
+  const int64_t tile = blockIdx.x;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t wave_chunk0 = tile * (kCompactWaves * kCompactChunks) + wave * kCompactChunks;
+  uint64_t words[kCompactChunks];
+#pragma unroll
+  for (int c = 0; c < kCompactChunks; ++c) words[c] = pred.word(wave_chunk0 + c, lane, n);
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int c = 0; c < kCompactChunks; ++c) cnt += __popcll(words[c]);
+  if (lane == 0) s_w[wave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kCompactWaves; ++w) t += s_w[w];
+    tile_cnt[tile] = t;
+  }
+}
 
+// Rank of this lane among the set bits of a wave-uniform word below it.
+__device__ __forceinline__ uint32_t rank_below(uint64_t word) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Copy pass: each selected row of the tile written at tile_base[tile] + its
+// rank in the tile; index_out (optional) gets the source row index of every
+// selected row, bitmap_out (optional) the selection bitmap.  The selection
+// words are wave-uniform and live in scalar registers; a lane's output
+// position is the scalar running count plus an mbcnt.
+template <class Pred>
+__global__ __launch_bounds__(256) void compact_copy_kernel(Pred pred, ColSet in, OutColSet out, int64_t n,
+                                                           uint8_t* __restrict__ bitmap_out,
+                                                           int64_t* __restrict__ index_out,
+                                                           const uint64_t* __restrict__ tile_base) {
+  __shared__ uint32_t s_wave_cnt[kCompactWaves];
+  const int64_t tile = blockIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t wave_chunk0 = tile * (kCompactWaves * kCompactChunks) + wave * kCompactChunks;
   uint64_t words[kCompactChunks];
   uint32_t cnt = 0;
 #pragma unroll
-  for (int c = 0; c < kCompactChunks; ++c) words[c] = pred.word(wave_chunk0 + c, lane, n);
+  for (int c = 0; c < kCompactChunks; ++c) words[c] = wave_uniform(pred.word(wave_chunk0 + c, lane, n));
 #pragma unroll
   for (int c = 0; c < kCompactChunks; ++c) cnt += __popcll(words[c]);
   if (bitmap_out) {
@@ -116,22 +196,10 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
   }
   if (lane == 0) s_wave_cnt[wave] = cnt;
   __syncthreads();
-  if (wave == 0) {
-    uint64_t agg = 0;
-    for (int w = 0; w < kCompactWaves; ++w) agg += s_wave_cnt[w];
-    int e = 0;
-    uint64_t excl = lookback_exclusive(lb.status, tile, s_epoch, agg, &e);
-    if (lane == 0) {
-      s_prefix = excl;
-      if (e) *err = 1;
-      if (tile == num_tiles - 1) *total = (int64_t)(excl + agg);
-    }
-  }
-  __syncthreads();
-  uint64_t pos0 = s_prefix;
+  uint64_t pos0 = tile_base[tile];
   for (int w = 0; w < wave; ++w) pos0 += s_wave_cnt[w];
-  const uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
-  // Column by column, 8 chunks at a time: the 8 source values are loaded
+  pos0 = wave_uniform(pos0);
+  // Column by column, G chunks at a time: the G source values are loaded
   // together (rows past the end read the last row) before the selected lanes
   // store theirs, so no load waits in its own branch.
   constexpr int G = 8;
@@ -139,6 +207,8 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
   const int64_t last = n - 1;
   for (int col = 0; col < in.n; ++col) {
     uint64_t pos = pos0;
+    const int64_t ioff = in.offset[col];
+    const int64_t ooff = out.offset[col];
 #pragma unroll
     for (int c0 = 0; c0 < kCompactChunks; c0 += G) {
       int64_t si[G];
@@ -146,14 +216,14 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         int64_t src = (wave_chunk0 + c0 + u) * 64 + lane;
-        si[u] = (src < n ? src : last) + in.offset[col];
+        si[u] = (src < n ? src : last) + ioff;
       }
       load_zext_n<G>(in.values[col], in.width[col], si, v);
 #pragma unroll
       for (int u = 0; u < G; ++u) {
-        uint64_t wc = words[c0 + u];
+        const uint64_t wc = words[c0 + u];
         if ((wc >> lane) & 1) {
-          int64_t di = (int64_t)(pos + __popcll(wc & lt_mask)) + out.offset[col];
+          int64_t di = (int64_t)(pos + rank_below(wc)) + ooff;
           store_w(out.values[col], out.width[col], di, v[u]);
           if (out.validity[col] && valid_bit(in.validity[col], si[u]))
             __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[col]) + (di >> 5), 1u << (di & 31),
@@ -167,12 +237,11 @@ __global__ __launch_bounds__(256) void compact_kernel(Pred pred, ColSet in, OutC
     uint64_t pos = pos0;
 #pragma unroll
     for (int c = 0; c < kCompactChunks; ++c) {
-      uint64_t wc = words[c];
-      if ((wc >> lane) & 1) index_out[pos + __popcll(wc & lt_mask)] = (wave_chunk0 + c) * 64 + lane;
+      const uint64_t wc = words[c];
+      if ((wc >> lane) & 1) index_out[pos + rank_below(wc)] = (wave_chunk0 + c) * 64 + lane;
       pos += __popcll(wc);
     }
   }
-  lb_end(lb);
 }
 
 inline int grid_for(Context* c, int64_t work_items, int per_block, int blocks_per_cu = 8) {
@@ -203,6 +272,7 @@ template <class Pred, bool NEED_MASKS>
 fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const OutColSet& out,
                         int64_t n, uint8_t* bitmap_out, int64_t* index_out, int64_t* num_out,
                         int64_t* d_num_out = nullptr, const char* prof_name = "filter_compact") {
+  static_assert(!NEED_MASKS, "compaction predicates evaluate without the mask table");
   FPDB_TRY(zero_out_validity(c, in, out, n));
   if (n == 0) {
     if (d_num_out) return launch_fill(c, d_num_out, 0, 8);
@@ -211,14 +281,24 @@ fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const Ou
   }
   int64_t tiles = ceil_div(n, kCompactTileRows);
   FPDB_CHECK(tiles < (int64_t(1) << 31), FPDB_ERR_INVALID, "compaction: too many rows");
-  LbArgs lb;
-  FPDB_TRY(lookback_prepare(c, tiles, &lb));
+  uint32_t* tcnt = nullptr;
+  uint64_t* tbase = nullptr;
+  fpdb_status st = ctx_alloc(c, (uint64_t)tiles * 4, reinterpret_cast<void**>(&tcnt));
+  if (st == FPDB_OK) st = ctx_alloc(c, (uint64_t)tiles * 8, reinterpret_cast<void**>(&tbase));
+  if (st != FPDB_OK) {
+    if (tcnt) ctx_free(c, tcnt);
+    return st;
+  }
+  int64_t* total = d_num_out ? d_num_out : c->d_counter;
   {
     ProfScope ps(c, prof_name);
-    hipLaunchKernelGGL((compact_kernel<Pred, NEED_MASKS>), dim3((unsigned)tiles), dim3(256), 0,
-                       c->stream, pred, in, out, n, bitmap_out, index_out, lb,
-                       d_num_out ? d_num_out : c->d_counter, ctx_err_flag(c));
+    hipLaunchKernelGGL((compact_count_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, n, tcnt);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tcnt, tiles, tbase, total);
+    hipLaunchKernelGGL((compact_copy_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, in, out,
+                       n, bitmap_out, index_out, tbase);
   }
+  ctx_free(c, tcnt);   // stream-ordered reuse: later work on the stream runs after these kernels
+  ctx_free(c, tbase);
   FPDB_HIP(hipGetLastError());
   if (d_num_out) return FPDB_OK;
   return read_count(c, num_out);

@@ -303,49 +303,6 @@ __global__ __launch_bounds__(256) void hj_match_kernel(const Slot* __restrict__ 
   }
 }
 
-// Exclusive scan of the tile counts (one workgroup); *total = the output rows.
-// Each thread owns kScanPerThread consecutive counts, all loaded before any
-// is used, so a scan of up to 1024 * kScanPerThread tiles waits for memory
-// once; longer inputs are taken in chunks of that size.
-constexpr int kScanPerThread = 16;
-__global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ cnt, int64_t tiles,
-                                                         uint64_t* __restrict__ base, int64_t* total) {
-  __shared__ uint64_t s_w[16];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  uint64_t carry = 0;
-  for (int64_t c0 = 0; c0 < tiles; c0 += (int64_t)blockDim.x * kScanPerThread) {
-    const int64_t t0 = c0 + (int64_t)threadIdx.x * kScanPerThread;
-    uint32_t v[kScanPerThread];
-#pragma unroll
-    for (int j = 0; j < kScanPerThread; ++j) v[j] = t0 + j < tiles ? cnt[t0 + j] : 0u;
-    uint64_t mine = 0;
-#pragma unroll
-    for (int j = 0; j < kScanPerThread; ++j) mine += v[j];
-    uint64_t incl = mine;
-    for (int off = 1; off < 64; off <<= 1) {
-      uint64_t u = __shfl_up(incl, off);
-      if (lane >= off) incl += u;
-    }
-    if (lane == 63) s_w[wave] = incl;
-    __syncthreads();
-    uint64_t before = carry, chunk = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-      if (w < wave) before += s_w[w];
-      chunk += s_w[w];
-    }
-    before += incl - mine;
-#pragma unroll
-    for (int j = 0; j < kScanPerThread; ++j) {
-      if (t0 + j < tiles) base[t0 + j] = before;
-      before += v[j];
-    }
-    carry += chunk;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = (int64_t)carry;
-}
-
 template <int KM, int JT>
 __global__ __launch_bounds__(256) void hj_emit_kernel(const Slot* __restrict__ slots, uint64_t mask, KeySpec pk,
                                                       int64_t n, ColSet pcols, ColSet bcols, OutColSet out,
