@@ -334,6 +334,40 @@ struct GbGeom {
   int32_t rec_words;    // 2 + nacc
 };
 
+// Applies the operands of ITEMS rows to their LDS slots, accumulator by
+// accumulator: the operation is uniform, so its switch runs once per
+// accumulator instead of once per row.
+template <int ITEMS, int NA>
+__device__ __forceinline__ void acc_apply_rows(const LdsTable& T, const AccSpec& A, int nacc, const int (&slot)[ITEMS],
+                                               const uint64_t (&vv)[NA][ITEMS], const bool (&okk)[NA][ITEMS]) {
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    if (a >= nacc) break;
+    uint64_t* acc = T.acc + (size_t)a * T.stride;
+    switch (A.op[a]) {
+      case kAccCountAll:
+      case kAccCountValid:
+      case kAccSumI:
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          if (slot[i] >= 0 && okk[a][i] && vv[a][i])
+            atomicAdd(reinterpret_cast<unsigned long long*>(acc + slot[i]), (unsigned long long)vv[a][i]);
+        break;
+      case kAccSumF:
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          if (slot[i] >= 0 && okk[a][i])
+            atomicAdd(reinterpret_cast<double*>(acc + slot[i]), __longlong_as_double((long long)vv[a][i]));
+        break;
+      default:
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i)
+          if (slot[i] >= 0 && okk[a][i]) acc_apply(acc + slot[i], A.op[a], vv[a][i]);
+        break;
+    }
+  }
+}
+
 // Local pass: LDS aggregation of the workgroup's rows; tail rows counted per
 // bucket; the table written out slot by slot (partial[b][slot]).
 template <int NACC>
@@ -348,13 +382,17 @@ __global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, 
   for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x) lhist[p] = 0;
   __syncthreads();
 
-  const int64_t begin = (int64_t)blockIdx.x * G.per_block;
-  const int64_t end = begin + G.per_block < G.n ? begin + G.per_block : G.n;
   bool out_of_range = false;
   // ITEMS rows per thread per step, their loads issued before any LDS work
   constexpr int ITEMS = NACC > 0 ? 4 : 1;
   constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
-  for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)ITEMS * blockDim.x) {
+  // blocks of ITEMS x blockDim rows are dealt round-robin to the workgroups,
+  // so all workgroups stream through neighbouring rows (HBM page locality)
+  const int64_t brows = (int64_t)ITEMS * blockDim.x;
+  for (int64_t b0 = (int64_t)blockIdx.x * brows; b0 < G.n; b0 += (int64_t)gridDim.x * brows) {
+    const int64_t end = b0 + brows < G.n ? b0 + brows : G.n;
+    const int64_t r0 = b0 + threadIdx.x;
+    if (r0 >= end) continue;
     // rows past the range re-read its last row (r0 < end), so every load is unconditional
     int64_t rc[ITEMS];
 #pragma unroll
@@ -367,21 +405,22 @@ __global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, 
     bool okk[NA][ITEMS];
     group_keys_n<ITEMS>(keys, rc, kk, &out_of_range);
     acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
+    int slot[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       int64_t r = r0 + (int64_t)i * blockDim.x;
+      slot[i] = -2;
       if (r >= end) continue;
       uint64_t h = fmix64(kk[i]);
-      int s = lds_find_or_insert(T, kk[i], h, kLdsProbes);
-      if (s < 0) {
-        atomicAdd(&lhist[bucket_of(h, G.bucket_shift)], 1u);
-        continue;
-      }
+      slot[i] = lds_find_or_insert(T, kk[i], h, kLdsProbes);
+      if (slot[i] < 0) atomicAdd(&lhist[bucket_of(h, G.bucket_shift)], 1u);
+    }
+    acc_apply_rows<ITEMS, NA>(T, A, nacc, slot, vv, okk);
 #pragma unroll
-      for (int a = 0; a < NA; ++a)
-        if (a < nacc && okk[a][i]) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], vv[a][i]);
-      long long grow = (long long)(G.row_base + r);
-      if (grow < T.first[s]) atomicMin(&T.first[s], grow);
+    for (int i = 0; i < ITEMS; ++i) {
+      if (slot[i] < 0) continue;
+      long long grow = (long long)(G.row_base + r0 + (int64_t)i * blockDim.x);
+      if (grow < T.first[slot[i]]) atomicMin(&T.first[slot[i]], grow);
     }
   }
   if (out_of_range) *key_range = 1;
@@ -485,12 +524,16 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
   T.acc = nullptr;
   T.nslots = G.nslots;
   T.stride = stride;
-  const int64_t begin = (int64_t)blockIdx.x * G.per_block;
-  const int64_t end = begin + G.per_block < G.n ? begin + G.per_block : G.n;
   bool dummy = false;
   constexpr int ITEMS = NACC > 0 ? 4 : 1;
   constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
-  for (int64_t r0 = begin + threadIdx.x; r0 < end; r0 += (int64_t)ITEMS * blockDim.x) {
+  // blocks of ITEMS x blockDim rows are dealt round-robin to the workgroups,
+  // so all workgroups stream through neighbouring rows (HBM page locality)
+  const int64_t brows = (int64_t)ITEMS * blockDim.x;
+  for (int64_t b0 = (int64_t)blockIdx.x * brows; b0 < G.n; b0 += (int64_t)gridDim.x * brows) {
+    const int64_t end = b0 + brows < G.n ? b0 + brows : G.n;
+    const int64_t r0 = b0 + threadIdx.x;
+    if (r0 >= end) continue;
     // unconditional loads of keys and operands (tail rows touch nearly every
     // line of the value columns anyway)
     int64_t rc[ITEMS];
