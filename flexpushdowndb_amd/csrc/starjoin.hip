@@ -47,12 +47,12 @@ struct StarDimDev {
   int64_t rows;
   int64_t cst;
   int32_t op;
-  int32_t pad;
+  int32_t ht_shift;          // hash-table capacity >= kept rows << ht_shift
   uint8_t* kept;             // rows bytes
   uint64_t* fbf;             // forward filter blocks (capacity for `rows` keys)
   uint64_t* bbf;             // backward filter blocks (capacity for all fact rows)
-  StarSlot* ht;              // hash table (capacity >= 2 x rows)
-  uint64_t ht_mask;
+  StarSlot* ht;              // hash table storage (ht_mask + 1 slots); a run uses the
+  uint64_t ht_mask;          // first star_ht_mask(...) + 1 of them
   const int64_t* fk;         // fact foreign key column
 };
 
@@ -123,13 +123,17 @@ __device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const u
 // sized from the occupancy calculator).  Sense-free: a generation counter
 // advances when the last workgroup arrives.  Bounded spin: a workgroup that
 // waits too long sets the error flag and continues, so the kernel always ends.
+// Only thread 0 fences: __syncthreads orders the workgroup's stores before
+// its agent-scope release (an L2 write-back on this part), and its acquire
+// before the workgroup's later loads; a fence in every thread repeated the
+// write-back per thread.
 __device__ __forceinline__ void star_barrier(unsigned long long* st) {
   unsigned long long* count = st + 16;
   unsigned long long* gen = st + 17;
   unsigned long long* err = st + 32;
-  __threadfence();    // every thread's writes of this phase become visible device-wide
   __syncthreads();
   if (threadIdx.x == 0) {
+    __threadfence();    // the workgroup's writes of this phase become visible device-wide
     unsigned long long g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long arrived =
         __hip_atomic_fetch_add(count, 1ULL, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
@@ -146,9 +150,18 @@ __device__ __forceinline__ void star_barrier(unsigned long long* st) {
         }
       }
     }
+    __threadfence();    // and the workgroup observes the other workgroups' writes
   }
   __syncthreads();
-  __threadfence();    // and every thread observes the other workgroups' writes
+}
+
+// Hash-table mask of a dimension for this run: capacity the smallest power of
+// two >= its kept rows << ht_shift (at least 64), within the storage.
+__device__ __forceinline__ uint64_t star_ht_mask(const StarDimDev& D, const unsigned long long* state, int d) {
+  const uint64_t kept = __hip_atomic_load(state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t cap = 64;
+  while (cap < (kept << D.ht_shift)) cap <<= 1;
+  return cap - 1 < D.ht_mask ? cap - 1 : D.ht_mask;
 }
 
 __device__ __forceinline__ void ht_insert(StarSlot* ht, uint64_t mask, int64_t key, uint32_t row, uint32_t payload) {
@@ -256,15 +269,17 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
     if (threadIdx.x == 0 && s_kept)
       __hip_atomic_fetch_add(P.state + 1 + d, s_kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-    for (int64_t s = gtid; s <= (int64_t)D.ht_mask; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
   }
   star_barrier(P.state);
-  // A2: zero the forward filters at their sizes
+  // A2: zero the forward filters at their sizes; clear the used part of the
+  // hash tables
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
     int lnb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     for (int64_t b = gtid; b < (int64_t(1) << lnb); b += gthreads) D.fbf[b] = 0;
+    const uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    const int64_t hm = (int64_t)star_ht_mask(D, P.state, d);
+    for (int64_t s = gtid; s <= hm; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
   }
   star_barrier(P.state);
   // A3: insert the kept keys into the forward filters
@@ -302,11 +317,12 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
     const int64_t rounded = (P.fact_rows + step - 1) / step * step;   // whole waves iterate together
     const int64_t last = P.fact_rows - 1;
     for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * kStarItems + threadIdx.x; r0 < rounded; r0 += step) {
-      int64_t fk[kStarItems];
+      int64_t rc[kStarItems], fk[kStarItems];
 #pragma unroll
       for (int j = 0; j < kStarItems; ++j) {
         int64_t r = r0 + (int64_t)j * blockDim.x;
-        fk[j] = P.dim[d0].fk[r <= last ? r : last];
+        rc[j] = r <= last ? r : last;
+        fk[j] = P.dim[d0].fk[rc[j]];
       }
       bool pass[kStarItems];
 #pragma unroll
@@ -316,14 +332,23 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
                                  : bf_find(P.dim[d0].fbf, lnb[d0], s_masks, fk[j]);
         pass[j] = r <= last && hit;
       }
+      // later filters: rows still passing read their key, the others all read
+      // row 0 (one line), so the loads of all items are issued together with
+      // no branch; a wave with no row left stops
       for (int i = 1; i < P.ndims; ++i) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < kStarItems; ++j) any |= pass[j];
+        if (!__ballot(any)) break;
         const int d = P.order[i];
+        const int64_t* col = P.dim[d].fk;
+#pragma unroll
+        for (int j = 0; j < kStarItems; ++j) fk[j] = col[pass[j] ? rc[j] : 0];
 #pragma unroll
         for (int j = 0; j < kStarItems; ++j) {
-          if (!pass[j]) continue;
-          int64_t key = P.dim[d].fk[r0 + (int64_t)j * blockDim.x];
-          pass[j] = loff[d] >= 0 ? bf_find(s_filt + loff[d], lnb[d], s_masks, key)
-                                 : bf_find(P.dim[d].fbf, lnb[d], s_masks, key);
+          const bool hit = loff[d] >= 0 ? bf_find(s_filt + loff[d], lnb[d], s_masks, fk[j])
+                                        : bf_find(P.dim[d].fbf, lnb[d], s_masks, fk[j]);
+          pass[j] = pass[j] & hit;
         }
       }
 #pragma unroll
@@ -344,22 +369,25 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
   // C: kept dimension rows that pass the backward filter go into the hash tables
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
+    const uint64_t hm = star_ht_mask(D, P.state, d);
     for (int64_t r = gtid; r < D.rows; r += gthreads) {
       if (!D.kept[r]) continue;
       int64_t k = D.key[r];
       if (!bf_find(D.bbf, blnb, s_masks, k)) continue;
-      ht_insert(D.ht, D.ht_mask, k, (uint32_t)r, D.gattr ? (uint32_t)D.gattr[r] : 0u);
+      ht_insert(D.ht, hm, k, (uint32_t)r, D.gattr ? (uint32_t)D.gattr[r] : 0u);
     }
   }
   star_barrier(P.state);
   // D: join the survivors with every dimension and aggregate
+  uint64_t hms[kStarMaxDims];
+  for (int d = 0; d < P.ndims; ++d) hms[d] = star_ht_mask(P.dim[d], P.state, d);
   for (int64_t i = gtid; i < nsurv; i += gthreads) {
     uint32_t g[2] = {0, 0};
     bool ok = true;
     for (int j = 0; j < P.ndims && ok; ++j) {
       int d = P.order[j];
       uint32_t pay = 0;
-      ok = ht_find(P.dim[d].ht, P.dim[d].ht_mask, P.s_fk[(int64_t)d * P.fact_rows + i], &pay);
+      ok = ht_find(P.dim[d].ht, hms[d], P.s_fk[(int64_t)d * P.fact_rows + i], &pay);
       if (d == P.gdim[0]) g[0] = pay;
       if (d == P.gdim[1]) g[1] = pay;
     }
@@ -823,6 +851,7 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
     int64_t cap = 64;
     while (cap < (X.rows << s->tune.ht_load_shift)) cap <<= 1;
     X.ht_mask = (uint64_t)cap - 1;
+    X.ht_shift = s->tune.ht_load_shift;
     st = star_alloc(s, (uint64_t)std::max<int64_t>(X.rows, 1), reinterpret_cast<void**>(&X.kept));
     if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << host_log_blocks(X.rows), reinterpret_cast<void**>(&X.fbf));
     if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << blnb, reinterpret_cast<void**>(&X.bbf));
