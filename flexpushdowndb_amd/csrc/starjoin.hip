@@ -111,6 +111,7 @@ __device__ __forceinline__ void bf_insert(uint64_t* blocks, int lnb, const uint6
 constexpr int kStarFilterLdsWords = 8192;   // 64 KiB
 constexpr size_t kStarDynamicLds = kStarFilterLdsWords * 8;
 constexpr int kStarItems = 8;
+constexpr int kStarThreads = 1024;   // threads per workgroup of the persistent kernel
 
 __device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const uint64_t* s_masks, int64_t key) {
   uint32_t h = arrow6_hash_word((uint64_t)key);
@@ -243,7 +244,7 @@ __device__ __forceinline__ void append_survivor(const StarDev& P, bool pass, int
   }
 }
 
-__global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
+__global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P) {
   __shared__ uint64_t s_masks[kMaskNum];
   for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
   __syncthreads();
@@ -295,20 +296,24 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
   // takes kStarItems rows per step with the first filter's keys loaded
   // together, and only rows still passing read the next filter's keys.
   {
+    // per-dimension filter size and LDS offset (-1: read from global memory),
+    // in LDS: they are indexed by runtime dimension numbers
     extern __shared__ uint64_t s_filt[];
-    int lnb[kStarMaxDims];
-    int64_t loff[kStarMaxDims];
+    __shared__ int lnb[kStarMaxDims];
+    __shared__ int64_t loff[kStarMaxDims];
     int64_t used = 0;
     for (int i = 0; i < P.ndims; ++i) {
       int d = P.order[i];
-      lnb[d] = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-      int64_t words = int64_t(1) << lnb[d];
-      if (used + words <= kStarFilterLdsWords) {
-        loff[d] = used;
+      const int lb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      int64_t words = int64_t(1) << lb;
+      const bool fits = used + words <= kStarFilterLdsWords;
+      if (threadIdx.x == 0) {
+        lnb[d] = lb;
+        loff[d] = fits ? used : -1;
+      }
+      if (fits) {
         for (int64_t b = threadIdx.x; b < words; b += blockDim.x) s_filt[used + b] = P.dim[d].fbf[b];
         used += words;
-      } else {
-        loff[d] = -1;
       }
     }
     __syncthreads();
@@ -379,8 +384,9 @@ __global__ __launch_bounds__(256) void star_persistent_kernel(StarDev P) {
   }
   star_barrier(P.state);
   // D: join the survivors with every dimension and aggregate
-  uint64_t hms[kStarMaxDims];
-  for (int d = 0; d < P.ndims; ++d) hms[d] = star_ht_mask(P.dim[d], P.state, d);
+  __shared__ uint64_t hms[kStarMaxDims];
+  if (threadIdx.x < P.ndims) hms[threadIdx.x] = star_ht_mask(P.dim[threadIdx.x], P.state, threadIdx.x);
+  __syncthreads();
   for (int64_t i = gtid; i < nsurv; i += gthreads) {
     uint32_t g[2] = {0, 0};
     bool ok = true;
@@ -764,7 +770,7 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
   FPDB_HIP(hipGetLastError());
   {
     ProfScope ps(c, "star_persistent");
-    hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(256), kStarDynamicLds, c->stream, s->dev);
+    hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(kStarThreads), kStarDynamicLds, c->stream, s->dev);
   }
   FPDB_HIP(hipGetLastError());
   hipLaunchKernelGGL(star_emit_kernel, dim3(1), dim3(1024), 0, c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt,
@@ -881,7 +887,8 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   }
   // persistent grid: every workgroup resident at once
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, star_persistent_kernel, 256, kStarDynamicLds);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, star_persistent_kernel, kStarThreads,
+                                                              kStarDynamicLds);
   if (e != hipSuccess || per_cu < 1) {
     fpdb_star_destroy(s);
     return fail(FPDB_ERR_HIP, "star join: occupancy query failed");
