@@ -1,6 +1,8 @@
 // Context, device memory accounting, copies, timing and column descriptors.
 #include "fpdb_internal.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 
 namespace fpdb {
@@ -137,14 +139,43 @@ fpdb_status ctx_free(Context* c, void* p) {
   return FPDB_OK;
 }
 
+// One wave: the values, then (after a system-scope fence that waits for the
+// wave's stores) the sequence number.
+__global__ void mail_kernel(const uint64_t* __restrict__ src, int nwords, uint64_t* mail, uint64_t seq) {
+  const int lane = threadIdx.x;
+  if (lane < nwords) __hip_atomic_store(mail + 1 + lane, src[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __threadfence_system();
+  if (lane == 0) __hip_atomic_store(mail, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+fpdb_status mail_read(Context* c, const void* d_src, int nwords, uint64_t* out) {
+  FPDB_CHECK(nwords >= 0 && nwords <= 7, FPDB_ERR_INTERNAL, "mail_read: too many words");
+  const uint64_t seq = ++c->mail_seq;
+  hipLaunchKernelGGL(mail_kernel, dim3(1), dim3(64), 0, c->stream, static_cast<const uint64_t*>(d_src), nwords,
+                     c->d_mail, seq);
+  FPDB_HIP(hipGetLastError());
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint64_t i = 0;; ++i) {
+    if (__atomic_load_n(c->h_mail, __ATOMIC_ACQUIRE) == seq) break;
+    if ((i & 4095) == 4095 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) {
+      FPDB_HIP(hipStreamSynchronize(c->stream));
+      FPDB_CHECK(__atomic_load_n(c->h_mail, __ATOMIC_ACQUIRE) == seq, FPDB_ERR_INTERNAL,
+                 "read-back did not arrive after the stream completed");
+      break;
+    }
+  }
+  for (int i = 0; i < nwords; ++i) out[i] = __atomic_load_n(c->h_mail + 1 + i, __ATOMIC_RELAXED);
+  return FPDB_OK;
+}
+
 fpdb_status read_count(Context* c, int64_t* count) {
-  FPDB_HIP(hipMemcpyAsync(c->h_counter, c->d_counter, 16, hipMemcpyDeviceToHost, c->stream));
-  FPDB_HIP(hipStreamSynchronize(c->stream));
-  if (reinterpret_cast<int*>(c->h_counter + 1)[0] != 0) {
+  uint64_t v[2];
+  FPDB_TRY(mail_read(c, c->d_counter, 2, v));
+  if ((int)(uint32_t)v[1] != 0) {
     FPDB_TRY(launch_fill(c, c->d_counter + 1, 0, 8));
     return fail(FPDB_ERR_INTERNAL, "a kernel reported a failure (look-back did not complete or a table overflowed)");
   }
-  if (count) *count = c->h_counter[0];
+  if (count) *count = (int64_t)v[0];
   return FPDB_OK;
 }
 
@@ -298,6 +329,9 @@ fpdb_status fpdb_context_create(int device, uint64_t hbm_budget_bytes, void* str
   if (s == FPDB_OK) s = launch_fill(c, c->d_counter, 0, 64);
   if (s != FPDB_OK) { delete c; return s; }
   FPDB_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->h_counter), 64, hipHostMallocDefault));
+  FPDB_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->h_mail), 64, hipHostMallocCoherent | hipHostMallocMapped));
+  std::memset(c->h_mail, 0, 64);
+  FPDB_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_mail), c->h_mail, 0));
   *out = reinterpret_cast<fpdb_context*>(c);
   return FPDB_OK;
 }
@@ -316,6 +350,7 @@ fpdb_status fpdb_context_destroy(fpdb_context* ctx) {
   }
   for (auto e : c->event_pool) (void)hipEventDestroy(e);
   if (c->h_counter) (void)hipHostFree(c->h_counter);
+  if (c->h_mail) (void)hipHostFree(c->h_mail);
   if (c->ev_start) (void)hipEventDestroy(c->ev_start);
   if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);

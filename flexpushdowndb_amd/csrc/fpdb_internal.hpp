@@ -98,6 +98,11 @@ struct Context {
   int64_t* d_counter = nullptr;     // small device scratch for counts (8 words)
   unsigned long long* d_partials = nullptr;   // per-workgroup partial counts (kMaxPartials)
   int64_t* h_counter = nullptr;     // pinned host mirror
+  // read-back mailbox: pinned, coherent host memory the device writes
+  // (word 0 a sequence number, words 1..7 values); see mail_read
+  uint64_t* h_mail = nullptr;
+  uint64_t* d_mail = nullptr;       // the same memory, device view
+  uint64_t mail_seq = 0;
 };
 
 fpdb_status ctx_alloc(Context* c, uint64_t bytes, void** out);
@@ -142,6 +147,14 @@ inline int* ctx_err_flag(Context* c) { return reinterpret_cast<int*>(c->d_counte
 // Synchronizes, reads the count a kernel left in d_counter[0] and checks (and
 // clears) the failure flag.
 fpdb_status read_count(Context* c, int64_t* count);
+// Reads nwords (<= 7) 8-byte words from device memory once the work queued on
+// the context stream before it has run: a one-wave kernel stores them and then
+// a sequence number into pinned host memory, and the host spins on the
+// sequence number -- about the latency of the kernel itself, where a copy and
+// a stream synchronize wait for the host thread to be woken.  After 50 ms of
+// spinning it falls back to synchronizing the stream (which reports a failed
+// kernel).
+fpdb_status mail_read(Context* c, const void* d_src, int nwords, uint64_t* out);
 
 inline Context* as_ctx(fpdb_context* c) { return reinterpret_cast<Context*>(c); }
 

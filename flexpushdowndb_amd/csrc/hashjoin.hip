@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void hj_dense_fill_kernel(uint32_t* __restrict
 
 // With an inline payload the scatter writes {row, payload} pairs into one
 // 8-byte array (one random store per row instead of two) and
-// hj_dense_split_kernel then separates the two halves.
+// hj_dense_finish_kernel then separates the two halves.
 template <int KM>
 __global__ __launch_bounds__(256) void hj_dense_scatter_kernel(uint32_t* __restrict__ rows, uint2* __restrict__ pairs,
                                                                uint64_t cap, const DenseMeta* meta, KeySpec k,
@@ -542,46 +542,49 @@ __global__ __launch_bounds__(256) void hj_dense_scatter_kernel(uint32_t* __restr
   }
 }
 
-__global__ __launch_bounds__(256) void hj_dense_split_kernel(const uint2* __restrict__ pairs, uint64_t cap,
-                                                             const DenseMeta* meta, uint32_t* __restrict__ rows,
-                                                             uint32_t* __restrict__ pay) {
-  const uint64_t span = dense_span(meta, cap);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < span;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint2 e = pairs[i];
-    rows[i] = e.x;
-    pay[i] = e.y;
-  }
-}
-
+// After the scatter, in one launch: the slots split into the row and payload
+// arrays (when the scatter wrote pairs), the presence bitmap (bit s of word
+// s / 64 set iff slot s holds a key), and the duplicate check (a duplicate
+// key leaves one of its rows not finding itself in its slot).
 template <int KM>
-__global__ __launch_bounds__(256) void hj_dense_verify_kernel(const uint32_t* __restrict__ rows, uint64_t cap,
-                                                              DenseMeta* meta, KeySpec k, int64_t n) {
-  if (dense_span(meta, cap) == 0) return;
-  const uint64_t kmin = (uint64_t)meta->kmin;
-  bool dup = false;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-    int64_t key;
-    if (!get_key<KM>(k, r, &key)) continue;
-    dup |= rows[(uint64_t)key - kmin] != (uint32_t)r;
-  }
-  if (__ballot(dup) && (threadIdx.x & 63) == 0) meta->dup = 1;   // every writer stores 1
-}
-
-// Presence bitmap: bit s of word s / 64 set iff slot s holds a key.
-__global__ __launch_bounds__(256) void hj_dense_bits_kernel(const uint32_t* __restrict__ rows, uint64_t cap,
-                                                            const DenseMeta* meta, uint64_t* __restrict__ bits) {
+__global__ __launch_bounds__(256) void hj_dense_finish_kernel(const uint2* __restrict__ pairs, uint64_t cap,
+                                                              DenseMeta* meta, uint32_t* __restrict__ rows,
+                                                              uint32_t* __restrict__ pay, uint64_t* __restrict__ bits,
+                                                              KeySpec k, int64_t n) {
   const uint64_t span = dense_span(meta, cap);
-  const uint64_t words = (span + 63) >> 6;
+  if (span == 0) return;
   const int lane = threadIdx.x & 63;
-  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
-  for (uint64_t w = wave; w < words; w += nwaves) {
+  const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // slots, a wave per 64 consecutive slots
+  const uint64_t words = (span + 63) >> 6;
+  for (uint64_t w = gtid >> 6; w < words; w += gthreads >> 6) {
     const uint64_t s = w * 64 + lane;
-    const uint64_t word = __ballot(s < span && rows[s < span ? s : 0] != kEmptyRow);
+    const uint64_t sc = s < span ? s : 0;
+    uint32_t row;
+    if (pairs) {
+      const uint2 e = pairs[sc];
+      row = e.x;
+      if (s < span) {
+        rows[s] = e.x;
+        pay[s] = e.y;
+      }
+    } else {
+      row = rows[sc];
+    }
+    const uint64_t word = __ballot(s < span && row != kEmptyRow);
     if (lane == 0) bits[w] = word;
   }
+  // build rows
+  const uint64_t kmin = (uint64_t)meta->kmin;
+  bool dup = false;
+  for (int64_t r = (int64_t)gtid; r < n; r += (int64_t)gthreads) {
+    int64_t key;
+    if (!get_key<KM>(k, r, &key)) continue;
+    const uint64_t slot = (uint64_t)key - kmin;
+    dup |= (pairs ? pairs[slot].x : rows[slot]) != (uint32_t)r;
+  }
+  if (__ballot(dup) && lane == 0) meta->dup = 1;   // every writer stores 1
 }
 
 // Dense probe, pass 1: per probe row whether its key is present (the hit
@@ -609,7 +612,7 @@ __global__ __launch_bounds__(kDenseCountThreads) void hj_dense_count_kernel(
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * (kDenseCountThreads / 64) + (threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * (kDenseCountThreads / 64);
-  constexpr int U = kDenseTileWords < 8 ? kDenseTileWords : 8;   // words whose keys are loaded together
+  constexpr int U = kDenseTileWords;   // a tile's keys are all loaded together
   for (int64_t t = wave; t < tiles; t += nwaves) {
     uint32_t cnt = 0;
     for (int w0 = 0; w0 < kDenseTileWords; w0 += U) {
@@ -1177,10 +1180,8 @@ static fpdb_status try_dense_build(Context* c, fpdb_hash_table* t, const KeySpec
   hipLaunchKernelGGL(hj_dense_fill_kernel, dim3(gf), dim3(256), 0, c->stream, arr, pairs, cap, meta);       \
   hipLaunchKernelGGL(hj_dense_scatter_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, pairs, cap, meta, k,  \
                      n, pv, poff);                                                                          \
-  if (pairs) hipLaunchKernelGGL(hj_dense_split_kernel, dim3(gf), dim3(256), 0, c->stream, pairs, cap, meta, \
-                                arr, pay);                                                                  \
-  hipLaunchKernelGGL(hj_dense_verify_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, cap, meta, k, n);   \
-  hipLaunchKernelGGL(hj_dense_bits_kernel, dim3(gf), dim3(256), 0, c->stream, arr, cap, meta, bits)
+  hipLaunchKernelGGL(hj_dense_finish_kernel<KM>, dim3(std::max(g, gf)), dim3(256), 0, c->stream, pairs, cap,    \
+                     meta, arr, pay, bits, k, n)
       case 1: FPDB_DENSE_BUILD(1); break;
       case 2: FPDB_DENSE_BUILD(2); break;
       default: FPDB_DENSE_BUILD(0); break;
@@ -1189,16 +1190,22 @@ static fpdb_status try_dense_build(Context* c, fpdb_hash_table* t, const KeySpec
   }
   DenseMeta h;
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(&h, meta, sizeof(h), hipMemcpyDeviceToHost, c->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  ctx_free(c, meta);
-  meta = nullptr;
-  if (pairs) ctx_free(c, pairs);   // stream-ordered: the split kernel has run
-  pairs = nullptr;
   if (e != hipSuccess) {
     drop();
     return fail(FPDB_ERR_HIP, std::string("hash join dense build: ") + hipGetErrorString(e));
   }
+  static_assert(sizeof(DenseMeta) == 32, "four words");
+  uint64_t words[4];
+  st = mail_read(c, meta, 4, words);
+  ctx_free(c, meta);
+  meta = nullptr;
+  if (pairs) ctx_free(c, pairs);   // stream-ordered: the split kernel has run
+  pairs = nullptr;
+  if (st != FPDB_OK) {
+    drop();
+    return st;
+  }
+  std::memcpy(&h, words, sizeof(h));
   const uint64_t span = h.count > 0 ? (uint64_t)h.kmax - (uint64_t)h.kmin + 1 : 0;
   if (h.count == 0 || span == 0 || span > cap || h.dup) {
     drop();
