@@ -341,6 +341,74 @@ def bench_serve(ctx, rates=SERVE_RATES):
     return out
 
 
+def bench_shuffle(ctx, steps, n=60_000_000, num_slots=64):
+    """Hash shuffle of n rows {int64 key (uniform), int32 row id, float64} on
+    the key into num_slots slots (ShuffleKernel2::shuffle).  Checked in full
+    through size-independent properties: every output row's key hashes to its
+    slot, row ids ascend within each slot (stable), and the row ids are a
+    permutation carrying their key and float64 values along."""
+    from flexpushdowndb_amd import _capi as C
+    from flexpushdowndb_amd import ops
+    from flexpushdowndb_amd.device import DeviceColumn
+    from oracle import oracle as O
+    L = C.lib()
+    log(f"shuffle: generating inputs ({n} rows)")
+    rng = np.random.default_rng(2024)
+    key = rng.integers(0, 2 ** 62, n, dtype=np.int64)
+    rid = np.arange(n, dtype=np.int32)
+    val = np.round(rng.uniform(900, 105000, n), 2)
+    cols = [DeviceColumn.from_numpy(ctx, a) for a in (key, rid, val)]
+    out = [DeviceColumn.empty(ctx, n, a.dtype) for a in (key, rid, val)]
+    a_keys, a_in = C.columns_array([cols[0].c()]), C.columns_array([c.c() for c in cols])
+    a_out = C.columns_array([o.c() for o in out])
+    d_off = ctx.empty(num_slots + 1, np.int64)
+    offsets = np.zeros(num_slots + 1, np.int64)
+    poff = offsets.ctypes.data_as(C._PI64)
+    for _ in range(2):
+        C.check(L.fpdb_shuffle(ctx.handle, a_keys, 1, a_in, 3, n, num_slots, a_out, d_off.ptr, poff))
+    L.fpdb_profile_reset(ctx.handle)
+    L.fpdb_context_set_profiling(ctx.handle, 1)
+    wall = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        C.check(L.fpdb_shuffle(ctx.handle, a_keys, 1, a_in, 3, n, num_slots, a_out, d_off.ptr, poff))
+        wall.append((time.perf_counter() - t0) * 1e3)
+    shuffle_ms, _ = kernel_ms(ctx, "shuffle")
+    scatter_ms, _ = kernel_ms(ctx, "shuffle_scatter")
+    L.fpdb_context_set_profiling(ctx.handle, 0)
+    ok_key, ok_rid, ok_val = out[0].numpy(n), out[1].numpy(n), out[2].numpy(n)
+    slot = np.repeat(np.arange(num_slots, dtype=np.uint32), np.diff(offsets))
+    ok = offsets[-1] == n and np.array_equal(O.arrow6_hash([(ok_key, None)]) % np.uint32(num_slots), slot)
+    ok = ok and np.array_equal(np.sort(ok_rid), rid)
+    ok = ok and np.array_equal(key[ok_rid], ok_key) and np.array_equal(val[ok_rid], ok_val)
+    d = np.diff(ok_rid.astype(np.int64))
+    inside = np.ones(n - 1, bool)
+    inside[offsets[1:-1] - 1] = False     # steps across a slot boundary may go down
+    ok = ok and bool(np.all(d[inside] > 0))
+    if not ok:
+        raise SystemExit("shuffle output does not match the oracle's properties")
+    alg = n * 20 * 2
+    scatter_alg = n * 20 * 2 + n        # columns read and written, plus one partition-id byte a row
+    scatter_gbs = scatter_alg / (scatter_ms * 1e-3) / 1e9
+    # CPU: the oracle's stable partition + take on a 6,000,000-row slice
+    m = 6_000_000
+    t0 = time.perf_counter()
+    order, _ = O.shuffle_order(O.arrow6_hash([(key[:m], None)]), num_slots)
+    _ = (key[:m][order], rid[:m][order], val[:m][order])
+    cpu_s = time.perf_counter() - t0
+    return {"workload": f"hash shuffle: {n} rows {{int64 key, int32, float64}} on the key into {num_slots} slots",
+            "kernels_ms": shuffle_ms, "scatter_kernel_ms": scatter_ms, "wall_ms": statistics.median(wall),
+            "grows_per_s": n / (shuffle_ms * 1e-3) / 1e9,
+            "alg_bytes": alg, "hbm_frac": alg / (shuffle_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "scatter_hbm_frac": alg / (scatter_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "kernel": "shuffle_scatter", "achieved": round(scatter_gbs, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(scatter_gbs / HBM_PEAK_GBS, 4),
+                         "traffic": load_traffic("shuffle_scatter"), "alg_bytes_per_launch": scatter_alg},
+            "cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "Grows/s", "cores": 1, "kind": "port",
+                             "sample": f"oracle stable partition (numpy argsort of hash % {num_slots}) + take "
+                                       f"of the three columns over the first {m} rows, {cpu_s:.2f} s"}}
+
+
 def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
     """A plain copy of n int64 values (8n bytes read, 8n written) with
     copy_rows_kernel: the reference point for the FETCH_SIZE / WRITE_SIZE
@@ -383,7 +451,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", choices=["join", "bloom", "groupby", "star", "serve", "calib"], default=None,
+    ap.add_argument("--only", choices=["join", "bloom", "groupby", "shuffle", "star", "serve", "calib"], default=None,
                     help="run one workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -413,6 +481,9 @@ def main():
     if args.only in (None, "groupby"):
         res["groupby"] = bench_groupby(ctx, max(2, args.steps // 5))
         log(f"groupby: consume {res['groupby']['consume_kernels_ms']:.4f} ms (local {res['groupby']['local_ms']:.3f}, scatter {res['groupby']['scatter_ms']:.3f}, compact {res['groupby']['compact_ms']:.3f}; wall {res['groupby']['consume_wall_ms']:.3f})")
+    if args.only in (None, "shuffle"):
+        res["shuffle"] = bench_shuffle(ctx, args.steps)
+        log(f"shuffle: {res['shuffle']['kernels_ms']:.4f} ms (scatter {res['shuffle']['scatter_kernel_ms']:.4f})")
     if args.only in (None, "star"):
         res["star"] = bench_star(ctx, max(3, args.steps // 2), f"gfx950-{info['num_cus']}cu")
         log(f"star: batched {res['star']['batched_ms']:.3f} ms, plain {res['star']['plain_ms']:.3f} ms")
@@ -462,6 +533,8 @@ def main():
         line["bloom_use"] = res["bloom"]
     if "groupby" in res:
         line["groupby"] = res["groupby"]
+    if "shuffle" in res:
+        line["hash_shuffle"] = res["shuffle"]
     if "star" in res:
         line["batched_star_join"] = res["star"]
     if "serve" in res:

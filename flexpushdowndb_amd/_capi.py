@@ -153,6 +153,7 @@ SIGNATURES = [
     ("fpdb_hj_build_side_rows", _I32, [_P, _P, _I, _P, _I64, _PI64]),
     ("fpdb_gather", _I32, [_P, _PCOL, _I, _P, _I64, _PCOL]),
     ("fpdb_copy_rows", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I64]),
+    ("fpdb_shuffle", _I32, [_P, _PCOL, _I, _PCOL, _I, _I64, _I, _PCOL, _P, _PI64]),
     ("fpdb_groupby_create", _I32, [_P, ctypes.POINTER(_I32), _I, ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),
     ("fpdb_groupby_create_packed", _I32, [_P, ctypes.POINTER(_I32), _I, _PI64, ctypes.POINTER(_I32),
                                           ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),

@@ -91,14 +91,19 @@ struct PredByteFlag {
 
 static __constant__ ExpandedMasks c_masks = expand_masks();
 
-// Exclusive scan of the tile counts (one workgroup); *total = the output rows.
-// Each thread owns kScanPerThread consecutive counts, all loaded before any
-// is used, so a scan of up to 1024 * kScanPerThread tiles waits for memory
-// once; longer inputs are taken in chunks of that size.
+// Exclusive scan of the tile counts (one workgroup per segment of `tiles`
+// counts: workgroup s scans cnt[s * tiles, (s + 1) * tiles) into the same
+// range of base and stores its sum in total[s]).  Each thread owns
+// kScanPerThread consecutive counts, all loaded before any is used, so a scan
+// of up to 1024 * kScanPerThread tiles waits for memory once; longer inputs
+// are taken in chunks of that size.
 constexpr int kScanPerThread = 16;
 static __global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ cnt, int64_t tiles,
                                                          uint64_t* __restrict__ base, int64_t* total) {
   __shared__ uint64_t s_w[16];
+  cnt += (int64_t)blockIdx.x * tiles;
+  base += (int64_t)blockIdx.x * tiles;
+  total += blockIdx.x;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   uint64_t carry = 0;

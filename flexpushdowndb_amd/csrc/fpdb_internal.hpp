@@ -279,6 +279,29 @@ __device__ __forceinline__ uint32_t hash_row(const ColSet& k, int64_t row) {
   return h;
 }
 
+// ---------------------------------------------------------------------------
+// key access modes: 0 = generic column set, 1 = one 8-byte column without
+// nulls, 2 = one 4-byte column without nulls.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ uint32_t key_hash(const ColSet& k, int64_t row) {
+  if constexpr (MODE == 1) {
+    return arrow6_hash_word(reinterpret_cast<const uint64_t*>(k.values[0])[row + k.offset[0]]);
+  } else if constexpr (MODE == 2) {
+    return arrow6_hash_word(reinterpret_cast<const uint32_t*>(k.values[0])[row + k.offset[0]]);
+  } else {
+    return hash_row(k, row);
+  }
+}
+
+inline int key_mode(const ColSet& k) {
+  if (k.n == 1 && k.validity[0] == nullptr) {
+    if (k.width[0] == 8) return 1;
+    if (k.width[0] == 4) return 2;
+  }
+  return 0;
+}
+
 // 64-bit finaliser (Murmur3 fmix64) used for the join and group tables.
 __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
   k ^= k >> 33;

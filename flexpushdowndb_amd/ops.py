@@ -164,6 +164,19 @@ def filter_compact(ctx, cols, n, bitmap, capacity=None):
     return out, nout.value
 
 
+def shuffle(ctx, key_cols, cols, n, num_slots):
+    """Hash shuffle (ShuffleKernel2::shuffle, ShuffleKernel2.cpp:28-126): the
+    columns reordered partition by partition, rows of partition p (key hash %
+    num_slots == p) in input order at [offsets[p], offsets[p + 1]).
+    Returns (output columns, offsets as an int64 numpy array)."""
+    out = [DeviceColumn.empty(ctx, n, c.dtype, c.validity is not None, c.type_id) for c in cols]
+    offsets = np.zeros(num_slots + 1, np.int64)
+    C.check(C.lib().fpdb_shuffle(ctx.handle, _cols(key_cols), len(key_cols), _cols(cols), len(cols), n, num_slots,
+                                 C.columns_array([o.c() for o in out]), None,
+                                 offsets.ctypes.data_as(C._PI64)))
+    return out, offsets
+
+
 class HashJoinTable:
     """Device hash table of the build side (HashJoinArrowKernel build phase)."""
 

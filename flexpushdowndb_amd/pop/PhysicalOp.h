@@ -19,7 +19,7 @@
 namespace fpdb::amd {
 
 enum class POpType {
-  HASH_JOIN_ARROW, BLOOM_FILTER_CREATE, BLOOM_FILTER_USE, GROUP, SOURCE, COLLATE, FILTER
+  HASH_JOIN_ARROW, BLOOM_FILTER_CREATE, BLOOM_FILTER_USE, GROUP, SHUFFLE, SOURCE, COLLATE, FILTER
 };
 
 enum class POpRelationshipType { Producer, Consumer };

@@ -73,10 +73,17 @@ fpdb_column Column::c() const {
   col.type = type;
   col.reserved = 0;
   col.length = length;
-  col.offset = 0;
+  col.offset = offset;
   col.values = values ? values->data() : nullptr;
   col.validity = validity ? static_cast<uint8_t*>(validity->data()) : nullptr;
   return col;
+}
+
+Column Column::slice(int64_t begin, int64_t rows) const {
+  Column s = *this;
+  s.offset = offset + begin;
+  s.length = rows;
+  return s;
 }
 
 Column Column::allocate(fpdb_context* ctx, fpdb_type type, int64_t length, bool nullable) {
@@ -226,13 +233,18 @@ std::vector<HostColumn> TupleSet::toHost() const {
     h.type = c.type;
     h.length = c.length;
     h.bytes.resize((size_t)c.length * typeWidth(c.type));
+    const int w = typeWidth(c.type);
     if (c.length)
-      checkStatus(fpdb_copy_to_host(ctx, h.bytes.data(), c.values->data(), h.bytes.size()), "copy");
+      checkStatus(fpdb_copy_to_host(ctx, h.bytes.data(), static_cast<uint8_t*>(c.values->data()) + c.offset * w,
+                                    h.bytes.size()),
+                  "copy");
     if (c.validity && c.length) {
-      std::vector<uint8_t> bits((c.length + 7) / 8);
-      checkStatus(fpdb_copy_to_host(ctx, bits.data(), c.validity->data(), bits.size()), "copy");
+      const int64_t b0 = c.offset >> 3, sh = c.offset & 7;
+      std::vector<uint8_t> bits((sh + c.length + 7) / 8);
+      checkStatus(fpdb_copy_to_host(ctx, bits.data(), static_cast<uint8_t*>(c.validity->data()) + b0, bits.size()),
+                  "copy");
       h.valid.resize(c.length);
-      for (int64_t r = 0; r < c.length; ++r) h.valid[r] = (bits[r >> 3] >> (r & 7)) & 1;
+      for (int64_t r = 0; r < c.length; ++r) h.valid[r] = (bits[(r + sh) >> 3] >> ((r + sh) & 7)) & 1;
     }
     out.push_back(std::move(h));
   }

@@ -50,16 +50,20 @@ struct Schema {
   std::string toString() const;
 };
 
-// One column of a TupleSet: a single contiguous device chunk.
+// One column of a TupleSet: a single contiguous device chunk, or a slice of
+// one (rows [offset, offset + length) of shared buffers, like arrow::Array::Slice).
 struct Column {
   fpdb_type type = FPDB_TYPE_INT64;
   int64_t length = 0;
+  int64_t offset = 0;
   std::shared_ptr<DeviceBuffer> values;
   std::shared_ptr<DeviceBuffer> validity;   // null: no nulls
   fpdb_column c() const;
 
   static Column allocate(fpdb_context* ctx, fpdb_type type, int64_t length, bool nullable);
   static Column allNull(fpdb_context* ctx, fpdb_type type, int64_t length);
+  // Rows [begin, begin + rows) of this column, sharing its buffers.
+  Column slice(int64_t begin, int64_t rows) const;
 };
 
 // Host copy of a column, for tests and display.

@@ -314,6 +314,27 @@ fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_col
 fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns,
                         const int64_t* d_rows, int64_t num_rows, fpdb_column* out);
 
+/* ---- hash shuffle ------------------------------------------------------------ */
+
+/* Splits num_rows rows into num_slots partitions: row r goes to partition
+ * hash(r) % num_slots, hash being the Arrow 6 row hash of the key columns (as
+ * fpdb_hash_keys computes it), and each partition keeps the input row order.
+ * Every column of `in` is written to the matching column of `out` (same
+ * width, room for num_rows rows) partition after partition, so partition p
+ * holds output rows [offsets[p], offsets[p + 1]).  The offsets (num_slots + 1
+ * int64) go to d_offsets (device, may be NULL) and to offsets (host, may be
+ * NULL; non-NULL synchronizes).  Output validity bitmaps (required for
+ * nullable inputs) need ceil(num_rows/32)*4 bytes and an offset that is a
+ * multiple of 32.  1 <= num_slots <= 4096.  Replaces ShuffleKernel2::shuffle
+ * (fpdb-executor/src/physical/shuffle/ShuffleKernel2.cpp:28-126: partition =
+ * hash % numSlots at :68, partitions concatenated in slot order at :85-97 and
+ * sliced into one TupleSet per slot at :113-119), which ShufflePOp::onTupleSet
+ * calls with numSlots = the number of consumers
+ * (fpdb-executor/src/physical/shuffle/ShufflePOp.cpp:135-170). */
+fpdb_status fpdb_shuffle(fpdb_context* ctx, const fpdb_column* keys, int num_keys,
+                         const fpdb_column* in, int num_columns, int64_t num_rows, int num_slots,
+                         fpdb_column* out, int64_t* d_offsets, int64_t* offsets);
+
 /* ---- hash group-by --------------------------------------------------------- */
 
 /* Aggregate kinds of fpdb::executor::physical::aggregate

@@ -224,3 +224,18 @@ def groupby(keys_u64, sums_i64=(), sums_f64=(), max_groups=None):
         "sum_f64": [osf[j, :ng].copy() for j in range(len(sf))],
         "first_row": ofr[:ng].copy(),
     }
+
+
+# --- hash shuffle -----------------------------------------------------------------
+
+def shuffle_order(hashes, num_slots):
+    """Row order and slot offsets of ShuffleKernel2::shuffle
+    (fpdb-executor/src/physical/shuffle/ShuffleKernel2.cpp:66-97): row r joins
+    partition hashes[r] % num_slots (:68), partitions keep row order and are
+    concatenated in slot order (:85-89).  Returns (row order, offsets)."""
+    part = np.asarray(hashes, dtype=np.uint32) % np.uint32(num_slots)
+    order = np.argsort(part, kind="stable")
+    counts = np.bincount(part, minlength=num_slots)
+    offsets = np.zeros(num_slots + 1, np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    return order, offsets

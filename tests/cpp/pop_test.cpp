@@ -20,6 +20,7 @@
 #include "Group.h"
 #include "HashJoin.h"
 #include "Plumbing.h"
+#include "Shuffle.h"
 
 using namespace fpdb::amd;
 
@@ -194,6 +195,95 @@ static void testGroup() {
   CHECK(h[1].int64At(0) == 27 && h[1].int64At(1) == 15);
   CHECK(h[2].int64At(0) == 2 && h[2].int64At(1) == 1);
   CHECK(h[3].doubleAt(0) == 16.0 && h[3].doubleAt(1) == 18.0);   // (16 + 17) / 2 truncates for ints
+}
+
+// Arrow 6 row hash of one non-null int64 key: bswap32 of the high 32 bits of
+// key * 0xC2B2AE3D27D4EB4F (the 8-byte branch of Hashing::HashFixed).
+static uint32_t keyHash(int64_t k) {
+  uint32_t hi = (uint32_t)(((uint64_t)k * 0xC2B2AE3D27D4EB4FULL) >> 32);
+  return (hi >> 24) | ((hi >> 8) & 0xff00u) | ((hi << 8) & 0xff0000u) | (hi << 24);
+}
+
+static void testShuffle() {
+  // ShufflePOp with three consumers: slot p gets the rows with hash % 3 == p,
+  // in input order; an empty input reaches every slot
+  const int64_t n = 5000;
+  std::vector<int64_t> k(n), v(n);
+  for (int64_t i = 0; i < n; ++i) {
+    k[i] = (i * 7919) % 1013 - 300;
+    v[i] = i;
+  }
+  Schema s;
+  s.fields = {Field{"k", FPDB_TYPE_INT64, false}, Field{"v", FPDB_TYPE_INT64, false}};
+  auto ts = TupleSet::fromHost(defaultContext(), s, {hostInt64(k), hostInt64(v)});
+  Executor ex;
+  auto src = std::make_shared<SourcePOp>("scan", std::vector<std::shared_ptr<TupleSet>>{ts, TupleSet::make(s)});
+  auto sh = std::make_shared<ShufflePOp>("shuffle", std::vector<std::string>{"k", "v"}, 0,
+                                         std::vector<std::string>{"k"});
+  ex.put(src);
+  ex.put(sh);
+  connect(src, sh);
+  std::vector<std::shared_ptr<CollatePOp>> outs;
+  for (int p = 0; p < 3; ++p) {
+    outs.push_back(std::make_shared<CollatePOp>("collate" + std::to_string(p)));
+    ex.put(outs.back());
+    connect(sh, outs.back());
+  }
+  CHECK(sh->getConsumerVec().size() == 3);
+  auto r = ex.execute();
+  CHECK(r.has_value());
+  if (!r) return;
+  int64_t total = 0;
+  for (int p = 0; p < 3; ++p) {
+    auto got = outs[p]->tupleSet();
+    CHECK(got.has_value());
+    if (!got) continue;
+    auto rows = rowsOf(*got);
+    std::vector<std::vector<int64_t>> want;
+    for (int64_t i = 0; i < n; ++i)
+      if (keyHash(k[i]) % 3 == (uint32_t)p) want.push_back({k[i], v[i]});
+    CHECK(rows == want);
+    total += (int64_t)rows.size();
+  }
+  CHECK(total == n);
+  // ShuffleKernel2 on its own: a missing key column is an error
+  auto bad = ShuffleKernel2::shuffle({"nope"}, 2, ts);
+  CHECK(!bad.has_value());
+}
+
+static void testShuffledJoin() {
+  // a two-instance hash join: both sides shuffled on the join key into two
+  // slots, one join per slot (the parallel join plans of
+  // PrePToPTransformer.cpp), with the same result as one join
+  Executor ex;
+  auto a = std::make_shared<SourcePOp>("scan_a", std::vector<std::shared_ptr<TupleSet>>{table3("a", kA)});
+  auto b = std::make_shared<SourcePOp>("scan_b", std::vector<std::shared_ptr<TupleSet>>{table3("b", kB)});
+  auto sa = std::make_shared<ShufflePOp>("shuffle_a", std::vector<std::string>{}, 0, std::vector<std::string>{"aa"});
+  auto sb = std::make_shared<ShufflePOp>("shuffle_b", std::vector<std::string>{}, 0, std::vector<std::string>{"ba"});
+  auto collate = std::make_shared<CollatePOp>("collate");
+  for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{a, b, sa, sb, collate}) ex.put(op);
+  connect(a, sa);
+  connect(b, sb);
+  for (int i = 0; i < 2; ++i) {
+    auto join = std::make_shared<HashJoinArrowPOp>("join" + std::to_string(i),
+                                                   std::vector<std::string>{"aa", "ab", "ac", "ba", "bb", "bc"}, 0,
+                                                   HashJoinPredicate{{"ba"}, {"aa"}}, INNER);
+    ex.put(join);
+    sb->produce(join);
+    join->addBuildProducer(sb);
+    sa->produce(join);
+    join->addProbeProducer(sa);
+    connect(join, collate);
+  }
+  auto r = ex.execute();
+  CHECK(r.has_value());
+  if (!r) return;
+  auto out = collate->tupleSet().value();
+  auto rows = rowsOf(out);
+  std::sort(rows.begin(), rows.end());
+  std::vector<std::vector<int64_t>> want = {{10, 13, 16, 10, 23, 26}, {10, 13, 16, 10, 25, 28},
+                                            {10, 14, 17, 10, 23, 26}, {10, 14, 17, 10, 25, 28}};
+  CHECK(rows == want);
 }
 
 static void testEmptyAndErrors() {
@@ -380,6 +470,8 @@ int main(int argc, char** argv) {
       {"bloom_join_legacy", [] { testBloomJoin(false); }},
       {"group", testGroup},
       {"empty_and_errors", testEmptyAndErrors},
+      {"shuffle", testShuffle},
+      {"shuffled_join", testShuffledJoin},
   };
   for (auto& t : tests) {
     int before = g_failures;

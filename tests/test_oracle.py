@@ -285,3 +285,41 @@ def test_groupby_closed_form_and_attic_vector():
     for j, g in enumerate(k[:50]):
         exact = math.fsum(fv[keys == g])
         assert abs(r["sum_f64"][0][j] - exact) <= 1e-12 * abs(exact)
+
+
+def _shuffle_by_appending(hashes, num_slots):
+    # the reference's loop: append each row id to its partition's list in row
+    # order, then concatenate the lists in slot order
+    parts = [[] for _ in range(num_slots)]
+    for row, h in enumerate(hashes):
+        parts[int(h) % num_slots].append(row)
+    order = [r for p in parts for r in p]
+    offsets = [0]
+    for p in parts:
+        offsets.append(offsets[-1] + len(p))
+    return order, offsets
+
+
+@pytest.mark.parametrize("n,num_slots", [(0, 3), (1, 1), (7, 10), (300, 1), (300, 2), (1000, 7), (2000, 64)])
+def test_shuffle_order_brute_force(n, num_slots):
+    rng = np.random.default_rng(n * 31 + num_slots)
+    hashes = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    order, offsets = O.shuffle_order(hashes, num_slots)
+    want_order, want_offsets = _shuffle_by_appending(hashes.tolist(), num_slots)
+    assert order.tolist() == want_order
+    assert offsets.tolist() == want_offsets
+
+
+def test_shuffle_order_properties():
+    # every row lands in partition hash % S, each partition ascends in row id,
+    # and the order is a permutation; uint32 hashes above 2^31 stay unsigned
+    hashes = np.array([0xFFFFFFFF, 5, 0x80000000, 3, 5, 0xFFFFFFFE], np.uint32)
+    order, offsets = O.shuffle_order(hashes, 3)
+    assert sorted(order.tolist()) == list(range(6))
+    for p in range(3):
+        rows = order[offsets[p]:offsets[p + 1]]
+        assert all(int(hashes[r]) % 3 == p for r in rows)
+        assert list(rows) == sorted(rows)
+    # 0xFFFFFFFF and 3 are 0 mod 3; 5, 0x80000000 and 0xFFFFFFFE are 2 mod 3
+    assert offsets.tolist() == [0, 2, 2, 6]
+    assert order.tolist() == [0, 3, 1, 2, 4, 5]
