@@ -409,6 +409,79 @@ def bench_shuffle(ctx, steps, n=60_000_000, num_slots=64):
                                        f"of the three columns over the first {m} rows, {cpu_s:.2f} s"}}
 
 
+def bench_cache(ctx, steps, w=None):
+    """The join workload's probe side served through the device segment cache:
+    a cold step stores the three probe columns from host memory (PCIe) into
+    the cache and then builds, probes and materialises; a warm step loads
+    them from the cache (a hit hands out the HBM buffers) and does the same
+    join.  The same output either way (checked)."""
+    from flexpushdowndb_amd import _capi as C
+    from flexpushdowndb_amd import datagen, ops
+    from flexpushdowndb_amd.device import DeviceColumn
+    L = C.lib()
+    n_build, n_probe = 1_000_000, 6_000_000
+    if w is None:
+        log("cache: generating inputs")
+        w = datagen.join_workload(n_build, n_probe)
+    host = {"lo_fk": np.ascontiguousarray(w["probe_fk"]), "lo_qty": np.ascontiguousarray(w["probe_qty"]),
+            "lo_price": np.ascontiguousarray(w["probe_price"])}
+    bk = DeviceColumn.from_numpy(ctx, w["build_key"])
+    bp = DeviceColumn.from_numpy(ctx, w["build_pay"])
+    outs = [DeviceColumn.empty(ctx, n_probe, np.int64), DeviceColumn.empty(ctx, n_probe, np.int32),
+            DeviceColumn.empty(ctx, n_probe, np.float64), DeviceColumn.empty(ctx, n_probe, np.int64),
+            DeviceColumn.empty(ctx, n_probe, np.int32)]
+    d_count = ctx.zeros(1, np.int64)
+    a_bkeys = C.columns_array([bk.c()])
+    a_pay = bp.c()
+    a_bout = C.columns_array([bk.c(), C.Column(C.TYPE_INT32, 0, 0, 0, None, None)])
+    a_out = C.columns_array([o.c() for o in outs])
+    table = ctypes.c_void_p()
+    cache = ops.SegmentCache(ctx, 1 << 30)
+
+    def join(cols):
+        a_pkeys = C.columns_array([cols[0].c()])
+        a_pout = C.columns_array([c.c() for c in cols])
+        C.check(L.fpdb_hj_build(ctx.handle, a_bkeys, 1, n_build, ctypes.byref(a_pay), 0, ctypes.byref(table)))
+        C.check(L.fpdb_hj_probe_async(ctx.handle, table, C.JOIN_INNER, a_pkeys, 1, n_probe, a_pout, 3,
+                                      a_bout, 2, a_out, n_probe, d_count.ptr))
+        L.fpdb_hj_table_destroy(table)
+
+    def cached_step():
+        cols = []
+        for name in host:
+            col = cache.load("lineorder/part0", name)
+            if col is None:
+                cache.store("lineorder/part0", name, 0, n_probe, host[name])
+                col = cache.load("lineorder/part0", name)
+            cols.append(col)
+        join(cols)
+        ctx.synchronize()
+        for col in cols:
+            cache.release(col)
+
+    times = {"cold": [], "warm": []}
+    for i in range(steps + 1):
+        cache.clear()
+        for kind in ("cold", "warm"):
+            t0 = time.perf_counter()
+            cached_step()
+            if i:
+                times[kind].append((time.perf_counter() - t0) * 1e3)
+    info = cache.info()
+    ok = int(d_count.numpy()[0]) == n_probe and np.array_equal(outs[0].numpy(), w["probe_fk"]) \
+        and np.array_equal(outs[2].numpy(), w["probe_price"])
+    if not ok:
+        raise SystemExit("join through the segment cache does not match the closed form")
+    cache.close()
+    cold, warm = statistics.median(times["cold"]), statistics.median(times["warm"])
+    probe_bytes = sum(a.nbytes for a in host.values())
+    return {"workload": "join workload (1,000,000-row build, 6,000,000-row probe) with the probe columns "
+                        "{int64 fk, int32 qty, float64 price} served through the device segment cache",
+            "cold_step_ms": cold, "warm_step_ms": warm, "speedup": cold / warm,
+            "segment_bytes": probe_bytes, "host_to_hbm_gbs": probe_bytes / ((cold - warm) * 1e-3) / 1e9,
+            "segments": info["segments"], "hits": info["hits"], "misses": info["misses"]}
+
+
 def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
     """A plain copy of n int64 values (8n bytes read, 8n written) with
     copy_rows_kernel: the reference point for the FETCH_SIZE / WRITE_SIZE
@@ -451,7 +524,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", choices=["join", "bloom", "groupby", "shuffle", "star", "serve", "calib"], default=None,
+    ap.add_argument("--only", choices=["join", "bloom", "groupby", "shuffle", "cache", "star", "serve", "calib"],
+                    default=None,
                     help="run one workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -484,6 +558,9 @@ def main():
     if args.only in (None, "shuffle"):
         res["shuffle"] = bench_shuffle(ctx, args.steps)
         log(f"shuffle: {res['shuffle']['kernels_ms']:.4f} ms (scatter {res['shuffle']['scatter_kernel_ms']:.4f})")
+    if args.only in (None, "cache"):
+        res["cache"] = bench_cache(ctx, max(3, args.steps // 4), res["join"]["inputs"] if "join" in res else None)
+        log(f"cache: cold {res['cache']['cold_step_ms']:.3f} ms, warm {res['cache']['warm_step_ms']:.3f} ms")
     if args.only in (None, "star"):
         res["star"] = bench_star(ctx, max(3, args.steps // 2), f"gfx950-{info['num_cus']}cu")
         log(f"star: batched {res['star']['batched_ms']:.3f} ms, plain {res['star']['plain_ms']:.3f} ms")
@@ -535,6 +612,8 @@ def main():
         line["groupby"] = res["groupby"]
     if "shuffle" in res:
         line["hash_shuffle"] = res["shuffle"]
+    if "cache" in res:
+        line["segment_cache"] = res["cache"]
     if "star" in res:
         line["batched_star_join"] = res["star"]
     if "serve" in res:

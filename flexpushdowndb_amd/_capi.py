@@ -154,6 +154,15 @@ SIGNATURES = [
     ("fpdb_gather", _I32, [_P, _PCOL, _I, _P, _I64, _PCOL]),
     ("fpdb_copy_rows", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I64]),
     ("fpdb_shuffle", _I32, [_P, _PCOL, _I, _PCOL, _I, _I64, _I, _PCOL, _P, _PI64]),
+    ("fpdb_cache_create", _I32, [_P, _U64, _I, ctypes.POINTER(_P)]),
+    ("fpdb_cache_destroy", _I32, [_P]),
+    ("fpdb_cache_store", _I32, [_P, ctypes.c_char_p, ctypes.c_char_p, _I64, _I64, _PCOL, ctypes.POINTER(_I)]),
+    ("fpdb_cache_load", _I32, [_P, ctypes.c_char_p, ctypes.c_char_p, _I64, _I64, _PCOL, ctypes.POINTER(_I)]),
+    ("fpdb_cache_release", _I32, [_P, _P]),
+    ("fpdb_cache_remove", _I32, [_P, ctypes.c_char_p, ctypes.c_char_p, _PI64]),
+    ("fpdb_cache_clear", _I32, [_P]),
+    ("fpdb_cache_info", _I32, [_P, _PI64, ctypes.POINTER(_U64), ctypes.POINTER(_U64), _PI64, _PI64]),
+    ("fpdb_cache_layout", _I32, [_P, ctypes.c_char_p, _U64, ctypes.POINTER(_U64)]),
     ("fpdb_groupby_create", _I32, [_P, ctypes.POINTER(_I32), _I, ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),
     ("fpdb_groupby_create_packed", _I32, [_P, ctypes.POINTER(_I32), _I, _PI64, ctypes.POINTER(_I32),
                                           ctypes.POINTER(AggSpec), _I, _I64, ctypes.POINTER(_P)]),

@@ -177,6 +177,99 @@ def shuffle(ctx, key_cols, cols, n, num_slots):
     return out, offsets
 
 
+class _CacheBuffer:
+    """A cache-owned device buffer (not freed by Python)."""
+
+    def __init__(self, ctx, ptr, nbytes, dtype, length):
+        self.ctx, self.ptr, self.nbytes, self.dtype, self.length = ctx, ptr, nbytes, np.dtype(dtype), length
+
+    def numpy(self, n=None):
+        return self.ctx.download(self.ptr, self.length if n is None else n, self.dtype)
+
+
+class SegmentCache:
+    """Device-resident segment cache (SegmentCache + LRUCachingPolicy,
+    fpdb-cache/src/SegmentCache.cpp, fpdb-cache/src/policy/LRUCachingPolicy.cpp)."""
+
+    def __init__(self, ctx, capacity_bytes, policy=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        C.check(C.lib().fpdb_cache_create(ctx.handle, int(capacity_bytes), policy, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            C.lib().fpdb_cache_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def store(self, partition, column, start, finish, data, valid=None):
+        """data: a DeviceColumn, or a numpy array (host memory, copied in
+        directly) with an optional boolean validity mask."""
+        if isinstance(data, DeviceColumn):
+            col = data.c()
+        else:
+            arr = np.ascontiguousarray(data)
+            self._host_keepalive = arr
+            vb = None
+            if valid is not None:
+                vb = np.packbits(np.asarray(valid, bool), bitorder="little")
+                self._host_valid_keepalive = vb
+            col = C.Column(C.NP_TO_TYPE[arr.dtype], 0, len(arr), 0, arr.ctypes.data,
+                           vb.ctypes.data if vb is not None else None)
+        stored = ctypes.c_int()
+        C.check(C.lib().fpdb_cache_store(self.handle, partition.encode(), column.encode(), int(start), int(finish),
+                                         ctypes.byref(col), ctypes.byref(stored)))
+        return bool(stored.value)
+
+    def load(self, partition, column, start=0, finish=0):
+        """A pinned DeviceColumn over the cached buffers, or None on a miss."""
+        out = C.Column()
+        hit = ctypes.c_int()
+        C.check(C.lib().fpdb_cache_load(self.handle, partition.encode(), column.encode(), int(start), int(finish),
+                                        ctypes.byref(out), ctypes.byref(hit)))
+        if not hit.value:
+            return None
+        dtype = C.TYPE_TO_NP[out.type]
+        vals = _CacheBuffer(self.ctx, out.values, out.length * dtype.itemsize, dtype, out.length)
+        valid = _CacheBuffer(self.ctx, out.validity, (out.length + 7) // 8, np.uint8, (out.length + 7) // 8) \
+            if out.validity else None
+        return DeviceColumn(vals, valid, out.type, out.length)
+
+    def release(self, col):
+        C.check(C.lib().fpdb_cache_release(self.handle, col.values.ptr))
+
+    def remove(self, partition, column=None):
+        n = ctypes.c_int64()
+        C.check(C.lib().fpdb_cache_remove(self.handle, partition.encode(),
+                                          column.encode() if column is not None else None, ctypes.byref(n)))
+        return n.value
+
+    def clear(self):
+        C.check(C.lib().fpdb_cache_clear(self.handle))
+
+    def info(self):
+        ns, hits, misses = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        used, free = ctypes.c_uint64(), ctypes.c_uint64()
+        C.check(C.lib().fpdb_cache_info(self.handle, ctypes.byref(ns), ctypes.byref(used), ctypes.byref(free),
+                                        ctypes.byref(hits), ctypes.byref(misses)))
+        return {"segments": ns.value, "used": used.value, "free": free.value, "hits": hits.value,
+                "misses": misses.value}
+
+    def layout(self):
+        """Keys in the usage queue, most recently used first."""
+        need = ctypes.c_uint64()
+        C.check(C.lib().fpdb_cache_layout(self.handle, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        C.check(C.lib().fpdb_cache_layout(self.handle, buf, need.value, ctypes.byref(need)))
+        return [tuple(line.split("\t")) for line in buf.value.decode().splitlines()]
+
+
 class HashJoinTable:
     """Device hash table of the build side (HashJoinArrowKernel build phase)."""
 

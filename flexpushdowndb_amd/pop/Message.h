@@ -5,13 +5,34 @@
 
 #include <memory>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "BloomFilter.h"
 #include "TupleSet.h"
 
 namespace fpdb::amd {
 
-enum class MessageType { START, TUPLESET, BLOOM_FILTER, COMPLETE, ERROR };
+enum class MessageType {
+  START, TUPLESET, BLOOM_FILTER, COMPLETE, ERROR,
+  // segment cache protocol (fpdb-executor/include/fpdb/executor/message/cache/*, ScanMessage.h)
+  SCAN, LOAD_REQUEST, LOAD_RESPONSE, STORE_REQUEST, CACHE_METRICS
+};
+
+// Cache key of one column segment of a table partition
+// (fpdb-cache/include/fpdb/cache/SegmentKey.h, SegmentRange.h): equality and
+// hashing use the partition and the column, not the range
+// (fpdb-cache/src/SegmentKey.cpp:58-72).
+struct SegmentKey {
+  std::string partition;
+  std::string columnName;
+  int64_t startOffset = 0, finishOffset = 0;
+  uint64_t size = 0;    // SegmentMetadata::size: Arrow buffer bytes
+  std::string toString() const {
+    return "{ partition: " + partition + ", column: " + columnName + ", range: (" + std::to_string(startOffset) + ":" +
+           std::to_string(finishOffset) + ") }";
+  }
+};
 
 class Message {
  public:
@@ -64,6 +85,72 @@ class ErrorMessage : public Message {
 
  private:
   std::string content_;
+};
+
+// ScanMessage (fpdb-executor/include/fpdb/executor/message/ScanMessage.h:17-38)
+class ScanMessage : public Message {
+ public:
+  ScanMessage(std::vector<std::string> scanColumnNames, std::vector<std::string> projectColumnNames,
+              std::string sender, bool resultNeeded)
+      : Message(MessageType::SCAN, std::move(sender)), scanColumnNames_(std::move(scanColumnNames)),
+        projectColumnNames_(std::move(projectColumnNames)), resultNeeded_(resultNeeded) {}
+  const std::vector<std::string>& getScanColumnNames() const { return scanColumnNames_; }
+  const std::vector<std::string>& getProjectColumnNames() const { return projectColumnNames_; }
+  bool isResultNeeded() const { return resultNeeded_; }
+
+ private:
+  std::vector<std::string> scanColumnNames_, projectColumnNames_;
+  bool resultNeeded_;
+};
+
+class LoadRequestMessage : public Message {
+ public:
+  LoadRequestMessage(std::vector<SegmentKey> keys, std::string sender)
+      : Message(MessageType::LOAD_REQUEST, std::move(sender)), keys_(std::move(keys)) {}
+  const std::vector<SegmentKey>& getSegmentKeys() const { return keys_; }
+
+ private:
+  std::vector<SegmentKey> keys_;
+};
+
+// Hit segments (device columns, pinned while held) in request order, and the
+// missed keys the policy wants cached.
+class LoadResponseMessage : public Message {
+ public:
+  LoadResponseMessage(std::vector<std::pair<SegmentKey, Column>> segments, std::vector<SegmentKey> keysToCache,
+                      std::string sender)
+      : Message(MessageType::LOAD_RESPONSE, std::move(sender)), segments_(std::move(segments)),
+        keysToCache_(std::move(keysToCache)) {}
+  const std::vector<std::pair<SegmentKey, Column>>& getSegments() const { return segments_; }
+  const std::vector<SegmentKey>& getSegmentKeysToCache() const { return keysToCache_; }
+
+ private:
+  std::vector<std::pair<SegmentKey, Column>> segments_;
+  std::vector<SegmentKey> keysToCache_;
+};
+
+class StoreRequestMessage : public Message {
+ public:
+  StoreRequestMessage(std::vector<std::pair<SegmentKey, Column>> segments, std::string sender)
+      : Message(MessageType::STORE_REQUEST, std::move(sender)), segments_(std::move(segments)) {}
+  const std::vector<std::pair<SegmentKey, Column>>& getSegments() const { return segments_; }
+
+ private:
+  std::vector<std::pair<SegmentKey, Column>> segments_;
+};
+
+class CacheMetricsMessage : public Message {
+ public:
+  CacheMetricsMessage(size_t hitNum, size_t missNum, size_t shardHitNum, size_t shardMissNum, std::string sender)
+      : Message(MessageType::CACHE_METRICS, std::move(sender)), hitNum_(hitNum), missNum_(missNum),
+        shardHitNum_(shardHitNum), shardMissNum_(shardMissNum) {}
+  size_t getHitNum() const { return hitNum_; }
+  size_t getMissNum() const { return missNum_; }
+  size_t getShardHitNum() const { return shardHitNum_; }
+  size_t getShardMissNum() const { return shardMissNum_; }
+
+ private:
+  size_t hitNum_, missNum_, shardHitNum_, shardMissNum_;
 };
 
 class Envelope {

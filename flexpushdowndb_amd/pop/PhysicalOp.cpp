@@ -12,6 +12,11 @@ std::string Message::getTypeString() const {
     case MessageType::BLOOM_FILTER: return "BloomFilterMessage";
     case MessageType::COMPLETE: return "CompleteMessage";
     case MessageType::ERROR: return "ErrorMessage";
+    case MessageType::SCAN: return "ScanMessage";
+    case MessageType::LOAD_REQUEST: return "LoadRequestMessage";
+    case MessageType::LOAD_RESPONSE: return "LoadResponseMessage";
+    case MessageType::STORE_REQUEST: return "StoreRequestMessage";
+    case MessageType::CACHE_METRICS: return "CacheMetricsMessage";
   }
   return "Message";
 }

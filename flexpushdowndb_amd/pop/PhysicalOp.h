@@ -19,7 +19,8 @@
 namespace fpdb::amd {
 
 enum class POpType {
-  HASH_JOIN_ARROW, BLOOM_FILTER_CREATE, BLOOM_FILTER_USE, GROUP, SHUFFLE, SOURCE, COLLATE, FILTER
+  HASH_JOIN_ARROW, BLOOM_FILTER_CREATE, BLOOM_FILTER_USE, GROUP, SHUFFLE, SOURCE, COLLATE, FILTER,
+  CACHE_LOAD, SEGMENT_CACHE, FILE_SCAN, MERGE
 };
 
 enum class POpRelationshipType { Producer, Consumer };

@@ -22,8 +22,14 @@ DeviceBuffer::DeviceBuffer(fpdb_context* ctx, uint64_t bytes) : ctx_(ctx), bytes
   checkStatus(fpdb_device_alloc(ctx, bytes ? bytes : 1, &ptr_), "fpdb_device_alloc");
 }
 
+DeviceBuffer::DeviceBuffer(fpdb_context* ctx, void* ptr, uint64_t bytes, std::function<void()> release)
+    : ctx_(ctx), ptr_(ptr), bytes_(bytes), release_(std::move(release)) {}
+
 DeviceBuffer::~DeviceBuffer() {
-  if (ptr_) fpdb_device_free(ctx_, ptr_);
+  if (release_)
+    release_();
+  else if (ptr_)
+    fpdb_device_free(ctx_, ptr_);
 }
 
 int typeWidth(fpdb_type t) {

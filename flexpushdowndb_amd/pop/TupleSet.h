@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -17,6 +18,9 @@ namespace fpdb::amd {
 class DeviceBuffer {
  public:
   DeviceBuffer(fpdb_context* ctx, uint64_t bytes);
+  // Borrows memory owned elsewhere (a cached segment); `release` runs instead
+  // of the free when the last holder lets go.
+  DeviceBuffer(fpdb_context* ctx, void* ptr, uint64_t bytes, std::function<void()> release);
   ~DeviceBuffer();
   DeviceBuffer(const DeviceBuffer&) = delete;
   DeviceBuffer& operator=(const DeviceBuffer&) = delete;
@@ -28,6 +32,7 @@ class DeviceBuffer {
   fpdb_context* ctx_;
   void* ptr_ = nullptr;
   uint64_t bytes_;
+  std::function<void()> release_;
 };
 
 // Device context shared by the operators of one executor.

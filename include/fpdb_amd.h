@@ -1,7 +1,8 @@
 /*
  * fpdb_amd.h -- C ABI of the MI355X (gfx950) relational hot path of
  * FlexPushdownDB: blocked and legacy bloom filters (predicate transfer),
- * hash-join build/probe, hash group-by, and filter compaction.
+ * hash-join build/probe, hash group-by, filter compaction, hash shuffle and
+ * a device-resident segment cache.
  *
  * Conventions
  *   - Every entry point returns an fpdb_status; FPDB_OK is 0.  On failure
@@ -313,6 +314,47 @@ fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_col
  * materialise build-side rows and the null probe side of outer joins). */
 fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns,
                         const int64_t* d_rows, int64_t num_rows, fpdb_column* out);
+
+/* ---- device-resident segment cache ---------------------------------------------- */
+
+/* Column segments of table partitions kept in HBM under (partition, column)
+ * keys -- the range is carried but, as in the reference, not part of a key's
+ * identity (fpdb-cache/src/SegmentKey.cpp:58-72) -- with a size-bounded
+ * replacement policy.  Segment size = values bytes + ceil(length/8) validity
+ * bytes (the Arrow buffer sizes the reference's SegmentMetadata counts).
+ * Host bookkeeping, device data: SegmentCache (fpdb-cache/src/SegmentCache.cpp)
+ * with LRUCachingPolicy (fpdb-cache/src/policy/LRUCachingPolicy.cpp). */
+typedef struct fpdb_segment_cache fpdb_segment_cache;
+typedef enum { FPDB_CACHE_LRU = 0 } fpdb_cache_policy;
+fpdb_status fpdb_cache_create(fpdb_context* ctx, uint64_t capacity_bytes, int policy, fpdb_segment_cache** out);
+fpdb_status fpdb_cache_destroy(fpdb_segment_cache* cache);
+/* SegmentCache::store (SegmentCache.cpp:21-30): copies `data` (host or device
+ * memory) into cache-owned HBM, evicting least recently used segments until
+ * it fits.  *stored = 0 when the segment is larger than the whole cache
+ * (LRUCachingPolicy::onStore returns no decision, LRUCachingPolicy.cpp:32-38).
+ * Storing a key that is present refreshes it. */
+fpdb_status fpdb_cache_store(fpdb_segment_cache* cache, const char* partition, const char* column,
+                             int64_t start_offset, int64_t finish_offset, const fpdb_column* data, int* stored);
+/* SegmentCache::load (SegmentCache.cpp:32-45): on a hit *hit = 1 and *out
+ * describes the cached device buffers, pinned until fpdb_cache_release (an
+ * eviction meanwhile drops the key but keeps the memory).  A load moves the
+ * key to the front of the usage queue. */
+fpdb_status fpdb_cache_load(fpdb_segment_cache* cache, const char* partition, const char* column,
+                            int64_t start_offset, int64_t finish_offset, fpdb_column* out, int* hit);
+fpdb_status fpdb_cache_release(fpdb_segment_cache* cache, const void* values);
+/* SegmentCache::remove(predicate) (SegmentCache.cpp:51-61) for one partition,
+ * or one column of it (column may be NULL). */
+fpdb_status fpdb_cache_remove(fpdb_segment_cache* cache, const char* partition, const char* column,
+                              int64_t* removed);
+/* SegmentCache::clear (SegmentCache.cpp:80-84): every segment and the counters. */
+fpdb_status fpdb_cache_clear(fpdb_segment_cache* cache);
+/* Segments held, policy bytes used / free, and load hits / misses. */
+fpdb_status fpdb_cache_info(fpdb_segment_cache* cache, int64_t* num_segments, uint64_t* used_bytes,
+                            uint64_t* free_bytes, int64_t* hits, int64_t* misses);
+/* The usage queue, most recent first, as "partition\tcolumn\n" lines
+ * (LRUCachingPolicy::showCurrentLayout, LRUCachingPolicy.cpp:79-85);
+ * *needed = bytes including the terminating NUL. */
+fpdb_status fpdb_cache_layout(fpdb_segment_cache* cache, char* buf, uint64_t buf_len, uint64_t* needed);
 
 /* ---- hash shuffle ------------------------------------------------------------ */
 

@@ -239,3 +239,61 @@ def shuffle_order(hashes, num_slots):
     offsets = np.zeros(num_slots + 1, np.int64)
     np.cumsum(counts, out=offsets[1:])
     return order, offsets
+
+
+# --- segment cache (LRU policy) -------------------------------------------------
+
+class LruCacheModel:
+    """SegmentCache::store/load (fpdb-cache/src/SegmentCache.cpp:21-45) with
+    LRUCachingPolicy (fpdb-cache/src/policy/LRUCachingPolicy.cpp:18-73): a
+    usage queue, most recently used first; a store evicts from the back until
+    the segment fits and declines a segment larger than the capacity; a load
+    moves a present key to the front.  Keys are (partition, column).  A store
+    of a key already present refreshes it (the reference would queue it
+    twice; the device cache and this model both refresh)."""
+
+    def __init__(self, capacity):
+        self.capacity = capacity
+        self.free = capacity
+        self.queue = []          # [(key, size)], front = most recent
+        self.hits = 0
+        self.misses = 0
+
+    def _find(self, key):
+        for i, (k, _) in enumerate(self.queue):
+            if k == key:
+                return i
+        return -1
+
+    def store(self, key, size):
+        i = self._find(key)
+        if i >= 0:
+            self.queue.insert(0, self.queue.pop(i))
+            return True
+        if size > self.capacity:
+            return False
+        while self.free < size:
+            _, s = self.queue.pop()
+            self.free += s
+        self.queue.insert(0, (key, size))
+        self.free -= size
+        return True
+
+    def load(self, key):
+        i = self._find(key)
+        if i < 0:
+            self.misses += 1
+            return False
+        self.queue.insert(0, self.queue.pop(i))
+        self.hits += 1
+        return True
+
+    def remove(self, partition, column=None):
+        keep = [(k, s) for k, s in self.queue if not (k[0] == partition and (column is None or k[1] == column))]
+        removed = len(self.queue) - len(keep)
+        self.free += sum(s for k, s in self.queue) - sum(s for k, s in keep)
+        self.queue = keep
+        return removed
+
+    def layout(self):
+        return [k for k, _ in self.queue]

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "BloomFilterPOps.h"
+#include "Cache.h"
 #include "Globals.h"
 #include "Group.h"
 #include "HashJoin.h"
@@ -286,6 +287,93 @@ static void testShuffledJoin() {
   CHECK(rows == want);
 }
 
+// One query over partition "lineorder/part0" through the cache: CacheLoadPOp
+// -> hit columns to a MergePOp, missed columns scanned from host memory (and
+// stored in the cache) -> MergePOp -> collate.
+struct CachedScanResult {
+  std::shared_ptr<TupleSet> out;
+  uint64_t bytesScanned = 0;
+};
+
+static CachedScanResult runCachedScan(const std::shared_ptr<SegmentCache>& cache, const Schema& schema,
+                                      const std::vector<HostColumn>& cols) {
+  Executor ex;
+  std::vector<std::string> names;
+  for (auto& f : schema.fields) names.push_back(f.name);
+  auto actor = std::make_shared<SegmentCacheActor>(cache);
+  auto load = std::make_shared<CacheLoadPOp>("cache_load", names, 0, std::vector<std::string>{}, names,
+                                             "lineorder/part0", 0, 4096);
+  auto scan = std::make_shared<HostScanPOp>("scan", names, 0, schema, cols, "lineorder/part0", 0, 4096,
+                                            /*scanOnStart*/ false, /*toCache*/ true);
+  auto merge = std::make_shared<MergePOp>("merge", names, 0);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{actor, load, scan, merge, collate}) ex.put(op);
+  load->setHitOperator(merge);
+  load->setMissOperatorToCache(scan);
+  merge->setLeftProducer(load);
+  scan->produce(merge);
+  merge->setRightProducer(scan);
+  connect(merge, collate);
+  auto r = ex.execute();
+  CachedScanResult res;
+  if (!r) {
+    std::printf("  execute failed: %s\n", r.error().c_str());
+    ++g_failures;
+    return res;
+  }
+  res.out = collate->tupleSet().value();
+  res.bytesScanned = scan->bytesRead();
+  return res;
+}
+
+static void testSegmentCache() {
+  const int64_t n = 3000;
+  std::vector<int64_t> a(n), b(n);
+  std::vector<int32_t> c(n);
+  for (int64_t i = 0; i < n; ++i) {
+    a[i] = i * 3 - 7;
+    b[i] = (i * 7919) % 1000;
+    c[i] = (int32_t)(i % 97);
+  }
+  Schema s;
+  s.fields = {Field{"a", FPDB_TYPE_INT64, false}, Field{"b", FPDB_TYPE_INT64, false},
+              Field{"c", FPDB_TYPE_INT32, true}};
+  HostColumn hc = hostInt32(c);
+  hc.valid.resize(n);
+  for (int64_t i = 0; i < n; ++i) hc.valid[i] = i % 5 != 0;
+  std::vector<HostColumn> cols = {hostInt64(a), hostInt64(b), hc};
+  const uint64_t sizeA = n * 8, sizeC = n * 4 + (n + 7) / 8;
+  auto want = rowsOf(TupleSet::fromHost(defaultContext(), s, cols));
+
+  // a cache that holds everything: cold query scans all three columns, warm
+  // query scans none; both return the same rows
+  auto cache = SegmentCache::make(1 << 20);
+  auto cold = runCachedScan(cache, s, cols);
+  CHECK(cold.out && rowsOf(cold.out) == want);
+  CHECK(cold.bytesScanned == 2 * sizeA + sizeC);
+  CHECK(cache->getSize() == 3 && cache->usedBytes() == 2 * sizeA + sizeC);
+  CHECK(cache->hitNum() == 0 && cache->missNum() == 3 && cache->shardMissNum() == 1);
+  cache->newQuery();
+  auto warm = runCachedScan(cache, s, cols);
+  CHECK(warm.out && rowsOf(warm.out) == want);
+  CHECK(warm.bytesScanned == 0);
+  CHECK(cache->hitNum() == 3 && cache->shardHitNum() == 1 && cache->crtQueryHitNum() == 3);
+  CHECK(cache->crtQueryMissNum() == 0);
+
+  // room for two of the three segments: storing c evicts a (least recent);
+  // the next query hits b and c, scans a again, and the merged columns come
+  // back in projection order
+  auto small = SegmentCache::make(sizeA + sizeA + sizeC - 1);
+  auto q1 = runCachedScan(small, s, cols);
+  CHECK(q1.out && rowsOf(q1.out) == want);
+  CHECK(small->showCurrentLayout() == "lineorder/part0\tc\nlineorder/part0\tb\n");
+  auto q2 = runCachedScan(small, s, cols);
+  CHECK(q2.out && rowsOf(q2.out) == want);
+  CHECK(q2.bytesScanned == sizeA);
+  CHECK(q2.out && q2.out->schema().fields[0].name == "a" && q2.out->schema().fields[2].name == "c");
+  CHECK(small->hitNum() == 2 && small->missNum() == 4 && small->shardMissNum() == 2);
+}
+
 static void testEmptyAndErrors() {
   // an empty build side still yields an (empty) result with the output schema
   Executor ex;
@@ -472,6 +560,7 @@ int main(int argc, char** argv) {
       {"empty_and_errors", testEmptyAndErrors},
       {"shuffle", testShuffle},
       {"shuffled_join", testShuffledJoin},
+      {"segment_cache", testSegmentCache},
   };
   for (auto& t : tests) {
     int before = g_failures;

@@ -323,3 +323,35 @@ def test_shuffle_order_properties():
     # 0xFFFFFFFF and 3 are 0 mod 3; 5, 0x80000000 and 0xFFFFFFFE are 2 mod 3
     assert offsets.tolist() == [0, 2, 2, 6]
     assert order.tolist() == [0, 3, 1, 2, 4, 5]
+
+
+def test_lru_cache_model_worked_example():
+    # capacity 100: a(40) b(30) c(20) fill 90; loading a makes b the least
+    # recent, so d(30) evicts b only; e(200) is declined
+    m = O.LruCacheModel(100)
+    for k, s in [("a", 40), ("b", 30), ("c", 20)]:
+        assert m.store(("p", k), s)
+    assert m.load(("p", "a")) and not m.load(("p", "z"))
+    assert m.store(("p", "d"), 30)
+    assert m.layout() == [("p", "d"), ("p", "a"), ("p", "c")]
+    assert m.free == 10
+    assert not m.store(("p", "e"), 200) and m.layout()[0] == ("p", "d")
+    # a store that needs two evictions takes the two least recent
+    assert m.store(("q", "f"), 55)
+    assert m.layout() == [("q", "f"), ("p", "d")] and m.free == 15
+    assert (m.hits, m.misses) == (1, 1)
+    assert m.remove("q") == 1 and m.layout() == [("p", "d")] and m.free == 70
+
+
+def test_lru_cache_model_invariants():
+    rng = np.random.default_rng(5)
+    m = O.LruCacheModel(1000)
+    for _ in range(2000):
+        key = ("p%d" % rng.integers(4), "c%d" % rng.integers(6))
+        if rng.random() < 0.5:
+            m.store(key, int(rng.integers(1, 400)))
+        else:
+            m.load(key)
+        sizes = [s for _, s in m.queue]
+        assert sum(sizes) + m.free == 1000 and m.free >= 0
+        assert len(set(m.layout())) == len(m.layout())
