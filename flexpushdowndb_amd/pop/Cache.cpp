@@ -98,7 +98,7 @@ void SegmentCache::clearCrtQueryMetrics() {
 
 // ---- SegmentCacheActor ---------------------------------------------------------------
 
-// SegmentCacheActor::load / store / metrics (SegmentCacheActor.cpp:18-72).  In
+// SegmentCacheActor::load / store / metrics (SegmentCacheActor.cpp:11-72).  In
 // caching-only mode every missed key is to be loaded and cached.
 void SegmentCacheActor::onReceive(const Envelope& msg) {
   const auto& m = msg.message();
@@ -168,7 +168,7 @@ void CacheLoadPOp::onReceive(const Envelope& msg) {
 }
 
 // CacheLoadPOp::onStart + CacheHelper::requestLoadSegmentsFromCache
-// (CacheLoadPOp.cpp:74-112, CacheHelper.cpp:17-37)
+// (CacheLoadPOp.cpp:75-113, CacheHelper.cpp:10-33)
 void CacheLoadPOp::onStart() {
   if (!hitOperatorName_) {
     ctx()->notifyError("Hit consumer not set ");
@@ -183,7 +183,7 @@ void CacheLoadPOp::onStart() {
   ctx()->tell(std::make_shared<LoadRequestMessage>(std::move(keys), name_), {"SegmentCache"});
 }
 
-// CacheLoadPOp::onCacheLoadResponse, caching-only branch (CacheLoadPOp.cpp:114-263)
+// CacheLoadPOp::onCacheLoadResponse, caching-only branch (CacheLoadPOp.cpp:115-259)
 void CacheLoadPOp::onCacheLoadResponse(const LoadResponseMessage& msg) {
   Schema hitSchema;
   std::vector<Column> hitColumns;
@@ -258,7 +258,7 @@ Expected<std::shared_ptr<TupleSet>> HostScanPOp::readTuples(const std::vector<st
     return make_unexpected(e.what());
   }
   if (toCache_ && ts->numColumns() > 0) {
-    // CacheHelper::requestStoreSegmentsInCache (CacheHelper.cpp:39-61)
+    // CacheHelper::requestStoreSegmentsInCache (CacheHelper.cpp:35-61)
     std::vector<std::pair<SegmentKey, Column>> segments;
     for (int c = 0; c < ts->numColumns(); ++c) {
       const Column& col = ts->column(c);

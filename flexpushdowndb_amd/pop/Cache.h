@@ -79,7 +79,7 @@ class SegmentCacheActor : public PhysicalOp {
 
 // Caching-only CacheLoadPOp: asks the cache for every column of its segment;
 // the hit columns go to the hit operator as one TupleSet, the missed ones are
-// requested from the miss operator with a ScanMessage (CacheLoadPOp.cpp:114-263;
+// requested from the miss operator with a ScanMessage (CacheLoadPOp.cpp:115-259;
 // hybrid pushdown needs the store and is not part of this layer).
 class CacheLoadPOp : public PhysicalOp {
  public:

@@ -10,8 +10,9 @@ namespace fpdb::amd {
 
 class SourcePOp : public PhysicalOp {
  public:
-  SourcePOp(const std::string& name, std::vector<std::shared_ptr<TupleSet>> batches)
-      : PhysicalOp(name, POpType::SOURCE, {}, 0), batches_(std::move(batches)) {}
+  SourcePOp(const std::string& name, std::vector<std::shared_ptr<TupleSet>> batches,
+            std::vector<std::string> projectColumnNames = {})
+      : PhysicalOp(name, POpType::SOURCE, std::move(projectColumnNames), 0), batches_(std::move(batches)) {}
   std::string getTypeString() const override { return "SourcePOp"; }
   void onReceive(const Envelope& msg) override;
   void clear() override { batches_.clear(); }

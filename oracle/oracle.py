@@ -297,3 +297,114 @@ class LruCacheModel:
 
     def layout(self):
         return [k for k, _ in self.queue]
+
+
+# --- predicate-transfer plan ----------------------------------------------------
+
+def pred_trans_plan(order, tables, joins):
+    """Connection steps of the predicate-transfer orderings
+    (fpdb-executor/src/physical/transform/pred-trans/SmallToLargePredTransOrder.cpp:37-214,
+    BFSPredTransOrder.cpp:40-227), with insertion order where the reference
+    iterates hash sets.
+
+    order: "s2l", "bfs" or "yan"; tables: [(name, rows)]; joins:
+    [(left, right, left_columns, right_columns, type)].  Returns (steps, ends)
+    with steps = [(transfer_op, create_op, src, tgt, src_attached, tgt_attached)]
+    and ends = {table: op its rows leave from}, or raises ValueError on a cycle
+    (s2l)."""
+    names = [t for t, _ in tables]
+    rows = dict(tables)
+    curr = {t: "scan_" + t for t in names}
+    steps = []
+
+    def pred(a, b):
+        return "-".join(x + "-" + y for x, y in zip(a, b))
+
+    def connect(create, use, src, tgt):
+        steps.append((use, create, src, tgt, curr[src], curr[tgt]))
+        curr[tgt] = use
+
+    if order == "s2l":
+        fw, bw = [], []
+        need = {"F": {t: 0 for t in names}, "B": {t: 0 for t in names}}
+        for i, (l, r, lc, rc, ty) in enumerate([j for j in joins if j[4] != "FULL"]):
+            p = pred(lc, rc)
+            if ty != "RIGHT":
+                fw.append(("BloomFilterCreate(F)<%d>-%s" % (i, p), "BloomFilterUse(F)<%d>-%s" % (i, p), l, r))
+                need["F"][r] += 1
+            if ty != "LEFT":
+                bw.append(("BloomFilterCreate(B)<%d>-%s" % (i, p), "BloomFilterUse(B)<%d>-%s" % (i, p), r, l))
+                need["B"][l] += 1
+        for tag, nodes in (("F", fw), ("B", bw)):
+            done = {t: 0 for t in names}
+            queue = [n for n in nodes if need[tag][n[2]] == 0]
+            seen = set(queue)
+            while queue:
+                create, use, src, tgt = queue.pop(0)
+                connect(create, use, src, tgt)
+                done[tgt] += 1
+                if done[tgt] == need[tag][tgt]:
+                    for n in nodes:
+                        if n[2] == tgt and n not in seen:
+                            queue.append(n)
+                            seen.add(n)
+            if len(seen) != len(nodes):
+                raise ValueError("cycle (%s predicate transfer)" % ("forward" if tag == "F" else "backward"))
+        return steps, curr
+
+    # BFS from the largest table (Yannakakis: the first one)
+    nbrs = {t: [] for t in names}
+    in_graph = []
+    for l, r, lc, rc, ty in joins:
+        if ty == "FULL":
+            continue
+        ld, rd = {"LEFT": ("fw", "bw"), "RIGHT": ("bw", "fw")}.get(ty, ("both", "both"))
+        nbrs[l].append((r, ld, lc, rc))
+        nbrs[r].append((l, rd, rc, lc))
+        for t in (l, r):
+            if t not in in_graph:
+                in_graph.append(t)
+    todo = [t for t in names if t in in_graph]
+    stack = []
+    while todo:
+        if order == "yan":
+            root = todo[0]
+        else:
+            root = todo[0]
+            for t in todo:
+                if rows[t] > rows[root]:
+                    root = t
+        todo.remove(root)
+        q = [root]
+        while q:
+            u = q.pop(0)
+            for v, d, own, other in nbrs[u]:
+                if v not in todo:
+                    continue
+                stack.append((v, u, other, own, d in ("both", "bw"), d in ("both", "fw")))
+                q.append(v)
+                todo.remove(v)
+    ident = [0]
+
+    def transfer(src, tgt, sc, tc, tag):
+        p = pred(sc, tc)
+        i = ident[0]
+        ident[0] += 1
+        if order == "yan":
+            name = "HashJoinArrow(%s)<%d>-%s" % (tag, i, p)
+            steps.append((name, name, src, tgt, curr[src], curr[tgt]))
+            curr[tgt] = name
+        else:
+            connect("BloomFilterCreate(%s)<%d>-%s" % (tag, i, p), "BloomFilterUse(%s)<%d>-%s" % (tag, i, p), src, tgt)
+
+    back = []
+    while stack:
+        src, tgt, sc, tc, fwd, bwd = stack.pop()
+        if fwd:
+            transfer(src, tgt, sc, tc, "F")
+        back.append((tgt, src, tc, sc, bwd, fwd))
+    while back:
+        src, tgt, sc, tc, fwd, _ = back.pop()
+        if fwd:
+            transfer(src, tgt, sc, tc, "B")
+    return steps, curr

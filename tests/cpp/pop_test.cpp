@@ -7,6 +7,7 @@
 //                                 N seeded fact rows; result rows to OUT.csv
 //                                 (checked against numpy by tests/test_pop_gpu.py)
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -21,6 +22,7 @@
 #include "Group.h"
 #include "HashJoin.h"
 #include "Plumbing.h"
+#include "PredTrans.h"
 #include "Shuffle.h"
 
 using namespace fpdb::amd;
@@ -545,8 +547,176 @@ static int starJoin(int64_t n, const char* outPath) {
   return 0;
 }
 
+static std::vector<std::string> split(const std::string& s, char sep) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == sep) {
+      out.push_back(cur);
+      cur.clear();
+    } else {
+      cur += ch;
+    }
+  }
+  if (!cur.empty()) out.push_back(cur);
+  return out;
+}
+
+// pop_test plan ORDER TABLES JOINS: builds only the predicate-transfer part of
+// a plan (no data, no device work) and prints its connection steps.
+//   ORDER   s2l | bfs | yan
+//   TABLES  name=rows,name=rows,...
+//   JOINS   left.c1+c2>right.d1+d2@TYPE;...   (TYPE INNER LEFT RIGHT FULL)
+static int planOnly(const std::string& order, const std::string& tablesSpec, const std::string& joinsSpec) {
+  Executor ex;
+  std::vector<PredTransTable> tables;
+  for (const auto& t : split(tablesSpec, ',')) {
+    auto kv = split(t, '=');
+    auto src = std::make_shared<SourcePOp>("scan_" + kv[0], std::vector<std::shared_ptr<TupleSet>>{});
+    ex.put(src);
+    tables.push_back(PredTransTable{kv[0], src, std::atof(kv[1].c_str())});
+  }
+  std::vector<JoinOrigin> joins;
+  for (const auto& j : split(joinsSpec, ';')) {
+    auto at = split(j, '@');
+    auto sides = split(at[0], '>');
+    auto l = split(sides[0], '.'), r = split(sides[1], '.');
+    JoinOrigin o;
+    o.left = l[0];
+    o.right = r[0];
+    o.leftColumns = split(l[1], '+');
+    o.rightColumns = split(r[1], '+');
+    const std::string& ty = at[1];
+    o.joinType = ty == "LEFT" ? LEFT : ty == "RIGHT" ? RIGHT : ty == "FULL" ? FULL : INNER;
+    joins.push_back(o);
+  }
+  std::vector<PredTransStep> steps;
+  auto r = PredTransOrder::orderPredTrans(order == "s2l" ? PredTransOrderType::SMALL_TO_LARGE : PredTransOrderType::BFS,
+                                          order == "yan", ex, tables, joins, &steps);
+  if (!r) {
+    std::printf("ERROR %s\n", r.error().c_str());
+    return 0;
+  }
+  for (const auto& st : steps)
+    std::printf("STEP %s|%s|%s|%s|%s|%s\n", st.transferOp.c_str(), st.createOp.c_str(), st.srcTable.c_str(),
+                st.tgtTable.c_str(), st.srcAttachedTo.c_str(), st.tgtAttachedTo.c_str());
+  for (const auto& kv : *r) std::printf("END %s|%s\n", kv.first.c_str(), kv.second->name().c_str());
+  return 0;
+}
+
+// The star join of starJoin with its predicate transfer laid out by the plan
+// builder (ORDER s2l | bfs | yan) instead of by hand; joins and group-by are
+// connected from the chain ends it returns.
+static int starJoinPlanned(const std::string& order, int64_t n, const char* outPath) {
+  const int64_t d1 = 1000, d2 = 500;
+  const uint64_t seed = 77;
+  fpdb_context* ctx = defaultContext();
+  std::vector<int64_t> fk1(n), fk2(n), m(n), k1(d1), k2(d2);
+  std::vector<int32_t> a1(d1), g1(d1), a2(d2), g2(d2);
+  for (int64_t i = 0; i < n; ++i) {
+    fk1[i] = uni(seed, 1, i, 0, d1 - 1);
+    fk2[i] = uni(seed, 2, i, 0, d2 - 1);
+    m[i] = uni(seed, 3, i, 1, 100);
+  }
+  for (int64_t i = 0; i < d1; ++i) {
+    k1[i] = i;
+    a1[i] = (int32_t)uni(seed, 4, i, 0, 99);
+    g1[i] = (int32_t)uni(seed, 5, i, 0, 9);
+  }
+  for (int64_t i = 0; i < d2; ++i) {
+    k2[i] = i;
+    a2[i] = (int32_t)uni(seed, 6, i, 0, 99);
+    g2[i] = (int32_t)uni(seed, 7, i, 0, 4);
+  }
+  Schema sf{{Field{"fk1", FPDB_TYPE_INT64}, Field{"fk2", FPDB_TYPE_INT64}, Field{"m", FPDB_TYPE_INT64}}};
+  Schema s1{{Field{"k1", FPDB_TYPE_INT64}, Field{"a1", FPDB_TYPE_INT32}, Field{"g1", FPDB_TYPE_INT32}}};
+  Schema s2{{Field{"k2", FPDB_TYPE_INT64}, Field{"a2", FPDB_TYPE_INT32}, Field{"g2", FPDB_TYPE_INT32}}};
+  std::vector<std::shared_ptr<TupleSet>> factBatches;
+  const int64_t chunk = 100000;
+  for (int64_t s = 0; s < n; s += chunk) {
+    int64_t e = std::min(n, s + chunk);
+    factBatches.push_back(TupleSet::fromHost(
+        ctx, sf,
+        {hostInt64({fk1.begin() + s, fk1.begin() + e}), hostInt64({fk2.begin() + s, fk2.begin() + e}),
+         hostInt64({m.begin() + s, m.begin() + e})}));
+  }
+  auto dim1 = TupleSet::fromHost(ctx, s1, {hostInt64(k1), hostInt32(a1), hostInt32(g1)});
+  auto dim2 = TupleSet::fromHost(ctx, s2, {hostInt64(k2), hostInt32(a2), hostInt32(g2)});
+
+  Executor ex;
+  auto scanF = std::make_shared<SourcePOp>("scan_fact", factBatches, std::vector<std::string>{"fk1", "fk2", "m"});
+  auto scan1 = std::make_shared<SourcePOp>("scan_dim1", std::vector<std::shared_ptr<TupleSet>>{dim1});
+  auto scan2 = std::make_shared<SourcePOp>("scan_dim2", std::vector<std::shared_ptr<TupleSet>>{dim2});
+  auto filt1 = std::make_shared<FilterPOp>("filter_dim1", std::vector<std::string>{"k1", "a1", "g1"}, 0, "a1",
+                                           FPDB_CMP_LT, 30);
+  auto filt2 = std::make_shared<FilterPOp>("filter_dim2", std::vector<std::string>{"k2", "a2", "g2"}, 0, "a2",
+                                           FPDB_CMP_LT, 10);
+  for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{scanF, scan1, scan2, filt1, filt2}) ex.put(op);
+  connect(scan1, filt1);
+  connect(scan2, filt2);
+  std::vector<PredTransTable> tables = {{"fact", scanF, (double)n}, {"dim1", filt1, (double)d1},
+                                        {"dim2", filt2, (double)d2}};
+  std::vector<JoinOrigin> joins = {{"dim1", "fact", {"k1"}, {"fk1"}, INNER}, {"dim2", "fact", {"k2"}, {"fk2"}, INNER}};
+  auto ends = PredTransOrder::orderPredTrans(order == "s2l" ? PredTransOrderType::SMALL_TO_LARGE
+                                                            : PredTransOrderType::BFS,
+                                             order == "yan", ex, tables, joins);
+  if (!ends) {
+    std::printf("plan failed: %s\n", ends.error().c_str());
+    return 1;
+  }
+  auto endF = ends->at("fact"), end1 = ends->at("dim1"), end2 = ends->at("dim2");
+  auto join1 = std::make_shared<HashJoinArrowPOp>("join_dim1", std::vector<std::string>{"fk2", "m", "g1"}, 0,
+                                                  HashJoinPredicate{{"k1"}, {"fk1"}}, INNER);
+  auto join2 = std::make_shared<HashJoinArrowPOp>("join_dim2", std::vector<std::string>{"m", "g1", "g2"}, 0,
+                                                  HashJoinPredicate{{"k2"}, {"fk2"}}, INNER);
+  std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum_m", "m"),
+                                                        std::make_shared<Count>("cnt")};
+  auto group = std::make_shared<GroupPOp>("group", std::vector<std::string>{"g1", "g2", "sum_m", "cnt"}, 0,
+                                          std::vector<std::string>{"g1", "g2"}, fns);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{join1, join2, group, collate}) ex.put(op);
+  end1->produce(join1);
+  join1->addBuildProducer(end1);
+  endF->produce(join1);
+  join1->addProbeProducer(endF);
+  end2->produce(join2);
+  join2->addBuildProducer(end2);
+  join1->produce(join2);
+  join2->addProbeProducer(join1);
+  connect(join2, group);
+  connect(group, collate);
+  auto t0 = std::chrono::steady_clock::now();
+  auto r = ex.execute();
+  checkStatus(fpdb_context_synchronize(ctx), "synchronize");
+  const double execMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (!r) {
+    std::printf("starjoin-planned failed: %s\n", r.error().c_str());
+    return 1;
+  }
+  auto out = collate->tupleSet().value();
+  auto h = out->toHost();
+  FILE* f = std::fopen(outPath, "w");
+  if (!f) return 1;
+  std::fprintf(f, "g1,g2,sum_m,cnt\n");
+  for (int64_t i = 0; i < out->numRows(); ++i)
+    std::fprintf(f, "%lld,%lld,%lld,%lld\n", (long long)h[0].int64At(i), (long long)h[1].int64At(i),
+                 (long long)h[2].int64At(i), (long long)h[3].int64At(i));
+  std::fclose(f);
+  std::printf("starjoin-planned order=%s rows=%lld groups=%lld execute_ms=%.2f\n", order.c_str(), (long long)n,
+              (long long)out->numRows(), execMs);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   std::string mode = argc > 1 ? argv[1] : "unit";
+  if (mode == "starjoin-planned") {
+    if (argc < 5) return 2;
+    return starJoinPlanned(argv[2], std::atoll(argv[3]), argv[4]);
+  }
+  if (mode == "plan") {
+    if (argc < 5) return 2;
+    return planOnly(argv[2], argv[3], argv[4]);
+  }
   if (mode == "starjoin") {
     if (argc < 4) return 2;
     return starJoin(std::atoll(argv[2]), argv[3]);

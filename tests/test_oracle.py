@@ -355,3 +355,50 @@ def test_lru_cache_model_invariants():
         sizes = [s for _, s in m.queue]
         assert sum(sizes) + m.free == 1000 and m.free >= 0
         assert len(set(m.layout())) == len(m.layout())
+
+
+STAR = ([("F", 6e6), ("D1", 1000), ("D2", 500)],
+        [("D1", "F", ["k1"], ["fk1"], "INNER"), ("D2", "F", ["k2"], ["fk2"], "INNER")])
+
+
+def test_pred_trans_small_to_large_star_by_hand():
+    # both forward filters are free at once (dimensions use nothing forward);
+    # F's backward filters wait for both forward uses on F
+    steps, ends = O.pred_trans_plan("s2l", *STAR)
+    assert steps == [
+        ("BloomFilterUse(F)<0>-k1-fk1", "BloomFilterCreate(F)<0>-k1-fk1", "D1", "F", "scan_D1", "scan_F"),
+        ("BloomFilterUse(F)<1>-k2-fk2", "BloomFilterCreate(F)<1>-k2-fk2", "D2", "F", "scan_D2",
+         "BloomFilterUse(F)<0>-k1-fk1"),
+        ("BloomFilterUse(B)<0>-k1-fk1", "BloomFilterCreate(B)<0>-k1-fk1", "F", "D1", "BloomFilterUse(F)<1>-k2-fk2",
+         "scan_D1"),
+        ("BloomFilterUse(B)<1>-k2-fk2", "BloomFilterCreate(B)<1>-k2-fk2", "F", "D2", "BloomFilterUse(F)<1>-k2-fk2",
+         "scan_D2")]
+    assert ends == {"F": "BloomFilterUse(F)<1>-k2-fk2", "D1": "BloomFilterUse(B)<0>-k1-fk1",
+                    "D2": "BloomFilterUse(B)<1>-k2-fk2"}
+
+
+def test_pred_trans_bfs_star_by_hand():
+    # root F (largest); BFS reaches D1 then D2; forward pops the stack (D2
+    # first), backward replays it in BFS order
+    steps, ends = O.pred_trans_plan("bfs", *STAR)
+    assert [s[0] for s in steps] == ["BloomFilterUse(F)<0>-k2-fk2", "BloomFilterUse(F)<1>-k1-fk1",
+                                     "BloomFilterUse(B)<2>-fk1-k1", "BloomFilterUse(B)<3>-fk2-k2"]
+    assert [(s[2], s[3]) for s in steps] == [("D2", "F"), ("D1", "F"), ("F", "D1"), ("F", "D2")]
+    assert ends["F"] == "BloomFilterUse(F)<1>-k1-fk1"
+
+
+def test_pred_trans_join_types_and_cycles():
+    tables = [("A", 10), ("B", 20), ("C", 30)]
+    # LEFT keeps the left (build) rows: no backward filter into it; RIGHT: no
+    # forward filter; FULL: nothing
+    steps, _ = O.pred_trans_plan("s2l", tables, [("A", "B", ["a"], ["b"], "LEFT"), ("B", "C", ["b"], ["c"], "RIGHT"),
+                                                 ("A", "C", ["a"], ["c"], "FULL")])
+    assert [(s[0][:17], s[2], s[3]) for s in steps] == [("BloomFilterUse(F)", "A", "B"),
+                                                         ("BloomFilterUse(B)", "C", "B")]
+    with pytest.raises(ValueError):
+        O.pred_trans_plan("s2l", tables, [("A", "B", ["a"], ["b"], "INNER"), ("B", "C", ["b"], ["c"], "INNER"),
+                                          ("C", "A", ["c"], ["a"], "INNER")])
+    # BFS tolerates the cycle: the edge closing it is never taken
+    steps, _ = O.pred_trans_plan("bfs", tables, [("A", "B", ["a"], ["b"], "INNER"), ("B", "C", ["b"], ["c"], "INNER"),
+                                                 ("C", "A", ["c"], ["a"], "INNER")])
+    assert len(steps) == 4

@@ -44,13 +44,7 @@ def _uni(seed, col, n, lo, hi):
     return (u % np.uint64(hi - lo + 1)).astype(np.int64) + lo
 
 
-@pytest.mark.parametrize("n", [1000, 250_000])
-def test_predicate_transfer_star_join(tmp_path, n):
-    out = tmp_path / "star.csv"
-    r = _run(["starjoin", str(n), str(out)])
-    print(r.stdout)
-    assert r.returncode == 0, r.stdout
-    got = np.loadtxt(out, delimiter=",", skiprows=1, dtype=np.int64, ndmin=2)
+def _star_expected(n):
     seed = 77
     fk1, fk2, m = _uni(seed, 1, n, 0, 999), _uni(seed, 2, n, 0, 499), _uni(seed, 3, n, 1, 100)
     a1, g1 = _uni(seed, 4, 1000, 0, 99), _uni(seed, 5, 1000, 0, 9)
@@ -65,5 +59,27 @@ def test_predicate_transfer_star_join(tmp_path, n):
     for i in order:
         sel = key == key[i]
         exp.append((kg1[i], kg2[i], km[sel].sum(), sel.sum()))
-    exp = np.array(exp, dtype=np.int64).reshape(-1, 4)
-    assert np.array_equal(got, exp)
+    return np.array(exp, dtype=np.int64).reshape(-1, 4)
+
+
+@pytest.mark.parametrize("n", [1000, 250_000])
+def test_predicate_transfer_star_join(tmp_path, n):
+    out = tmp_path / "star.csv"
+    r = _run(["starjoin", str(n), str(out)])
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    got = np.loadtxt(out, delimiter=",", skiprows=1, dtype=np.int64, ndmin=2)
+    assert np.array_equal(got, _star_expected(n))
+
+
+@pytest.mark.parametrize("order", ["s2l", "bfs", "yan"])
+def test_planned_predicate_transfer_star_join(tmp_path, order):
+    # the same query with its filters (or Yannakakis semi-joins) laid out by
+    # the plan builder: the same groups in the same order
+    n = 250_000
+    out = tmp_path / f"star_{order}.csv"
+    r = _run(["starjoin-planned", order, str(n), str(out)])
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    got = np.loadtxt(out, delimiter=",", skiprows=1, dtype=np.int64, ndmin=2)
+    assert np.array_equal(got, _star_expected(n))
