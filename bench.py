@@ -482,6 +482,69 @@ def bench_cache(ctx, steps, w=None):
             "segments": info["segments"], "hits": info["hits"], "misses": info["misses"]}
 
 
+def bench_wire(ctx, steps, n=60_000_000):
+    """Wire forms: a 60M-row selection bitmap into the int64 words of bitmap
+    pushdown (in 100,000-row batches at running offsets, as FilterPOp buffers
+    it, and in one call) and back per batch; a 2^16-block filter exported as
+    its two record batches, framed as an Arrow IPC stream, parsed and
+    imported.  Checked against the oracle's words / the original blocks."""
+    from flexpushdowndb_amd import datagen, ops, wire
+    from flexpushdowndb_amd.device import DeviceColumn
+    from oracle import oracle as O
+    rng = np.random.default_rng(12)
+    sel = rng.random(n) < 0.25
+    packed = np.packbits(sel, bitorder="little")
+    bm = np.zeros((n + 63) // 64 * 8, np.uint8)
+    bm[:len(packed)] = packed
+    d_bm = ctx.upload(bm)
+    nwords = (n + 63) // 64
+    d_words = ctx.zeros(nwords, np.int64)
+    out = ctx.zeros((n + 63) // 64 * 8, np.uint8)
+    wire.bitmap_to_words(ctx, d_bm.ptr, n, 0, d_words.ptr)
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(steps):
+        wire.bitmap_to_words(ctx, d_bm.ptr, n, 0, d_words.ptr)
+    to_ms = ctx.timer_stop() / steps
+    ctx.timer_start()
+    for _ in range(steps):
+        wire.bitmap_from_words(ctx, d_words.ptr, nwords, 0, n, out.ptr)
+    from_ms = ctx.timer_stop() / steps
+    words = d_words.numpy()
+    ok = np.array_equal(words.view(np.uint8)[:len(packed)], packed) and np.array_equal(out.numpy(len(packed)), packed)
+    # batch by batch at running offsets (FilterPOp::bufferBitMap)
+    chunk = 100_000
+    d_words2 = ctx.zeros(nwords, np.int64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s0 in range(0, n, chunk):
+        wire.bitmap_to_words(ctx, d_bm.ptr + s0 // 8, min(chunk, n - s0), s0, d_words2.ptr)
+    ctx.synchronize()
+    batched_ms = (time.perf_counter() - t0) * 1e3
+    ok = ok and np.array_equal(d_words2.numpy(), words)
+    head = O.bitmap_words([sel[:64 * 1562]])      # whole words only
+    ok = ok and np.array_equal(words[:len(head)], head)
+    # the 2^16-block filter of the bloom workload through an IPC stream
+    w = datagen.bloom_workload()
+    f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, w["kept"])], len(w["kept"]))
+    times = []
+    for _ in range(max(3, steps // 4)):
+        t0 = time.perf_counter()
+        data = wire.ipc_bytes(wire.blocked_filter_record_batches(ctx, f.blocks.ptr, f.log_num_blocks))
+        lnb, d_blocks = wire.blocked_filter_from_record_batches(ctx, wire.ipc_read(data))
+        ctx.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    ok = ok and lnb == f.log_num_blocks and np.array_equal(d_blocks.numpy(1 << lnb), f.blocks_numpy())
+    if not ok:
+        raise SystemExit("wire forms do not match the oracle")
+    alg = nwords * 8 * 2
+    return {"workload": f"{n}-row selection bitmap <-> int64 bitmap words; 2^{f.log_num_blocks}-block filter "
+                        f"through an Arrow IPC stream ({len(data)} bytes)",
+            "to_words_ms": to_ms, "from_words_ms": from_ms, "to_words_batched_ms": batched_ms,
+            "to_words_hbm_frac": alg / (to_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "filter_round_trip_ms": statistics.median(times), "ipc_bytes": len(data)}
+
+
 def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
     """A plain copy of n int64 values (8n bytes read, 8n written) with
     copy_rows_kernel: the reference point for the FETCH_SIZE / WRITE_SIZE
@@ -524,7 +587,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--only", choices=["join", "bloom", "groupby", "shuffle", "cache", "star", "serve", "calib"],
+    ap.add_argument("--only", choices=["join", "bloom", "groupby", "shuffle", "cache", "wire", "star", "serve",
+                                       "calib"],
                     default=None,
                     help="run one workload only (for profiler runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -561,6 +625,10 @@ def main():
     if args.only in (None, "cache"):
         res["cache"] = bench_cache(ctx, max(3, args.steps // 4), res["join"]["inputs"] if "join" in res else None)
         log(f"cache: cold {res['cache']['cold_step_ms']:.3f} ms, warm {res['cache']['warm_step_ms']:.3f} ms")
+    if args.only in (None, "wire"):
+        res["wire"] = bench_wire(ctx, args.steps)
+        log(f"wire: to words {res['wire']['to_words_ms']:.4f} ms, filter round trip "
+            f"{res['wire']['filter_round_trip_ms']:.3f} ms")
     if args.only in (None, "star"):
         res["star"] = bench_star(ctx, max(3, args.steps // 2), f"gfx950-{info['num_cus']}cu")
         log(f"star: batched {res['star']['batched_ms']:.3f} ms, plain {res['star']['plain_ms']:.3f} ms")
@@ -614,6 +682,8 @@ def main():
         line["hash_shuffle"] = res["shuffle"]
     if "cache" in res:
         line["segment_cache"] = res["cache"]
+    if "wire" in res:
+        line["wire_forms"] = res["wire"]
     if "star" in res:
         line["batched_star_join"] = res["star"]
     if "serve" in res:

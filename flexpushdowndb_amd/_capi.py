@@ -154,6 +154,10 @@ SIGNATURES = [
     ("fpdb_gather", _I32, [_P, _PCOL, _I, _P, _I64, _PCOL]),
     ("fpdb_copy_rows", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I64]),
     ("fpdb_shuffle", _I32, [_P, _PCOL, _I, _PCOL, _I, _I64, _I, _PCOL, _P, _PI64]),
+    ("fpdb_bitmap_to_words", _I32, [_P, _P, _I64, _I64, _P]),
+    ("fpdb_bitmap_from_words", _I32, [_P, _P, _I64, _I64, _I64, _P]),
+    ("fpdb_bloom_blocked_export", _I32, [_P, _P, _I, _P, _P]),
+    ("fpdb_bloom_blocked_import", _I32, [_P, _P, _P, _U64, ctypes.POINTER(_I), _P]),
     ("fpdb_cache_create", _I32, [_P, _U64, _I, ctypes.POINTER(_P)]),
     ("fpdb_cache_destroy", _I32, [_P]),
     ("fpdb_cache_store", _I32, [_P, ctypes.c_char_p, ctypes.c_char_p, _I64, _I64, _PCOL, ctypes.POINTER(_I)]),

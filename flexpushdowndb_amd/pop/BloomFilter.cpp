@@ -1,8 +1,11 @@
 #include "BloomFilter.h"
 
+#include <cstring>
 #include <sstream>
 
 #include "Globals.h"
+
+#include <algorithm>
 
 namespace fpdb::amd {
 
@@ -70,6 +73,114 @@ std::string ArrowBloomFilter::toJson() const {
   return o.str();
 }
 
+Expected<std::vector<BitmapRecordBatch>> ArrowBloomFilter::makeBitmapRecordBatches() const {
+  if (!valid_) return make_unexpected("ArrowBloomFilter: an invalid filter has no bit array");
+  BitmapRecordBatch masks, blocks;
+  masks.length = 136;
+  blocks.length = (int64_t)8 << logNumBlocks_;
+  masks.bytes.resize(136);
+  blocks.bytes.resize((size_t)blocks.length);
+  if (fpdb_bloom_blocked_export(defaultContext(), deviceBlocks(), logNumBlocks_, masks.bytes.data(),
+                                blocks.bytes.data()) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  return std::vector<BitmapRecordBatch>{masks, blocks};
+}
+
+// ArrowBloomFilter::saveBitmapRecordBatches (ArrowBloomFilter.cpp:77-95)
+Expected<void> ArrowBloomFilter::saveBitmapRecordBatches(const std::vector<BitmapRecordBatch>& batches) {
+  if (batches.size() != 2)
+    return make_unexpected("RecordBatch stream for ArrowBloomFilter's bitmap should contain two recordBatches");
+  const auto& m = batches[0];
+  const auto& b = batches[1];
+  if (m.type != FPDB_TYPE_UINT8 || b.type != FPDB_TYPE_UINT8 || m.bytes.size() != 136)
+    return make_unexpected("ArrowBloomFilter bitmap batches must be uint8, the first of 136 mask bytes");
+  fpdb_context* ctx = defaultContext();
+  int lnb = 0;
+  if (fpdb_bloom_blocked_import(ctx, m.bytes.data(), b.bytes.data(), b.bytes.size(), &lnb, nullptr) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  if (!blocks_ || lnb != logNumBlocks_) blocks_ = std::make_shared<DeviceBuffer>(ctx, b.bytes.size());
+  if (fpdb_bloom_blocked_import(ctx, m.bytes.data(), b.bytes.data(), b.bytes.size(), &lnb,
+                                static_cast<uint64_t*>(blocks_->data())) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  logNumBlocks_ = lnb;
+  valid_ = true;
+  return {};
+}
+
+// ---- wire form of bitmaps ------------------------------------------------------
+
+BitmapRecordBatch bitmapToRecordBatch(const std::vector<int64_t>& bitmap) {
+  BitmapRecordBatch b;
+  b.type = FPDB_TYPE_INT64;
+  b.length = (int64_t)bitmap.size();
+  b.bytes.resize(bitmap.size() * 8);
+  if (!bitmap.empty()) std::memcpy(b.bytes.data(), bitmap.data(), b.bytes.size());
+  return b;
+}
+
+Expected<std::vector<int64_t>> recordBatchToBitmap(const BitmapRecordBatch& batch) {
+  if (batch.fieldName != "bitmap") return make_unexpected("Bitmap array not found in the recordBatch");
+  if (batch.type != FPDB_TYPE_INT64) return make_unexpected("Type of bitmap array is not int64");
+  std::vector<int64_t> out((size_t)batch.length);
+  if (batch.bytes.size() != out.size() * 8) return make_unexpected("Bitmap batch has the wrong number of bytes");
+  if (!out.empty()) std::memcpy(out.data(), batch.bytes.data(), batch.bytes.size());
+  return out;
+}
+
+Expected<void> FilterBitmap::reserve(int64_t words) {
+  if (words <= capacityWords_) return {};
+  fpdb_context* ctx = defaultContext();
+  const int64_t cap = std::max<int64_t>(words, capacityWords_ * 2);
+  auto grown = std::make_shared<DeviceBuffer>(ctx, (uint64_t)cap * 8);
+  if (fpdb_fill(ctx, grown->data(), 0, grown->size()) != FPDB_OK) return make_unexpected(fpdb_last_error());
+  if (numWords_ && fpdb_copy_device(ctx, grown->data(), words_->data(), (uint64_t)numWords_ * 8) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  words_ = grown;
+  capacityWords_ = cap;
+  return {};
+}
+
+Expected<void> FilterBitmap::bufferBitMap(const uint8_t* d_bitmap, int64_t rowOffset, int64_t n) {
+  const int64_t words = (rowOffset + n + 63) / 64;   // resize(ceil((rowOffset + n) / 64)), FilterPOp.cpp:274
+  if (auto r = reserve(words); !r) return r;
+  numWords_ = std::max(numWords_, words);
+  numRows_ = std::max(numRows_, rowOffset + n);
+  if (fpdb_bitmap_to_words(defaultContext(), d_bitmap, n, rowOffset, static_cast<int64_t*>(words_->data())) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  return {};
+}
+
+Expected<std::shared_ptr<DeviceBuffer>> FilterBitmap::makeSelectionBitmap(int64_t startRowOffset,
+                                                                          int64_t numRows) const {
+  fpdb_context* ctx = defaultContext();
+  auto bm = std::make_shared<DeviceBuffer>(ctx, (uint64_t)(numRows + 63) / 64 * 8);
+  if (numRows == 0) return bm;
+  if (!words_) return make_unexpected("Bitmap not set");
+  if (fpdb_bitmap_from_words(ctx, static_cast<const int64_t*>(words_->data()), numWords_, startRowOffset, numRows,
+                             static_cast<uint8_t*>(bm->data())) != FPDB_OK)
+    return make_unexpected(fpdb_last_error());
+  return bm;
+}
+
+std::vector<int64_t> FilterBitmap::toHost() const {
+  std::vector<int64_t> out((size_t)numWords_);
+  if (numWords_)
+    checkStatus(fpdb_copy_to_host(defaultContext(), out.data(), words_->data(), out.size() * 8), "copy");
+  return out;
+}
+
+Expected<FilterBitmap> FilterBitmap::fromRecordBatch(const BitmapRecordBatch& batch) {
+  auto words = recordBatchToBitmap(batch);
+  if (!words) return make_unexpected(words.error());
+  FilterBitmap fb;
+  if (auto r = fb.reserve((int64_t)words->size()); !r) return make_unexpected(r.error());
+  fb.numWords_ = (int64_t)words->size();
+  fb.numRows_ = fb.numWords_ * 64;
+  if (!words->empty())
+    checkStatus(fpdb_copy_to_device(defaultContext(), fb.words_->data(), words->data(), words->size() * 8), "copy");
+  return fb;
+}
+
 // ---- legacy filter -------------------------------------------------------------
 
 BloomFilter::BloomFilter(int64_t capacity, double falsePositiveRate)
@@ -121,6 +232,23 @@ std::vector<int64_t> BloomFilter::getBitArray() const {
   std::vector<int64_t> out((size_t)(numBits_ / 64 + 1));
   checkStatus(fpdb_copy_to_host(defaultContext(), out.data(), words_->data(), out.size() * 8), "copy");
   return out;
+}
+
+// BloomFilter::makeBitmapRecordBatches / saveBitmapRecordBatches (BloomFilter.cpp:123-144)
+Expected<std::vector<BitmapRecordBatch>> BloomFilter::makeBitmapRecordBatches() const {
+  return std::vector<BitmapRecordBatch>{bitmapToRecordBatch(getBitArray())};
+}
+
+Expected<void> BloomFilter::saveBitmapRecordBatches(const std::vector<BitmapRecordBatch>& batches) {
+  if (batches.size() != 1)
+    return make_unexpected("RecordBatch stream for BloomFilter's bitmap should only contain one recordBatch");
+  auto words = recordBatchToBitmap(batches[0]);
+  if (!words) return make_unexpected(words.error());
+  if ((int64_t)words->size() != numBits_ / 64 + 1)
+    return make_unexpected("BloomFilter bitmap has " + std::to_string(words->size()) + " words, expected " +
+                           std::to_string(numBits_ / 64 + 1));
+  checkStatus(fpdb_copy_to_device(defaultContext(), words_->data(), words->data(), words->size() * 8), "copy");
+  return {};
 }
 
 std::string BloomFilter::toJson() const {

@@ -16,6 +16,22 @@ namespace fpdb::amd {
 
 enum BloomFilterType { BLOOM_FILTER, ARROW_BLOOM_FILTER };
 
+// One record batch of the bitmap wire form: a single column named "bitmap"
+// (ArrowSerializer::BITMAP_FIELD_NAME, fpdb-tuple/include/fpdb/tuple/serialization/ArrowSerializer.h:68)
+// of uint8 or int64 values, kept as its raw little-endian value bytes; the
+// Arrow IPC framing around it is the transport's.
+struct BitmapRecordBatch {
+  std::string fieldName = "bitmap";
+  fpdb_type type = FPDB_TYPE_UINT8;
+  int64_t length = 0;                 // values
+  std::vector<uint8_t> bytes;         // length * width bytes
+};
+
+// ArrowSerializer::bitmap_to_recordBatch / recordBatch_to_bitmap
+// (fpdb-tuple/src/serialization/ArrowSerializer.cpp:330-365).
+BitmapRecordBatch bitmapToRecordBatch(const std::vector<int64_t>& bitmap);
+Expected<std::vector<int64_t>> recordBatchToBitmap(const BitmapRecordBatch& batch);
+
 class BloomFilterBase {
  public:
   BloomFilterBase(BloomFilterType type, int64_t capacity, bool valid)
@@ -25,6 +41,10 @@ class BloomFilterBase {
   int64_t capacity() const { return capacity_; }
   bool valid() const { return valid_; }
   virtual std::string toJson() const = 0;
+  // The filter's bit array as record batches, and back
+  // (BloomFilterBase.h:32-33; ArrowBloomFilter.cpp:77-116, BloomFilter.cpp:123-144).
+  virtual Expected<std::vector<BitmapRecordBatch>> makeBitmapRecordBatches() const = 0;
+  virtual Expected<void> saveBitmapRecordBatches(const std::vector<BitmapRecordBatch>& batches) = 0;
 
  protected:
   BloomFilterType type_;
@@ -51,6 +71,11 @@ class ArrowBloomFilter : public BloomFilterBase {
   // makeBitmapRecordBatches (ArrowBloomFilter.cpp:77-116), as raw bytes.
   std::vector<uint8_t> masksBytes() const;
   std::string toJson() const override;
+  // Two uint8 batches: the 136 mask bytes, then the blocks.
+  Expected<std::vector<BitmapRecordBatch>> makeBitmapRecordBatches() const override;
+  // Takes the blocks of a received filter (its size may differ from this
+  // filter's); masks other than BloomFilterMasks() are an error.
+  Expected<void> saveBitmapRecordBatches(const std::vector<BitmapRecordBatch>& batches) override;
 
  private:
   std::vector<std::string> columnNames_;
@@ -76,12 +101,36 @@ class BloomFilter : public BloomFilterBase {
   const int64_t* deviceWords() const { return static_cast<const int64_t*>(words_->data()); }
   std::vector<int64_t> getBitArray() const;
   std::string toJson() const override;
+  // One int64 batch of the bit array words.
+  Expected<std::vector<BitmapRecordBatch>> makeBitmapRecordBatches() const override;
+  Expected<void> saveBitmapRecordBatches(const std::vector<BitmapRecordBatch>& batches) override;
 
  private:
   double fpr_;
   int64_t numBits_ = 0;
   std::vector<fpdb_universal_hash> hashFunctions_;
   std::shared_ptr<DeviceBuffer> words_;
+};
+
+// A filter's selection over all its input batches as int64 words in HBM,
+// the bitmap of FilterPOp's bitmap pushdown (FPDBStoreFilterBitmapWrapper::bitmap_,
+// fpdb-executor/src/physical/filter/FilterPOp.cpp:264-297).
+class FilterBitmap {
+ public:
+  // ORs the n-row selection bitmap of one batch in at rowOffset (bufferBitMap).
+  Expected<void> bufferBitMap(const uint8_t* d_bitmap, int64_t rowOffset, int64_t n);
+  // Rows [startRowOffset, startRowOffset + numRows) as a device selection
+  // bitmap of (numRows + 7) / 8 bytes (makeSelectionVector).
+  Expected<std::shared_ptr<DeviceBuffer>> makeSelectionBitmap(int64_t startRowOffset, int64_t numRows) const;
+  int64_t numRows() const { return numRows_; }
+  std::vector<int64_t> toHost() const;
+  BitmapRecordBatch toRecordBatch() const { return bitmapToRecordBatch(toHost()); }
+  static Expected<FilterBitmap> fromRecordBatch(const BitmapRecordBatch& batch);
+
+ private:
+  Expected<void> reserve(int64_t words);
+  std::shared_ptr<DeviceBuffer> words_;
+  int64_t numWords_ = 0, capacityWords_ = 0, numRows_ = 0;
 };
 
 // Resolves key column names to columns of a TupleSet.

@@ -315,6 +315,38 @@ fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_col
 fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns,
                         const int64_t* d_rows, int64_t num_rows, fpdb_column* out);
 
+/* ---- wire forms (filters and bitmaps shipped to the store) --------------------- */
+
+/* ORs an n-row selection bitmap (LSB-first bytes, as produced by the find and
+ * compare entry points) into int64 bitmap words at row row_offset, bit r % 64
+ * of word r / 64 being row r: FilterPOp::bufferBitMap
+ * (fpdb-executor/src/physical/filter/FilterPOp.cpp:264-279) with
+ * fpdb::util::setBit (fpdb-util/src/Util.cpp:115-117).  d_words must hold
+ * ceil((row_offset + n) / 64) words; they are not cleared (batch after batch
+ * at a growing row_offset builds the whole bitmap).  The words are the int64
+ * "bitmap" column of ArrowSerializer::bitmap_to_recordBatch
+ * (fpdb-tuple/src/serialization/ArrowSerializer.cpp:330-348). */
+fpdb_status fpdb_bitmap_to_words(fpdb_context* ctx, const uint8_t* d_bitmap, int64_t n, int64_t row_offset,
+                                 int64_t* d_words);
+/* Rows [row_offset, row_offset + n) of int64 bitmap words as a selection
+ * bitmap ((n + 7) / 8 bytes): FilterPOp::makeSelectionVector
+ * (FilterPOp.cpp:281-297) with fpdb::util::getBit (Util.cpp:123-125). */
+fpdb_status fpdb_bitmap_from_words(fpdb_context* ctx, const int64_t* d_words, int64_t num_words,
+                                   int64_t row_offset, int64_t n, uint8_t* d_bitmap);
+/* The two record batches of ArrowBloomFilter::makeBitmapRecordBatches
+ * (fpdb-executor/src/physical/bloomfilter/ArrowBloomFilter.cpp:97-116), each
+ * one uint8 column: the 136 mask bytes, then the 8 << log_num_blocks block
+ * bytes (little-endian uint64 blocks).  Either output may be NULL; copying the
+ * blocks synchronizes. */
+fpdb_status fpdb_bloom_blocked_export(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
+                                      uint8_t* masks_out, uint8_t* blocks_out);
+/* ArrowBloomFilter::saveBitmapRecordBatches (ArrowBloomFilter.cpp:77-95): the
+ * masks must equal BloomFilterMasks() (this library's find uses that table;
+ * anything else is FPDB_ERR_INVALID); block_bytes must be 8 x a power of two.
+ * Sets *log_num_blocks and, if d_blocks is not NULL, uploads the blocks. */
+fpdb_status fpdb_bloom_blocked_import(fpdb_context* ctx, const uint8_t* masks, const uint8_t* blocks,
+                                      uint64_t block_bytes, int* log_num_blocks, uint64_t* d_blocks);
+
 /* ---- device-resident segment cache ---------------------------------------------- */
 
 /* Column segments of table partitions kept in HBM under (partition, column)

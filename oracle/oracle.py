@@ -408,3 +408,29 @@ def pred_trans_plan(order, tables, joins):
         if fwd:
             transfer(src, tgt, sc, tc, "B")
     return steps, curr
+
+
+# --- wire forms -------------------------------------------------------------------
+
+def bitmap_words(batches):
+    """int64 words of a filter's selection over consecutive batches
+    (FilterPOp::bufferBitMap, fpdb-executor/src/physical/filter/FilterPOp.cpp:264-279,
+    with fpdb::util::setBit, fpdb-util/src/Util.cpp:115-117): row r of the
+    concatenation is bit r % 64 of word r / 64; ceil(rows / 64) words.
+    batches: list of boolean selection arrays."""
+    total = sum(len(b) for b in batches)
+    words = [0] * ((total + 63) // 64)
+    row = 0
+    for b in batches:
+        for i in np.nonzero(np.asarray(b, bool))[0]:
+            r = row + int(i)
+            words[r // 64] |= 1 << (r % 64)
+        row += len(b)
+    return np.array([w - (1 << 64) if w >= 1 << 63 else w for w in words], dtype=np.int64)
+
+
+def selection_from_words(words, start, n):
+    """FilterPOp::makeSelectionVector (FilterPOp.cpp:280-297): rows
+    [start, start + n) as a boolean array (fpdb::util::getBit, Util.cpp:123-125)."""
+    w = [int(x) & ((1 << 64) - 1) for x in words]
+    return np.array([(w[r // 64] >> (r % 64)) & 1 for r in range(start, start + n)], dtype=bool)

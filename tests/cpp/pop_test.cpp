@@ -376,6 +376,79 @@ static void testSegmentCache() {
   CHECK(small->hitNum() == 2 && small->missNum() == 4 && small->shardMissNum() == 2);
 }
 
+static void testWireForms() {
+  // a blocked filter built from 5000 keys, shipped as its two record batches
+  // and restored into a filter of another capacity: same blocks
+  std::vector<int64_t> keys(5000);
+  for (int64_t i = 0; i < 5000; ++i) keys[i] = i * 7919 % 100003;
+  Schema s;
+  s.fields = {Field{"k", FPDB_TYPE_INT64, false}};
+  auto ts = TupleSet::fromHost(defaultContext(), s, {hostInt64(keys)});
+  auto f = ArrowBloomFilter::make(5000, {"k"});
+  CHECK(f->build(ts).has_value());
+  auto batches = f->makeBitmapRecordBatches();
+  CHECK(batches.has_value());
+  if (!batches) return;
+  CHECK(batches->size() == 2 && (*batches)[0].bytes.size() == 136 &&
+        (*batches)[1].bytes.size() == ((size_t)8 << f->logNumBlocks()));
+  auto g = ArrowBloomFilter::make(10, {"k"});
+  CHECK(g->saveBitmapRecordBatches(*batches).has_value());
+  CHECK(g->logNumBlocks() == f->logNumBlocks() && g->blocksToHost() == f->blocksToHost());
+  auto bad = *batches;
+  bad[0].bytes[0] ^= 1;
+  CHECK(!g->saveBitmapRecordBatches(bad).has_value());
+  CHECK(!g->saveBitmapRecordBatches({(*batches)[1]}).has_value());
+
+  // the legacy filter's words as one int64 batch
+  auto lf = BloomFilter::make(5000, 0.01);
+  lf->init(11);
+  CHECK(lf->build(ts, {"k"}).has_value());
+  auto lb = lf->makeBitmapRecordBatches();
+  CHECK(lb.has_value() && lb->size() == 1 && (*lb)[0].type == FPDB_TYPE_INT64);
+  auto lg = BloomFilter::make(5000, 0.01);
+  lg->init(11);
+  CHECK(lg->saveBitmapRecordBatches(*lb).has_value() && lg->getBitArray() == lf->getBitArray());
+
+  // a filter's selection over three batches as bitmap words, then per batch back
+  fpdb_context* ctx = defaultContext();
+  std::vector<int64_t> sizes = {70, 0, 200};
+  std::vector<std::vector<uint8_t>> sel;
+  FilterBitmap fb;
+  int64_t row = 0;
+  std::vector<int64_t> want(5, 0);
+  for (int64_t n : sizes) {
+    std::vector<uint8_t> bits((n + 63) / 64 * 8 + 8, 0);
+    for (int64_t i = 0; i < n; ++i)
+      if ((i * 37 + n) % 5 == 0) {
+        bits[i / 8] |= (uint8_t)(1u << (i % 8));
+        want[(row + i) / 64] |= (int64_t)(1ULL << ((row + i) % 64));
+      }
+    DeviceBuffer d(ctx, bits.size());
+    checkStatus(fpdb_copy_to_device(ctx, d.data(), bits.data(), bits.size()), "copy");
+    CHECK(fb.bufferBitMap(static_cast<uint8_t*>(d.data()), row, n).has_value());
+    sel.push_back(bits);
+    row += n;
+  }
+  want.resize((row + 63) / 64);
+  CHECK(fb.toHost() == want);
+  auto rb = fb.toRecordBatch();
+  auto back = FilterBitmap::fromRecordBatch(rb);
+  CHECK(back.has_value());
+  if (!back) return;
+  row = 0;
+  for (size_t b = 0; b < sizes.size(); ++b) {
+    const int64_t n = sizes[b];
+    auto slice = back->makeSelectionBitmap(row, n);
+    CHECK(slice.has_value());
+    if (slice && n) {
+      std::vector<uint8_t> got((n + 7) / 8);
+      checkStatus(fpdb_copy_to_host(ctx, got.data(), (*slice)->data(), got.size()), "copy");
+      CHECK(std::equal(got.begin(), got.end(), sel[b].begin()));
+    }
+    row += n;
+  }
+}
+
 static void testEmptyAndErrors() {
   // an empty build side still yields an (empty) result with the output schema
   Executor ex;
@@ -731,6 +804,7 @@ int main(int argc, char** argv) {
       {"shuffle", testShuffle},
       {"shuffled_join", testShuffledJoin},
       {"segment_cache", testSegmentCache},
+      {"wire_forms", testWireForms},
   };
   for (auto& t : tests) {
     int before = g_failures;

@@ -402,3 +402,34 @@ def test_pred_trans_join_types_and_cycles():
     steps, _ = O.pred_trans_plan("bfs", tables, [("A", "B", ["a"], ["b"], "INNER"), ("B", "C", ["b"], ["c"], "INNER"),
                                                  ("C", "A", ["c"], ["a"], "INNER")])
     assert len(steps) == 4
+
+
+def test_bitmap_words_by_hand_and_framing():
+    # rows 0 and 63 of batch one, row 1 of batch two (row 65 overall), row 2
+    # of a third batch starting at row 130 (row 132): words 0, 1, 2
+    b1 = np.zeros(66, bool)
+    b1[[0, 63]] = True
+    b2 = np.zeros(64, bool)
+    b2[1] = True   # rows 66..129 -> row 67
+    b3 = np.zeros(5, bool)
+    b3[2] = True   # rows 130..134 -> row 132
+    words = O.bitmap_words([b1, b2, b3])
+    assert len(words) == 3
+    assert int(words[0]) & ((1 << 64) - 1) == 1 | (1 << 63)
+    assert int(words[1]) == 1 << (67 - 64)
+    assert int(words[2]) == 1 << (132 - 128)
+    assert np.array_equal(O.selection_from_words(words, 66, 64), b2)
+    assert np.array_equal(O.selection_from_words(words, 130, 5), b3)
+    # the Arrow framing of the int64 bitmap and the blocked filter's two uint8 batches
+    pa = pytest.importorskip("pyarrow")
+    from flexpushdowndb_amd import wire
+    batch = wire.bitmap_record_batch(words)
+    assert batch.schema.names == ["bitmap"] and batch.schema.field(0).type == pa.int64()
+    back = wire.ipc_read(wire.ipc_bytes([batch]))
+    assert len(back) == 1 and np.array_equal(wire.record_batch_to_bitmap(back[0]), words)
+    masks = O.bloom_masks()
+    blocks = np.arange(64, dtype=np.uint8)
+    b = wire.ipc_read(wire.ipc_bytes([wire._batch(masks), wire._batch(blocks)]))
+    assert [x.num_rows for x in b] == [136, 64] and b[0].schema.field(0).type == pa.uint8()
+    with pytest.raises(ValueError):
+        wire.record_batch_to_bitmap(b[0])       # uint8, not the int64 bitmap
