@@ -449,6 +449,68 @@ static void testWireForms() {
   }
 }
 
+static void testTwoPhaseGroupBy() {
+  // the parallel group-by plan (PrePToPTransformer.cpp:434-560): a partial
+  // GroupPOp per input partition, a ShufflePOp on the group key, a reducing
+  // GroupPOp per slot (Sum of sums, Count becomes a Sum of counts); the same
+  // groups as one GroupPOp over all rows
+  const int64_t n = 20000;
+  const int parts = 4, slots = 3;
+  std::vector<int64_t> k(n), v(n);
+  for (int64_t i = 0; i < n; ++i) {
+    k[i] = (i * 7919) % 257;
+    v[i] = (i * 31) % 1000 - 500;
+  }
+  Schema s;
+  s.fields = {Field{"k", FPDB_TYPE_INT64, false}, Field{"v", FPDB_TYPE_INT64, false}};
+  std::map<int64_t, std::pair<int64_t, int64_t>> want;
+  for (int64_t i = 0; i < n; ++i) {
+    want[k[i]].first += v[i];
+    want[k[i]].second += 1;
+  }
+  Executor ex;
+  auto collate = std::make_shared<CollatePOp>("collate");
+  ex.put(collate);
+  std::vector<std::shared_ptr<GroupPOp>> reducers;
+  for (int r = 0; r < slots; ++r) {
+    std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum_v", "sum_v"),
+                                                          std::make_shared<Sum>("cnt", "cnt")};
+    reducers.push_back(std::make_shared<GroupPOp>("reduce" + std::to_string(r),
+                                                  std::vector<std::string>{"k", "sum_v", "cnt"}, 0,
+                                                  std::vector<std::string>{"k"}, fns));
+    ex.put(reducers.back());
+    connect(reducers.back(), collate);
+  }
+  for (int p = 0; p < parts; ++p) {
+    const int64_t b = n * p / parts, e = n * (p + 1) / parts;
+    auto ts = TupleSet::fromHost(defaultContext(), s,
+                                 {hostInt64({k.begin() + b, k.begin() + e}), hostInt64({v.begin() + b, v.begin() + e})});
+    auto src = std::make_shared<SourcePOp>("scan" + std::to_string(p), std::vector<std::shared_ptr<TupleSet>>{ts});
+    std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum_v", "v"),
+                                                          std::make_shared<Count>("cnt")};
+    auto partial = std::make_shared<GroupPOp>("partial" + std::to_string(p),
+                                              std::vector<std::string>{"k", "sum_v", "cnt"}, 0,
+                                              std::vector<std::string>{"k"}, fns);
+    auto sh = std::make_shared<ShufflePOp>("shuffle" + std::to_string(p), std::vector<std::string>{}, 0,
+                                           std::vector<std::string>{"k"});
+    for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{src, partial, sh}) ex.put(op);
+    connect(src, partial);
+    connect(partial, sh);
+    for (auto& r : reducers) connect(sh, r);
+  }
+  auto r = ex.execute();
+  CHECK(r.has_value());
+  if (!r) return;
+  auto out = collate->tupleSet().value();
+  auto h = out->toHost();
+  std::map<int64_t, std::pair<int64_t, int64_t>> got;
+  for (int64_t i = 0; i < out->numRows(); ++i) {
+    CHECK(got.count(h[0].int64At(i)) == 0);     // each key reduced in exactly one slot
+    got[h[0].int64At(i)] = {h[1].int64At(i), h[2].int64At(i)};
+  }
+  CHECK(got == want);
+}
+
 static void testEmptyAndErrors() {
   // an empty build side still yields an (empty) result with the output schema
   Executor ex;
@@ -805,6 +867,7 @@ int main(int argc, char** argv) {
       {"shuffled_join", testShuffledJoin},
       {"segment_cache", testSegmentCache},
       {"wire_forms", testWireForms},
+      {"two_phase_group_by", testTwoPhaseGroupBy},
   };
   for (auto& t : tests) {
     int before = g_failures;
