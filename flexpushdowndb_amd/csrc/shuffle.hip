@@ -43,6 +43,7 @@ constexpr int kShufMaxSlots = 4096;
 #define FPDB_SHUF_RUNS_PER_CU 0
 #endif
 constexpr int kShufRunsPerCu = FPDB_SHUF_RUNS_PER_CU;
+constexpr int64_t kShufMaxCounts = int64_t(1) << 24;   // 16 Mi (slot, run) entries: 192 MiB of scratch
 #ifndef FPDB_SHUF_MIN_WAVES
 #define FPDB_SHUF_MIN_WAVES 3
 #endif
@@ -405,9 +406,12 @@ static fpdb_status run_shuffle(Context* c, const ColSet& k, const ColSet& ic, co
   const int bits = bits_for(S);
   const ShufCountFn<PID> count_fn = shuffle_count_for<PID>(key_mode(k), (S & (S - 1)) == 0);
   const ShufScatterFn<PID> scatter_fn = shuffle_scatter_kernel<PID>;
-  // runs: kShufRunsPerCu x the scatter workgroups that stay resident at
-  // once, or one tile per workgroup when that is 0
+  // runs: one tile per workgroup (kShufRunsPerCu == 0, the measured best),
+  // or kShufRunsPerCu x the scatter workgroups that stay resident at once.
+  // The per-(slot, run) counts and bases are bounded by kShufMaxCounts
+  // entries: beyond that the runs grow longer instead.
   int64_t runs = tiles;
+  if (S * tiles > kShufMaxCounts) runs = std::max<int64_t>(1, kShufMaxCounts / S);
   if (kShufRunsPerCu > 0) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(scatter_fn),

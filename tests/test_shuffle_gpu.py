@@ -113,3 +113,19 @@ def test_shuffle_rejects_bad_arguments(ctx):
     st = C.lib().fpdb_shuffle(ctx.handle, C.columns_array([k.c()]), 1, C.columns_array([kv.c()]), 1, 10, 2,
                               C.columns_array([o.c() for o in out]), None, offs.ctypes.data_as(C._PI64))
     assert st == C.FPDB_ERR_INVALID
+
+
+def test_shuffle_many_slots_long_input(ctx):
+    # 4096 slots over 17M rows: the per-(slot, tile) bookkeeping would pass
+    # its bound, so workgroups take runs of several tiles; same output
+    n, slots = 17_000_000, 4096
+    print(f"shuffle {n} rows {slots} slots", flush=True)
+    rng = np.random.default_rng(21)
+    k = rng.integers(0, 2 ** 62, n, dtype=np.int64)
+    rid = np.arange(n, dtype=np.int32)
+    kc, rc = DeviceColumn.from_numpy(ctx, k), DeviceColumn.from_numpy(ctx, rid)
+    out, offsets = ops.shuffle(ctx, [kc], [kc, rc], n, slots)
+    order, want = O.shuffle_order(O.arrow6_hash([(k, None)]), slots)
+    assert np.array_equal(offsets, want)
+    assert np.array_equal(out[1].numpy(n), rid[order])
+    assert np.array_equal(out[0].numpy(n), k[order])
