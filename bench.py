@@ -195,7 +195,15 @@ def bench_bloom(ctx, steps):
     if not (np.array_equal(fs.blocks_numpy(), ref_s) and np.array_equal(
             bm.numpy(125_000), O.blocked_find(fs.log_num_blocks, ref_s, O.arrow6_hash([(w["probe"][:1_000_000], None)])))):
         raise SystemExit("bloom filter (LDS-resident) output does not match the oracle")
+    # CPU: the oracle's hash + FindImp over the same 60M probe keys
+    m = n
+    t0 = time.perf_counter()
+    O.blocked_find(f.log_num_blocks, ref, O.arrow6_hash([(w["probe"][:m], None)]))
+    cpu_s = time.perf_counter() - t0
     return {
+        "cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
+                         "sample": f"the whole workload: oracle Arrow 6 hash + blocked-filter Find (C++) over all "
+                                   f"{m} probe keys against the same 2^{f.log_num_blocks}-block filter, {cpu_s:.2f} s"},
         "workload": f"blocked bloom use: filter from {len(kept)} keys (2^{f.log_num_blocks} blocks), "
                     f"{n} int64 probe keys",
         "find_kernel_ms": find_ms, "find_gkeys_per_s": n / (find_ms * 1e-3) / 1e9,
@@ -254,7 +262,18 @@ def bench_groupby(ctx, steps):
     if not ok:
         raise SystemExit("group-by output does not match the oracle's closed form")
     alg = n * (4 + 8 + 8 + 8)
-    return {"workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
+    # CPU: the oracle's hash group-by (C++ unordered_map, first-appearance
+    # order) over the same 60M rows
+    from oracle import oracle as O
+    m = n
+    t0 = time.perf_counter()
+    O.groupby(w["key"][:m].astype(np.uint64), sums_i64=(w["qty"][:m], w["price"][:m]), sums_f64=(w["disc"][:m],),
+              max_groups=100_000)
+    cpu_s = time.perf_counter() - t0
+    return {"cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "Grows/s", "cores": 1, "kind": "port",
+                             "sample": f"the whole workload: oracle group-by (COUNT, SUM int64 x2, SUM float64) "
+                                       f"over all {m} rows, {cpu_s:.2f} s"},
+            "workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
                         "COUNT, SUM int64 x2, SUM float64", "groups": int(len(k)),
             "consume_kernels_ms": cons_ms, "local_ms": local_ms, "scatter_ms": scatter_ms,
             "compact_ms": compact_ms, "consume_wall_ms": statistics.median(wall),
@@ -295,7 +314,31 @@ def bench_star(ctx, steps, device_name):
         raise SystemExit("star join result does not match the expected join + group-by")
     plain_ms = statistics.median(plain)
     fact_bytes = len(w["measure"]) * 8 * len(w["dims"])
-    return {"workload": "star join, 60,000,000 fact rows in 64 lognormal(0,1) partitions, 4 dimensions "
+    # CPU: the same query over all fact rows with the oracle's hash joins
+    # (one per dimension, filtered dimension rows as the build side) and its
+    # group-by
+    from oracle import oracle as O
+    m = len(w["measure"])
+    t0 = time.perf_counter()
+    keep = np.ones(m, bool)
+    grp = {}
+    for d, dim in enumerate(w["dims"]):
+        rows = np.nonzero(dim["attr"] < dim["keep_below"])[0]
+        op, ob = O.join_pairs(dim["key"][rows], w["fact_fk"][d][:m])
+        hit = np.zeros(m, bool)
+        hit[op] = True
+        keep &= hit
+        if d in (0, 3):
+            g = np.zeros(m, np.int64)
+            g[op] = dim["group"][rows[ob]]
+            grp[d] = g
+    key = (grp[0][keep].astype(np.uint64) << np.uint64(32)) | grp[3][keep].astype(np.uint64)
+    O.groupby(key, sums_i64=(w["measure"][:m][keep],))
+    cpu_s = time.perf_counter() - t0
+    return {"cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "G fact rows/s", "cores": 1, "kind": "port",
+                             "sample": f"the whole workload, all {m} fact rows (no predicate transfer): 4 oracle "
+                                       f"hash joins (C++ unordered_map) + oracle group-by, {cpu_s:.2f} s"},
+            "workload": "star join, 60,000,000 fact rows in 64 lognormal(0,1) partitions, 4 dimensions "
                         "(30,000/200,000/2,000,000/2,556 rows kept at 20%/4%/1%/14%), forward + backward "
                         "blocked bloom filters, 4 joins, group-by on 2 dimension attributes",
             "batched_ms": batched_ms, "plain_ms": plain_ms, "speedup": plain_ms / batched_ms,
