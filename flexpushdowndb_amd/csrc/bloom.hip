@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kSliceThreads) void blocked_find_kernel(
   const uint64_t lo = (uint64_t)q << kSliceLog;
   const uint64_t cnt = reach - lo < (1ULL << kSliceLog) ? reach - lo : (1ULL << kSliceLog);
   for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
-  for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) s_blk[i] = blocks[lo + i];
+  stage_words(s_blk, blocks + lo, cnt);
   __syncthreads();
   const uint32_t bmask = (uint32_t)(reach - 1);
   const int lane = threadIdx.x & 63;

@@ -16,6 +16,7 @@
 #include "bloom_masks.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fpdb {
 
@@ -91,27 +92,58 @@ struct PredByteFlag {
 
 static __constant__ ExpandedMasks c_masks = expand_masks();
 
-// Exclusive scan of the tile counts (one workgroup per segment of `tiles`
-// counts: workgroup s scans cnt[s * tiles, (s + 1) * tiles) into the same
-// range of base and stores its sum in total[s]).  Each thread owns
-// kScanPerThread consecutive counts, all loaded before any is used, so a scan
-// of up to 1024 * kScanPerThread tiles waits for memory once; longer inputs
-// are taken in chunks of that size.
-constexpr int kScanPerThread = 16;
-static __global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* __restrict__ cnt, int64_t tiles,
-                                                         uint64_t* __restrict__ base, int64_t* total) {
-  __shared__ uint64_t s_w[16];
+// Exclusive scan of the tile counts (one workgroup of kScanThreads threads per
+// segment of `tiles` counts: workgroup s scans cnt[s * tiles, (s + 1) * tiles)
+// into the same range of base and stores its sum in total[s]).  Each thread
+// owns kScanPerThread consecutive counts; global memory is read and written
+// coalesced (consecutive threads, consecutive counts) and the counts change
+// hands through LDS, since one workgroup's strided accesses would keep its
+// CU's texture-address unit busy on one cache line per lane.  Inputs longer
+// than kTileScanChunk counts are taken in chunks of that size, the next
+// chunk's loads in flight while one is scanned.  (16 counts per thread, one chunk of
+// 128 KiB for 16,384 tiles, measured slower on MI355X: 8.5 against 5.2 us for
+// 5,859 tiles.)
+constexpr int kScanThreads = 1024;
+constexpr int kScanPerThread = 8;
+constexpr int kTileScanChunk = kScanThreads * kScanPerThread;
+static __global__ __launch_bounds__(kScanThreads) void tile_scan_kernel(const uint32_t* __restrict__ cnt,
+                                                                        int64_t tiles, uint64_t* __restrict__ base,
+                                                                        int64_t* total) {
+  __shared__ __align__(16) uint64_t s_stage[kTileScanChunk];   // the chunk's counts (as uint32), then its bases
+  __shared__ uint64_t s_w[kScanThreads / 64];
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_stage);
   cnt += (int64_t)blockIdx.x * tiles;
   base += (int64_t)blockIdx.x * tiles;
   total += blockIdx.x;
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
   uint64_t carry = 0;
-  for (int64_t c0 = 0; c0 < tiles; c0 += (int64_t)blockDim.x * kScanPerThread) {
-    const int64_t t0 = c0 + (int64_t)threadIdx.x * kScanPerThread;
+  uint32_t g[kScanPerThread];
+  auto load_chunk = [&](int64_t c0) {
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) {
+      const int64_t i = c0 + j * kScanThreads + tid;
+      g[j] = i < tiles ? cnt[i] : 0u;
+    }
+  };
+  load_chunk(0);
+  for (int64_t c0 = 0; c0 < tiles; c0 += kTileScanChunk) {
+    const int len = tiles - c0 < kTileScanChunk ? (int)(tiles - c0) : kTileScanChunk;
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) s_cnt[j * kScanThreads + tid] = g[j];
+    __syncthreads();
+    // the next chunk's counts are in flight while this one is scanned
+    if (c0 + kTileScanChunk < tiles) load_chunk(c0 + kTileScanChunk);
     uint32_t v[kScanPerThread];
 #pragma unroll
-    for (int j = 0; j < kScanPerThread; ++j) v[j] = t0 + j < tiles ? cnt[t0 + j] : 0u;
+    for (int j = 0; j < kScanPerThread; j += 4) {
+      const uint4 q = reinterpret_cast<const uint4*>(s_cnt)[(tid * kScanPerThread + j) >> 2];
+      v[j] = q.x;
+      v[j + 1] = q.y;
+      v[j + 2] = q.z;
+      v[j + 3] = q.w;
+    }
     uint64_t mine = 0;
 #pragma unroll
     for (int j = 0; j < kScanPerThread; ++j) mine += v[j];
@@ -121,22 +153,35 @@ static __global__ __launch_bounds__(1024) void tile_scan_kernel(const uint32_t* 
       if (lane >= off) incl += u;
     }
     if (lane == 63) s_w[wave] = incl;
-    __syncthreads();
+    __syncthreads();   // also: every thread has read its counts from the stage
     uint64_t before = carry, chunk = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
       if (w < wave) before += s_w[w];
       chunk += s_w[w];
     }
     before += incl - mine;
 #pragma unroll
     for (int j = 0; j < kScanPerThread; ++j) {
-      if (t0 + j < tiles) base[t0 + j] = before;
+      s_stage[tid * kScanPerThread + j] = before;
       before += v[j];
     }
-    carry += chunk;
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScanPerThread; ++j) {
+      const int i = j * kScanThreads + tid;
+      if (i < len) base[c0 + i] = s_stage[i];
+    }
+    carry += chunk;
+    __syncthreads();   // the stage and s_w are reused by the next chunk
   }
-  if (threadIdx.x == 0) *total = (int64_t)carry;
+  if (tid == 0) *total = (int64_t)carry;
+}
+
+// Scans `segments` segments of `tiles` counts each (see tile_scan_kernel).
+inline void launch_tile_scan(Context* c, int segments, const uint32_t* cnt, int64_t tiles, uint64_t* base,
+                             int64_t* total) {
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(segments), dim3(kScanThreads), 0, c->stream, cnt, tiles, base, total);
 }
 
 // Count pass: selected rows of each tile.
@@ -159,6 +204,47 @@ __global__ __launch_bounds__(256) void compact_count_kernel(Pred pred, int64_t n
     uint32_t t = 0;
     for (int w = 0; w < kCompactWaves; ++w) t += s_w[w];
     tile_cnt[tile] = t;
+  }
+}
+
+// Count pass for a packed bitmap: a wave counts whole tiles, lane l holding
+// word l of each 64 words of the tile, so a tile costs one coalesced load per
+// 64 words; waves loop over tiles, kBitmapCountUnroll tiles in flight.  (One
+// small workgroup per tile, as in compact_count_kernel, is bound by workgroup
+// dispatch when each reads only a few hundred bytes.)
+constexpr int kCompactTileWords = kCompactWaves * kCompactChunks;
+static_assert(kCompactTileWords % 64 == 0, "a tile is whole wave-loads of bitmap words");
+constexpr int kBitmapCountUnroll = 2;
+static __global__ __launch_bounds__(256) void compact_count_bitmap_kernel(const uint8_t* __restrict__ bm, int64_t n,
+                                                                          int64_t tiles,
+                                                                          uint32_t* __restrict__ tile_cnt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const bool aligned = (reinterpret_cast<uintptr_t>(bm) & 7) == 0;
+  const int64_t full_words = n >> 6;   // words whose 64 rows all exist
+  for (int64_t t0 = wave; t0 < tiles; t0 += nwaves * kBitmapCountUnroll) {
+    uint32_t cnt[kBitmapCountUnroll];
+#pragma unroll
+    for (int u = 0; u < kBitmapCountUnroll; ++u) {
+      const int64_t t = t0 + u * nwaves;
+      cnt[u] = 0;
+#pragma unroll
+      for (int j = 0; j < kCompactTileWords; j += 64) {
+        const int64_t w = t * kCompactTileWords + j + lane;
+        // past the last row (t >= tiles included) the slow path reads nothing and gives 0
+        const uint64_t word = aligned && w < full_words ? reinterpret_cast<const uint64_t*>(bm)[w]
+                                                        : bitmap_word_slow(bm, w, n);
+        cnt[u] += (uint32_t)__popcll(word);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBitmapCountUnroll; ++u) {
+      uint32_t v = cnt[u];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      const int64_t t = t0 + u * nwaves;
+      if (lane == 0 && t < tiles) tile_cnt[t] = v;
+    }
   }
 }
 
@@ -297,8 +383,14 @@ fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const Ou
   int64_t* total = d_num_out ? d_num_out : c->d_counter;
   {
     ProfScope ps(c, prof_name);
-    hipLaunchKernelGGL((compact_count_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, n, tcnt);
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tcnt, tiles, tbase, total);
+    if constexpr (std::is_same<Pred, PredBitmap>::value) {
+      const int g = grid_for(c, ceil_div(tiles, kBitmapCountUnroll), 256 / 64, 8);
+      hipLaunchKernelGGL(compact_count_bitmap_kernel, dim3(g), dim3(256), 0, c->stream, pred.bm, n, tiles, tcnt);
+    } else {
+      hipLaunchKernelGGL((compact_count_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, n,
+                         tcnt);
+    }
+    launch_tile_scan(c, 1, tcnt, tiles, tbase, total);
     hipLaunchKernelGGL((compact_copy_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, in, out,
                        n, bitmap_out, index_out, tbase);
   }

@@ -218,6 +218,29 @@ __device__ __forceinline__ void store_w(uint8_t* base, int width, int64_t i, uin
   }
 }
 
+// Copies count 8-byte words src[i * src_stride] into dst[i] (LDS) with the
+// whole workgroup.  Each thread issues kStageWords loads before its first LDS
+// write: a loop of one load and one write per trip would wait for memory once
+// per trip.  The caller synchronises the workgroup before reading dst.
+constexpr int kStageWords = 8;
+__device__ __forceinline__ void stage_words(uint64_t* dst, const uint64_t* __restrict__ src, uint64_t count,
+                                            uint64_t src_stride = 1) {
+  const uint64_t nt = blockDim.x;
+  for (uint64_t i0 = threadIdx.x; i0 < count; i0 += nt * kStageWords) {
+    uint64_t v[kStageWords];
+#pragma unroll
+    for (int j = 0; j < kStageWords; ++j) {
+      const uint64_t i = i0 + (uint64_t)j * nt;
+      v[j] = src[(i < count ? i : i0) * src_stride];
+    }
+#pragma unroll
+    for (int j = 0; j < kStageWords; ++j) {
+      const uint64_t i = i0 + (uint64_t)j * nt;
+      if (i < count) dst[i] = v[j];
+    }
+  }
+}
+
 // Loads N values of one column at once (indices already clamped to valid
 // rows): the width switch sits outside the per-value loop, so the N loads are
 // issued back to back instead of each in a branch that waits for it.

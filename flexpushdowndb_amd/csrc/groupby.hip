@@ -546,7 +546,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
   uint64_t* skeys = lds;
   uint32_t* lcnt = reinterpret_cast<uint32_t*>(lds + stride);
   const uint64_t* part = partial + (size_t)blockIdx.x * stride * G.rec_words;
-  for (uint32_t s = threadIdx.x; s < stride; s += blockDim.x) skeys[s] = part[(size_t)s * G.rec_words];
+  stage_words(skeys, part, stride, G.rec_words);
   for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x) lcnt[p] = 0;
   __syncthreads();
   LdsTable T;

@@ -428,18 +428,38 @@ __global__ __launch_bounds__(256) void hj_emit_kernel(const Slot* __restrict__ s
 // (surrogate keys, permutations) gets a collision-free table indexed by
 // slot = key - kmin: build row (kEmptyRow where no key), inline payload, and a
 // presence bitmap of one bit per slot.  No hashing, no chains.  The build runs
-// without atomics: min/max by per-workgroup partials, a plain scatter, then a
-// verify pass that flags a duplicate key (two rows scattered to one slot: one
-// of them does not find itself there), then the bitmap.
+// without atomics in three launches: per-workgroup min/max partials together
+// with the clear of every slot of the capacity; a plain scatter; a verify pass
+// that flags a duplicate key (two rows scattered to one slot: one of them does
+// not find itself there) together with the bitmap.  The scatter and verify
+// workgroups each reduce the few minmax partials themselves, which costs less
+// than a launch of its own.
 struct DenseMeta {
   int64_t kmin, kmax, count, dup;
 };
 
+constexpr int kMinMaxThreads = 1024;
+
+// Per-workgroup {min, max, count} of the non-null build keys into partials
+// (3 words per workgroup), and every slot of the capacity cleared (build row
+// kEmptyRow, in the row array or in the row half of the {row, payload} pairs).
 template <int KM>
-__global__ __launch_bounds__(256) void hj_minmax_kernel(KeySpec k, int64_t n, int64_t* __restrict__ partials) {
+__global__ __launch_bounds__(kMinMaxThreads) void hj_minmax_kernel(KeySpec k, int64_t n,
+                                                                   int64_t* __restrict__ partials,
+                                                                   uint32_t* __restrict__ rows,
+                                                                   uint2* __restrict__ pairs, uint64_t cap,
+                                                                   DenseMeta* meta) {
+  const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = gtid; i < cap; i += gthreads) {
+    if (pairs)
+      pairs[i] = make_uint2(kEmptyRow, 0u);
+    else
+      rows[i] = kEmptyRow;
+  }
+  if (gtid == 0) meta->dup = 0;
   int64_t lo = INT64_MAX, hi = INT64_MIN, cnt = 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+  for (int64_t r = (int64_t)gtid; r < n; r += (int64_t)gthreads) {
     int64_t key;
     if (!get_key<KM>(k, r, &key)) continue;
     lo = key < lo ? key : lo;
@@ -452,7 +472,7 @@ __global__ __launch_bounds__(256) void hj_minmax_kernel(KeySpec k, int64_t n, in
     hi = h2 > hi ? h2 : hi;
     cnt += __shfl_xor(cnt, off);
   }
-  __shared__ int64_t s[3][4];
+  __shared__ int64_t s[3][kMinMaxThreads / 64];
   const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) { s[0][wave] = lo; s[1][wave] = hi; s[2][wave] = cnt; }
   __syncthreads();
@@ -468,56 +488,44 @@ __global__ __launch_bounds__(256) void hj_minmax_kernel(KeySpec k, int64_t n, in
   }
 }
 
-__global__ __launch_bounds__(256) void hj_minmax_reduce_kernel(const int64_t* __restrict__ partials, int blocks,
-                                                               DenseMeta* meta) {
-  int64_t lo = INT64_MAX, hi = INT64_MIN, cnt = 0;
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-    lo = partials[3 * b] < lo ? partials[3 * b] : lo;
-    hi = partials[3 * b + 1] > hi ? partials[3 * b + 1] : hi;
-    cnt += partials[3 * b + 2];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    int64_t l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
-    lo = l2 < lo ? l2 : lo;
-    hi = h2 > hi ? h2 : hi;
-    cnt += __shfl_xor(cnt, off);
-  }
-  __shared__ int64_t s[3][4];
-  const int wave = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { s[0][wave] = lo; s[1][wave] = hi; s[2][wave] = cnt; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
-      lo = s[0][w] < lo ? s[0][w] : lo;
-      hi = s[1][w] > hi ? s[1][w] : hi;
-      cnt += s[2][w];
+struct DenseRange {
+  int64_t kmin, kmax, count;
+  uint64_t span;   // slots the keys span; 0 when there is no key or they do not fit cap slots
+};
+
+// The build keys' range from the minmax partials, reduced by the first wave of
+// the calling workgroup and shared through LDS.  Every thread of the
+// workgroup must call it (it synchronises the workgroup).
+__device__ __forceinline__ DenseRange dense_range(const int64_t* __restrict__ partials, int parts, uint64_t cap) {
+  __shared__ int64_t s_r[3];
+  if (threadIdx.x < 64) {
+    int64_t lo = INT64_MAX, hi = INT64_MIN, cnt = 0;
+    for (int b = threadIdx.x; b < parts; b += 64) {
+      const int64_t l = partials[3 * b], h = partials[3 * b + 1];
+      lo = l < lo ? l : lo;
+      hi = h > hi ? h : hi;
+      cnt += partials[3 * b + 2];
     }
-    meta->kmin = lo;
-    meta->kmax = hi;
-    meta->count = cnt;
-    meta->dup = 0;
+    for (int off = 32; off > 0; off >>= 1) {
+      int64_t l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
+      lo = l2 < lo ? l2 : lo;
+      hi = h2 > hi ? h2 : hi;
+      cnt += __shfl_xor(cnt, off);
+    }
+    if (threadIdx.x == 0) {
+      s_r[0] = lo;
+      s_r[1] = hi;
+      s_r[2] = cnt;
+    }
   }
-}
-
-// Number of slots the keys span (0 when they do not fit `cap` slots).
-__device__ __forceinline__ uint64_t dense_span(const DenseMeta* meta, uint64_t cap) {
-  if (meta->count == 0) return 0;
-  uint64_t span = (uint64_t)meta->kmax - (uint64_t)meta->kmin + 1;
-  return (span != 0 && span <= cap) ? span : 0;
-}
-
-// Clears the slots of the span: build row kEmptyRow (in the row array, or in
-// the row half of the {row, payload} pairs).
-__global__ __launch_bounds__(256) void hj_dense_fill_kernel(uint32_t* __restrict__ rows, uint2* __restrict__ pairs,
-                                                            uint64_t cap, const DenseMeta* meta) {
-  const uint64_t span = dense_span(meta, cap);
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < span;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    if (pairs)
-      pairs[i] = make_uint2(kEmptyRow, 0u);
-    else
-      rows[i] = kEmptyRow;
-  }
+  __syncthreads();
+  DenseRange r;
+  r.kmin = s_r[0];
+  r.kmax = s_r[1];
+  r.count = s_r[2];
+  const uint64_t span = (uint64_t)r.kmax - (uint64_t)r.kmin + 1;
+  r.span = (r.count > 0 && span != 0 && span <= cap) ? span : 0;
+  return r;
 }
 
 // With an inline payload the scatter writes {row, payload} pairs into one
@@ -525,11 +533,13 @@ __global__ __launch_bounds__(256) void hj_dense_fill_kernel(uint32_t* __restrict
 // hj_dense_finish_kernel then separates the two halves.
 template <int KM>
 __global__ __launch_bounds__(256) void hj_dense_scatter_kernel(uint32_t* __restrict__ rows, uint2* __restrict__ pairs,
-                                                               uint64_t cap, const DenseMeta* meta, KeySpec k,
-                                                               int64_t n, const uint8_t* __restrict__ payload,
+                                                               uint64_t cap, const int64_t* __restrict__ partials,
+                                                               int parts, KeySpec k, int64_t n,
+                                                               const uint8_t* __restrict__ payload,
                                                                int64_t payload_off) {
-  if (dense_span(meta, cap) == 0) return;
-  const uint64_t kmin = (uint64_t)meta->kmin;
+  const DenseRange dr = dense_range(partials, parts, cap);
+  if (dr.span == 0) return;
+  const uint64_t kmin = (uint64_t)dr.kmin;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
     int64_t key;
@@ -545,13 +555,21 @@ __global__ __launch_bounds__(256) void hj_dense_scatter_kernel(uint32_t* __restr
 // After the scatter, in one launch: the slots split into the row and payload
 // arrays (when the scatter wrote pairs), the presence bitmap (bit s of word
 // s / 64 set iff slot s holds a key), and the duplicate check (a duplicate
-// key leaves one of its rows not finding itself in its slot).
+// key leaves one of its rows not finding itself in its slot).  Workgroup 0
+// stores the range into meta for the host's one read-back.
 template <int KM>
 __global__ __launch_bounds__(256) void hj_dense_finish_kernel(const uint2* __restrict__ pairs, uint64_t cap,
+                                                              const int64_t* __restrict__ partials, int parts,
                                                               DenseMeta* meta, uint32_t* __restrict__ rows,
                                                               uint32_t* __restrict__ pay, uint64_t* __restrict__ bits,
                                                               KeySpec k, int64_t n) {
-  const uint64_t span = dense_span(meta, cap);
+  const DenseRange dr = dense_range(partials, parts, cap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    meta->kmin = dr.kmin;
+    meta->kmax = dr.kmax;
+    meta->count = dr.count;
+  }
+  const uint64_t span = dr.span;
   if (span == 0) return;
   const int lane = threadIdx.x & 63;
   const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
@@ -576,7 +594,7 @@ __global__ __launch_bounds__(256) void hj_dense_finish_kernel(const uint2* __res
     if (lane == 0) bits[w] = word;
   }
   // build rows
-  const uint64_t kmin = (uint64_t)meta->kmin;
+  const uint64_t kmin = (uint64_t)dr.kmin;
   bool dup = false;
   for (int64_t r = (int64_t)gtid; r < n; r += (int64_t)gthreads) {
     int64_t key;
@@ -591,7 +609,9 @@ __global__ __launch_bounds__(256) void hj_dense_finish_kernel(const uint2* __res
 // bitmap, 64 rows per word, LSB first) and per tile of kDenseTileRows rows its
 // output count.  The presence bitmap is staged in LDS when it fits
 // (BITS_IN_LDS), else read from L2.  Workgroups loop over tiles; one wave
-// counts one tile at a time.
+// counts one tile at a time.  A wave loads its first tile's keys before the
+// bitmap is staged and each next tile's keys before it tests the current one,
+// so it waits for memory about once instead of once per tile.
 constexpr int kDenseItems = 4;
 constexpr int64_t kDenseTileRows = (int64_t)kDenseItems * kProbeThreads;
 constexpr int kDensePrefetch = 4;   // probe columns loaded before the in-tile scan
@@ -603,42 +623,55 @@ __global__ __launch_bounds__(kDenseCountThreads) void hj_dense_count_kernel(
     const uint64_t* __restrict__ bits, int64_t kmin, uint64_t span, KeySpec pk, int64_t n, int64_t tiles,
     uint64_t* __restrict__ hitmap, uint32_t* __restrict__ tile_cnt) {
   extern __shared__ uint64_t s_bits[];
-  const uint64_t nbw = (span + 63) >> 6;
-  if (BITS_IN_LDS) {
-    for (uint64_t i = threadIdx.x; i < nbw; i += blockDim.x) s_bits[i] = bits[i];
-    __syncthreads();
-  }
-  const uint64_t* __restrict__ bw = BITS_IN_LDS ? s_bits : bits;
+  constexpr int U = kDenseTileWords;   // a tile's keys are all loaded together
   const int lane = threadIdx.x & 63;
   const int64_t wave = (int64_t)blockIdx.x * (kDenseCountThreads / 64) + (threadIdx.x >> 6);
   const int64_t nwaves = (int64_t)gridDim.x * (kDenseCountThreads / 64);
-  constexpr int U = kDenseTileWords;   // a tile's keys are all loaded together
+  int64_t key[U], rc[U];
+  bool live[U];
+  if (wave < tiles) probe_keys_n<KM, U, 64>(pk, n, wave * kDenseTileRows + lane, rc, key, live);
+  const uint64_t nbw = (span + 63) >> 6;
+  if (BITS_IN_LDS) {
+    stage_words(s_bits, bits, nbw);
+    __syncthreads();
+  }
+  const uint64_t* __restrict__ bw = BITS_IN_LDS ? s_bits : bits;
   for (int64_t t = wave; t < tiles; t += nwaves) {
+    int64_t nkey[U], nrc[U];
+    bool nlive[U];
+    const int64_t tn = t + nwaves;
+    if (tn < tiles) probe_keys_n<KM, U, 64>(pk, n, tn * kDenseTileRows + lane, nrc, nkey, nlive);
+    // every lane reads its word (slot 0's for keys outside the span), so the
+    // U reads are issued together
+    uint64_t hw[U], pw[U];
+    bool in[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t slot = (uint64_t)key[u] - (uint64_t)kmin;
+      in[u] = live[u] && slot < span;
+      const uint64_t s = in[u] ? slot : 0;
+      pw[u] = bw[s >> 6] >> (s & 63);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) hw[u] = __ballot(in[u] & (bool)(pw[u] & 1));
     uint32_t cnt = 0;
-    for (int w0 = 0; w0 < kDenseTileWords; w0 += U) {
-      const int64_t row0 = (t * kDenseTileWords + w0) * 64 + lane;
-      int64_t key[U], rc[U];
-      bool live[U];
-      probe_keys_n<KM, U, 64>(pk, n, row0, rc, key, live);
-      uint64_t hw[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const uint64_t slot = (uint64_t)key[u] - (uint64_t)kmin;
-        const bool in = live[u] && slot < span;
-        const uint64_t s = in ? slot : 0;
-        hw[u] = __ballot(in && ((bw[s >> 6] >> (s & 63)) & 1));
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int64_t w = t * kDenseTileWords + w0 + u;
-        const int64_t r0 = w * 64;
-        if (lane == 0 && r0 < n) hitmap[w] = hw[u];
-        const int64_t rows_here = r0 >= n ? 0 : (n - r0 < 64 ? n - r0 : 64);
-        if (JT == kJtOuter) cnt += (uint32_t)rows_here;
-        else if (JT != kJtMark) cnt += (uint32_t)__popcll(hw[u]);
-      }
+    for (int u = 0; u < U; ++u) {
+      const int64_t w = t * kDenseTileWords + u;
+      const int64_t r0 = w * 64;
+      if (lane == 0 && r0 < n) hitmap[w] = hw[u];
+      const int64_t rows_here = r0 >= n ? 0 : (n - r0 < 64 ? n - r0 : 64);
+      if (JT == kJtOuter) cnt += (uint32_t)rows_here;
+      else if (JT != kJtMark) cnt += (uint32_t)__popcll(hw[u]);
     }
     if (lane == 0) tile_cnt[t] = cnt;
+    if (tn < tiles) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        key[u] = nkey[u];
+        live[u] = nlive[u];
+      }
+    }
   }
 }
 
@@ -940,7 +973,7 @@ static fpdb_status dense_probe(Context* c, fpdb_hash_table* t, int jt, bool trac
     }
 #undef FPDB_DCOUNT_JT
 #undef FPDB_DCOUNT
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tcnt, tiles, tbase, total);
+    launch_tile_scan(c, 1, tcnt, tiles, tbase, total);
     if ((jt != kJtMark && oc.n > 0 && out_cap > 0) || track) {
       dim3 g((unsigned)tiles), b(kProbeThreads);
 #define FPDB_DEMIT(KM, JT, TR, NR)                                                                             \
@@ -1111,7 +1144,7 @@ static fpdb_status probe_impl(Context* c, fpdb_hash_table* t, int join_type, con
     }
 #undef FPDB_MATCH_JT
 #undef FPDB_MATCH_LAUNCH
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tcnt, tiles, tbase, total);
+    launch_tile_scan(c, 1, tcnt, tiles, tbase, total);
     if (jt != kJtMark && oc.n > 0 && out_cap > 0) {
 #define FPDB_EMIT_LAUNCH(KM, JT)                                                                      \
   hipLaunchKernelGGL((hj_emit_kernel<KM, JT>), g, b, 0, c->stream, t->slots, mask, pk, n, pc, bc, oc, out_cap, \
@@ -1168,20 +1201,23 @@ static fpdb_status try_dense_build(Context* c, fpdb_hash_table* t, const KeySpec
     return st;
   }
   uint32_t* pay = pv ? arr + cap : nullptr;
-  const int g = grid_for(c, n, 256);
-  const int gf = grid_for(c, (int64_t)cap, 256);
+  // one minmax workgroup per CU at most, so the scatter and finish
+  // workgroups each reduce at most num_cus partials
+  const int gm = grid_for(c, std::max<int64_t>(n, (int64_t)cap), kMinMaxThreads, 1);
+  const int g = grid_for(c, n, 256, 4);
+  const int gf = grid_for(c, (int64_t)cap, 256, 4);
   int64_t* parts = reinterpret_cast<int64_t*>(c->d_partials);   // 3 words per workgroup
+  FPDB_CHECK(3 * gm <= kMaxPartials, FPDB_ERR_INTERNAL, "hash join dense build: too many minmax workgroups");
   {
     ProfScope ps(c, "hj_build");
     switch (key_mode_of(k)) {
-#define FPDB_DENSE_BUILD(KM)                                                                                \
-  hipLaunchKernelGGL(hj_minmax_kernel<KM>, dim3(g), dim3(256), 0, c->stream, k, n, parts);                  \
-  hipLaunchKernelGGL(hj_minmax_reduce_kernel, dim3(1), dim3(256), 0, c->stream, parts, g, meta);            \
-  hipLaunchKernelGGL(hj_dense_fill_kernel, dim3(gf), dim3(256), 0, c->stream, arr, pairs, cap, meta);       \
-  hipLaunchKernelGGL(hj_dense_scatter_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, pairs, cap, meta, k,  \
-                     n, pv, poff);                                                                          \
-  hipLaunchKernelGGL(hj_dense_finish_kernel<KM>, dim3(std::max(g, gf)), dim3(256), 0, c->stream, pairs, cap,    \
-                     meta, arr, pay, bits, k, n)
+#define FPDB_DENSE_BUILD(KM)                                                                                   \
+  hipLaunchKernelGGL(hj_minmax_kernel<KM>, dim3(gm), dim3(kMinMaxThreads), 0, c->stream, k, n, parts, arr,     \
+                     pairs, cap, meta);                                                                        \
+  hipLaunchKernelGGL(hj_dense_scatter_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, pairs, cap, parts, gm, \
+                     k, n, pv, poff);                                                                          \
+  hipLaunchKernelGGL(hj_dense_finish_kernel<KM>, dim3(std::max(g, gf)), dim3(256), 0, c->stream, pairs, cap,     \
+                     parts, gm, meta, arr, pay, bits, k, n)
       case 1: FPDB_DENSE_BUILD(1); break;
       case 2: FPDB_DENSE_BUILD(2); break;
       default: FPDB_DENSE_BUILD(0); break;

@@ -434,7 +434,7 @@ static fpdb_status run_shuffle(Context* c, const ColSet& k, const ColSet& ic, co
       ProfScope ps1(c, "shuffle_count");
       hipLaunchKernelGGL(count_fn, grid, block, sizeof(uint32_t) * S, c->stream, k, n, S, bits, tiles, cnt, pids);
     }
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(S), dim3(1024), 0, c->stream, cnt, runs, tbase, totals);
+    launch_tile_scan(c, S, cnt, runs, tbase, totals);
     hipLaunchKernelGGL(shuffle_offsets_kernel, dim3(1), dim3(1024), 0, c->stream, totals, (int)S, doff);
     {
       ProfScope ps2(c, "shuffle_scatter");

@@ -312,7 +312,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
         loff[d] = fits ? used : -1;
       }
       if (fits) {
-        for (int64_t b = threadIdx.x; b < words; b += blockDim.x) s_filt[used + b] = P.dim[d].fbf[b];
+        stage_words(s_filt + used, P.dim[d].fbf, (uint64_t)words);
         used += words;
       }
     }
