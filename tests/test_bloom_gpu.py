@@ -6,6 +6,8 @@ restatement of the Arrow 6 key hash (parity with the reference binary's hash is
 unpinned; the filter arithmetic given the hash is pinned).  Legacy filter:
 fully pinned, compared bit for bit.
 """
+import types
+
 import numpy as np
 import pytest
 
@@ -174,6 +176,22 @@ def test_filter_compact_plain(ctx):
     assert np.array_equal(out[0].numpy(), a[sel]) and np.array_equal(out[1].numpy(), b[sel])
 
 
+@pytest.mark.parametrize("misalign", [1, 3, 8])
+def test_filter_compact_unaligned_bitmap(ctx, misalign):
+    """A selection bitmap that does not start on an 8-byte boundary: the count
+    and copy passes assemble each 64-row word from bytes."""
+    rng = np.random.default_rng(40 + misalign)
+    n = 300_001
+    sel = rng.random(n) > 0.45
+    a = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64)
+    packed = np.packbits(sel, bitorder="little")
+    buf = ctx.upload(np.concatenate([np.full(misalign, 0xFF, np.uint8), packed, np.full(8, 0xFF, np.uint8)]))
+    out, cnt = ops.filter_compact(ctx, [DeviceColumn.from_numpy(ctx, a)], n,
+                                  types.SimpleNamespace(ptr=buf.ptr + misalign))
+    assert cnt == int(sel.sum())
+    assert np.array_equal(out[0].numpy(cnt), a[sel])
+
+
 def test_legacy_build_find_bit_exact(ctx):
     rng = np.random.default_rng(21)
     n = 40_000
@@ -231,5 +249,8 @@ def test_bloom_workload_full_size_properties(ctx):
     assert cnt == int(np.unpackbits(got, bitorder="little")[:len(probe)].sum())
     out, c2, bm2 = f.filter([pc], [0], len(probe))
     assert c2 == cnt and np.array_equal(bm2.numpy(len(got)), got)
+    # 14,649 compaction tiles (two scan chunks): every kept key at its rank
+    sel = np.unpackbits(got, bitorder="little")[:len(probe)].astype(bool)
+    assert np.array_equal(out[0].numpy(c2), probe[sel])
     head_bits = np.unpackbits(got[:250_000], bitorder="little").astype(bool)
     assert head_bits[np.isin(probe[:2_000_000], kept)].all()   # no false negatives

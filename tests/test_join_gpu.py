@@ -283,3 +283,31 @@ def test_dense_and_hash_same_output(ctx):
     assert cnt_d == cnt_h == len(pk)
     for a, b in zip(out_d, out_h):
         assert np.array_equal(a.numpy(cnt_d), b.numpy(cnt_h))
+
+
+@pytest.mark.parametrize("spread", [1, 3])
+def test_probe_longer_than_one_scan_chunk(ctx, spread):
+    """10M probe rows are 9,766 tiles of 1,024 rows: the tile-count scan takes
+    two chunks of 8,192 tiles.  Dense form (keys a permutation) and hash form
+    (keys spread 3x); build keys k, probe keys uniform over twice the key
+    range, so about half the probe rows miss.  Closed form: the output rows
+    are the hitting probe rows in probe order, the payload that of the build
+    row holding the key."""
+    rng = np.random.default_rng(31 + spread)
+    nb, npr = 500_000, 10_000_000
+    perm = rng.permutation(nb).astype(np.int64)
+    bk = perm * spread + (spread > 1)
+    pay = rng.integers(-2 ** 31, 2 ** 31, nb).astype(np.int32)
+    raw = rng.integers(0, 2 * nb, npr).astype(np.int64)
+    pk = raw * spread + (spread > 1)
+    dbk, dpay, dpk = (DeviceColumn.from_numpy(ctx, a) for a in (bk, pay, pk))
+    t = ops.HashJoinTable(ctx, [dbk], nb, payload=dpay)
+    assert t.form() == ("dense" if spread == 1 else "hash")
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], npr, [dpk], [dbk, "payload"], capacity=npr)
+    hit = raw < nb
+    assert cnt == int(hit.sum())
+    inv = np.empty(nb, np.int64)
+    inv[perm] = np.arange(nb)
+    assert np.array_equal(out[0].numpy(cnt), pk[hit])
+    assert np.array_equal(out[1].numpy(cnt), pk[hit])
+    assert np.array_equal(out[2].numpy(cnt), pay[inv[raw[hit]]])
