@@ -54,19 +54,21 @@ def load_traffic(name):
         return json.load(f).get("bytes_per_launch")
 
 
-def bench_join(ctx, steps, warmup, spread=1, w=None):
+def bench_join(ctx, steps, warmup, spread=1, w=None, n_build=1_000_000):
     """spread = 1: the workload as specified (build keys a permutation of
     0..n-1, so the table takes its dense form); spread = 3: every key k
     becomes 3k + 1 (same rows, same matches), so the keys span 3x the build
-    rows and the table takes its open-addressing hash form."""
+    rows and the table takes its open-addressing hash form.  n_build: build
+    rows of a generated workload (the probe keys stay uniform over them)."""
     from flexpushdowndb_amd import _capi as C
     from flexpushdowndb_amd import datagen
     from flexpushdowndb_amd.device import DeviceColumn
     L = C.lib()
-    n_build, n_probe = 1_000_000, 6_000_000
+    n_probe = 6_000_000
     if w is None:
         log("join: generating inputs")
         w = datagen.join_workload(n_build, n_probe)
+    n_build = len(w["build_key"])
     if spread != 1:
         w = dict(w)
         w["build_key"] = w["build_key"] * spread + 1
@@ -656,6 +658,10 @@ def main():
         res["join_hash"] = {k: v for k, v in hj.items() if k != "inputs"}
         log(f"join (keys spread 3x): {hj['ms_per_step']:.4f} ms/step, probe kernel {hj['probe_ms']:.4f} ms "
             f"({hj['table_form']} table)")
+        hs = bench_join(ctx, args.steps, args.warmup, spread=3, n_build=4_000)
+        res["join_small_hash"] = {k: v for k, v in hs.items() if k != "inputs"}
+        log(f"join (4,000 build rows, keys spread 3x): {hs['ms_per_step']:.4f} ms/step, probe kernel "
+            f"{hs['probe_ms']:.4f} ms ({hs['table_form']} table)")
     if args.only in (None, "bloom"):
         res["bloom"] = bench_bloom(ctx, args.steps)
         log(f"bloom: find {res['bloom']['find_kernel_ms']:.4f} ms")
@@ -710,6 +716,15 @@ def main():
         line["join_hash_table"] = {
             "workload": "the same join with every key k replaced by 3k + 1 (keys span 3x the build rows, so the "
                         "build takes the open-addressing hash form)",
+            "value_grows_per_s": round(h["value"], 4), "ms_per_step": round(h["ms_per_step"], 5),
+            "probe_kernel_ms": round(h["probe_ms"], 5), "build_kernel_ms": round(h["build_ms"], 5),
+            "probe_hbm_frac": round(h["alg_bytes"] / (h["probe_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "table_form": h["table_form"]}
+    if "join_small_hash" in res:
+        h = res["join_small_hash"]
+        line["join_small_hash_table"] = {
+            "workload": "build 4,000 rows {int64 key 3k + 1 over a permutation, int32 pay}, the same 6,000,000 "
+                        "probe rows shape (keys uniform over the build keys); an 8,192-slot (128 KiB) hash table",
             "value_grows_per_s": round(h["value"], 4), "ms_per_step": round(h["ms_per_step"], 5),
             "probe_kernel_ms": round(h["probe_ms"], 5), "build_kernel_ms": round(h["build_ms"], 5),
             "probe_hbm_frac": round(h["alg_bytes"] / (h["probe_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

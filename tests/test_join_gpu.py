@@ -311,3 +311,21 @@ def test_probe_longer_than_one_scan_chunk(ctx, spread):
     assert np.array_equal(out[0].numpy(cnt), pk[hit])
     assert np.array_equal(out[1].numpy(cnt), pk[hit])
     assert np.array_equal(out[2].numpy(cnt), pay[inv[raw[hit]]])
+
+
+@pytest.mark.parametrize("nb,dup", [(2, True), (97, True), (3000, False), (4096, True), (4096, False),
+                                    (4097, True)])
+def test_small_hash_tables(ctx, nb, dup):
+    """Small build sides (tables of 64 to 16,384 slots, the sizes of filtered
+    dimension tables).  Keys spread over +-2^40 (never the dense form),
+    duplicate or distinct build keys, nulls on both sides, probe keys that hit
+    and miss; every join type against the oracle."""
+    rng = np.random.default_rng(50 + nb + dup)
+    if dup:
+        bk = rng.choice(rng.integers(-2 ** 40, 2 ** 40, max(1, nb // 2)), nb).astype(np.int64)
+    else:
+        bk = rng.permutation(np.unique(rng.integers(-2 ** 40, 2 ** 40, 2 * nb))[:nb]).astype(np.int64)
+    pk = np.concatenate([rng.choice(bk, 150_000), rng.integers(-2 ** 40, 2 ** 40, 50_001)]).astype(np.int64)
+    rng.shuffle(pk)
+    _check_all_join_types(ctx, bk, pk, rng.random(nb) > 0.1, rng.random(len(pk)) > 0.1, "hash")
+    _check_all_join_types(ctx, bk, pk, None, None, "hash")
