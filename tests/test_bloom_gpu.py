@@ -254,3 +254,18 @@ def test_bloom_workload_full_size_properties(ctx):
     assert np.array_equal(out[0].numpy(c2), probe[sel])
     head_bits = np.unpackbits(got[:250_000], bitorder="little").astype(bool)
     assert head_bits[np.isin(probe[:2_000_000], kept)].all()   # no false negatives
+
+
+@pytest.mark.parametrize("extra", [-1, 0, 1])
+def test_filter_compact_at_scan_chunk_boundary(ctx, extra):
+    """8,192 compaction tiles of 4,096 rows fill exactly one chunk of the
+    tile-count scan; one more row starts a second chunk, one fewer leaves the
+    last tile partial."""
+    n = 8192 * 4096 + extra
+    rng = np.random.default_rng(70 + extra)
+    sel = rng.random(n) < 0.3
+    a = np.arange(n, dtype=np.int64) * 7 - 5
+    out, cnt = ops.filter_compact(ctx, [DeviceColumn.from_numpy(ctx, a)], n,
+                                  ctx.upload(np.packbits(sel, bitorder="little")))
+    assert cnt == int(sel.sum())
+    assert np.array_equal(out[0].numpy(cnt), a[sel])

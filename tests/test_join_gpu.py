@@ -329,3 +329,24 @@ def test_small_hash_tables(ctx, nb, dup):
     rng.shuffle(pk)
     _check_all_join_types(ctx, bk, pk, rng.random(nb) > 0.1, rng.random(len(pk)) > 0.1, "hash")
     _check_all_join_types(ctx, bk, pk, None, None, "hash")
+
+
+@pytest.mark.parametrize("extra", [0, 1])
+def test_dense_probe_at_scan_chunk_boundary(ctx, extra):
+    """8,192 probe tiles of 1,024 rows fill exactly one chunk of the tile-count
+    scan; one more row starts a second chunk."""
+    rng = np.random.default_rng(80 + extra)
+    nb, npr = 100_000, 8192 * 1024 + extra
+    bk = rng.permutation(nb).astype(np.int64)
+    pay = rng.integers(0, 2 ** 31, nb).astype(np.int32)
+    pk = rng.integers(0, 2 * nb, npr).astype(np.int64)
+    dbk, dpay, dpk = (DeviceColumn.from_numpy(ctx, a) for a in (bk, pay, pk))
+    t = ops.HashJoinTable(ctx, [dbk], nb, payload=dpay)
+    assert t.form() == "dense"
+    out, cnt = t.probe(C.JOIN_INNER, [dpk], npr, [dpk], ["payload"], capacity=npr)
+    hit = pk < nb
+    inv = np.empty(nb, np.int64)
+    inv[bk] = np.arange(nb)
+    assert cnt == int(hit.sum())
+    assert np.array_equal(out[0].numpy(cnt), pk[hit])
+    assert np.array_equal(out[1].numpy(cnt), pay[inv[pk[hit]]])
