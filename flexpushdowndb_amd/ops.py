@@ -198,9 +198,10 @@ class SegmentCache:
         self.handle = h
 
     def close(self):
-        if self.handle:
+        # after its context is closed the native object is gone with it
+        if self.handle and self.ctx.handle:
             C.lib().fpdb_cache_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -294,9 +295,10 @@ class HashJoinTable:
         return "dense" if f.value == 1 else "hash"
 
     def close(self):
-        if self.handle:
+        # after its context is closed the native object is gone with it
+        if self.handle and self.ctx.handle:
             C.lib().fpdb_hj_table_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -400,9 +402,10 @@ class GroupBy:
                                                        max_groups, ctypes.byref(self.handle)))
 
     def close(self):
-        if self.handle:
+        # after its context is closed the native object is gone with it
+        if self.handle and self.ctx.handle:
             C.lib().fpdb_groupby_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

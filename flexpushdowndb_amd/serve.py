@@ -37,9 +37,10 @@ class Server:
         C.check(C.lib().fpdb_serve_create(ctx.handle, ctypes.byref(S), ctypes.byref(cfg), ctypes.byref(self.handle)))
 
     def close(self):
-        if self.handle:
+        # after its context is closed the native object is gone with it
+        if self.handle and self.ctx.handle:
             C.lib().fpdb_serve_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

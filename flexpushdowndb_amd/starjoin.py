@@ -80,9 +80,10 @@ class StarJoin:
         self.tuning = tuning
 
     def close(self):
-        if self.handle:
+        # after its context is closed the native object is gone with it
+        if self.handle and self.ctx.handle:
             C.lib().fpdb_star_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

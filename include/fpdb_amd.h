@@ -69,6 +69,9 @@ typedef struct fpdb_context fpdb_context;
  * reference binds in POpContext (fpdb-executor/src/physical/POpContext.cpp:25-58). */
 fpdb_status fpdb_context_create(int device, uint64_t hbm_budget_bytes, void* stream,
                                 fpdb_context** out);
+/* Destroy every object created on a context (tables, filters, group tables,
+ * caches, star joins, servers) before the context itself: their destroy calls
+ * return memory to it. */
 fpdb_status fpdb_context_destroy(fpdb_context* ctx);
 void* fpdb_context_stream(fpdb_context* ctx);
 fpdb_status fpdb_context_synchronize(fpdb_context* ctx);
