@@ -350,3 +350,19 @@ def test_dense_probe_at_scan_chunk_boundary(ctx, extra):
     assert cnt == int(hit.sum())
     assert np.array_equal(out[0].numpy(cnt), pk[hit])
     assert np.array_equal(out[1].numpy(cnt), pay[inv[pk[hit]]])
+
+
+def test_objects_outliving_their_context():
+    """Closing a context before the tables made on it: the wrappers' later
+    close (or garbage collection) must not touch the destroyed context."""
+    from flexpushdowndb_amd.device import Context
+    c2 = Context(0)
+    keys = DeviceColumn.from_numpy(c2, np.arange(1000, dtype=np.int64))
+    t = ops.HashJoinTable(c2, [keys], 1000)
+    g = ops.GroupBy(c2, [C.TYPE_INT64], [(C.AGG_COUNT_ALL, None, False)], 100)
+    cache = ops.SegmentCache(c2, 1 << 20)
+    c2.close()
+    t.close()
+    g.close()
+    cache.close()
+    del t, g, cache, keys
