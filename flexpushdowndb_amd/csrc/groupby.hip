@@ -544,10 +544,13 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
   const int nacc = NACC > 0 ? NACC : A.n;
   const uint32_t stride = G.nslots + 1;
   uint64_t* skeys = lds;
-  uint32_t* lcnt = reinterpret_cast<uint32_t*>(lds + stride);
+  // next record position of this workgroup in each bucket, starting at its
+  // scanned base: one LDS atomic per record, no global read
+  unsigned long long* lpos = reinterpret_cast<unsigned long long*>(lds + stride);
   const uint64_t* part = partial + (size_t)blockIdx.x * stride * G.rec_words;
   stage_words(skeys, part, stride, G.rec_words);
-  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x) lcnt[p] = 0;
+  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x)
+    lpos[p] = base[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x];
   __syncthreads();
   LdsTable T;
   T.keys = skeys;
@@ -586,7 +589,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
       uint64_t h = fmix64(k);
       if (lds_find(T, k, h, kLdsProbes) >= 0) continue;
       uint32_t p = bucket_of(h, G.bucket_shift);
-      uint64_t at = base[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
+      uint64_t at = atomicAdd(&lpos[p], 1ULL);
       uint64_t* rec = pool + at * G.rec_words;
       if constexpr (NACC > 0) {
         uint64_t w[2 + NACC];
@@ -608,7 +611,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
     const uint64_t* src = part + (size_t)s * G.rec_words;
     if ((long long)src[1] == LLONG_MAX) continue;
     uint32_t p = bucket_of(fmix64(src[0]), G.bucket_shift);
-    uint64_t at = base[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x] + atomicAdd(&lcnt[p], 1u);
+    uint64_t at = atomicAdd(&lpos[p], 1ULL);
     uint64_t* rec = pool + at * G.rec_words;
     if constexpr (NACC > 0) {
       uint64_t w[2 + NACC];
@@ -1051,9 +1054,9 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   }
   t->local_slots = (uint32_t)((budget - hist_bytes) / per_slot) - 1;
   t->local_lds = (size_t)(t->local_slots + 1) * per_slot + hist_bytes;
-  t->scatter_lds = (size_t)(t->local_slots + 1) * 8 + hist_bytes;
+  t->scatter_lds = (size_t)(t->local_slots + 1) * 8 + (size_t)nb * 8;   // keys + 64-bit bucket positions
   t->max_blocks = std::max(1, c->num_cus);
-  if (t->local_slots < 64 || cslots < 64) {
+  if (t->local_slots < 64 || cslots < 64 || t->scatter_lds > budget) {
     delete t;
     return fail(FPDB_ERR_INVALID, "group-by: too many aggregates for the LDS table");
   }
