@@ -2,8 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
 // calls this file.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
-// baseline use it, and only as the checker (or the timed CPU baseline), never
-// as the thing measured on the GPU.
+// baseline use it, and only to check results (or as the timed CPU baseline),
+// never as the thing measured on the GPU.
 //
 // Every function cites the reference file:line whose behaviour it restates.
 // The reference paths are relative to the FlexPushdownDB source tree.

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the checker.
+"""Pins the CPU oracle before it is trusted to check the device results.
 
 Each check is chosen so that a plausible restatement mistake (wrong seeding,
 wrong draw composition, an off-by-one in sizing, a transposed operand) fails:
