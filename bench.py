@@ -202,7 +202,9 @@ def bench_bloom(ctx, steps):
     t0 = time.perf_counter()
     O.blocked_find(f.log_num_blocks, ref, O.arrow6_hash([(w["probe"][:m], None)]))
     cpu_s = time.perf_counter() - t0
+    legacy = bench_bloom_legacy(ctx, steps, w, pc, a_keys, bm, cnt)
     return {
+        "legacy_filter": legacy,
         "cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
                          "sample": f"the whole workload: oracle Arrow 6 hash + blocked-filter Find (C++) over all "
                                    f"{m} probe keys against the same 2^{f.log_num_blocks}-block filter, {cpu_s:.2f} s"},
@@ -217,6 +219,60 @@ def bench_bloom(ctx, steps):
                                    "held whole in every workgroup's LDS)",
                        "find_kernel_ms": small_ms, "find_hbm_frac": find_bytes / (small_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
+
+
+def bench_bloom_legacy(ctx, steps, w, pc, a_keys, bm, cnt):
+    """The universal-hash filter on the same edge: built from the kept keys
+    at a 1% target false-positive rate (k = 7), applied to the 60M probe
+    keys.  Bit array and the whole selection bitmap checked against the
+    oracle."""
+    from flexpushdowndb_amd import _capi as C
+    from flexpushdowndb_amd import ops
+    from flexpushdowndb_amd.device import DeviceColumn
+    from oracle import oracle as O
+    L = C.lib()
+    kept, probe = w["kept"], w["probe"]
+    n = len(probe)
+    lf = ops.LegacyBloomFilter(ctx, len(kept), 0.01, seed=2024)
+    kc = DeviceColumn.from_numpy(ctx, kept)
+    a_kept = C.columns_array([kc.c()])
+    hf = lf._hf()
+
+    def find():
+        C.check(L.fpdb_bloom_legacy_find(ctx.handle, hf, lf.k, lf.capacity, lf.num_bits, lf.words.ptr, a_keys, 1, n,
+                                         bm.ptr, cnt.ptr))
+
+    lf.build([kc], len(kept))
+    find()
+    ctx.synchronize()
+    L.fpdb_profile_reset(ctx.handle)
+    L.fpdb_context_set_profiling(ctx.handle, 1)
+    for _ in range(steps):
+        find()
+        C.check(L.fpdb_bloom_legacy_build(ctx.handle, a_kept, 1, len(kept), None, hf, lf.k, lf.num_bits,
+                                          lf.words.ptr))
+    ctx.synchronize()
+    find_ms, _ = kernel_ms(ctx, "bloom_legacy_find")
+    build_ms, _ = kernel_ms(ctx, "bloom_legacy_build")
+    L.fpdb_context_set_profiling(ctx.handle, 0)
+    find()
+    okeys = O.legacy_keys([probe])
+    ref_words = O.legacy_build(lf.hash_functions, lf.num_bits, O.legacy_keys([kept]))
+    t0 = time.perf_counter()
+    ref_bm = O.legacy_find(lf.hash_functions, len(kept), ref_words, okeys)
+    cpu_s = time.perf_counter() - t0
+    if not (np.array_equal(lf.words.numpy(), ref_words) and np.array_equal(bm.numpy(len(ref_bm)), ref_bm)):
+        raise SystemExit("legacy bloom filter output does not match the oracle")
+    find_bytes = n * 8 + (n + 7) // 8
+    return {"workload": f"legacy universal-hash filter: {len(kept)} kept keys, p = 0.01 (k = {lf.k}, "
+                        f"m = {lf.num_bits} bits), {n} int64 probe keys",
+            "find_kernel_ms": find_ms, "find_gkeys_per_s": n / (find_ms * 1e-3) / 1e9,
+            "find_hbm_frac": find_bytes / (find_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "build_kernel_ms": build_ms,
+            "selected": int(np.unpackbits(ref_bm, bitorder="little")[:n].sum()),
+            "cpu_baseline": {"value": n / cpu_s / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
+                             "sample": f"the whole workload: oracle legacy find (hashCombine key, k universal "
+                                       f"hashes with 128-bit products) over all {n} probe keys, keys hashed "
+                                       f"beforehand, {cpu_s:.2f} s"}}
 
 
 def bench_groupby(ctx, steps):

@@ -2,8 +2,8 @@
 //   * key hashing (Arrow 6 HashMultiColumn on fixed-width keys),
 //   * the blocked filter (one 64-bit block per key, 57-bit masks),
 //   * the legacy universal-hash filter,
-//   * order-preserving filter compaction (single pass, decoupled look-back),
-//     used plain (bitmap in) and fused with the bloom probe.
+//   * order-preserving filter compaction (count pass, tile scan, copy pass),
+//     used plain (bitmap in) and after the bloom probe.
 //
 // Every kernel here streams its key columns once, coalesced: one wavefront
 // covers 64 consecutive rows and the per-row test result becomes one 64-bit

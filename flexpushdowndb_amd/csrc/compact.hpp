@@ -263,12 +263,14 @@ __device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
 // rank in the tile; index_out (optional) gets the source row index of every
 // selected row, bitmap_out (optional) the selection bitmap.  The selection
 // words are wave-uniform and live in scalar registers; a lane's output
-// position is the scalar running count plus an mbcnt.
+// position is the scalar running count plus an mbcnt.  Positions at or past
+// `cap` (the rows the outputs hold) are not written; the caller compares the
+// total with cap.
 template <class Pred>
 __global__ __launch_bounds__(256) void compact_copy_kernel(Pred pred, ColSet in, OutColSet out, int64_t n,
                                                            uint8_t* __restrict__ bitmap_out,
                                                            int64_t* __restrict__ index_out,
-                                                           const uint64_t* __restrict__ tile_base) {
+                                                           const uint64_t* __restrict__ tile_base, int64_t cap) {
   __shared__ uint32_t s_wave_cnt[kCompactWaves];
   const int64_t tile = blockIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(256) void compact_copy_kernel(Pred pred, ColSet in,
 #pragma unroll
       for (int u = 0; u < G; ++u) {
         const uint64_t wc = words[c0 + u];
-        if ((wc >> lane) & 1) {
+        if (((wc >> lane) & 1) && (int64_t)(pos + rank_below(wc)) < cap) {
           int64_t di = (int64_t)(pos + rank_below(wc)) + ooff;
           store_w(out.values[col], out.width[col], di, v[u]);
           if (out.validity[col] && valid_bit(in.validity[col], si[u]))
@@ -329,7 +331,8 @@ __global__ __launch_bounds__(256) void compact_copy_kernel(Pred pred, ColSet in,
 #pragma unroll
     for (int c = 0; c < kCompactChunks; ++c) {
       const uint64_t wc = words[c];
-      if ((wc >> lane) & 1) index_out[pos + rank_below(wc)] = (wave_chunk0 + c) * 64 + lane;
+      if (((wc >> lane) & 1) && (int64_t)(pos + rank_below(wc)) < cap)
+        index_out[pos + rank_below(wc)] = (wave_chunk0 + c) * 64 + lane;
       pos += __popcll(wc);
     }
   }
@@ -358,12 +361,17 @@ inline fpdb_status zero_out_validity(Context* c, const ColSet& in, const OutColS
 
 // Runs the compaction.  With d_num_out == nullptr the selected count is read
 // back into *num_out (synchronizes); otherwise it is written to d_num_out on
-// the device and the call returns without waiting.
+// the device and the call returns without waiting.  Rows past `capacity`
+// (the rows the output columns, and index_out, hold) are not written: the
+// synchronous form then fails with FPDB_ERR_CAPACITY and *num_out holds the
+// count the outputs need; the caller of the asynchronous form compares.
 template <class Pred, bool NEED_MASKS>
 fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const OutColSet& out,
                         int64_t n, uint8_t* bitmap_out, int64_t* index_out, int64_t* num_out,
-                        int64_t* d_num_out = nullptr, const char* prof_name = "filter_compact") {
+                        int64_t* d_num_out = nullptr, const char* prof_name = "filter_compact",
+                        int64_t capacity = INT64_MAX) {
   static_assert(!NEED_MASKS, "compaction predicates evaluate without the mask table");
+  capacity = std::min(capacity, out.capacity);
   FPDB_TRY(zero_out_validity(c, in, out, n));
   if (n == 0) {
     if (d_num_out) return launch_fill(c, d_num_out, 0, 8);
@@ -392,13 +400,17 @@ fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const Ou
     }
     launch_tile_scan(c, 1, tcnt, tiles, tbase, total);
     hipLaunchKernelGGL((compact_copy_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, in, out,
-                       n, bitmap_out, index_out, tbase);
+                       n, bitmap_out, index_out, tbase, capacity);
   }
   ctx_free(c, tcnt);   // stream-ordered reuse: later work on the stream runs after these kernels
   ctx_free(c, tbase);
   FPDB_HIP(hipGetLastError());
   if (d_num_out) return FPDB_OK;
-  return read_count(c, num_out);
+  FPDB_TRY(read_count(c, num_out));
+  FPDB_CHECK(*num_out <= capacity, FPDB_ERR_CAPACITY,
+             std::string(prof_name) + ": " + std::to_string(*num_out) + " rows selected, the output holds " +
+                 std::to_string(capacity));
+  return FPDB_OK;
 }
 
 }  // namespace fpdb

@@ -65,10 +65,14 @@ fpdb_status make_outcolset(const fpdb_column* cols, int n, OutColSet* out, const
   FPDB_CHECK(n == 0 || cols != nullptr, FPDB_ERR_INVALID, std::string(what) + ": columns is NULL");
   std::memset(out, 0, sizeof(*out));
   out->n = n;
+  out->capacity = INT64_MAX;
   for (int i = 0; i < n; ++i) {
     int w = type_width(cols[i].type);
     FPDB_CHECK(w > 0, FPDB_ERR_INVALID,
                std::string(what) + ": output column " + std::to_string(i) + " has unsupported type");
+    FPDB_CHECK(cols[i].length >= 0, FPDB_ERR_INVALID,
+               std::string(what) + ": output column " + std::to_string(i) + " has a negative length");
+    out->capacity = std::min<int64_t>(out->capacity, cols[i].length);
     out->values[i] = static_cast<uint8_t*>(cols[i].values);
     out->validity[i] = cols[i].validity;
     out->offset[i] = cols[i].offset;
@@ -173,7 +177,7 @@ fpdb_status read_count(Context* c, int64_t* count) {
   FPDB_TRY(mail_read(c, c->d_counter, 2, v));
   if ((int)(uint32_t)v[1] != 0) {
     FPDB_TRY(launch_fill(c, c->d_counter + 1, 0, 8));
-    return fail(FPDB_ERR_INTERNAL, "a kernel reported a failure (look-back did not complete or a table overflowed)");
+    return fail(FPDB_ERR_INTERNAL, "a kernel reported a failure (a bounded wait gave up or a table overflowed)");
   }
   if (count) *count = (int64_t)v[0];
   return FPDB_OK;
@@ -215,27 +219,6 @@ fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes) {
   return FPDB_OK;
 }
 
-__global__ void lb_reset_kernel(LbState* st) {
-  st->reserved = 0;
-  st->reserved2 = 0;
-  st->epoch = 0;
-}
-
-// Advances the epoch before a look-back launch (one workgroup); on wrap-around
-// the status words are cleared so no old word can carry the new epoch.
-__global__ void lb_advance_kernel(LbState* st, uint64_t* status, int64_t status_cap) {
-  __shared__ unsigned s_e;
-  if (threadIdx.x == 0) s_e = st->epoch + 1;
-  __syncthreads();
-  unsigned e = s_e;
-  if (e >= (1u << 20)) {
-    for (int64_t i = threadIdx.x; i < status_cap; i += blockDim.x) status[i] = 0;
-    e = 1;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) st->epoch = e;
-}
-
 __global__ __launch_bounds__(1024) void sum_partials_kernel(const unsigned long long* __restrict__ partials,
                                                              int n, int64_t* dst) {
   __shared__ unsigned long long s_w[16];
@@ -255,33 +238,6 @@ fpdb_status launch_sum_partials(Context* c, int blocks, int64_t* dst) {
   FPDB_CHECK(blocks >= 1 && blocks <= kMaxPartials, FPDB_ERR_INTERNAL, "too many workgroups for the partial counts");
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(1024), 0, c->stream, c->d_partials, blocks, dst);
   FPDB_HIP(hipGetLastError());
-  return FPDB_OK;
-}
-
-fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args) {
-  LookbackScratch& lb = c->lb;
-  if (!lb.state) {
-    FPDB_TRY(ctx_alloc(c, sizeof(LbState), reinterpret_cast<void**>(&lb.state)));
-    hipLaunchKernelGGL(lb_reset_kernel, dim3(1), dim3(1), 0, c->stream, lb.state);
-    FPDB_HIP(hipGetLastError());
-  }
-  if (tiles > lb.tiles) {
-    // grow: a fresh zeroed status array; the epoch restarts
-    int64_t want = std::max<int64_t>(tiles, 4096);
-    if (lb.status) FPDB_TRY(ctx_free(c, lb.status));
-    lb.status = nullptr;
-    FPDB_TRY(ctx_alloc(c, (uint64_t)want * 8, reinterpret_cast<void**>(&lb.status)));
-    FPDB_TRY(launch_fill(c, lb.status, 0, (uint64_t)want * 8));
-    hipLaunchKernelGGL(lb_reset_kernel, dim3(1), dim3(1), 0, c->stream, lb.state);
-    FPDB_HIP(hipGetLastError());
-    lb.tiles = want;
-  }
-  hipLaunchKernelGGL(lb_advance_kernel, dim3(1), dim3(256), 0, c->stream, lb.state, lb.status, lb.tiles);
-  FPDB_HIP(hipGetLastError());
-  args->state = lb.state;
-  args->status = lb.status;
-  args->status_cap = lb.tiles;
-  args->num_tiles = tiles;
   return FPDB_OK;
 }
 
@@ -369,7 +325,7 @@ fpdb_status fpdb_context_synchronize(fpdb_context* ctx) {
   if (flag) {
     FPDB_TRY(launch_fill(c, ctx_err_flag(c), 0, 8));
     FPDB_HIP(hipStreamSynchronize(c->stream));
-    return fail(FPDB_ERR_INTERNAL, "a kernel reported a failure (look-back did not complete or a table overflowed)");
+    return fail(FPDB_ERR_INTERNAL, "a kernel reported a failure (a bounded wait gave up or a table overflowed)");
   }
   return FPDB_OK;
 }

@@ -40,35 +40,6 @@ fpdb_status fail(fpdb_status code, const std::string& msg);
 
 // ---- context ---------------------------------------------------------------
 
-// Decoupled look-back state, kept entirely on the device so a launch can be
-// replayed from a captured graph.  Workgroup b handles tile b: each XCD
-// dispatches its workgroups in index order, so the lowest-numbered unfinished
-// tile always runs and every look-back wait ends (a bounded spin still turns a
-// stuck wait into an error).  A one-workgroup kernel before each launch
-// advances `epoch`; status words carry it, [epoch:20][flag:2][value:42], so
-// they never need clearing between launches (only when the 20-bit epoch
-// wraps).  No per-tile atomic on a shared word: one same-address device
-// atomic per workgroup serializes at about 12 ns each on gfx950.
-struct LbState {
-  unsigned long long reserved;
-  unsigned int reserved2;
-  unsigned int epoch;
-};
-
-struct LookbackScratch {
-  uint64_t* status = nullptr;       // device, `tiles` words
-  LbState* state = nullptr;         // device
-  int64_t tiles = 0;
-};
-
-// Kernel-argument view of the look-back scratch for one launch.
-struct LbArgs {
-  LbState* state;
-  uint64_t* status;
-  int64_t status_cap;
-  int64_t num_tiles;
-};
-
 struct ProfRecord {
   const char* name;
   hipEvent_t start, stop;
@@ -94,7 +65,6 @@ struct Context {
   uint64_t hbm_bytes = 0;
   size_t max_lds_per_block = 64 * 1024;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
-  LookbackScratch lb;
   int64_t* d_counter = nullptr;     // small device scratch for counts (8 words)
   unsigned long long* d_partials = nullptr;   // per-workgroup partial counts (kMaxPartials)
   int64_t* h_counter = nullptr;     // pinned host mirror
@@ -131,9 +101,6 @@ struct ProfScope {
     }
   }
 };
-// Ensures look-back scratch for `tiles` tiles and returns the launch arguments.
-fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args);
-
 // Per-workgroup partial counts.  A same-address device atomic serializes at
 // about 12 ns each on gfx950, so kernels that count (selected rows, inserted
 // keys) store one partial per workgroup and a one-workgroup kernel adds the
@@ -141,7 +108,7 @@ fpdb_status lookback_prepare(Context* c, int64_t tiles, LbArgs* args);
 constexpr int kMaxPartials = 1 << 14;
 fpdb_status launch_sum_partials(Context* c, int blocks, int64_t* dst);
 fpdb_status launch_fill(Context* c, void* d, uint8_t v, uint64_t bytes);
-// Kernel-side failure flag (look-back spin exhausted, table overflow); zero
+// Kernel-side failure flag (a bounded wait gave up, a table overflowed); zero
 // unless a kernel reported a failure.
 inline int* ctx_err_flag(Context* c) { return reinterpret_cast<int*>(c->d_counter + 1); }
 // Synchronizes, reads the count a kernel left in d_counter[0] and checks (and
@@ -182,6 +149,7 @@ struct OutColSet {
   int64_t offset[kMaxCols];
   int32_t width[kMaxCols];
   int32_t n;
+  int64_t capacity;   // rows every output column has room for (INT64_MAX with no columns)
 };
 
 fpdb_status make_colset(const fpdb_column* cols, int n, int64_t num_rows, ColSet* out,
@@ -334,86 +302,6 @@ __host__ __device__ __forceinline__ uint64_t fmix64(uint64_t k) {
   k ^= k >> 33;
   return k;
 }
-
-// Look-back status word packing.
-__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint32_t flag, uint64_t value) {
-  return ((uint64_t)epoch << 44) | ((uint64_t)flag << 42) | (value & ((1ULL << 42) - 1));
-}
-
-constexpr uint32_t kLbAggregate = 1;
-constexpr uint32_t kLbPrefix = 2;
-constexpr int64_t kLbSpinLimit = int64_t(1) << 28;
-
-// Decoupled look-back executed by one full wavefront (all 64 lanes active).
-// Publishes this tile's aggregate, walks predecessors 64 at a time and returns
-// the exclusive prefix of the tile (identical in every lane).  `err` is set
-// (and 0 returned) if the bounded spin runs out, which only happens if an
-// earlier tile never publishes.
-// The status words carry their own payload, so relaxed device-scope atomics
-// suffice: acquire/release at device scope would write back or invalidate the
-// whole L2 of the XCD on every publish and poll.
-__device__ __forceinline__ uint64_t lookback_exclusive(uint64_t* status, int64_t tile,
-                                                       uint32_t epoch, uint64_t aggregate,
-                                                       int* err) {
-  const int lane = threadIdx.x & 63;
-  if (tile == 0) {
-    if (lane == 0)
-      __hip_atomic_store(&status[0], lb_pack(epoch, kLbPrefix, aggregate), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[tile], lb_pack(epoch, kLbAggregate, aggregate), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t exclusive = 0;
-  int64_t window_end = tile;   // predecessors [window_end-64, window_end)
-  int64_t spins = 0;
-  while (true) {
-    int64_t idx = window_end - 1 - lane;   // lane 0 = nearest predecessor
-    uint64_t w = 0;
-    uint32_t flag = 0;
-    if (idx >= 0) {
-      w = __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      flag = ((uint32_t)(w >> 44) == epoch) ? (uint32_t)((w >> 42) & 3) : 0;
-    } else {
-      flag = kLbPrefix;   // virtual prefix of zero before tile 0
-    }
-    // Lanes up to (and including) the first prefix lane must all be ready.
-    uint64_t prefix_mask = __ballot(flag == kLbPrefix);
-    uint64_t notready_mask = __ballot(flag == 0);
-    int first_prefix = prefix_mask ? __builtin_ctzll(prefix_mask) : 64;
-    uint64_t upto = (first_prefix >= 63) ? ~0ULL : ((2ULL << first_prefix) - 1);
-    if (notready_mask & upto) {
-      if (++spins > kLbSpinLimit) { *err = 1; return 0; }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    uint64_t v = (lane <= first_prefix && idx >= 0) ? (w & ((1ULL << 42) - 1)) : 0;
-    // wave reduction
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    exclusive += v;
-    if (first_prefix < 64) break;
-    window_end -= 64;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[tile], lb_pack(epoch, kLbPrefix, exclusive + aggregate),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return exclusive;
-}
-
-// Start of a look-back launch: the tile is the workgroup index; thread 0 reads
-// the epoch the advance kernel wrote before this launch.
-__device__ __forceinline__ void lb_begin(const LbArgs& lb, int64_t* s_tile, uint32_t* s_epoch) {
-  if (threadIdx.x == 0) {
-    *s_epoch = __hip_atomic_load(&lb.state->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *s_tile = (int64_t)blockIdx.x;
-  }
-  __syncthreads();
-}
-
-// End of a look-back launch (nothing to do: the next launch's advance kernel
-// moves the epoch on).
-__device__ __forceinline__ void lb_end(const LbArgs&) {}
 
 // Stores this workgroup's count (the sum of every thread's `mine`) into
 // partials[blockIdx.x]; every thread of the workgroup must call it.

@@ -9,10 +9,10 @@
 // Build: one thread per build row claims a slot with a 32-bit CAS on the row
 // field, then writes key and payload; keys are compared only by later probes.
 //
-// Probe: a tile of 1024 probe rows (256 threads x 4 rows, coalesced) counts
-// its matches, chains its output offset to earlier tiles by decoupled
-// look-back, then writes probe columns and gathered build columns at their
-// final positions -- one pass over the probe columns, output in probe order.
+// Probe: a match (or count) pass over tiles of 1024 probe rows records each
+// row's first match and each tile's output count, a one-workgroup scan turns
+// the counts into output offsets, and an emit pass writes probe columns and
+// gathered build columns at their final positions, in probe-row order.
 #include "fpdb_internal.hpp"
 #include "compact.hpp"
 
@@ -168,8 +168,8 @@ __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bco
   }
 }
 
-// The probe runs as two passes so the random table reads are not interleaved
-// with a look-back wait:
+// The probe runs as passes with known output offsets, so the random table
+// reads never wait behind the output positions of earlier tiles:
 //   match  tiles of 1024 rows (256 threads x 4): keys, table lookups (the home
 //          slot and the next one loaded together), per row the match count and
 //          first match (build row, inline payload), per tile the output count;
@@ -1423,8 +1423,10 @@ fpdb_status fpdb_hj_build_side_rows(fpdb_context* ctx, fpdb_hash_table* t, int m
   OutColSet onone;
   std::memset(&none, 0, sizeof(none));
   std::memset(&onone, 0, sizeof(onone));
+  onone.capacity = INT64_MAX;
   return run_compact<PredByteFlag, false>(c, PredByteFlag{t->matched, (uint8_t)(matched ? 1 : 0)}, none,
-                                          onone, t->build_rows, nullptr, d_rows, num_out);
+                                          onone, t->build_rows, nullptr, d_rows, num_out, nullptr,
+                                          "build side rows", capacity);
 }
 
 fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_columns, int64_t num_rows,

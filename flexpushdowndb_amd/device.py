@@ -141,9 +141,12 @@ class DeviceColumn:
         return cls(vals, vb, type_id, len(arr))
 
     @classmethod
-    def empty(cls, ctx, n, dtype, nullable=False, type_id=None):
+    def empty(cls, ctx, n, dtype, nullable=False, type_id=None, validity_rows=None):
+        """n rows of values; a validity bitmap (when nullable) of
+        max(n, validity_rows) rows."""
         vals = ctx.empty(max(1, n), dtype)
-        vb = ctx.zeros(((max(1, n) + 31) // 32) * 4, np.uint8) if nullable else None
+        vrows = max(1, n, validity_rows or 0)
+        vb = ctx.zeros(((vrows + 31) // 32) * 4, np.uint8) if nullable else None
         return cls(vals, vb, type_id, n)
 
     @property
@@ -159,9 +162,16 @@ class DeviceColumn:
         return self.values.ctx.download(self.values.ptr + self.offset * self.dtype.itemsize, n, self.dtype)
 
     def valid_mask(self, n=None):
+        """Validity of rows [0, n) of the column: bits offset .. offset + n of
+        the validity bitmap (LSB-first)."""
         n = self.length if n is None else int(n)
         if self.validity is None:
             return np.ones(n, bool)
-        nbytes = (n + 7) // 8
+        nbytes = (self.offset + n + 7) // 8
         bits = self.values.ctx.download(self.validity.ptr, nbytes, np.uint8)
-        return np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+        return unpack_validity(bits, self.offset, n)
+
+
+def unpack_validity(bits, offset, n):
+    """Rows [offset, offset + n) of an LSB-first validity bitmap as booleans."""
+    return np.unpackbits(np.asarray(bits, np.uint8), bitorder="little")[offset:offset + n].astype(bool)

@@ -153,9 +153,12 @@ def compare_bitmap(ctx, col, op, constant, n, combine=None, combine_or=False, ou
 
 
 def filter_compact(ctx, cols, n, bitmap, capacity=None):
+    """Order-preserving compaction of the rows whose bit is set into output
+    columns of `capacity` rows (default n); raises FpdbError with
+    FPDB_ERR_CAPACITY when more rows are selected than fit."""
     cap = n if capacity is None else capacity
-    out = [DeviceColumn.empty(ctx, max(cap, n) if c.validity is not None else cap, c.dtype,
-                              c.validity is not None, c.type_id) for c in cols]
+    out = [DeviceColumn.empty(ctx, cap, c.dtype, c.validity is not None, c.type_id, validity_rows=n)
+           for c in cols]
     nout = ctypes.c_int64()
     C.check(C.lib().fpdb_filter_compact(ctx.handle, _cols(cols), len(cols), n, bitmap.ptr,
                                         C.columns_array([o.c() for o in out]), ctypes.byref(nout)))

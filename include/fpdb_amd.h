@@ -201,10 +201,12 @@ fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, in
 /* Keeps the rows whose bitmap bit is set, in order, for every column
  * (arrow::compute::Filter at BloomFilterUseKernel.cpp:195; gandiva
  * SelectionVector filtering at BloomFilterUseKernel.cpp:170-177).
- * out[i] must have room for the selected rows; an output validity bitmap
- * (required when the input column has one) must hold ceil(num_rows/32)*4
- * bytes and its offset must be a multiple of 32.  *num_out (host) receives the
- * number of rows written (synchronizes). */
+ * out[i].length is the number of rows out[i] has room for; rows past the
+ * smallest of them are not written, and the call then fails with
+ * FPDB_ERR_CAPACITY with *num_out set to the number of rows selected.  An
+ * output validity bitmap (required when the input column has one) must hold
+ * ceil(num_rows/32)*4 bytes and its offset must be a multiple of 32.
+ * *num_out (host) receives the number of rows written (synchronizes). */
 fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int num_columns,
                                 int64_t num_rows, const uint8_t* d_bitmap,
                                 fpdb_column* out, int64_t* num_out);
@@ -212,8 +214,10 @@ fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int nu
 /* Fused blocked-bloom use: Find + compaction in one pass over the columns
  * (BloomFilterUseKernel::filterRecordBatch, BloomFilterUseKernel.cpp:180-207).
  * key_index[j] names which of `in` are the key columns.  d_bitmap may be NULL.
- * out[i] must have room for num_rows rows (the count is not known up front).
- * *num_out (host) receives the selected row count (synchronizes). */
+ * out[i].length is the rows out[i] has room for (num_rows always suffices);
+ * with fewer than the selected count the call writes what fits and fails with
+ * FPDB_ERR_CAPACITY, *num_out holding the selected count.  *num_out (host)
+ * receives the selected row count (synchronizes). */
 fpdb_status fpdb_bloom_blocked_filter(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
                                       const fpdb_column* in, int num_columns,
                                       const int* key_index, int num_keys, int64_t num_rows,

@@ -6,6 +6,7 @@ restatement of the Arrow 6 key hash (parity with the reference binary's hash is
 unpinned; the filter arithmetic given the hash is pinned).  Legacy filter:
 fully pinned, compared bit for bit.
 """
+import ctypes
 import types
 
 import numpy as np
@@ -229,10 +230,11 @@ def test_legacy_two_columns_and_capacity_zero(ctx):
     assert cnt == 0
 
 
-def test_bloom_workload_full_size_properties(ctx):
+def test_bloom_workload_full_size_exact(ctx):
     """Full single-edge predicate-transfer workload: 2M keys filtered to ~400k,
-    60M probe keys.  Bit array against the oracle; bitmap by no-false-negative
-    and popcount identities (size-independent checks)."""
+    60M probe keys.  The 2^16-block bit array and all 60M bits of the selection
+    bitmap equal the oracle's; the compacted column holds every selected key
+    at its rank."""
     w = datagen.bloom_workload()
     kept = w["kept"]
     f = ops.BlockedBloomFilter.build(ctx, [DeviceColumn.from_numpy(ctx, kept)], len(kept))
@@ -243,17 +245,65 @@ def test_bloom_workload_full_size_properties(ctx):
     pc = DeviceColumn.from_numpy(ctx, probe)
     bm, cnt = f.find([pc], len(probe))
     got = bm.numpy((len(probe) + 7) // 8)
-    # exact on a 2M-row slice, counting identity on all of it
-    ref_head = O.blocked_find(lnb, ref, O.arrow6_hash([(probe[:2_000_000], None)]))
-    assert np.array_equal(got[:250_000], ref_head)
-    assert cnt == int(np.unpackbits(got, bitorder="little")[:len(probe)].sum())
+    ref_bm = O.blocked_find(lnb, ref, O.arrow6_hash([(probe, None)]))
+    assert np.array_equal(got, ref_bm)
+    sel = np.unpackbits(got, bitorder="little")[:len(probe)].astype(bool)
+    assert cnt == int(sel.sum())
     out, c2, bm2 = f.filter([pc], [0], len(probe))
     assert c2 == cnt and np.array_equal(bm2.numpy(len(got)), got)
     # 14,649 compaction tiles (two scan chunks): every kept key at its rank
-    sel = np.unpackbits(got, bitorder="little")[:len(probe)].astype(bool)
     assert np.array_equal(out[0].numpy(c2), probe[sel])
-    head_bits = np.unpackbits(got[:250_000], bitorder="little").astype(bool)
-    assert head_bits[np.isin(probe[:2_000_000], kept)].all()   # no false negatives
+    assert sel[np.isin(probe, kept)].all()   # no false negatives
+
+
+def test_legacy_filter_c2_size_exact(ctx):
+    """The universal-hash filter at the workload's size: the ~400k kept keys
+    at a 1% target false-positive rate (k = 7, m = ceil(n |ln p| / ln^2 2))
+    and 60M probe keys.  Bit array words and all 60M selection bits equal the
+    oracle's (BloomFilter.cpp:62-91, UniversalHashFunction.cpp:51-57)."""
+    w = datagen.bloom_workload()
+    kept, probe = w["kept"], w["probe"]
+    n = len(kept)
+    lf = ops.LegacyBloomFilter(ctx, n, 0.01, seed=2024)
+    assert lf.k == 7 and lf.num_bits == O.legacy_m(n, 0.01)
+    lf.build([DeviceColumn.from_numpy(ctx, kept)], n)
+    ref_words = O.legacy_build(lf.hash_functions, lf.num_bits, O.legacy_keys([kept]))
+    assert np.array_equal(lf.words.numpy(), ref_words)
+    bm, cnt = lf.find([DeviceColumn.from_numpy(ctx, probe)], len(probe))
+    ref_bm = O.legacy_find(lf.hash_functions, n, ref_words, O.legacy_keys([probe]))
+    assert np.array_equal(bm.numpy(len(ref_bm)), ref_bm)
+    sel = np.unpackbits(ref_bm, bitorder="little")[:len(probe)].astype(bool)
+    assert cnt == int(sel.sum())
+    member = np.isin(probe, kept)
+    assert sel[member].all()
+    assert sel[~member].mean() < 0.012   # false positives near the 1% target
+
+
+def test_filter_compact_capacity(ctx):
+    """Outputs shorter than the selected count: the rows that fit are written,
+    nothing past them, and the call reports FPDB_ERR_CAPACITY with the count."""
+    n = 100_000
+    rng = np.random.default_rng(90)
+    sel = rng.random(n) < 0.5
+    a = np.arange(n, dtype=np.int64) + 11
+    need = int(sel.sum())
+    cap = need - 1000
+    src = DeviceColumn.from_numpy(ctx, a)
+    bmd = ctx.upload(np.packbits(sel, bitorder="little"))
+    buf = ctx.upload(np.full(need + 64, -7, np.int64))
+    dst = DeviceColumn(buf, None, C.TYPE_INT64, cap)
+    nout = ctypes.c_int64()
+    st = C.lib().fpdb_filter_compact(ctx.handle, C.columns_array([src.c()]), 1, n, bmd.ptr,
+                                     C.columns_array([dst.c()]), ctypes.byref(nout))
+    assert st == C.FPDB_ERR_CAPACITY and nout.value == need
+    got = buf.numpy()
+    assert np.array_equal(got[:cap], a[sel][:cap])
+    assert (got[cap:] == -7).all()
+    with pytest.raises(C.FpdbError) as e:
+        ops.filter_compact(ctx, [src], n, bmd, capacity=cap)
+    assert e.value.code == C.FPDB_ERR_CAPACITY
+    out, cnt = ops.filter_compact(ctx, [src], n, bmd, capacity=need)
+    assert cnt == need and np.array_equal(out[0].numpy(cnt), a[sel])
 
 
 @pytest.mark.parametrize("extra", [-1, 0, 1])

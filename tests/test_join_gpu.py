@@ -18,12 +18,17 @@ def _rows(cols, n, masks=None):
     return sorted(zip(*[a.tolist() for a in arrs]))
 
 
-def test_c1_shape_inner_join_closed_form(ctx):
-    w = datagen.join_workload(n_build=200_000, n_probe=1_200_000, seed=5)
+@pytest.mark.parametrize("n_build,n_probe,seed", [(200_000, 1_200_000, 5), (1_000_000, 6_000_000, 1)])
+def test_c1_inner_join_closed_form(ctx, n_build, n_probe, seed):
+    """The join workload (the full 1M-row build / 6M-row probe case with the
+    benchmark's seed, and a fifth of it): the build keys are a permutation, so
+    every probe row matches exactly one build row and the output is fixed by
+    direct indexing -- compared column by column in probe-row order."""
+    w = datagen.join_workload(n_build=n_build, n_probe=n_probe, seed=seed)
     bk = DeviceColumn.from_numpy(ctx, w["build_key"])
     bp = DeviceColumn.from_numpy(ctx, w["build_pay"])
     t = ops.HashJoinTable(ctx, [bk], len(w["build_key"]), payload=bp)
-    assert t.info()["unique"] and t.info()["inserted"] == 200_000
+    assert t.info()["unique"] and t.info()["inserted"] == n_build
     pf = DeviceColumn.from_numpy(ctx, w["probe_fk"])
     pq = DeviceColumn.from_numpy(ctx, w["probe_qty"])
     pp = DeviceColumn.from_numpy(ctx, w["probe_price"])
@@ -31,8 +36,8 @@ def test_c1_shape_inner_join_closed_form(ctx):
     out, cnt = t.probe(C.JOIN_INNER, [pf], n, [pf, pq, pp], [bk, "payload"], capacity=n)
     assert cnt == n
     # output is in probe-row order and every probe row has exactly one match
-    inv = np.empty(200_000, np.int64)
-    inv[w["build_key"]] = np.arange(200_000)
+    inv = np.empty(n_build, np.int64)
+    inv[w["build_key"]] = np.arange(n_build)
     assert np.array_equal(out[0].numpy(), w["probe_fk"])
     assert np.array_equal(out[1].numpy(), w["probe_qty"])
     assert np.array_equal(out[2].numpy(), w["probe_price"])

@@ -80,3 +80,18 @@ def test_errors(ctx):
     bad["dims"][1]["key"] = bad["dims"][1]["key"][::-1].copy()    # keys not dense
     with pytest.raises(C.FpdbError):
         Server(ctx, bad)
+
+
+def test_full_size_queries_exact(ctx):
+    """The serving workload at its full size (60M fact rows, four dimensions
+    of 30,000 / 200,000 / 2,000,000 / 2,556 rows): a mix of queries admitted
+    into shared scans, each result equal to the direct-indexing expectation."""
+    w = datagen.serve_workload()
+    qs = datagen.serve_queries(6, seed=31)
+    srv = Server(ctx, w, max_active=8)
+    ids = [srv.submit(q) for q in qs]
+    while any(srv.status(i) != 2 for i in ids):
+        assert srv.step() > 0
+    for q, i in zip(qs, ids):
+        _same(srv.result(i), datagen.serve_expected(w, q))
+    srv.close()
