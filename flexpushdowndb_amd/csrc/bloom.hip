@@ -720,6 +720,33 @@ fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int nu
   return run_compact<PredBitmap, false>(c, PredBitmap{d_bitmap}, ic, oc, num_rows, nullptr, nullptr, num_out);
 }
 
+fpdb_status fpdb_filter_compact_noalloc(fpdb_context* ctx, const fpdb_column* in, int num_columns,
+                                        int64_t num_rows, const uint8_t* d_bitmap, fpdb_column* out,
+                                        int64_t* d_num_out) {
+  FPDB_CHECK(ctx && d_num_out, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(d_bitmap || num_rows == 0, FPDB_ERR_INVALID, "NULL bitmap");
+  FPDB_CHECK(num_rows >= 0 && num_rows <= kNoAllocMaxRows, FPDB_ERR_INVALID,
+             "filter compact (no allocation): " + std::to_string(num_rows) +
+                 " rows exceed the maximum of " + std::to_string(kNoAllocMaxRows) +
+                 " (the library's static scratch); use fpdb_filter_compact");
+  Context* c = as_ctx(ctx);
+  FPDB_CHECK(c->noalloc_scratch, FPDB_ERR_INTERNAL, "no-allocation scratch address not set up");
+  ColSet ic;
+  OutColSet oc;
+  FPDB_TRY(make_colset(in, num_columns, num_rows, &ic, "filter compact"));
+  FPDB_TRY(make_outcolset(out, num_columns, &oc, "filter compact"));
+  FPDB_TRY(check_outputs_match(ic, oc));
+  FPDB_TRY(zero_out_validity(c, ic, oc, num_rows));
+  if (num_rows == 0) return launch_fill(c, d_num_out, 0, 8);
+  // scratch: tile counts (uint32) then tile bases (uint64, 8-byte aligned)
+  const int64_t tiles = ceil_div(num_rows, kCompactTileRows);
+  uint32_t* tcnt = reinterpret_cast<uint32_t*>(c->noalloc_scratch);
+  uint64_t* tbase = reinterpret_cast<uint64_t*>(c->noalloc_scratch + (uint64_t)kNoAllocMaxTiles * 4);
+  static_assert(kNoAllocMaxTiles * kCompactTileRows >= kNoAllocMaxRows, "scratch holds the tiles of the maximum");
+  return compact_launch(c, PredBitmap{d_bitmap}, ic, oc, num_rows, nullptr, nullptr, tcnt, tbase, d_num_out,
+                        oc.capacity, nullptr);
+}
+
 fpdb_status fpdb_bloom_blocked_filter(fpdb_context* ctx, const uint64_t* d_blocks, int log_num_blocks,
                                       const fpdb_column* in, int num_columns,
                                       const int* key_index, int num_keys, int64_t num_rows,

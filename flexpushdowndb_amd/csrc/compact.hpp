@@ -359,6 +359,37 @@ inline fpdb_status zero_out_validity(Context* c, const ColSet& in, const OutColS
   return FPDB_OK;
 }
 
+// The three launches of a compaction of n > 0 rows with caller-provided
+// scratch: tile counts (tiles uint32) and tile bases (tiles uint64).  The
+// selected count goes to the device int64 at `total`.  prof_name NULL: no
+// timing scope (it would create events).
+template <class Pred>
+fpdb_status compact_launch(Context* c, const Pred& pred, const ColSet& in, const OutColSet& out, int64_t n,
+                           uint8_t* bitmap_out, int64_t* index_out, uint32_t* tcnt, uint64_t* tbase,
+                           int64_t* total, int64_t capacity, const char* prof_name) {
+  const int64_t tiles = ceil_div(n, kCompactTileRows);
+  auto launches = [&]() {
+    if constexpr (std::is_same<Pred, PredBitmap>::value) {
+      const int g = grid_for(c, ceil_div(tiles, kBitmapCountUnroll), 256 / 64, 8);
+      hipLaunchKernelGGL(compact_count_bitmap_kernel, dim3(g), dim3(256), 0, c->stream, pred.bm, n, tiles, tcnt);
+    } else {
+      hipLaunchKernelGGL((compact_count_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, n,
+                         tcnt);
+    }
+    launch_tile_scan(c, 1, tcnt, tiles, tbase, total);
+    hipLaunchKernelGGL((compact_copy_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, in, out,
+                       n, bitmap_out, index_out, tbase, capacity);
+  };
+  if (prof_name) {
+    ProfScope ps(c, prof_name);
+    launches();
+  } else {
+    launches();
+  }
+  FPDB_HIP(hipGetLastError());
+  return FPDB_OK;
+}
+
 // Runs the compaction.  With d_num_out == nullptr the selected count is read
 // back into *num_out (synchronizes); otherwise it is written to d_num_out on
 // the device and the call returns without waiting.  Rows past `capacity`
@@ -389,22 +420,10 @@ fpdb_status run_compact(Context* c, const Pred& pred, const ColSet& in, const Ou
     return st;
   }
   int64_t* total = d_num_out ? d_num_out : c->d_counter;
-  {
-    ProfScope ps(c, prof_name);
-    if constexpr (std::is_same<Pred, PredBitmap>::value) {
-      const int g = grid_for(c, ceil_div(tiles, kBitmapCountUnroll), 256 / 64, 8);
-      hipLaunchKernelGGL(compact_count_bitmap_kernel, dim3(g), dim3(256), 0, c->stream, pred.bm, n, tiles, tcnt);
-    } else {
-      hipLaunchKernelGGL((compact_count_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, n,
-                         tcnt);
-    }
-    launch_tile_scan(c, 1, tcnt, tiles, tbase, total);
-    hipLaunchKernelGGL((compact_copy_kernel<Pred>), dim3((unsigned)tiles), dim3(256), 0, c->stream, pred, in, out,
-                       n, bitmap_out, index_out, tbase, capacity);
-  }
+  st = compact_launch(c, pred, in, out, n, bitmap_out, index_out, tcnt, tbase, total, capacity, prof_name);
   ctx_free(c, tcnt);   // stream-ordered reuse: later work on the stream runs after these kernels
   ctx_free(c, tbase);
-  FPDB_HIP(hipGetLastError());
+  FPDB_TRY(st);
   if (d_num_out) return FPDB_OK;
   FPDB_TRY(read_count(c, num_out));
   FPDB_CHECK(*num_out <= capacity, FPDB_ERR_CAPACITY,

@@ -9,6 +9,32 @@ namespace fpdb {
 
 static thread_local std::string g_last_error;
 
+// Scratch of the no-allocation entry points (fpdb_filter_compact_noalloc):
+// every GPU the library runs on holds its own copy until the process ends.
+// Host code never uses this name as an address: the per-device address comes
+// from hipGetSymbolAddress (noalloc_scratch_address).
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, uint8_t*> g_scratch_addr;
+
+// Address of `device`'s copy of the scratch array; the device must be current.
+static fpdb_status noalloc_scratch_address(int device, uint8_t** out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it != g_scratch_addr.end()) {
+    *out = it->second;
+    return FPDB_OK;
+  }
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    return fail(FPDB_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_addr[device] = static_cast<uint8_t*>(p);
+  *out = static_cast<uint8_t*>(p);
+  return FPDB_OK;
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 fpdb_status fail(fpdb_status code, const std::string& msg) {
@@ -274,6 +300,14 @@ fpdb_status fpdb_context_create(int device, uint64_t hbm_budget_bytes, void* str
   }
   hipDeviceProp_t prop;
   FPDB_HIP(hipGetDeviceProperties(&prop, device));
+  {
+    fpdb_status s = noalloc_scratch_address(device, &c->noalloc_scratch);
+    if (s != FPDB_OK) {
+      if (c->own_stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+      return s;
+    }
+  }
   c->num_cus = prop.multiProcessorCount;
   c->hbm_bytes = prop.totalGlobalMem;
   c->max_lds_per_block = prop.sharedMemPerBlock;

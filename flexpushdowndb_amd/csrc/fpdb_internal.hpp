@@ -73,7 +73,17 @@ struct Context {
   uint64_t* h_mail = nullptr;
   uint64_t* d_mail = nullptr;       // the same memory, device view
   uint64_t mail_seq = 0;
+  // Device address of this device's copy of the library's static no-allocation
+  // scratch array (kNoAllocScratchBytes), looked up once per device when the
+  // first context on it is created.  Shared by every context on the device.
+  uint8_t* noalloc_scratch = nullptr;
 };
+
+// The static scratch array of the no-allocation entry points: tile counts and
+// tile bases of a compaction of at most kNoAllocMaxRows rows.
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 28;
+constexpr int64_t kNoAllocMaxTiles = kNoAllocMaxRows / 4096;
+constexpr uint64_t kNoAllocScratchBytes = (uint64_t)kNoAllocMaxTiles * 12;
 
 fpdb_status ctx_alloc(Context* c, uint64_t bytes, void** out);
 fpdb_status ctx_free(Context* c, void* p);

@@ -211,6 +211,21 @@ fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int nu
                                 int64_t num_rows, const uint8_t* d_bitmap,
                                 fpdb_column* out, int64_t* num_out);
 
+/* fpdb_filter_compact for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Allocates nothing: the tile counts and
+ * bases live in the library's static per-device scratch array (768 KiB), so
+ * num_rows may be at most 268,435,456 (2^28; larger inputs fail with
+ * FPDB_ERR_INVALID).  Asynchronous: the selected count is written to
+ * d_num_out (device int64); rows past the smallest out[i].length are not
+ * written, and the caller compares *d_num_out with it after synchronizing.
+ * Every call on one GPU shares that GPU's scratch array: calls that overlap
+ * on one GPU (different streams or threads) race; calls on one stream are
+ * ordered and safe. */
+fpdb_status fpdb_filter_compact_noalloc(fpdb_context* ctx, const fpdb_column* in, int num_columns,
+                                        int64_t num_rows, const uint8_t* d_bitmap, fpdb_column* out,
+                                        int64_t* d_num_out);
+
 /* Fused blocked-bloom use: Find + compaction in one pass over the columns
  * (BloomFilterUseKernel::filterRecordBatch, BloomFilterUseKernel.cpp:180-207).
  * key_index[j] names which of `in` are the key columns.  d_bitmap may be NULL.
