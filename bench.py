@@ -303,11 +303,11 @@ def bench_groupby(ctx, steps):
         ctx.synchronize()
         if i >= 1:
             wall.append((time.perf_counter() - t0) * 1e3)
-    # consume = local pass + scan, scatter pass, compact pass (one host sync between)
-    local_ms, _ = kernel_ms(ctx, "groupby_local")
-    scatter_ms, _ = kernel_ms(ctx, "groupby_scatter")
+    # consume = sample + count pass + offsets, main pass, head merge + compact pass
+    count_ms, _ = kernel_ms(ctx, "groupby_count")
+    main_ms, _ = kernel_ms(ctx, "groupby_main")
     compact_ms, _ = kernel_ms(ctx, "groupby_compact")
-    cons_ms = local_ms + scatter_ms + compact_ms
+    cons_ms = count_ms + main_ms + compact_ms
     t0 = time.perf_counter()
     keys, out = g.finalize()
     finalize_ms = (time.perf_counter() - t0) * 1e3
@@ -333,11 +333,11 @@ def bench_groupby(ctx, steps):
                                        f"over all {m} rows, {cpu_s:.2f} s"},
             "workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
                         "COUNT, SUM int64 x2, SUM float64", "groups": int(len(k)),
-            "consume_kernels_ms": cons_ms, "local_ms": local_ms, "scatter_ms": scatter_ms,
+            "consume_kernels_ms": cons_ms, "count_ms": count_ms, "main_ms": main_ms,
             "compact_ms": compact_ms, "consume_wall_ms": statistics.median(wall),
             "grows_per_s": n / (cons_ms * 1e-3) / 1e9, "finalize_ms": finalize_ms,
             "hbm_frac": alg / (cons_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic_bytes": load_traffic("groupby_local")}
+            "traffic_bytes": load_traffic("groupby_main")}
 
 
 def bench_star(ctx, steps, device_name):
@@ -723,7 +723,9 @@ def main():
         log(f"bloom: find {res['bloom']['find_kernel_ms']:.4f} ms")
     if args.only in (None, "groupby"):
         res["groupby"] = bench_groupby(ctx, max(2, args.steps // 5))
-        log(f"groupby: consume {res['groupby']['consume_kernels_ms']:.4f} ms (local {res['groupby']['local_ms']:.3f}, scatter {res['groupby']['scatter_ms']:.3f}, compact {res['groupby']['compact_ms']:.3f}; wall {res['groupby']['consume_wall_ms']:.3f})")
+        gb = res["groupby"]
+        log(f"groupby: consume {gb['consume_kernels_ms']:.4f} ms (count {gb['count_ms']:.3f}, main {gb['main_ms']:.3f}, "
+            f"compact {gb['compact_ms']:.3f}; wall {gb['consume_wall_ms']:.3f})")
     if args.only in (None, "shuffle"):
         res["shuffle"] = bench_shuffle(ctx, args.steps)
         log(f"shuffle: {res['shuffle']['kernels_ms']:.4f} ms (scatter {res['shuffle']['scatter_kernel_ms']:.4f})")

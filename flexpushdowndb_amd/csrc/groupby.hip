@@ -40,11 +40,10 @@ namespace fpdb {
 
 constexpr uint64_t kEmptyKey = ~0ULL;
 constexpr int kMaxAggs = 16;          // visible + hidden accumulators
-constexpr int kLdsProbes = 8;
 constexpr int kGroupThreads = 1024;
-constexpr int kMinBuckets = 64;
-constexpr int kMaxBuckets = 1 << 14;
-constexpr int64_t kRowsPerWorkgroup = 16384;   // fewer rows per workgroup: fewer workgroups
+constexpr int kMinBuckets = 256;     // at least one compact workgroup per CU
+constexpr int kMaxBuckets = 1 << 12;
+constexpr int64_t kMaxBatchRows = int64_t(1) << 30;   // consume splits larger batches (32-bit batch rows in LDS)
 
 __device__ __forceinline__ bool type_is_signed_int_d(int32_t t) {
   return t == FPDB_TYPE_INT8 || t == FPDB_TYPE_INT16 || t == FPDB_TYPE_INT32 || t == FPDB_TYPE_INT64 ||
@@ -354,12 +353,40 @@ __device__ __forceinline__ void load_rec(const uint64_t* rec, uint64_t (&w)[RW])
 }
 
 // ---- kernels ---------------------------------------------------------------
+//
+// A consume runs as:
+//   sample   (one workgroup) counts the keys of up to 65,536 sampled rows in
+//            LDS and picks the most frequent ones as this batch's head keys
+//            (as many as the main pass's LDS holds), each with its bucket and
+//            its rank among the bucket's head keys;
+//   count    (one workgroup per CU) looks every row's key up among the head
+//            keys and counts the other ("tail") rows per bucket;
+//   offsets  (one workgroup) turns the counts into record positions in the
+//            new pool: bucket-major, old records, then head records, then
+//            each workgroup's tail rows;
+//   main     (same workgroups, same rows) aggregates head rows in LDS and
+//            writes each tail row as a record at its position; then writes
+//            its head partials;
+//   merge    adds each head key's partials in workgroup order and stores its
+//            record at its position;
+//   compact  (one workgroup per bucket) aggregates old records, head records
+//            and tail rows of the bucket into one record per group.
+// A row is a head row iff its key is a head key, a fixed function of the key
+// during the batch, so the count and main passes agree on every row without
+// the main pass re-deriving anything from the input.  The input is read once
+// in full (main) plus its key column once more (count).
+
+constexpr int kSampleBlocks = 64;
+constexpr int kSampleBlockRows = 1024;
+constexpr int kSampleSlots = 8192;           // the sampler's LDS count table
+constexpr int kHeadIndexSlots = 8192;        // head index: head id per slot
+constexpr uint32_t kNoHead = 0xFFFFFFFFu;
+constexpr int kCountBins = 512;
 
 struct GbGeom {
   int64_t n;            // rows in this batch
   int64_t row_base;     // global row index of the batch's first row
-  int64_t per_block;    // rows per workgroup
-  uint32_t nslots;      // LDS slots of the local / scatter passes
+  int32_t max_heads;    // head capacity of the main pass's LDS
   int32_t nbuckets;
   int32_t bucket_shift; // 64 - log2(nbuckets)
   int32_t rec_words;    // 2 + nacc
@@ -399,181 +426,308 @@ __device__ __forceinline__ void acc_apply_rows(const LdsTable& T, const AccSpec&
   }
 }
 
-// Local pass: LDS aggregation of the workgroup's rows; tail rows counted per
-// bucket; the table written out slot by slot (partial[b][slot]).
-template <int NACC>
-__global__ __launch_bounds__(kGroupThreads) void gb_local_kernel(GKeySpec keys, ColSet vals, AccSpec A, GbGeom G,
-                                                                 uint32_t* __restrict__ hist,
-                                                                 uint64_t* __restrict__ partial, int* key_range) {
-  extern __shared__ uint64_t lds[];
-  const int nacc = NACC > 0 ? NACC : A.n;
-  LdsTable T = lds_table(lds, G.nslots);
-  uint32_t* lhist = reinterpret_cast<uint32_t*>(lds + lds_table_words(G.nslots, nacc));
-  lds_table_init(T, A, nacc);
-  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x) lhist[p] = 0;
-  __syncthreads();
+__device__ __forceinline__ uint32_t head_home(uint64_t h) { return (uint32_t)h & (kHeadIndexSlots - 1); }
 
+// Head id of key k (hash h), or -1: the index holds at most a third as many
+// ids as slots, so every walk ends at an empty slot within a few probes.
+__device__ __forceinline__ int head_lookup(const uint32_t* s_idx, const uint64_t* s_hkeys, uint64_t k, uint64_t h) {
+  uint32_t s = head_home(h);
+  for (int p = 0; p < kHeadIndexSlots; ++p) {
+    const uint32_t id = s_idx[s];
+    if (id == kNoHead) return -1;
+    if (s_hkeys[id] == k) return (int)id;
+    s = (s + 1) & (kHeadIndexSlots - 1);
+  }
+  return -1;
+}
+
+// Every workgroup builds the head index in its LDS from the head key list.
+__device__ __forceinline__ void head_index_build(uint32_t* s_idx, uint64_t* s_hkeys, const uint64_t* __restrict__ hkeys,
+                                                 int nhead) {
+  for (int s = threadIdx.x; s < kHeadIndexSlots; s += blockDim.x) s_idx[s] = kNoHead;
+  for (int h = threadIdx.x; h < nhead; h += blockDim.x) s_hkeys[h] = hkeys[h];
+  __syncthreads();
+  for (int h = threadIdx.x; h < nhead; h += blockDim.x) {
+    uint32_t s = head_home(fmix64(s_hkeys[h]));
+    while (atomicCAS(&s_idx[s], kNoHead, (uint32_t)h) != kNoHead) s = (s + 1) & (kHeadIndexSlots - 1);
+  }
+  __syncthreads();
+}
+
+// Sample: keys of up to kSampleBlocks x kSampleBlockRows rows (all rows of a
+// smaller batch; otherwise kSampleBlocks evenly spaced runs of consecutive
+// rows) counted in an LDS table; the max_heads keys with the highest counts
+// (ties at the threshold count in any order) become the head keys.  Per head
+// key: its bucket's head count is incremented and its rank there recorded.
+// One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G, uint64_t* __restrict__ head_keys,
+                                                         uint32_t* __restrict__ head_rank,
+                                                         unsigned int* __restrict__ head_count, int* nhead_out) {
+  extern __shared__ uint64_t lds[];
+  uint64_t* s_keys = lds;                                                     // kSampleSlots + 1
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kSampleSlots + 1);    // kSampleSlots + 1
+  uint32_t* s_hist = s_cnt + kSampleSlots + 1;                               // kCountBins
+  __shared__ int s_thresh, s_take, s_n, s_tied;
+  for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x) {
+    s_keys[s] = kEmptyKey;
+    s_cnt[s] = 0;
+  }
+  for (int b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) head_count[b] = 0;
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_tied = 0;
+  }
+  __syncthreads();
+  const int64_t n = G.n;
+  const bool all = n <= (int64_t)kSampleBlocks * kSampleBlockRows;
+  constexpr int U = 8;
+  bool dummy = false;
+  for (int j0 = 0; j0 < kSampleBlocks; j0 += U) {
+    int64_t rc[U];
+    bool live[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t j = j0 + u;
+      int64_t r = all ? j * kSampleBlockRows + threadIdx.x
+                      : (n - kSampleBlockRows) * j / (kSampleBlocks - 1) + threadIdx.x;
+      live[u] = r < n;
+      rc[u] = live[u] ? r : n - 1;
+    }
+    uint64_t kk[U];
+    group_keys_n<U>(keys, rc, kk, &dummy);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!live[u]) continue;
+      const uint64_t k = kk[u];
+      int slot = -1;
+      if (k == kEmptyKey) {
+        slot = kSampleSlots;
+      } else {
+        uint32_t s = (uint32_t)fmix64(k) & (kSampleSlots - 1);
+        for (int p = 0; p < 16; ++p) {
+          uint64_t cur = s_keys[s];
+          if (cur == k) { slot = (int)s; break; }
+          if (cur == kEmptyKey) {
+            unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[s]), kEmptyKey, k);
+            if (prev == kEmptyKey || prev == k) { slot = (int)s; break; }
+          }
+          s = (s + 1) & (kSampleSlots - 1);
+        }
+      }
+      if (slot >= 0) atomicAdd(&s_cnt[slot], 1u);
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x)
+    if (s_cnt[s]) atomicAdd(&s_hist[min(s_cnt[s], (uint32_t)(kCountBins - 1))], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // threshold bin T: every key in a bin above T is taken, s_take of bin T
+    int acc = 0, T = 1, take = 0;
+    for (int b = kCountBins - 1; b >= 1; --b) {
+      const int hb = (int)s_hist[b];
+      if (acc + hb >= G.max_heads) {
+        T = b;
+        take = G.max_heads - acc;
+        break;
+      }
+      acc += hb;
+      if (b == 1) {
+        T = 1;
+        take = hb;
+      }
+    }
+    s_thresh = T;
+    s_take = take;
+  }
+  __syncthreads();
+  const uint32_t T = (uint32_t)s_thresh;
+  for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x) {
+    const uint32_t c = s_cnt[s];
+    if (!c) continue;
+    const uint32_t bin = min(c, (uint32_t)(kCountBins - 1));
+    bool take = bin > T;
+    if (bin == T) take = atomicAdd(&s_tied, 1) < s_take;
+    if (!take) continue;
+    const int id = atomicAdd(&s_n, 1);
+    const uint64_t k = s == kSampleSlots ? kEmptyKey : s_keys[s];
+    head_keys[id] = k;
+    head_rank[id] = atomicAdd(&head_count[bucket_of(fmix64(k), G.bucket_shift)], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *nhead_out = s_n;
+}
+
+// Rows are dealt to the count and main passes' workgroups in blocks of
+// ITEMS x 1024 consecutive rows, round-robin, so all workgroups stream through
+// neighbouring rows (HBM page locality) and both passes give every workgroup
+// the same rows.
+template <int ITEMS>
+__device__ __forceinline__ int64_t gb_block_rows() { return (int64_t)ITEMS * kGroupThreads; }
+
+// Count pass: tail rows per bucket of this workgroup's rows, written to
+// hist[bucket][workgroup].
+template <int ITEMS>
+__global__ __launch_bounds__(kGroupThreads) void gb_count_kernel(GKeySpec keys, GbGeom G,
+                                                                 const uint64_t* __restrict__ head_keys,
+                                                                 const int* __restrict__ nhead_p,
+                                                                 uint32_t* __restrict__ hist, int* key_range) {
+  extern __shared__ uint64_t lds[];
+  uint32_t* s_idx = reinterpret_cast<uint32_t*>(lds);
+  uint64_t* s_hkeys = lds + kHeadIndexSlots / 2;
+  uint32_t* s_hist = reinterpret_cast<uint32_t*>(s_hkeys + G.max_heads);
+  const int nhead = *nhead_p;
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) s_hist[b] = 0;
+  head_index_build(s_idx, s_hkeys, head_keys, nhead);
   bool out_of_range = false;
-  // ITEMS rows per thread per step, their loads issued before any LDS work
-  constexpr int ITEMS = NACC > 0 ? 4 : 1;
-  constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
-  // blocks of ITEMS x blockDim rows are dealt round-robin to the workgroups,
-  // so all workgroups stream through neighbouring rows (HBM page locality)
-  const int64_t brows = (int64_t)ITEMS * blockDim.x;
+  const int64_t brows = gb_block_rows<ITEMS>();
   for (int64_t b0 = (int64_t)blockIdx.x * brows; b0 < G.n; b0 += (int64_t)gridDim.x * brows) {
     const int64_t end = b0 + brows < G.n ? b0 + brows : G.n;
     const int64_t r0 = b0 + threadIdx.x;
     if (r0 >= end) continue;
-    // rows past the range re-read its last row (r0 < end), so every load is unconditional
     int64_t rc[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      int64_t r = r0 + (int64_t)i * blockDim.x;
+      const int64_t r = r0 + (int64_t)i * kGroupThreads;
       rc[i] = r < end ? r : end - 1;
     }
     uint64_t kk[ITEMS];
-    uint64_t vv[NA][ITEMS];
-    bool okk[NA][ITEMS];
     group_keys_n<ITEMS>(keys, rc, kk, &out_of_range);
-    acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
-    int slot[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      int64_t r = r0 + (int64_t)i * blockDim.x;
-      slot[i] = -2;
-      if (r >= end) continue;
-      uint64_t h = fmix64(kk[i]);
-      slot[i] = lds_find_or_insert(T, kk[i], h, kLdsProbes);
-      if (slot[i] < 0) atomicAdd(&lhist[bucket_of(h, G.bucket_shift)], 1u);
-    }
-    acc_apply_rows<ITEMS, NA>(T, A, nacc, slot, vv, okk);
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if (slot[i] < 0) continue;
-      long long grow = (long long)(G.row_base + r0 + (int64_t)i * blockDim.x);
-      if (grow < T.first[slot[i]]) atomicMin(&T.first[slot[i]], grow);
+      if (r0 + (int64_t)i * kGroupThreads >= end) continue;
+      const uint64_t h = fmix64(kk[i]);
+      if (head_lookup(s_idx, s_hkeys, kk[i], h) < 0) atomicAdd(&s_hist[bucket_of(h, G.bucket_shift)], 1u);
     }
   }
   if (out_of_range) *key_range = 1;
   __syncthreads();
-  // write the table out slot by slot; used slots count towards their bucket
-  uint64_t* out = partial + (size_t)blockIdx.x * T.stride * G.rec_words;
-  for (uint32_t s = threadIdx.x; s < T.stride; s += blockDim.x) {
-    long long f = T.first[s];
-    uint64_t k = s == T.nslots ? kEmptyKey : T.keys[s];
-    uint64_t* rec = out + (size_t)s * G.rec_words;
-    rec[0] = k;
-    rec[1] = (uint64_t)f;
-    for (int a = 0; a < nacc; ++a) rec[2 + a] = T.acc[(size_t)a * T.stride + s];
-    if (f != LLONG_MAX) atomicAdd(&lhist[bucket_of(fmix64(k), G.bucket_shift)], 1u);
-  }
-  __syncthreads();
-  // bucket-major: entry 0 of each bucket's row is left for its old record count
-  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x)
-    hist[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x] = lhist[p];
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) hist[(size_t)b * gridDim.x + blockIdx.x] = s_hist[b];
 }
 
-// Bucket-major exclusive scan over rows of (old record count, per-workgroup
-// counts) of every bucket: base[p][0] = nstart[p], the start of bucket p in the
-// new pool; base[p][1 + b] = where workgroup b writes its bucket-p records,
-// after the bucket's old records.  One workgroup, each thread scanning a
-// contiguous run; also resets the group counter the compact pass adds to.
-__global__ __launch_bounds__(1024) void gb_scan_kernel(const uint32_t* __restrict__ hist, int blocks, int nbuckets,
-                                                       const int64_t* __restrict__ ocount,
-                                                       uint64_t* __restrict__ base, int64_t* __restrict__ nstart,
-                                                       int64_t* __restrict__ newcount, unsigned long long* totals) {
+// Offsets (one workgroup): for bucket b, its range of the new pool starts at
+// nstart[b] and holds its old records (ocount[b], aggregated in place by the
+// compact pass), its head records (head_count[b]), then the tail rows of
+// workgroup 0, 1, ...; base[b][w] is where workgroup w writes its first
+// tail row of bucket b.  newcount[b] = head + tail records of b; totals[0] =
+// records of the new pool; totals[1] (groups) reset for the compact pass.
+__global__ __launch_bounds__(1024) void gb_offsets_kernel(const uint32_t* __restrict__ hist, int blocks, int nbuckets,
+                                                          const int64_t* __restrict__ ocount,
+                                                          const unsigned int* __restrict__ head_count,
+                                                          uint64_t* __restrict__ base, int64_t* __restrict__ nstart,
+                                                          int64_t* __restrict__ newcount,
+                                                          unsigned long long* totals) {
+  __shared__ uint64_t s_tail[kMaxBuckets];
   __shared__ uint64_t s_w[16];
-  const int64_t row = blocks + 1;
-  const int64_t len = (int64_t)nbuckets * row;
-  const int64_t per = (len + blockDim.x - 1) / blockDim.x;
-  const int64_t lo = (int64_t)threadIdx.x * per;
-  const int64_t hi = lo + per < len ? lo + per : len;
-  // walk (bucket, column) along with the flat index: one division per thread
-  const int64_t p_lo = lo / row, b_lo = lo - p_lo * row;
-  uint64_t mine = 0;
-  {
-    int64_t p = p_lo, b = b_lo;
-    for (int64_t i = lo; i < hi; ++i) {
-      mine += b == 0 ? (uint64_t)ocount[p] : (uint64_t)hist[i];
-      if (++b == row) { b = 0; ++p; }
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int per = (blocks + 63) / 64;   // consecutive counts per lane
+  // per bucket: exclusive scan of its workgroups' tail counts (relative)
+  for (int b = wave; b < nbuckets; b += nwaves) {
+    const uint32_t* row = hist + (size_t)b * blocks;
+    uint64_t mine = 0;
+    for (int j = 0; j < per; ++j) {
+      const int w = lane * per + j;
+      if (w < blocks) mine += row[w];
     }
+    uint64_t incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint64_t t = __shfl_up(incl, off);
+      if (lane >= off) incl += t;
+    }
+    uint64_t run = incl - mine;
+    for (int j = 0; j < per; ++j) {
+      const int w = lane * per + j;
+      if (w < blocks) {
+        base[(size_t)b * blocks + w] = run;
+        run += row[w];
+      }
+    }
+    if (lane == 63) s_tail[b] = incl;
   }
-  // exclusive scan of the per-thread sums
+  __syncthreads();
+  // exclusive scan over buckets of old + head + tail records
+  const int bper = (nbuckets + (int)blockDim.x - 1) / (int)blockDim.x;
+  uint64_t mine = 0;
+  for (int j = 0; j < bper; ++j) {
+    const int b = threadIdx.x * bper + j;
+    if (b < nbuckets) mine += (uint64_t)ocount[b] + head_count[b] + s_tail[b];
+  }
   uint64_t incl = mine;
   for (int off = 1; off < 64; off <<= 1) {
-    uint64_t t = __shfl_up(incl, off);
-    if ((threadIdx.x & 63) >= off) incl += t;
+    const uint64_t t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
   }
-  if ((threadIdx.x & 63) == 63) s_w[threadIdx.x >> 6] = incl;
+  if (lane == 63) s_w[wave] = incl;
   __syncthreads();
   uint64_t run = incl - mine;
-  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += s_w[w];
-  {
-    int64_t p = p_lo, b = b_lo;
-    for (int64_t i = lo; i < hi; ++i) {
-      base[i] = run;
-      if (b == 0) nstart[p] = (int64_t)run;
-      run += b == 0 ? (uint64_t)ocount[p] : (uint64_t)hist[i];
-      if (++b == row) { b = 0; ++p; }
+  for (int w = 0; w < wave; ++w) run += s_w[w];
+  for (int j = 0; j < bper; ++j) {
+    const int b = threadIdx.x * bper + j;
+    if (b < nbuckets) {
+      nstart[b] = (int64_t)run;
+      newcount[b] = (int64_t)(head_count[b] + s_tail[b]);
+      const uint64_t rec = (uint64_t)ocount[b] + head_count[b] + s_tail[b];
+      s_tail[b] = run + (uint64_t)ocount[b] + head_count[b];   // where the bucket's tail rows start
+      run += rec;
     }
   }
   if (threadIdx.x == blockDim.x - 1) {
-    totals[0] = (unsigned long long)run;   // records in the new pool
-    totals[1] = 0;                         // groups (the compact pass adds them up)
+    totals[0] = (unsigned long long)run;
+    totals[1] = 0;
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < nbuckets; p += blockDim.x) {
-    uint64_t end = p + 1 < nbuckets ? base[(int64_t)(p + 1) * row] : 0;
-    if (p + 1 == nbuckets) {
-      end = base[(int64_t)p * row] + (uint64_t)ocount[p];
-      for (int b = 1; b <= blocks; ++b) end += hist[(int64_t)p * row + b];
-    }
-    newcount[p] = (int64_t)(end - base[(int64_t)p * row]) - ocount[p];
+  for (int b = wave; b < nbuckets; b += nwaves) {
+    const uint64_t start = s_tail[b];
+    for (int w = lane; w < blocks; w += 64) base[(size_t)b * blocks + w] += start;
   }
 }
 
-// Scatter pass: the workgroup's LDS table is rebuilt at its old slots (keys
-// only), so lookups within the same probe window classify every row exactly as
-// the local pass did; tail rows and the partial records go to their buckets.
+// Main pass: head rows aggregated in LDS (slot = head id), tail rows written
+// as records (key, row, operands) at their positions; then the head partials
+// (first row, accumulators) to hpart[workgroup][1 + nacc][max_heads].
 template <int NACC>
-__global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys, ColSet vals, AccSpec A, GbGeom G,
-                                                                   const uint64_t* __restrict__ partial,
-                                                                   const uint64_t* __restrict__ base,
-                                                                   uint64_t* __restrict__ pool) {
+__global__ __launch_bounds__(kGroupThreads) void gb_main_kernel(GKeySpec keys, ColSet vals, AccSpec A, GbGeom G,
+                                                                const uint64_t* __restrict__ head_keys,
+                                                                const int* __restrict__ nhead_p,
+                                                                const uint64_t* __restrict__ base,
+                                                                uint64_t* __restrict__ pool,
+                                                                uint64_t* __restrict__ hpart) {
   extern __shared__ uint64_t lds[];
   const int nacc = NACC > 0 ? NACC : A.n;
-  const uint32_t stride = G.nslots + 1;
-  uint64_t* skeys = lds;
-  // next record position of this workgroup in each bucket, starting at its
-  // scanned base: one LDS atomic per record, no global read
-  unsigned long long* lpos = reinterpret_cast<unsigned long long*>(lds + stride);
-  const uint64_t* part = partial + (size_t)blockIdx.x * stride * G.rec_words;
-  stage_words(skeys, part, stride, G.rec_words);
-  for (int p = threadIdx.x; p < G.nbuckets; p += blockDim.x)
-    lpos[p] = base[(size_t)p * (gridDim.x + 1) + 1 + blockIdx.x];
-  __syncthreads();
+  const int H = G.max_heads;
+  uint32_t* s_idx = reinterpret_cast<uint32_t*>(lds);
+  uint64_t* s_hkeys = lds + kHeadIndexSlots / 2;
+  uint64_t* s_acc = s_hkeys + H;                                                     // nacc x H
+  unsigned long long* s_pos = reinterpret_cast<unsigned long long*>(s_acc + (size_t)nacc * H);   // nbuckets
+  uint32_t* s_first = reinterpret_cast<uint32_t*>(s_pos + G.nbuckets);              // H, batch-relative rows
+  const int nhead = *nhead_p;
+  for (int h = threadIdx.x; h < nhead; h += blockDim.x) {
+    s_first[h] = 0xFFFFFFFFu;
+    for (int a = 0; a < nacc; ++a) s_acc[(size_t)a * H + h] = acc_identity(A.op[a]);
+  }
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) s_pos[b] = base[(size_t)b * gridDim.x + blockIdx.x];
+  head_index_build(s_idx, s_hkeys, head_keys, nhead);
   LdsTable T;
-  T.keys = skeys;
+  T.keys = s_hkeys;
   T.first = nullptr;
-  T.acc = nullptr;
-  T.nslots = G.nslots;
-  T.stride = stride;
+  T.acc = s_acc;
+  T.nslots = (uint32_t)H;
+  T.stride = (uint32_t)H;
   bool dummy = false;
   constexpr int ITEMS = NACC > 0 ? 4 : 1;
   constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
-  // blocks of ITEMS x blockDim rows are dealt round-robin to the workgroups,
-  // so all workgroups stream through neighbouring rows (HBM page locality)
-  const int64_t brows = (int64_t)ITEMS * blockDim.x;
+  const int64_t brows = gb_block_rows<ITEMS>();
   for (int64_t b0 = (int64_t)blockIdx.x * brows; b0 < G.n; b0 += (int64_t)gridDim.x * brows) {
     const int64_t end = b0 + brows < G.n ? b0 + brows : G.n;
     const int64_t r0 = b0 + threadIdx.x;
     if (r0 >= end) continue;
-    // unconditional loads of keys and operands (tail rows touch nearly every
-    // line of the value columns anyway)
     int64_t rc[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      int64_t r = r0 + (int64_t)i * blockDim.x;
+      const int64_t r = r0 + (int64_t)i * kGroupThreads;
       rc[i] = r < end ? r : end - 1;
     }
     uint64_t kk[ITEMS];
@@ -581,46 +735,111 @@ __global__ __launch_bounds__(kGroupThreads) void gb_scatter_kernel(GKeySpec keys
     bool okk[NA][ITEMS];
     group_keys_n<ITEMS>(keys, rc, kk, &dummy);
     acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
+    int slot[ITEMS];
+    uint64_t hh[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      int64_t r = r0 + (int64_t)i * blockDim.x;
-      if (r >= end) continue;
-      uint64_t k = kk[i];
-      uint64_t h = fmix64(k);
-      if (lds_find(T, k, h, kLdsProbes) >= 0) continue;
-      uint32_t p = bucket_of(h, G.bucket_shift);
-      uint64_t at = atomicAdd(&lpos[p], 1ULL);
-      uint64_t* rec = pool + at * G.rec_words;
-      if constexpr (NACC > 0) {
-        uint64_t w[2 + NACC];
-        w[0] = k;
-        w[1] = (uint64_t)(G.row_base + r);
+      hh[i] = fmix64(kk[i]);
+      slot[i] = r0 + (int64_t)i * kGroupThreads < end ? head_lookup(s_idx, s_hkeys, kk[i], hh[i]) : -2;
+    }
+    acc_apply_rows<ITEMS, NA>(T, A, nacc, slot, vv, okk);
 #pragma unroll
-        for (int a = 0; a < NACC; ++a) w[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
-        store_rec<2 + NACC>(rec, w);
-      } else {
-        rec[0] = k;
-        rec[1] = (uint64_t)(G.row_base + r);
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t rel = (uint32_t)(r0 + (int64_t)i * kGroupThreads);
+      if (slot[i] >= 0) {
+        if (rel < s_first[slot[i]]) atomicMin(&s_first[slot[i]], rel);
+      } else if (slot[i] == -1) {
+        const uint64_t at = atomicAdd(&s_pos[bucket_of(hh[i], G.bucket_shift)], 1ULL);
+        uint64_t* rec = pool + at * G.rec_words;
+        if constexpr (NACC > 0) {
+          uint64_t w[2 + NACC];
+          w[0] = kk[i];
+          w[1] = (uint64_t)(G.row_base + rel);
 #pragma unroll
-        for (int a = 0; a < NA; ++a)
-          if (a < nacc) rec[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+          for (int a = 0; a < NACC; ++a) w[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+          store_rec<2 + NACC>(rec, w);
+        } else {
+          rec[0] = kk[i];
+          rec[1] = (uint64_t)(G.row_base + rel);
+#pragma unroll
+          for (int a = 0; a < NA; ++a)
+            if (a < nacc) rec[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+        }
       }
     }
   }
-  for (uint32_t s = threadIdx.x; s < stride; s += blockDim.x) {
-    const uint64_t* src = part + (size_t)s * G.rec_words;
-    if ((long long)src[1] == LLONG_MAX) continue;
-    uint32_t p = bucket_of(fmix64(src[0]), G.bucket_shift);
-    uint64_t at = atomicAdd(&lpos[p], 1ULL);
-    uint64_t* rec = pool + at * G.rec_words;
-    if constexpr (NACC > 0) {
-      uint64_t w[2 + NACC];
-      load_rec<2 + NACC>(src, w);
-      store_rec<2 + NACC>(rec, w);
-    } else {
-      for (int w = 0; w < G.rec_words; ++w) rec[w] = src[w];
+  __syncthreads();
+  uint64_t* out = hpart + (size_t)blockIdx.x * (1 + nacc) * H;
+  for (int h = threadIdx.x; h < nhead; h += blockDim.x) {
+    const uint32_t f = s_first[h];
+    out[h] = f == 0xFFFFFFFFu ? (uint64_t)LLONG_MAX : (uint64_t)(G.row_base + f);
+    for (int a = 0; a < nacc; ++a) out[(size_t)(1 + a) * H + h] = s_acc[(size_t)a * H + h];
+  }
+}
+
+// Applies an accumulator value to a register accumulator (merge of partials).
+__device__ __forceinline__ uint64_t acc_combine(int32_t op, uint64_t x, uint64_t v) {
+  switch (op) {
+    case kAccCountAll:
+    case kAccCountValid:
+    case kAccSumI: return x + v;
+    case kAccSumF: return (uint64_t)__double_as_longlong(__longlong_as_double((long long)x) +
+                                                          __longlong_as_double((long long)v));
+    case kAccMinI: return (long long)v < (long long)x ? v : x;
+    case kAccMaxI: return (long long)v > (long long)x ? v : x;
+    case kAccMinU: return v < x ? v : x;
+    case kAccMaxU: return v > x ? v : x;
+    case kAccMinF: return __longlong_as_double((long long)v) < __longlong_as_double((long long)x) ? v : x;
+    case kAccMaxF: return __longlong_as_double((long long)v) > __longlong_as_double((long long)x) ? v : x;
+    default: return x;
+  }
+}
+
+// Merge: 64 head keys per workgroup of 4 waves; wave q adds the partials of
+// workgroups [q * blocks / 4, (q + 1) * blocks / 4) in order, then the four
+// sums are added in wave order: a fixed order, so head keys' FP64 sums do
+// not depend on scheduling across workgroups.  Each head key's record goes to
+// nstart[b] + ocount[b] + its rank among bucket b's head keys.
+constexpr int kMergeWaves = 4;
+__global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
+    AccSpec A, GbGeom G, int blocks, const uint64_t* __restrict__ head_keys, const uint32_t* __restrict__ head_rank,
+    const int* __restrict__ nhead_p, const uint64_t* __restrict__ hpart, const int64_t* __restrict__ nstart,
+    const int64_t* __restrict__ ocount, uint64_t* __restrict__ pool) {
+  __shared__ uint64_t s_part[kMergeWaves][1 + kMaxAggs][64];
+  const int lane = threadIdx.x & 63;
+  const int q = threadIdx.x >> 6;
+  const int nacc = A.n;
+  const int H = G.max_heads;
+  const int h = blockIdx.x * 64 + lane;
+  const int nhead = *nhead_p;
+  const bool live = h < nhead;
+  const int w0 = (int)((int64_t)blocks * q / kMergeWaves), w1 = (int)((int64_t)blocks * (q + 1) / kMergeWaves);
+  uint64_t first = (uint64_t)LLONG_MAX;
+  uint64_t acc[kMaxAggs];
+  for (int a = 0; a < nacc; ++a) acc[a] = acc_identity(A.op[a]);
+  if (live) {
+    for (int w = w0; w < w1; ++w) {
+      const uint64_t* p = hpart + (size_t)w * (1 + nacc) * H + h;
+      const uint64_t f = p[0];
+      first = (long long)f < (long long)first ? f : first;
+      for (int a = 0; a < nacc; ++a) acc[a] = acc_combine(A.op[a], acc[a], p[(size_t)(1 + a) * H]);
     }
   }
+  s_part[q][0][lane] = first;
+  for (int a = 0; a < nacc; ++a) s_part[q][1 + a][lane] = acc[a];
+  __syncthreads();
+  if (q != 0 || !live) return;
+  for (int r = 1; r < kMergeWaves; ++r) {
+    const uint64_t f = s_part[r][0][lane];
+    first = (long long)f < (long long)first ? f : first;
+    for (int a = 0; a < nacc; ++a) acc[a] = acc_combine(A.op[a], acc[a], s_part[r][1 + a][lane]);
+  }
+  const uint64_t k = head_keys[h];
+  const uint32_t b = bucket_of(fmix64(k), G.bucket_shift);
+  uint64_t* rec = pool + (size_t)(nstart[b] + ocount[b] + head_rank[h]) * G.rec_words;
+  rec[0] = k;
+  rec[1] = first;
+  for (int a = 0; a < nacc; ++a) rec[2 + a] = acc[a];
 }
 
 // Compact pass, one workgroup per bucket: old records (previous batches, one
@@ -912,10 +1131,11 @@ struct fpdb_group_table {
   int64_t max_groups = 0;
   int rec_words = 0;                 // 2 + accumulators
   int nbuckets = 1, bucket_shift = 64;
-  uint32_t local_slots = 0;          // LDS slots of the local and scatter passes
+  int max_heads = 0;                 // head keys the main pass's LDS holds
   uint32_t compact_slots = 0;        // LDS slots of the compact pass
-  size_t local_lds = 0, scatter_lds = 0, compact_lds = 0;
-  int max_blocks = 0;
+  size_t sample_lds = 0, count_lds = 0, main_lds = 0, compact_lds = 0;
+  int max_blocks = 0;                // workgroups of the count and main passes (one per CU)
+  int64_t pool_records_bound = 0;    // records the pool can hold after the last consume (an upper bound)
   // device buffers
   uint64_t* pool = nullptr;          // one record per group so far (bucket ranges ostart/ocount)
   int64_t pool_cap = 0;              // records
@@ -925,9 +1145,13 @@ struct fpdb_group_table {
   int64_t* ocount = nullptr;         // nbuckets
   int64_t* nstart = nullptr;         // nbuckets
   int64_t* newcount = nullptr;       // nbuckets
-  uint64_t* base = nullptr;          // nbuckets x (max_blocks + 1), bucket-major
-  uint32_t* hist = nullptr;          // nbuckets x (max_blocks + 1), bucket-major
-  uint64_t* partial = nullptr;       // max_blocks x (local_slots + 1) records
+  uint64_t* base = nullptr;          // nbuckets x max_blocks tail positions, bucket-major
+  uint32_t* hist = nullptr;          // nbuckets x max_blocks tail counts, bucket-major
+  uint64_t* head_keys = nullptr;     // max_heads
+  uint32_t* head_rank = nullptr;     // max_heads: rank among its bucket's head keys
+  unsigned int* head_count = nullptr;   // nbuckets
+  int* nhead = nullptr;              // head keys of the current batch
+  uint64_t* hpart = nullptr;         // max_blocks x (1 + accumulators) x max_heads head partials
   unsigned long long* totals = nullptr;   // [0] records of the new pool, [1] groups
   int* flags = nullptr;              // [0] bucket overflow, [1] key outside its packed range
   int64_t rows = 0;
@@ -1047,16 +1271,19 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   t->bucket_shift = 64 - lg;
   t->compact_slots = cslots;
   t->compact_lds = (size_t)(cslots + 1) * per_slot;
-  size_t hist_bytes = (size_t)nb * 4;
-  if (budget < hist_bytes + 65 * per_slot) {
-    delete t;
-    return fail(FPDB_ERR_INVALID, "group-by: too many aggregates for the LDS table");
-  }
-  t->local_slots = (uint32_t)((budget - hist_bytes) / per_slot) - 1;
-  t->local_lds = (size_t)(t->local_slots + 1) * per_slot + hist_bytes;
-  t->scatter_lds = (size_t)(t->local_slots + 1) * 8 + (size_t)nb * 8;   // keys + 64-bit bucket positions
+  // main pass LDS: head index (kHeadIndexSlots x 4 B), per head key its key,
+  // batch-relative first row (4 B) and accumulators, per bucket a 64-bit
+  // record position; at most a third of the index slots hold head keys
+  const size_t fixed = (size_t)kHeadIndexSlots * 4 + (size_t)nb * 8;
+  const size_t per_head = 8 + 4 + 8 * (size_t)nacc;
+  int64_t heads = budget > fixed ? (int64_t)((budget - fixed) / per_head) : 0;
+  heads = std::min<int64_t>(heads, kHeadIndexSlots / 3);
+  t->max_heads = (int)heads;
+  t->main_lds = fixed + per_head * (size_t)heads;
+  t->count_lds = (size_t)kHeadIndexSlots * 4 + (size_t)heads * 8 + (size_t)nb * 4;
+  t->sample_lds = (size_t)(kSampleSlots + 1) * 12 + (size_t)kCountBins * 4;
   t->max_blocks = std::max(1, c->num_cus);
-  if (t->local_slots < 64 || cslots < 64 || t->scatter_lds > budget) {
+  if (heads < 64 || cslots < 64 || t->sample_lds > budget) {
     delete t;
     return fail(FPDB_ERR_INVALID, "group-by: too many aggregates for the LDS table");
   }
@@ -1064,11 +1291,14 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->ocount));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->nstart));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->newcount));
-  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * (t->max_blocks + 1) * 8, reinterpret_cast<void**>(&t->base));
-  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * (t->max_blocks + 1) * 4, reinterpret_cast<void**>(&t->hist));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 8, reinterpret_cast<void**>(&t->base));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 4, reinterpret_cast<void**>(&t->hist));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)heads * 8, reinterpret_cast<void**>(&t->head_keys));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)heads * 4, reinterpret_cast<void**>(&t->head_rank));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 4, reinterpret_cast<void**>(&t->head_count));
+  if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->nhead));
   if (s == FPDB_OK)
-    s = gb_alloc(t, (uint64_t)t->max_blocks * (t->local_slots + 1) * t->rec_words * 8,
-                 reinterpret_cast<void**>(&t->partial));
+    s = gb_alloc(t, (uint64_t)t->max_blocks * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hpart));
   if (s == FPDB_OK) s = gb_alloc(t, 16, reinterpret_cast<void**>(&t->totals));
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->flags));
   if (s == FPDB_OK) s = launch_fill(c, t->ostart, 0, (uint64_t)nb * 8);
@@ -1084,16 +1314,10 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
 }
 
 template <int NACC>
-static void launch_local(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
-                         const GbGeom& G) {
-  hipLaunchKernelGGL(gb_local_kernel<NACC>, dim3(blocks), dim3(kGroupThreads), t->local_lds, c->stream, k, vals,
-                     t->acc, G, t->hist, t->partial, t->flags + 1);
-}
-template <int NACC>
-static void launch_scatter(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
-                           const GbGeom& G) {
-  hipLaunchKernelGGL(gb_scatter_kernel<NACC>, dim3(blocks), dim3(kGroupThreads), t->scatter_lds, c->stream, k, vals,
-                     t->acc, G, t->partial, t->base, t->spare);
+static void launch_main(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
+                        const GbGeom& G) {
+  hipLaunchKernelGGL(gb_main_kernel<NACC>, dim3(blocks), dim3(kGroupThreads), t->main_lds, c->stream, k, vals,
+                     t->acc, G, t->head_keys, t->nhead, t->base, t->spare, t->hpart);
 }
 template <int NACC>
 static void launch_compact(Context* c, fpdb_group_table* t) {
@@ -1111,6 +1335,76 @@ static void launch_compact(Context* c, fpdb_group_table* t) {
     case 5: FN<5>(__VA_ARGS__); break;            \
     default: FN<0>(__VA_ARGS__); break;           \
   }
+
+// One batch of n <= kMaxBatchRows rows: sample, count, offsets, main, merge,
+// compact, all on the context stream (no host synchronization unless the
+// worst-case size of the next pool does not fit the HBM budget).
+static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const GKeySpec& k, const ColSet& vals,
+                                         int64_t n) {
+  const int blocks = t->max_blocks;
+  GbGeom G;
+  G.n = n;
+  G.row_base = t->rows;
+  G.max_heads = t->max_heads;
+  G.nbuckets = t->nbuckets;
+  G.bucket_shift = t->bucket_shift;
+  G.rec_words = t->rec_words;
+  // The next pool holds at most: the groups so far (at most the records of
+  // the current pool), every head key and every row of the batch.
+  const int64_t bound = t->pool_records_bound + t->max_heads + n;
+  bool exact = false;
+  if (bound > t->spare_cap) {
+    if (t->spare) FPDB_TRY(ctx_free(c, t->spare));
+    t->spare = nullptr;
+    t->spare_cap = 0;
+    const int64_t want = std::max<int64_t>(bound, 1024);
+    fpdb_status s = gb_alloc(t, (uint64_t)want * t->rec_words * 8, reinterpret_cast<void**>(&t->spare));
+    if (s == FPDB_ERR_BUDGET) {
+      (void)fpdb_last_error();
+      exact = true;   // size it from the count pass instead
+    } else {
+      FPDB_TRY(s);
+      t->spare_cap = want;
+    }
+  }
+  {
+    ProfScope ps(c, "groupby_count");
+    hipLaunchKernelGGL(gb_sample_kernel, dim3(1), dim3(1024), t->sample_lds, c->stream, k, G, t->head_keys,
+                       t->head_rank, t->head_count, t->nhead);
+    hipLaunchKernelGGL(gb_count_kernel<4>, dim3(blocks), dim3(kGroupThreads), t->count_lds, c->stream, k, G,
+                       t->head_keys, t->nhead, t->hist, t->flags + 1);
+    hipLaunchKernelGGL(gb_offsets_kernel, dim3(1), dim3(1024), 0, c->stream, t->hist, blocks, t->nbuckets, t->ocount,
+                       t->head_count, t->base, t->nstart, t->newcount, t->totals);
+    FPDB_HIP(hipGetLastError());
+  }
+  if (exact) {
+    uint64_t total = 0;
+    FPDB_TRY(mail_read(c, t->totals, 1, &total));
+    const int64_t want = std::max<int64_t>((int64_t)total, 1024);
+    FPDB_TRY(gb_alloc(t, (uint64_t)want * t->rec_words * 8, reinterpret_cast<void**>(&t->spare)));
+    t->spare_cap = want;
+  }
+  {
+    ProfScope ps(c, "groupby_main");
+    FPDB_GB_DISPATCH(launch_main, c, t, blocks, k, vals, G);
+    FPDB_HIP(hipGetLastError());
+  }
+  {
+    ProfScope ps(c, "groupby_compact");
+    hipLaunchKernelGGL(gb_head_merge_kernel, dim3((unsigned)ceil_div(t->max_heads, 64)), dim3(64 * kMergeWaves), 0,
+                       c->stream, t->acc, G, blocks, t->head_keys, t->head_rank, t->nhead, t->hpart, t->nstart,
+                       t->ocount, t->spare);
+    FPDB_GB_DISPATCH(launch_compact, c, t);
+    FPDB_HIP(hipGetLastError());
+  }
+  std::swap(t->pool, t->spare);
+  std::swap(t->pool_cap, t->spare_cap);
+  // groups after the batch: at most the compact workgroups' slots, and at
+  // most the old groups plus the batch's rows
+  t->pool_records_bound = std::min<int64_t>(t->pool_records_bound + n, (int64_t)t->nbuckets * (t->compact_slots + 1));
+  t->rows += n;
+  return FPDB_OK;
+}
 
 }  // namespace fpdb
 
@@ -1187,8 +1481,8 @@ fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, in
 
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
-  void* bufs[] = {t->pool, t->spare, t->ostart, t->ocount, t->nstart, t->newcount,
-                  t->base, t->hist, t->partial, t->totals, t->flags};
+  void* bufs[] = {t->pool, t->spare, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
+                  t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->totals, t->flags};
   for (void* b : bufs)
     if (b) ctx_free(t->ctx, b);
   delete t;
@@ -1238,48 +1532,16 @@ fpdb_status fpdb_groupby_consume(fpdb_context* ctx, fpdb_group_table* t, const f
   }
   if (n == 0) return FPDB_OK;
   FPDB_CHECK(t->rows + n < (int64_t(1) << 62), FPDB_ERR_INVALID, "group-by: too many rows");
-  int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(t->max_blocks, ceil_div(n, kRowsPerWorkgroup)));
-  GbGeom G;
-  G.n = n;
-  G.row_base = t->rows;
-  G.per_block = ceil_div(n, blocks);
-  G.nslots = t->local_slots;
-  G.nbuckets = t->nbuckets;
-  G.bucket_shift = t->bucket_shift;
-  G.rec_words = t->rec_words;
-  {
-    ProfScope ps(c, "groupby_local");
-    FPDB_GB_DISPATCH(launch_local, c, t, blocks, k, vals, G);
-    FPDB_HIP(hipGetLastError());
-    hipLaunchKernelGGL(gb_scan_kernel, dim3(1), dim3(1024), 0, c->stream, t->hist, blocks, t->nbuckets, t->ocount,
-                       t->base, t->nstart, t->newcount, t->totals);
-    FPDB_HIP(hipGetLastError());
+  // batches beyond kMaxBatchRows rows are consumed in pieces (rows are
+  // batch-relative 32-bit values in the main pass's LDS)
+  for (int64_t done = 0; done < n; done += kMaxBatchRows) {
+    const int64_t m = std::min<int64_t>(kMaxBatchRows, n - done);
+    GKeySpec kp = k;
+    ColSet vp = vals;
+    for (int i = 0; i < kp.n; ++i) kp.off[i] += done;
+    for (int a = 0; a < vp.n; ++a) vp.offset[a] += done;
+    FPDB_TRY(groupby_consume_batch(c, t, kp, vp, m));
   }
-  // the new pool's size decides its allocation
-  unsigned long long total = 0;
-  FPDB_HIP(hipMemcpyAsync(&total, t->totals, 8, hipMemcpyDeviceToHost, c->stream));
-  FPDB_HIP(hipStreamSynchronize(c->stream));
-  if ((int64_t)total > t->spare_cap) {
-    if (t->spare) FPDB_TRY(ctx_free(c, t->spare));
-    t->spare = nullptr;
-    t->spare_cap = 0;
-    int64_t want = std::max<int64_t>((int64_t)total, 1024);
-    FPDB_TRY(gb_alloc(t, (uint64_t)want * t->rec_words * 8, reinterpret_cast<void**>(&t->spare)));
-    t->spare_cap = want;
-  }
-  {
-    ProfScope ps(c, "groupby_scatter");
-    FPDB_GB_DISPATCH(launch_scatter, c, t, blocks, k, vals, G);
-    FPDB_HIP(hipGetLastError());
-  }
-  {
-    ProfScope ps(c, "groupby_compact");
-    FPDB_GB_DISPATCH(launch_compact, c, t);
-    FPDB_HIP(hipGetLastError());
-  }
-  std::swap(t->pool, t->spare);
-  std::swap(t->pool_cap, t->spare_cap);
-  t->rows += n;
   return FPDB_OK;
 }
 
