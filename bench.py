@@ -303,11 +303,12 @@ def bench_groupby(ctx, steps):
         ctx.synchronize()
         if i >= 1:
             wall.append((time.perf_counter() - t0) * 1e3)
-    # consume = sample + count pass + offsets, main pass, head merge + compact pass
+    # consume = sample, count pass + offsets, main pass, head merge + compact pass
+    sample_ms, _ = kernel_ms(ctx, "groupby_sample")
     count_ms, _ = kernel_ms(ctx, "groupby_count")
     main_ms, _ = kernel_ms(ctx, "groupby_main")
     compact_ms, _ = kernel_ms(ctx, "groupby_compact")
-    cons_ms = count_ms + main_ms + compact_ms
+    cons_ms = sample_ms + count_ms + main_ms + compact_ms
     t0 = time.perf_counter()
     keys, out = g.finalize()
     finalize_ms = (time.perf_counter() - t0) * 1e3
@@ -333,7 +334,7 @@ def bench_groupby(ctx, steps):
                                        f"over all {m} rows, {cpu_s:.2f} s"},
             "workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
                         "COUNT, SUM int64 x2, SUM float64", "groups": int(len(k)),
-            "consume_kernels_ms": cons_ms, "count_ms": count_ms, "main_ms": main_ms,
+            "consume_kernels_ms": cons_ms, "sample_ms": sample_ms, "count_ms": count_ms, "main_ms": main_ms,
             "compact_ms": compact_ms, "consume_wall_ms": statistics.median(wall),
             "grows_per_s": n / (cons_ms * 1e-3) / 1e9, "finalize_ms": finalize_ms,
             "hbm_frac": alg / (cons_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -724,7 +725,8 @@ def main():
     if args.only in (None, "groupby"):
         res["groupby"] = bench_groupby(ctx, max(2, args.steps // 5))
         gb = res["groupby"]
-        log(f"groupby: consume {gb['consume_kernels_ms']:.4f} ms (count {gb['count_ms']:.3f}, main {gb['main_ms']:.3f}, "
+        log(f"groupby: consume {gb['consume_kernels_ms']:.4f} ms (sample {gb['sample_ms']:.3f}, count {gb['count_ms']:.3f}, "
+            f"main {gb['main_ms']:.3f}, "
             f"compact {gb['compact_ms']:.3f}; wall {gb['consume_wall_ms']:.3f})")
     if args.only in (None, "shuffle"):
         res["shuffle"] = bench_shuffle(ctx, args.steps)

@@ -376,12 +376,22 @@ __device__ __forceinline__ void load_rec(const uint64_t* rec, uint64_t (&w)[RW])
 // the main pass re-deriving anything from the input.  The input is read once
 // in full (main) plus its key column once more (count).
 
-constexpr int kSampleBlocks = 64;
+constexpr int kSampleBlocks = 32;
 constexpr int kSampleBlockRows = 1024;
 constexpr int kSampleSlots = 8192;           // the sampler's LDS count table
-constexpr int kHeadIndexSlots = 8192;        // head index: head id per slot
+constexpr int kHeadIndexSlots = 16384;       // head index: 16-bit head id per slot
 constexpr uint32_t kNoHead = 0xFFFFFFFFu;
+constexpr uint32_t kNoHead16 = 0xFFFFu;
 constexpr int kCountBins = 512;
+constexpr int kSampleProbes = 8;
+constexpr int64_t kDealRows = 4096;          // rows dealt to a workgroup at a time (count and main passes)
+
+// Hash of a group key for its bucket (top bits) and its head-index home
+// (bits 40..52): one multiply by 2^64 / golden ratio after folding the high
+// half onto the low half, so every key bit reaches the bits used.
+__host__ __device__ __forceinline__ uint64_t gb_hash(uint64_t k) {
+  return (k ^ (k >> 31)) * 0x9E3779B97F4A7C15ULL;
+}
 
 struct GbGeom {
   int64_t n;            // rows in this batch
@@ -426,57 +436,77 @@ __device__ __forceinline__ void acc_apply_rows(const LdsTable& T, const AccSpec&
   }
 }
 
-__device__ __forceinline__ uint32_t head_home(uint64_t h) { return (uint32_t)h & (kHeadIndexSlots - 1); }
+// Head index: kHeadIndexSlots 16-bit head ids (0xFFFF = empty).  A head key
+// sits in one of two slots, home1 or home2 of its hash, so a lookup reads two
+// slots and (for occupied ones) two keys, all independent: no probe loop.  The
+// sampler builds the index once per batch (a key whose two slots are both
+// taken is dropped from the head set) and every workgroup of the count and
+// main passes copies the same index into its LDS, so both passes classify
+// every row identically.
+__device__ __forceinline__ uint32_t head_home1(uint64_t h) { return (uint32_t)(h >> 40) & (kHeadIndexSlots - 1); }
+__device__ __forceinline__ uint32_t head_home2(uint64_t h) { return (uint32_t)(h >> 20) & (kHeadIndexSlots - 1); }
 
-// Head id of key k (hash h), or -1: the index holds at most a third as many
-// ids as slots, so every walk ends at an empty slot within a few probes.
-__device__ __forceinline__ int head_lookup(const uint32_t* s_idx, const uint64_t* s_hkeys, uint64_t k, uint64_t h) {
-  uint32_t s = head_home(h);
-  for (int p = 0; p < kHeadIndexSlots; ++p) {
-    const uint32_t id = s_idx[s];
-    if (id == kNoHead) return -1;
-    if (s_hkeys[id] == k) return (int)id;
-    s = (s + 1) & (kHeadIndexSlots - 1);
-  }
+// Head id of key k (hash h), or -1.
+__device__ __forceinline__ int head_find(const uint16_t* s_idx, const uint64_t* s_hkeys, uint64_t k, uint64_t h) {
+  const uint32_t a = s_idx[head_home1(h)];
+  const uint32_t b = s_idx[head_home2(h)];
+  const uint64_t ka = s_hkeys[a == kNoHead16 ? 0 : a];
+  const uint64_t kb = s_hkeys[b == kNoHead16 ? 0 : b];
+  if (a != kNoHead16 && ka == k) return (int)a;
+  if (b != kNoHead16 && kb == k) return (int)b;
   return -1;
 }
 
-// Every workgroup builds the head index in its LDS from the head key list.
-__device__ __forceinline__ void head_index_build(uint32_t* s_idx, uint64_t* s_hkeys, const uint64_t* __restrict__ hkeys,
-                                                 int nhead) {
-  for (int s = threadIdx.x; s < kHeadIndexSlots; s += blockDim.x) s_idx[s] = kNoHead;
+// Copies the batch's head index and head keys into LDS (the whole workgroup).
+__device__ __forceinline__ void head_index_load(uint16_t* s_idx, uint64_t* s_hkeys, const uint16_t* __restrict__ g_idx,
+                                                const uint64_t* __restrict__ hkeys, int nhead) {
+  const uint4* src = reinterpret_cast<const uint4*>(g_idx);
+  uint4* dst = reinterpret_cast<uint4*>(s_idx);
+  for (int i = threadIdx.x; i < kHeadIndexSlots * 2 / 16; i += blockDim.x) dst[i] = src[i];
   for (int h = threadIdx.x; h < nhead; h += blockDim.x) s_hkeys[h] = hkeys[h];
-  __syncthreads();
-  for (int h = threadIdx.x; h < nhead; h += blockDim.x) {
-    uint32_t s = head_home(fmix64(s_hkeys[h]));
-    while (atomicCAS(&s_idx[s], kNoHead, (uint32_t)h) != kNoHead) s = (s + 1) & (kHeadIndexSlots - 1);
-  }
+  if (threadIdx.x == 0 && nhead == 0) s_hkeys[0] = 0;   // head_find reads slot 0 for empty slots
   __syncthreads();
 }
 
+__device__ __forceinline__ uint32_t sample_home(uint64_t h) { return (uint32_t)(h >> 40) & (kSampleSlots - 1); }
+
 // Sample: keys of up to kSampleBlocks x kSampleBlockRows rows (all rows of a
 // smaller batch; otherwise kSampleBlocks evenly spaced runs of consecutive
-// rows) counted in an LDS table; the max_heads keys with the highest counts
-// (ties at the threshold count in any order) become the head keys.  Per head
-// key: its bucket's head count is incremented and its rank there recorded.
-// One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G, uint64_t* __restrict__ head_keys,
+// rows) counted in an LDS table; up to max_heads keys with the highest counts
+// (ties at the threshold count in any order) are placed in the head index.
+// Outputs: the index, the head keys (ids 0 .. nhead - 1), per head key its
+// rank among its bucket's head keys, per bucket its head count.  One
+// workgroup of 1024 threads.  LDS: the count table's keys (later the 32-bit
+// index under construction), its counts (later the head ids' new numbers),
+// the count histogram, the bucket counts and the selected keys.
+__global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G, uint16_t* __restrict__ g_idx,
+                                                         uint64_t* __restrict__ head_keys,
                                                          uint32_t* __restrict__ head_rank,
                                                          unsigned int* __restrict__ head_count, int* nhead_out) {
   extern __shared__ uint64_t lds[];
   uint64_t* s_keys = lds;                                                     // kSampleSlots + 1
   uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kSampleSlots + 1);    // kSampleSlots + 1
   uint32_t* s_hist = s_cnt + kSampleSlots + 1;                               // kCountBins
-  __shared__ int s_thresh, s_take, s_n, s_tied;
+  uint32_t* s_bcount = s_hist + kCountBins;                                  // nbuckets
+  // max_heads selected keys, 8-byte aligned after the bucket counts
+  const size_t sel_off = ((reinterpret_cast<uintptr_t>(s_bcount + G.nbuckets) - reinterpret_cast<uintptr_t>(lds)) + 7) & ~(size_t)7;
+  uint64_t* s_sel = reinterpret_cast<uint64_t*>(reinterpret_cast<uint8_t*>(lds) + sel_off);
+  uint32_t* s_idx32 = reinterpret_cast<uint32_t*>(s_keys);                  // kHeadIndexSlots, after selection
+  uint32_t* s_newid = s_cnt;                                                 // max_heads, after selection
+  __shared__ int s_thresh, s_take, s_n, s_tied, s_kept;
+  const int lane = threadIdx.x & 63;
   for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x) {
     s_keys[s] = kEmptyKey;
     s_cnt[s] = 0;
   }
   for (int b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
-  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) head_count[b] = 0;
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) s_bcount[b] = 0;
   if (threadIdx.x == 0) {
     s_n = 0;
     s_tied = 0;
+    s_kept = 0;
+    s_thresh = 1;
+    s_take = 0;
   }
   __syncthreads();
   const int64_t n = G.n;
@@ -484,13 +514,19 @@ __global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G
   constexpr int U = 8;
   bool dummy = false;
   for (int j0 = 0; j0 < kSampleBlocks; j0 += U) {
+    // keys enter the table only from the first half of the sample (about
+    // 4,600 distinct keys on a Zipf(1.1) column over 100,000 values); later
+    // samples count the keys already there (a key not seen early is rare
+    // enough to rank below the head keys), so the table stays under 60% full
+    // and walks stay short
+    const bool insert = j0 < kSampleBlocks / 2;
     int64_t rc[U];
     bool live[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t j = j0 + u;
-      int64_t r = all ? j * kSampleBlockRows + threadIdx.x
-                      : (n - kSampleBlockRows) * j / (kSampleBlocks - 1) + threadIdx.x;
+      const int64_t r = all ? j * kSampleBlockRows + threadIdx.x
+                            : (n - kSampleBlockRows) * j / (kSampleBlocks - 1) + threadIdx.x;
       live[u] = r < n;
       rc[u] = live[u] ? r : n - 1;
     }
@@ -498,107 +534,189 @@ __global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G
     group_keys_n<U>(keys, rc, kk, &dummy);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (!live[u]) continue;
       const uint64_t k = kk[u];
       int slot = -1;
-      if (k == kEmptyKey) {
-        slot = kSampleSlots;
-      } else {
-        uint32_t s = (uint32_t)fmix64(k) & (kSampleSlots - 1);
-        for (int p = 0; p < 16; ++p) {
-          uint64_t cur = s_keys[s];
-          if (cur == k) { slot = (int)s; break; }
-          if (cur == kEmptyKey) {
-            unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[s]), kEmptyKey, k);
-            if (prev == kEmptyKey || prev == k) { slot = (int)s; break; }
+      if (live[u]) {
+        if (k == kEmptyKey) {
+          slot = kSampleSlots;
+        } else {
+          uint32_t s = sample_home(gb_hash(k));
+          for (int p = 0; p < kSampleProbes; ++p) {
+            const uint64_t cur = s_keys[s];
+            if (cur == k) { slot = (int)s; break; }
+            if (cur == kEmptyKey) {
+              if (!insert) break;
+              const unsigned long long prev =
+                  atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[s]), kEmptyKey, k);
+              if (prev == kEmptyKey || prev == k) { slot = (int)s; break; }
+            }
+            s = (s + 1) & (kSampleSlots - 1);
           }
-          s = (s + 1) & (kSampleSlots - 1);
         }
       }
-      if (slot >= 0) atomicAdd(&s_cnt[slot], 1u);
+      // skewed keys repeat within a wave: twice, the lanes sharing the slot of
+      // the first lane still to count add with one atomic
+      bool todo = slot >= 0;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const uint64_t act = __ballot(todo);
+        if (!act) break;
+        const int leader = __builtin_ctzll(act);
+        const int ls = __shfl(slot, leader);
+        const bool same = todo && slot == ls;
+        const uint64_t grp = __ballot(same);
+        if (same) {
+          if (lane == leader) atomicAdd(&s_cnt[ls], (uint32_t)__popcll(grp));
+          todo = false;
+        }
+      }
+      if (todo) atomicAdd(&s_cnt[slot], 1u);
     }
   }
   __syncthreads();
   for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x)
     if (s_cnt[s]) atomicAdd(&s_hist[min(s_cnt[s], (uint32_t)(kCountBins - 1))], 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    // threshold bin T: every key in a bin above T is taken, s_take of bin T
-    int acc = 0, T = 1, take = 0;
-    for (int b = kCountBins - 1; b >= 1; --b) {
-      const int hb = (int)s_hist[b];
-      if (acc + hb >= G.max_heads) {
-        T = b;
-        take = G.max_heads - acc;
-        break;
-      }
-      acc += hb;
-      if (b == 1) {
-        T = 1;
-        take = hb;
-      }
+  // threshold bin T = the highest bin with (keys in higher bins) + (keys in
+  // T) >= max_heads; every key above T is taken and max_heads - (keys above)
+  // of bin T.  Wave 0: lane l holds bins 8l .. 8l + 7.
+  if (threadIdx.x < 64) {
+    constexpr int PB = kCountBins / 64;
+    uint32_t hb[PB];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < PB; ++j) {
+      hb[j] = s_hist[lane * PB + j];
+      mine += hb[j];
     }
-    s_thresh = T;
-    s_take = take;
+    uint32_t incl = mine;   // keys in bins of lanes >= lane
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t t = __shfl_down(incl, off);
+      if (lane + off < 64) incl += t;
+    }
+    uint32_t above = incl - mine;
+    int found = -1, take = 0;
+#pragma unroll
+    for (int j = PB - 1; j >= 0; --j) {
+      const int bin = lane * PB + j;
+      if (found < 0 && bin >= 1 && above + hb[j] >= (uint32_t)G.max_heads) {
+        found = bin;
+        take = G.max_heads - (int)above;
+      }
+      above += hb[j];
+    }
+    const uint64_t with = __ballot(found >= 0);
+    if (with) {
+      const int top = 63 - __clzll((long long)with);
+      if (lane == top) {
+        s_thresh = found;
+        s_take = take;
+      }
+    } else if (lane == 0) {
+      s_thresh = 1;                  // fewer sampled keys than head slots: take them all
+      s_take = (int)hb[1];
+    }
   }
   __syncthreads();
   const uint32_t T = (uint32_t)s_thresh;
+  const int take_at_t = s_take;
   for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x) {
     const uint32_t c = s_cnt[s];
     if (!c) continue;
     const uint32_t bin = min(c, (uint32_t)(kCountBins - 1));
     bool take = bin > T;
-    if (bin == T) take = atomicAdd(&s_tied, 1) < s_take;
+    if (bin == T) take = atomicAdd(&s_tied, 1) < take_at_t;
     if (!take) continue;
-    const int id = atomicAdd(&s_n, 1);
-    const uint64_t k = s == kSampleSlots ? kEmptyKey : s_keys[s];
-    head_keys[id] = k;
-    head_rank[id] = atomicAdd(&head_count[bucket_of(fmix64(k), G.bucket_shift)], 1u);
+    s_sel[atomicAdd(&s_n, 1)] = s == kSampleSlots ? kEmptyKey : s_keys[s];
   }
   __syncthreads();
-  if (threadIdx.x == 0) *nhead_out = s_n;
+  // index under construction (32-bit entries over the count table's keys)
+  for (int s = threadIdx.x; s < kHeadIndexSlots; s += blockDim.x) s_idx32[s] = kNoHead;
+  __syncthreads();
+  const int nsel = s_n;
+  for (int j = threadIdx.x; j < nsel; j += blockDim.x) {
+    const uint64_t h = gb_hash(s_sel[j]);
+    bool placed = atomicCAS(&s_idx32[head_home1(h)], kNoHead, (uint32_t)j) == kNoHead;
+    if (!placed) placed = atomicCAS(&s_idx32[head_home2(h)], kNoHead, (uint32_t)j) == kNoHead;
+    s_newid[j] = placed ? (uint32_t)atomicAdd(&s_kept, 1) : kNoHead;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < nsel; j += blockDim.x) {
+    const uint32_t id = s_newid[j];
+    if (id == kNoHead) continue;
+    const uint64_t k = s_sel[j];
+    head_keys[id] = k;
+    head_rank[id] = atomicAdd(&s_bcount[bucket_of(gb_hash(k), G.bucket_shift)], 1u);
+  }
+  for (int s = threadIdx.x; s < kHeadIndexSlots; s += blockDim.x) {
+    const uint32_t j = s_idx32[s];
+    g_idx[s] = (uint16_t)(j == kNoHead ? kNoHead16 : s_newid[j]);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) head_count[b] = s_bcount[b];
+  if (threadIdx.x == 0) *nhead_out = s_kept;
 }
 
-// Rows are dealt to the count and main passes' workgroups in blocks of
-// ITEMS x 1024 consecutive rows, round-robin, so all workgroups stream through
-// neighbouring rows (HBM page locality) and both passes give every workgroup
-// the same rows.
-template <int ITEMS>
-__device__ __forceinline__ int64_t gb_block_rows() { return (int64_t)ITEMS * kGroupThreads; }
+// Rows are dealt to the count and main passes' workgroups in units of
+// kDealRows consecutive rows, round-robin (unit u to workgroup u % grid), so
+// all workgroups stream through neighbouring rows (HBM page locality) and both
+// passes give every workgroup the same rows.
+__device__ __forceinline__ int64_t gb_units(int64_t n) { return (n + kDealRows - 1) / kDealRows; }
 
 // Count pass: tail rows per bucket of this workgroup's rows, written to
-// hist[bucket][workgroup].
-template <int ITEMS>
+// hist[bucket][workgroup].  Each thread loads the keys of U units at once
+// (4 rows per unit).
+constexpr int kCountUnits = 2;
+template <int KW>
 __global__ __launch_bounds__(kGroupThreads) void gb_count_kernel(GKeySpec keys, GbGeom G,
+                                                                 const uint16_t* __restrict__ g_idx,
                                                                  const uint64_t* __restrict__ head_keys,
                                                                  const int* __restrict__ nhead_p,
                                                                  uint32_t* __restrict__ hist, int* key_range) {
   extern __shared__ uint64_t lds[];
-  uint32_t* s_idx = reinterpret_cast<uint32_t*>(lds);
-  uint64_t* s_hkeys = lds + kHeadIndexSlots / 2;
+  uint16_t* s_idx = reinterpret_cast<uint16_t*>(lds);
+  uint64_t* s_hkeys = lds + kHeadIndexSlots / 4;
   uint32_t* s_hist = reinterpret_cast<uint32_t*>(s_hkeys + G.max_heads);
   const int nhead = *nhead_p;
   for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) s_hist[b] = 0;
-  head_index_build(s_idx, s_hkeys, head_keys, nhead);
+  head_index_load(s_idx, s_hkeys, g_idx, head_keys, nhead);
   bool out_of_range = false;
-  const int64_t brows = gb_block_rows<ITEMS>();
-  for (int64_t b0 = (int64_t)blockIdx.x * brows; b0 < G.n; b0 += (int64_t)gridDim.x * brows) {
-    const int64_t end = b0 + brows < G.n ? b0 + brows : G.n;
-    const int64_t r0 = b0 + threadIdx.x;
-    if (r0 >= end) continue;
-    int64_t rc[ITEMS];
+  constexpr int PER_UNIT = (int)(kDealRows / kGroupThreads);
+  constexpr int CU = KW ? 2 * kCountUnits : kCountUnits;   // the direct key path has registers for more
+  constexpr int N = CU * PER_UNIT;
+  const int64_t units = gb_units(G.n);
+  for (int64_t u0 = blockIdx.x; u0 < units; u0 += (int64_t)gridDim.x * CU) {
+    int64_t rc[N];
+    bool live[N];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const int64_t r = r0 + (int64_t)i * kGroupThreads;
-      rc[i] = r < end ? r : end - 1;
+    for (int j = 0; j < CU; ++j) {
+#pragma unroll
+      for (int i = 0; i < PER_UNIT; ++i) {
+        const int64_t r = (u0 + (int64_t)j * gridDim.x) * kDealRows + (int64_t)i * kGroupThreads + threadIdx.x;
+        live[j * PER_UNIT + i] = r < G.n;
+        rc[j * PER_UNIT + i] = r < G.n ? r : G.n - 1;
+      }
     }
-    uint64_t kk[ITEMS];
-    group_keys_n<ITEMS>(keys, rc, kk, &out_of_range);
+    uint64_t kk[N];
+    if constexpr (KW == 8) {
+      const uint64_t* kp = reinterpret_cast<const uint64_t*>(keys.v[0]) + keys.off[0];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if (r0 + (int64_t)i * kGroupThreads >= end) continue;
-      const uint64_t h = fmix64(kk[i]);
-      if (head_lookup(s_idx, s_hkeys, kk[i], h) < 0) atomicAdd(&s_hist[bucket_of(h, G.bucket_shift)], 1u);
+      for (int i = 0; i < N; ++i) kk[i] = kp[rc[i]];
+    } else if constexpr (KW == 4) {
+      const uint32_t* kp = reinterpret_cast<const uint32_t*>(keys.v[0]) + keys.off[0];
+      const bool sgn = keys.sgn[0] != 0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const uint32_t x = kp[rc[i]];
+        kk[i] = sgn ? (uint64_t)(int64_t)(int32_t)x : (uint64_t)x;
+      }
+    } else {
+      group_keys_n<N>(keys, rc, kk, &out_of_range);
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const uint64_t h = gb_hash(kk[i]);
+      if (live[i] && head_find(s_idx, s_hkeys, kk[i], h) < 0) atomicAdd(&s_hist[bucket_of(h, G.bucket_shift)], 1u);
     }
   }
   if (out_of_range) *key_range = 1;
@@ -606,54 +724,64 @@ __global__ __launch_bounds__(kGroupThreads) void gb_count_kernel(GKeySpec keys, 
   for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) hist[(size_t)b * gridDim.x + blockIdx.x] = s_hist[b];
 }
 
-// Offsets (one workgroup): for bucket b, its range of the new pool starts at
-// nstart[b] and holds its old records (ocount[b], aggregated in place by the
-// compact pass), its head records (head_count[b]), then the tail rows of
-// workgroup 0, 1, ...; base[b][w] is where workgroup w writes its first
-// tail row of bucket b.  newcount[b] = head + tail records of b; totals[0] =
-// records of the new pool; totals[1] (groups) reset for the compact pass.
-__global__ __launch_bounds__(1024) void gb_offsets_kernel(const uint32_t* __restrict__ hist, int blocks, int nbuckets,
-                                                          const int64_t* __restrict__ ocount,
-                                                          const unsigned int* __restrict__ head_count,
-                                                          uint64_t* __restrict__ base, int64_t* __restrict__ nstart,
-                                                          int64_t* __restrict__ newcount,
-                                                          unsigned long long* totals) {
-  __shared__ uint64_t s_tail[kMaxBuckets];
+// Offsets, first kernel (a wave per bucket): base[b][w] = tail rows of bucket
+// b in workgroups before w; tail_total[b] = all of them.
+__global__ __launch_bounds__(1024) void gb_bucket_scan_kernel(const uint32_t* __restrict__ hist, int blocks,
+                                                              int nbuckets, uint64_t* __restrict__ base,
+                                                              uint64_t* __restrict__ tail_total) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= nbuckets) return;
+  const uint32_t* row = hist + (size_t)b * blocks;
+  constexpr int kMaxPer = 16;               // up to 1024 workgroups
+  const int per = (blocks + 63) / 64;       // consecutive counts per lane
+  uint32_t v[kMaxPer];
+  uint64_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxPer; ++j) {
+    const int w = lane * per + j;
+    v[j] = (j < per && w < blocks) ? row[w] : 0u;
+    mine += v[j];
+  }
+  uint64_t incl = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  uint64_t run = incl - mine;
+#pragma unroll
+  for (int j = 0; j < kMaxPer; ++j) {
+    const int w = lane * per + j;
+    if (j < per && w < blocks) base[(size_t)b * blocks + w] = run;
+    run += v[j];
+  }
+  if (lane == 63) tail_total[b] = incl;
+}
+
+// Offsets, second kernel (one workgroup): for bucket b, its range of the new
+// pool starts at nstart[b] and holds its old records (ocount[b], aggregated in
+// place by the compact pass), its head records (head_count[b]), then the tail
+// rows of workgroup 0, 1, ... from tail_start[b] on.  newcount[b] = head +
+// tail records of b; totals[0] = records of the new pool; totals[1] (groups)
+// reset for the compact pass.
+__global__ __launch_bounds__(1024) void gb_bucket_offsets_kernel(int nbuckets, const int64_t* __restrict__ ocount,
+                                                                 const unsigned int* __restrict__ head_count,
+                                                                 const uint64_t* __restrict__ tail_total,
+                                                                 int64_t* __restrict__ nstart,
+                                                                 int64_t* __restrict__ newcount,
+                                                                 uint64_t* __restrict__ tail_start,
+                                                                 unsigned long long* totals) {
   __shared__ uint64_t s_w[16];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int nwaves = blockDim.x >> 6;
-  const int per = (blocks + 63) / 64;   // consecutive counts per lane
-  // per bucket: exclusive scan of its workgroups' tail counts (relative)
-  for (int b = wave; b < nbuckets; b += nwaves) {
-    const uint32_t* row = hist + (size_t)b * blocks;
-    uint64_t mine = 0;
-    for (int j = 0; j < per; ++j) {
-      const int w = lane * per + j;
-      if (w < blocks) mine += row[w];
-    }
-    uint64_t incl = mine;
-    for (int off = 1; off < 64; off <<= 1) {
-      const uint64_t t = __shfl_up(incl, off);
-      if (lane >= off) incl += t;
-    }
-    uint64_t run = incl - mine;
-    for (int j = 0; j < per; ++j) {
-      const int w = lane * per + j;
-      if (w < blocks) {
-        base[(size_t)b * blocks + w] = run;
-        run += row[w];
-      }
-    }
-    if (lane == 63) s_tail[b] = incl;
-  }
-  __syncthreads();
-  // exclusive scan over buckets of old + head + tail records
-  const int bper = (nbuckets + (int)blockDim.x - 1) / (int)blockDim.x;
+  constexpr int kPer = kMaxBuckets / 1024;
+  uint64_t rec[kPer];
   uint64_t mine = 0;
-  for (int j = 0; j < bper; ++j) {
-    const int b = threadIdx.x * bper + j;
-    if (b < nbuckets) mine += (uint64_t)ocount[b] + head_count[b] + s_tail[b];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int b = threadIdx.x * kPer + j;
+    rec[j] = b < nbuckets ? (uint64_t)ocount[b] + head_count[b] + tail_total[b] : 0;
+    mine += rec[j];
   }
   uint64_t incl = mine;
   for (int off = 1; off < 64; off <<= 1) {
@@ -664,42 +792,83 @@ __global__ __launch_bounds__(1024) void gb_offsets_kernel(const uint32_t* __rest
   __syncthreads();
   uint64_t run = incl - mine;
   for (int w = 0; w < wave; ++w) run += s_w[w];
-  for (int j = 0; j < bper; ++j) {
-    const int b = threadIdx.x * bper + j;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int b = threadIdx.x * kPer + j;
     if (b < nbuckets) {
       nstart[b] = (int64_t)run;
-      newcount[b] = (int64_t)(head_count[b] + s_tail[b]);
-      const uint64_t rec = (uint64_t)ocount[b] + head_count[b] + s_tail[b];
-      s_tail[b] = run + (uint64_t)ocount[b] + head_count[b];   // where the bucket's tail rows start
-      run += rec;
+      newcount[b] = (int64_t)(head_count[b] + tail_total[b]);
+      tail_start[b] = run + (uint64_t)ocount[b] + head_count[b];
     }
+    run += rec[j];
   }
   if (threadIdx.x == blockDim.x - 1) {
     totals[0] = (unsigned long long)run;
     totals[1] = 0;
-  }
-  __syncthreads();
-  for (int b = wave; b < nbuckets; b += nwaves) {
-    const uint64_t start = s_tail[b];
-    for (int w = lane; w < blocks; w += 64) base[(size_t)b * blocks + w] += start;
   }
 }
 
 // Main pass: head rows aggregated in LDS (slot = head id), tail rows written
 // as records (key, row, operands) at their positions; then the head partials
 // (first row, accumulators) to hpart[workgroup][1 + nacc][max_heads].
-template <int NACC>
+// Key and operand loads of ITEMS rows.  KW = 4 or 8: the key is one unpacked
+// column of that width (the value itself, sign-extended when signed); V8:
+// every aggregate input is an 8-byte column without nulls (its raw bits are
+// the operand, COUNT operands are 1).  Otherwise the generic column walks.
+template <int ITEMS, int NA, int KW, bool V8>
+__device__ __forceinline__ void gb_load_rows(const GKeySpec& keys, const ColSet& vals, const AccSpec& A, int nacc,
+                                             const int64_t (&rc)[ITEMS], uint64_t (&kk)[ITEMS],
+                                             uint64_t (&vv)[NA][ITEMS], bool (&okk)[NA][ITEMS], bool* oor) {
+  if constexpr (KW == 8) {
+    const uint64_t* kp = reinterpret_cast<const uint64_t*>(keys.v[0]) + keys.off[0];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) kk[i] = kp[rc[i]];
+  } else if constexpr (KW == 4) {
+    const uint32_t* kp = reinterpret_cast<const uint32_t*>(keys.v[0]) + keys.off[0];
+    const bool sgn = keys.sgn[0] != 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t x = kp[rc[i]];
+      kk[i] = sgn ? (uint64_t)(int64_t)(int32_t)x : (uint64_t)x;
+    }
+  } else {
+    group_keys_n<ITEMS>(keys, rc, kk, oor);
+  }
+  if constexpr (V8) {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      if (a >= nacc) break;
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) okk[a][i] = true;
+      if (A.op[a] == kAccCountAll || A.op[a] == kAccCountValid) {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) vv[a][i] = 1;
+      } else {
+        const int in = A.input[a];
+        const uint64_t* vp = reinterpret_cast<const uint64_t*>(vals.values[in]) + vals.offset[in];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) vv[a][i] = vp[rc[i]];
+      }
+    }
+  } else {
+    acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
+  }
+}
+
+template <int NACC, int KW, bool V8>
 __global__ __launch_bounds__(kGroupThreads) void gb_main_kernel(GKeySpec keys, ColSet vals, AccSpec A, GbGeom G,
+                                                                const uint16_t* __restrict__ g_idx,
                                                                 const uint64_t* __restrict__ head_keys,
                                                                 const int* __restrict__ nhead_p,
                                                                 const uint64_t* __restrict__ base,
+                                                                const uint64_t* __restrict__ tail_start,
                                                                 uint64_t* __restrict__ pool,
                                                                 uint64_t* __restrict__ hpart) {
   extern __shared__ uint64_t lds[];
   const int nacc = NACC > 0 ? NACC : A.n;
   const int H = G.max_heads;
-  uint32_t* s_idx = reinterpret_cast<uint32_t*>(lds);
-  uint64_t* s_hkeys = lds + kHeadIndexSlots / 2;
+  uint16_t* s_idx = reinterpret_cast<uint16_t*>(lds);
+  uint64_t* s_hkeys = lds + kHeadIndexSlots / 4;
   uint64_t* s_acc = s_hkeys + H;                                                     // nacc x H
   unsigned long long* s_pos = reinterpret_cast<unsigned long long*>(s_acc + (size_t)nacc * H);   // nbuckets
   uint32_t* s_first = reinterpret_cast<uint32_t*>(s_pos + G.nbuckets);              // H, batch-relative rows
@@ -708,8 +877,9 @@ __global__ __launch_bounds__(kGroupThreads) void gb_main_kernel(GKeySpec keys, C
     s_first[h] = 0xFFFFFFFFu;
     for (int a = 0; a < nacc; ++a) s_acc[(size_t)a * H + h] = acc_identity(A.op[a]);
   }
-  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) s_pos[b] = base[(size_t)b * gridDim.x + blockIdx.x];
-  head_index_build(s_idx, s_hkeys, head_keys, nhead);
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x)
+    s_pos[b] = tail_start[b] + base[(size_t)b * gridDim.x + blockIdx.x];
+  head_index_load(s_idx, s_hkeys, g_idx, head_keys, nhead);
   LdsTable T;
   T.keys = s_hkeys;
   T.first = nullptr;
@@ -719,51 +889,53 @@ __global__ __launch_bounds__(kGroupThreads) void gb_main_kernel(GKeySpec keys, C
   bool dummy = false;
   constexpr int ITEMS = NACC > 0 ? 4 : 1;
   constexpr int NA = NACC > 0 ? NACC : kMaxAggs;
-  const int64_t brows = gb_block_rows<ITEMS>();
-  for (int64_t b0 = (int64_t)blockIdx.x * brows; b0 < G.n; b0 += (int64_t)gridDim.x * brows) {
-    const int64_t end = b0 + brows < G.n ? b0 + brows : G.n;
-    const int64_t r0 = b0 + threadIdx.x;
-    if (r0 >= end) continue;
-    int64_t rc[ITEMS];
+  const int64_t units = gb_units(G.n);
+  for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+    const int64_t ub = u * kDealRows;
+    const int64_t end = ub + kDealRows < G.n ? ub + kDealRows : G.n;
+    for (int64_t s0 = ub; s0 < end; s0 += (int64_t)ITEMS * kGroupThreads) {
+      const int64_t r0 = s0 + threadIdx.x;
+      if (r0 >= end) continue;
+      int64_t rc[ITEMS];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const int64_t r = r0 + (int64_t)i * kGroupThreads;
-      rc[i] = r < end ? r : end - 1;
-    }
-    uint64_t kk[ITEMS];
-    uint64_t vv[NA][ITEMS];
-    bool okk[NA][ITEMS];
-    group_keys_n<ITEMS>(keys, rc, kk, &dummy);
-    acc_operands_n<ITEMS, NA>(vals, A, nacc, rc, vv, okk);
-    int slot[ITEMS];
-    uint64_t hh[ITEMS];
+      for (int i = 0; i < ITEMS; ++i) {
+        const int64_t r = r0 + (int64_t)i * kGroupThreads;
+        rc[i] = r < end ? r : end - 1;
+      }
+      uint64_t kk[ITEMS];
+      uint64_t vv[NA][ITEMS];
+      bool okk[NA][ITEMS];
+      gb_load_rows<ITEMS, NA, KW, V8>(keys, vals, A, nacc, rc, kk, vv, okk, &dummy);
+      int slot[ITEMS];
+      uint64_t hh[ITEMS];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      hh[i] = fmix64(kk[i]);
-      slot[i] = r0 + (int64_t)i * kGroupThreads < end ? head_lookup(s_idx, s_hkeys, kk[i], hh[i]) : -2;
-    }
-    acc_apply_rows<ITEMS, NA>(T, A, nacc, slot, vv, okk);
+      for (int i = 0; i < ITEMS; ++i) {
+        hh[i] = gb_hash(kk[i]);
+        slot[i] = r0 + (int64_t)i * kGroupThreads < end ? head_find(s_idx, s_hkeys, kk[i], hh[i]) : -2;
+      }
+      acc_apply_rows<ITEMS, NA>(T, A, nacc, slot, vv, okk);
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t rel = (uint32_t)(r0 + (int64_t)i * kGroupThreads);
-      if (slot[i] >= 0) {
-        if (rel < s_first[slot[i]]) atomicMin(&s_first[slot[i]], rel);
-      } else if (slot[i] == -1) {
-        const uint64_t at = atomicAdd(&s_pos[bucket_of(hh[i], G.bucket_shift)], 1ULL);
-        uint64_t* rec = pool + at * G.rec_words;
-        if constexpr (NACC > 0) {
-          uint64_t w[2 + NACC];
-          w[0] = kk[i];
-          w[1] = (uint64_t)(G.row_base + rel);
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint32_t rel = (uint32_t)(r0 + (int64_t)i * kGroupThreads);
+        if (slot[i] >= 0) {
+          if (rel < s_first[slot[i]]) atomicMin(&s_first[slot[i]], rel);
+        } else if (slot[i] == -1) {
+          const uint64_t at = atomicAdd(&s_pos[bucket_of(hh[i], G.bucket_shift)], 1ULL);
+          uint64_t* rec = pool + at * G.rec_words;
+          if constexpr (NACC > 0) {
+            uint64_t w[2 + NACC];
+            w[0] = kk[i];
+            w[1] = (uint64_t)(G.row_base + rel);
 #pragma unroll
-          for (int a = 0; a < NACC; ++a) w[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
-          store_rec<2 + NACC>(rec, w);
-        } else {
-          rec[0] = kk[i];
-          rec[1] = (uint64_t)(G.row_base + rel);
+            for (int a = 0; a < NACC; ++a) w[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+            store_rec<2 + NACC>(rec, w);
+          } else {
+            rec[0] = kk[i];
+            rec[1] = (uint64_t)(G.row_base + rel);
 #pragma unroll
-          for (int a = 0; a < NA; ++a)
-            if (a < nacc) rec[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+            for (int a = 0; a < NA; ++a)
+              if (a < nacc) rec[2 + a] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+          }
         }
       }
     }
@@ -795,51 +967,61 @@ __device__ __forceinline__ uint64_t acc_combine(int32_t op, uint64_t x, uint64_t
   }
 }
 
-// Merge: 64 head keys per workgroup of 4 waves; wave q adds the partials of
-// workgroups [q * blocks / 4, (q + 1) * blocks / 4) in order, then the four
-// sums are added in wave order: a fixed order, so head keys' FP64 sums do
-// not depend on scheduling across workgroups.  Each head key's record goes to
-// nstart[b] + ocount[b] + its rank among bucket b's head keys.
-constexpr int kMergeWaves = 4;
+// Merge: 64 head keys per workgroup of 16 waves; wave q adds the partials of
+// workgroups [q * blocks / 16, (q + 1) * blocks / 16) in order (the field's
+// loads of four workgroups in flight at a time), then wave 0 adds the 16
+// sums in wave order: a fixed order, so head keys' FP64 sums do not depend on
+// scheduling across workgroups.  Each head key's record goes to nstart[b] +
+// ocount[b] + its rank among bucket b's head keys.  Dynamic LDS:
+// kMergeWaves x (1 + nacc) x 64 words.
+constexpr int kMergeWaves = 16;
 __global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
     AccSpec A, GbGeom G, int blocks, const uint64_t* __restrict__ head_keys, const uint32_t* __restrict__ head_rank,
     const int* __restrict__ nhead_p, const uint64_t* __restrict__ hpart, const int64_t* __restrict__ nstart,
     const int64_t* __restrict__ ocount, uint64_t* __restrict__ pool) {
-  __shared__ uint64_t s_part[kMergeWaves][1 + kMaxAggs][64];
+  extern __shared__ uint64_t s_part[];   // [wave][field][lane]
   const int lane = threadIdx.x & 63;
   const int q = threadIdx.x >> 6;
   const int nacc = A.n;
+  const int nf = 1 + nacc;
   const int H = G.max_heads;
   const int h = blockIdx.x * 64 + lane;
   const int nhead = *nhead_p;
   const bool live = h < nhead;
+  const int hc = live ? h : 0;
   const int w0 = (int)((int64_t)blocks * q / kMergeWaves), w1 = (int)((int64_t)blocks * (q + 1) / kMergeWaves);
-  uint64_t first = (uint64_t)LLONG_MAX;
-  uint64_t acc[kMaxAggs];
-  for (int a = 0; a < nacc; ++a) acc[a] = acc_identity(A.op[a]);
-  if (live) {
-    for (int w = w0; w < w1; ++w) {
-      const uint64_t* p = hpart + (size_t)w * (1 + nacc) * H + h;
-      const uint64_t f = p[0];
-      first = (long long)f < (long long)first ? f : first;
-      for (int a = 0; a < nacc; ++a) acc[a] = acc_combine(A.op[a], acc[a], p[(size_t)(1 + a) * H]);
+  for (int f = 0; f < nf; ++f) {
+    const int32_t op = f == 0 ? kAccMinI : A.op[f - 1];
+    uint64_t x = f == 0 ? (uint64_t)LLONG_MAX : acc_identity(op);
+    const uint64_t* p = hpart + (size_t)f * H + hc;
+    int w = w0;
+    for (; w + 4 <= w1; w += 4) {
+      uint64_t v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = p[(size_t)(w + j) * nf * H];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) x = acc_combine(op, x, v[j]);
     }
+    for (; w < w1; ++w) x = acc_combine(op, x, p[(size_t)w * nf * H]);
+    s_part[((size_t)q * nf + f) * 64 + lane] = x;
   }
-  s_part[q][0][lane] = first;
-  for (int a = 0; a < nacc; ++a) s_part[q][1 + a][lane] = acc[a];
   __syncthreads();
   if (q != 0 || !live) return;
+  uint64_t first = s_part[lane];
   for (int r = 1; r < kMergeWaves; ++r) {
-    const uint64_t f = s_part[r][0][lane];
+    const uint64_t f = s_part[((size_t)r * nf) * 64 + lane];
     first = (long long)f < (long long)first ? f : first;
-    for (int a = 0; a < nacc; ++a) acc[a] = acc_combine(A.op[a], acc[a], s_part[r][1 + a][lane]);
   }
   const uint64_t k = head_keys[h];
-  const uint32_t b = bucket_of(fmix64(k), G.bucket_shift);
+  const uint32_t b = bucket_of(gb_hash(k), G.bucket_shift);
   uint64_t* rec = pool + (size_t)(nstart[b] + ocount[b] + head_rank[h]) * G.rec_words;
   rec[0] = k;
   rec[1] = first;
-  for (int a = 0; a < nacc; ++a) rec[2 + a] = acc[a];
+  for (int a = 0; a < nacc; ++a) {
+    uint64_t x = s_part[(1 + a) * 64 + lane];
+    for (int r = 1; r < kMergeWaves; ++r) x = acc_combine(A.op[a], x, s_part[((size_t)r * nf + 1 + a) * 64 + lane]);
+    rec[2 + a] = x;
+  }
 }
 
 // Compact pass, one workgroup per bucket: old records (previous batches, one
@@ -868,31 +1050,46 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, ui
   const int64_t nold = ocount[p], nnew = newcount[p];
   const int64_t start = nstart[p];
   const uint64_t* news = pool + (size_t)(start + nold) * rec_words;
-  for (int64_t i = threadIdx.x; i < nold + nnew; i += blockDim.x) {
-    const uint64_t* rec = i < nold ? oldpool + (size_t)(ostart[p] + i) * rec_words
-                                   : news + (size_t)(i - nold) * rec_words;
-    constexpr int RW = NACC > 0 ? 2 + NACC : 2 + kMaxAggs;
-    uint64_t w[RW];
-    if constexpr (NACC > 0) {
-      load_rec<RW>(rec, w);
-    } else {
-      for (int j = 0; j < rec_words; ++j) w[j] = rec[j];
-    }
-    uint64_t k = w[0];
-    int s = lds_find_or_insert(T, k, fmix64(k), nslots);
-    if (s < 0) {
-      s_over = 1;
-      continue;
-    }
+  constexpr int RW = NACC > 0 ? 2 + NACC : 2 + kMaxAggs;
+  // R records per thread per trip, all loaded before any is aggregated
+  constexpr int R = NACC > 0 ? 4 : 1;
+  const int64_t total = nold + nnew;
+  for (int64_t t0 = 0; t0 < total; t0 += (int64_t)R * blockDim.x) {
+    const int64_t i0 = t0 + threadIdx.x;
+    uint64_t w[R][RW];
 #pragma unroll
-    for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-      if (a < nacc) {
-        uint64_t v = w[2 + a];
-        if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
+    for (int j = 0; j < R; ++j) {
+      int64_t i = i0 + (int64_t)j * blockDim.x;
+      i = i < total ? i : total - 1;
+      const uint64_t* rec = i < nold ? oldpool + (size_t)(ostart[p] + i) * rec_words
+                                     : news + (size_t)(i - nold) * rec_words;
+      if constexpr (NACC > 0) {
+        load_rec<RW>(rec, w[j]);
+      } else {
+        for (int x = 0; x < rec_words; ++x) w[j][x] = rec[x];
       }
     }
-    long long f = (long long)w[1];
-    if (f < T.first[s]) atomicMin(&T.first[s], f);
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const bool todo = i0 + (int64_t)j * blockDim.x < total;
+      if (!todo) continue;
+      const uint64_t k = w[j][0];
+      const int s = lds_find_or_insert(T, k, fmix64(k), nslots);
+      if (s < 0) {
+        s_over = 1;
+        continue;
+      }
+#pragma unroll
+      for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
+        if (a < nacc) {
+          const uint64_t v = w[j][2 + a];
+          if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF)
+            acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
+        }
+      }
+      const long long f = (long long)w[j][1];
+      if (f < T.first[s]) atomicMin(&T.first[s], f);
+    }
   }
   __syncthreads();   // every record of the bucket is read before any is overwritten
   uint64_t* out = pool + (size_t)start * rec_words;
@@ -1147,11 +1344,14 @@ struct fpdb_group_table {
   int64_t* newcount = nullptr;       // nbuckets
   uint64_t* base = nullptr;          // nbuckets x max_blocks tail positions, bucket-major
   uint32_t* hist = nullptr;          // nbuckets x max_blocks tail counts, bucket-major
+  uint16_t* head_index = nullptr;    // kHeadIndexSlots head ids (two-choice index of the head keys)
   uint64_t* head_keys = nullptr;     // max_heads
   uint32_t* head_rank = nullptr;     // max_heads: rank among its bucket's head keys
   unsigned int* head_count = nullptr;   // nbuckets
   int* nhead = nullptr;              // head keys of the current batch
   uint64_t* hpart = nullptr;         // max_blocks x (1 + accumulators) x max_heads head partials
+  uint64_t* tail_total = nullptr;    // nbuckets: tail rows of the batch per bucket
+  uint64_t* tail_start = nullptr;    // nbuckets: pool position of the bucket's first tail row
   unsigned long long* totals = nullptr;   // [0] records of the new pool, [1] groups
   int* flags = nullptr;              // [0] bucket overflow, [1] key outside its packed range
   int64_t rows = 0;
@@ -1271,18 +1471,19 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   t->bucket_shift = 64 - lg;
   t->compact_slots = cslots;
   t->compact_lds = (size_t)(cslots + 1) * per_slot;
-  // main pass LDS: head index (kHeadIndexSlots x 4 B), per head key its key,
+  // main pass LDS: head index (kHeadIndexSlots x 2 B), per head key its key,
   // batch-relative first row (4 B) and accumulators, per bucket a 64-bit
-  // record position; at most a third of the index slots hold head keys
-  const size_t fixed = (size_t)kHeadIndexSlots * 4 + (size_t)nb * 8;
+  // record position; at most a sixth of the index slots hold head keys (the
+  // sampler's two-choice placement then rarely drops one)
+  const size_t fixed = (size_t)kHeadIndexSlots * 2 + (size_t)nb * 8;
   const size_t per_head = 8 + 4 + 8 * (size_t)nacc;
   int64_t heads = budget > fixed ? (int64_t)((budget - fixed) / per_head) : 0;
-  heads = std::min<int64_t>(heads, kHeadIndexSlots / 3);
+  heads = std::min<int64_t>(heads, kHeadIndexSlots / 6);
   t->max_heads = (int)heads;
   t->main_lds = fixed + per_head * (size_t)heads;
-  t->count_lds = (size_t)kHeadIndexSlots * 4 + (size_t)heads * 8 + (size_t)nb * 4;
-  t->sample_lds = (size_t)(kSampleSlots + 1) * 12 + (size_t)kCountBins * 4;
-  t->max_blocks = std::max(1, c->num_cus);
+  t->count_lds = (size_t)kHeadIndexSlots * 2 + (size_t)heads * 8 + (size_t)nb * 4;
+  t->sample_lds = (size_t)(kSampleSlots + 1) * 12 + (size_t)kCountBins * 4 + (size_t)nb * 4 + 8 + (size_t)heads * 8;
+  t->max_blocks = std::min(1024, std::max(1, c->num_cus));   // the bucket scan takes up to 1024 workgroups
   if (heads < 64 || cslots < 64 || t->sample_lds > budget) {
     delete t;
     return fail(FPDB_ERR_INVALID, "group-by: too many aggregates for the LDS table");
@@ -1293,12 +1494,15 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->newcount));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 8, reinterpret_cast<void**>(&t->base));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 4, reinterpret_cast<void**>(&t->hist));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)kHeadIndexSlots * 2, reinterpret_cast<void**>(&t->head_index));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)heads * 8, reinterpret_cast<void**>(&t->head_keys));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)heads * 4, reinterpret_cast<void**>(&t->head_rank));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 4, reinterpret_cast<void**>(&t->head_count));
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->nhead));
   if (s == FPDB_OK)
     s = gb_alloc(t, (uint64_t)t->max_blocks * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hpart));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_total));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_start));
   if (s == FPDB_OK) s = gb_alloc(t, 16, reinterpret_cast<void**>(&t->totals));
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->flags));
   if (s == FPDB_OK) s = launch_fill(c, t->ostart, 0, (uint64_t)nb * 8);
@@ -1313,11 +1517,45 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   return FPDB_OK;
 }
 
+template <int NACC, int KW, bool V8>
+static void launch_main_kv(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
+                           const GbGeom& G) {
+  hipLaunchKernelGGL((gb_main_kernel<NACC, KW, V8>), dim3(blocks), dim3(kGroupThreads), t->main_lds, c->stream, k,
+                     vals, t->acc, G, t->head_index, t->head_keys, t->nhead, t->base, t->tail_start, t->spare,
+                     t->hpart);
+}
+
+// The key width of the direct key path (one unpacked column of 4 or 8 bytes),
+// or 0 for the generic key walk.
+static int gb_key_width(const fpdb_group_table* t, const GKeySpec& k) {
+  if (k.n != 1 || k.check || k.bits[0] != 64 || k.shift[0] != 0 || k.bias[0] != 0) return 0;
+  return (k.w[0] == 4 || k.w[0] == 8) ? k.w[0] : 0;
+}
+
+// True when every aggregate input is an 8-byte column without nulls.
+static bool gb_values_8(const fpdb_group_table* t, const ColSet& vals) {
+  for (int a = 0; a < t->acc.n; ++a) {
+    const int32_t op = t->acc.op[a];
+    if (op == kAccCountAll) continue;
+    const int in = t->acc.input[a];
+    if (vals.width[in] != 8 || vals.validity[in]) return false;
+    if (op != kAccCountValid && vals.type[in] != FPDB_TYPE_INT64 && vals.type[in] != FPDB_TYPE_UINT64 &&
+        vals.type[in] != FPDB_TYPE_FLOAT64 && vals.type[in] != FPDB_TYPE_DATE64)
+      return false;
+  }
+  return true;
+}
+
 template <int NACC>
 static void launch_main(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
                         const GbGeom& G) {
-  hipLaunchKernelGGL(gb_main_kernel<NACC>, dim3(blocks), dim3(kGroupThreads), t->main_lds, c->stream, k, vals,
-                     t->acc, G, t->head_keys, t->nhead, t->base, t->spare, t->hpart);
+  const int kw = gb_key_width(t, k);
+  if (NACC > 0 && kw == 4 && gb_values_8(t, vals))
+    launch_main_kv<NACC, 4, true>(c, t, blocks, k, vals, G);
+  else if (NACC > 0 && kw == 8 && gb_values_8(t, vals))
+    launch_main_kv<NACC, 8, true>(c, t, blocks, k, vals, G);
+  else
+    launch_main_kv<NACC, 0, false>(c, t, blocks, k, vals, G);
 }
 template <int NACC>
 static void launch_compact(Context* c, fpdb_group_table* t) {
@@ -1368,13 +1606,21 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
     }
   }
   {
+    {
+      ProfScope ps(c, "groupby_sample");
+      hipLaunchKernelGGL(gb_sample_kernel, dim3(1), dim3(1024), t->sample_lds, c->stream, k, G, t->head_index,
+                         t->head_keys,
+                         t->head_rank, t->head_count, t->nhead);
+    }
     ProfScope ps(c, "groupby_count");
-    hipLaunchKernelGGL(gb_sample_kernel, dim3(1), dim3(1024), t->sample_lds, c->stream, k, G, t->head_keys,
-                       t->head_rank, t->head_count, t->nhead);
-    hipLaunchKernelGGL(gb_count_kernel<4>, dim3(blocks), dim3(kGroupThreads), t->count_lds, c->stream, k, G,
-                       t->head_keys, t->nhead, t->hist, t->flags + 1);
-    hipLaunchKernelGGL(gb_offsets_kernel, dim3(1), dim3(1024), 0, c->stream, t->hist, blocks, t->nbuckets, t->ocount,
-                       t->head_count, t->base, t->nstart, t->newcount, t->totals);
+    const int kw = gb_key_width(t, k);
+    auto count_kernel = kw == 8 ? gb_count_kernel<8> : (kw == 4 ? gb_count_kernel<4> : gb_count_kernel<0>);
+    hipLaunchKernelGGL(count_kernel, dim3(blocks), dim3(kGroupThreads), t->count_lds, c->stream, k, G,
+                       t->head_index, t->head_keys, t->nhead, t->hist, t->flags + 1);
+    hipLaunchKernelGGL(gb_bucket_scan_kernel, dim3((unsigned)ceil_div(t->nbuckets, 16)), dim3(1024), 0, c->stream,
+                       t->hist, blocks, t->nbuckets, t->base, t->tail_total);
+    hipLaunchKernelGGL(gb_bucket_offsets_kernel, dim3(1), dim3(1024), 0, c->stream, t->nbuckets, t->ocount,
+                       t->head_count, t->tail_total, t->nstart, t->newcount, t->tail_start, t->totals);
     FPDB_HIP(hipGetLastError());
   }
   if (exact) {
@@ -1391,7 +1637,8 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
   }
   {
     ProfScope ps(c, "groupby_compact");
-    hipLaunchKernelGGL(gb_head_merge_kernel, dim3((unsigned)ceil_div(t->max_heads, 64)), dim3(64 * kMergeWaves), 0,
+    hipLaunchKernelGGL(gb_head_merge_kernel, dim3((unsigned)ceil_div(t->max_heads, 64)), dim3(64 * kMergeWaves),
+                       (size_t)kMergeWaves * (1 + t->acc.n) * 64 * 8,
                        c->stream, t->acc, G, blocks, t->head_keys, t->head_rank, t->nhead, t->hpart, t->nstart,
                        t->ocount, t->spare);
     FPDB_GB_DISPATCH(launch_compact, c, t);
@@ -1482,7 +1729,8 @@ fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, in
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
   void* bufs[] = {t->pool, t->spare, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
-                  t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->totals, t->flags};
+                  t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->tail_total, t->tail_start,
+                  t->totals, t->flags};
   for (void* b : bufs)
     if (b) ctx_free(t->ctx, b);
   delete t;

@@ -49,6 +49,32 @@ def test_zipf_groupby_vs_oracle(ctx, n, batches):
     _check(w["key"], {"sum_i": [a[1], a[2]], "sum_f": [a[3]]}, k, a[0], [w["qty"], w["price"]], [w["disc"]])
 
 
+@pytest.mark.parametrize("layout", ["sorted", "hidden_heavy_key"])
+def test_skewed_layouts_vs_oracle(ctx, layout):
+    """Inputs the head-key sample does not see as the whole does: sorted keys
+    (long runs of one key), and a key holding a quarter of the rows placed
+    only between the sampled stretches of rows, so all of its rows reach the
+    per-bucket aggregation as tail rows."""
+    n = 3_000_000
+    w = datagen.groupby_workload(n=n, seed=21)
+    key = w["key"].copy()
+    if layout == "sorted":
+        order = np.argsort(key, kind="stable")
+        key = key[order]
+        qty, price, disc = w["qty"][order], w["price"][order], w["disc"][order]
+    else:
+        qty, price, disc = w["qty"], w["price"], w["disc"]
+        # the sampler reads 64 runs of 1,024 rows at (n - 1024) * j / 63
+        gap = np.ones(n, bool)
+        for j in range(64):
+            s = (n - 1024) * j // 63
+            gap[s:s + 1024] = False
+        pick = np.flatnonzero(gap)[::3]
+        key[pick] = 123_456_789
+    k, a = _run(ctx, key, qty, price, disc)
+    _check(key, {"sum_i": [a[1], a[2]], "sum_f": [a[3]]}, k, a[0], [qty, price], [disc])
+
+
 def test_attic_group_vector(ctx):
     # fpdb-executor/ATTIC/test/GroupTest.cpp:137-150: sum(ab) by aa -> 10: 27, 12: 15
     g = ops.GroupBy(ctx, [C.TYPE_INT64], [(C.AGG_SUM, C.TYPE_INT64, False)], 16)
