@@ -69,7 +69,7 @@ class StarQuery(ctypes.Structure):
 
 
 class StarTuning(ctypes.Structure):
-    _fields_ = [("blocks_per_cu", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+    _fields_ = [("blocks_per_cu", ctypes.c_int32), ("barrier_timeout_us", ctypes.c_int32),
                 ("ht_load_shift", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
 
 

@@ -77,6 +77,7 @@ struct StarDev {
   long long* g_first;
   uint64_t g_mask;
   int32_t tile_rows;
+  uint64_t barrier_ticks;          // grid barrier wait bound, device wall-clock ticks
 };
 
 __device__ __forceinline__ int log_blocks_for(int64_t n) {
@@ -112,6 +113,7 @@ constexpr int kStarFilterLdsWords = 8192;   // 64 KiB
 constexpr size_t kStarDynamicLds = kStarFilterLdsWords * 8;
 constexpr int kStarItems = 8;
 constexpr int kStarThreads = 1024;   // threads per workgroup of the persistent kernel
+constexpr uint64_t kStarBarrierTimeoutUs = 200000;   // grid barrier wait bound (a normal wait is < 1 ms)
 
 __device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const uint64_t* s_masks, int64_t key) {
   uint32_t h = arrow6_hash_word((uint64_t)key);
@@ -121,39 +123,53 @@ __device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const u
 }
 
 // Grid barrier for a launch whose workgroups are all resident (the grid is
-// sized from the occupancy calculator).  Sense-free: a generation counter
-// advances when the last workgroup arrives.  Bounded spin: a workgroup that
-// waits too long sets the error flag and continues, so the kernel always ends.
-// Only thread 0 fences: __syncthreads orders the workgroup's stores before
-// its agent-scope release (an L2 write-back on this part), and its acquire
-// before the workgroup's later loads; a fence in every thread repeated the
-// write-back per thread.
-__device__ __forceinline__ void star_barrier(unsigned long long* st) {
-  unsigned long long* count = st + 16;
-  unsigned long long* gen = st + 17;
-  unsigned long long* err = st + 32;
+// sized from the occupancy calculator and checked against it at every
+// launch).  Sense-free: a generation counter advances when the last workgroup
+// arrives.  The wait is bounded by wall-clock time (P.barrier_ticks of the
+// device's constant-rate clock): a workgroup that waits longer sets the error
+// word.  Once the error word is set every workgroup stops at its next barrier
+// (returns false, and the kernel returns), so a missed barrier ends the launch
+// within about one bound, with FPDB_ERR_INTERNAL reported by the result call.
+// Only thread 0 fences: __syncthreads orders the workgroup's stores before its
+// agent-scope release (an L2 write-back on this part), and its acquire before
+// the workgroup's later loads.
+__device__ __forceinline__ bool star_barrier(const StarDev& P) {
+  unsigned long long* count = P.state + 16;
+  unsigned long long* gen = P.state + 17;
+  unsigned long long* err = P.state + 32;
+  __shared__ int s_ok;
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();    // the workgroup's writes of this phase become visible device-wide
-    unsigned long long g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long arrived =
-        __hip_atomic_fetch_add(count, 1ULL, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
-    if (arrived == gridDim.x) {
-      __hip_atomic_store(count, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(gen, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      int64_t spins = 0;
-      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-        __builtin_amdgcn_s_sleep(2);
-        if (++spins > (int64_t(1) << 26)) {
-          __hip_atomic_store(err, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
+    int ok = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+    if (ok) {
+      const unsigned long long g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long arrived =
+          __hip_atomic_fetch_add(count, 1ULL, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+      if (arrived == gridDim.x) {
+        __hip_atomic_store(count, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gen, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+          __builtin_amdgcn_s_sleep(2);
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            ok = 0;   // another workgroup gave up
+            break;
+          }
+          if (wall_clock64() - t0 > P.barrier_ticks) {
+            __hip_atomic_store(err, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
         }
       }
     }
     __threadfence();    // and the workgroup observes the other workgroups' writes
+    s_ok = ok;
   }
   __syncthreads();
+  return s_ok != 0;
 }
 
 // Hash-table mask of a dimension for this run: capacity the smallest power of
@@ -181,7 +197,7 @@ __device__ __forceinline__ void ht_insert(StarSlot* ht, uint64_t mask, int64_t k
 // Returns true and the payload if key is present (keys are unique per dimension).
 __device__ __forceinline__ bool ht_find(const StarSlot* ht, uint64_t mask, int64_t key, uint32_t* payload) {
   uint64_t s = fmix64((uint64_t)key) & mask;
-  while (true) {
+  for (uint64_t p = 0; p <= mask; ++p) {   // at most every slot once, even in a table left full
     uint4 v = reinterpret_cast<const uint4*>(ht)[s];
     if (v.z == kStarEmptyRow) return false;
     int64_t k = (int64_t)((uint64_t)v.x | ((uint64_t)v.y << 32));
@@ -191,6 +207,7 @@ __device__ __forceinline__ bool ht_find(const StarSlot* ht, uint64_t mask, int64
     }
     s = (s + 1) & mask;
   }
+  return false;
 }
 
 // Group slots 0..g_mask hold keys; slot g_mask + 1 is reserved for the one
@@ -271,7 +288,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
       __hip_atomic_fetch_add(P.state + 1 + d, s_kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
   }
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   // A2: zero the forward filters at their sizes; clear the used part of the
   // hash tables
   for (int d = 0; d < P.ndims; ++d) {
@@ -282,7 +299,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     const int64_t hm = (int64_t)star_ht_mask(D, P.state, d);
     for (int64_t s = gtid; s <= hm; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
   }
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   // A3: insert the kept keys into the forward filters
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
@@ -290,7 +307,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     for (int64_t r = gtid; r < D.rows; r += gthreads)
       if (D.kept[r]) bf_insert(D.fbf, lnb, s_masks, D.key[r]);
   }
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   // B: forward pass over every fact row (all partitions), filters in `order`.
   // The filters that fit (in that order) are copied into LDS; each thread
   // takes kStarItems rows per step with the first filter's keys loaded
@@ -360,17 +377,17 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
       for (int j = 0; j < kStarItems; ++j) append_survivor(P, pass[j], r0 + (int64_t)j * blockDim.x);
     }
   }
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   // B2: backward filters over the survivors
   const int64_t nsurv = (int64_t)__hip_atomic_load(P.state + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int blnb = log_blocks_for(nsurv);
   for (int d = 0; d < P.ndims; ++d)
     for (int64_t b = gtid; b < (int64_t(1) << blnb); b += gthreads) P.dim[d].bbf[b] = 0;
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   for (int d = 0; d < P.ndims; ++d)
     for (int64_t i = gtid; i < nsurv; i += gthreads)
       bf_insert(P.dim[d].bbf, blnb, s_masks, P.s_fk[(int64_t)d * P.fact_rows + i]);
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   // C: kept dimension rows that pass the backward filter go into the hash tables
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
@@ -382,7 +399,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
       ht_insert(D.ht, hm, k, (uint32_t)r, D.gattr ? (uint32_t)D.gattr[r] : 0u);
     }
   }
-  star_barrier(P.state);
+  if (!star_barrier(P)) return;
   // D: join the survivors with every dimension and aggregate
   __shared__ uint64_t hms[kStarMaxDims];
   if (threadIdx.x < P.ndims) hms[threadIdx.x] = star_ht_mask(P.dim[threadIdx.x], P.state, threadIdx.x);
@@ -498,6 +515,7 @@ struct fpdb_star {
   fpdb::StarDev dev;
   std::vector<void*> owned;
   int grid = 0;
+  int resident_per_cu = 0;             // workgroups of the persistent kernel one CU holds
   // result (device)
   int32_t* out_g0 = nullptr;
   int32_t* out_g1 = nullptr;
@@ -766,6 +784,11 @@ static fpdb_status star_plain(fpdb_star* s) {
 
 static fpdb_status star_batched_enqueue(fpdb_star* s) {
   Context* c = s->ctx;
+  // every workgroup of the persistent kernel must be resident at once for its
+  // grid barriers: the grid never exceeds what the CUs hold at its LDS and
+  // register use
+  FPDB_CHECK(s->grid >= 1 && s->grid <= s->resident_per_cu * std::max(1, c->num_cus), FPDB_ERR_INTERNAL,
+             "star join: persistent grid exceeds the resident workgroups");
   hipLaunchKernelGGL(star_init_kernel, dim3(64), dim3(256), 0, c->stream, s->dev);
   FPDB_HIP(hipGetLastError());
   {
@@ -833,6 +856,7 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   s->q.partition_offsets = nullptr;
   s->tune = tuning ? *tuning : fpdb_star_tuning{0, 0, 1, 0};
   if (s->tune.ht_load_shift < 1 || s->tune.ht_load_shift > 3) s->tune.ht_load_shift = 1;
+  FPDB_CHECK(s->tune.barrier_timeout_us >= 0, FPDB_ERR_INVALID, "star join: negative barrier timeout");
   s->max_groups = q->max_groups;
   StarDev& P = s->dev;
   std::memset(&P, 0, sizeof(P));
@@ -894,7 +918,17 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
     return fail(FPDB_ERR_HIP, "star join: occupancy query failed");
   }
   int want = s->tune.blocks_per_cu > 0 ? s->tune.blocks_per_cu : 4;
+  s->resident_per_cu = per_cu;
   s->grid = std::max(1, c->num_cus) * std::min(want, per_cu);
+  // grid barrier wait bound in ticks of the device's constant-rate wall clock
+  int clock_khz = 0;
+  e = hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device);
+  if (e != hipSuccess || clock_khz <= 0) {
+    fpdb_star_destroy(s);
+    return fail(FPDB_ERR_HIP, "star join: wall clock rate query failed");
+  }
+  const uint64_t us = s->tune.barrier_timeout_us > 0 ? (uint64_t)s->tune.barrier_timeout_us : kStarBarrierTimeoutUs;
+  s->dev.barrier_ticks = us * (uint64_t)clock_khz / 1000;
   *out = s;
   return FPDB_OK;
 }

@@ -23,8 +23,10 @@ CANDIDATES = [(bpc, shift) for bpc in (1, 2, 4, 8) for shift in (1, 2)]
 class StarJoin:
     """One prepared star-join plan over device-resident columns."""
 
-    def __init__(self, ctx, workload, group_dims=(0, 3), filter_order=None, tuning=None, max_groups=4096):
+    def __init__(self, ctx, workload, group_dims=(0, 3), filter_order=None, tuning=None, max_groups=4096,
+                 barrier_timeout_us=0):
         self.ctx = ctx
+        self.barrier_timeout_us = int(barrier_timeout_us)
         self.w = workload
         dims = workload["dims"]
         self.cols = []          # keep device columns alive
@@ -71,8 +73,9 @@ class StarJoin:
     def prepare(self, tuning=None):
         self.close()
         t = None
-        if tuning is not None:
-            t = C.StarTuning(int(tuning[0]), 0, int(tuning[1]), 0)
+        if tuning is not None or self.barrier_timeout_us:
+            bpc, shift = tuning if tuning is not None else (0, 1)
+            t = C.StarTuning(int(bpc), self.barrier_timeout_us, int(shift), 0)
         h = ctypes.c_void_p()
         C.check(C.lib().fpdb_star_prepare(self.ctx.handle, ctypes.byref(self.q), ctypes.byref(t) if t else None,
                                           ctypes.byref(h)))

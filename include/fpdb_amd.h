@@ -544,10 +544,13 @@ typedef struct fpdb_star_query {
 
 /* Launch shape of the batched form (0 = default): workgroups per CU of the
  * persistent kernel, and hash-table capacity as a power-of-two multiple
- * (2^ht_load_shift) of the dimension rows.  Chosen by the autotuner. */
+ * (2^ht_load_shift) of the dimension rows.  Chosen by the autotuner.
+ * barrier_timeout_us bounds each grid-barrier wait of the persistent kernel
+ * (0 = 200 ms); a wait past it ends the launch early and the result call
+ * reports FPDB_ERR_INTERNAL. */
 typedef struct fpdb_star_tuning {
   int32_t blocks_per_cu;
-  int32_t reserved0;
+  int32_t barrier_timeout_us;
   int32_t ht_load_shift;
   int32_t reserved1;
 } fpdb_star_tuning;

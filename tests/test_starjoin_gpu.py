@@ -92,3 +92,26 @@ def test_star_join_autotune_and_full_size(ctx, tmp_path):
     # a second autotune call is a cache hit
     best2, _ = autotune(sj, "test-device", reps=1, cache_path=str(tmp_path / "tune.json"))
     assert best2 == best
+
+
+def test_barrier_wait_bound_reports_error_and_next_run_is_exact(ctx):
+    """A grid-barrier wait bound of 1 us (set from the host) makes the
+    persistent kernel's waiting workgroups give up: the launch still ends
+    (every workgroup stops at its next barrier) and the result call reports
+    FPDB_ERR_INTERNAL.  The next run, with the default bound, is exact."""
+    from flexpushdowndb_amd import _capi as C
+    w = datagen.star_workload(n_fact=2_000_000, n_parts=8, seed=9, keep_below=(30, 20, 10, 40))
+    exp = datagen.star_expected(w)
+    tight = StarJoin(ctx, w, barrier_timeout_us=1)
+    tight.run_batched()
+    with pytest.raises(C.FpdbError) as e:
+        tight.result()
+    assert e.value.code == C.FPDB_ERR_INTERNAL
+    tight.run_plain()                  # the plain form has no grid barrier
+    _same(tight.result(), exp)
+    tight.close()
+    normal = StarJoin(ctx, w)
+    normal.run_batched()
+    _same(normal.result(), exp)
+    normal.run_batched()
+    _same(normal.result(), exp)
