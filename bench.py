@@ -197,17 +197,24 @@ def bench_bloom(ctx, steps):
     if not (np.array_equal(fs.blocks_numpy(), ref_s) and np.array_equal(
             bm.numpy(125_000), O.blocked_find(fs.log_num_blocks, ref_s, O.arrow6_hash([(w["probe"][:1_000_000], None)])))):
         raise SystemExit("bloom filter (LDS-resident) output does not match the oracle")
-    # CPU: the oracle's hash + FindImp over the same 60M probe keys
+    # CPU: the oracle's hash + FindImp over the same 60M probe keys, on one
+    # host thread and on P threads over P slices of the probe column
     m = n
     t0 = time.perf_counter()
     O.blocked_find(f.log_num_blocks, ref, O.arrow6_hash([(w["probe"][:m], None)]))
     cpu_s = time.perf_counter() - t0
+    p = host_cores()
+    slices = np.array_split(w["probe"], p)
+    cpu_p = run_parallel(lambda sl: O.blocked_find(f.log_num_blocks, ref, O.arrow6_hash([(sl, None)])), slices)
     legacy = bench_bloom_legacy(ctx, steps, w, pc, a_keys, bm, cnt)
     return {
         "legacy_filter": legacy,
-        "cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
+        "cpu_baseline": {"value": m / cpu_p / 1e9, "unit": "Gkeys/s", "cores": p, "kind": "port",
                          "sample": f"the whole workload: oracle Arrow 6 hash + blocked-filter Find (C++) over all "
-                                   f"{m} probe keys against the same 2^{f.log_num_blocks}-block filter, {cpu_s:.2f} s"},
+                                   f"{m} probe keys against the same 2^{f.log_num_blocks}-block filter, {p} host "
+                                   f"threads over {p} slices, {cpu_p:.2f} s",
+                         "single_core": {"value": m / cpu_s / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
+                                         "sample": f"the same on one host thread, {cpu_s:.2f} s"}},
         "workload": f"blocked bloom use: filter from {len(kept)} keys (2^{f.log_num_blocks} blocks), "
                     f"{n} int64 probe keys",
         "find_kernel_ms": find_ms, "find_gkeys_per_s": n / (find_ms * 1e-3) / 1e9,
@@ -322,16 +329,39 @@ def bench_groupby(ctx, steps):
         raise SystemExit("group-by output does not match the oracle's closed form")
     alg = n * (4 + 8 + 8 + 8)
     # CPU: the oracle's hash group-by (C++ unordered_map, first-appearance
-    # order) over the same 60M rows
+    # order) over the same 60M rows, on one host thread and on P threads over
+    # P key partitions (each a partial group-by of its partition's groups);
+    # pyarrow's acero group-by on all host cores when present
     from oracle import oracle as O
     m = n
     t0 = time.perf_counter()
     O.groupby(w["key"][:m].astype(np.uint64), sums_i64=(w["qty"][:m], w["price"][:m]), sums_f64=(w["disc"][:m],),
               max_groups=100_000)
     cpu_s = time.perf_counter() - t0
-    return {"cpu_baseline": {"value": m / cpu_s / 1e9, "unit": "Grows/s", "cores": 1, "kind": "port",
-                             "sample": f"the whole workload: oracle group-by (COUNT, SUM int64 x2, SUM float64) "
-                                       f"over all {m} rows, {cpu_s:.2f} s"},
+    p = host_cores()
+    parts = [(w["key"][i].astype(np.uint64), w["qty"][i], w["price"][i], w["disc"][i]) for i in hash_parts(w["key"], p)]
+    cpu_p = run_parallel(lambda q: O.groupby(q[0], sums_i64=(q[1], q[2]), sums_f64=(q[3],), max_groups=100_000), parts)
+    del parts
+    cpu = {"value": m / cpu_p / 1e9, "unit": "Grows/s", "cores": p, "kind": "port",
+           "sample": f"the whole workload: oracle group-by (COUNT, SUM int64 x2, SUM float64) over all {m} rows, "
+                     f"{p} host threads over {p} key partitions, {cpu_p:.2f} s",
+           "single_core": {"value": m / cpu_s / 1e9, "unit": "Grows/s", "cores": 1, "kind": "port",
+                           "sample": f"the same on one host thread, {cpu_s:.2f} s"}}
+    try:
+        import pyarrow as pa
+        tbl = pa.table({"key": w["key"], "qty": w["qty"], "price": w["price"], "disc": w["disc"]})
+        t0 = time.perf_counter()
+        res = tbl.group_by("key", use_threads=True).aggregate(
+            [("key", "count"), ("qty", "sum"), ("price", "sum"), ("disc", "sum")])
+        ta = time.perf_counter() - t0
+        assert res.num_rows == len(k)
+        cpu["acero"] = {"value": m / ta / 1e9, "unit": "Grows/s", "cores": pa.cpu_count(), "kind": "acero",
+                        "sample": f"pyarrow {pa.__version__} Table.group_by(...).aggregate (acero hash aggregate, "
+                                  f"COUNT, SUM x3, use_threads) over all {m} rows, {ta:.2f} s"}
+        del tbl, res
+    except ImportError:
+        pass
+    return {"cpu_baseline": cpu,
             "workload": f"hash group-by: {n} rows, int32 Zipf(1.1) key over 100000 values, "
                         "COUNT, SUM int64 x2, SUM float64", "groups": int(len(k)),
             "consume_kernels_ms": cons_ms, "sample_ms": sample_ms, "count_ms": count_ms, "main_ms": main_ms,
@@ -372,7 +402,7 @@ def bench_star(ctx, steps, device_name):
     if not ok:
         raise SystemExit("star join result does not match the expected join + group-by")
     plain_ms = statistics.median(plain)
-    fact_bytes = len(w["measure"]) * 8 * len(w["dims"])
+    stage_bytes = star_stage_bytes(w, sj.default_order())
     # CPU: the same query over all fact rows with the oracle's hash joins
     # (one per dimension, filtered dimension rows as the build side) and its
     # group-by
@@ -402,7 +432,40 @@ def bench_star(ctx, steps, device_name):
                         "blocked bloom filters, 4 joins, group-by on 2 dimension attributes",
             "batched_ms": batched_ms, "plain_ms": plain_ms, "speedup": plain_ms / batched_ms,
             "groups": int(len(got["count"])), "survivors": exp["survivors"], "tuning": list(best),
-            "fact_fk_hbm_frac": fact_bytes / (batched_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            "alg_bytes": stage_bytes["total"], "alg_bytes_by_stage": stage_bytes,
+            "hbm_frac": stage_bytes["total"] / (batched_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
+def star_stage_bytes(w, order):
+    """Algorithmic bytes of one batched star join, stage by stage from the
+    workload's actual row counts: each dimension's key, filter and group
+    columns read once; the fact pass reads the first filter's key column in
+    full and each later filter's key column only for the rows still passing
+    (the forward filters' real pass counts, false positives included, from the
+    oracle's filters); survivors are written once (row, keys, measure) and read
+    back by the backward-filter build and by the joins and group-by."""
+    from oracle import oracle as O
+    n = len(w["measure"])
+    nd = len(w["dims"])
+    dims = sum(len(d["key"]) * (8 + 4 + (4 if "group" in d else 0)) for d in w["dims"])
+    passing = np.ones(n, bool)
+    fact = 0
+    for i, d in enumerate(order):
+        dim = w["dims"][d]
+        kept = dim["key"][dim["attr"] < dim["keep_below"]]
+        lnb, blocks = O.blocked_build(O.arrow6_hash([(kept, None)]) if len(kept) else np.zeros(0, np.uint32),
+                                      n_capacity=len(kept))
+        fact += 8 * (n if i == 0 else int(passing.sum()))
+        bits = O.blocked_find(lnb, blocks, O.arrow6_hash([(w["fact_fk"][d], None)]))
+        passing &= np.unpackbits(bits, bitorder="little")[:n].astype(bool)
+    surv = int(passing.sum())
+    surv_write = surv * (8 + 8 * nd + 8) + surv * 8          # row, keys, measure written; measure read
+    backward = surv * 8 * nd                                   # keys read for the backward filters
+    joins = surv * (8 * nd + 8 + 8)                             # keys, measure and row read by the joins
+    out = {"dimensions": dims, "fact_pass": fact, "survivors_written": surv_write, "backward_filters": backward,
+           "joins_and_groupby": joins, "survivors": surv}
+    out["total"] = dims + fact + surv_write + backward + joins
+    return out
 
 
 SERVE_RATES = ((50, 150), (200, 400), (800, 800))   # (queries/s, queries) per arrival rate
@@ -440,6 +503,17 @@ def bench_serve(ctx, rates=SERVE_RATES):
                              "throughput_qps": n / (makespan / 1e9)})
         log(f"serve: {rate} q/s: p50 {out['rates'][-1]['p50_ms']:.3f} ms, p99 {out['rates'][-1]['p99_ms']:.3f} ms")
     srv.close()
+    # CPU: the same kind of query answered on the host by direct indexing of
+    # the dense dimension keys (numpy, one thread), one query after another
+    qs = datagen.serve_queries(12, seed=300)
+    t0 = time.perf_counter()
+    for q in qs:
+        datagen.serve_expected(w, q)
+    cpu_s = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": len(qs) / cpu_s, "unit": "queries/s", "cores": 1, "kind": "port",
+                           "sample": f"{len(qs)} serving queries over the 60M-row fact table answered one after "
+                                     f"another by numpy direct indexing of the dense dimension keys (filters, join "
+                                     f"by key index, group-by by first appearance), {cpu_s:.2f} s"}
     return out
 
 
@@ -667,9 +741,39 @@ def bench_copy_calibration(ctx, steps, n=64 * 1024 * 1024):
             "ms": ms, "gbs": 16 * n / (ms * 1e-3) / 1e9}
 
 
+def host_cores():
+    """Host CPUs this process may run on (the GPU box's share, not the whole machine)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def run_parallel(fn, parts):
+    """Runs fn over the parts on one host thread each (the oracle's C++ calls
+    release the GIL) and returns the wall time in seconds."""
+    import concurrent.futures
+    with concurrent.futures.ThreadPoolExecutor(len(parts)) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(fn, parts))
+        return time.perf_counter() - t0
+
+
+def hash_parts(key, p):
+    """Row indices of each of p partitions of key % p (the reference's
+    PARALLEL_DEGREE operator instances each get the rows of one partition)."""
+    slot = (np.asarray(key).astype(np.int64) % p).astype(np.int64)
+    order = np.argsort(slot, kind="stable")
+    bounds = np.searchsorted(slot[order], np.arange(p + 1))
+    return [order[bounds[i]:bounds[i + 1]] for i in range(p)]
+
+
 def cpu_baseline_join(w, runs=3):
     """The oracle's hash join (C++ unordered_map build + probe, pairs only) on
-    the whole join workload, single-threaded on the host."""
+    the whole join workload: one host thread, P host threads over P key
+    partitions (P = host cores, inputs partitioned beforehand, as the
+    reference's parallel operator instances receive shuffled inputs), and
+    pyarrow's acero hash join on all host cores when pyarrow is present."""
     from oracle import oracle as O
     times = []
     for _ in range(runs):
@@ -678,10 +782,42 @@ def cpu_baseline_join(w, runs=3):
         times.append(time.perf_counter() - t0)
         assert len(op) == len(w["probe_fk"])
     t = statistics.median(times)
-    return {"value": len(w["probe_fk"]) / t / 1e9, "unit": "Grows/s", "cores": 1, "kind": "port",
-            "sample": f"whole join workload (1,000,000-row build, 6,000,000-row probe), oracle C++ "
-                      f"unordered_map hash join producing (probe,build) row pairs; median of {runs} runs, "
-                      f"{statistics.median(times):.3f} s each; host has {os.cpu_count()} logical CPUs"}
+    n = len(w["probe_fk"])
+    p = host_cores()
+    bparts, pparts = hash_parts(w["build_key"], p), hash_parts(w["probe_fk"], p)
+    parts = [(w["build_key"][b], w["probe_fk"][q]) for b, q in zip(bparts, pparts)]
+    tp = statistics.median(run_parallel(lambda bp: O.join_pairs(bp[0], bp[1]), parts) for _ in range(runs))
+    out = {"value": n / tp / 1e9, "unit": "Grows/s", "cores": p, "kind": "port",
+           "sample": f"whole join workload (1,000,000-row build, 6,000,000-row probe), oracle C++ unordered_map "
+                     f"hash join producing (probe,build) row pairs, {p} host threads each joining one of {p} key "
+                     f"partitions; median of {runs} runs, {tp:.3f} s each",
+           "single_core": {"value": n / t / 1e9, "unit": "Grows/s", "cores": 1, "kind": "port",
+                           "sample": f"the same oracle join on one host thread, {t:.3f} s"}}
+    acero = acero_join(w, runs)
+    if acero:
+        out["acero"] = acero
+    return out
+
+
+def acero_join(w, runs=3):
+    """pyarrow's acero hash join (the successor of the reference's Arrow 6
+    delegate) on the join workload, all host cores; None without pyarrow."""
+    try:
+        import pyarrow as pa
+    except ImportError:
+        return None
+    build = pa.table({"key": w["build_key"], "pay": w["build_pay"]})
+    probe = pa.table({"fk": w["probe_fk"], "qty": w["probe_qty"], "price": w["probe_price"]})
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        res = probe.join(build, keys="fk", right_keys="key", join_type="inner", use_threads=True)
+        times.append(time.perf_counter() - t0)
+        assert res.num_rows == len(w["probe_fk"])
+    t = statistics.median(times)
+    return {"value": len(w["probe_fk"]) / t / 1e9, "unit": "Grows/s", "cores": pa.cpu_count(), "kind": "acero",
+            "sample": f"pyarrow {pa.__version__} Table.join (acero hash join, inner, use_threads) over the whole "
+                      f"join workload; median of {runs} runs, {t:.3f} s each"}
 
 
 def main():
