@@ -143,6 +143,7 @@ SIGNATURES = [
     ("fpdb_compare_bitmap", _I32, [_P, _PCOL, _I, _I64, ctypes.c_double, _P, _I64, _P, _P]),
     ("fpdb_filter_compact", _I32, [_P, _PCOL, _I, _I64, _P, _PCOL, _PI64]),
     ("fpdb_filter_compact_noalloc", _I32, [_P, _PCOL, _I, _I64, _P, _PCOL, _P]),
+    ("fpdb_divide_to_float64", _I32, [_P, _PCOL, _PCOL, _I64, _PCOL]),
     ("fpdb_bloom_blocked_filter", _I32, [_P, _P, _I, _PCOL, _I, ctypes.POINTER(_I), _I, _I64, _P, _PCOL, _PI64]),
     ("fpdb_hj_build", _I32, [_P, _PCOL, _I, _I64, _PCOL, _I, ctypes.POINTER(_P)]),
     ("fpdb_hj_table_destroy", _I32, [_P]),

@@ -1885,3 +1885,90 @@ fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* t, fpdb_c
 }
 
 }  // extern "C"
+
+// ---- averages from intermediate sums and counts ------------------------------
+
+namespace fpdb {
+
+// One wave per 64 rows: quotient, validity word from a ballot.
+__global__ __launch_bounds__(256) void divide_kernel(ColSet in, int64_t n, double* __restrict__ out,
+                                                     uint8_t* __restrict__ out_validity, int64_t out_off,
+                                                     int* div_zero) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const bool int_num = !(in.type[0] == FPDB_TYPE_FLOAT32 || in.type[0] == FPDB_TYPE_FLOAT64);
+  const bool int_den = !(in.type[1] == FPDB_TYPE_FLOAT32 || in.type[1] == FPDB_TYPE_FLOAT64);
+  bool zero = false;
+  for (int64_t c = wave; c < (n + 63) / 64; c += nwaves) {
+    const int64_t r = c * 64 + lane;
+    bool valid = false;
+    double q = 0.0;
+    if (r < n) {
+      const int64_t i0 = r + in.offset[0], i1 = r + in.offset[1];
+      valid = valid_bit(in.validity[0], i0) && valid_bit(in.validity[1], i1);
+      auto as_f64 = [&](int k, int64_t i) -> double {
+        if (in.type[k] == FPDB_TYPE_FLOAT64) return reinterpret_cast<const double*>(in.values[k])[i];
+        if (in.type[k] == FPDB_TYPE_FLOAT32) return (double)reinterpret_cast<const float*>(in.values[k])[i];
+        if (type_is_signed_int_d(in.type[k])) return (double)load_sext(in.values[k], in.width[k], i);
+        return (double)load_zext(in.values[k], in.width[k], i);
+      };
+      if (valid) {
+        if (int_num && int_den) {
+          const bool sn = type_is_signed_int_d(in.type[0]), sd = type_is_signed_int_d(in.type[1]);
+          const int64_t a = sn ? load_sext(in.values[0], in.width[0], i0) : (int64_t)load_zext(in.values[0], in.width[0], i0);
+          const int64_t b = sd ? load_sext(in.values[1], in.width[1], i1) : (int64_t)load_zext(in.values[1], in.width[1], i1);
+          if (b == 0) {
+            zero = true;
+          } else {
+            q = (double)(b == -1 ? (int64_t)(0 - (uint64_t)a) : a / b);
+          }
+        } else {
+          q = as_f64(0, i0) / as_f64(1, i1);
+        }
+      }
+      out[r] = q;
+    }
+    const uint64_t word = __ballot(valid);
+    if (out_validity && lane < 2) {
+      const int64_t bit0 = c * 64 + out_off + 32 * lane;   // out_off is a multiple of 32
+      if (c * 64 + 32 * lane < n)
+        reinterpret_cast<uint32_t*>(out_validity)[bit0 >> 5] = (uint32_t)(word >> (32 * lane));
+    }
+  }
+  if (zero) *div_zero = 1;
+}
+
+}  // namespace fpdb
+
+extern "C" {
+
+fpdb_status fpdb_divide_to_float64(fpdb_context* ctx, const fpdb_column* num, const fpdb_column* den,
+                                   int64_t num_rows, fpdb_column* out) {
+  FPDB_CHECK(ctx && num && den && out, FPDB_ERR_INVALID, "NULL argument");
+  FPDB_CHECK(out->type == FPDB_TYPE_FLOAT64 && (num_rows == 0 || out->values), FPDB_ERR_INVALID,
+             "divide: the output must be a float64 column");
+  FPDB_CHECK(out->length >= num_rows, FPDB_ERR_INVALID, "divide: output column too short");
+  FPDB_CHECK(out->validity == nullptr || (out->offset & 31) == 0, FPDB_ERR_INVALID,
+             "divide: output validity offset must be a multiple of 32");
+  fpdb_column both[2] = {*num, *den};
+  ColSet in;
+  FPDB_TRY(make_colset(both, 2, num_rows, &in, "divide"));
+  FPDB_CHECK(out->validity || (!num->validity && !den->validity), FPDB_ERR_INVALID,
+             "divide: nullable inputs need an output validity bitmap");
+  if (num_rows == 0) return FPDB_OK;
+  Context* c = as_ctx(ctx);
+  int* flag = reinterpret_cast<int*>(c->d_counter + 2);
+  FPDB_TRY(launch_fill(c, flag, 0, 4));
+  const int grid = grid_for(c, num_rows, 256, 8);
+  hipLaunchKernelGGL(divide_kernel, dim3(grid), dim3(256), 0, c->stream, in, num_rows,
+                     static_cast<double*>(out->values) + out->offset, out->validity, out->offset, flag);
+  FPDB_HIP(hipGetLastError());
+  int zero = 0;
+  FPDB_HIP(hipMemcpyAsync(&zero, flag, 4, hipMemcpyDeviceToHost, c->stream));
+  FPDB_HIP(hipStreamSynchronize(c->stream));
+  FPDB_CHECK(!zero, FPDB_ERR_INVALID, "divide by zero");
+  return FPDB_OK;
+}
+
+}  // extern "C"

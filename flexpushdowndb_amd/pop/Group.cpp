@@ -33,22 +33,43 @@ Expected<void> GroupArrowKernel::makeOutputSchema(const Schema& in) {
   packed_ = keyTypes_.size() > 2;
   if (keyTypes_.size() == 2)
     for (auto t : keyTypes_) packed_ = packed_ || typeWidth((fpdb_type)t) > 4;
-  for (auto& f : aggregateFunctions_) {
+  specInputs_.clear();
+  firstSpecOf_.clear();
+  auto addSpec = [&](fpdb_agg_kind kind, const std::string& input) -> Expected<fpdb_agg_spec> {
     fpdb_agg_spec s{};
-    s.kind = f->kind();
-    if (!f->getInputColumnName().empty()) {
-      int i = in.getFieldIndex(f->getInputColumnName());
-      if (i < 0) return make_unexpected("Aggregate input '" + f->getInputColumnName() + "' not found");
+    s.kind = kind;
+    if (!input.empty()) {
+      int i = in.getFieldIndex(input);
+      if (i < 0) return make_unexpected("Aggregate input '" + input + "' not found");
       s.input_type = in.fields[i].type;
       s.input_nullable = in.fields[i].nullable ? 1 : 0;
     } else {
       s.input_type = FPDB_TYPE_INT64;
     }
     specs_.push_back(s);
+    specInputs_.push_back(input);
+    return s;
+  };
+  for (auto& f : aggregateFunctions_) {
+    firstSpecOf_.push_back((int)specs_.size());
     Field of;
     of.name = f->getOutputColumnName();
-    of.type = (fpdb_type)fpdb_groupby_output_type(s);
-    of.nullable = s.kind != FPDB_AGG_COUNT_ALL && s.kind != FPDB_AGG_COUNT_VALID;
+    if (f->getType() == AVG_REDUCE) {
+      // hash_sum of both intermediate columns (AvgReduce.cpp:84-106); the
+      // average is their quotient, float64 (AvgBase.cpp:71-98)
+      auto ar = std::static_pointer_cast<AvgReduce>(f);
+      auto s1 = addSpec(FPDB_AGG_SUM, ar->getIntermediateSumColumnName());
+      if (!s1) return make_unexpected(s1.error());
+      auto s2 = addSpec(FPDB_AGG_SUM, ar->getIntermediateCountColumnName());
+      if (!s2) return make_unexpected(s2.error());
+      of.type = FPDB_TYPE_FLOAT64;
+      of.nullable = true;
+    } else {
+      auto s = addSpec(f->kind(), f->getInputColumnName());
+      if (!s) return make_unexpected(s.error());
+      of.type = (fpdb_type)fpdb_groupby_output_type(*s);
+      of.nullable = s->kind != FPDB_AGG_COUNT_ALL && s->kind != FPDB_AGG_COUNT_VALID;
+    }
     out.fields.push_back(of);
   }
   outputSchema_ = out;
@@ -119,11 +140,11 @@ Expected<bool> GroupArrowKernel::widenKeyRanges(const TupleSet& ts) {
 Expected<void> GroupArrowKernel::consume(const std::shared_ptr<TupleSet>& ts) {
   std::vector<fpdb_column> keys, vals;
   for (auto& g : groupColumnNames_) keys.push_back(ts->column(ts->schema().getFieldIndex(g)).c());
-  for (auto& f : aggregateFunctions_) {
-    if (f->getInputColumnName().empty()) {
+  for (auto& input : specInputs_) {
+    if (input.empty()) {
       vals.push_back(fpdb_column{});
     } else {
-      vals.push_back(ts->column(ts->schema().getFieldIndex(f->getInputColumnName())).c());
+      vals.push_back(ts->column(ts->schema().getFieldIndex(input)).c());
     }
   }
   if (fpdb_groupby_consume(defaultContext(), table_, keys.data(), vals.data(), ts->numRows()) != FPDB_OK)
@@ -178,16 +199,34 @@ Expected<std::shared_ptr<TupleSet>> GroupArrowKernel::finalise() {
   fpdb_context* ctx = defaultContext();
   int64_t ng = 0;
   if (fpdb_groupby_num_groups(ctx, table_, &ng) != FPDB_OK) return make_unexpected(fpdb_last_error());
-  std::vector<Column> cols;
+  std::vector<Column> cols, aggCols;
   std::vector<fpdb_column> outKeys, outAggs;
   size_t nk = groupColumnNames_.size();
-  for (size_t i = 0; i < os.fields.size(); ++i) {
+  for (size_t i = 0; i < nk; ++i) {
     cols.push_back(Column::allocate(ctx, os.fields[i].type, ng, os.fields[i].nullable));
-    (i < nk ? outKeys : outAggs).push_back(cols.back().c());
+    outKeys.push_back(cols.back().c());
+  }
+  for (auto& s : specs_) {
+    const bool nullable = s.kind != FPDB_AGG_COUNT_ALL && s.kind != FPDB_AGG_COUNT_VALID;
+    aggCols.push_back(Column::allocate(ctx, (fpdb_type)fpdb_groupby_output_type(s), ng, nullable));
+    outAggs.push_back(aggCols.back().c());
   }
   int64_t written = 0;
   if (fpdb_groupby_finalize(ctx, table_, outKeys.data(), outAggs.data(), &written) != FPDB_OK)
     return make_unexpected(fpdb_last_error());
+  // [group columns..., aggregates in function order]; an AvgReduce divides its
+  // two sums (GroupArrowKernel::finalizeAvg, GroupArrowKernel.cpp:156-200)
+  for (size_t f = 0; f < aggregateFunctions_.size(); ++f) {
+    const int s = firstSpecOf_[f];
+    if (aggregateFunctions_[f]->getType() != AVG_REDUCE) {
+      cols.push_back(aggCols[s]);
+      continue;
+    }
+    Column avg = Column::allocate(ctx, FPDB_TYPE_FLOAT64, ng, true);
+    fpdb_column num = aggCols[s].c(), den = aggCols[s + 1].c(), o = avg.c();
+    if (fpdb_divide_to_float64(ctx, &num, &den, ng, &o) != FPDB_OK) return make_unexpected(fpdb_last_error());
+    cols.push_back(avg);
+  }
   return TupleSet::make(os, cols);
 }
 

@@ -10,7 +10,12 @@
 
 namespace fpdb::amd {
 
-enum AggregateFunctionType { SUM, COUNT, MIN_MAX, AVG };
+enum AggregateFunctionType { SUM, COUNT, MIN_MAX, AVG, AVG_REDUCE };
+
+// Intermediate column prefixes of a two-phase AVG
+// (fpdb-plan/include/fpdb/plan/prephysical/AggregatePrePFunction.h:25-26).
+constexpr const char* kAvgIntermediateSumPrefix = "AVG_INTERMEDIATE_SUM_";
+constexpr const char* kAvgIntermediateCountPrefix = "AVG_INTERMEDIATE_COUNT_";
 
 // An aggregate over one input column (the reference evaluates a gandiva
 // expression; here the expression is a column reference, empty for count(*)).
@@ -56,6 +61,25 @@ struct Avg : AggregateFunction {
   fpdb_agg_kind kind() const override { return FPDB_AGG_MEAN; }
 };
 
+// The reducing half of a two-phase AVG (AvgReduce.cpp:12-115): sums the
+// partial operators' intermediate sum and count columns per group and divides
+// them (AvgBase::finalize, AvgBase.cpp:71-98).  Its inputs are the columns
+// AVG_INTERMEDIATE_SUM_<out> and AVG_INTERMEDIATE_COUNT_<out>.
+struct AvgReduce : AggregateFunction {
+  explicit AvgReduce(std::string out) : AggregateFunction(AVG_REDUCE, std::move(out), "") {}
+  std::string getIntermediateSumColumnName() const { return kAvgIntermediateSumPrefix + outputColumnName_; }
+  std::string getIntermediateCountColumnName() const { return kAvgIntermediateCountPrefix + outputColumnName_; }
+  fpdb_agg_kind kind() const override { return FPDB_AGG_SUM; }
+};
+
+// The partial half of a two-phase AVG over `in` (PrePToPTransformerUtil.cpp:79-87):
+// a Sum into AVG_INTERMEDIATE_SUM_<out> and a Count into
+// AVG_INTERMEDIATE_COUNT_<out>, which the reducing AvgReduce(out) reads.
+inline std::vector<std::shared_ptr<AggregateFunction>> makeAvgPartial(const std::string& out, const std::string& in) {
+  return {std::make_shared<Sum>(kAvgIntermediateSumPrefix + out, in),
+          std::make_shared<Count>(kAvgIntermediateCountPrefix + out, in)};
+}
+
 class GroupArrowKernel {
  public:
   GroupArrowKernel(std::vector<std::string> groupColumnNames,
@@ -80,7 +104,11 @@ class GroupArrowKernel {
   std::vector<std::shared_ptr<AggregateFunction>> aggregateFunctions_;
   std::optional<Schema> outputSchema_;
   std::vector<int32_t> keyTypes_;
+  // one device aggregate per function, two for an AvgReduce (sum of the
+  // intermediate sums, then sum of the intermediate counts)
   std::vector<fpdb_agg_spec> specs_;
+  std::vector<std::string> specInputs_;        // input column of each device aggregate ("" for count(*))
+  std::vector<int> firstSpecOf_;               // per function: its first device aggregate
   std::vector<std::shared_ptr<TupleSet>> inputs_;   // kept to rebuild a table that overflowed
   fpdb_group_table* table_ = nullptr;
   int64_t maxGroups_ = 1 << 16;

@@ -504,6 +504,18 @@ fpdb_status fpdb_groupby_finalize(fpdb_context* ctx, fpdb_group_table* table,
                                   fpdb_column* out_keys, fpdb_column* out_aggs,
                                   int64_t* num_groups);
 
+/* Row-wise num / den as float64: an average from its intermediate sum and
+ * count columns, as AvgBase::finalize computes it for Avg and AvgReduce
+ * (fpdb-executor/src/physical/aggregate/function/AvgBase.cpp:71-98: arrow
+ * "divide", then a cast to float64).  Two integer columns divide as integers
+ * (truncating toward zero) before the cast; otherwise the quotient is taken in
+ * float64.  A null num or den gives a null; an integer division by zero fails
+ * with FPDB_ERR_INVALID, as arrow's integer "divide" does.  out is float64
+ * with a validity bitmap of ceil(n/32)*4 bytes at an offset that is a multiple
+ * of 32.  Synchronizes. */
+fpdb_status fpdb_divide_to_float64(fpdb_context* ctx, const fpdb_column* num, const fpdb_column* den,
+                                   int64_t num_rows, fpdb_column* out);
+
 /* ---- batched star join with predicate transfer ------------------------------ */
 
 /* One dimension of a star join: dimension keys (non-null int64), a filter

@@ -452,8 +452,9 @@ static void testWireForms() {
 static void testTwoPhaseGroupBy() {
   // the parallel group-by plan (PrePToPTransformer.cpp:434-560): a partial
   // GroupPOp per input partition, a ShufflePOp on the group key, a reducing
-  // GroupPOp per slot (Sum of sums, Count becomes a Sum of counts); the same
-  // groups as one GroupPOp over all rows
+  // GroupPOp per slot (Sum of sums, Count becomes a Sum of counts, Avg becomes
+  // a partial Sum + Count and an AvgReduce, PrePToPTransformerUtil.cpp:79-118);
+  // the same groups as one GroupPOp over all rows
   const int64_t n = 20000;
   const int parts = 4, slots = 3;
   std::vector<int64_t> k(n), v(n);
@@ -474,9 +475,10 @@ static void testTwoPhaseGroupBy() {
   std::vector<std::shared_ptr<GroupPOp>> reducers;
   for (int r = 0; r < slots; ++r) {
     std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum_v", "sum_v"),
-                                                          std::make_shared<Sum>("cnt", "cnt")};
+                                                          std::make_shared<Sum>("cnt", "cnt"),
+                                                          std::make_shared<AvgReduce>("avg_v")};
     reducers.push_back(std::make_shared<GroupPOp>("reduce" + std::to_string(r),
-                                                  std::vector<std::string>{"k", "sum_v", "cnt"}, 0,
+                                                  std::vector<std::string>{"k", "sum_v", "cnt", "avg_v"}, 0,
                                                   std::vector<std::string>{"k"}, fns));
     ex.put(reducers.back());
     connect(reducers.back(), collate);
@@ -488,9 +490,11 @@ static void testTwoPhaseGroupBy() {
     auto src = std::make_shared<SourcePOp>("scan" + std::to_string(p), std::vector<std::shared_ptr<TupleSet>>{ts});
     std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("sum_v", "v"),
                                                           std::make_shared<Count>("cnt")};
-    auto partial = std::make_shared<GroupPOp>("partial" + std::to_string(p),
-                                              std::vector<std::string>{"k", "sum_v", "cnt"}, 0,
-                                              std::vector<std::string>{"k"}, fns);
+    for (auto& f : makeAvgPartial("avg_v", "v")) fns.push_back(f);
+    auto partial = std::make_shared<GroupPOp>(
+        "partial" + std::to_string(p),
+        std::vector<std::string>{"k", "sum_v", "cnt", "AVG_INTERMEDIATE_SUM_avg_v", "AVG_INTERMEDIATE_COUNT_avg_v"}, 0,
+        std::vector<std::string>{"k"}, fns);
     auto sh = std::make_shared<ShufflePOp>("shuffle" + std::to_string(p), std::vector<std::string>{}, 0,
                                            std::vector<std::string>{"k"});
     for (auto op : std::vector<std::shared_ptr<PhysicalOp>>{src, partial, sh}) ex.put(op);
@@ -504,9 +508,15 @@ static void testTwoPhaseGroupBy() {
   auto out = collate->tupleSet().value();
   auto h = out->toHost();
   std::map<int64_t, std::pair<int64_t, int64_t>> got;
+  CHECK(out->schema().fields.size() == 4 && out->schema().fields[3].name == "avg_v" &&
+        out->schema().fields[3].type == FPDB_TYPE_FLOAT64);
   for (int64_t i = 0; i < out->numRows(); ++i) {
-    CHECK(got.count(h[0].int64At(i)) == 0);     // each key reduced in exactly one slot
-    got[h[0].int64At(i)] = {h[1].int64At(i), h[2].int64At(i)};
+    const int64_t key = h[0].int64At(i);
+    CHECK(got.count(key) == 0);     // each key reduced in exactly one slot
+    got[key] = {h[1].int64At(i), h[2].int64At(i)};
+    // AVG of int64 values: integer quotient of the sums, then float64
+    const auto& w = want[key];
+    CHECK(h[3].isValid(i) && h[3].doubleAt(i) == (double)(w.first / w.second));
   }
   CHECK(got == want);
 }

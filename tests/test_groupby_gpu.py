@@ -64,10 +64,10 @@ def test_skewed_layouts_vs_oracle(ctx, layout):
         qty, price, disc = w["qty"][order], w["price"][order], w["disc"][order]
     else:
         qty, price, disc = w["qty"], w["price"], w["disc"]
-        # the sampler reads 64 runs of 1,024 rows at (n - 1024) * j / 63
+        # the sampler reads 32 runs of 1,024 rows at (n - 1024) * j / 31
         gap = np.ones(n, bool)
-        for j in range(64):
-            s = (n - 1024) * j // 63
+        for j in range(32):
+            s = (n - 1024) * j // 31
             gap[s:s + 1024] = False
         pick = np.flatnonzero(gap)[::3]
         key[pick] = 123_456_789
@@ -168,3 +168,36 @@ def test_groupby_full_size_exact(ctx):
         assert a[2][i] == int(w["price"][m].sum())
         fs = math.fsum(w["disc"][m])
         assert abs(a[3][i] - fs) <= REL_TOL * abs(fs)
+
+
+def test_divide_to_float64_as_avg_finalize(ctx):
+    """fpdb_divide_to_float64, the average of AvgBase::finalize: integer sums
+    divide as integers (truncating toward zero) before the cast, float sums in
+    float64, a null on either side gives a null, and an integer division by
+    zero is an error."""
+    import ctypes
+    rng = np.random.default_rng(12)
+    n = 10_007
+    num = rng.integers(-10 ** 12, 10 ** 12, n).astype(np.int64)
+    den = rng.integers(1, 1000, n).astype(np.int64)
+    nvalid = rng.random(n) < 0.9
+    out = DeviceColumn.empty(ctx, n, np.float64, nullable=True)
+
+    def divide(a, b):
+        C.check(C.lib().fpdb_divide_to_float64(ctx.handle, ctypes.byref(a.c()), ctypes.byref(b.c()), n,
+                                               ctypes.byref(out.c())))
+
+    divide(DeviceColumn.from_numpy(ctx, num, nvalid), DeviceColumn.from_numpy(ctx, den))
+    # truncation toward zero: sign times the quotient of the magnitudes, in Python integers
+    want = np.array([float((abs(int(a)) // abs(int(b))) * (1 if (a >= 0) == (b > 0) else -1))
+                     for a, b in zip(num, den)])
+    assert np.array_equal(out.valid_mask(), nvalid)
+    assert np.array_equal(out.numpy()[nvalid], want[nvalid])
+    f = rng.random(n) * 100
+    divide(DeviceColumn.from_numpy(ctx, f), DeviceColumn.from_numpy(ctx, den))
+    assert np.array_equal(out.numpy(), f / den) and out.valid_mask().all()
+    zero = den.copy()
+    zero[17] = 0
+    with pytest.raises(C.FpdbError) as e:
+        divide(DeviceColumn.from_numpy(ctx, num), DeviceColumn.from_numpy(ctx, zero))
+    assert e.value.code == C.FPDB_ERR_INVALID
