@@ -1,5 +1,6 @@
 #include "BloomFilter.h"
 
+#include <cstdio>
 #include <cstring>
 #include <sstream>
 
@@ -65,12 +66,49 @@ std::vector<uint8_t> ArrowBloomFilter::masksBytes() const {
   return m;
 }
 
+static std::string jsonString(const std::string& v) {
+  std::string o = "\"";
+  for (char c : v) {
+    if (c == '"' || c == '\\') o += '\\';
+    o += c;
+  }
+  return o + "\"";
+}
+
+// ArrowBloomFilter::toJson (ArrowBloomFilter.cpp:118-125); the blocked
+// filter's own object carries log_num_blocks (arrow BlockedBloomFilter::toJson,
+// fpdb-tuple/src/arrow/exec/BloomFilter.cpp:289-293).
 std::string ArrowBloomFilter::toJson() const {
   std::ostringstream o;
   o << "{\"type\":" << (int)type_ << ",\"capacity\":" << capacity_ << ",\"columnNames\":[";
-  for (size_t i = 0; i < columnNames_.size(); ++i) o << (i ? "," : "") << "\"" << columnNames_[i] << "\"";
-  o << "],\"blockedBloomFilter\":{\"logNumBlocks\":" << logNumBlocks_ << "}}";
+  for (size_t i = 0; i < columnNames_.size(); ++i) o << (i ? "," : "") << jsonString(columnNames_[i]);
+  o << "],\"blockedBloomFilter\":{\"log_num_blocks\":" << logNumBlocks_ << "}}";
   return o.str();
+}
+
+Expected<std::shared_ptr<ArrowBloomFilter>> ArrowBloomFilter::fromJson(const JsonValue& jObj) {
+  auto capacity = jsonInt(jObj, "capacity", "ArrowBloomFilter");
+  if (!capacity) return make_unexpected(capacity.error());
+  if (!jObj.contains("columnNames") || jObj["columnNames"].kind != JsonValue::ARRAY)
+    return make_unexpected("ColumnNames not specified in ArrowBloomFilter JSON");
+  std::vector<std::string> names;
+  for (auto& v : jObj["columnNames"].items) {
+    if (v.kind != JsonValue::STRING) return make_unexpected("ColumnNames in ArrowBloomFilter JSON must be strings");
+    names.push_back(v.s);
+  }
+  if (!jObj.contains("blockedBloomFilter"))
+    return make_unexpected("BlockedBloomFilter not specified in ArrowBloomFilter JSON");
+  auto lnb = jsonInt(jObj["blockedBloomFilter"], "log_num_blocks", "BlockedBloomFilter");
+  if (!lnb) return make_unexpected(lnb.error());
+  if (*lnb < 0 || *lnb > 40) return make_unexpected("log_num_blocks out of range in BlockedBloomFilter JSON");
+  auto bf = make(*capacity, names);
+  if (bf->valid_ && *lnb != bf->logNumBlocks_) {
+    fpdb_context* ctx = defaultContext();
+    bf->blocks_ = std::make_shared<DeviceBuffer>(ctx, (uint64_t)8 << *lnb);
+    checkStatus(fpdb_fill(ctx, bf->blocks_->data(), 0, bf->blocks_->size()), "fpdb_fill");
+  }
+  bf->logNumBlocks_ = (int)*lnb;
+  return bf;
 }
 
 Expected<std::vector<BitmapRecordBatch>> ArrowBloomFilter::makeBitmapRecordBatches() const {
@@ -190,10 +228,83 @@ std::shared_ptr<BloomFilter> BloomFilter::make(int64_t capacity, double falsePos
   return std::make_shared<BloomFilter>(capacity, falsePositiveRate);
 }
 
+std::shared_ptr<BloomFilter> BloomFilter::make(int64_t capacity, double falsePositiveRate, bool valid,
+                                               int64_t numHashFunctions, int64_t numBits,
+                                               std::vector<fpdb_universal_hash> hashFunctions) {
+  auto bf = std::make_shared<BloomFilter>(capacity, falsePositiveRate);
+  bf->valid_ = valid;
+  bf->numHashFunctions_ = numHashFunctions;
+  bf->numBits_ = numBits;
+  bf->hashFunctions_ = std::move(hashFunctions);
+  fpdb_context* ctx = defaultContext();
+  bf->words_ = std::make_shared<DeviceBuffer>(ctx, (uint64_t)(numBits / 64 + 1) * 8);
+  checkStatus(fpdb_fill(ctx, bf->words_->data(), 0, bf->words_->size()), "fpdb_fill");
+  return bf;
+}
+
+Expected<std::shared_ptr<BloomFilter>> BloomFilter::fromJson(const JsonValue& jObj) {
+  const std::string what = "bloom filter";
+  auto capacity = jsonInt(jObj, "capacity", what);
+  if (!capacity) return make_unexpected(capacity.error());
+  auto fpr = jsonDouble(jObj, "falsePositiveRate", what);
+  if (!fpr) return make_unexpected(fpr.error());
+  auto valid = jsonBool(jObj, "valid", what);
+  if (!valid) return make_unexpected(valid.error());
+  auto k = jsonInt(jObj, "numHashFunctions", what);
+  if (!k) return make_unexpected(k.error());
+  auto numBits = jsonInt(jObj, "numBits", what);
+  if (!numBits) return make_unexpected(numBits.error());
+  if (*numBits < 0 || *numBits > ((int64_t)1 << 40)) return make_unexpected("numBits out of range in bloom filter JSON");
+  if (!jObj.contains("hashFunctions") || jObj["hashFunctions"].kind != JsonValue::ARRAY)
+    return make_unexpected("HashFunctions not specified in bloom filter JSON");
+  std::vector<fpdb_universal_hash> hfs;
+  for (auto& h : jObj["hashFunctions"].items) {
+    // UniversalHashFunction::fromJson (UniversalHashFunction.cpp:68-92)
+    fpdb_universal_hash u{};
+    int64_t* dst[4] = {&u.a, &u.b, &u.m, &u.p};
+    const char* keys[4] = {"a", "b", "m", "p"};
+    for (int j = 0; j < 4; ++j) {
+      auto v = jsonInt(h, keys[j], "universal hash function");
+      if (!v) return make_unexpected(v.error());
+      *dst[j] = *v;
+    }
+    if (u.p <= 0 || u.m <= 0 || u.m > *numBits)
+      return make_unexpected("universal hash function needs p > 0 and 0 < m <= numBits");
+    hfs.push_back(u);
+  }
+  return make(*capacity, *fpr, *valid, *k, *numBits, std::move(hfs));
+}
+
+Expected<std::shared_ptr<BloomFilterBase>> BloomFilterBase::fromJson(const JsonValue& jObj) {
+  if (!jObj.contains("type")) return make_unexpected("Type not specified in BloomFilterBase JSON");
+  auto type = jsonInt(jObj, "type", "BloomFilterBase");
+  if (!type) return make_unexpected(type.error());
+  switch (*type) {
+    case BLOOM_FILTER: {
+      auto r = BloomFilter::fromJson(jObj);
+      if (!r) return make_unexpected(r.error());
+      return std::shared_ptr<BloomFilterBase>(*r);
+    }
+    case ARROW_BLOOM_FILTER: {
+      auto r = ArrowBloomFilter::fromJson(jObj);
+      if (!r) return make_unexpected(r.error());
+      return std::shared_ptr<BloomFilterBase>(*r);
+    }
+    default: return make_unexpected("Unknown bloom filter type: " + std::to_string(*type));
+  }
+}
+
+Expected<std::shared_ptr<BloomFilterBase>> BloomFilterBase::fromJson(const std::string& text) {
+  auto j = parseJson(text);
+  if (!j) return make_unexpected(j.error());
+  return fromJson(*j);
+}
+
 void BloomFilter::init(uint32_t seed) {
   int64_t k = 0, m = 0;
   checkStatus(fpdb_bloom_legacy_params(capacity_, fpr_, &k, &m), "fpdb_bloom_legacy_params");
   numBits_ = m;
+  numHashFunctions_ = k;
   hashFunctions_.clear();
   if (capacity_ > 0) {   // no hash functions for an empty input (BloomFilter.cpp:209-213)
     hashFunctions_.resize((size_t)k);
@@ -253,8 +364,10 @@ Expected<void> BloomFilter::saveBitmapRecordBatches(const std::vector<BitmapReco
 
 std::string BloomFilter::toJson() const {
   std::ostringstream o;
-  o << "{\"type\":" << (int)type_ << ",\"capacity\":" << capacity_ << ",\"falsePositiveRate\":" << fpr_
-    << ",\"valid\":" << (valid_ ? "true" : "false") << ",\"numHashFunctions\":" << hashFunctions_.size()
+  char fpr[32];
+  std::snprintf(fpr, sizeof fpr, "%.17g", fpr_);
+  o << "{\"type\":" << (int)type_ << ",\"capacity\":" << capacity_ << ",\"falsePositiveRate\":" << fpr
+    << ",\"valid\":" << (valid_ ? "true" : "false") << ",\"numHashFunctions\":" << numHashFunctions_
     << ",\"numBits\":" << numBits_ << ",\"hashFunctions\":[";
   for (size_t i = 0; i < hashFunctions_.size(); ++i) {
     const auto& h = hashFunctions_[i];

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "Expected.h"
+#include "Json.h"
 #include "TupleSet.h"
 
 namespace fpdb::amd {
@@ -41,6 +42,11 @@ class BloomFilterBase {
   int64_t capacity() const { return capacity_; }
   bool valid() const { return valid_; }
   virtual std::string toJson() const = 0;
+  // The filter a toJson text describes, of the class its "type" names
+  // (BloomFilterBase::fromJson, BloomFilterBase.cpp:25-43).  Its bit array is
+  // zero until saveBitmapRecordBatches fills it: the bits travel separately.
+  static Expected<std::shared_ptr<BloomFilterBase>> fromJson(const std::string& text);
+  static Expected<std::shared_ptr<BloomFilterBase>> fromJson(const JsonValue& jObj);
   // The filter's bit array as record batches, and back
   // (BloomFilterBase.h:32-33; ArrowBloomFilter.cpp:77-116, BloomFilter.cpp:123-144).
   virtual Expected<std::vector<BitmapRecordBatch>> makeBitmapRecordBatches() const = 0;
@@ -71,6 +77,8 @@ class ArrowBloomFilter : public BloomFilterBase {
   // makeBitmapRecordBatches (ArrowBloomFilter.cpp:77-116), as raw bytes.
   std::vector<uint8_t> masksBytes() const;
   std::string toJson() const override;
+  // ArrowBloomFilter::fromJson (ArrowBloomFilter.cpp:127-149)
+  static Expected<std::shared_ptr<ArrowBloomFilter>> fromJson(const JsonValue& jObj);
   // Two uint8 batches: the 136 mask bytes, then the blocks.
   Expected<std::vector<BitmapRecordBatch>> makeBitmapRecordBatches() const override;
   // Takes the blocks of a received filter (its size may differ from this
@@ -87,6 +95,13 @@ class BloomFilter : public BloomFilterBase {
  public:
   BloomFilter(int64_t capacity, double falsePositiveRate);
   static std::shared_ptr<BloomFilter> make(int64_t capacity, double falsePositiveRate);
+  // A filter with the given parameters and a zeroed bit array
+  // (BloomFilter::make, BloomFilter.cpp:32-53).
+  static std::shared_ptr<BloomFilter> make(int64_t capacity, double falsePositiveRate, bool valid,
+                                           int64_t numHashFunctions, int64_t numBits,
+                                           std::vector<fpdb_universal_hash> hashFunctions);
+  // BloomFilter::fromJson (BloomFilter.cpp:166-205)
+  static Expected<std::shared_ptr<BloomFilter>> fromJson(const JsonValue& jObj);
 
   // k, m, hash functions and a zeroed bit array (BloomFilter::init, BloomFilter.cpp:55-60).
   void init(uint32_t seed);
@@ -95,7 +110,8 @@ class BloomFilter : public BloomFilterBase {
   Expected<void> merge(const std::shared_ptr<BloomFilter>& other);
 
   double falsePositiveRate() const { return fpr_; }
-  int64_t numHashFunctions() const { return (int64_t)hashFunctions_.size(); }
+  // k; an empty input keeps k but has no hash functions (BloomFilter.cpp:209-213)
+  int64_t numHashFunctions() const { return numHashFunctions_; }
   int64_t numBits() const { return numBits_; }
   const std::vector<fpdb_universal_hash>& hashFunctions() const { return hashFunctions_; }
   const int64_t* deviceWords() const { return static_cast<const int64_t*>(words_->data()); }
@@ -107,6 +123,7 @@ class BloomFilter : public BloomFilterBase {
 
  private:
   double fpr_;
+  int64_t numHashFunctions_ = 0;
   int64_t numBits_ = 0;
   std::vector<fpdb_universal_hash> hashFunctions_;
   std::shared_ptr<DeviceBuffer> words_;

@@ -83,7 +83,7 @@ Expected<std::shared_ptr<TupleSet>> BloomFilterUseKernel::filter(const std::shar
     std::vector<fpdb_column> keys;
     for (int i : keyIndex) keys.push_back(in[i]);
     DeviceBuffer bitmap(ctx, (uint64_t)((n + 63) / 64) * 8);
-    fpdb_status s = fpdb_bloom_legacy_find(ctx, bf->hashFunctions().data(), (int)bf->numHashFunctions(),
+    fpdb_status s = fpdb_bloom_legacy_find(ctx, bf->hashFunctions().data(), (int)bf->hashFunctions().size(),
                                            bf->capacity(), bf->numBits(), bf->deviceWords(), keys.data(),
                                            (int)keys.size(), n, static_cast<uint8_t*>(bitmap.data()), nullptr);
     if (s == FPDB_OK)

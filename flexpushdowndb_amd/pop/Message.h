@@ -1,6 +1,6 @@
 // Messages between physical operators (fpdb-executor/include/fpdb/executor/message/
 // MessageType.h:10-35, TupleSetMessage.h:19, BloomFilterMessage.h:18,
-// CompleteMessage, ErrorMessage), delivered in Envelopes.
+// CompleteMessage, ErrorMessage, TupleSetIndexMessage.h:17), delivered in Envelopes.
 #pragma once
 
 #include <memory>
@@ -10,11 +10,12 @@
 
 #include "BloomFilter.h"
 #include "TupleSet.h"
+#include "TupleSetIndex.h"
 
 namespace fpdb::amd {
 
 enum class MessageType {
-  START, TUPLESET, BLOOM_FILTER, COMPLETE, ERROR,
+  START, TUPLESET, TUPLESET_INDEX, BLOOM_FILTER, COMPLETE, ERROR,
   // segment cache protocol (fpdb-executor/include/fpdb/executor/message/cache/*, ScanMessage.h)
   SCAN, LOAD_REQUEST, LOAD_RESPONSE, STORE_REQUEST, CACHE_METRICS
 };
@@ -60,6 +61,18 @@ class TupleSetMessage : public Message {
 
  private:
   std::shared_ptr<TupleSet> tuples_;
+};
+
+// A hash join build side on its way from HashJoinBuildPOp to HashJoinProbePOp
+// (fpdb-executor/include/fpdb/executor/message/TupleSetIndexMessage.h:17-31).
+class TupleSetIndexMessage : public Message {
+ public:
+  TupleSetIndexMessage(std::shared_ptr<TupleSetIndex> index, std::string sender)
+      : Message(MessageType::TUPLESET_INDEX, std::move(sender)), tupleSetIndex_(std::move(index)) {}
+  const std::shared_ptr<TupleSetIndex>& getTupleSetIndex() const { return tupleSetIndex_; }
+
+ private:
+  std::shared_ptr<TupleSetIndex> tupleSetIndex_;
 };
 
 class BloomFilterMessage : public Message {

@@ -9,6 +9,7 @@ std::string Message::getTypeString() const {
   switch (type_) {
     case MessageType::START: return "StartMessage";
     case MessageType::TUPLESET: return "TupleSetMessage";
+    case MessageType::TUPLESET_INDEX: return "TupleSetIndexMessage";
     case MessageType::BLOOM_FILTER: return "BloomFilterMessage";
     case MessageType::COMPLETE: return "CompleteMessage";
     case MessageType::ERROR: return "ErrorMessage";

@@ -19,7 +19,7 @@
 namespace fpdb::amd {
 
 enum class POpType {
-  HASH_JOIN_ARROW, BLOOM_FILTER_CREATE, BLOOM_FILTER_USE, GROUP, SHUFFLE, SOURCE, COLLATE, FILTER,
+  HASH_JOIN_ARROW, HASH_JOIN_BUILD, HASH_JOIN_PROBE, BLOOM_FILTER_CREATE, BLOOM_FILTER_USE, GROUP, SHUFFLE, SOURCE, COLLATE, FILTER,
   CACHE_LOAD, SEGMENT_CACHE, FILE_SCAN, MERGE
 };
 

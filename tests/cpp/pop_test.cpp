@@ -21,6 +21,7 @@
 #include "Globals.h"
 #include "Group.h"
 #include "HashJoin.h"
+#include "HashJoinBuildProbe.h"
 #include "Plumbing.h"
 #include "PredTrans.h"
 #include "Shuffle.h"
@@ -163,6 +164,135 @@ static void testOuterSemiJoins() {
   if (rsemi) {
     CHECK(rsemi->numRows() == 2 && rsemi->schema().fields[0].name == "aa");
   }
+}
+
+// HashJoinBuildPOp -> TupleSetIndexMessage -> HashJoinProbePOp.  With
+// `probeLate` the probe rows pass one extra hop, so they reach the probe
+// operator after the build indexes; otherwise they arrive first.
+static std::shared_ptr<TupleSet> runBuildProbeJoin(JoinType jt, const std::vector<std::shared_ptr<TupleSet>>& build,
+                                                   const std::vector<std::shared_ptr<TupleSet>>& probe,
+                                                   std::vector<std::string> project, bool probeLate) {
+  Executor ex;
+  auto b = std::make_shared<SourcePOp>("scan_b", build);
+  auto a = std::make_shared<SourcePOp>("scan_a", probe);
+  auto hb = std::make_shared<HashJoinBuildPOp>("build", std::vector<std::string>{}, 0, std::vector<std::string>{"ba"});
+  auto hp = std::make_shared<HashJoinProbePOp>("probe", project, 0, HashJoinPredicate{{"ba"}, {"aa"}}, jt);
+  auto collate = std::make_shared<CollatePOp>("collate");
+  ex.put(b);
+  ex.put(a);
+  ex.put(hb);
+  ex.put(hp);
+  ex.put(collate);
+  connect(b, hb);
+  connect(hb, hp);
+  if (probeLate) {
+    auto hop = std::make_shared<FilterPOp>("hop", std::vector<std::string>{}, 0, "aa", FPDB_CMP_GE, INT64_MIN);
+    ex.put(hop);
+    connect(a, hop);
+    connect(hop, hp);
+  } else {
+    connect(a, hp);
+  }
+  connect(hp, collate);
+  auto r = ex.execute();
+  if (!r) {
+    std::printf("  execute failed: %s\n", r.error().c_str());
+    ++g_failures;
+    return nullptr;
+  }
+  return collate->tupleSet().value();
+}
+
+static void testBuildProbeJoin() {
+  // JoinTest.cpp:82-123 through the two-operator join: build (b) columns first
+  std::vector<std::string> all = {"aa", "ab", "ac", "ba", "bb", "bc"};
+  auto out = runBuildProbeJoin(INNER, {table3("b", kB)}, {table3("a", kA)}, all, false);
+  if (out) {
+    CHECK(out->numRows() == 4 && out->numColumns() == 6);
+    CHECK(out->schema().fields[0].name == "ba" && out->schema().fields[3].name == "aa");
+    auto rows = rowsOf(out);
+    std::sort(rows.begin(), rows.end());
+    std::vector<std::vector<int64_t>> want = {{10, 23, 26, 10, 13, 16}, {10, 23, 26, 10, 14, 17},
+                                              {10, 25, 28, 10, 13, 16}, {10, 25, 28, 10, 14, 17}};
+    CHECK(rows == want);
+  }
+  auto left = runBuildProbeJoin(LEFT, {table3("b", kB)}, {table3("a", kA)}, all, false);
+  if (left) {
+    int found = 0;
+    for (auto& r : rowsOf(left)) found += (r[0] == 11 && r[3] == INT64_MIN);
+    CHECK(left->numRows() == 5 && found == 1);
+  }
+  auto right = runBuildProbeJoin(RIGHT, {table3("b", kB)}, {table3("a", kA)}, all, false);
+  if (right) {
+    int found = 0;
+    for (auto& r : rowsOf(right)) found += (r[0] == INT64_MIN && r[3] == 12);
+    CHECK(right->numRows() == 5 && found == 1);
+  }
+  auto full = runBuildProbeJoin(FULL, {table3("b", kB)}, {table3("a", kA)}, all, false);
+  if (full) CHECK(full->numRows() == 6);
+  auto semi = runBuildProbeJoin(LEFT_SEMI, {table3("b", kB)}, {table3("a", kA)}, {"ba", "bb", "bc"}, false);
+  if (semi) {
+    CHECK(semi->numRows() == 2 && semi->numColumns() == 3);
+    for (auto& r : rowsOf(semi)) CHECK(r[0] == 10);
+  }
+  // an empty probe side: the output schema still arrives, with no rows
+  auto none = runBuildProbeJoin(INNER, {table3("b", kB)}, {table3("a", {})}, all, false);
+  if (none) CHECK(none->numRows() == 0 && none->numColumns() == 6 && none->schema().fields[0].name == "ba");
+
+  // Several build indexes (DefaultBufferSize rows each) and probe batches with
+  // duplicate keys, both arrival orders, every join type: the rows as a
+  // multiset equal a nested-loop join.
+  int64_t savedBuffer = DefaultBufferSize;
+  DefaultBufferSize = 7;
+  uint64_t s = 12345;
+  auto rnd = [&s](int64_t m) {
+    s = s * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (int64_t)((s >> 33) % (uint64_t)m);
+  };
+  std::vector<std::vector<int64_t>> brows, prows;
+  std::vector<std::shared_ptr<TupleSet>> bsets, psets;
+  for (int t = 0; t < 5; ++t) {
+    std::vector<std::vector<int64_t>> part;
+    for (int i = 0; i < 5; ++i) part.push_back({rnd(12), (int64_t)brows.size() + (int64_t)part.size(), rnd(100)});
+    brows.insert(brows.end(), part.begin(), part.end());
+    bsets.push_back(table3("b", part));
+  }
+  for (int t = 0; t < 4; ++t) {
+    std::vector<std::vector<int64_t>> part;
+    for (int i = 0; i < 6; ++i) part.push_back({rnd(15), 1000 + (int64_t)prows.size() + (int64_t)part.size(), rnd(100)});
+    prows.insert(prows.end(), part.begin(), part.end());
+    psets.push_back(table3("a", part));
+  }
+  const int64_t N = INT64_MIN;
+  for (JoinType jt : {INNER, LEFT, RIGHT, FULL, LEFT_SEMI}) {
+    std::vector<std::vector<int64_t>> want;
+    std::vector<bool> pm(prows.size(), false);
+    for (auto& br : brows) {
+      bool matched = false;
+      for (size_t j = 0; j < prows.size(); ++j) {
+        if (br[0] != prows[j][0]) continue;
+        matched = pm[j] = true;
+        if (jt != LEFT_SEMI) want.push_back({br[0], br[1], br[2], prows[j][0], prows[j][1], prows[j][2]});
+      }
+      if (jt == LEFT_SEMI && matched) want.push_back(br);
+      if ((jt == LEFT || jt == FULL) && !matched) want.push_back({br[0], br[1], br[2], N, N, N});
+    }
+    if (jt == RIGHT || jt == FULL)
+      for (size_t j = 0; j < prows.size(); ++j)
+        if (!pm[j]) want.push_back({N, N, N, prows[j][0], prows[j][1], prows[j][2]});
+    std::sort(want.begin(), want.end());
+    for (bool late : {false, true}) {
+      auto got = runBuildProbeJoin(jt, bsets, psets, jt == LEFT_SEMI ? std::vector<std::string>{"ba", "bb", "bc"} : all,
+                                   late);
+      if (!got) continue;
+      auto rows = rowsOf(got);
+      std::sort(rows.begin(), rows.end());
+      if (rows != want) std::printf("  join type %d, probe late %d: %zu rows, want %zu\n", (int)jt, (int)late,
+                                    rows.size(), want.size());
+      CHECK(rows == want);
+    }
+  }
+  DefaultBufferSize = savedBuffer;
 }
 
 static void testBloomJoin(bool arrowImpl) {
@@ -408,6 +538,45 @@ static void testWireForms() {
   auto lg = BloomFilter::make(5000, 0.01);
   lg->init(11);
   CHECK(lg->saveBitmapRecordBatches(*lb).has_value() && lg->getBitArray() == lf->getBitArray());
+
+  // fromJson of each filter's toJson, then its bits from the record batches:
+  // the same description, the same bits (BloomFilterBase::fromJson,
+  // BloomFilterBase.cpp:25-43; the bits travel separately, BloomFilter.cpp:158)
+  {
+    auto lj = BloomFilterBase::fromJson(lf->toJson());
+    CHECK(lj.has_value());
+    if (lj) {
+      CHECK((*lj)->getType() == BLOOM_FILTER && (*lj)->toJson() == lf->toJson());
+      auto lr = std::static_pointer_cast<BloomFilter>(*lj);
+      CHECK(lr->numBits() == lf->numBits() && lr->numHashFunctions() == lf->numHashFunctions() &&
+            lr->falsePositiveRate() == 0.01);
+      CHECK(lr->saveBitmapRecordBatches(*lb).has_value() && lr->getBitArray() == lf->getBitArray());
+    }
+    auto aj = BloomFilterBase::fromJson(f->toJson());
+    CHECK(aj.has_value());
+    if (aj) {
+      CHECK((*aj)->getType() == ARROW_BLOOM_FILTER && (*aj)->toJson() == f->toJson());
+      auto ar = std::static_pointer_cast<ArrowBloomFilter>(*aj);
+      CHECK(ar->columnNames() == std::vector<std::string>{"k"} && ar->logNumBlocks() == f->logNumBlocks());
+      CHECK(ar->saveBitmapRecordBatches(*batches).has_value() && ar->blocksToHost() == f->blocksToHost());
+    }
+    // the reference's field names, written by hand
+    auto hand = BloomFilterBase::fromJson(
+        "{\"type\": 0, \"capacity\": 10, \"falsePositiveRate\": 0.5, \"valid\": true, \"numHashFunctions\": 1,"
+        " \"numBits\": 15, \"hashFunctions\": [{\"a\": 3, \"b\": 4, \"m\": 15, \"p\": 17}]}");
+    CHECK(hand.has_value() && std::static_pointer_cast<BloomFilter>(*hand)->hashFunctions().size() == 1);
+    CHECK(BloomFilterBase::fromJson("{\"type\": 1, \"capacity\": 10, \"columnNames\": [\"x\"],"
+                                    " \"blockedBloomFilter\": {\"log_num_blocks\": 3}}")
+              .has_value());
+    CHECK(!BloomFilterBase::fromJson("{\"capacity\": 10}").has_value());                  // no type
+    CHECK(!BloomFilterBase::fromJson("{\"type\": 7}").has_value());                       // unknown type
+    CHECK(!BloomFilterBase::fromJson("{\"type\": 0, \"capacity\": 10}").has_value());     // fields missing
+    CHECK(!BloomFilterBase::fromJson("{\"type\": 0, \"capacity\": 10, \"falsePositiveRate\": 0.5, \"valid\": true,"
+                                     " \"numHashFunctions\": 1, \"numBits\": 15,"
+                                     " \"hashFunctions\": [{\"a\": 3, \"b\": 4, \"m\": 99, \"p\": 101}]}")
+               .has_value());                                                             // m beyond the bits
+    CHECK(!BloomFilterBase::fromJson("{\"type\": 1, \"capacity\": 10").has_value());      // truncated text
+  }
 
   // a filter's selection over three batches as bitmap words, then per batch back
   fpdb_context* ctx = defaultContext();
@@ -869,6 +1038,7 @@ int main(int argc, char** argv) {
   std::vector<std::pair<std::string, std::function<void()>>> tests = {
       {"join", testJoin},
       {"outer_semi_joins", testOuterSemiJoins},
+      {"build_probe_join", testBuildProbeJoin},
       {"bloom_join_blocked", [] { testBloomJoin(true); }},
       {"bloom_join_legacy", [] { testBloomJoin(false); }},
       {"group", testGroup},
