@@ -22,6 +22,7 @@ FPDB_ERR_INTERNAL = 5
 TYPE_INT8, TYPE_INT16, TYPE_INT32, TYPE_INT64 = 1, 2, 3, 4
 TYPE_UINT8, TYPE_UINT16, TYPE_UINT32, TYPE_UINT64 = 5, 6, 7, 8
 TYPE_FLOAT32, TYPE_FLOAT64, TYPE_DATE32, TYPE_DATE64 = 9, 10, 11, 12
+TYPE_UTF8 = 13
 
 NP_TO_TYPE = {
     np.dtype(np.int8): TYPE_INT8, np.dtype(np.int16): TYPE_INT16,
@@ -41,7 +42,8 @@ AGG_COUNT_ALL, AGG_COUNT_VALID, AGG_SUM, AGG_MIN, AGG_MAX = range(5)
 class Column(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("length", ctypes.c_int64), ("offset", ctypes.c_int64),
-                ("values", ctypes.c_void_p), ("validity", ctypes.c_void_p)]
+                ("values", ctypes.c_void_p), ("validity", ctypes.c_void_p),
+                ("data", ctypes.c_void_p), ("data_bytes", ctypes.c_int64)]
 
 
 class UniversalHash(ctypes.Structure):
@@ -193,6 +195,16 @@ SIGNATURES = [
     ("fpdb_serve_status", _I32, [_P, _I64, ctypes.POINTER(_I)]),
     ("fpdb_serve_result", _I32, [_P, _I64, _P, _P, _P, _P, _I64, _PI64]),
     ("fpdb_serve_run_trace", _I32, [_P, ctypes.POINTER(ServeQuery), _PI64, _I64, _PI64, _PI64, _PI64]),
+    ("fpdb_utf8_take_bytes", _I32, [_P, _PCOL, _PCOL, _I64, _PI64]),
+    ("fpdb_utf8_take", _I32, [_P, _PCOL, _PCOL, _I64, _PCOL]),
+    ("fpdb_iota_int64", _I32, [_P, _P, _I64, _I64]),
+    ("fpdb_compare_bitmap_utf8", _I32, [_P, _PCOL, _I, ctypes.c_char_p, _I64, _P, _I64, _P, _P]),
+    ("fpdb_keydict_create", _I32, [_P, ctypes.POINTER(_I32), _I, _I, ctypes.POINTER(_P)]),
+    ("fpdb_keydict_destroy", _I32, [_P]),
+    ("fpdb_keydict_size", _I32, [_P, _PI64]),
+    ("fpdb_keydict_encode", _I32, [_P, _P, _PCOL, _I, _I64, _I, _I, _PCOL, _PI64]),
+    ("fpdb_keydict_decode_bytes", _I32, [_P, _P, _I, _PCOL, _I64, _PI64]),
+    ("fpdb_keydict_decode", _I32, [_P, _P, _I, _PCOL, _I64, _PCOL]),
 ]
 
 _lib = None

@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FPDB_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["context.hip", "bloom.hip", "hashjoin.hip", "groupby.hip", "starjoin.hip", "serve.hip", "shuffle.hip",
-               "segcache.hip", "wire.hip"]
+               "segcache.hip", "wire.hip", "strings.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

@@ -706,18 +706,36 @@ fpdb_status fpdb_compare_bitmap(fpdb_context* ctx, const fpdb_column* column, in
   return FPDB_OK;
 }
 
-fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in, int num_columns,
+fpdb_status fpdb_filter_compact(fpdb_context* ctx, const fpdb_column* in_all, int num_all,
                                 int64_t num_rows, const uint8_t* d_bitmap,
-                                fpdb_column* out, int64_t* num_out) {
+                                fpdb_column* out_all, int64_t* num_out) {
   FPDB_CHECK(ctx && num_out, FPDB_ERR_INVALID, "NULL argument");
   FPDB_CHECK(d_bitmap || num_rows == 0, FPDB_ERR_INVALID, "NULL bitmap");
   Context* c = as_ctx(ctx);
+  std::vector<fpdb_column> fin, fout;
+  std::vector<int> utf8;
+  FPDB_TRY(split_utf8(in_all, out_all, num_all, &fin, &fout, &utf8, "filter compact"));
   ColSet ic;
   OutColSet oc;
-  FPDB_TRY(make_colset(in, num_columns, num_rows, &ic, "filter compact"));
-  FPDB_TRY(make_outcolset(out, num_columns, &oc, "filter compact"));
+  FPDB_TRY(make_colset(fin.data(), (int)fin.size(), num_rows, &ic, "filter compact"));
+  FPDB_TRY(make_outcolset(fout.data(), (int)fout.size(), &oc, "filter compact"));
   FPDB_TRY(check_outputs_match(ic, oc));
-  return run_compact<PredBitmap, false>(c, PredBitmap{d_bitmap}, ic, oc, num_rows, nullptr, nullptr, num_out);
+  if (utf8.empty())
+    return run_compact<PredBitmap, false>(c, PredBitmap{d_bitmap}, ic, oc, num_rows, nullptr, nullptr, num_out);
+  // utf8 columns: the selected row numbers, then a take of each
+  int64_t cap = oc.capacity;
+  for (int i : utf8) {
+    FPDB_CHECK(in_all[i].length >= num_rows, FPDB_ERR_INVALID, "filter compact: input column too short");
+    cap = std::min<int64_t>(cap, out_all[i].length);
+  }
+  void* rows = nullptr;
+  FPDB_TRY(ctx_alloc(c, (uint64_t)std::max<int64_t>(num_rows, 1) * 8, &rows));
+  fpdb_status s = run_compact<PredBitmap, false>(c, PredBitmap{d_bitmap}, ic, oc, num_rows, nullptr,
+                                                 static_cast<int64_t*>(rows), num_out, nullptr, "filter_compact", cap);
+  for (size_t j = 0; s == FPDB_OK && j < utf8.size(); ++j)
+    s = utf8_gather(c, in_all[utf8[j]], static_cast<int64_t*>(rows), *num_out, out_all[utf8[j]]);
+  ctx_free(c, rows);
+  return s;
 }
 
 fpdb_status fpdb_filter_compact_noalloc(fpdb_context* ctx, const fpdb_column* in, int num_columns,

@@ -164,6 +164,30 @@ struct OutColSet {
 
 fpdb_status make_colset(const fpdb_column* cols, int n, int64_t num_rows, ColSet* out,
                         const char* what);
+
+// utf8 columns (strings.hip).  utf8_gather: rows d_rows[0, n) of `in` (-1 =
+// null) into `out` from row 0; FPDB_ERR_CAPACITY if the bytes pass
+// out.data_bytes (synchronizes).  utf8_copy_rows: rows [0, n) of `in` after
+// output row out_row, whose offsets up to out_row are written (synchronizes).
+fpdb_status utf8_gather(Context* c, const fpdb_column& in, const int64_t* d_rows, int64_t n, const fpdb_column& out);
+fpdb_status utf8_copy_rows(Context* c, const fpdb_column& in, int64_t n, const fpdb_column& out, int64_t out_row);
+
+// Splits an input/output column list into its fixed-width columns (in order)
+// and the positions of its utf8 columns.
+inline fpdb_status split_utf8(const fpdb_column* in, const fpdb_column* out, int n, std::vector<fpdb_column>* fin,
+                              std::vector<fpdb_column>* fout, std::vector<int>* utf8, const char* what) {
+  FPDB_CHECK(n >= 0 && (n == 0 || (in && out)), FPDB_ERR_INVALID, std::string(what) + ": NULL columns");
+  for (int i = 0; i < n; ++i) {
+    if (in[i].type == FPDB_TYPE_UTF8 || out[i].type == FPDB_TYPE_UTF8) {
+      FPDB_CHECK(in[i].type == out[i].type, FPDB_ERR_INVALID, std::string(what) + ": utf8 column into another type");
+      utf8->push_back(i);
+    } else {
+      fin->push_back(in[i]);
+      fout->push_back(out[i]);
+    }
+  }
+  return FPDB_OK;
+}
 fpdb_status make_outcolset(const fpdb_column* cols, int n, OutColSet* out, const char* what);
 
 // ---- device helpers ------------------------------------------------------------

@@ -1429,11 +1429,22 @@ fpdb_status fpdb_hj_build_side_rows(fpdb_context* ctx, fpdb_hash_table* t, int m
                                           "build side rows", capacity);
 }
 
-fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_columns, int64_t num_rows,
-                           fpdb_column* out, int64_t out_row) {
+fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in_all, int num_all, int64_t num_rows,
+                           fpdb_column* out_all, int64_t out_row) {
   FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
   FPDB_CHECK(out_row >= 0, FPDB_ERR_INVALID, "copy rows: negative output row");
   Context* c = as_ctx(ctx);
+  std::vector<fpdb_column> fin, fout;
+  std::vector<int> utf8;
+  FPDB_TRY(split_utf8(in_all, out_all, num_all, &fin, &fout, &utf8, "copy rows"));
+  for (int i : utf8) {
+    FPDB_CHECK(in_all[i].length >= num_rows && out_all[i].length >= out_row + num_rows, FPDB_ERR_INVALID,
+               "copy rows: column too short");
+    FPDB_TRY(utf8_copy_rows(c, in_all[i], num_rows, out_all[i], out_row));
+  }
+  const fpdb_column* in = fin.data();
+  fpdb_column* out = fout.data();
+  const int num_columns = (int)fin.size();
   ColSet ic;
   OutColSet oc;
   FPDB_TRY(make_colset(in, num_columns, num_rows, &ic, "copy rows"));
@@ -1444,18 +1455,25 @@ fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_col
     FPDB_CHECK(oc.validity[i] || !ic.validity[i], FPDB_ERR_INVALID,
                "copy rows: nullable input needs an output validity bitmap");
   }
-  if (num_rows == 0) return FPDB_OK;
+  if (num_rows == 0 || num_columns == 0) return FPDB_OK;
   hipLaunchKernelGGL(copy_rows_kernel, dim3(grid_for(c, num_rows, 256)), dim3(256), 0, c->stream, ic, num_rows,
                      oc, out_row);
   FPDB_HIP(hipGetLastError());
   return FPDB_OK;
 }
 
-fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns, const int64_t* d_rows,
-                        int64_t num_rows, fpdb_column* out) {
+fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in_all, int num_all, const int64_t* d_rows,
+                        int64_t num_rows, fpdb_column* out_all) {
   FPDB_CHECK(ctx, FPDB_ERR_INVALID, "NULL context");
   FPDB_CHECK(d_rows || num_rows == 0, FPDB_ERR_INVALID, "NULL rows");
   Context* c = as_ctx(ctx);
+  std::vector<fpdb_column> fin, fout;
+  std::vector<int> utf8;
+  FPDB_TRY(split_utf8(in_all, out_all, num_all, &fin, &fout, &utf8, "gather"));
+  for (int i : utf8) FPDB_TRY(utf8_gather(c, in_all[i], d_rows, num_rows, out_all[i]));
+  const fpdb_column* in = fin.data();
+  fpdb_column* out = fout.data();
+  const int num_columns = (int)fin.size();
   ColSet ic;
   OutColSet oc;
   FPDB_TRY(make_colset(in, num_columns, 0, &ic, "gather"));
@@ -1467,7 +1485,7 @@ fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_column
     FPDB_CHECK((oc.offset[i] & 31) == 0, FPDB_ERR_INVALID, "output validity offset must be a multiple of 32");
     FPDB_TRY(launch_fill(c, oc.validity[i] + oc.offset[i] / 8, 0, (uint64_t)((num_rows + 31) / 32) * 4));
   }
-  if (num_rows == 0) return FPDB_OK;
+  if (num_rows == 0 || num_columns == 0) return FPDB_OK;
   hipLaunchKernelGGL(gather_kernel, dim3(grid_for(c, num_rows, 256)), dim3(256), 0, c->stream, ic, d_rows,
                      num_rows, oc);
   FPDB_HIP(hipGetLastError());

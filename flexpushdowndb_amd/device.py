@@ -175,3 +175,66 @@ class DeviceColumn:
 def unpack_validity(bits, offset, n):
     """Rows [offset, offset + n) of an LSB-first validity bitmap as booleans."""
     return np.unpackbits(np.asarray(bits, np.uint8), bitorder="little")[offset:offset + n].astype(bool)
+
+
+class Utf8Column:
+    """An Arrow utf8 column on the device: int32 offsets, character bytes and
+    an optional validity bitmap."""
+
+    type_id = C.TYPE_UTF8
+
+    def __init__(self, offsets, data, data_bytes, validity=None, length=None, offset=0):
+        self.offsets = offsets
+        self.data = data
+        self.data_bytes = int(data_bytes)
+        self.validity = validity
+        self.length = offsets.length - 1 if length is None else int(length)
+        self.offset = int(offset)
+
+    @classmethod
+    def from_strings(cls, ctx, strings):
+        """strings: a sequence of str / bytes, None for a null row."""
+        enc = [None if s is None else (s.encode() if isinstance(s, str) else bytes(s)) for s in strings]
+        lens = np.array([0 if b is None else len(b) for b in enc], np.int64)
+        offs = np.zeros(len(enc) + 1, np.int64)
+        np.cumsum(lens, out=offs[1:])
+        if offs[-1] > np.iinfo(np.int32).max:
+            raise ValueError("more than 2^31 - 1 bytes in one utf8 column")
+        data = np.frombuffer(b"".join(b for b in enc if b is not None) or b"\0", np.uint8)
+        vb = None
+        if any(b is None for b in enc):
+            bits = np.packbits(np.array([b is not None for b in enc], bool), bitorder="little")
+            pad = np.zeros(((len(enc) + 31) // 32) * 4, np.uint8)
+            pad[:len(bits)] = bits
+            vb = ctx.upload(pad)
+        return cls(ctx.upload(offs.astype(np.int32)), ctx.upload(data), int(offs[-1]), vb, len(enc))
+
+    @classmethod
+    def empty(cls, ctx, n, data_bytes, nullable=False):
+        offs = ctx.zeros(n + 1, np.int32)
+        data = ctx.empty(max(1, int(data_bytes)), np.uint8)
+        vb = ctx.zeros(((max(1, n) + 31) // 32) * 4, np.uint8) if nullable else None
+        return cls(offs, data, int(data_bytes), vb, n)
+
+    def c(self, length=None):
+        return C.Column(C.TYPE_UTF8, 0, self.length if length is None else int(length), self.offset,
+                        self.offsets.ptr, self.validity.ptr if self.validity is not None else None,
+                        self.data.ptr, self.data_bytes)
+
+    def valid_mask(self, n=None):
+        n = self.length if n is None else int(n)
+        if self.validity is None:
+            return np.ones(n, bool)
+        nbytes = (self.offset + n + 7) // 8
+        bits = self.offsets.ctx.download(self.validity.ptr, nbytes, np.uint8)
+        return unpack_validity(bits, self.offset, n)
+
+    def to_list(self, n=None):
+        """Rows [0, n) as str (None for null)."""
+        n = self.length if n is None else int(n)
+        ctx = self.offsets.ctx
+        offs = ctx.download(self.offsets.ptr + 4 * self.offset, n + 1, np.int32).astype(np.int64)
+        lo, hi = (int(offs[0]), int(offs[-1])) if n else (0, 0)
+        raw = ctx.download(self.data.ptr + lo, hi - lo, np.uint8).tobytes() if hi > lo else b""
+        valid = self.valid_mask(n)
+        return [raw[offs[i] - lo:offs[i + 1] - lo].decode() if valid[i] else None for i in range(n)]

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _capi as C
-from .device import DeviceColumn
+from .device import DeviceColumn, Utf8Column
 
 
 def _cols(cols, n=None):
@@ -157,7 +157,8 @@ def filter_compact(ctx, cols, n, bitmap, capacity=None):
     columns of `capacity` rows (default n); raises FpdbError with
     FPDB_ERR_CAPACITY when more rows are selected than fit."""
     cap = n if capacity is None else capacity
-    out = [DeviceColumn.empty(ctx, cap, c.dtype, c.validity is not None, c.type_id, validity_rows=n)
+    out = [Utf8Column.empty(ctx, cap, c.data_bytes, c.validity is not None) if isinstance(c, Utf8Column)
+           else DeviceColumn.empty(ctx, cap, c.dtype, c.validity is not None, c.type_id, validity_rows=n)
            for c in cols]
     nout = ctypes.c_int64()
     C.check(C.lib().fpdb_filter_compact(ctx.handle, _cols(cols), len(cols), n, bitmap.ptr,
@@ -439,3 +440,98 @@ class GroupBy:
         for c in keys + aggs:
             c.length = n.value
         return keys, aggs
+
+
+# ---- utf8 columns and composite keys -------------------------------------------------
+
+def _rows_col(rows, n):
+    """A row-number column: a DeviceColumn (INT64/INT32) or a DeviceArray of int64."""
+    if isinstance(rows, DeviceColumn):
+        return rows.c(n)
+    return C.Column(C.TYPE_INT64, 0, int(n), 0, rows.ptr, None)
+
+
+def utf8_take(ctx, col, rows, n):
+    """Rows `rows` (-1 or null = null row) of a utf8 column, as a new Utf8Column."""
+    cin = col.c()
+    rc = _rows_col(rows, n)
+    nbytes = ctypes.c_int64()
+    C.check(C.lib().fpdb_utf8_take_bytes(ctx.handle, ctypes.byref(cin), ctypes.byref(rc), n, ctypes.byref(nbytes)))
+    out = Utf8Column.empty(ctx, n, nbytes.value, nullable=True)
+    oc = out.c()
+    C.check(C.lib().fpdb_utf8_take(ctx.handle, ctypes.byref(cin), ctypes.byref(rc), n, ctypes.byref(oc)))
+    return out
+
+
+def iota(ctx, n, start=0):
+    out = ctx.empty(max(1, n), np.int64)
+    C.check(C.lib().fpdb_iota_int64(ctx.handle, out.ptr, n, start))
+    return out
+
+
+def compare_bitmap_utf8(ctx, col, op, constant, n, combine=None, combine_or=False, out=None):
+    """Selection bitmap of `col op constant` (bytewise string order)."""
+    b = constant.encode() if isinstance(constant, str) else bytes(constant)
+    bm = out if out is not None else ctx.zeros(max(8, ((n + 63) // 64) * 8), np.uint8)
+    cc = col.c()
+    C.check(C.lib().fpdb_compare_bitmap_utf8(ctx.handle, ctypes.byref(cc), op | (C.CMP_OR if combine_or else 0),
+                                             b, len(b), combine.ptr if combine is not None else None, n, bm.ptr,
+                                             None))
+    return bm
+
+
+class KeyDict:
+    """Dense int32 ids for key tuples of any width and type (utf8 included),
+    in HBM: the key encoder the join and group-by operators use for keys their
+    integer kernels do not take directly."""
+
+    def __init__(self, ctx, key_types, hash_bits=64):
+        self.ctx = ctx
+        self.key_types = list(key_types)
+        types = (ctypes.c_int32 * len(self.key_types))(*self.key_types)
+        h = ctypes.c_void_p()
+        C.check(C.lib().fpdb_keydict_create(ctx.handle, types, len(self.key_types), hash_bits, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            C.lib().fpdb_keydict_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def size(self):
+        n = ctypes.c_int64()
+        C.check(C.lib().fpdb_keydict_size(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def encode(self, key_cols, n, insert=True, null_is_key=False):
+        """INT32 ids of the rows (-1 and a cleared validity bit: absent, or a
+        null key with null_is_key False)."""
+        ids = DeviceColumn.empty(self.ctx, n, np.int32, nullable=True)
+        oc = ids.c()
+        total = ctypes.c_int64()
+        C.check(C.lib().fpdb_keydict_encode(self.ctx.handle, self.handle, _cols(key_cols, n), len(key_cols), n,
+                                            1 if insert else 0, 1 if null_is_key else 0, ctypes.byref(oc),
+                                            ctypes.byref(total)))
+        return ids
+
+    def decode(self, key, ids, n):
+        """Values of key column `key` for the given ids (a DeviceColumn)."""
+        t = self.key_types[key]
+        rc = ids.c(n)
+        if t == C.TYPE_UTF8:
+            nbytes = ctypes.c_int64()
+            C.check(C.lib().fpdb_keydict_decode_bytes(self.ctx.handle, self.handle, key, ctypes.byref(rc), n,
+                                                      ctypes.byref(nbytes)))
+            out = Utf8Column.empty(self.ctx, n, nbytes.value, nullable=True)
+        else:
+            out = DeviceColumn.empty(self.ctx, n, C.TYPE_TO_NP[t], nullable=True, type_id=t)
+        oc = out.c()
+        C.check(C.lib().fpdb_keydict_decode(self.ctx.handle, self.handle, key, ctypes.byref(rc), n,
+                                            ctypes.byref(oc)))
+        return out

@@ -44,19 +44,27 @@ enum {
 typedef enum {
   FPDB_TYPE_INT8 = 1, FPDB_TYPE_INT16 = 2, FPDB_TYPE_INT32 = 3, FPDB_TYPE_INT64 = 4,
   FPDB_TYPE_UINT8 = 5, FPDB_TYPE_UINT16 = 6, FPDB_TYPE_UINT32 = 7, FPDB_TYPE_UINT64 = 8,
-  FPDB_TYPE_FLOAT32 = 9, FPDB_TYPE_FLOAT64 = 10, FPDB_TYPE_DATE32 = 11, FPDB_TYPE_DATE64 = 12
+  FPDB_TYPE_FLOAT32 = 9, FPDB_TYPE_FLOAT64 = 10, FPDB_TYPE_DATE32 = 11, FPDB_TYPE_DATE64 = 12,
+  FPDB_TYPE_UTF8 = 13
 } fpdb_type;
 
 /* One Arrow-layout column on the device.  For outputs, values/validity are
  * caller-allocated buffers the library writes into (validity may be NULL when
- * the output cannot hold nulls). */
+ * the output cannot hold nulls).
+ * A utf8 column (Arrow "utf8": int32 offsets + character bytes) keeps its
+ * offsets in `values` (offset + length + 1 entries; row r spans bytes
+ * [values[offset + r], values[offset + r + 1]) of `data`) and its bytes in
+ * `data`; `data_bytes` is the size of `data` (for outputs: the room in it).
+ * Entry points that take utf8 columns say so; the others reject them. */
 typedef struct fpdb_column {
   int32_t type;             /* fpdb_type                                   */
   int32_t reserved;
   int64_t length;           /* rows                                        */
   int64_t offset;           /* element offset into values and validity     */
-  void* values;             /* device pointer                              */
+  void* values;             /* device pointer (utf8: int32 offsets)        */
   uint8_t* validity;        /* device pointer or NULL (all valid)          */
+  void* data;               /* utf8: character bytes (device); else NULL   */
+  int64_t data_bytes;       /* utf8: bytes in (room of) data; else 0       */
 } fpdb_column;
 
 typedef struct fpdb_context fpdb_context;
@@ -336,6 +344,81 @@ fpdb_status fpdb_copy_rows(fpdb_context* ctx, const fpdb_column* in, int num_col
  * materialise build-side rows and the null probe side of outer joins). */
 fpdb_status fpdb_gather(fpdb_context* ctx, const fpdb_column* in, int num_columns,
                         const int64_t* d_rows, int64_t num_rows, fpdb_column* out);
+
+/* ---- utf8 columns and composite keys ---------------------------------------- */
+
+/* The bytes fpdb_utf8_take would write: the sum of the lengths of the rows
+ * named by `rows` (an INT64 or INT32 column; a null or negative entry names
+ * no row).  *bytes (host) receives it (synchronizes). */
+fpdb_status fpdb_utf8_take_bytes(fpdb_context* ctx, const fpdb_column* in, const fpdb_column* rows,
+                                 int64_t num_rows, int64_t* bytes);
+
+/* Gathers rows of a utf8 column: out row i is in row rows[i], or null where
+ * rows[i] is null or negative (out then needs a validity bitmap of
+ * ceil(num_rows/32)*4 bytes; it is needed too when `in` has one).  out->values
+ * needs num_rows + 1 int32 offsets (out->offset must be 0) and out->data room
+ * for the bytes (out->data_bytes; fpdb_utf8_take_bytes gives the count, more
+ * is an FPDB_ERR_CAPACITY failure before anything is written; synchronizes).
+ * The take/append of string arrays the reference leaves to Arrow
+ * (arrow::compute::Take in the acero hash join's materialisation,
+ * HashJoinArrowKernel.cpp:126-206; ArrayAppender in RecordBatchHashJoiner.cpp:83-119;
+ * arrow::ConcatenateTables in TupleSet::append, fpdb-tuple/src/TupleSet.cpp). */
+fpdb_status fpdb_utf8_take(fpdb_context* ctx, const fpdb_column* in, const fpdb_column* rows,
+                           int64_t num_rows, fpdb_column* out);
+
+/* Writes start, start + 1, ... into n int64 values (row numbers to carry
+ * through a join or compaction and resolve utf8 columns with afterwards). */
+fpdb_status fpdb_iota_int64(fpdb_context* ctx, int64_t* d_out, int64_t n, int64_t start);
+
+/* Comparison of a utf8 column with a constant string (bytewise, as unsigned
+ * bytes, a prefix before any longer string: Arrow's binary ordering) into a
+ * selection bitmap; the other arguments and the result as for
+ * fpdb_compare_bitmap (gandiva's string comparisons in FilterPOp,
+ * fpdb-executor/src/physical/filter/FilterPOp.cpp). */
+fpdb_status fpdb_compare_bitmap_utf8(fpdb_context* ctx, const fpdb_column* column, int op,
+                                     const char* constant, int64_t constant_len,
+                                     const uint8_t* d_and_bitmap, int64_t num_rows, uint8_t* d_bitmap,
+                                     int64_t* d_num_selected);
+
+/* A dictionary of key tuples in HBM: it gives each distinct row of its key
+ * columns (integer, float, date and utf8 columns in any number, up to 8) a
+ * dense int32 id, in order of first occurrence (a tuple whose 64-bit hash
+ * equals another's may take its id a little later).  Equality is exact:
+ * tuples are compared value by value, integers by value whatever their width,
+ * floats by value (0.0 == -0.0), strings byte by byte.  It lets the join and
+ * group-by kernels, which take one integer key, work on keys of any width and
+ * type: the role of Arrow's Grouper in the acero aggregate node
+ * (GroupArrowKernel.cpp:245-291) and of the key encoder of the acero hash
+ * join (HashJoinArrowKernel.cpp:138-145), both in the absent Arrow library.
+ * hash_bits < 64 keeps only that many bits of each tuple's hash (tests use it
+ * to make distinct tuples share hashes); pass 64. */
+typedef struct fpdb_keydict fpdb_keydict;
+fpdb_status fpdb_keydict_create(fpdb_context* ctx, const int32_t* key_types, int num_keys, int hash_bits,
+                                fpdb_keydict** out);
+fpdb_status fpdb_keydict_destroy(fpdb_keydict* dict);
+fpdb_status fpdb_keydict_size(fpdb_keydict* dict, int64_t* num_ids);
+
+/* Maps num_rows rows of the key columns (same count as the dictionary's; an
+ * integer column may differ in width from the dictionary's but not in
+ * signedness) to ids in `ids` (an INT32 output column).  insert != 0 adds the
+ * tuples not yet present (new ids); insert == 0 gives -1 for them.
+ * null_is_key != 0 makes null a value like any other (a group-by key);
+ * null_is_key == 0 gives -1 to a row with a null key column (a join key never
+ * matches).  Where an id is -1 the ids column's validity bit (if it has a
+ * bitmap) is cleared.  *num_ids (host, may be NULL) receives the dictionary's
+ * size afterwards (synchronizes). */
+fpdb_status fpdb_keydict_encode(fpdb_context* ctx, fpdb_keydict* dict, const fpdb_column* keys, int num_keys,
+                                int64_t num_rows, int insert, int null_is_key, fpdb_column* ids,
+                                int64_t* num_ids);
+
+/* The values of key column `key` of the tuples with the given ids (an INT32
+ * or INT64 column; null or negative gives null): a column of the
+ * dictionary's type for that key.  A utf8 output follows fpdb_utf8_take;
+ * fpdb_keydict_decode_bytes gives its byte count (synchronizes). */
+fpdb_status fpdb_keydict_decode_bytes(fpdb_context* ctx, fpdb_keydict* dict, int key, const fpdb_column* ids,
+                                      int64_t num_rows, int64_t* bytes);
+fpdb_status fpdb_keydict_decode(fpdb_context* ctx, fpdb_keydict* dict, int key, const fpdb_column* ids,
+                                int64_t num_rows, fpdb_column* out);
 
 /* ---- wire forms (filters and bitmaps shipped to the store) --------------------- */
 

@@ -434,3 +434,42 @@ def selection_from_words(words, start, n):
     [start, start + n) as a boolean array (fpdb::util::getBit, Util.cpp:123-125)."""
     w = [int(x) & ((1 << 64) - 1) for x in words]
     return np.array([(w[r // 64] >> (r % 64)) & 1 for r in range(start, start + n)], dtype=bool)
+
+
+def factorize_tuples(columns, null_is_key=True):
+    """Dense ids of key tuples in order of first appearance -- the group ids
+    Arrow's Grouper hands the acero aggregate node (GroupArrowKernel.cpp:245-291)
+    and that the group-by's first-appearance group order follows.  `columns`:
+    sequences of equal length whose None entries are nulls.  Integers compare by
+    value, floats by value (0.0 == -0.0), strings by their characters.  With
+    null_is_key False a tuple holding a null gets id -1 (a join key never
+    matches).  Pure Python: small inputs only."""
+    ids = {}
+    out = []
+    n = len(columns[0]) if columns else 0
+    for r in range(n):
+        t = tuple(c[r] for c in columns)
+        if not null_is_key and any(v is None for v in t):
+            out.append(-1)
+            continue
+        t = tuple(0.0 if isinstance(v, float) and v == 0.0 else v for v in t)
+        if t not in ids:
+            ids[t] = len(ids)
+        out.append(ids[t])
+    return np.array(out, np.int64), list(ids.keys())
+
+
+def lookup_tuples(dictionary, columns):
+    """Ids of the tuples in `dictionary` (the key list factorize_tuples
+    returned), -1 for tuples absent from it or holding a null."""
+    index = {t: i for i, t in enumerate(dictionary)}
+    n = len(columns[0]) if columns else 0
+    out = []
+    for r in range(n):
+        t = tuple(c[r] for c in columns)
+        if any(v is None for v in t):
+            out.append(-1)
+            continue
+        t = tuple(0.0 if isinstance(v, float) and v == 0.0 else v for v in t)
+        out.append(index.get(t, -1))
+    return np.array(out, np.int64)

@@ -433,3 +433,25 @@ def test_bitmap_words_by_hand_and_framing():
     assert [x.num_rows for x in b] == [136, 64] and b[0].schema.field(0).type == pa.uint8()
     with pytest.raises(ValueError):
         wire.record_batch_to_bitmap(b[0])       # uint8, not the int64 bitmap
+
+
+def test_factorize_tuples_against_pandas_and_by_hand():
+    """Pinned against pandas.factorize (first-appearance codes, a library
+    routine) on random multi-column keys, and by hand on nulls, -0.0 and
+    strings."""
+    import pandas as pd
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 7, 500).tolist()
+    b = [f"s{v}" for v in rng.integers(0, 5, 500)]
+    ids, keys = O.factorize_tuples([a, b])
+    codes, _ = pd.factorize(pd.Series(list(zip(a, b))))
+    assert np.array_equal(ids, codes)
+    assert keys == list(dict.fromkeys(zip(a, b)))
+    # nulls are one value as group keys, and exclude a row as join keys
+    ids, keys = O.factorize_tuples([[1, None, 1, None, 2], ["x", "y", "x", "y", None]])
+    assert ids.tolist() == [0, 1, 0, 1, 2] and keys[1] == (None, "y")
+    ids, _ = O.factorize_tuples([[1, None, 1, None, 2], ["x", "y", "x", "y", None]], null_is_key=False)
+    assert ids.tolist() == [0, -1, 0, -1, -1]
+    ids, keys = O.factorize_tuples([[0.0, -0.0, 1.5]])
+    assert ids.tolist() == [0, 0, 1]
+    assert O.lookup_tuples(keys, [[-0.0, 2.0, None, 1.5]]).tolist() == [0, -1, -1, 1]
