@@ -60,9 +60,11 @@ Expected<std::shared_ptr<TupleSet>> BloomFilterUseKernel::filter(const std::shar
   for (int i = 0; i < tupleSet->numColumns(); ++i) {
     const Column& c = tupleSet->column(i);
     in.push_back(c.c());
-    outCols.push_back(Column::allocate(ctx, c.type, n, c.validity != nullptr));
+    outCols.push_back(Column::allocateLike(ctx, c, n));
     out.push_back(outCols.back().c());
   }
+  bool anyUtf8 = false;
+  for (auto& c : tupleSet->columns()) anyUtf8 = anyUtf8 || c.isUtf8();
   std::vector<int> keyIndex;
   for (auto& name : columnNames) {
     int i = tupleSet->schema().getFieldIndex(name);
@@ -73,9 +75,22 @@ Expected<std::shared_ptr<TupleSet>> BloomFilterUseKernel::filter(const std::shar
   if (bloomFilter->getType() == ARROW_BLOOM_FILTER) {
     // BloomFilterUseKernel.cpp:98-138,180-207: hash, Find, arrow::compute::Filter
     auto bf = std::static_pointer_cast<ArrowBloomFilter>(bloomFilter);
-    fpdb_status s = fpdb_bloom_blocked_filter(ctx, bf->deviceBlocks(), bf->logNumBlocks(), in.data(),
-                                              (int)in.size(), keyIndex.data(), (int)keyIndex.size(), n, nullptr,
-                                              out.data(), &kept);
+    fpdb_status s;
+    if (!anyUtf8) {
+      s = fpdb_bloom_blocked_filter(ctx, bf->deviceBlocks(), bf->logNumBlocks(), in.data(), (int)in.size(),
+                                    keyIndex.data(), (int)keyIndex.size(), n, nullptr, out.data(), &kept);
+    } else {
+      // the fused pass moves fixed-width columns only: Find, then a compaction
+      // that carries the utf8 columns too
+      std::vector<fpdb_column> keys;
+      for (int i : keyIndex) keys.push_back(in[i]);
+      DeviceBuffer bitmap(ctx, (uint64_t)((n + 63) / 64) * 8);
+      s = fpdb_bloom_blocked_find(ctx, bf->deviceBlocks(), bf->logNumBlocks(), keys.data(), (int)keys.size(), n,
+                                  static_cast<uint8_t*>(bitmap.data()), nullptr);
+      if (s == FPDB_OK)
+        s = fpdb_filter_compact(ctx, in.data(), (int)in.size(), n, static_cast<uint8_t*>(bitmap.data()), out.data(),
+                                &kept);
+    }
     if (s != FPDB_OK) return make_unexpected(fpdb_last_error());
   } else {
     // BloomFilterUseKernel.cpp:140-178: contains() per row, selection vector, filter

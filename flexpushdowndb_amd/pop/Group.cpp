@@ -12,6 +12,8 @@ void GroupArrowKernel::clear() {
   if (table_) fpdb_groupby_destroy(table_);
   table_ = nullptr;
   inputs_.clear();
+  inputIds_.clear();
+  keyDict_.reset();
   haveRange_ = false;
   keyMin_.clear();
   keyMax_.clear();
@@ -33,6 +35,11 @@ Expected<void> GroupArrowKernel::makeOutputSchema(const Schema& in) {
   packed_ = keyTypes_.size() > 2;
   if (keyTypes_.size() == 2)
     for (auto t : keyTypes_) packed_ = packed_ || typeWidth((fpdb_type)t) > 4;
+  for (auto& g : groupColumnNames_) {
+    const Field& f = in.fields[in.getFieldIndex(g)];
+    if (f.type == FPDB_TYPE_UTF8 || f.nullable) useDict_ = true;
+  }
+  if (useDict_) packed_ = false;
   specInputs_.clear();
   firstSpecOf_.clear();
   auto addSpec = [&](fpdb_agg_kind kind, const std::string& input) -> Expected<fpdb_agg_spec> {
@@ -80,6 +87,13 @@ Expected<void> GroupArrowKernel::recreate(int64_t maxGroups) {
   if (table_) fpdb_groupby_destroy(table_);
   table_ = nullptr;
   maxGroups_ = maxGroups;
+  if (useDict_) {
+    const int32_t idType = FPDB_TYPE_INT32;
+    if (fpdb_groupby_create(defaultContext(), &idType, 1, specs_.data(), (int)specs_.size(), maxGroups_, &table_) !=
+        FPDB_OK)
+      return make_unexpected(fpdb_last_error());
+    return {};
+  }
   fpdb_status s = packed_ ? fpdb_groupby_create_packed(defaultContext(), keyTypes_.data(), (int)keyTypes_.size(),
                                                        packMin_.data(), packBits_.data(), specs_.data(),
                                                        (int)specs_.size(), maxGroups_, &table_)
@@ -137,9 +151,40 @@ Expected<bool> GroupArrowKernel::widenKeyRanges(const TupleSet& ts) {
   return true;
 }
 
-Expected<void> GroupArrowKernel::consume(const std::shared_ptr<TupleSet>& ts) {
+// Packed keys that outgrow 64 bits: group by key-dictionary ids from here on
+// (the batches so far are encoded and consumed again).
+Expected<void> GroupArrowKernel::switchToDictionary() {
+  useDict_ = true;
+  packed_ = false;
+  keyDict_.reset();
+  inputIds_.clear();
+  return {};
+}
+
+Expected<void> GroupArrowKernel::consume(size_t input) {
+  const auto& ts = inputs_[input];
   std::vector<fpdb_column> keys, vals;
   for (auto& g : groupColumnNames_) keys.push_back(ts->column(ts->schema().getFieldIndex(g)).c());
+  if (useDict_) {
+    if (!keyDict_) {
+      std::vector<fpdb_type> kt;
+      for (auto t : keyTypes_) kt.push_back((fpdb_type)t);
+      try {
+        keyDict_ = std::make_unique<KeyDictionary>(kt);
+      } catch (const std::exception& e) {
+        return make_unexpected(e.what());
+      }
+    }
+    if (inputIds_.size() <= input) {
+      auto ids = keyDict_->encode(keys, ts->numRows(), true, true);
+      if (!ids) return make_unexpected(ids.error());
+      inputIds_.push_back(*ids);
+    }
+    // null_is_key ids are never null: the group-by takes them without a bitmap
+    fpdb_column ids = inputIds_[input].c();
+    ids.validity = nullptr;
+    keys = {ids};
+  }
   for (auto& input : specInputs_) {
     if (input.empty()) {
       vals.push_back(fpdb_column{});
@@ -162,19 +207,24 @@ Expected<void> GroupArrowKernel::group(const std::shared_ptr<TupleSet>& tupleSet
   bool rebuild = !table_;
   if (packed_) {
     auto w = widenKeyRanges(*tupleSet);
-    if (!w) return make_unexpected(w.error());
-    rebuild = rebuild || *w;
+    if (w) {
+      rebuild = rebuild || *w;
+    } else {
+      r = switchToDictionary();
+      if (!r) return r;
+      rebuild = true;
+    }
   }
   inputs_.push_back(tupleSet);
   if (rebuild) {
     r = recreate(std::max<int64_t>(maxGroups_, 1));
     if (!r) return r;
-    for (auto& in : inputs_) {
-      r = consume(in);
+    for (size_t i = 0; i < inputs_.size(); ++i) {
+      r = consume(i);
       if (!r) return r;
     }
   } else {
-    r = consume(tupleSet);
+    r = consume(inputs_.size() - 1);
     if (!r) return r;
   }
   int64_t groups = 0;
@@ -182,8 +232,8 @@ Expected<void> GroupArrowKernel::group(const std::shared_ptr<TupleSet>& tupleSet
   while (s == FPDB_ERR_CAPACITY) {
     r = recreate(maxGroups_ * 8);
     if (!r) return r;
-    for (auto& in : inputs_) {
-      r = consume(in);
+    for (size_t i = 0; i < inputs_.size(); ++i) {
+      r = consume(i);
       if (!r) return r;
     }
     s = fpdb_groupby_num_groups(defaultContext(), table_, &groups);
@@ -202,9 +252,15 @@ Expected<std::shared_ptr<TupleSet>> GroupArrowKernel::finalise() {
   std::vector<Column> cols, aggCols;
   std::vector<fpdb_column> outKeys, outAggs;
   size_t nk = groupColumnNames_.size();
-  for (size_t i = 0; i < nk; ++i) {
-    cols.push_back(Column::allocate(ctx, os.fields[i].type, ng, os.fields[i].nullable));
-    outKeys.push_back(cols.back().c());
+  Column groupIds;
+  if (useDict_) {
+    groupIds = Column::allocate(ctx, FPDB_TYPE_INT32, ng, false);
+    outKeys.push_back(groupIds.c());
+  } else {
+    for (size_t i = 0; i < nk; ++i) {
+      cols.push_back(Column::allocate(ctx, os.fields[i].type, ng, os.fields[i].nullable));
+      outKeys.push_back(cols.back().c());
+    }
   }
   for (auto& s : specs_) {
     const bool nullable = s.kind != FPDB_AGG_COUNT_ALL && s.kind != FPDB_AGG_COUNT_VALID;
@@ -214,6 +270,14 @@ Expected<std::shared_ptr<TupleSet>> GroupArrowKernel::finalise() {
   int64_t written = 0;
   if (fpdb_groupby_finalize(ctx, table_, outKeys.data(), outAggs.data(), &written) != FPDB_OK)
     return make_unexpected(fpdb_last_error());
+  if (useDict_) {
+    // each group's key values from its id
+    for (size_t i = 0; i < nk; ++i) {
+      auto k = keyDict_->decode((int)i, groupIds, ng);
+      if (!k) return make_unexpected(k.error());
+      cols.push_back(*k);
+    }
+  }
   // [group columns..., aggregates in function order]; an AvgReduce divides its
   // two sums (GroupArrowKernel::finalizeAvg, GroupArrowKernel.cpp:156-200)
   for (size_t f = 0; f < aggregateFunctions_.size(); ++f) {

@@ -6,6 +6,7 @@
 
 #include <optional>
 
+#include "KeyDictionary.h"
 #include "PhysicalOp.h"
 
 namespace fpdb::amd {
@@ -96,7 +97,8 @@ class GroupArrowKernel {
 
  private:
   Expected<void> makeOutputSchema(const Schema& in);
-  Expected<void> consume(const std::shared_ptr<TupleSet>& ts);
+  Expected<void> consume(size_t input);
+  Expected<void> switchToDictionary();
   Expected<void> recreate(int64_t maxGroups);
   Expected<bool> widenKeyRanges(const TupleSet& ts);
 
@@ -120,6 +122,12 @@ class GroupArrowKernel {
   std::vector<int64_t> keyMin_, keyMax_;     // value range seen so far
   std::vector<int64_t> packMin_;             // packing of the current table
   std::vector<int32_t> packBits_;
+  // Keys of other kinds -- utf8, nullable (null is a group, as in Arrow's
+  // Grouper), or too wide to pack into 64 bits -- are grouped by their
+  // key-dictionary id (one INT32 key); the dictionary turns ids back into keys.
+  bool useDict_ = false;
+  std::unique_ptr<KeyDictionary> keyDict_;
+  std::vector<Column> inputIds_;             // ids of inputs_[i], kept for rebuilds
 };
 
 class GroupPOp : public PhysicalOp {

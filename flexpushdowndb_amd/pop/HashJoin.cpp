@@ -13,6 +13,9 @@ HashJoinArrowKernel::~HashJoinArrowKernel() { clear(); }
 void HashJoinArrowKernel::clear() {
   if (table_) fpdb_hj_table_destroy(table_);
   table_ = nullptr;
+  keyDict_.reset();
+  buildKeyIds_ = Column();
+  buildRows_ = Column();
   buildInput_.clear();
   pendingProbe_.clear();
   buildTable_.reset();
@@ -88,12 +91,29 @@ Expected<void> HashJoinArrowKernel::buildTable() {
     buildInput_.clear();
   }
   std::vector<fpdb_column> keys;
-  for (auto& n : pred_.leftColumnNames) keys.push_back(buildTable_->column(buildSchema_->getFieldIndex(n)).c());
+  std::vector<fpdb_type> buildTypes, probeTypes;
+  for (auto& n : pred_.leftColumnNames) {
+    keys.push_back(buildTable_->column(buildSchema_->getFieldIndex(n)).c());
+    buildTypes.push_back(buildSchema_->fields[buildSchema_->getFieldIndex(n)].type);
+  }
+  for (auto& n : pred_.rightColumnNames) probeTypes.push_back(probeSchema_->fields[probeSchema_->getFieldIndex(n)].type);
+  if (!joinKeysNative(buildTypes) || !joinKeysNative(probeTypes)) {
+    // utf8, float or wide composite keys: one int32 id per distinct key tuple
+    try {
+      keyDict_ = std::make_unique<KeyDictionary>(buildTypes);
+    } catch (const std::exception& e) {
+      return make_unexpected(e.what());
+    }
+    auto ids = keyDict_->encode(keys, buildTable_->numRows(), true, false);
+    if (!ids) return make_unexpected(ids.error());
+    buildKeyIds_ = *ids;
+    keys = {buildKeyIds_.c()};
+  }
   // A single 4-byte non-null build output column travels inline in the table slots.
   payloadColumn_ = -1;
   if (buildOut_.size() == 1) {
     const Column& c = buildTable_->column(buildOut_[0]);
-    if (typeWidth(c.type) == 4 && !c.validity) payloadColumn_ = buildOut_[0];
+    if (!c.isUtf8() && typeWidth(c.type) == 4 && !c.validity) payloadColumn_ = buildOut_[0];
   }
   fpdb_column payload;
   if (payloadColumn_ >= 0) payload = buildTable_->column(payloadColumn_).c();
@@ -114,15 +134,39 @@ Expected<void> HashJoinArrowKernel::probe(const std::shared_ptr<TupleSet>& ts) {
   int64_t n = ts->numRows();
   std::vector<fpdb_column> keys, pcols, bcols;
   for (auto& name : pred_.rightColumnNames) keys.push_back(ts->column(ts->schema().getFieldIndex(name)).c());
-  for (int i : probeOut_) pcols.push_back(ts->column(i).c());
-  for (int i : buildOut_) {
-    if (i == payloadColumn_) {
-      fpdb_column pc{};
-      pc.type = buildTable_->column(i).type;
-      bcols.push_back(pc);     // values == NULL: the inline payload
-    } else {
-      bcols.push_back(buildTable_->column(i).c());
+  Column probeIds;
+  if (keyDict_) {
+    auto ids = keyDict_->encode(keys, n, false, false);
+    if (!ids) return make_unexpected(ids.error());
+    probeIds = *ids;
+    keys = {probeIds.c()};
+  }
+  // utf8 output columns go through the probe as row numbers (probe rows, or
+  // build rows) and are taken afterwards
+  Column probeRows;
+  try {
+    for (int i : probeOut_) {
+      if (ts->column(i).isUtf8()) {
+        if (!probeRows.values) probeRows = rowNumbers(n);
+        pcols.push_back(probeRows.c());
+      } else {
+        pcols.push_back(ts->column(i).c());
+      }
     }
+    for (int i : buildOut_) {
+      if (i == payloadColumn_) {
+        fpdb_column pc{};
+        pc.type = buildTable_->column(i).type;
+        bcols.push_back(pc);     // values == NULL: the inline payload
+      } else if (buildTable_->column(i).isUtf8()) {
+        if (!buildRows_.values) buildRows_ = rowNumbers(buildTable_->numRows());
+        bcols.push_back(buildRows_.c());
+      } else {
+        bcols.push_back(buildTable_->column(i).c());
+      }
+    }
+  } catch (const std::exception& e) {
+    return make_unexpected(e.what());
   }
   int jt = (int)joinType_;
   int unique = 0;
@@ -138,7 +182,8 @@ Expected<void> HashJoinArrowKernel::probe(const std::shared_ptr<TupleSet>& ts) {
   std::vector<Column> outCols;
   std::vector<fpdb_column> out;
   for (size_t i = 0; i < os.fields.size(); ++i) {
-    outCols.push_back(Column::allocate(ctx, os.fields[i].type, cap, os.fields[i].nullable));
+    const fpdb_type t = os.fields[i].type == FPDB_TYPE_UTF8 ? FPDB_TYPE_INT64 : os.fields[i].type;
+    outCols.push_back(Column::allocate(ctx, t, cap, os.fields[i].nullable));
     out.push_back(outCols.back().c());
   }
   int64_t produced = 0;
@@ -147,6 +192,15 @@ Expected<void> HashJoinArrowKernel::probe(const std::shared_ptr<TupleSet>& ts) {
   if (s != FPDB_OK) return make_unexpected(fpdb_last_error());
   if (joinType_ == LEFT_SEMI) return {};
   for (auto& c : outCols) c.length = produced;
+  // resolve the utf8 columns from the row numbers the probe wrote
+  const size_t nProbe = probeOut_.size();
+  for (size_t i = 0; i < os.fields.size(); ++i) {
+    if (os.fields[i].type != FPDB_TYPE_UTF8) continue;
+    const Column& src = i < nProbe ? ts->column(probeOut_[i]) : buildTable_->column(buildOut_[i - nProbe]);
+    auto t = takeUtf8(src, outCols[i].c(), produced);
+    if (!t) return make_unexpected(t.error());
+    outCols[i] = *t;
+  }
   return bufferOutput(TupleSet::make(os, outCols), false);
 }
 
@@ -169,10 +223,21 @@ Expected<void> HashJoinArrowKernel::finish() {
     for (size_t i = 0; i < nProbe; ++i) outCols.push_back(Column::allNull(ctx, os.fields[i].type, count));
     std::vector<fpdb_column> in, out;
     std::vector<Column> gathered;
+    fpdb_column rowCol{};
+    rowCol.type = FPDB_TYPE_INT64;
+    rowCol.length = count;
+    rowCol.values = rows.data();
     for (size_t j = 0; j < buildOut_.size(); ++j) {
       const Field& f = os.fields[nProbe + j];
+      const Column& src = buildTable_->column(buildOut_[j]);
+      if (src.isUtf8()) {
+        auto t = takeUtf8(src, rowCol, count);
+        if (!t) return make_unexpected(t.error());
+        gathered.push_back(*t);
+        continue;
+      }
       gathered.push_back(Column::allocate(ctx, f.type, count, true));
-      in.push_back(buildTable_->column(buildOut_[j]).c());
+      in.push_back(src.c());
       out.push_back(gathered.back().c());
     }
     if (!in.empty()) {

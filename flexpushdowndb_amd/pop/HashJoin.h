@@ -6,6 +6,7 @@
 #include <optional>
 #include <set>
 
+#include "KeyDictionary.h"
 #include "PhysicalOp.h"
 
 namespace fpdb::amd {
@@ -55,6 +56,9 @@ class HashJoinArrowKernel {
   std::shared_ptr<TupleSet> buildTable_;      // concatenated build side
   fpdb_hash_table* table_ = nullptr;
   int payloadColumn_ = -1;                    // build column stored inline in the table
+  std::unique_ptr<KeyDictionary> keyDict_;    // keys the join kernels do not take directly
+  Column buildKeyIds_;                        // the build keys' ids while the table lives
+  Column buildRows_;                          // build row numbers, for utf8 build outputs
   bool buildFinalized_ = false, probeFinalized_ = false, finished_ = false;
   std::shared_ptr<TupleSet> pendingOutput_;
   std::optional<std::shared_ptr<TupleSet>> outputBuffer_;

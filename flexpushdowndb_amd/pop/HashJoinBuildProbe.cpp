@@ -138,17 +138,49 @@ Expected<void> HashJoinProbeKernel::bufferOutputSchema(const Schema& build, cons
 // Matched pairs of an index's rows and probe rows, build columns first: the
 // work of RecordBatchHashJoiner::join (RecordBatchHashJoiner.cpp:38-131), one
 // fpdb_hj_probe over the whole probe tuple set.
+// Key types of the named columns.
+static std::vector<fpdb_type> keyTypes(const Schema& schema, const std::vector<std::string>& names) {
+  std::vector<fpdb_type> t;
+  for (auto& n : names) t.push_back(schema.fields[schema.getFieldIndex(n)].type);
+  return t;
+}
+
 Expected<void> HashJoinProbeKernel::joinPairs(TupleSetIndex& build, const std::shared_ptr<TupleSet>& probe) {
   if (build.size() == 0 || probe->numRows() == 0) return {};
-  auto table = build.table(false);
+  auto table = build.table(false, keyTypes(probe->schema(), pred_.rightColumnNames));
   if (!table) return make_unexpected(table.error());
   fpdb_context* ctx = defaultContext();
   int64_t n = probe->numRows();
-  std::vector<fpdb_column> keys, cols;
-  for (auto& name : pred_.rightColumnNames) keys.push_back(probe->column(probe->schema().getFieldIndex(name)).c());
-  for (int i : probeOut_) cols.push_back(probe->column(i).c());
-  size_t nProbe = cols.size();
-  for (int i : buildOut_) cols.push_back(build.getTupleSet()->column(i).c());
+  std::vector<fpdb_column> rawKeys, cols;
+  for (auto& name : pred_.rightColumnNames) rawKeys.push_back(probe->column(probe->schema().getFieldIndex(name)).c());
+  Column keyIds;
+  auto pk = build.probeKeys(rawKeys, n, &keyIds);
+  if (!pk) return make_unexpected(pk.error());
+  std::vector<fpdb_column> keys = *pk;
+  // utf8 output columns travel as row numbers and are taken afterwards
+  Column probeRows, buildRows;
+  const TupleSet& bts = *build.getTupleSet();
+  try {
+    for (int i : probeOut_) {
+      if (probe->column(i).isUtf8()) {
+        if (!probeRows.values) probeRows = rowNumbers(n);
+        cols.push_back(probeRows.c());
+      } else {
+        cols.push_back(probe->column(i).c());
+      }
+    }
+    for (int i : buildOut_) {
+      if (bts.column(i).isUtf8()) {
+        if (!buildRows.values) buildRows = rowNumbers(bts.numRows());
+        cols.push_back(buildRows.c());
+      } else {
+        cols.push_back(bts.column(i).c());
+      }
+    }
+  } catch (const std::exception& e) {
+    return make_unexpected(e.what());
+  }
+  size_t nProbe = probeOut_.size();
   int64_t cap = n;
   int unique = 0;
   if (fpdb_hj_table_info(*table, nullptr, nullptr, nullptr, &unique) != FPDB_OK)
@@ -157,7 +189,8 @@ Expected<void> HashJoinProbeKernel::joinPairs(TupleSetIndex& build, const std::s
     return make_unexpected(fpdb_last_error());
   const Schema& os = *outputSchema_;
   std::vector<Column> outCols;
-  for (auto& f : os.fields) outCols.push_back(Column::allocate(ctx, f.type, cap, f.nullable));
+  for (auto& f : os.fields)
+    outCols.push_back(Column::allocate(ctx, f.type == FPDB_TYPE_UTF8 ? FPDB_TYPE_INT64 : f.type, cap, f.nullable));
   // fpdb_hj_probe writes probe columns, then build columns
   std::vector<fpdb_column> out;
   size_t nBuild = buildOut_.size();
@@ -169,6 +202,13 @@ Expected<void> HashJoinProbeKernel::joinPairs(TupleSetIndex& build, const std::s
   if (s != FPDB_OK) return make_unexpected(fpdb_last_error());
   if (produced == 0) return {};
   for (auto& c : outCols) c.length = produced;
+  for (size_t j = 0; j < os.fields.size(); ++j) {
+    if (os.fields[j].type != FPDB_TYPE_UTF8) continue;
+    const Column& src = j < nBuild ? bts.column(buildOut_[j]) : probe->column(probeOut_[j - nBuild]);
+    auto t = takeUtf8(src, outCols[j].c(), produced);
+    if (!t) return make_unexpected(t.error());
+    outCols[j] = *t;
+  }
   return buffer(TupleSet::make(os, outCols));
 }
 
@@ -209,8 +249,18 @@ Expected<std::vector<Column>> HashJoinProbeKernel::gatherSide(const TupleSet& in
   fpdb_context* ctx = defaultContext();
   std::vector<Column> outCols;
   std::vector<fpdb_column> src, dst;
+  fpdb_column rowCol{};
+  rowCol.type = FPDB_TYPE_INT64;
+  rowCol.length = count;
+  rowCol.values = const_cast<int64_t*>(d_rows);
   for (size_t j = 0; j < cols.size(); ++j) {
     const Field& f = outputSchema_->fields[firstField + j];
+    if (in.column(cols[j]).isUtf8()) {
+      auto t = takeUtf8(in.column(cols[j]), rowCol, count);
+      if (!t) return make_unexpected(t.error());
+      outCols.push_back(*t);
+      continue;
+    }
     outCols.push_back(Column::allocate(ctx, f.type, count, f.nullable));
     src.push_back(in.column(cols[j]).c());
     dst.push_back(outCols.back().c());
@@ -228,11 +278,15 @@ Expected<std::vector<Column>> HashJoinProbeKernel::gatherSide(const TupleSet& in
 static Expected<int64_t> sideRows(TupleSetIndex& side, const TupleSet* other, const std::vector<std::string>& otherKeys,
                                   bool matched, DeviceBuffer& rows) {
   fpdb_context* ctx = defaultContext();
-  auto table = side.table(true);
+  auto table = side.table(true, keyTypes(other->schema(), otherKeys));
   if (!table) return make_unexpected(table.error());
-  if (other && other->numRows() > 0) {
-    std::vector<fpdb_column> keys;
-    for (auto& n : otherKeys) keys.push_back(other->column(other->schema().getFieldIndex(n)).c());
+  if (other->numRows() > 0) {
+    std::vector<fpdb_column> raw;
+    for (auto& n : otherKeys) raw.push_back(other->column(other->schema().getFieldIndex(n)).c());
+    Column keyIds;
+    auto pk = side.probeKeys(raw, other->numRows(), &keyIds);
+    if (!pk) return make_unexpected(pk.error());
+    std::vector<fpdb_column> keys = *pk;
     int64_t produced = 0;
     if (fpdb_hj_probe(ctx, *table, FPDB_JOIN_LEFT_SEMI, keys.data(), (int)keys.size(), other->numRows(), nullptr, 0,
                       nullptr, 0, nullptr, 0, &produced) != FPDB_OK)

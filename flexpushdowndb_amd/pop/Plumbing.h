@@ -28,17 +28,25 @@ class FilterPOp : public PhysicalOp {
             std::string column, fpdb_cmp_op op, int64_t intConstant, double floatConstant = 0.0)
       : PhysicalOp(name, POpType::FILTER, projectColumnNames, nodeId), column_(std::move(column)), op_(op),
         intConstant_(intConstant), floatConstant_(floatConstant) {}
+  // `column op stringConstant` on a utf8 column (bytewise order)
+  FilterPOp(const std::string& name, const std::vector<std::string>& projectColumnNames, int nodeId,
+            std::string column, fpdb_cmp_op op, std::string stringConstant)
+      : PhysicalOp(name, POpType::FILTER, projectColumnNames, nodeId), column_(std::move(column)), op_(op),
+        intConstant_(0), floatConstant_(0.0), hasString_(true), stringConstant_(std::move(stringConstant)) {}
   std::string getTypeString() const override { return "FilterPOp"; }
   void onReceive(const Envelope& msg) override;
   void clear() override {}
   static Expected<std::shared_ptr<TupleSet>> filter(const std::shared_ptr<TupleSet>& ts, const std::string& column,
-                                                    fpdb_cmp_op op, int64_t intConstant, double floatConstant);
+                                                    fpdb_cmp_op op, int64_t intConstant, double floatConstant,
+                                                    const std::string* stringConstant = nullptr);
 
  private:
   std::string column_;
   fpdb_cmp_op op_;
   int64_t intConstant_;
   double floatConstant_;
+  bool hasString_ = false;
+  std::string stringConstant_;
 };
 
 class CollatePOp : public PhysicalOp {

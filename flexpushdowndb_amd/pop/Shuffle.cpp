@@ -1,6 +1,7 @@
 #include "Shuffle.h"
 
 #include "Globals.h"
+#include "KeyDictionary.h"
 
 namespace fpdb::amd {
 
@@ -11,6 +12,7 @@ Expected<std::vector<std::shared_ptr<TupleSet>>> ShuffleKernel2::shuffle(
   for (const auto& name : columnNames) {
     int i = tupleSet->schema().getFieldIndex(name);
     if (i < 0) return make_unexpected("Column '" + name + "' not found when creating RecordBatchHasher");
+    if (tupleSet->column(i).isUtf8()) return make_unexpected("Shuffle on the utf8 column '" + name + "' is not supported");
     keys.push_back(tupleSet->column(i).c());
   }
   if (numSlots == 0) return make_unexpected("ShuffleKernel2: no slots");
@@ -18,12 +20,29 @@ Expected<std::vector<std::shared_ptr<TupleSet>>> ShuffleKernel2::shuffle(
   const int64_t n = tupleSet->numRows();
   std::vector<fpdb_column> in, out;
   std::vector<Column> outCols;
+  // utf8 columns travel as row numbers through the shuffle and are taken
+  // afterwards in the shuffled order
+  int rowsAt = -1;
+  Column rows, rowsOut;
   try {
     for (int i = 0; i < tupleSet->numColumns(); ++i) {
       const Column& c = tupleSet->column(i);
+      if (c.isUtf8()) {
+        outCols.push_back(Column());
+        continue;
+      }
       in.push_back(c.c());
       outCols.push_back(Column::allocate(ctx, c.type, n, c.validity != nullptr));
       out.push_back(outCols.back().c());
+    }
+    for (auto& c : tupleSet->columns())
+      if (c.isUtf8()) rowsAt = 0;
+    if (rowsAt == 0) {
+      rows = rowNumbers(n);
+      rowsOut = Column::allocate(ctx, FPDB_TYPE_INT64, n, false);
+      rowsAt = (int)in.size();
+      in.push_back(rows.c());
+      out.push_back(rowsOut.c());
     }
   } catch (const std::exception& e) {
     return make_unexpected(e.what());
@@ -32,6 +51,15 @@ Expected<std::vector<std::shared_ptr<TupleSet>>> ShuffleKernel2::shuffle(
   if (fpdb_shuffle(ctx, keys.data(), (int)keys.size(), in.data(), (int)in.size(), n, (int)numSlots, out.data(),
                    nullptr, offsets.data()) != FPDB_OK)
     return make_unexpected(fpdb_last_error());
+  if (rowsAt >= 0) {
+    fpdb_column rc = rowsOut.c();
+    for (int i = 0; i < tupleSet->numColumns(); ++i) {
+      if (!tupleSet->column(i).isUtf8()) continue;
+      auto t = takeUtf8(tupleSet->column(i), rc, n);
+      if (!t) return make_unexpected(t.error());
+      outCols[i] = *t;
+    }
+  }
   // slice the shuffled table into one TupleSet per slot (ShuffleKernel2.cpp:107-119)
   std::vector<std::shared_ptr<TupleSet>> slots;
   for (size_t p = 0; p < numSlots; ++p) {

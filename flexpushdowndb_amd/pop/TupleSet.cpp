@@ -1,5 +1,6 @@
 #include "TupleSet.h"
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <sstream>
@@ -55,6 +56,7 @@ std::string typeName(fpdb_type t) {
     case FPDB_TYPE_FLOAT64: return "double";
     case FPDB_TYPE_DATE32: return "date32";
     case FPDB_TYPE_DATE64: return "date64";
+    case FPDB_TYPE_UTF8: return "utf8";
   }
   return "?";
 }
@@ -82,7 +84,18 @@ fpdb_column Column::c() const {
   col.offset = offset;
   col.values = values ? values->data() : nullptr;
   col.validity = validity ? static_cast<uint8_t*>(validity->data()) : nullptr;
+  col.data = data ? data->data() : nullptr;
+  col.data_bytes = data ? (int64_t)data->size() : 0;
   return col;
+}
+
+int64_t Column::utf8Bytes() const {
+  if (!isUtf8() || length == 0) return 0;
+  int32_t ends[2];
+  const uint8_t* offs = static_cast<const uint8_t*>(values->data());
+  checkStatus(fpdb_copy_to_host(defaultContext(), &ends[0], offs + offset * 4, 4), "copy");
+  checkStatus(fpdb_copy_to_host(defaultContext(), &ends[1], offs + (offset + length) * 4, 4), "copy");
+  return (int64_t)ends[1] - ends[0];
 }
 
 Column Column::slice(int64_t begin, int64_t rows) const {
@@ -92,7 +105,27 @@ Column Column::slice(int64_t begin, int64_t rows) const {
   return s;
 }
 
+Column Column::allocateUtf8(fpdb_context* ctx, int64_t length, int64_t dataBytes, bool nullable) {
+  Column c;
+  c.type = FPDB_TYPE_UTF8;
+  c.length = length;
+  c.values = std::make_shared<DeviceBuffer>(ctx, (uint64_t)(length + 1) * 4);
+  checkStatus(fpdb_fill(ctx, c.values->data(), 0, c.values->size()), "fpdb_fill");
+  c.data = std::make_shared<DeviceBuffer>(ctx, (uint64_t)std::max<int64_t>(dataBytes, 1));
+  if (nullable) {
+    c.validity = std::make_shared<DeviceBuffer>(ctx, validityBytes(length));
+    checkStatus(fpdb_fill(ctx, c.validity->data(), 0, c.validity->size()), "fpdb_fill");
+  }
+  return c;
+}
+
+Column Column::allocateLike(fpdb_context* ctx, const Column& src, int64_t length) {
+  if (src.isUtf8()) return allocateUtf8(ctx, length, src.utf8Bytes(), src.validity != nullptr);
+  return allocate(ctx, src.type, length, src.validity != nullptr);
+}
+
 Column Column::allocate(fpdb_context* ctx, fpdb_type type, int64_t length, bool nullable) {
+  if (type == FPDB_TYPE_UTF8) return allocateUtf8(ctx, length, 0, nullable);
   Column c;
   c.type = type;
   c.length = length;
@@ -108,6 +141,23 @@ Column Column::allNull(fpdb_context* ctx, fpdb_type type, int64_t length) {
   Column c = allocate(ctx, type, length, true);
   checkStatus(fpdb_fill(ctx, c.values->data(), 0, c.values->size()), "fpdb_fill");
   return c;
+}
+
+HostColumn HostColumn::utf8(const std::vector<std::string>& strings) {
+  HostColumn h;
+  h.type = FPDB_TYPE_UTF8;
+  h.length = (int64_t)strings.size();
+  h.offsets.push_back(0);
+  for (auto& s : strings) {
+    h.bytes.insert(h.bytes.end(), s.begin(), s.end());
+    h.offsets.push_back((int32_t)h.bytes.size());
+  }
+  return h;
+}
+
+std::string HostColumn::stringAt(int64_t i) const {
+  if (type != FPDB_TYPE_UTF8) return std::to_string(int64At(i));
+  return std::string(reinterpret_cast<const char*>(bytes.data()) + offsets[i], (size_t)(offsets[i + 1] - offsets[i]));
 }
 
 int64_t HostColumn::int64At(int64_t i) const {
@@ -155,8 +205,21 @@ std::shared_ptr<TupleSet> TupleSet::fromHost(fpdb_context* ctx, const Schema& sc
   for (size_t i = 0; i < hcols.size(); ++i) {
     const HostColumn& h = hcols[i];
     bool nullable = !h.valid.empty();
-    Column c = Column::allocate(ctx, schema.fields[i].type, h.length, nullable);
-    checkStatus(fpdb_copy_to_device(ctx, c.values->data(), h.bytes.data(), h.bytes.size()), "copy");
+    Column c;
+    if (schema.fields[i].type == FPDB_TYPE_UTF8) {
+      if ((int64_t)h.offsets.size() != h.length + 1) throw std::runtime_error("TupleSet::fromHost: utf8 offsets");
+      c = Column::allocateUtf8(ctx, h.length, (int64_t)h.bytes.size(), nullable);
+      std::vector<int32_t> offs(h.offsets.begin(), h.offsets.end());
+      for (auto& o : offs) o -= h.offsets[0];
+      checkStatus(fpdb_copy_to_device(ctx, c.values->data(), offs.data(), offs.size() * 4), "copy");
+      if (!h.bytes.empty())
+        checkStatus(fpdb_copy_to_device(ctx, c.data->data(), h.bytes.data() + h.offsets[0],
+                                        (uint64_t)(h.offsets.back() - h.offsets[0])),
+                    "copy");
+    } else {
+      c = Column::allocate(ctx, schema.fields[i].type, h.length, nullable);
+      checkStatus(fpdb_copy_to_device(ctx, c.values->data(), h.bytes.data(), h.bytes.size()), "copy");
+    }
     if (nullable) {
       std::vector<uint8_t> bits(validityBytes(h.length), 0);
       for (int64_t r = 0; r < h.length; ++r)
@@ -178,7 +241,8 @@ Expected<Column> TupleSet::getColumnByName(const std::string& name) const {
 static Column concatColumns(fpdb_context* ctx, const Column& a, const Column& b) {
   int64_t n = a.length + b.length;
   bool nullable = a.validity || b.validity;
-  Column out = Column::allocate(ctx, a.type, n, nullable);
+  Column out = a.isUtf8() ? Column::allocateUtf8(ctx, n, a.utf8Bytes() + b.utf8Bytes(), nullable)
+                          : Column::allocate(ctx, a.type, n, nullable);
   fpdb_column o = out.c();
   fpdb_column ia = a.c(), ib = b.c();
   checkStatus(fpdb_copy_rows(ctx, &ia, 1, a.length, &o, 0), "fpdb_copy_rows");
@@ -238,12 +302,26 @@ std::vector<HostColumn> TupleSet::toHost() const {
     HostColumn h;
     h.type = c.type;
     h.length = c.length;
-    h.bytes.resize((size_t)c.length * typeWidth(c.type));
-    const int w = typeWidth(c.type);
-    if (c.length)
-      checkStatus(fpdb_copy_to_host(ctx, h.bytes.data(), static_cast<uint8_t*>(c.values->data()) + c.offset * w,
-                                    h.bytes.size()),
+    if (c.isUtf8()) {
+      h.offsets.resize((size_t)c.length + 1, 0);
+      checkStatus(fpdb_copy_to_host(ctx, h.offsets.data(), static_cast<uint8_t*>(c.values->data()) + c.offset * 4,
+                                    h.offsets.size() * 4),
                   "copy");
+      const int32_t base = h.offsets[0];
+      for (auto& o : h.offsets) o -= base;
+      h.bytes.resize((size_t)h.offsets.back());
+      if (!h.bytes.empty())
+        checkStatus(fpdb_copy_to_host(ctx, h.bytes.data(), static_cast<uint8_t*>(c.data->data()) + base,
+                                      h.bytes.size()),
+                    "copy");
+    } else {
+      h.bytes.resize((size_t)c.length * typeWidth(c.type));
+      const int w = typeWidth(c.type);
+      if (c.length)
+        checkStatus(fpdb_copy_to_host(ctx, h.bytes.data(), static_cast<uint8_t*>(c.values->data()) + c.offset * w,
+                                      h.bytes.size()),
+                    "copy");
+    }
     if (c.validity && c.length) {
       const int64_t b0 = c.offset >> 3, sh = c.offset & 7;
       std::vector<uint8_t> bits((sh + c.length + 7) / 8);
@@ -265,6 +343,7 @@ std::string TupleSet::showString(int64_t maxRows) const {
     for (size_t c = 0; c < host.size(); ++c) {
       o << (c ? " | " : "");
       if (!host[c].isValid(r)) o << "null";
+      else if (host[c].type == FPDB_TYPE_UTF8) o << host[c].stringAt(r);
       else if (host[c].type == FPDB_TYPE_FLOAT64 || host[c].type == FPDB_TYPE_FLOAT32) o << host[c].doubleAt(r);
       else o << host[c].int64At(r);
     }

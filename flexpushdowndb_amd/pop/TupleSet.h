@@ -57,29 +57,46 @@ struct Schema {
 
 // One column of a TupleSet: a single contiguous device chunk, or a slice of
 // one (rows [offset, offset + length) of shared buffers, like arrow::Array::Slice).
+// A utf8 column keeps Arrow's layout: `values` holds int32 offsets (one more
+// than its rows), `data` the characters.
 struct Column {
   fpdb_type type = FPDB_TYPE_INT64;
   int64_t length = 0;
   int64_t offset = 0;
   std::shared_ptr<DeviceBuffer> values;
   std::shared_ptr<DeviceBuffer> validity;   // null: no nulls
+  std::shared_ptr<DeviceBuffer> data;       // utf8 characters
   fpdb_column c() const;
+  bool isUtf8() const { return type == FPDB_TYPE_UTF8; }
+  // Bytes of characters in rows [offset, offset + length) (reads two offsets
+  // back from the device); 0 for other types.
+  int64_t utf8Bytes() const;
 
   static Column allocate(fpdb_context* ctx, fpdb_type type, int64_t length, bool nullable);
+  // A utf8 column with room for `dataBytes` characters.
+  static Column allocateUtf8(fpdb_context* ctx, int64_t length, int64_t dataBytes, bool nullable);
+  // Room for `length` rows taken from `src` without repeats (a compaction or a
+  // reordering): same type and nullability, and for utf8 src's characters.
+  static Column allocateLike(fpdb_context* ctx, const Column& src, int64_t length);
   static Column allNull(fpdb_context* ctx, fpdb_type type, int64_t length);
   // Rows [begin, begin + rows) of this column, sharing its buffers.
   Column slice(int64_t begin, int64_t rows) const;
 };
 
-// Host copy of a column, for tests and display.
+// Host copy of a column, for tests and display.  utf8: `bytes` holds the
+// characters and `offsets` (length + 1 entries, from 0) the row boundaries.
 struct HostColumn {
   fpdb_type type;
-  std::vector<uint8_t> bytes;   // length * width
+  std::vector<uint8_t> bytes;   // length * width (utf8: the characters)
   std::vector<bool> valid;      // empty: all valid
+  std::vector<int32_t> offsets; // utf8 only
   int64_t length = 0;
   int64_t int64At(int64_t i) const;
   double doubleAt(int64_t i) const;
+  std::string stringAt(int64_t i) const;
   bool isValid(int64_t i) const { return valid.empty() || valid[i]; }
+  // A utf8 column of the given strings (nullopt-free: use `valid` for nulls).
+  static HostColumn utf8(const std::vector<std::string>& strings);
 };
 
 class TupleSet {

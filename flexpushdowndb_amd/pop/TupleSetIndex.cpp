@@ -11,6 +11,8 @@ void TupleSetIndex::dropTable() {
   if (table_) fpdb_hj_table_destroy(table_);
   table_ = nullptr;
   tableTracks_ = false;
+  dict_.reset();
+  keyIds_ = Column();
 }
 
 Expected<std::shared_ptr<TupleSetIndex>> TupleSetIndex::make(const std::vector<std::string>& columnNames,
@@ -44,11 +46,27 @@ Expected<void> TupleSetIndex::merge(const std::shared_ptr<TupleSetIndex>& other)
   return put(other->tupleSet_);
 }
 
-Expected<fpdb_hash_table*> TupleSetIndex::table(bool trackMatches) {
+Expected<fpdb_hash_table*> TupleSetIndex::table(bool trackMatches, const std::vector<fpdb_type>& probeTypes) {
   if (table_ && (tableTracks_ || !trackMatches)) return table_;
   dropTable();
   std::vector<fpdb_column> keys;
-  for (auto& n : columnNames_) keys.push_back(tupleSet_->column(tupleSet_->schema().getFieldIndex(n)).c());
+  std::vector<fpdb_type> types;
+  for (auto& n : columnNames_) {
+    const int i = tupleSet_->schema().getFieldIndex(n);
+    keys.push_back(tupleSet_->column(i).c());
+    types.push_back(tupleSet_->schema().fields[i].type);
+  }
+  if (!joinKeysNative(types) || !joinKeysNative(probeTypes)) {
+    try {
+      dict_ = std::make_unique<KeyDictionary>(types);
+    } catch (const std::exception& e) {
+      return make_unexpected(e.what());
+    }
+    auto ids = dict_->encode(keys, size(), true, false);
+    if (!ids) return make_unexpected(ids.error());
+    keyIds_ = *ids;
+    keys = {keyIds_.c()};
+  }
   fpdb_status s = fpdb_hj_build(defaultContext(), keys.data(), (int)keys.size(), size(), nullptr,
                                 trackMatches ? 1 : 0, &table_);
   if (s != FPDB_OK) {
@@ -57,6 +75,15 @@ Expected<fpdb_hash_table*> TupleSetIndex::table(bool trackMatches) {
   }
   tableTracks_ = trackMatches;
   return table_;
+}
+
+Expected<std::vector<fpdb_column>> TupleSetIndex::probeKeys(const std::vector<fpdb_column>& keys, int64_t n,
+                                                            Column* holder) {
+  if (!dict_) return keys;
+  auto ids = dict_->encode(keys, n, false, false);
+  if (!ids) return make_unexpected(ids.error());
+  *holder = *ids;
+  return std::vector<fpdb_column>{holder->c()};
 }
 
 }  // namespace fpdb::amd

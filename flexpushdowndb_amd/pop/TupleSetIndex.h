@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "KeyDictionary.h"
 #include "TupleSet.h"
 
 namespace fpdb::amd {
@@ -39,8 +40,14 @@ class TupleSetIndex {
   // The device hash table over the key columns; trackMatches asks for one
   // matched flag per row (fpdb_hj_build's track_build_matches).  A table built
   // with tracking is returned for either request; asking for tracking after a
-  // table without it was built rebuilds it.  Owned by the index.
-  Expected<fpdb_hash_table*> table(bool trackMatches);
+  // table without it was built rebuilds it.  probeTypes are the key types of
+  // the side that will probe it: when either side's keys are not ones the
+  // join kernels take directly, the table is built over key-dictionary ids
+  // and probes go through probeKeys.  Owned by the index.
+  Expected<fpdb_hash_table*> table(bool trackMatches, const std::vector<fpdb_type>& probeTypes);
+  // Probe key columns as the table takes them: the columns themselves, or
+  // their dictionary ids (held in *holder, which must outlive the probe).
+  Expected<std::vector<fpdb_column>> probeKeys(const std::vector<fpdb_column>& keys, int64_t n, Column* holder);
   // Drops the device table (its matched flags included).
   void dropTable();
 
@@ -49,6 +56,8 @@ class TupleSetIndex {
   std::shared_ptr<TupleSet> tupleSet_;
   fpdb_hash_table* table_ = nullptr;
   bool tableTracks_ = false;
+  std::unique_ptr<KeyDictionary> dict_;
+  Column keyIds_;
 };
 
 }  // namespace fpdb::amd

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -293,6 +294,339 @@ static void testBuildProbeJoin() {
     }
   }
   DefaultBufferSize = savedBuffer;
+}
+
+// ---- utf8 columns and keys the join kernels do not take directly ----------------
+
+// A host table of int64 / utf8 columns; null cells are "NULL", strings "s:..."
+struct HTable {
+  Schema schema;
+  std::vector<std::vector<std::string>> cells;   // [column][row]; "NULL" = null
+  int64_t rows() const { return cells.empty() ? 0 : (int64_t)cells[0].size(); }
+  const std::string& cell(const std::string& name, int64_t r) const {
+    return cells[schema.getFieldIndex(name)][r];
+  }
+  std::shared_ptr<TupleSet> device(int64_t lo, int64_t hi) const {
+    std::vector<HostColumn> hc;
+    for (size_t c = 0; c < schema.fields.size(); ++c) {
+      HostColumn h;
+      if (schema.fields[c].type == FPDB_TYPE_UTF8) {
+        std::vector<std::string> v;
+        for (int64_t r = lo; r < hi; ++r) v.push_back(cells[c][r] == "NULL" ? "" : cells[c][r].substr(2));
+        h = HostColumn::utf8(v);
+      } else {
+        std::vector<int64_t> v;
+        for (int64_t r = lo; r < hi; ++r) v.push_back(cells[c][r] == "NULL" ? 0 : std::stoll(cells[c][r]));
+        h = hostInt64(v);
+      }
+      bool anyNull = false;
+      for (int64_t r = lo; r < hi; ++r) anyNull = anyNull || cells[c][r] == "NULL";
+      if (anyNull)
+        for (int64_t r = lo; r < hi; ++r) h.valid.push_back(cells[c][r] != "NULL");
+      hc.push_back(h);
+    }
+    return TupleSet::fromHost(defaultContext(), schema, hc);
+  }
+  std::vector<std::shared_ptr<TupleSet>> batches(int64_t size) const {
+    std::vector<std::shared_ptr<TupleSet>> out;
+    for (int64_t lo = 0; lo < rows(); lo += size) out.push_back(device(lo, std::min(rows(), lo + size)));
+    return out;
+  }
+};
+
+static std::vector<std::vector<std::string>> cellsOf(const std::shared_ptr<TupleSet>& ts) {
+  auto h = ts->toHost();
+  std::vector<std::vector<std::string>> rows(ts->numRows());
+  for (int64_t r = 0; r < ts->numRows(); ++r)
+    for (auto& c : h)
+      rows[r].push_back(!c.isValid(r) ? "NULL"
+                                      : c.type == FPDB_TYPE_UTF8 ? "s:" + c.stringAt(r) : std::to_string(c.int64At(r)));
+  return rows;
+}
+
+static uint64_t g_rng = 88172645463325252ULL;
+static int64_t rnd(int64_t m) {
+  g_rng ^= g_rng << 13;
+  g_rng ^= g_rng >> 7;
+  g_rng ^= g_rng << 17;
+  return (int64_t)(g_rng % (uint64_t)m);
+}
+
+// Probe table a(aname utf8 null, ak1, ak2, ak3, acity utf8) and build table
+// b(bname utf8 null, bk1, bk2, bk3, bregion utf8); wide keys repeat so joins match.
+static HTable makeJoinTable(const std::string& p, int64_t n, int names) {
+  HTable t;
+  t.schema.fields = {Field{p + "name", FPDB_TYPE_UTF8, true}, Field{p + "k1", FPDB_TYPE_INT64, false},
+                     Field{p + "k2", FPDB_TYPE_INT64, false}, Field{p + "k3", FPDB_TYPE_INT64, false},
+                     Field{p + "txt", FPDB_TYPE_UTF8, false}};
+  t.cells.resize(5);
+  for (int64_t r = 0; r < n; ++r) {
+    int64_t x = rnd(names);
+    t.cells[0].push_back(rnd(10) == 0 ? "NULL" : x == 0 ? "s:" : "s:n\xc3\xa9" + std::to_string(x));
+    t.cells[1].push_back(std::to_string((int64_t)(rnd(3)) * 4000000000LL));   // > 32 bits: not packable in pairs
+    t.cells[2].push_back(std::to_string(-rnd(2)));
+    t.cells[3].push_back(std::to_string(rnd(2)));
+    t.cells[4].push_back("s:" + p + std::to_string(r) + std::string((size_t)rnd(5), 'x'));
+  }
+  return t;
+}
+
+// Expected join rows by nested loop: pairs (probe row, build row), -1 for a
+// missing side, rendered in the output schema's column order.
+static std::vector<std::vector<std::string>> nestedLoopJoin(const HTable& a, const HTable& b, JoinType jt,
+                                                            const std::vector<std::string>& akeys,
+                                                            const std::vector<std::string>& bkeys,
+                                                            const Schema& out) {
+  std::vector<std::pair<int64_t, int64_t>> pairs;
+  std::vector<bool> am(a.rows(), false), bm(b.rows(), false);
+  for (int64_t i = 0; i < a.rows(); ++i)
+    for (int64_t j = 0; j < b.rows(); ++j) {
+      bool eq = true;
+      for (size_t k = 0; k < akeys.size() && eq; ++k) {
+        const std::string &x = a.cell(akeys[k], i), &y = b.cell(bkeys[k], j);
+        eq = x != "NULL" && x == y;
+      }
+      if (!eq) continue;
+      am[i] = bm[j] = true;
+      if (jt == INNER || jt == LEFT || jt == RIGHT || jt == FULL) pairs.push_back({i, j});
+    }
+  for (int64_t j = 0; j < b.rows(); ++j)
+    if ((jt == LEFT || jt == FULL) ? !bm[j] : (jt == LEFT_SEMI && bm[j])) pairs.push_back({-1, j});
+  for (int64_t i = 0; i < a.rows(); ++i)
+    if ((jt == RIGHT || jt == FULL) ? !am[i] : (jt == RIGHT_SEMI && am[i])) pairs.push_back({i, -1});
+  std::vector<std::vector<std::string>> rows;
+  for (auto [i, j] : pairs) {
+    std::vector<std::string> row;
+    for (auto& f : out.fields) {
+      bool probeSide = a.schema.getFieldIndex(f.name) >= 0;
+      int64_t r = probeSide ? i : j;
+      row.push_back(r < 0 ? "NULL" : (probeSide ? a : b).cell(f.name, r));
+    }
+    rows.push_back(row);
+  }
+  std::sort(rows.begin(), rows.end());
+  return rows;
+}
+
+static std::shared_ptr<TupleSet> runGenericJoin(bool legacy, JoinType jt, const HTable& a, const HTable& b,
+                                                const std::vector<std::string>& akeys,
+                                                const std::vector<std::string>& bkeys,
+                                                const std::vector<std::string>& project) {
+  Executor ex;
+  auto sa = std::make_shared<SourcePOp>("scan_a", a.batches(40));
+  auto sb = std::make_shared<SourcePOp>("scan_b", b.batches(35));
+  auto collate = std::make_shared<CollatePOp>("collate");
+  ex.put(sa);
+  ex.put(sb);
+  ex.put(collate);
+  HashJoinPredicate pred{bkeys, akeys};
+  if (legacy) {
+    auto hb = std::make_shared<HashJoinBuildPOp>("build", std::vector<std::string>{}, 0, bkeys);
+    auto hp = std::make_shared<HashJoinProbePOp>("probe", project, 0, pred, jt);
+    ex.put(hb);
+    ex.put(hp);
+    connect(sb, hb);
+    connect(hb, hp);
+    connect(sa, hp);
+    connect(hp, collate);
+  } else {
+    auto join = std::make_shared<HashJoinArrowPOp>("join", project, 0, pred, jt);
+    ex.put(join);
+    sb->produce(join);
+    join->addBuildProducer(sb);
+    sa->produce(join);
+    join->addProbeProducer(sa);
+    connect(join, collate);
+  }
+  auto r = ex.execute();
+  if (!r) {
+    std::printf("  execute failed: %s\n", r.error().c_str());
+    ++g_failures;
+    return nullptr;
+  }
+  return collate->tupleSet().value();
+}
+
+static void testUtf8AndWideKeyJoins() {
+  HTable a = makeJoinTable("a", 150, 12), b = makeJoinTable("b", 120, 15);
+  std::vector<std::string> acols = {"aname", "ak1", "ak2", "ak3", "atxt"}, bcols = {"bname", "bk1", "bk2", "bk3", "btxt"};
+  struct KeyCase {
+    std::vector<std::string> akeys, bkeys;
+  };
+  std::vector<KeyCase> cases = {{{"aname"}, {"bname"}},                       // one utf8 key
+                                {{"ak1", "ak2", "ak3"}, {"bk1", "bk2", "bk3"}},  // three integer keys
+                                {{"ak1", "ak2"}, {"bk1", "bk2"}},                // two 8-byte keys
+                                {{"aname", "ak3"}, {"bname", "bk3"}}};          // utf8 + integer
+  for (bool legacy : {false, true}) {
+    for (size_t kc = 0; kc < cases.size(); ++kc) {
+      for (JoinType jt : {INNER, LEFT, RIGHT, FULL, LEFT_SEMI, RIGHT_SEMI}) {
+        if (legacy && jt == RIGHT_SEMI) continue;
+        std::vector<std::string> project;
+        if (jt != LEFT_SEMI) project.insert(project.end(), acols.begin(), acols.end());
+        if (jt != RIGHT_SEMI) project.insert(project.end(), bcols.begin(), bcols.end());
+        auto got = runGenericJoin(legacy, jt, a, b, cases[kc].akeys, cases[kc].bkeys, project);
+        if (!got) continue;
+        auto rows = cellsOf(got);
+        std::sort(rows.begin(), rows.end());
+        auto want = nestedLoopJoin(a, b, jt, cases[kc].akeys, cases[kc].bkeys, got->schema());
+        if (rows != want)
+          std::printf("  legacy %d keys %zu join %d: %zu rows, want %zu\n", (int)legacy, kc, (int)jt, rows.size(),
+                      want.size());
+        CHECK(rows == want);
+        // output column order: probe first (HashJoinArrowPOp), build first (HashJoinProbePOp)
+        if (jt != LEFT_SEMI && jt != RIGHT_SEMI)
+          CHECK(got->schema().fields[0].name == (legacy ? "bname" : "aname"));
+      }
+    }
+  }
+}
+
+static void testUtf8GroupFilterShuffle() {
+  // group by (utf8 with nulls, int with nulls) over three batches: groups in
+  // first-appearance order, null a group of its own
+  HTable g;
+  g.schema.fields = {Field{"g1", FPDB_TYPE_UTF8, true}, Field{"g2", FPDB_TYPE_INT64, true},
+                     Field{"v", FPDB_TYPE_INT64, false}};
+  g.cells.resize(3);
+  const int64_t n = 3000;
+  for (int64_t r = 0; r < n; ++r) {
+    int64_t x = rnd(9);
+    g.cells[0].push_back(x == 0 ? "NULL" : "s:grp\xe2\x82\xac" + std::to_string(x));
+    g.cells[1].push_back(rnd(7) == 0 ? "NULL" : std::to_string(rnd(3)));
+    g.cells[2].push_back(std::to_string(rnd(1000)));
+  }
+  std::vector<std::pair<std::string, std::string>> order;
+  std::map<std::pair<std::string, std::string>, std::pair<int64_t, int64_t>> agg;
+  for (int64_t r = 0; r < n; ++r) {
+    auto k = std::make_pair(g.cells[0][r], g.cells[1][r]);
+    if (!agg.count(k)) order.push_back(k);
+    agg[k].first += std::stoll(g.cells[2][r]);
+    agg[k].second += 1;
+  }
+  {
+    Executor ex;
+    auto src = std::make_shared<SourcePOp>("scan", g.batches(1000));
+    std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Sum>("s", "v"),
+                                                          std::make_shared<Count>("c")};
+    auto group = std::make_shared<GroupPOp>("group", std::vector<std::string>{"g1", "g2", "s", "c"}, 0,
+                                            std::vector<std::string>{"g1", "g2"}, fns);
+    auto collate = std::make_shared<CollatePOp>("collate");
+    ex.put(src);
+    ex.put(group);
+    ex.put(collate);
+    connect(src, group);
+    connect(group, collate);
+    auto r = ex.execute();
+    if (!r) std::printf("  execute failed: %s\n", r.error().c_str());
+    CHECK(r.has_value());
+    if (r) {
+      auto rows = cellsOf(collate->tupleSet().value());
+      CHECK(rows.size() == order.size());
+      for (size_t i = 0; i < rows.size() && i < order.size(); ++i) {
+        auto& w = agg[order[i]];
+        CHECK(rows[i][0] == order[i].first && rows[i][1] == order[i].second);
+        CHECK(rows[i][2] == std::to_string(w.first) && rows[i][3] == std::to_string(w.second));
+      }
+    }
+  }
+  // three int64 keys whose ranges need more than 64 bits together: the group
+  // operator moves from packed keys to dictionary ids mid-stream
+  {
+    HTable w;
+    w.schema.fields = {Field{"x", FPDB_TYPE_INT64, false}, Field{"y", FPDB_TYPE_INT64, false},
+                       Field{"z", FPDB_TYPE_INT64, false}};
+    w.cells.resize(3);
+    for (int64_t r = 0; r < 2000; ++r) {
+      const int64_t big = r < 1000 ? 1 : (int64_t)1 << 40;   // the second half widens the ranges
+      w.cells[0].push_back(std::to_string(rnd(3) * big));
+      w.cells[1].push_back(std::to_string(-rnd(3) * big));
+      w.cells[2].push_back(std::to_string(rnd(2) * big));
+    }
+    std::vector<std::vector<std::string>> ord;
+    std::map<std::vector<std::string>, int64_t> cnt;
+    for (int64_t r = 0; r < 2000; ++r) {
+      std::vector<std::string> k = {w.cells[0][r], w.cells[1][r], w.cells[2][r]};
+      if (!cnt.count(k)) ord.push_back(k);
+      ++cnt[k];
+    }
+    Executor ex;
+    auto src = std::make_shared<SourcePOp>("scan", w.batches(1000));
+    std::vector<std::shared_ptr<AggregateFunction>> fns = {std::make_shared<Count>("c")};
+    auto group = std::make_shared<GroupPOp>("group", std::vector<std::string>{"x", "y", "z", "c"}, 0,
+                                            std::vector<std::string>{"x", "y", "z"}, fns);
+    auto collate = std::make_shared<CollatePOp>("collate");
+    ex.put(src);
+    ex.put(group);
+    ex.put(collate);
+    connect(src, group);
+    connect(group, collate);
+    auto r = ex.execute();
+    if (!r) std::printf("  execute failed: %s\n", r.error().c_str());
+    CHECK(r.has_value());
+    if (r) {
+      auto rows = cellsOf(collate->tupleSet().value());
+      CHECK(rows.size() == ord.size());
+      for (size_t i = 0; i < rows.size() && i < ord.size(); ++i) {
+        CHECK(std::vector<std::string>(rows[i].begin(), rows[i].begin() + 3) == ord[i]);
+        CHECK(rows[i][3] == std::to_string(cnt[ord[i]]));
+      }
+    }
+  }
+  // a filter on a utf8 column, and a shuffle carrying utf8 columns
+  {
+    auto ts = g.device(0, n);
+    std::string c = "grp\xe2\x82\xac" "3";
+    auto f = FilterPOp::filter(ts, "g1", FPDB_CMP_GE, 0, 0.0, &c);
+    CHECK(f.has_value());
+    if (f) {
+      auto rows = cellsOf(*f);
+      std::vector<std::vector<std::string>> want;
+      auto all = cellsOf(ts);
+      for (auto& row : all)
+        if (row[0] != "NULL" && row[0].substr(2) >= c) want.push_back(row);
+      CHECK(rows == want);
+    }
+    CHECK(!FilterPOp::filter(ts, "g1", FPDB_CMP_EQ, 3, 0.0).has_value());   // a number against utf8
+    auto slots = ShuffleKernel2::shuffle({"v"}, 3, ts);
+    CHECK(slots.has_value());
+    if (slots) {
+      std::vector<std::vector<std::string>> got, want = cellsOf(ts);
+      for (auto& sl : *slots)
+        for (auto& row : cellsOf(sl)) got.push_back(row);
+      std::sort(got.begin(), got.end());
+      std::sort(want.begin(), want.end());
+      CHECK(got == want);
+    }
+    CHECK(!ShuffleKernel2::shuffle({"g1"}, 3, ts).has_value());   // utf8 shuffle keys are refused
+  }
+  // a bloom filter applied to a batch with utf8 columns keeps every matching row whole
+  {
+    auto ts = g.device(0, n);
+    Schema ks;
+    ks.fields = {Field{"v", FPDB_TYPE_INT64, false}};
+    auto keys = TupleSet::fromHost(defaultContext(), ks, {hostInt64({1, 2, 3, 500, 999})});
+    auto bf = ArrowBloomFilter::make(5, {"v"});
+    CHECK(bf->build(keys).has_value());
+    auto kept = BloomFilterUseKernel::filter(ts, bf, {"v"});
+    CHECK(kept.has_value());
+    if (kept) {
+      auto rows = cellsOf(*kept);
+      auto all = cellsOf(ts);
+      std::set<std::vector<std::string>> allSet(all.begin(), all.end());
+      int64_t matches = 0;
+      for (auto& row : all) {
+        int64_t v = std::stoll(row[2]);
+        matches += v == 1 || v == 2 || v == 3 || v == 500 || v == 999;
+      }
+      int64_t keptMatches = 0;
+      for (auto& row : rows) {
+        CHECK(allSet.count(row) == 1);
+        int64_t v = std::stoll(row[2]);
+        keptMatches += v == 1 || v == 2 || v == 3 || v == 500 || v == 999;
+      }
+      CHECK(keptMatches == matches);
+    }
+  }
 }
 
 static void testBloomJoin(bool arrowImpl) {
@@ -1039,6 +1373,8 @@ int main(int argc, char** argv) {
       {"join", testJoin},
       {"outer_semi_joins", testOuterSemiJoins},
       {"build_probe_join", testBuildProbeJoin},
+      {"utf8_and_wide_key_joins", testUtf8AndWideKeyJoins},
+      {"utf8_group_filter_shuffle", testUtf8GroupFilterShuffle},
       {"bloom_join_blocked", [] { testBloomJoin(true); }},
       {"bloom_join_legacy", [] { testBloomJoin(false); }},
       {"group", testGroup},
