@@ -88,6 +88,12 @@ def build_pop_tests():
     _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT}/include", f"-I{POP}", src, "-o", out + ".tmp",
           f"-L{LIB}", "-lfpdb_pop", "-lfpdb_amd", f"-Wl,-rpath,{LIB}", "-Wl,-rpath,$ORIGIN/../../flexpushdowndb_amd/lib"])
     os.replace(out + ".tmp", out)
+    # the Arrow C Data Interface hooks tests/test_arrow_c_data.py loads with ctypes
+    hook = os.path.join(ROOT, "tests", "cpp", "libcdata_roundtrip.so")
+    _run(["g++", "-O2", "-shared", "-fPIC", "-std=c++17", "-Wall", f"-I{ROOT}/include", f"-I{POP}",
+          os.path.join(ROOT, "tests", "cpp", "cdata_roundtrip.cpp"), "-o", hook + ".tmp",
+          f"-L{LIB}", "-lfpdb_pop", "-lfpdb_amd", "-Wl,-rpath,$ORIGIN/../../flexpushdowndb_amd/lib"])
+    os.replace(hook + ".tmp", hook)
     return out
 
 

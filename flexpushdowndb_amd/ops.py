@@ -67,7 +67,12 @@ class BlockedBloomFilter:
         return bm, int(cnt.numpy()[0])
 
     def filter(self, cols, key_index, n, want_bitmap=True):
-        """Fused Find + compaction of all columns (BloomFilterUseKernel.cpp:180-207)."""
+        """Fused Find + compaction of all columns (BloomFilterUseKernel.cpp:180-207);
+        with utf8 columns, Find and then a compaction that carries them."""
+        if any(isinstance(c, Utf8Column) for c in cols):
+            bm, _ = self.find([cols[i] for i in key_index], n)
+            out, kept = filter_compact(self.ctx, cols, n, bm)
+            return out, kept, bm if want_bitmap else None
         out = [DeviceColumn.empty(self.ctx, n, c.dtype, c.validity is not None, c.type_id) for c in cols]
         bm = self.ctx.zeros(max(8, ((n + 63) // 64) * 8), np.uint8) if want_bitmap else None
         ki = (ctypes.c_int * len(key_index))(*key_index)
@@ -145,6 +150,12 @@ def compare_bitmap(ctx, col, op, constant, n, combine=None, combine_or=False, ou
         code |= C.CMP_OR
     bm = out if out is not None else ctx.empty(max(8, (n + 63) // 64 * 8), np.uint8)
     cnt = ctx.zeros(1, np.int64)
+    if isinstance(col, Utf8Column):
+        b = constant.encode() if isinstance(constant, str) else bytes(constant)
+        cc = col.c()
+        C.check(C.lib().fpdb_compare_bitmap_utf8(ctx.handle, ctypes.byref(cc), code, b, len(b),
+                                                 combine.ptr if combine is not None else None, n, bm.ptr, cnt.ptr))
+        return bm, int(cnt.numpy()[0])
     is_f = np.dtype(col.dtype).kind == "f"
     C.check(C.lib().fpdb_compare_bitmap(ctx.handle, ctypes.byref(col.c()), code, 0 if is_f else int(constant),
                                         float(constant) if is_f else 0.0, combine.ptr if combine is not None else None,
@@ -317,12 +328,28 @@ class HashJoinTable:
         return cnt.value
 
     def probe(self, join_type, key_cols, n, probe_out, build_out, capacity=None, out=None):
-        """build_out entries: DeviceColumn, or the string 'payload' for the inline payload."""
+        """build_out entries: DeviceColumn, or the string 'payload' for the inline payload.
+        utf8 columns on either side go through the probe as row numbers and
+        are taken afterwards."""
         outer_build = join_type in (C.JOIN_RIGHT, C.JOIN_FULL)
         if join_type in (C.JOIN_RIGHT_SEMI, C.JOIN_LEFT_SEMI):
             build_out = []
         if join_type == C.JOIN_LEFT_SEMI:
             probe_out = []
+        if out is None and any(isinstance(c, Utf8Column) for c in list(probe_out) + list(build_out)):
+            rows_p = iota(self.ctx, n)
+            nb = self.info()["build_rows"]
+            rows_b = iota(self.ctx, nb)
+            rp = DeviceColumn(rows_p, None, C.TYPE_INT64, n)
+            rb = DeviceColumn(rows_b, None, C.TYPE_INT64, nb)
+            po = [rp if isinstance(c, Utf8Column) else c for c in probe_out]
+            bo = [rb if isinstance(c, Utf8Column) else c for c in build_out]
+            res, m = self.probe(join_type, key_cols, n, po, bo, capacity)
+            src = list(probe_out) + list(build_out)
+            for i, c in enumerate(src):
+                if isinstance(c, Utf8Column):
+                    res[i] = utf8_take(self.ctx, c, res[i], m)
+            return res, m
         if capacity is None:
             capacity = self.probe_count(join_type, key_cols, n)
         bcols = []
@@ -470,14 +497,8 @@ def iota(ctx, n, start=0):
 
 
 def compare_bitmap_utf8(ctx, col, op, constant, n, combine=None, combine_or=False, out=None):
-    """Selection bitmap of `col op constant` (bytewise string order)."""
-    b = constant.encode() if isinstance(constant, str) else bytes(constant)
-    bm = out if out is not None else ctx.zeros(max(8, ((n + 63) // 64) * 8), np.uint8)
-    cc = col.c()
-    C.check(C.lib().fpdb_compare_bitmap_utf8(ctx.handle, ctypes.byref(cc), op | (C.CMP_OR if combine_or else 0),
-                                             b, len(b), combine.ptr if combine is not None else None, n, bm.ptr,
-                                             None))
-    return bm
+    """Selection bitmap of `col op constant` over a utf8 column (bytewise order)."""
+    return compare_bitmap(ctx, col, op, constant, n, combine, combine_or, out)[0]
 
 
 class KeyDict:

@@ -214,24 +214,36 @@ def _device_select(ctx, ops, dcols, clauses, n):
     return bm
 
 
-def device_run(ctx, cols, plan):
+def device_run(ctx, cols, plan, strings=False):
     """The plan on the device; returns the sorted result rows and the row
-    counts after each stage."""
+    counts after each stage.  With strings=True the string columns go to the
+    device as utf8 columns (their dictionary values) and are filtered, joined
+    and grouped as strings: string constants compare bytewise on the device
+    and string group keys go through the key dictionary."""
     from flexpushdowndb_amd import ops, _capi as C
-    from flexpushdowndb_amd.device import DeviceColumn
+    from flexpushdowndb_amd.device import DeviceColumn, Utf8Column
     stats = {}
+
+    def is_str(c):
+        return strings and c + "__dict" in cols
+
+    def up_col(c):
+        if is_str(c):
+            d = cols[c + "__dict"]
+            return Utf8Column.from_strings(ctx, [str(d[v]) for v in cols[c]])
+        return DeviceColumn.from_numpy(ctx, np.ascontiguousarray(cols[c], np.int64))
 
     def up(a):
         return DeviceColumn.from_numpy(ctx, np.ascontiguousarray(a, np.int64))
 
     def enc(clauses):
-        return [[(c, op, encode(cols, c, op, k)) for c, op, k in cl] for cl in clauses]
+        return [[(c, op, (op, k) if is_str(c) else encode(cols, c, op, k)) for c, op, k in cl] for cl in clauses]
 
     # fact table: the foreign keys, the measure and the columns fact predicates use
     n = len(cols["lo_orderdate"])
     fcols = [JOIN_KEYS[t][0] for t in plan["dims"]]
     fpred = sorted({a[0] for cl in clauses_of(plan, "lineorder") for a in cl})
-    dev = {c: up(cols[c]) for c in fcols + fpred}
+    dev = {c: up_col(c) for c in fcols + fpred}
     dev["measure"] = up(measure(cols, plan))
     names = fcols + ["measure"]
     fcl = clauses_of(plan, "lineorder")
@@ -250,7 +262,7 @@ def device_run(ctx, cols, plan):
         dn = len(cols[pk])
         gcols = [g for g in plan["group"] if table_of(g) == t]
         need = sorted({a[0] for cl in clauses_of(plan, t) for a in cl})
-        dd = {c: up(cols[c]) for c in set([pk] + gcols + need)}
+        dd = {c: up_col(c) for c in set([pk] + gcols + need)}
         dcl = clauses_of(plan, t)
         keep = [pk] + gcols
         if dcl:
@@ -306,6 +318,24 @@ def device_run(ctx, cols, plan):
     # group-by
     keys = [cur[g] for g in plan["group"]]
     ktypes = [k.type_id for k in keys]
+    if any(isinstance(k, Utf8Column) for k in keys):
+        # string keys: group by key-dictionary id, then decode the ids
+        kd = ops.KeyDict(ctx, ktypes)
+        ids = kd.encode(keys, n, insert=True, null_is_key=True)
+        ids.validity = None
+        gb = ops.GroupBy(ctx, [C.TYPE_INT32], [(C.AGG_SUM, C.TYPE_INT64, False)], max_groups=max(1, n))
+        gb.consume([ids], [cur["measure"]], n)
+        kcols, aggs = gb.finalize()
+        gb.close()
+        sums = aggs[0].numpy()
+        ng = len(sums)
+        vals = []
+        for j, k in enumerate(keys):
+            dk = kd.decode(j, kcols[0], ng)
+            vals.append(dk.to_list() if isinstance(dk, Utf8Column) else [int(v) for v in dk.numpy()])
+        kd.close()
+        rows = [tuple(v[i] for v in vals) + (int(sums[i]),) for i in range(ng)]
+        return sorted(rows), stats
     packed = None
     if len(keys) > 2 or (len(keys) == 2 and any(np.dtype(k.dtype).itemsize > 4 for k in keys)):
         ranges = [ops.column_min_max(ctx, k, n)[:2] for k in keys]

@@ -20,3 +20,20 @@ def test_device_plan_matches_reference_answer(ctx, q):
     rows, stats = ssb.device_run(ctx, cols, plan)
     print(q, stats, flush=True)
     assert rows == ssb.expected_rows(plan, answers[q])
+
+
+@pytest.mark.parametrize("q", ssb.QUERIES)
+def test_device_plan_with_string_columns_matches_reference_answer(ctx, q):
+    """The same plans with the string columns on the device as utf8 (their
+    values, not dictionary codes): string predicates compare bytewise on the
+    device (Q2.1 p_category = 'MFGR#12', Q2.2 p_brand1 between ...), string
+    columns travel through compaction, bloom filtering and the joins, and
+    string group keys (Q2.1 p_brand1; Q3.1 c_nation, s_nation) are grouped
+    through the key dictionary."""
+    if not CASE:
+        CASE["cols"], CASE["answers"] = ssb.load()
+    cols, answers = CASE["cols"], CASE["answers"]
+    plan = ssb.PLANS[q]
+    rows, stats = ssb.device_run(ctx, cols, plan, strings=True)
+    print(q, stats, flush=True)
+    assert rows == ssb.expected_rows(plan, answers[q])
