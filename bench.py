@@ -483,8 +483,7 @@ def bench_serve(ctx, rates=SERVE_RATES):
     # warm-up, and two queries checked against direct indexing
     warm = datagen.serve_queries(8, seed=99)
     ids = [srv.submit(q) for q in warm]
-    while srv.status(ids[-1]) != 2:
-        srv.step()
+    srv.drain(ids)
     for q, i in list(zip(warm, ids))[:2]:
         got, exp = srv.result(i), datagen.serve_expected(w, q)
         if not all(np.array_equal(got[k], exp[k]) for k in ("g0", "g1", "count", "sum")):
@@ -492,7 +491,8 @@ def bench_serve(ctx, rates=SERVE_RATES):
     out = {"workload": "serving: star schema resident in HBM (60,000,000 fact rows, 4 dimensions of "
                        "30,000/200,000/2,000,000/2,556 rows); each query filters 1-4 dimensions at log-uniform "
                        "selectivity in [0.1%, 50%] and groups by 1-2 dimension attributes; up to 8 queries "
-                       "share one scan", "rates": []}
+                       "share a circular scan of the fact table in 8 chunks, admitted at any chunk",
+           "rates": []}
     for k, (rate, n) in enumerate(rates):
         qs = datagen.serve_queries(n, seed=100 + k)
         arr = poisson_arrivals(n, rate, seed=200 + k)

@@ -86,7 +86,7 @@ class ServeSchema(ctypes.Structure):
 
 
 class ServeConfig(ctypes.Structure):
-    _fields_ = [("max_active", ctypes.c_int32), ("reserved", ctypes.c_int32), ("workspace_bytes", ctypes.c_uint64)]
+    _fields_ = [("max_active", ctypes.c_int32), ("chunks", ctypes.c_int32), ("workspace_bytes", ctypes.c_uint64)]
 
 
 class ServeQuery(ctypes.Structure):

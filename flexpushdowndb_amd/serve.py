@@ -1,6 +1,7 @@
 """Serving loop over a star schema resident in HBM: queries are submitted,
-admitted in FIFO order into shared scans of the fact table (continuous
-batching), and finished together with their scan (fpdb_serve_* entry points)."""
+admitted in FIFO order at any tick of a circular shared scan of the fact
+table (continuous batching), and finished once they have seen every chunk
+(fpdb_serve_* entry points)."""
 import ctypes
 
 import numpy as np
@@ -9,15 +10,20 @@ from . import _capi as C
 from .device import DeviceColumn
 
 
+QUEUED, SCANNING, FINISHED, FAILED = 0, 1, 2, 3
+
+
 class Server:
-    def __init__(self, ctx, w, max_active=8, workspace_bytes=0):
+    def __init__(self, ctx, w, max_active=8, workspace_bytes=0, chunks=0, data_ctx=None):
+        """data_ctx: the context holding the schema's columns (default ctx)."""
         self.ctx = ctx
         self.w = w
         self._cols = []
         n = len(w["measure"])
+        dctx = data_ctx if data_ctx is not None else ctx
 
         def dc(a):
-            c = DeviceColumn.from_numpy(ctx, np.ascontiguousarray(a))
+            c = DeviceColumn.from_numpy(dctx, np.ascontiguousarray(a))
             self._cols.append(c)
             return c.c()
 
@@ -32,7 +38,7 @@ class Server:
             D.fact_fk = dc(w["fact_fk"][d])
         S.fact_measure = dc(w["measure"])
         S.fact_rows = n
-        cfg = C.ServeConfig(max_active, 0, workspace_bytes)
+        cfg = C.ServeConfig(max_active, chunks, workspace_bytes)
         self.handle = ctypes.c_void_p()
         C.check(C.lib().fpdb_serve_create(ctx.handle, ctypes.byref(S), ctypes.byref(cfg), ctypes.byref(self.handle)))
 
@@ -75,6 +81,16 @@ class Server:
         st = ctypes.c_int()
         C.check(C.lib().fpdb_serve_status(self.handle, qid, ctypes.byref(st)))
         return st.value
+
+    def drain(self, ids, max_steps=1_000_000):
+        """Steps until every query in ids is finished or failed; returns the steps taken."""
+        steps = 0
+        while any(self.status(i) < FINISHED for i in ids):
+            self.step()
+            steps += 1
+            if steps > max_steps:
+                raise RuntimeError("serving queries did not finish")
+        return steps
 
     def result(self, qid, capacity=4096):
         g0 = np.zeros(capacity, np.int32)

@@ -699,9 +699,9 @@ typedef struct fpdb_serve_schema {
 } fpdb_serve_schema;
 
 typedef struct fpdb_serve_config {
-  int32_t max_active;         /* queries sharing one scan, 1..8 */
-  int32_t reserved;
-  uint64_t workspace_bytes;   /* HBM budget for the queries in a scan (0: none) */
+  int32_t max_active;         /* queries sharing the scan at once, 1..8 */
+  int32_t chunks;             /* chunks per pass over the fact table (0: 8) */
+  uint64_t workspace_bytes;   /* HBM budget for the queries in the scan (0: none) */
 } fpdb_serve_config;
 
 /* SELECT g0, g1, COUNT(*), SUM(measure) FROM fact JOIN dims WHERE <filters>
@@ -721,25 +721,40 @@ typedef struct fpdb_serve_query {
 typedef struct fpdb_serve fpdb_serve;
 
 /* Registers the resident schema (borrowed: the columns must outlive the
- * server) and checks that dimension keys are dense. */
+ * server) and checks that dimension keys are dense.  The server keeps a 4-byte
+ * copy of each fact foreign-key column (4 B x fact rows per dimension of HBM,
+ * counted against the context's budget) for the scan to read. */
 fpdb_status fpdb_serve_create(fpdb_context* ctx, const fpdb_serve_schema* schema,
                               const fpdb_serve_config* config, fpdb_serve** out);
 fpdb_status fpdb_serve_destroy(fpdb_serve* server);
-/* Queues a query (thread-safe); *query_id identifies it from then on. */
+/* Queues a query (thread-safe, and never waits for a step in progress: the
+ * step holds the server's lock only while it admits and books queries);
+ * *query_id identifies it from then on. */
 fpdb_status fpdb_serve_submit(fpdb_serve* server, const fpdb_serve_query* query, int64_t* query_id);
-/* Admits queued queries (FIFO, within max_active, 8192 groups and the
- * workspace budget), runs one shared scan for them and finishes them;
- * *completed = queries finished (synchronizes). */
+/* One tick of the circular shared scan.  The fact table is read in `chunks`
+ * chunks, one per tick, round and round; a query is admitted at any tick
+ * (FIFO, within max_active, 8192 groups and the workspace budget), joins the
+ * scan at the current chunk and is finished after one full pass, however
+ * many other queries come and go meanwhile.  Its finalize (partials added in
+ * a fixed order, groups sorted by first appearance) and the copy of its
+ * groups to the host run on a second stream, overlapped with the next ticks;
+ * a step collects the queries whose copies have landed.  A query whose
+ * workspace cannot be allocated at admission fails on its own (state 3,
+ * fpdb_serve_result returns the error) without holding up the others.
+ * *completed = queries finished or failed during this call.  With nothing
+ * queued or scanning, a step waits for the finalizes still in flight. */
 fpdb_status fpdb_serve_step(fpdb_serve* server, int* completed);
-/* 0 queued, 1 in a scan, 2 finished. */
+/* 0 queued, 1 in the scan (or being finalized), 2 finished, 3 failed. */
 fpdb_status fpdb_serve_status(fpdb_serve* server, int64_t query_id, int* state);
 /* A finished query's groups in first-appearance order (host arrays; g1 is 0
- * for one group dimension, g0 and g1 are 0 for none). */
+ * for one group dimension, g0 and g1 are 0 for none); for a failed query,
+ * the error that failed it. */
 fpdb_status fpdb_serve_result(fpdb_serve* server, int64_t query_id, int32_t* g0, int32_t* g1, int64_t* count,
                               int64_t* sum, int64_t capacity, int64_t* num_groups);
 /* Replays an arrival trace in real time (arrival_ns non-decreasing, relative
- * to the call): submits each query at its arrival, steps while work is queued,
- * and reports each query's id and latency (completion minus arrival) and the
+ * to the call): submits each query at its arrival, steps while work is queued
+ * or in flight, and reports each query's id and latency (the moment its
+ * result is on the host, minus its arrival; -1 for a failed query) and the
  * whole trace's makespan. */
 fpdb_status fpdb_serve_run_trace(fpdb_serve* server, const fpdb_serve_query* queries, const int64_t* arrival_ns,
                                  int64_t num_queries, int64_t* query_ids, int64_t* latency_ns,
