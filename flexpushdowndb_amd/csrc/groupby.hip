@@ -759,52 +759,76 @@ __global__ __launch_bounds__(1024) void gb_bucket_scan_kernel(const uint32_t* __
 }
 
 // Offsets, second kernel (one workgroup): for bucket b, its range of the new
-// pool starts at nstart[b] and holds its old records (ocount[b], aggregated in
-// place by the compact pass), its head records (head_count[b]), then the tail
-// rows of workgroup 0, 1, ... from tail_start[b] on.  newcount[b] = head +
-// tail records of b; totals[0] = records of the new pool; totals[1] (groups)
-// reset for the compact pass.
-__global__ __launch_bounds__(1024) void gb_bucket_offsets_kernel(int nbuckets, const int64_t* __restrict__ ocount,
+// pool starts at nstart[b] and has room for its old records (ocount[b],
+// aggregated in place by the compact pass), its head records (head_count[b])
+// and its tail rows.  Tail rows either follow the head records in the pool
+// (separate == 0: tail_start[b] = their first pool position, newcount[b] =
+// head + tail records) or live in the tail pool as raw rows (separate == 1:
+// tail_start[b] = their first tail-pool position, bucket-major, newcount[b] =
+// head records).  Each workgroup's tail rows of b start at tail_start[b] +
+// base[b][w].  totals[0] = records of the new pool; totals[1] (groups) reset
+// for the compact pass; totals[2] = tail rows of the batch.
+__global__ __launch_bounds__(1024) void gb_bucket_offsets_kernel(int nbuckets, int separate,
+                                                                 const int64_t* __restrict__ ocount,
                                                                  const unsigned int* __restrict__ head_count,
                                                                  const uint64_t* __restrict__ tail_total,
                                                                  int64_t* __restrict__ nstart,
                                                                  int64_t* __restrict__ newcount,
                                                                  uint64_t* __restrict__ tail_start,
                                                                  unsigned long long* totals) {
-  __shared__ uint64_t s_w[16];
+  __shared__ uint64_t s_w[16], s_t[16];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   constexpr int kPer = kMaxBuckets / 1024;
-  uint64_t rec[kPer];
-  uint64_t mine = 0;
+  uint64_t rec[kPer], tl[kPer];
+  uint64_t mine = 0, mine_t = 0;
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int b = threadIdx.x * kPer + j;
-    rec[j] = b < nbuckets ? (uint64_t)ocount[b] + head_count[b] + tail_total[b] : 0;
+    tl[j] = b < nbuckets ? tail_total[b] : 0;
+    rec[j] = b < nbuckets ? (uint64_t)ocount[b] + head_count[b] + tl[j] : 0;
     mine += rec[j];
+    mine_t += tl[j];
   }
-  uint64_t incl = mine;
+  uint64_t incl = mine, incl_t = mine_t;
   for (int off = 1; off < 64; off <<= 1) {
     const uint64_t t = __shfl_up(incl, off);
-    if (lane >= off) incl += t;
+    const uint64_t tt = __shfl_up(incl_t, off);
+    if (lane >= off) {
+      incl += t;
+      incl_t += tt;
+    }
   }
-  if (lane == 63) s_w[wave] = incl;
+  if (lane == 63) {
+    s_w[wave] = incl;
+    s_t[wave] = incl_t;
+  }
   __syncthreads();
-  uint64_t run = incl - mine;
-  for (int w = 0; w < wave; ++w) run += s_w[w];
+  uint64_t run = incl - mine, trun = incl_t - mine_t;
+  for (int w = 0; w < wave; ++w) {
+    run += s_w[w];
+    trun += s_t[w];
+  }
 #pragma unroll
   for (int j = 0; j < kPer; ++j) {
     const int b = threadIdx.x * kPer + j;
     if (b < nbuckets) {
       nstart[b] = (int64_t)run;
-      newcount[b] = (int64_t)(head_count[b] + tail_total[b]);
-      tail_start[b] = run + (uint64_t)ocount[b] + head_count[b];
+      if (separate) {
+        newcount[b] = (int64_t)head_count[b];
+        tail_start[b] = trun;
+      } else {
+        newcount[b] = (int64_t)(head_count[b] + tl[j]);
+        tail_start[b] = run + (uint64_t)ocount[b] + head_count[b];
+      }
     }
     run += rec[j];
+    trun += tl[j];
   }
   if (threadIdx.x == blockDim.x - 1) {
     totals[0] = (unsigned long long)run;
     totals[1] = 0;
+    totals[2] = (unsigned long long)trun;
   }
 }
 
@@ -949,6 +973,218 @@ __global__ __launch_bounds__(kGroupThreads) void gb_main_kernel(GKeySpec keys, C
   }
 }
 
+// Main pass with the tail rows staged in LDS (fixed accumulator count).
+// A lane-scattered record store costs the texture-address unit about as much
+// as the whole wave's data whatever the number of active lanes, and only about
+// a fifth of the lanes hold a tail row, so storing each tail row from the lane
+// that loaded it makes the stores the bound of the pass.  Instead each wave
+// appends its tail rows, in their raw tail-pool form, to a stage of its own in
+// LDS, and as soon as the stage holds rpw rows (one store instruction's worth)
+// the wave writes them out with every lane active, consecutive lanes writing
+// consecutive parts (16 bytes, or 8 when a row has an odd number of words) of
+// one row; the lane of a row's first part takes the row's tail-pool position
+// (an LDS atomic on its bucket's counter) and passes it to the row's other
+// lanes.  Stages are per wave, so the pass has no workgroup barriers.  A raw
+// tail row is RK + nv words: the key and the batch-relative row in one word
+// (RK = 1, a 4-byte key column: low half the key, high half the row) or two
+// (RK = 2), then the operands of the accumulators that are not COUNT_ALL
+// (those are 1).
+__host__ __device__ constexpr int stage_rows_per_wave(int rr) {
+  // rows a store instruction takes (64 / parts a row), plus one item's 64
+  return 64 / ((rr & 1) ? rr : rr / 2) + 63;
+}
+
+template <int NACC, int KW, bool V8>
+__global__ __launch_bounds__(kGroupThreads) void gb_main_staged_kernel(
+    GKeySpec keys, ColSet vals, AccSpec A, GbGeom G, const uint16_t* __restrict__ g_idx,
+    const uint64_t* __restrict__ head_keys, const int* __restrict__ nhead_p, const uint64_t* __restrict__ base,
+    const uint64_t* __restrict__ tail_start, uint64_t* __restrict__ tpool, uint64_t* __restrict__ hpart) {
+  static_assert(NACC > 0, "fixed accumulator count");
+  constexpr int RK = KW == 4 ? 1 : 2;
+  extern __shared__ uint4 lds_v4[];   // 16-byte aligned: the stages move 16-byte parts
+  uint64_t* lds = reinterpret_cast<uint64_t*>(lds_v4);
+  const int H = G.max_heads;
+  uint16_t* s_idx = reinterpret_cast<uint16_t*>(lds);
+  uint64_t* s_hkeys = lds + kHeadIndexSlots / 4;
+  uint64_t* s_acc = s_hkeys + H;                                                     // NACC x H
+  unsigned long long* s_pos = reinterpret_cast<unsigned long long*>(s_acc + (size_t)NACC * H);   // nbuckets
+  // raw word of each accumulator's operand (-1: COUNT_ALL, operand 1)
+  int vcol[NACC];
+  int nv = 0;
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) vcol[a] = A.op[a] == kAccCountAll ? -1 : RK + nv++;
+  const int rr = RK + nv;   // words of a raw tail row
+  // flush geometry: parts of pw words, ppr parts a row, rpw rows a store
+  // instruction; lane / ppr by a multiply (exact for lane < 64, ppr <= 7)
+  const int pw = (rr & 1) ? 1 : 2;
+  const uint32_t ppr = (uint32_t)(rr / pw);
+  const uint32_t magic = ppr == 1 ? 0u : 0xFFFFFFFFu / ppr + 1u;
+  const uint32_t rpw = 64u / ppr;
+  const int wave_rows = stage_rows_per_wave(rr);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  // stages: 16-byte aligned (H and nbuckets are even, wave_rows * rr is
+  // rounded up to an even word count)
+  const size_t wave_words = ((size_t)wave_rows * rr + 1) & ~(size_t)1;
+  uint64_t* s_stage = reinterpret_cast<uint64_t*>(s_pos + G.nbuckets) + (size_t)wave * wave_words;
+  uint32_t* s_first = reinterpret_cast<uint32_t*>(reinterpret_cast<uint64_t*>(s_pos + G.nbuckets) +
+                                                  (size_t)(kGroupThreads / 64) * wave_words);   // H
+  const int nhead = *nhead_p;
+  for (int h = threadIdx.x; h < nhead; h += blockDim.x) {
+    s_first[h] = 0xFFFFFFFFu;
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) s_acc[(size_t)a * H + h] = acc_identity(A.op[a]);
+  }
+  for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x)
+    s_pos[b] = tail_start[b] + base[(size_t)b * gridDim.x + blockIdx.x];
+  head_index_load(s_idx, s_hkeys, g_idx, head_keys, nhead);   // ends with a barrier
+  LdsTable T;
+  T.keys = s_hkeys;
+  T.first = nullptr;
+  T.acc = s_acc;
+  T.nslots = (uint32_t)H;
+  T.stride = (uint32_t)H;
+  bool dummy = false;
+  constexpr int ITEMS = (int)(kDealRows / kGroupThreads);
+  const uint32_t jl = ppr == 1 ? (uint32_t)lane : __umulhi((uint32_t)lane, magic);   // lane / ppr
+  const uint32_t pp = (uint32_t)lane - jl * ppr;                                      // lane % ppr
+  // writes the stage's first `rows` rows (rows <= rpw) to the tail pool
+  auto flush = [&](uint32_t j0, uint32_t rows) {
+    const bool live = jl < rows;
+    const uint64_t* src = s_stage + (size_t)(j0 + (live ? jl : 0u)) * rr + pp * pw;
+    uint64_t w0 = 0, w1 = 0;
+    if (live) {
+      if (pw == 2) {
+        const uint4 v = *reinterpret_cast<const uint4*>(src);
+        w0 = (uint64_t)v.x | ((uint64_t)v.y << 32);
+        w1 = (uint64_t)v.z | ((uint64_t)v.w << 32);
+      } else {
+        w0 = *src;
+      }
+    }
+    uint64_t pos = 0;
+    if (live && pp == 0) {
+      const uint64_t key = RK == 1 ? (KW == 4 && keys.sgn[0] ? (uint64_t)(int64_t)(int32_t)(uint32_t)w0
+                                                              : (w0 & 0xFFFFFFFFULL))
+                                   : w0;
+      pos = atomicAdd(&s_pos[bucket_of(gb_hash(key), G.bucket_shift)], 1ULL);
+    }
+    pos = __shfl(pos, lane - (int)pp);   // from the lane of the row's first part
+    if (live) {
+      uint64_t* dst = tpool + pos * rr + pp * pw;
+      if (pw == 2)
+        *reinterpret_cast<uint4*>(dst) = make_uint4((uint32_t)w0, (uint32_t)(w0 >> 32), (uint32_t)w1,
+                                                    (uint32_t)(w1 >> 32));
+      else
+        *dst = w0;
+    }
+  };
+  uint32_t fill = 0;   // rows in this wave's stage (wave-uniform, < rpw between items)
+  const int64_t units = gb_units(G.n);
+  // the rows of a step are loaded while the previous step is aggregated
+  uint64_t kk[ITEMS];
+  uint64_t vv[NACC][ITEMS];
+  bool okk[NACC][ITEMS];
+  auto load_unit = [&](int64_t u, uint64_t (&k)[ITEMS], uint64_t (&v)[NACC][ITEMS], bool (&ok)[NACC][ITEMS]) {
+    const int64_t ub = u * kDealRows;
+    const int64_t end = ub + kDealRows < G.n ? ub + kDealRows : G.n;
+    int64_t rc[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int64_t r = ub + (int64_t)i * kGroupThreads + threadIdx.x;
+      rc[i] = r < end ? r : end - 1;
+    }
+    gb_load_rows<ITEMS, NACC, KW, V8>(keys, vals, A, NACC, rc, k, v, ok, &dummy);
+  };
+  if ((int64_t)blockIdx.x < units) load_unit(blockIdx.x, kk, vv, okk);
+  for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+    const int64_t ub = u * kDealRows;
+    const int64_t end = ub + kDealRows < G.n ? ub + kDealRows : G.n;
+    uint64_t nk[ITEMS];
+    uint64_t nv_[NACC][ITEMS];
+    bool nok[NACC][ITEMS];
+    const bool more = u + gridDim.x < units;
+    if (more) load_unit(u + gridDim.x, nk, nv_, nok);
+    int slot[ITEMS];
+    uint64_t hh[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      hh[i] = gb_hash(kk[i]);
+      slot[i] = ub + (int64_t)i * kGroupThreads + threadIdx.x < end ? head_find(s_idx, s_hkeys, kk[i], hh[i]) : -2;
+    }
+    acc_apply_rows<ITEMS, NACC>(T, A, NACC, slot, vv, okk);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const uint32_t rel = (uint32_t)(ub + (int64_t)i * kGroupThreads + threadIdx.x);
+      if (slot[i] >= 0 && rel < s_first[slot[i]]) atomicMin(&s_first[slot[i]], rel);
+      const uint64_t tails = __ballot(slot[i] == -1);
+      if (!tails) continue;
+      if (slot[i] == -1) {
+        uint64_t* rec = s_stage + (size_t)(fill + rank_below(tails)) * rr;
+        if (RK == 1) {
+          rec[0] = (kk[i] & 0xFFFFFFFFULL) | ((uint64_t)rel << 32);
+        } else {
+          rec[0] = kk[i];
+          rec[1] = rel;
+        }
+#pragma unroll
+        for (int a = 0; a < NACC; ++a)
+          if (vcol[a] >= 0) rec[vcol[a]] = okk[a][i] ? vv[a][i] : acc_identity(A.op[a]);
+      }
+      fill += (uint32_t)__popcll(tails);
+      if (fill >= rpw) {
+        uint32_t done = 0;
+        for (; done + rpw <= fill; done += rpw) flush(done, rpw);
+        // move the rest (fewer than rpw rows, all past the flushed ones) to
+        // the front of the stage
+        const uint32_t rest = fill - done;
+        uint64_t w0 = 0, w1 = 0;
+        const bool mv = jl < rest;
+        const uint64_t* src = s_stage + (size_t)(done + (mv ? jl : 0u)) * rr + pp * pw;
+        if (mv) {
+          w0 = src[0];
+          if (pw == 2) w1 = src[1];
+        }
+        if (mv) {
+          uint64_t* dst = s_stage + (size_t)jl * rr + pp * pw;
+          dst[0] = w0;
+          if (pw == 2) dst[1] = w1;
+        }
+        fill = rest;
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        kk[i] = nk[i];
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) {
+          vv[a][i] = nv_[a][i];
+          okk[a][i] = nok[a][i];
+        }
+      }
+    }
+  }
+  if (fill) flush(0, fill);
+  __syncthreads();   // every head row is aggregated
+  uint64_t* out = hpart + (size_t)blockIdx.x * (1 + NACC) * H;
+  for (int h = threadIdx.x; h < nhead; h += blockDim.x) {
+    const uint32_t f = s_first[h];
+    out[h] = f == 0xFFFFFFFFu ? (uint64_t)LLONG_MAX : (uint64_t)(G.row_base + f);
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) out[(size_t)(1 + a) * H + h] = s_acc[(size_t)a * H + h];
+  }
+}
+
+// LDS bytes of the staged main pass's per-wave stages for an accumulator set
+// and key form (RK = 1 for a 4-byte key column, else 2).
+static size_t stage_bytes(const AccSpec& A, int rk) {
+  int nv = 0;
+  for (int a = 0; a < A.n; ++a) nv += A.op[a] == kAccCountAll ? 0 : 1;
+  const int rr = rk + nv;
+  return (size_t)(kGroupThreads / 64) * ((((size_t)stage_rows_per_wave(rr) * rr + 1) & ~(size_t)1) * 8);
+}
+
 // Applies an accumulator value to a register accumulator (merge of partials).
 __device__ __forceinline__ uint64_t acc_combine(int32_t op, uint64_t x, uint64_t v) {
   switch (op) {
@@ -1026,7 +1262,12 @@ __global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
 
 // Compact pass, one workgroup per bucket: old records (previous batches, one
 // per group) and the new ones are aggregated in LDS and written back, one per
-// group, at the start of the bucket's range in the new pool.
+// group, at the start of the bucket's range in the new pool.  The new records
+// are the head records in the pool and the tail rows: after the head records
+// in the pool (tpool == nullptr) or raw in the tail pool (rr words each, key
+// and batch-relative row packed as the staged main pass writes them: rk = 1 a
+// 4-byte key zero-extended, rk = 3 a 4-byte key sign-extended, rk = 2 key and
+// row in two words).
 template <int NACC>
 __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, uint32_t nslots, int rec_words,
                                                                    const uint64_t* __restrict__ oldpool,
@@ -1034,6 +1275,10 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, ui
                                                                    int64_t* __restrict__ ocount,
                                                                    uint64_t* pool, const int64_t* __restrict__ nstart,
                                                                    const int64_t* __restrict__ newcount,
+                                                                   const uint64_t* __restrict__ tpool,
+                                                                   const uint64_t* __restrict__ tail_start,
+                                                                   const uint64_t* __restrict__ tail_total, int rk,
+                                                                   int rr, int64_t row_base,
                                                                    unsigned long long* groups, int* overflow) {
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t s_n;
@@ -1048,19 +1293,46 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, ui
   }
   __syncthreads();
   const int64_t nold = ocount[p], nnew = newcount[p];
+  const int64_t ntail = tpool ? (int64_t)tail_total[p] : 0;
   const int64_t start = nstart[p];
   const uint64_t* news = pool + (size_t)(start + nold) * rec_words;
+  const uint64_t* tails = tpool ? tpool + (size_t)tail_start[p] * rr : nullptr;
   constexpr int RW = NACC > 0 ? 2 + NACC : 2 + kMaxAggs;
+  // raw operand word of each accumulator (-1: COUNT_ALL, operand 1)
+  int vcol[NACC > 0 ? NACC : 1];
+  if constexpr (NACC > 0) {
+    int nv = 0;
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) vcol[a] = A.op[a] == kAccCountAll ? -1 : (rk == 2 ? 2 : 1) + nv++;
+  }
   // R records per thread per trip, all loaded before any is aggregated
   constexpr int R = NACC > 0 ? 4 : 1;
-  const int64_t total = nold + nnew;
-  for (int64_t t0 = 0; t0 < total; t0 += (int64_t)R * blockDim.x) {
+  const int64_t nrec = nold + nnew;
+  auto apply = [&](const uint64_t* w) {
+    const uint64_t k = w[0];
+    const int s = lds_find_or_insert(T, k, fmix64(k), nslots);
+    if (s < 0) {
+      s_over = 1;
+      return;
+    }
+#pragma unroll
+    for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
+      if (a < nacc) {
+        const uint64_t v = w[2 + a];
+        if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
+      }
+    }
+    const long long f = (long long)w[1];
+    if (f < T.first[s]) atomicMin(&T.first[s], f);
+  };
+  // records: old ones, then the new ones in the pool
+  for (int64_t t0 = 0; t0 < nrec; t0 += (int64_t)R * blockDim.x) {
     const int64_t i0 = t0 + threadIdx.x;
     uint64_t w[R][RW];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
       int64_t i = i0 + (int64_t)j * blockDim.x;
-      i = i < total ? i : total - 1;
+      i = i < nrec ? i : nrec - 1;
       const uint64_t* rec = i < nold ? oldpool + (size_t)(ostart[p] + i) * rec_words
                                      : news + (size_t)(i - nold) * rec_words;
       if constexpr (NACC > 0) {
@@ -1070,25 +1342,39 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, ui
       }
     }
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const bool todo = i0 + (int64_t)j * blockDim.x < total;
-      if (!todo) continue;
-      const uint64_t k = w[j][0];
-      const int s = lds_find_or_insert(T, k, fmix64(k), nslots);
-      if (s < 0) {
-        s_over = 1;
-        continue;
+    for (int j = 0; j < R; ++j)
+      if (i0 + (int64_t)j * blockDim.x < nrec) apply(w[j]);
+  }
+  // raw tail rows (every row's words loaded before any is converted; words
+  // are loaded by their runtime position, not picked from a register array,
+  // which would put the array in scratch memory)
+  if constexpr (NACC > 0) {
+    for (int64_t t0 = 0; t0 < ntail; t0 += (int64_t)R * blockDim.x) {
+      const int64_t i0 = t0 + threadIdx.x;
+      uint64_t w[R][RW];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        int64_t i = i0 + (int64_t)j * blockDim.x;
+        i = i < ntail ? i : ntail - 1;
+        const uint64_t* raw = tails + (size_t)i * rr;
+        w[j][0] = raw[0];
+        w[j][1] = rk == 2 ? raw[1] : 0;
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) w[j][2 + a] = vcol[a] < 0 ? 1ULL : raw[vcol[a]];
       }
 #pragma unroll
-      for (int a = 0; a < (NACC > 0 ? NACC : kMaxAggs); ++a) {
-        if (a < nacc) {
-          const uint64_t v = w[j][2 + a];
-          if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF)
-            acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
+      for (int j = 0; j < R; ++j) {
+        if (i0 + (int64_t)j * blockDim.x >= ntail) continue;
+        if (rk == 2) {
+          w[j][1] = (uint64_t)(row_base + (int64_t)(uint32_t)w[j][1]);
+        } else {
+          const uint64_t x0 = w[j][0];
+          const uint32_t k32 = (uint32_t)x0;
+          w[j][0] = rk == 3 ? (uint64_t)(int64_t)(int32_t)k32 : (uint64_t)k32;
+          w[j][1] = (uint64_t)(row_base + (int64_t)(x0 >> 32));
         }
+        apply(w[j]);
       }
-      const long long f = (long long)w[j][1];
-      if (f < T.first[s]) atomicMin(&T.first[s], f);
     }
   }
   __syncthreads();   // every record of the bucket is read before any is overwritten
@@ -1338,6 +1624,8 @@ struct fpdb_group_table {
   int64_t pool_cap = 0;              // records
   uint64_t* spare = nullptr;         // the next pool
   int64_t spare_cap = 0;
+  uint64_t* tpool = nullptr;         // raw tail rows of the current batch (staged main pass)
+  int64_t tpool_words = 0;
   int64_t* ostart = nullptr;         // nbuckets
   int64_t* ocount = nullptr;         // nbuckets
   int64_t* nstart = nullptr;         // nbuckets
@@ -1475,10 +1763,12 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   // batch-relative first row (4 B) and accumulators, per bucket a 64-bit
   // record position; at most a sixth of the index slots hold head keys (the
   // sampler's two-choice placement then rarely drops one)
-  const size_t fixed = (size_t)kHeadIndexSlots * 2 + (size_t)nb * 8;
+  // (plus, with at most 5 accumulators, the staged main pass's tail-row stage)
+  const size_t stage = nacc >= 1 && nacc <= 5 ? std::max(stage_bytes(t->acc, 1), stage_bytes(t->acc, 2)) : 0;
+  const size_t fixed = (size_t)kHeadIndexSlots * 2 + (size_t)nb * 8 + stage;
   const size_t per_head = 8 + 4 + 8 * (size_t)nacc;
   int64_t heads = budget > fixed ? (int64_t)((budget - fixed) / per_head) : 0;
-  heads = std::min<int64_t>(heads, kHeadIndexSlots / 6);
+  heads = std::min<int64_t>(heads, kHeadIndexSlots / 6) & ~int64_t(1);   // even: the stage after them stays 16-byte aligned
   t->max_heads = (int)heads;
   t->main_lds = fixed + per_head * (size_t)heads;
   t->count_lds = (size_t)kHeadIndexSlots * 2 + (size_t)heads * 8 + (size_t)nb * 4;
@@ -1503,11 +1793,11 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
     s = gb_alloc(t, (uint64_t)t->max_blocks * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hpart));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_total));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_start));
-  if (s == FPDB_OK) s = gb_alloc(t, 16, reinterpret_cast<void**>(&t->totals));
+  if (s == FPDB_OK) s = gb_alloc(t, 32, reinterpret_cast<void**>(&t->totals));
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->flags));
   if (s == FPDB_OK) s = launch_fill(c, t->ostart, 0, (uint64_t)nb * 8);
   if (s == FPDB_OK) s = launch_fill(c, t->ocount, 0, (uint64_t)nb * 8);
-  if (s == FPDB_OK) s = launch_fill(c, t->totals, 0, 16);
+  if (s == FPDB_OK) s = launch_fill(c, t->totals, 0, 32);
   if (s == FPDB_OK) s = launch_fill(c, t->flags, 0, 8);
   if (s != FPDB_OK) {
     fpdb_groupby_destroy(t);
@@ -1520,9 +1810,14 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
 template <int NACC, int KW, bool V8>
 static void launch_main_kv(Context* c, fpdb_group_table* t, int blocks, const GKeySpec& k, const ColSet& vals,
                            const GbGeom& G) {
-  hipLaunchKernelGGL((gb_main_kernel<NACC, KW, V8>), dim3(blocks), dim3(kGroupThreads), t->main_lds, c->stream, k,
-                     vals, t->acc, G, t->head_index, t->head_keys, t->nhead, t->base, t->tail_start, t->spare,
-                     t->hpart);
+  if constexpr (NACC > 0)
+    hipLaunchKernelGGL((gb_main_staged_kernel<NACC, KW, V8>), dim3(blocks), dim3(kGroupThreads), t->main_lds,
+                       c->stream, k, vals, t->acc, G, t->head_index, t->head_keys, t->nhead, t->base, t->tail_start,
+                       t->tpool, t->hpart);
+  else
+    hipLaunchKernelGGL((gb_main_kernel<NACC, KW, V8>), dim3(blocks), dim3(kGroupThreads), t->main_lds, c->stream, k,
+                       vals, t->acc, G, t->head_index, t->head_keys, t->nhead, t->base, t->tail_start, t->spare,
+                       t->hpart);
 }
 
 // The key width of the direct key path (one unpacked column of 4 or 8 bytes),
@@ -1558,10 +1853,11 @@ static void launch_main(Context* c, fpdb_group_table* t, int blocks, const GKeyS
     launch_main_kv<NACC, 0, false>(c, t, blocks, k, vals, G);
 }
 template <int NACC>
-static void launch_compact(Context* c, fpdb_group_table* t) {
+static void launch_compact(Context* c, fpdb_group_table* t, int rk, int rr, int64_t row_base) {
   hipLaunchKernelGGL(gb_compact_kernel<NACC>, dim3(t->nbuckets), dim3(kGroupThreads), t->compact_lds, c->stream,
                      t->acc, t->compact_slots, t->rec_words, t->pool, t->ostart, t->ocount, t->spare, t->nstart,
-                     t->newcount, t->totals + 1, t->flags);
+                     t->newcount, NACC > 0 ? t->tpool : nullptr, t->tail_start, t->tail_total, rk, rr, row_base,
+                     t->totals + 1, t->flags);
 }
 
 #define FPDB_GB_DISPATCH(FN, ...)                 \
@@ -1591,6 +1887,27 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
   // the current pool), every head key and every row of the batch.
   const int64_t bound = t->pool_records_bound + t->max_heads + n;
   bool exact = false;
+  // tail rows raw in the tail pool (the staged main pass: 1 to 5 accumulators)
+  const bool separate = t->acc.n >= 1 && t->acc.n <= 5;
+  const int kw = gb_key_width(t, k);
+  int nv = 0;
+  for (int a = 0; a < t->acc.n; ++a) nv += t->acc.op[a] == kAccCountAll ? 0 : 1;
+  const int rr = (kw == 4 ? 1 : 2) + nv;
+  const int rk = kw == 4 ? (k.sgn[0] ? 3 : 1) : 2;
+  bool exact_tail = false;
+  if (separate && n * rr > t->tpool_words) {
+    if (t->tpool) FPDB_TRY(ctx_free(c, t->tpool));
+    t->tpool = nullptr;
+    t->tpool_words = 0;
+    fpdb_status s = gb_alloc(t, (uint64_t)n * rr * 8, reinterpret_cast<void**>(&t->tpool));
+    if (s == FPDB_ERR_BUDGET) {
+      (void)fpdb_last_error();
+      exact_tail = true;   // size it from the count pass instead
+    } else {
+      FPDB_TRY(s);
+      t->tpool_words = n * rr;
+    }
+  }
   if (bound > t->spare_cap) {
     if (t->spare) FPDB_TRY(ctx_free(c, t->spare));
     t->spare = nullptr;
@@ -1613,22 +1930,29 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
                          t->head_rank, t->head_count, t->nhead);
     }
     ProfScope ps(c, "groupby_count");
-    const int kw = gb_key_width(t, k);
     auto count_kernel = kw == 8 ? gb_count_kernel<8> : (kw == 4 ? gb_count_kernel<4> : gb_count_kernel<0>);
     hipLaunchKernelGGL(count_kernel, dim3(blocks), dim3(kGroupThreads), t->count_lds, c->stream, k, G,
                        t->head_index, t->head_keys, t->nhead, t->hist, t->flags + 1);
     hipLaunchKernelGGL(gb_bucket_scan_kernel, dim3((unsigned)ceil_div(t->nbuckets, 16)), dim3(1024), 0, c->stream,
                        t->hist, blocks, t->nbuckets, t->base, t->tail_total);
-    hipLaunchKernelGGL(gb_bucket_offsets_kernel, dim3(1), dim3(1024), 0, c->stream, t->nbuckets, t->ocount,
+    hipLaunchKernelGGL(gb_bucket_offsets_kernel, dim3(1), dim3(1024), 0, c->stream, t->nbuckets, separate ? 1 : 0,
+                       t->ocount,
                        t->head_count, t->tail_total, t->nstart, t->newcount, t->tail_start, t->totals);
     FPDB_HIP(hipGetLastError());
   }
-  if (exact) {
-    uint64_t total = 0;
-    FPDB_TRY(mail_read(c, t->totals, 1, &total));
-    const int64_t want = std::max<int64_t>((int64_t)total, 1024);
-    FPDB_TRY(gb_alloc(t, (uint64_t)want * t->rec_words * 8, reinterpret_cast<void**>(&t->spare)));
-    t->spare_cap = want;
+  if (exact || exact_tail) {
+    uint64_t tot[3] = {0, 0, 0};
+    FPDB_TRY(mail_read(c, t->totals, 3, tot));
+    if (exact) {
+      const int64_t want = std::max<int64_t>((int64_t)tot[0], 1024);
+      FPDB_TRY(gb_alloc(t, (uint64_t)want * t->rec_words * 8, reinterpret_cast<void**>(&t->spare)));
+      t->spare_cap = want;
+    }
+    if (exact_tail) {
+      const int64_t want = std::max<int64_t>((int64_t)tot[2], 1) * rr;
+      FPDB_TRY(gb_alloc(t, (uint64_t)want * 8, reinterpret_cast<void**>(&t->tpool)));
+      t->tpool_words = want;
+    }
   }
   {
     ProfScope ps(c, "groupby_main");
@@ -1641,7 +1965,7 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
                        (size_t)kMergeWaves * (1 + t->acc.n) * 64 * 8,
                        c->stream, t->acc, G, blocks, t->head_keys, t->head_rank, t->nhead, t->hpart, t->nstart,
                        t->ocount, t->spare);
-    FPDB_GB_DISPATCH(launch_compact, c, t);
+    FPDB_GB_DISPATCH(launch_compact, c, t, rk, rr, G.row_base);
     FPDB_HIP(hipGetLastError());
   }
   std::swap(t->pool, t->spare);
@@ -1728,7 +2052,7 @@ fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, in
 
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
-  void* bufs[] = {t->pool, t->spare, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
+  void* bufs[] = {t->pool, t->spare, t->tpool, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
                   t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->tail_total, t->tail_start,
                   t->totals, t->flags};
   for (void* b : bufs)
