@@ -839,6 +839,9 @@ __global__ __launch_bounds__(1024) void gb_bucket_offsets_kernel(int nbuckets, i
 // column of that width (the value itself, sign-extended when signed); V8:
 // every aggregate input is an 8-byte column without nulls (its raw bits are
 // the operand, COUNT operands are 1).  Otherwise the generic column walks.
+// The direct loads are non-temporal: the input streams through once, and
+// keeping it out of the L2 lets the tail rows' partly written lines stay there
+// until they are whole (main pass 0.57 -> 0.45 ms on 60M rows).
 template <int ITEMS, int NA, int KW, bool V8>
 __device__ __forceinline__ void gb_load_rows(const GKeySpec& keys, const ColSet& vals, const AccSpec& A, int nacc,
                                              const int64_t (&rc)[ITEMS], uint64_t (&kk)[ITEMS],
@@ -846,13 +849,13 @@ __device__ __forceinline__ void gb_load_rows(const GKeySpec& keys, const ColSet&
   if constexpr (KW == 8) {
     const uint64_t* kp = reinterpret_cast<const uint64_t*>(keys.v[0]) + keys.off[0];
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) kk[i] = kp[rc[i]];
+    for (int i = 0; i < ITEMS; ++i) kk[i] = __builtin_nontemporal_load(kp + rc[i]);
   } else if constexpr (KW == 4) {
     const uint32_t* kp = reinterpret_cast<const uint32_t*>(keys.v[0]) + keys.off[0];
     const bool sgn = keys.sgn[0] != 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      const uint32_t x = kp[rc[i]];
+      const uint32_t x = __builtin_nontemporal_load(kp + rc[i]);
       kk[i] = sgn ? (uint64_t)(int64_t)(int32_t)x : (uint64_t)x;
     }
   } else {
@@ -871,7 +874,7 @@ __device__ __forceinline__ void gb_load_rows(const GKeySpec& keys, const ColSet&
         const int in = A.input[a];
         const uint64_t* vp = reinterpret_cast<const uint64_t*>(vals.values[in]) + vals.offset[in];
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) vv[a][i] = vp[rc[i]];
+        for (int i = 0; i < ITEMS; ++i) vv[a][i] = __builtin_nontemporal_load(vp + rc[i]);
       }
     }
   } else {
