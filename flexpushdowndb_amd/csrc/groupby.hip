@@ -378,12 +378,10 @@ __device__ __forceinline__ void load_rec(const uint64_t* rec, uint64_t (&w)[RW])
 
 constexpr int kSampleBlocks = 32;
 constexpr int kSampleBlockRows = 1024;
-constexpr int kSampleSlots = 8192;           // the sampler's LDS count table
 constexpr int kHeadIndexSlots = 16384;       // head index: 16-bit head id per slot
 constexpr uint32_t kNoHead = 0xFFFFFFFFu;
 constexpr uint32_t kNoHead16 = 0xFFFFu;
 constexpr int kCountBins = 512;
-constexpr int kSampleProbes = 8;
 constexpr int64_t kDealRows = 4096;          // rows dealt to a workgroup at a time (count and main passes)
 
 // Hash of a group key for its bucket (top bits) and its head-index home
@@ -468,39 +466,147 @@ __device__ __forceinline__ void head_index_load(uint16_t* s_idx, uint64_t* s_hke
   __syncthreads();
 }
 
-__device__ __forceinline__ uint32_t sample_home(uint64_t h) { return (uint32_t)(h >> 40) & (kSampleSlots - 1); }
 
-// Sample: keys of up to kSampleBlocks x kSampleBlockRows rows (all rows of a
-// smaller batch; otherwise kSampleBlocks evenly spaced runs of consecutive
-// rows) counted in an LDS table; up to max_heads keys with the highest counts
-// (ties at the threshold count in any order) are placed in the head index.
+// Sample, in three kernels over kSampleBlocks x kSampleBlockRows rows (all
+// rows of a smaller batch; otherwise kSampleBlocks evenly spaced runs of
+// consecutive rows).
+//   count  (one workgroup per run) counts its run's keys in an LDS table, then
+//          adds each distinct key's count to the global count table (open
+//          addressing on a 64-bit CAS of the key; the key equal to the empty
+//          marker counts in the reserved last slot);
+//   gather (a workgroup per 1024 slots) empties the count table into a list
+//          of (key, count) and a global histogram of the counts;
+//   select (one workgroup) picks up to max_heads keys with the highest counts
+//          (ties at the threshold count in any order), places them in the head
+//          index and empties the list and histogram for the next batch.
 // Outputs: the index, the head keys (ids 0 .. nhead - 1), per head key its
-// rank among its bucket's head keys, per bucket its head count.  One
-// workgroup of 1024 threads.  LDS: the count table's keys (later the 32-bit
-// index under construction), its counts (later the head ids' new numbers),
-// the count histogram, the bucket counts and the selected keys.
-__global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G, uint16_t* __restrict__ g_idx,
-                                                         uint64_t* __restrict__ head_keys,
-                                                         uint32_t* __restrict__ head_rank,
-                                                         unsigned int* __restrict__ head_count, int* nhead_out) {
+// rank among its bucket's head keys, per bucket its head count.  A single
+// workgroup reading the whole count table would wait for memory once per
+// handful of slots; the gather spreads that over the CUs.
+constexpr int kSampleLocal = 2048;           // a run's LDS count table (>= 2 x kSampleBlockRows)
+constexpr int kSampleTable = 1 << 16;        // global count table (>= 2 x the sampled rows)
+constexpr int kSampleListMax = kSampleBlocks * kSampleBlockRows + 1;   // distinct sampled keys, at most
+__global__ __launch_bounds__(kSampleBlockRows) void gb_sample_count_kernel(GKeySpec keys, GbGeom G,
+                                                                           unsigned long long* __restrict__ g_keys,
+                                                                           uint32_t* __restrict__ g_cnt) {
+  __shared__ uint64_t s_k[kSampleLocal + 1];
+  __shared__ uint32_t s_c[kSampleLocal + 1];
+  const int lane = threadIdx.x & 63;
+  for (int s = threadIdx.x; s <= kSampleLocal; s += blockDim.x) {
+    s_k[s] = kEmptyKey;
+    s_c[s] = 0;
+  }
+  __syncthreads();
+  const int64_t n = G.n;
+  const bool all = n <= (int64_t)kSampleBlocks * kSampleBlockRows;
+  const int64_t j = blockIdx.x;
+  const int64_t r = all ? j * kSampleBlockRows + threadIdx.x
+                        : (n - kSampleBlockRows) * j / (kSampleBlocks - 1) + threadIdx.x;
+  const bool live = r < n;
+  int64_t rc[1] = {live ? r : n - 1};
+  uint64_t kk[1];
+  bool dummy = false;
+  group_keys_n<1>(keys, rc, kk, &dummy);
+  const uint64_t k = kk[0];
+  int slot = -1;
+  if (live) {
+    if (k == kEmptyKey) {
+      slot = kSampleLocal;
+    } else {
+      uint32_t s = (uint32_t)(gb_hash(k) >> 40) & (kSampleLocal - 1);
+      for (int p = 0; p < kSampleLocal; ++p) {   // at most kSampleBlockRows keys: always a slot
+        const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&s_k[s]), kEmptyKey, k);
+        if (prev == kEmptyKey || prev == k) { slot = (int)s; break; }
+        s = (s + 1) & (kSampleLocal - 1);
+      }
+    }
+  }
+  // skewed keys repeat within a wave: twice, the lanes sharing the slot of the
+  // first lane still to count add with one atomic
+  bool todo = slot >= 0;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const uint64_t act = __ballot(todo);
+    if (!act) break;
+    const int leader = __builtin_ctzll(act);
+    const int ls = __shfl(slot, leader);
+    const bool same = todo && slot == ls;
+    const uint64_t grp = __ballot(same);
+    if (same) {
+      if (lane == leader) atomicAdd(&s_c[ls], (uint32_t)__popcll(grp));
+      todo = false;
+    }
+  }
+  if (todo) atomicAdd(&s_c[slot], 1u);
+  __syncthreads();
+  for (int s = threadIdx.x; s <= kSampleLocal; s += blockDim.x) {
+    const uint32_t c = s_c[s];
+    if (!c) continue;
+    uint32_t gs = kSampleTable;
+    if (s < kSampleLocal) {
+      const uint64_t key = s_k[s];
+      gs = (uint32_t)(gb_hash(key) >> 40) & (kSampleTable - 1);
+      for (int p = 0; p < kSampleTable; ++p) {   // at most the sampled rows' keys: always a slot
+        const unsigned long long prev = atomicCAS(&g_keys[gs], kEmptyKey, key);
+        if (prev == kEmptyKey || prev == key) break;
+        gs = (gs + 1) & (kSampleTable - 1);
+      }
+    }
+    atomicAdd(&g_cnt[gs], c);
+  }
+}
+
+__global__ __launch_bounds__(1024) void gb_sample_gather_kernel(unsigned long long* __restrict__ g_keys,
+                                                                uint32_t* __restrict__ g_cnt,
+                                                                uint64_t* __restrict__ l_keys,
+                                                                uint32_t* __restrict__ l_cnt,
+                                                                uint32_t* __restrict__ l_hist) {
+  __shared__ uint32_t s_hist[kCountBins];
+  const int lane = threadIdx.x & 63;
+  for (int b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
+  __syncthreads();
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;   // grid covers the slots, plus one for the reserved
+  const bool in = s <= (uint32_t)kSampleTable;
+  const uint32_t c = in ? g_cnt[s] : 0u;
+  const uint64_t key = in && s < (uint32_t)kSampleTable ? g_keys[s] : kEmptyKey;
+  if (in) {
+    g_cnt[s] = 0;
+    if (s < (uint32_t)kSampleTable) g_keys[s] = kEmptyKey;
+  }
+  if (c) atomicAdd(&s_hist[min(c, (uint32_t)(kCountBins - 1))], 1u);
+  const uint64_t m = __ballot(c != 0);
+  if (m) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&l_hist[kCountBins], (uint32_t)__popcll(m));   // list length after the bins
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (c) {
+      const uint32_t at = base + rank_below(m);
+      l_keys[at] = key;
+      l_cnt[at] = c;
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kCountBins; b += blockDim.x)
+    if (s_hist[b]) atomicAdd(&l_hist[b], s_hist[b]);
+}
+
+__global__ __launch_bounds__(1024) void gb_sample_select_kernel(GbGeom G, const uint64_t* __restrict__ l_keys,
+                                                                const uint32_t* __restrict__ l_cnt,
+                                                                uint32_t* __restrict__ l_hist,
+                                                                uint16_t* __restrict__ g_idx,
+                                                                uint64_t* __restrict__ head_keys,
+                                                                uint32_t* __restrict__ head_rank,
+                                                                unsigned int* __restrict__ head_count,
+                                                                int* nhead_out) {
   extern __shared__ uint64_t lds[];
-  uint64_t* s_keys = lds;                                                     // kSampleSlots + 1
-  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(lds + kSampleSlots + 1);    // kSampleSlots + 1
-  uint32_t* s_hist = s_cnt + kSampleSlots + 1;                               // kCountBins
-  uint32_t* s_bcount = s_hist + kCountBins;                                  // nbuckets
-  // max_heads selected keys, 8-byte aligned after the bucket counts
-  const size_t sel_off = ((reinterpret_cast<uintptr_t>(s_bcount + G.nbuckets) - reinterpret_cast<uintptr_t>(lds)) + 7) & ~(size_t)7;
-  uint64_t* s_sel = reinterpret_cast<uint64_t*>(reinterpret_cast<uint8_t*>(lds) + sel_off);
-  uint32_t* s_idx32 = reinterpret_cast<uint32_t*>(s_keys);                  // kHeadIndexSlots, after selection
-  uint32_t* s_newid = s_cnt;                                                 // max_heads, after selection
+  uint32_t* s_idx32 = reinterpret_cast<uint32_t*>(lds);                   // kHeadIndexSlots
+  uint64_t* s_sel = lds + kHeadIndexSlots / 2;                             // max_heads selected keys
+  uint32_t* s_newid = reinterpret_cast<uint32_t*>(s_sel + G.max_heads);   // max_heads
+  uint32_t* s_bcount = s_newid + G.max_heads;                              // nbuckets
   __shared__ int s_thresh, s_take, s_n, s_tied, s_kept;
   const int lane = threadIdx.x & 63;
-  for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x) {
-    s_keys[s] = kEmptyKey;
-    s_cnt[s] = 0;
-  }
-  for (int b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
   for (int b = threadIdx.x; b < G.nbuckets; b += blockDim.x) s_bcount[b] = 0;
+  for (int s = threadIdx.x; s < kHeadIndexSlots; s += blockDim.x) s_idx32[s] = kNoHead;
   if (threadIdx.x == 0) {
     s_n = 0;
     s_tied = 0;
@@ -508,75 +614,7 @@ __global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G
     s_thresh = 1;
     s_take = 0;
   }
-  __syncthreads();
-  const int64_t n = G.n;
-  const bool all = n <= (int64_t)kSampleBlocks * kSampleBlockRows;
-  constexpr int U = 8;
-  bool dummy = false;
-  for (int j0 = 0; j0 < kSampleBlocks; j0 += U) {
-    // keys enter the table only from the first half of the sample (about
-    // 4,600 distinct keys on a Zipf(1.1) column over 100,000 values); later
-    // samples count the keys already there (a key not seen early is rare
-    // enough to rank below the head keys), so the table stays under 60% full
-    // and walks stay short
-    const bool insert = j0 < kSampleBlocks / 2;
-    int64_t rc[U];
-    bool live[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t j = j0 + u;
-      const int64_t r = all ? j * kSampleBlockRows + threadIdx.x
-                            : (n - kSampleBlockRows) * j / (kSampleBlocks - 1) + threadIdx.x;
-      live[u] = r < n;
-      rc[u] = live[u] ? r : n - 1;
-    }
-    uint64_t kk[U];
-    group_keys_n<U>(keys, rc, kk, &dummy);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t k = kk[u];
-      int slot = -1;
-      if (live[u]) {
-        if (k == kEmptyKey) {
-          slot = kSampleSlots;
-        } else {
-          uint32_t s = sample_home(gb_hash(k));
-          for (int p = 0; p < kSampleProbes; ++p) {
-            const uint64_t cur = s_keys[s];
-            if (cur == k) { slot = (int)s; break; }
-            if (cur == kEmptyKey) {
-              if (!insert) break;
-              const unsigned long long prev =
-                  atomicCAS(reinterpret_cast<unsigned long long*>(&s_keys[s]), kEmptyKey, k);
-              if (prev == kEmptyKey || prev == k) { slot = (int)s; break; }
-            }
-            s = (s + 1) & (kSampleSlots - 1);
-          }
-        }
-      }
-      // skewed keys repeat within a wave: twice, the lanes sharing the slot of
-      // the first lane still to count add with one atomic
-      bool todo = slot >= 0;
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const uint64_t act = __ballot(todo);
-        if (!act) break;
-        const int leader = __builtin_ctzll(act);
-        const int ls = __shfl(slot, leader);
-        const bool same = todo && slot == ls;
-        const uint64_t grp = __ballot(same);
-        if (same) {
-          if (lane == leader) atomicAdd(&s_cnt[ls], (uint32_t)__popcll(grp));
-          todo = false;
-        }
-      }
-      if (todo) atomicAdd(&s_cnt[slot], 1u);
-    }
-  }
-  __syncthreads();
-  for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x)
-    if (s_cnt[s]) atomicAdd(&s_hist[min(s_cnt[s], (uint32_t)(kCountBins - 1))], 1u);
-  __syncthreads();
+  const uint32_t nlist = l_hist[kCountBins];
   // threshold bin T = the highest bin with (keys in higher bins) + (keys in
   // T) >= max_heads; every key above T is taken and max_heads - (keys above)
   // of bin T.  Wave 0: lane l holds bins 8l .. 8l + 7.
@@ -586,7 +624,7 @@ __global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G
     uint32_t mine = 0;
 #pragma unroll
     for (int j = 0; j < PB; ++j) {
-      hb[j] = s_hist[lane * PB + j];
+      hb[j] = l_hist[lane * PB + j];
       mine += hb[j];
     }
     uint32_t incl = mine;   // keys in bins of lanes >= lane
@@ -620,19 +658,46 @@ __global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G
   __syncthreads();
   const uint32_t T = (uint32_t)s_thresh;
   const int take_at_t = s_take;
-  for (int s = threadIdx.x; s <= kSampleSlots; s += blockDim.x) {
-    const uint32_t c = s_cnt[s];
-    if (!c) continue;
+  // a wave's selections take their places with one LDS atomic per wave (not
+  // per key: thousands of keys on one counter serialise)
+  auto consider = [&](uint32_t c, uint64_t key) {
     const uint32_t bin = min(c, (uint32_t)(kCountBins - 1));
-    bool take = bin > T;
-    if (bin == T) take = atomicAdd(&s_tied, 1) < take_at_t;
-    if (!take) continue;
-    s_sel[atomicAdd(&s_n, 1)] = s == kSampleSlots ? kEmptyKey : s_keys[s];
+    const bool tie = c != 0 && bin == T;
+    bool take = c != 0 && bin > T;
+    const uint64_t mt = __ballot(tie);
+    if (mt) {
+      uint32_t base = 0;
+      if (lane == 0) base = (uint32_t)atomicAdd(&s_tied, (int)__popcll(mt));
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (tie) take = (int)(base + rank_below(mt)) < take_at_t;
+    }
+    const uint64_t mk = __ballot(take);
+    if (mk) {
+      uint32_t base = 0;
+      if (lane == 0) base = (uint32_t)atomicAdd(&s_n, (int)__popcll(mk));
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (take) s_sel[base + rank_below(mk)] = key;
+    }
+  };
+  // the list, kListPer entries a thread a round, their loads issued together
+  constexpr int kListPer = 16;
+  for (uint32_t r0 = 0; r0 < nlist; r0 += kListPer * blockDim.x) {
+    uint32_t c[kListPer];
+    uint64_t k[kListPer];
+#pragma unroll
+    for (int q = 0; q < kListPer; ++q) {
+      const uint32_t i = r0 + q * blockDim.x + threadIdx.x;
+      const uint32_t ic = i < nlist ? i : nlist - 1;
+      c[q] = l_cnt[ic];
+      k[q] = l_keys[ic];
+      if (i >= nlist) c[q] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kListPer; ++q) consider(c[q], k[q]);
   }
   __syncthreads();
-  // index under construction (32-bit entries over the count table's keys)
-  for (int s = threadIdx.x; s < kHeadIndexSlots; s += blockDim.x) s_idx32[s] = kNoHead;
-  __syncthreads();
+  // the list and histogram are empty again for the next batch
+  for (int b = threadIdx.x; b <= kCountBins; b += blockDim.x) l_hist[b] = 0;
   const int nsel = s_n;
   for (int j = threadIdx.x; j < nsel; j += blockDim.x) {
     const uint64_t h = gb_hash(s_sel[j]);
@@ -664,9 +729,13 @@ __global__ __launch_bounds__(1024) void gb_sample_kernel(GKeySpec keys, GbGeom G
 __device__ __forceinline__ int64_t gb_units(int64_t n) { return (n + kDealRows - 1) / kDealRows; }
 
 // Count pass: tail rows per bucket of this workgroup's rows, written to
-// hist[bucket][workgroup].  Each thread loads the keys of U units at once
+// hist[bucket][workgroup].  It runs kCountSplit workgroups per main-pass
+// workgroup (its LDS allows several per CU, and more waves hide the latency of
+// its key loads and index lookups); rows are dealt the same way, so count
+// workgroup c sees a share of the rows of main-pass workgroup c % blocks.  Each thread loads the keys of U units at once
 // (4 rows per unit).
 constexpr int kCountUnits = 2;
+constexpr int kCountSplit = 2;
 template <int KW>
 __global__ __launch_bounds__(kGroupThreads) void gb_count_kernel(GKeySpec keys, GbGeom G,
                                                                  const uint16_t* __restrict__ g_idx,
@@ -725,14 +794,17 @@ __global__ __launch_bounds__(kGroupThreads) void gb_count_kernel(GKeySpec keys, 
 }
 
 // Offsets, first kernel (a wave per bucket): base[b][w] = tail rows of bucket
-// b in workgroups before w; tail_total[b] = all of them.
+// b in main-pass workgroups before w; tail_total[b] = all of them.  The count
+// pass ran kCountSplit workgroups per main-pass workgroup: hist[b][c] with
+// c < kCountSplit * blocks, and count workgroup c counted rows of main-pass
+// workgroup c % blocks.
 __global__ __launch_bounds__(1024) void gb_bucket_scan_kernel(const uint32_t* __restrict__ hist, int blocks,
                                                               int nbuckets, uint64_t* __restrict__ base,
                                                               uint64_t* __restrict__ tail_total) {
   const int lane = threadIdx.x & 63;
   const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (b >= nbuckets) return;
-  const uint32_t* row = hist + (size_t)b * blocks;
+  const uint32_t* row = hist + (size_t)b * blocks * kCountSplit;
   constexpr int kMaxPer = 16;               // up to 1024 workgroups
   const int per = (blocks + 63) / 64;       // consecutive counts per lane
   uint32_t v[kMaxPer];
@@ -740,7 +812,10 @@ __global__ __launch_bounds__(1024) void gb_bucket_scan_kernel(const uint32_t* __
 #pragma unroll
   for (int j = 0; j < kMaxPer; ++j) {
     const int w = lane * per + j;
-    v[j] = (j < per && w < blocks) ? row[w] : 0u;
+    uint32_t x = 0;
+#pragma unroll
+    for (int q = 0; q < kCountSplit; ++q) x += (j < per && w < blocks) ? row[w + q * blocks] : 0u;
+    v[j] = x;
     mine += v[j];
   }
   uint64_t incl = mine;
@@ -1627,6 +1702,11 @@ struct fpdb_group_table {
   int64_t pool_cap = 0;              // records
   uint64_t* spare = nullptr;         // the next pool
   int64_t spare_cap = 0;
+  unsigned long long* sample_keys = nullptr;   // the sampler's count table (kSampleTable + 1), empty between batches
+  uint32_t* sample_cnt = nullptr;
+  uint64_t* list_keys = nullptr;     // the sampled keys with their counts, and the counts' histogram
+  uint32_t* list_cnt = nullptr;      // (kCountBins bins, then the list length); empty between batches
+  uint32_t* list_hist = nullptr;
   uint64_t* tpool = nullptr;         // raw tail rows of the current batch (staged main pass)
   int64_t tpool_words = 0;
   int64_t* ostart = nullptr;         // nbuckets
@@ -1634,7 +1714,7 @@ struct fpdb_group_table {
   int64_t* nstart = nullptr;         // nbuckets
   int64_t* newcount = nullptr;       // nbuckets
   uint64_t* base = nullptr;          // nbuckets x max_blocks tail positions, bucket-major
-  uint32_t* hist = nullptr;          // nbuckets x max_blocks tail counts, bucket-major
+  uint32_t* hist = nullptr;          // nbuckets x (kCountSplit x max_blocks) tail counts, bucket-major
   uint16_t* head_index = nullptr;    // kHeadIndexSlots head ids (two-choice index of the head keys)
   uint64_t* head_keys = nullptr;     // max_heads
   uint32_t* head_rank = nullptr;     // max_heads: rank among its bucket's head keys
@@ -1775,7 +1855,7 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   t->max_heads = (int)heads;
   t->main_lds = fixed + per_head * (size_t)heads;
   t->count_lds = (size_t)kHeadIndexSlots * 2 + (size_t)heads * 8 + (size_t)nb * 4;
-  t->sample_lds = (size_t)(kSampleSlots + 1) * 12 + (size_t)kCountBins * 4 + (size_t)nb * 4 + 8 + (size_t)heads * 8;
+  t->sample_lds = (size_t)kHeadIndexSlots * 4 + (size_t)heads * 12 + (size_t)nb * 4;
   t->max_blocks = std::min(1024, std::max(1, c->num_cus));   // the bucket scan takes up to 1024 workgroups
   if (heads < 64 || cslots < 64 || t->sample_lds > budget) {
     delete t;
@@ -1786,7 +1866,7 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->nstart));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->newcount));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 8, reinterpret_cast<void**>(&t->base));
-  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * 4, reinterpret_cast<void**>(&t->hist));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * t->max_blocks * kCountSplit * 4, reinterpret_cast<void**>(&t->hist));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)kHeadIndexSlots * 2, reinterpret_cast<void**>(&t->head_index));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)heads * 8, reinterpret_cast<void**>(&t->head_keys));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)heads * 4, reinterpret_cast<void**>(&t->head_rank));
@@ -1798,6 +1878,14 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_start));
   if (s == FPDB_OK) s = gb_alloc(t, 32, reinterpret_cast<void**>(&t->totals));
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->flags));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)(kSampleTable + 1) * 8, reinterpret_cast<void**>(&t->sample_keys));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)(kSampleTable + 1) * 4, reinterpret_cast<void**>(&t->sample_cnt));
+  if (s == FPDB_OK) s = launch_fill(c, t->sample_keys, 0xFF, (uint64_t)(kSampleTable + 1) * 8);
+  if (s == FPDB_OK) s = launch_fill(c, t->sample_cnt, 0, (uint64_t)(kSampleTable + 1) * 4);
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)kSampleListMax * 8, reinterpret_cast<void**>(&t->list_keys));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)kSampleListMax * 4, reinterpret_cast<void**>(&t->list_cnt));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)(kCountBins + 1) * 4, reinterpret_cast<void**>(&t->list_hist));
+  if (s == FPDB_OK) s = launch_fill(c, t->list_hist, 0, (uint64_t)(kCountBins + 1) * 4);
   if (s == FPDB_OK) s = launch_fill(c, t->ostart, 0, (uint64_t)nb * 8);
   if (s == FPDB_OK) s = launch_fill(c, t->ocount, 0, (uint64_t)nb * 8);
   if (s == FPDB_OK) s = launch_fill(c, t->totals, 0, 32);
@@ -1928,13 +2016,17 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
   {
     {
       ProfScope ps(c, "groupby_sample");
-      hipLaunchKernelGGL(gb_sample_kernel, dim3(1), dim3(1024), t->sample_lds, c->stream, k, G, t->head_index,
-                         t->head_keys,
-                         t->head_rank, t->head_count, t->nhead);
+      hipLaunchKernelGGL(gb_sample_count_kernel, dim3(kSampleBlocks), dim3(kSampleBlockRows), 0, c->stream, k, G,
+                         t->sample_keys, t->sample_cnt);
+      hipLaunchKernelGGL(gb_sample_gather_kernel, dim3(kSampleTable / 1024 + 1), dim3(1024), 0, c->stream,
+                         t->sample_keys, t->sample_cnt, t->list_keys, t->list_cnt, t->list_hist);
+      hipLaunchKernelGGL(gb_sample_select_kernel, dim3(1), dim3(1024), t->sample_lds, c->stream, G, t->list_keys,
+                         t->list_cnt, t->list_hist, t->head_index, t->head_keys, t->head_rank, t->head_count,
+                         t->nhead);
     }
     ProfScope ps(c, "groupby_count");
     auto count_kernel = kw == 8 ? gb_count_kernel<8> : (kw == 4 ? gb_count_kernel<4> : gb_count_kernel<0>);
-    hipLaunchKernelGGL(count_kernel, dim3(blocks), dim3(kGroupThreads), t->count_lds, c->stream, k, G,
+    hipLaunchKernelGGL(count_kernel, dim3(blocks * kCountSplit), dim3(kGroupThreads), t->count_lds, c->stream, k, G,
                        t->head_index, t->head_keys, t->nhead, t->hist, t->flags + 1);
     hipLaunchKernelGGL(gb_bucket_scan_kernel, dim3((unsigned)ceil_div(t->nbuckets, 16)), dim3(1024), 0, c->stream,
                        t->hist, blocks, t->nbuckets, t->base, t->tail_total);
@@ -2055,7 +2147,8 @@ fpdb_status fpdb_column_min_max(fpdb_context* ctx, const fpdb_column* column, in
 
 fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
-  void* bufs[] = {t->pool, t->spare, t->tpool, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
+  void* bufs[] = {t->pool, t->spare, t->tpool, t->sample_keys, t->sample_cnt, t->list_keys, t->list_cnt,
+                  t->list_hist, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
                   t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->tail_total, t->tail_start,
                   t->totals, t->flags};
   for (void* b : bufs)
