@@ -276,6 +276,29 @@ __device__ __forceinline__ void load_zext_n(const uint8_t* base, int width, cons
   }
 }
 
+// load_zext_n with non-temporal loads for 4- and 8-byte columns: for columns
+// that stream through a kernel once, so they do not push out of the L2 what
+// the kernel gathers at random.
+template <int N>
+__device__ __forceinline__ void load_zext_n_nt(const uint8_t* base, int width, const int64_t (&idx)[N],
+                                               uint64_t (&out)[N]) {
+  switch (width) {
+    case 8: {
+      const uint64_t* p = reinterpret_cast<const uint64_t*>(base);
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] = __builtin_nontemporal_load(p + idx[i]);
+      break;
+    }
+    case 4: {
+      const uint32_t* p = reinterpret_cast<const uint32_t*>(base);
+#pragma unroll
+      for (int i = 0; i < N; ++i) out[i] = __builtin_nontemporal_load(p + idx[i]);
+      break;
+    }
+    default: load_zext_n<N>(base, width, idx, out); break;
+  }
+}
+
 __device__ __forceinline__ bool valid_bit(const uint8_t* validity, int64_t i) {
   return validity == nullptr || ((validity[i >> 3] >> (i & 7)) & 1);
 }

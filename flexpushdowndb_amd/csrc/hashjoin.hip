@@ -181,7 +181,7 @@ __device__ __forceinline__ void emit_pair(const ColSet& pcols, const ColSet& bco
 
 // Keys of N rows row0 + i * stride (rows past the end read row n - 1 and are
 // not live; a null key is not live).
-template <int KM, int N, int STRIDE>
+template <int KM, int N, int STRIDE, bool NT = false>
 __device__ __forceinline__ void probe_keys_n(const KeySpec& pk, int64_t n, int64_t row0, int64_t (&rc)[N],
                                              int64_t (&key)[N], bool (&live)[N]) {
 #pragma unroll
@@ -194,7 +194,8 @@ __device__ __forceinline__ void probe_keys_n(const KeySpec& pk, int64_t n, int64
     uint64_t kv[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) ki[i] = rc[i] + pk.off[0];
-    load_zext_n<N>(pk.v[0], KM == 1 ? 8 : 4, ki, kv);
+    if (NT) load_zext_n_nt<N>(pk.v[0], KM == 1 ? 8 : 4, ki, kv);
+    else load_zext_n<N>(pk.v[0], KM == 1 ? 8 : 4, ki, kv);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       key[i] = KM == 1 ? (int64_t)kv[i] : (int64_t)(int32_t)(uint32_t)kv[i];
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(kDenseCountThreads) void hj_dense_count_kernel(
   const int64_t nwaves = (int64_t)gridDim.x * (kDenseCountThreads / 64);
   int64_t key[U], rc[U];
   bool live[U];
-  if (wave < tiles) probe_keys_n<KM, U, 64>(pk, n, wave * kDenseTileRows + lane, rc, key, live);
+  if (wave < tiles) probe_keys_n<KM, U, 64, true>(pk, n, wave * kDenseTileRows + lane, rc, key, live);
   const uint64_t nbw = (span + 63) >> 6;
   if (BITS_IN_LDS) {
     stage_words(s_bits, bits, nbw);
@@ -640,7 +641,7 @@ __global__ __launch_bounds__(kDenseCountThreads) void hj_dense_count_kernel(
     int64_t nkey[U], nrc[U];
     bool nlive[U];
     const int64_t tn = t + nwaves;
-    if (tn < tiles) probe_keys_n<KM, U, 64>(pk, n, tn * kDenseTileRows + lane, nrc, nkey, nlive);
+    if (tn < tiles) probe_keys_n<KM, U, 64, true>(pk, n, tn * kDenseTileRows + lane, nrc, nkey, nlive);
     // every lane reads its word (slot 0's for keys outside the span), so the
     // U reads are issued together
     uint64_t hw[U], pw[U];
@@ -694,20 +695,28 @@ __global__ __launch_bounds__(256) void hj_dense_emit_kernel(
   const int64_t row0 = tile * kDenseTileRows + threadIdx.x;
   int64_t key[kDenseItems], rc[kDenseItems];
   bool live[kDenseItems];
-  probe_keys_n<KM, kDenseItems, kProbeThreads>(pk, n, row0, rc, key, live);
+  probe_keys_n<KM, kDenseItems, kProbeThreads, true>(pk, n, row0, rc, key, live);
   uint64_t hw[kDenseItems];
 #pragma unroll
   for (int i = 0; i < kDenseItems; ++i) hw[i] = hitmap[rc[i] >> 6];
   // probe columns that are written whatever the scan says
   const bool emit_cols = !(JT == kJtMark || out.n == 0);
+  // (a probe column that is the key column itself, non-null, takes the loaded
+  // keys instead of a second load; the others load non-temporally)
   uint64_t pre[kDensePrefetch][kDenseItems];
 #pragma unroll
   for (int c = 0; c < kDensePrefetch; ++c) {
     if (emit_cols && c < pcols.n) {
-      int64_t si[kDenseItems];
+      if ((KM == 1 || KM == 2) && pcols.values[c] == pk.v[0] && pcols.offset[c] == pk.off[0] &&
+          pcols.width[c] == (KM == 1 ? 8 : 4) && pcols.validity[c] == nullptr) {
 #pragma unroll
-      for (int i = 0; i < kDenseItems; ++i) si[i] = rc[i] + pcols.offset[c];
-      load_zext_n<kDenseItems>(pcols.values[c], pcols.width[c], si, pre[c]);
+        for (int i = 0; i < kDenseItems; ++i) pre[c][i] = KM == 1 ? (uint64_t)key[i] : (uint64_t)(uint32_t)key[i];
+      } else {
+        int64_t si[kDenseItems];
+#pragma unroll
+        for (int i = 0; i < kDenseItems; ++i) si[i] = rc[i] + pcols.offset[c];
+        load_zext_n_nt<kDenseItems>(pcols.values[c], pcols.width[c], si, pre[c]);
+      }
     }
   }
   uint32_t brow[kDenseItems], bpay[kDenseItems];
