@@ -44,9 +44,10 @@ def kernel_ms(ctx, name):
 
 
 def load_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), committed under
-    profiles/ as a small JSON summary."""
+    """HBM bytes per launch of a kernel group, measured with rocprofv3 --pmc
+    (the L2's memory-side read and write requests by size) by
+    `python -m flexpushdowndb_amd.profiling` and committed under profiles/ as a
+    small JSON summary."""
     p = os.path.join(ROOT, "profiles", f"{name}.traffic.json")
     if not os.path.exists(p):
         return None
@@ -368,7 +369,7 @@ def bench_groupby(ctx, steps):
             "compact_ms": compact_ms, "consume_wall_ms": statistics.median(wall),
             "grows_per_s": n / (cons_ms * 1e-3) / 1e9, "finalize_ms": finalize_ms,
             "hbm_frac": alg / (cons_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic_bytes": load_traffic("groupby_main")}
+            "traffic_bytes": load_traffic("groupby_consume")}
 
 
 def bench_star(ctx, steps, device_name):
@@ -915,6 +916,7 @@ def main():
             "value_grows_per_s": round(h["value"], 4), "ms_per_step": round(h["ms_per_step"], 5),
             "probe_kernel_ms": round(h["probe_ms"], 5), "build_kernel_ms": round(h["build_ms"], 5),
             "probe_hbm_frac": round(h["alg_bytes"] / (h["probe_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "probe_traffic": load_traffic("hj_probe_hash"),
             "table_form": h["table_form"]}
     if "join_small_hash" in res:
         h = res["join_small_hash"]
