@@ -67,7 +67,8 @@ class StarQuery(ctypes.Structure):
                 ("filter_order", ctypes.c_int32 * 8), ("dims", StarDimension * 8),
                 ("fact_measure", Column), ("fact_rows", ctypes.c_int64), ("num_partitions", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("partition_offsets", ctypes.POINTER(ctypes.c_int64)),
-                ("max_groups", ctypes.c_int64)]
+                ("max_groups", ctypes.c_int64), ("partition_fact_fk", ctypes.POINTER(Column)),
+                ("partition_measure", ctypes.POINTER(Column))]
 
 
 class StarTuning(ctypes.Structure):

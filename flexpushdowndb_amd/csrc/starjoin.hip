@@ -53,7 +53,15 @@ struct StarDimDev {
   uint64_t* bbf;             // backward filter blocks (capacity for all fact rows)
   StarSlot* ht;              // hash table storage (ht_mask + 1 slots); a run uses the
   uint64_t ht_mask;          // first star_ht_mask(...) + 1 of them
-  const int64_t* fk;         // fact foreign key column
+};
+
+// A run of fact rows in one set of buffers: the whole fact table, or one
+// partition held in buffers of its own.
+struct StarSeg {
+  int64_t base;              // global number of its first row
+  int64_t rows;
+  const int64_t* fk[kStarMaxDims];
+  const int64_t* meas;
 };
 
 struct StarDev {
@@ -61,7 +69,8 @@ struct StarDev {
   int32_t ndims;
   int32_t order[kStarMaxDims];     // forward filter order over the fact columns
   int32_t gdim[2];                 // dimensions whose group attribute forms the key
-  const int64_t* measure;
+  const StarSeg* segs;             // device: the fact rows' segments, in row order
+  int32_t nsegs;
   int64_t fact_rows;
   // survivors of the forward pass
   int64_t* s_idx;                  // original fact row
@@ -242,7 +251,7 @@ __device__ __forceinline__ void group_add(const StarDev& P, uint64_t gkey, int64
 // ---- phases shared by the persistent kernel and the plain path ---------------
 
 // Forward-pass survivors are appended with one atomic per wavefront.
-__device__ __forceinline__ void append_survivor(const StarDev& P, bool pass, int64_t r) {
+__device__ __forceinline__ void append_survivor(const StarDev& P, const StarSeg& S, bool pass, int64_t lr) {
   uint64_t mask = __ballot(pass);
   if (!mask) return;
   const int lane = threadIdx.x & 63;
@@ -255,9 +264,9 @@ __device__ __forceinline__ void append_survivor(const StarDev& P, bool pass, int
   if (pass) {
     uint64_t below = lane ? (mask & (~0ULL >> (64 - lane))) : 0ULL;
     int64_t pos = (int64_t)(base + __popcll(below));
-    P.s_idx[pos] = r;
-    for (int d = 0; d < P.ndims; ++d) P.s_fk[(int64_t)d * P.fact_rows + pos] = P.dim[d].fk[r];
-    P.s_meas[pos] = P.measure[r];
+    P.s_idx[pos] = S.base + lr;
+    for (int d = 0; d < P.ndims; ++d) P.s_fk[(int64_t)d * P.fact_rows + pos] = S.fk[d][lr];
+    P.s_meas[pos] = S.meas[lr];
   }
 }
 
@@ -336,15 +345,20 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     __syncthreads();
     const int d0 = P.order[0];
     const int64_t step = gthreads * kStarItems;
-    const int64_t rounded = (P.fact_rows + step - 1) / step * step;   // whole waves iterate together
-    const int64_t last = P.fact_rows - 1;
+    // segment by segment (one for a contiguous fact table, else one per
+    // partition), rows numbered within the segment
+    for (int sg = 0; sg < P.nsegs; ++sg) {
+    const StarSeg S = P.segs[sg];
+    if (S.rows == 0) continue;
+    const int64_t rounded = (S.rows + step - 1) / step * step;   // whole waves iterate together
+    const int64_t last = S.rows - 1;
     for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * kStarItems + threadIdx.x; r0 < rounded; r0 += step) {
       int64_t rc[kStarItems], fk[kStarItems];
 #pragma unroll
       for (int j = 0; j < kStarItems; ++j) {
         int64_t r = r0 + (int64_t)j * blockDim.x;
         rc[j] = r <= last ? r : last;
-        fk[j] = P.dim[d0].fk[rc[j]];
+        fk[j] = S.fk[d0][rc[j]];
       }
       bool pass[kStarItems];
 #pragma unroll
@@ -363,7 +377,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
         for (int j = 0; j < kStarItems; ++j) any |= pass[j];
         if (!__ballot(any)) break;
         const int d = P.order[i];
-        const int64_t* col = P.dim[d].fk;
+        const int64_t* col = S.fk[d];
 #pragma unroll
         for (int j = 0; j < kStarItems; ++j) fk[j] = col[pass[j] ? rc[j] : 0];
 #pragma unroll
@@ -374,7 +388,8 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
         }
       }
 #pragma unroll
-      for (int j = 0; j < kStarItems; ++j) append_survivor(P, pass[j], r0 + (int64_t)j * blockDim.x);
+      for (int j = 0; j < kStarItems; ++j) append_survivor(P, S, pass[j], r0 + (int64_t)j * blockDim.x);
+    }
     }
   }
   if (!star_barrier(P)) return;
@@ -511,6 +526,8 @@ struct fpdb_star {
   fpdb::Context* ctx = nullptr;
   fpdb_star_query q;
   std::vector<int64_t> parts;          // partition offsets (num_partitions + 1)
+  std::vector<fpdb::StarSeg> segs;     // the fact rows' segments (host copy of dev.segs)
+  bool split = false;                  // partitions in buffers of their own (one segment each)
   fpdb_star_tuning tune;
   fpdb::StarDev dev;
   std::vector<void*> owned;
@@ -534,6 +551,17 @@ struct fpdb_star {
 };
 
 namespace fpdb {
+
+// Partition p's fact foreign keys for dimension d, and its measures (in the
+// partition's own buffers, or its rows of the contiguous table).
+static const int64_t* fact_fk_of(const fpdb_star* s, int p, int d) {
+  if (s->split) return s->segs[p].fk[d];
+  return s->segs[0].fk[d] + s->parts[p];
+}
+static const int64_t* fact_measure_of(const fpdb_star* s, int p) {
+  if (s->split) return s->segs[p].meas;
+  return s->segs[0].meas + s->parts[p];
+}
 
 static fpdb_status star_alloc(fpdb_star* s, uint64_t bytes, void** p) {
   FPDB_TRY(ctx_alloc(s->ctx, bytes, p));
@@ -621,8 +649,8 @@ static fpdb_status star_plain(fpdb_star* s) {
   for (int p = 0; p < np; ++p) {
     int64_t lo = s->parts[p], n = s->parts[p + 1] - lo;
     std::vector<fpdb_column> cur;
-    for (int d = 0; d < nd; ++d) cur.push_back(col_i64(static_cast<const int64_t*>(q.dims[d].fact_fk.values) + lo, n));
-    cur.push_back(col_i64(static_cast<const int64_t*>(q.fact_measure.values) + lo, n));
+    for (int d = 0; d < nd; ++d) cur.push_back(col_i64(fact_fk_of(s, p, d), n));
+    cur.push_back(col_i64(fact_measure_of(s, p), n));
     for (int i = 0; i < nd && n > 0; ++i) {
       int d = q.filter_order[i];
       std::vector<int64_t*> outp(nd + 1);
@@ -819,8 +847,25 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   for (int p = 0; p < q->num_partitions; ++p)
     FPDB_CHECK(q->partition_offsets[p] <= q->partition_offsets[p + 1], FPDB_ERR_INVALID,
                "star join: partition offsets must not decrease");
-  FPDB_CHECK(q->fact_measure.type == FPDB_TYPE_INT64 && q->fact_measure.length >= q->fact_rows && q->fact_measure.values,
-             FPDB_ERR_INVALID, "star join: int64 measure column needed");
+  const bool split = q->partition_fact_fk != nullptr;
+  if (split) {
+    FPDB_CHECK(q->partition_measure, FPDB_ERR_INVALID, "star join: partition_fact_fk without partition_measure");
+    for (int p = 0; p < q->num_partitions; ++p) {
+      const int64_t rows = q->partition_offsets[p + 1] - q->partition_offsets[p];
+      const fpdb_column& m = q->partition_measure[p];
+      FPDB_CHECK(m.type == FPDB_TYPE_INT64 && m.length >= rows && (m.values || rows == 0) && !m.validity,
+                 FPDB_ERR_INVALID, "star join: non-null int64 partition measure columns needed");
+      for (int d = 0; d < q->num_dims; ++d) {
+        const fpdb_column& f = q->partition_fact_fk[(size_t)p * q->num_dims + d];
+        FPDB_CHECK(f.type == FPDB_TYPE_INT64 && f.length >= rows && (f.values || rows == 0) && !f.validity,
+                   FPDB_ERR_INVALID, "star join: non-null int64 partition foreign-key columns needed");
+      }
+    }
+  } else {
+    FPDB_CHECK(q->fact_measure.type == FPDB_TYPE_INT64 && q->fact_measure.length >= q->fact_rows &&
+                   q->fact_measure.values,
+               FPDB_ERR_INVALID, "star join: int64 measure column needed");
+  }
   FPDB_CHECK(q->group_dims[0] >= 0 && q->group_dims[0] < q->num_dims && q->group_dims[1] >= 0 &&
                  q->group_dims[1] < q->num_dims && q->group_dims[0] != q->group_dims[1],
              FPDB_ERR_INVALID, "star join: two distinct group dimensions");
@@ -836,8 +881,8 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
                    D.filter_attr.offset == 0,
                FPDB_ERR_INVALID, "star join: int32 filter attribute needed");
     FPDB_CHECK(D.filter_op >= FPDB_CMP_LT && D.filter_op <= FPDB_CMP_NE, FPDB_ERR_INVALID, "star join: bad filter op");
-    FPDB_CHECK(D.fact_fk.type == FPDB_TYPE_INT64 && D.fact_fk.length >= q->fact_rows && D.fact_fk.values &&
-                   D.fact_fk.offset == 0 && !D.fact_fk.validity,
+    FPDB_CHECK(split || (D.fact_fk.type == FPDB_TYPE_INT64 && D.fact_fk.length >= q->fact_rows &&
+                         D.fact_fk.values && D.fact_fk.offset == 0 && !D.fact_fk.validity),
                FPDB_ERR_INVALID, "star join: non-null int64 fact foreign keys needed");
     FPDB_CHECK(D.key.length < (int64_t)kStarEmptyRow, FPDB_ERR_INVALID, "star join: dimension too large");
   }
@@ -854,6 +899,31 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   s->q = *q;
   s->parts.assign(q->partition_offsets, q->partition_offsets + q->num_partitions + 1);
   s->q.partition_offsets = nullptr;
+  // the fact rows' segments: the contiguous table, or one per partition
+  s->split = split;
+  if (split) {
+    for (int p = 0; p < q->num_partitions; ++p) {
+      StarSeg g{};
+      g.base = s->parts[p];
+      g.rows = s->parts[p + 1] - s->parts[p];
+      const fpdb_column& m = q->partition_measure[p];
+      g.meas = static_cast<const int64_t*>(m.values) + m.offset;
+      for (int d = 0; d < q->num_dims; ++d) {
+        const fpdb_column& f = q->partition_fact_fk[(size_t)p * q->num_dims + d];
+        g.fk[d] = static_cast<const int64_t*>(f.values) + f.offset;
+      }
+      s->segs.push_back(g);
+    }
+  } else {
+    StarSeg g{};
+    g.base = 0;
+    g.rows = q->fact_rows;
+    g.meas = static_cast<const int64_t*>(q->fact_measure.values);
+    for (int d = 0; d < q->num_dims; ++d) g.fk[d] = static_cast<const int64_t*>(q->dims[d].fact_fk.values);
+    s->segs.push_back(g);
+  }
+  s->q.partition_fact_fk = nullptr;   // caller memory: only the segments are kept
+  s->q.partition_measure = nullptr;
   s->tune = tuning ? *tuning : fpdb_star_tuning{0, 0, 1, 0};
   if (s->tune.ht_load_shift < 1 || s->tune.ht_load_shift > 3) s->tune.ht_load_shift = 1;
   FPDB_CHECK(s->tune.barrier_timeout_us >= 0, FPDB_ERR_INVALID, "star join: negative barrier timeout");
@@ -864,8 +934,8 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   for (int i = 0; i < q->num_dims; ++i) P.order[i] = q->filter_order[i];
   P.gdim[0] = q->group_dims[0];
   P.gdim[1] = q->group_dims[1];
-  P.measure = static_cast<const int64_t*>(q->fact_measure.values);
   P.fact_rows = q->fact_rows;
+  P.nsegs = (int32_t)s->segs.size();
   fpdb_status st = FPDB_OK;
   int blnb = host_log_blocks(q->fact_rows);
   for (int d = 0; d < q->num_dims && st == FPDB_OK; ++d) {
@@ -877,7 +947,6 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
     X.rows = D.key.length;
     X.cst = D.filter_constant;
     X.op = D.filter_op;
-    X.fk = static_cast<const int64_t*>(D.fact_fk.values);
     int64_t cap = 64;
     while (cap < (X.rows << s->tune.ht_load_shift)) cap <<= 1;
     X.ht_mask = (uint64_t)cap - 1;
@@ -892,6 +961,9 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8 * q->num_dims, reinterpret_cast<void**>(&P.s_fk));
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8, reinterpret_cast<void**>(&P.s_meas));
   if (st == FPDB_OK) st = star_alloc(s, 64 * 8, reinterpret_cast<void**>(&P.state));
+  if (st == FPDB_OK) st = star_alloc(s, s->segs.size() * sizeof(StarSeg), reinterpret_cast<void**>(const_cast<StarSeg**>(&P.segs)));
+  if (st == FPDB_OK) st = hipMemcpy(const_cast<StarSeg*>(P.segs), s->segs.data(), s->segs.size() * sizeof(StarSeg),
+                                    hipMemcpyHostToDevice) == hipSuccess ? FPDB_OK : fail(FPDB_ERR_HIP, "star join: segment copy");
   int64_t gcap = 64;
   while (gcap < 2 * q->max_groups) gcap <<= 1;
   P.g_mask = (uint64_t)gcap - 1;

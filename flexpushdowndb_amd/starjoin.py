@@ -24,7 +24,10 @@ class StarJoin:
     """One prepared star-join plan over device-resident columns."""
 
     def __init__(self, ctx, workload, group_dims=(0, 3), filter_order=None, tuning=None, max_groups=4096,
-                 barrier_timeout_us=0):
+                 barrier_timeout_us=0, split_partitions=False):
+        """split_partitions: hand each partition's fact columns over in
+        device buffers of their own (one column per partition and dimension)
+        instead of one contiguous fact table."""
         self.ctx = ctx
         self.barrier_timeout_us = int(barrier_timeout_us)
         self.w = workload
@@ -59,6 +62,22 @@ class StarJoin:
         self._offs = offs
         q.num_partitions = len(offs) - 1
         q.partition_offsets = offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        if split_partitions:
+            nd, npart = len(dims), len(offs) - 1
+            fk_arr = (C.Column * (npart * nd))()
+            m_arr = (C.Column * npart)()
+            for p in range(npart):
+                lo, hi = int(offs[p]), int(offs[p + 1])
+                for d in range(nd):
+                    pc = DeviceColumn.from_numpy(ctx, np.ascontiguousarray(workload["fact_fk"][d][lo:hi]))
+                    self.cols.append(pc)
+                    fk_arr[p * nd + d] = pc.c()
+                pm = DeviceColumn.from_numpy(ctx, np.ascontiguousarray(workload["measure"][lo:hi]))
+                self.cols.append(pm)
+                m_arr[p] = pm.c()
+            self._split = (fk_arr, m_arr)
+            q.partition_fact_fk = ctypes.cast(fk_arr, ctypes.POINTER(C.Column))
+            q.partition_measure = ctypes.cast(m_arr, ctypes.POINTER(C.Column))
         q.max_groups = max_groups
         self.q = q
         self.max_groups = max_groups

@@ -635,6 +635,16 @@ typedef struct fpdb_star_query {
   int32_t reserved;
   const int64_t* partition_offsets;   /* host, num_partitions + 1 row offsets */
   int64_t max_groups;
+  /* Optional: partitions held in buffers of their own (the reference's one
+   * TupleSet per partition).  When partition_fact_fk is not NULL it holds
+   * num_partitions x num_dims foreign-key columns (partition p's column for
+   * dimension d at p * num_dims + d) and partition_measure num_partitions
+   * measure columns, each with at least its partition's rows
+   * (partition_offsets[p + 1] - partition_offsets[p]); dims[d].fact_fk and
+   * fact_measure are then not read.  Rows keep their global numbers (offset
+   * + row in partition) for the first-appearance order of the groups. */
+  const fpdb_column* partition_fact_fk;
+  const fpdb_column* partition_measure;
 } fpdb_star_query;
 
 /* Launch shape of the batched form (0 = default): workgroups per CU of the
