@@ -73,7 +73,7 @@ class StarQuery(ctypes.Structure):
 
 class StarTuning(ctypes.Structure):
     _fields_ = [("blocks_per_cu", ctypes.c_int32), ("barrier_timeout_us", ctypes.c_int32),
-                ("ht_load_shift", ctypes.c_int32), ("reserved1", ctypes.c_int32)]
+                ("ht_load_shift", ctypes.c_int32), ("items_per_thread", ctypes.c_int32)]
 
 
 class ServeDimension(ctypes.Structure):

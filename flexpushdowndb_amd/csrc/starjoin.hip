@@ -120,7 +120,7 @@ __device__ __forceinline__ void bf_insert(uint64_t* blocks, int lnb, const uint6
 // read from HBM / L2), and fact rows per thread per step.
 constexpr int kStarFilterLdsWords = 8192;   // 64 KiB
 constexpr size_t kStarDynamicLds = kStarFilterLdsWords * 8;
-constexpr int kStarItems = 8;
+constexpr int kStarItems = 8;      // default fact rows per thread per step (tuning: 4, 8 or 16)
 constexpr int kStarThreads = 1024;   // threads per workgroup of the persistent kernel
 constexpr uint64_t kStarBarrierTimeoutUs = 200000;   // grid barrier wait bound (a normal wait is < 1 ms)
 
@@ -270,6 +270,7 @@ __device__ __forceinline__ void append_survivor(const StarDev& P, const StarSeg&
   }
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P) {
   __shared__ uint64_t s_masks[kMaskNum];
   for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
   if (!star_barrier(P)) return;
   // B: forward pass over every fact row (all partitions), filters in `order`.
   // The filters that fit (in that order) are copied into LDS; each thread
-  // takes kStarItems rows per step with the first filter's keys loaded
+  // takes ITEMS rows per step with the first filter's keys loaded
   // together, and only rows still passing read the next filter's keys.
   {
     // per-dimension filter size and LDS offset (-1: read from global memory),
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     }
     __syncthreads();
     const int d0 = P.order[0];
-    const int64_t step = gthreads * kStarItems;
+    const int64_t step = gthreads * ITEMS;
     // segment by segment (one for a contiguous fact table, else one per
     // partition), rows numbered within the segment
     for (int sg = 0; sg < P.nsegs; ++sg) {
@@ -352,17 +353,17 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     if (S.rows == 0) continue;
     const int64_t rounded = (S.rows + step - 1) / step * step;   // whole waves iterate together
     const int64_t last = S.rows - 1;
-    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * kStarItems + threadIdx.x; r0 < rounded; r0 += step) {
-      int64_t rc[kStarItems], fk[kStarItems];
+    for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * ITEMS + threadIdx.x; r0 < rounded; r0 += step) {
+      int64_t rc[ITEMS], fk[ITEMS];
 #pragma unroll
-      for (int j = 0; j < kStarItems; ++j) {
+      for (int j = 0; j < ITEMS; ++j) {
         int64_t r = r0 + (int64_t)j * blockDim.x;
         rc[j] = r <= last ? r : last;
         fk[j] = S.fk[d0][rc[j]];
       }
-      bool pass[kStarItems];
+      bool pass[ITEMS];
 #pragma unroll
-      for (int j = 0; j < kStarItems; ++j) {
+      for (int j = 0; j < ITEMS; ++j) {
         int64_t r = r0 + (int64_t)j * blockDim.x;
         bool hit = loff[d0] >= 0 ? bf_find(s_filt + loff[d0], lnb[d0], s_masks, fk[j])
                                  : bf_find(P.dim[d0].fbf, lnb[d0], s_masks, fk[j]);
@@ -374,21 +375,21 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
       for (int i = 1; i < P.ndims; ++i) {
         bool any = false;
 #pragma unroll
-        for (int j = 0; j < kStarItems; ++j) any |= pass[j];
+        for (int j = 0; j < ITEMS; ++j) any |= pass[j];
         if (!__ballot(any)) break;
         const int d = P.order[i];
         const int64_t* col = S.fk[d];
 #pragma unroll
-        for (int j = 0; j < kStarItems; ++j) fk[j] = col[pass[j] ? rc[j] : 0];
+        for (int j = 0; j < ITEMS; ++j) fk[j] = col[pass[j] ? rc[j] : 0];
 #pragma unroll
-        for (int j = 0; j < kStarItems; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
           const bool hit = loff[d] >= 0 ? bf_find(s_filt + loff[d], lnb[d], s_masks, fk[j])
                                         : bf_find(P.dim[d].fbf, lnb[d], s_masks, fk[j]);
           pass[j] = pass[j] & hit;
         }
       }
 #pragma unroll
-      for (int j = 0; j < kStarItems; ++j) append_survivor(P, S, pass[j], r0 + (int64_t)j * blockDim.x);
+      for (int j = 0; j < ITEMS; ++j) append_survivor(P, S, pass[j], r0 + (int64_t)j * blockDim.x);
     }
     }
   }
@@ -533,6 +534,7 @@ struct fpdb_star {
   std::vector<void*> owned;
   int grid = 0;
   int resident_per_cu = 0;             // workgroups of the persistent kernel one CU holds
+  void (*kernel)(fpdb::StarDev) = nullptr;   // the persistent kernel for the tuned items per thread
   // result (device)
   int32_t* out_g0 = nullptr;
   int32_t* out_g1 = nullptr;
@@ -821,7 +823,7 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
   FPDB_HIP(hipGetLastError());
   {
     ProfScope ps(c, "star_persistent");
-    hipLaunchKernelGGL(star_persistent_kernel, dim3(s->grid), dim3(kStarThreads), kStarDynamicLds, c->stream, s->dev);
+    hipLaunchKernelGGL(s->kernel, dim3(s->grid), dim3(kStarThreads), kStarDynamicLds, c->stream, s->dev);
   }
   FPDB_HIP(hipGetLastError());
   hipLaunchKernelGGL(star_emit_kernel, dim3(1), dim3(1024), 0, c->stream, s->dev, s->out_g0, s->out_g1, s->out_cnt,
@@ -926,6 +928,9 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   s->q.partition_measure = nullptr;
   s->tune = tuning ? *tuning : fpdb_star_tuning{0, 0, 1, 0};
   if (s->tune.ht_load_shift < 1 || s->tune.ht_load_shift > 3) s->tune.ht_load_shift = 1;
+  FPDB_CHECK(s->tune.items_per_thread == 0 || s->tune.items_per_thread == 4 || s->tune.items_per_thread == 8 ||
+                 s->tune.items_per_thread == 16,
+             FPDB_ERR_INVALID, "star join: items_per_thread must be 0, 4, 8 or 16");
   FPDB_CHECK(s->tune.barrier_timeout_us >= 0, FPDB_ERR_INVALID, "star join: negative barrier timeout");
   s->max_groups = q->max_groups;
   StarDev& P = s->dev;
@@ -982,9 +987,13 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
     return st;
   }
   // persistent grid: every workgroup resident at once
+  switch (s->tune.items_per_thread) {
+    case 4: s->kernel = star_persistent_kernel<4>; break;
+    case 16: s->kernel = star_persistent_kernel<16>; break;
+    default: s->kernel = star_persistent_kernel<kStarItems>; break;
+  }
   int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, star_persistent_kernel, kStarThreads,
-                                                              kStarDynamicLds);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, s->kernel, kStarThreads, kStarDynamicLds);
   if (e != hipSuccess || per_cu < 1) {
     fpdb_star_destroy(s);
     return fail(FPDB_ERR_HIP, "star join: occupancy query failed");

@@ -6,6 +6,7 @@ inputs, keeps the fastest, and caches it per (workload signature, device) in a
 JSON file (FPDB_AUTOTUNE_CACHE, default flexpushdowndb_amd/_build/autotune.json).
 """
 import ctypes
+import itertools
 import json
 import os
 
@@ -17,7 +18,12 @@ from .device import DeviceColumn
 _PKG = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_CACHE = os.path.join(_PKG, "_build", "autotune.json")
 
-CANDIDATES = [(bpc, shift) for bpc in (1, 2, 4, 8) for shift in (1, 2)]
+# the autotuner's search, knob by knob: launch shape (workgroups per CU x
+# hash-table load shift), then fact rows per thread per step, then the order of
+# the forward filters over the fact rows
+SHAPES = [(bpc, shift) for bpc in (1, 2, 4, 8) for shift in (1, 2)]
+ITEMS = (4, 8, 16)
+MAX_ORDERS = 24
 
 
 class StarJoin:
@@ -89,12 +95,24 @@ class StarJoin:
         sel = [dim["keep_below"] / 100.0 for dim in self.w["dims"]]
         return list(np.argsort(sel, kind="stable"))
 
+    def set_order(self, order):
+        for i, d in enumerate(order):
+            self.q.filter_order[i] = int(d)
+
+    def order(self):
+        return [int(self.q.filter_order[i]) for i in range(self.q.num_dims)]
+
     def prepare(self, tuning=None):
+        """tuning: (blocks per CU, hash-table load shift[, items per thread[,
+        forward filter order]]) or None for the defaults."""
         self.close()
         t = None
+        if tuning is not None and len(tuning) > 3:
+            self.set_order(tuning[3])
         if tuning is not None or self.barrier_timeout_us:
-            bpc, shift = tuning if tuning is not None else (0, 1)
-            t = C.StarTuning(int(bpc), self.barrier_timeout_us, int(shift), 0)
+            bpc, shift = tuning[:2] if tuning is not None else (0, 1)
+            items = tuning[2] if tuning is not None and len(tuning) > 2 else 0
+            t = C.StarTuning(int(bpc), self.barrier_timeout_us, int(shift), int(items))
         h = ctypes.c_void_p()
         C.check(C.lib().fpdb_star_prepare(self.ctx.handle, ctypes.byref(self.q), ctypes.byref(t) if t else None,
                                           ctypes.byref(h)))
@@ -159,8 +177,12 @@ def _time_batched(sj, reps):
 
 
 def autotune(sj, device_name, reps=3, cache_path=None):
-    """Times every candidate (blocks per CU, hash-table load shift) on the
-    prepared plan and keeps the fastest; cached per (signature, device)."""
+    """Tunes the batched form on the prepared plan, one knob at a time, each
+    candidate timed over `reps` runs: the launch shape (blocks per CU x
+    hash-table load shift), then fact rows per thread per step, then the
+    forward filters' order (every order for up to four dimensions, else the
+    default and its neighbours by one swap).  The result (blocks per CU, load
+    shift, items, order) is cached per (signature, device)."""
     path = cache_path or os.environ.get("FPDB_AUTOTUNE_CACHE", DEFAULT_CACHE)
     key = f"{sj.signature()}@{device_name}"
     cache = {}
@@ -170,19 +192,35 @@ def autotune(sj, device_name, reps=3, cache_path=None):
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-    if key in cache:
-        best = tuple(cache[key]["tuning"])
+    if key in cache and len(cache[key]["tuning"]) == 4:
+        t = cache[key]["tuning"]
+        best = (t[0], t[1], t[2], tuple(t[3]))
         sj.prepare(best)
         return best, cache[key]
     timings = {}
-    for cand in CANDIDATES:
-        sj.prepare(cand)
-        sj.run_batched()          # warm
-        timings[cand] = _time_batched(sj, reps)
-    best = min(timings, key=timings.get)
+
+    def measure(cand):
+        if cand not in timings:
+            sj.prepare(cand)
+            sj.run_batched()          # warm
+            timings[cand] = _time_batched(sj, reps)
+        return timings[cand]
+
+    base_order = tuple(sj.order())
+    shape = min(SHAPES, key=lambda c: measure((c[0], c[1], 8, base_order)))
+    items = min(ITEMS, key=lambda i: measure((shape[0], shape[1], i, base_order)))
+    nd = len(base_order)
+    if nd <= 4:
+        orders = list(itertools.permutations(base_order))[:MAX_ORDERS]
+    else:
+        orders = [base_order] + [tuple(base_order[:i] + (base_order[i + 1], base_order[i]) + base_order[i + 2:])
+                                 for i in range(nd - 1)]
+    order = min(orders, key=lambda o: measure((shape[0], shape[1], items, tuple(o))))
+    best = (shape[0], shape[1], items, tuple(order))
     sj.prepare(best)
-    entry = {"tuning": list(best), "ms": timings[best],
-             "candidates": {f"{b}/{s}": ms for (b, s), ms in timings.items()}}
+    entry = {"tuning": [best[0], best[1], best[2], list(best[3])], "ms": timings[best],
+             "candidates": {"/".join(str(x) for x in c[:3]) + "/" + "".join(str(d) for d in c[3]): ms
+                            for c, ms in timings.items()}}
     cache[key] = entry
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)

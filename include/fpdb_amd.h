@@ -648,16 +648,17 @@ typedef struct fpdb_star_query {
 } fpdb_star_query;
 
 /* Launch shape of the batched form (0 = default): workgroups per CU of the
- * persistent kernel, and hash-table capacity as a power-of-two multiple
- * (2^ht_load_shift) of the dimension rows.  Chosen by the autotuner.
- * barrier_timeout_us bounds each grid-barrier wait of the persistent kernel
- * (0 = 200 ms); a wait past it ends the launch early and the result call
- * reports FPDB_ERR_INTERNAL. */
+ * persistent kernel, hash-table capacity as a power-of-two multiple
+ * (2^ht_load_shift) of the dimension rows, and fact rows per thread per step
+ * of its fact pass (4, 8 or 16; 0 = 8).  Chosen by the autotuner (with the
+ * query's filter_order).  barrier_timeout_us bounds each grid-barrier wait of
+ * the persistent kernel (0 = 200 ms); a wait past it ends the launch early
+ * and the result call reports FPDB_ERR_INTERNAL. */
 typedef struct fpdb_star_tuning {
   int32_t blocks_per_cu;
   int32_t barrier_timeout_us;
   int32_t ht_load_shift;
-  int32_t reserved1;
+  int32_t items_per_thread;
 } fpdb_star_tuning;
 
 typedef struct fpdb_star fpdb_star;

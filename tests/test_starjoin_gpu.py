@@ -113,7 +113,9 @@ def test_star_join_autotune_and_full_size(ctx, tmp_path):
     exp = datagen.star_expected(w)
     sj = StarJoin(ctx, w)
     best, entry = autotune(sj, "test-device", reps=1, cache_path=str(tmp_path / "tune.json"))
-    assert tuple(entry["tuning"]) == best
+    t = entry["tuning"]
+    assert (t[0], t[1], t[2], tuple(t[3])) == best
+    assert sorted(best[3]) == list(range(4)) and best[2] in (4, 8, 16)
     sj.run_batched()
     _same(sj.result(), exp)
     sj.run_plain()
