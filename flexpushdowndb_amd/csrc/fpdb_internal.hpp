@@ -220,6 +220,16 @@ __device__ __forceinline__ void store_w(uint8_t* base, int width, int64_t i, uin
   }
 }
 
+// store_w with non-temporal 8- and 4-byte stores: for output columns written
+// once, which then do not push gathered or reused lines out of the L2.
+__device__ __forceinline__ void store_w_nt(uint8_t* base, int width, int64_t i, uint64_t v) {
+  switch (width) {
+    case 8: __builtin_nontemporal_store(v, reinterpret_cast<uint64_t*>(base) + i); break;
+    case 4: __builtin_nontemporal_store((uint32_t)v, reinterpret_cast<uint32_t*>(base) + i); break;
+    default: store_w(base, width, i, v); break;
+  }
+}
+
 // Copies count 8-byte words src[i * src_stride] into dst[i] (LDS) with the
 // whole workgroup.  Each thread issues kStageWords loads before its first LDS
 // write: a loop of one load and one write per trip would wait for memory once

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void hj_emit_kernel(const Slot* __restrict__ s
     for (int i = 0; i < kProbeItems; ++i) {
       if (!emit[i]) continue;
       int64_t di = dst[i] + out.offset[c];
-      store_w(out.values[c], out.width[c], di, v[i]);
+      store_w_nt(out.values[c], out.width[c], di, v[i]);
       if (out.validity[c] && valid_bit(pcols.validity[c], si[i]))
         __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[c]) + (di >> 5), 1u << (di & 31),
                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void hj_emit_kernel(const Slot* __restrict__ s
       if (!emit[i]) continue;
       int64_t di = dst[i] + out.offset[oc];
       bool m = cnt[i] != 0;
-      store_w(out.values[oc], out.width[oc], di, m ? v[i] : 0);
+      store_w_nt(out.values[oc], out.width[oc], di, m ? v[i] : 0);
       bool valid = m && (from_key || bcols.values[c] == nullptr || valid_bit(bcols.validity[c], bi[i]));
       if (out.validity[oc] && valid)
         __hip_atomic_fetch_or(reinterpret_cast<uint32_t*>(out.validity[oc]) + (di >> 5), 1u << (di & 31),
