@@ -1281,18 +1281,24 @@ __device__ __forceinline__ uint64_t acc_combine(int32_t op, uint64_t x, uint64_t
   }
 }
 
-// Merge: 64 head keys per workgroup of 16 waves; wave q adds the partials of
-// workgroups [q * blocks / 16, (q + 1) * blocks / 16) in order (the field's
-// loads of four workgroups in flight at a time), then wave 0 adds the 16
-// sums in wave order: a fixed order, so head keys' FP64 sums do not depend on
-// scheduling across workgroups.  Each head key's record goes to nstart[b] +
-// ocount[b] + its rank among bucket b's head keys.  Dynamic LDS:
+// Head merge, in two launches so that enough workgroups share the reads: the
+// first reduces the main pass's per-workgroup partials in `splits` ranges of
+// workgroups (grid: 64 head keys x splits) into one partial per range, the
+// second reduces those (splits = 1) and writes the records.  Within a launch a
+// workgroup of 16 waves takes 64 head keys; wave q adds the partials of
+// parts [q * n / 16, (q + 1) * n / 16) of its range in order (the field's
+// loads of four parts in flight at a time), then wave 0 adds the 16 sums in
+// wave order: a fixed order throughout, so head keys' FP64 sums do not depend
+// on scheduling across workgroups.  Partials: [part][1 + nacc][max_heads]
+// (first row, accumulators).  out_part null: each head key's record goes to
+// nstart[b] + ocount[b] + its rank among bucket b's head keys.  Dynamic LDS:
 // kMergeWaves x (1 + nacc) x 64 words.
 constexpr int kMergeWaves = 16;
+constexpr int kMergeSplits = 16;
 __global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
-    AccSpec A, GbGeom G, int blocks, const uint64_t* __restrict__ head_keys, const uint32_t* __restrict__ head_rank,
-    const int* __restrict__ nhead_p, const uint64_t* __restrict__ hpart, const int64_t* __restrict__ nstart,
-    const int64_t* __restrict__ ocount, uint64_t* __restrict__ pool) {
+    AccSpec A, GbGeom G, int nparts, const uint64_t* __restrict__ head_keys, const uint32_t* __restrict__ head_rank,
+    const int* __restrict__ nhead_p, const uint64_t* __restrict__ part, uint64_t* __restrict__ out_part,
+    const int64_t* __restrict__ nstart, const int64_t* __restrict__ ocount, uint64_t* __restrict__ pool) {
   extern __shared__ uint64_t s_part[];   // [wave][field][lane]
   const int lane = threadIdx.x & 63;
   const int q = threadIdx.x >> 6;
@@ -1303,11 +1309,14 @@ __global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
   const int nhead = *nhead_p;
   const bool live = h < nhead;
   const int hc = live ? h : 0;
-  const int w0 = (int)((int64_t)blocks * q / kMergeWaves), w1 = (int)((int64_t)blocks * (q + 1) / kMergeWaves);
+  const int splits = gridDim.y;
+  const int r0 = (int)((int64_t)nparts * blockIdx.y / splits), r1 = (int)((int64_t)nparts * (blockIdx.y + 1) / splits);
+  const int w0 = r0 + (int)((int64_t)(r1 - r0) * q / kMergeWaves);
+  const int w1 = r0 + (int)((int64_t)(r1 - r0) * (q + 1) / kMergeWaves);
   for (int f = 0; f < nf; ++f) {
     const int32_t op = f == 0 ? kAccMinI : A.op[f - 1];
     uint64_t x = f == 0 ? (uint64_t)LLONG_MAX : acc_identity(op);
-    const uint64_t* p = hpart + (size_t)f * H + hc;
+    const uint64_t* p = part + (size_t)f * H + hc;
     int w = w0;
     for (; w + 4 <= w1; w += 4) {
       uint64_t v[4];
@@ -1326,15 +1335,24 @@ __global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
     const uint64_t f = s_part[((size_t)r * nf) * 64 + lane];
     first = (long long)f < (long long)first ? f : first;
   }
-  const uint64_t k = head_keys[h];
-  const uint32_t b = bucket_of(gb_hash(k), G.bucket_shift);
-  uint64_t* rec = pool + (size_t)(nstart[b] + ocount[b] + head_rank[h]) * G.rec_words;
-  rec[0] = k;
-  rec[1] = first;
+  uint64_t* rec;
+  size_t stride;
+  if (out_part) {
+    rec = out_part + (size_t)blockIdx.y * nf * H + h;
+    stride = (size_t)H;
+  } else {
+    const uint64_t k = head_keys[h];
+    const uint32_t b = bucket_of(gb_hash(k), G.bucket_shift);
+    rec = pool + (size_t)(nstart[b] + ocount[b] + head_rank[h]) * G.rec_words;
+    rec[0] = k;
+    ++rec;
+    stride = 1;
+  }
+  rec[0] = first;
   for (int a = 0; a < nacc; ++a) {
     uint64_t x = s_part[(1 + a) * 64 + lane];
     for (int r = 1; r < kMergeWaves; ++r) x = acc_combine(A.op[a], x, s_part[((size_t)r * nf + 1 + a) * 64 + lane]);
-    rec[2 + a] = x;
+    rec[(size_t)(1 + a) * stride] = x;
   }
 }
 
@@ -1721,6 +1739,7 @@ struct fpdb_group_table {
   unsigned int* head_count = nullptr;   // nbuckets
   int* nhead = nullptr;              // head keys of the current batch
   uint64_t* hpart = nullptr;         // max_blocks x (1 + accumulators) x max_heads head partials
+  uint64_t* hmerge = nullptr;        // kMergeSplits x (1 + accumulators) x max_heads: the first merge's output
   uint64_t* tail_total = nullptr;    // nbuckets: tail rows of the batch per bucket
   uint64_t* tail_start = nullptr;    // nbuckets: pool position of the bucket's first tail row
   unsigned long long* totals = nullptr;   // [0] records of the new pool, [1] groups
@@ -1874,6 +1893,8 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->nhead));
   if (s == FPDB_OK)
     s = gb_alloc(t, (uint64_t)t->max_blocks * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hpart));
+  if (s == FPDB_OK)
+    s = gb_alloc(t, (uint64_t)kMergeSplits * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hmerge));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_total));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_start));
   if (s == FPDB_OK) s = gb_alloc(t, 32, reinterpret_cast<void**>(&t->totals));
@@ -2056,10 +2077,15 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
   }
   {
     ProfScope ps(c, "groupby_compact");
-    hipLaunchKernelGGL(gb_head_merge_kernel, dim3((unsigned)ceil_div(t->max_heads, 64)), dim3(64 * kMergeWaves),
-                       (size_t)kMergeWaves * (1 + t->acc.n) * 64 * 8,
-                       c->stream, t->acc, G, blocks, t->head_keys, t->head_rank, t->nhead, t->hpart, t->nstart,
-                       t->ocount, t->spare);
+    const unsigned hgroups = (unsigned)ceil_div(t->max_heads, 64);
+    const size_t mlds = (size_t)kMergeWaves * (1 + t->acc.n) * 64 * 8;
+    const int splits = std::min(kMergeSplits, blocks);
+    hipLaunchKernelGGL(gb_head_merge_kernel, dim3(hgroups, (unsigned)splits), dim3(64 * kMergeWaves), mlds,
+                       c->stream, t->acc, G, blocks, t->head_keys, t->head_rank, t->nhead, t->hpart, t->hmerge,
+                       t->nstart, t->ocount, t->spare);
+    hipLaunchKernelGGL(gb_head_merge_kernel, dim3(hgroups, 1), dim3(64 * kMergeWaves), mlds, c->stream, t->acc, G,
+                       splits, t->head_keys, t->head_rank, t->nhead, t->hmerge, nullptr, t->nstart, t->ocount,
+                       t->spare);
     FPDB_GB_DISPATCH(launch_compact, c, t, rk, rr, G.row_base);
     FPDB_HIP(hipGetLastError());
   }
@@ -2149,8 +2175,8 @@ fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
   void* bufs[] = {t->pool, t->spare, t->tpool, t->sample_keys, t->sample_cnt, t->list_keys, t->list_cnt,
                   t->list_hist, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
-                  t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->tail_total, t->tail_start,
-                  t->totals, t->flags};
+                  t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->hmerge,
+                  t->tail_total, t->tail_start, t->totals, t->flags};
   for (void* b : bufs)
     if (b) ctx_free(t->ctx, b);
   delete t;

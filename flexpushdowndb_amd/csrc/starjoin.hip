@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
       for (int j = 0; j < ITEMS; ++j) {
         int64_t r = r0 + (int64_t)j * blockDim.x;
         rc[j] = r <= last ? r : last;
-        fk[j] = S.fk[d0][rc[j]];
+        fk[j] = __builtin_nontemporal_load(S.fk[d0] + rc[j]);   // streamed once
       }
       bool pass[ITEMS];
 #pragma unroll
