@@ -841,8 +841,9 @@ __global__ __launch_bounds__(1024) void gb_bucket_scan_kernel(const uint32_t* __
 // head + tail records) or live in the tail pool as raw rows (separate == 1:
 // tail_start[b] = their first tail-pool position, bucket-major, newcount[b] =
 // head records).  Each workgroup's tail rows of b start at tail_start[b] +
-// base[b][w].  totals[0] = records of the new pool; totals[1] (groups) reset
-// for the compact pass; totals[2] = tail rows of the batch.
+// base[b][w].  totals[0] = records of the new pool; totals[2] = tail rows of
+// the batch (totals[1], the group count, is summed from the buckets' record
+// counts when it is read).
 __global__ __launch_bounds__(1024) void gb_bucket_offsets_kernel(int nbuckets, int separate,
                                                                  const int64_t* __restrict__ ocount,
                                                                  const unsigned int* __restrict__ head_count,
@@ -902,7 +903,6 @@ __global__ __launch_bounds__(1024) void gb_bucket_offsets_kernel(int nbuckets, i
   }
   if (threadIdx.x == blockDim.x - 1) {
     totals[0] = (unsigned long long)run;
-    totals[1] = 0;
     totals[2] = (unsigned long long)trun;
   }
 }
@@ -1281,79 +1281,51 @@ __device__ __forceinline__ uint64_t acc_combine(int32_t op, uint64_t x, uint64_t
   }
 }
 
-// Head merge, in two launches so that enough workgroups share the reads: the
-// first reduces the main pass's per-workgroup partials in `splits` ranges of
-// workgroups (grid: 64 head keys x splits) into one partial per range, the
-// second reduces those (splits = 1) and writes the records.  Within a launch a
-// workgroup of 16 waves takes 64 head keys; wave q adds the partials of
-// parts [q * n / 16, (q + 1) * n / 16) of its range in order (the field's
-// loads of four parts in flight at a time), then wave 0 adds the 16 sums in
-// wave order: a fixed order throughout, so head keys' FP64 sums do not depend
-// on scheduling across workgroups.  Partials: [part][1 + nacc][max_heads]
-// (first row, accumulators).  out_part null: each head key's record goes to
-// nstart[b] + ocount[b] + its rank among bucket b's head keys.  Dynamic LDS:
-// kMergeWaves x (1 + nacc) x 64 words.
+// Head merge, one launch: workgroup (g, f) reduces field f (0: first row,
+// 1 + a: accumulator a) of head keys [64 g, 64 g + 64) over the main pass's
+// per-workgroup partials [workgroup][1 + nacc][max_heads].  Wave q takes the
+// partials of workgroups [q * blocks / 16, (q + 1) * blocks / 16), all loaded
+// at once, and adds them in order; wave 0 then adds the 16 sums in wave order:
+// a fixed order, so head keys' FP64 sums do not depend on scheduling across
+// workgroups.  Each head key's record goes to nstart[b] + ocount[b] + its
+// rank among bucket b's head keys (field 0's workgroup writes the key too).
 constexpr int kMergeWaves = 16;
-constexpr int kMergeSplits = 16;
+constexpr int kMergeLoads = 16;   // partials a lane loads before adding (more go in further rounds)
 __global__ __launch_bounds__(64 * kMergeWaves) void gb_head_merge_kernel(
-    AccSpec A, GbGeom G, int nparts, const uint64_t* __restrict__ head_keys, const uint32_t* __restrict__ head_rank,
-    const int* __restrict__ nhead_p, const uint64_t* __restrict__ part, uint64_t* __restrict__ out_part,
-    const int64_t* __restrict__ nstart, const int64_t* __restrict__ ocount, uint64_t* __restrict__ pool) {
-  extern __shared__ uint64_t s_part[];   // [wave][field][lane]
+    AccSpec A, GbGeom G, int blocks, const uint64_t* __restrict__ head_keys, const uint32_t* __restrict__ head_rank,
+    const int* __restrict__ nhead_p, const uint64_t* __restrict__ hpart, const int64_t* __restrict__ nstart,
+    const int64_t* __restrict__ ocount, uint64_t* __restrict__ pool) {
+  __shared__ uint64_t s_sum[kMergeWaves][64];
   const int lane = threadIdx.x & 63;
   const int q = threadIdx.x >> 6;
-  const int nacc = A.n;
-  const int nf = 1 + nacc;
+  const int nf = 1 + A.n;
+  const int f = blockIdx.y;
   const int H = G.max_heads;
   const int h = blockIdx.x * 64 + lane;
   const int nhead = *nhead_p;
   const bool live = h < nhead;
   const int hc = live ? h : 0;
-  const int splits = gridDim.y;
-  const int r0 = (int)((int64_t)nparts * blockIdx.y / splits), r1 = (int)((int64_t)nparts * (blockIdx.y + 1) / splits);
-  const int w0 = r0 + (int)((int64_t)(r1 - r0) * q / kMergeWaves);
-  const int w1 = r0 + (int)((int64_t)(r1 - r0) * (q + 1) / kMergeWaves);
-  for (int f = 0; f < nf; ++f) {
-    const int32_t op = f == 0 ? kAccMinI : A.op[f - 1];
-    uint64_t x = f == 0 ? (uint64_t)LLONG_MAX : acc_identity(op);
-    const uint64_t* p = part + (size_t)f * H + hc;
-    int w = w0;
-    for (; w + 4 <= w1; w += 4) {
-      uint64_t v[4];
+  const int32_t op = f == 0 ? kAccMinI : A.op[f - 1];
+  const int w0 = (int)((int64_t)blocks * q / kMergeWaves), w1 = (int)((int64_t)blocks * (q + 1) / kMergeWaves);
+  const uint64_t* p = hpart + (size_t)f * H + hc;
+  uint64_t x = f == 0 ? (uint64_t)LLONG_MAX : acc_identity(op);
+  for (int wb = w0; wb < w1; wb += kMergeLoads) {
+    uint64_t v[kMergeLoads];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) v[j] = p[(size_t)(w + j) * nf * H];
+    for (int j = 0; j < kMergeLoads; ++j) v[j] = p[(size_t)(wb + j < w1 ? wb + j : w1 - 1) * nf * H];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) x = acc_combine(op, x, v[j]);
-    }
-    for (; w < w1; ++w) x = acc_combine(op, x, p[(size_t)w * nf * H]);
-    s_part[((size_t)q * nf + f) * 64 + lane] = x;
+    for (int j = 0; j < kMergeLoads; ++j)
+      if (wb + j < w1) x = acc_combine(op, x, v[j]);
   }
+  s_sum[q][lane] = x;
   __syncthreads();
   if (q != 0 || !live) return;
-  uint64_t first = s_part[lane];
-  for (int r = 1; r < kMergeWaves; ++r) {
-    const uint64_t f = s_part[((size_t)r * nf) * 64 + lane];
-    first = (long long)f < (long long)first ? f : first;
-  }
-  uint64_t* rec;
-  size_t stride;
-  if (out_part) {
-    rec = out_part + (size_t)blockIdx.y * nf * H + h;
-    stride = (size_t)H;
-  } else {
-    const uint64_t k = head_keys[h];
-    const uint32_t b = bucket_of(gb_hash(k), G.bucket_shift);
-    rec = pool + (size_t)(nstart[b] + ocount[b] + head_rank[h]) * G.rec_words;
-    rec[0] = k;
-    ++rec;
-    stride = 1;
-  }
-  rec[0] = first;
-  for (int a = 0; a < nacc; ++a) {
-    uint64_t x = s_part[(1 + a) * 64 + lane];
-    for (int r = 1; r < kMergeWaves; ++r) x = acc_combine(A.op[a], x, s_part[((size_t)r * nf + 1 + a) * 64 + lane]);
-    rec[(size_t)(1 + a) * stride] = x;
-  }
+  for (int r = 1; r < kMergeWaves; ++r) x = acc_combine(op, x, s_sum[r][lane]);
+  const uint64_t k = head_keys[h];
+  const uint32_t b = bucket_of(gb_hash(k), G.bucket_shift);
+  uint64_t* rec = pool + (size_t)(nstart[b] + ocount[b] + head_rank[h]) * G.rec_words;
+  if (f == 0) rec[0] = k;
+  rec[1 + f] = x;
 }
 
 // Compact pass, one workgroup per bucket: old records (previous batches, one
@@ -1375,7 +1347,7 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, ui
                                                                    const uint64_t* __restrict__ tail_start,
                                                                    const uint64_t* __restrict__ tail_total, int rk,
                                                                    int rr, int64_t row_base,
-                                                                   unsigned long long* groups, int* overflow) {
+                                                                   int* overflow) {
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t s_n;
   __shared__ int s_over;
@@ -1488,8 +1460,254 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_kernel(AccSpec A, ui
   if (threadIdx.x == 0) {
     ostart[p] = start;
     ocount[p] = s_n;
-    if (s_n) atomicAdd(groups, (unsigned long long)s_n);
     if (s_over) *overflow = 1;
+  }
+}
+
+// Balanced compact, for the staged main pass's layout (raw tail rows in the
+// tail pool).  The buckets' items -- old records, head records, raw tail rows,
+// in that order, bucket after bucket -- are numbered by record position:
+// bucket b's items are the positions [nstart[b], nstart[b] + its item count)
+// of the new pool, all of them [0, total).  Piece q (one workgroup per CU)
+// takes positions [total * q / P, total * (q + 1) / P), so every CU streams
+// the same number of items whatever the bucket sizes (with one workgroup per
+// bucket the largest bucket set the time).  For each bucket it overlaps, a
+// piece aggregates the overlapping items in an LDS table and writes one
+// record per group at the segment's first position (a segment of m items has
+// at most m groups, so it never writes past its own positions).  A bucket
+// inside one piece is then final (its record count in bcount[b]); a bucket
+// split over pieces q0..q1 has one partial record set per piece, counted in
+// seg_last[q0], seg_first[q] = seg_last[q] for q0 < q < q1, and seg_first[q1],
+// which gb_compact_merge_kernel combines.
+__device__ __forceinline__ int64_t piece_lo(int64_t total, int q, int pieces) {
+  return (int64_t)((uint64_t)total * (uint64_t)q / (uint64_t)pieces);   // total < 2^40, q <= 1024
+}
+
+__device__ __forceinline__ int piece_of(int64_t x, int64_t total, int pieces) {
+  int q = (int)((uint64_t)x * (uint64_t)pieces / (uint64_t)total);
+  if (q >= pieces) q = pieces - 1;
+  while (q + 1 < pieces && piece_lo(total, q + 1, pieces) <= x) ++q;
+  while (q > 0 && piece_lo(total, q, pieces) > x) --q;
+  return q;
+}
+
+template <int NACC>
+__global__ __launch_bounds__(kGroupThreads) void gb_compact_piece_kernel(
+    AccSpec A, uint32_t nslots, int rec_words, int nbuckets, const unsigned long long* __restrict__ totals,
+    const uint64_t* __restrict__ oldpool, const int64_t* __restrict__ ostart, const int64_t* __restrict__ ocount,
+    uint64_t* pool, const int64_t* __restrict__ nstart, const int64_t* __restrict__ newcount,
+    const uint64_t* __restrict__ tpool, const uint64_t* __restrict__ tail_start,
+    const uint64_t* __restrict__ tail_total, int rk, int rr, int64_t row_base, int64_t* __restrict__ bcount,
+    int64_t* __restrict__ seg_first, int64_t* __restrict__ seg_last, int* overflow) {
+  static_assert(NACC > 0, "fixed accumulator count");
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t s_n;
+  __shared__ int s_over;
+  const int pieces = gridDim.x, q = blockIdx.x;
+  const int64_t total = (int64_t)totals[0];
+  const int64_t lo = piece_lo(total, q, pieces), hi = piece_lo(total, q + 1, pieces);
+  if (lo >= hi) return;
+  // the bucket holding position lo: the last one starting at or before it
+  int b = 0;
+  for (int b1 = nbuckets - 1; b < b1;) {
+    const int m = (b + b1 + 1) >> 1;
+    if (nstart[m] <= lo) b = m;
+    else b1 = m - 1;
+  }
+  LdsTable T = lds_table(lds, nslots);
+  constexpr int RW = 2 + NACC;
+  int vcol[NACC];   // raw operand word of each accumulator (-1: COUNT_ALL, operand 1)
+  {
+    int nv = 0;
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) vcol[a] = A.op[a] == kAccCountAll ? -1 : (rk == 2 ? 2 : 1) + nv++;
+  }
+  constexpr int R = 4;   // items per thread per trip, all loaded before any is aggregated
+  if (threadIdx.x == 0) s_over = 0;
+  auto apply = [&](const uint64_t* w) {
+    const uint64_t k = w[0];
+    const int s = lds_find_or_insert(T, k, fmix64(k), nslots);
+    if (s < 0) {
+      s_over = 1;
+      return;
+    }
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) {
+      const uint64_t v = w[2 + a];
+      if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
+    }
+    const long long f = (long long)w[1];
+    if (f < T.first[s]) atomicMin(&T.first[s], f);
+  };
+  for (; b < nbuckets && nstart[b] < hi; ++b) {
+    const int64_t nold = ocount[b], nnew = newcount[b], ntail = (int64_t)tail_total[b];
+    const int64_t bs = nstart[b], be = bs + nold + nnew + ntail;
+    const int64_t s0 = lo > bs ? lo : bs, s1 = hi < be ? hi : be;
+    if (s0 >= s1) continue;
+    lds_table_init(T, A, NACC);
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    // records: old ones (previous batches), then the head records in the pool
+    const int64_t nrec = nold + nnew;
+    const int64_t r0 = s0 - bs, r1 = s1 - bs < nrec ? s1 - bs : nrec;
+    for (int64_t t0 = r0; t0 < r1; t0 += (int64_t)R * blockDim.x) {
+      const int64_t i0 = t0 + threadIdx.x;
+      uint64_t w[R][RW];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        int64_t i = i0 + (int64_t)j * blockDim.x;
+        i = i < r1 ? i : r1 - 1;
+        const uint64_t* rec = i < nold ? oldpool + (size_t)(ostart[b] + i) * rec_words : pool + (size_t)(bs + i) * rec_words;
+        load_rec<RW>(rec, w[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j)
+        if (i0 + (int64_t)j * blockDim.x < r1) apply(w[j]);
+    }
+    // raw tail rows (words loaded by their runtime position, not picked from
+    // a register array, which would put the array in scratch memory)
+    const int64_t j0 = s0 - bs - nrec > 0 ? s0 - bs - nrec : 0, j1 = s1 - bs - nrec;
+    const uint64_t* tails = tpool + (size_t)tail_start[b] * rr;
+    for (int64_t t0 = j0; t0 < j1; t0 += (int64_t)R * blockDim.x) {
+      const int64_t i0 = t0 + threadIdx.x;
+      uint64_t w[R][RW];
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        int64_t i = i0 + (int64_t)j * blockDim.x;
+        i = i < j1 ? i : j1 - 1;
+        const uint64_t* raw = tails + (size_t)i * rr;
+        w[j][0] = raw[0];
+        w[j][1] = rk == 2 ? raw[1] : 0;
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) w[j][2 + a] = vcol[a] < 0 ? 1ULL : raw[vcol[a]];
+      }
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if (i0 + (int64_t)j * blockDim.x >= j1) continue;
+        if (rk == 2) {
+          w[j][1] = (uint64_t)(row_base + (int64_t)(uint32_t)w[j][1]);
+        } else {
+          const uint64_t x0 = w[j][0];
+          const uint32_t k32 = (uint32_t)x0;
+          w[j][0] = rk == 3 ? (uint64_t)(int64_t)(int32_t)k32 : (uint64_t)k32;
+          w[j][1] = (uint64_t)(row_base + (int64_t)(x0 >> 32));
+        }
+        apply(w[j]);
+      }
+    }
+    __syncthreads();   // every item of the segment is read before any record is written
+    uint64_t* out = pool + (size_t)s0 * rec_words;
+    for (uint32_t s = threadIdx.x; s < T.stride; s += blockDim.x) {
+      const long long f = T.first[s];
+      if (f == LLONG_MAX) continue;
+      const uint32_t idx = atomicAdd(&s_n, 1u);
+      uint64_t* rec = out + (size_t)idx * rec_words;
+      rec[0] = s == T.nslots ? kEmptyKey : T.keys[s];
+      rec[1] = (uint64_t)f;
+      for (int a = 0; a < NACC; ++a) rec[2 + a] = T.acc[(size_t)a * T.stride + s];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const bool before = bs < lo, after = be > hi;
+      if (!before && !after) bcount[b] = s_n;
+      if (before) seg_first[q] = s_n;
+      if (after) seg_last[q] = s_n;
+    }
+  }
+  if (threadIdx.x == 0 && s_over) *overflow = 1;
+}
+
+// After the pieces, one workgroup per bucket: a bucket inside one piece gets
+// its count; a split one has its partial record sets aggregated in LDS and
+// written back, one record per group, at the start of its range.
+template <int NACC>
+__global__ __launch_bounds__(kGroupThreads) void gb_compact_merge_kernel(
+    AccSpec A, uint32_t nslots, int rec_words, int pieces, const unsigned long long* __restrict__ totals,
+    uint64_t* pool, const int64_t* __restrict__ nstart, int64_t* __restrict__ ostart, int64_t* __restrict__ ocount,
+    const int64_t* __restrict__ newcount, const uint64_t* __restrict__ tail_total,
+    const int64_t* __restrict__ bcount, const int64_t* __restrict__ seg_first,
+    const int64_t* __restrict__ seg_last, int* overflow) {
+  static_assert(NACC > 0, "fixed accumulator count");
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t s_n;
+  __shared__ int s_over;
+  const int b = blockIdx.x;
+  const int64_t bs = nstart[b];
+  const int64_t items = ocount[b] + newcount[b] + (int64_t)tail_total[b];
+  const int64_t total = (int64_t)totals[0];
+  const int q0 = items ? piece_of(bs, total, pieces) : 0, q1 = items ? piece_of(bs + items - 1, total, pieces) : 0;
+  if (items == 0 || q0 == q1) {
+    if (threadIdx.x == 0) {
+      ostart[b] = bs;
+      ocount[b] = items ? bcount[b] : 0;
+    }
+    return;
+  }
+  LdsTable T = lds_table(lds, nslots);
+  lds_table_init(T, A, NACC);
+  if (threadIdx.x == 0) {
+    s_n = 0;
+    s_over = 0;
+  }
+  __syncthreads();
+  constexpr int RW = 2 + NACC;
+  for (int q = q0; q <= q1; ++q) {
+    const int64_t lo = piece_lo(total, q, pieces);
+    if (lo == piece_lo(total, q + 1, pieces)) continue;   // an empty piece (fewer items than pieces) wrote nothing
+    const int64_t s0 = lo > bs ? lo : bs;
+    const int64_t cnt = q == q0 ? seg_last[q] : seg_first[q];
+    for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+      uint64_t w[RW];
+      load_rec<RW>(pool + (size_t)(s0 + i) * rec_words, w);
+      const uint64_t k = w[0];
+      const int s = lds_find_or_insert(T, k, fmix64(k), nslots);
+      if (s < 0) {
+        s_over = 1;
+        continue;
+      }
+#pragma unroll
+      for (int a = 0; a < NACC; ++a) {
+        const uint64_t v = w[2 + a];
+        if (v != acc_identity(A.op[a]) || A.op[a] == kAccSumF) acc_apply(&T.acc[(size_t)a * T.stride + s], A.op[a], v);
+      }
+      const long long f = (long long)w[1];
+      if (f < T.first[s]) atomicMin(&T.first[s], f);
+    }
+  }
+  __syncthreads();   // every partial record is read before any is overwritten
+  uint64_t* out = pool + (size_t)bs * rec_words;
+  for (uint32_t s = threadIdx.x; s < T.stride; s += blockDim.x) {
+    const long long f = T.first[s];
+    if (f == LLONG_MAX) continue;
+    const uint32_t idx = atomicAdd(&s_n, 1u);
+    uint64_t* rec = out + (size_t)idx * rec_words;
+    rec[0] = s == T.nslots ? kEmptyKey : T.keys[s];
+    rec[1] = (uint64_t)f;
+    for (int a = 0; a < NACC; ++a) rec[2 + a] = T.acc[(size_t)a * T.stride + s];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ostart[b] = bs;
+    ocount[b] = s_n;
+    if (s_over) *overflow = 1;
+  }
+}
+
+// Groups so far: the sum of the buckets' record counts into *out (one
+// workgroup; the compact passes keep per-bucket counts only, so that their
+// workgroups do not queue on one global counter as they finish together).
+__global__ __launch_bounds__(1024) void gb_sum_groups_kernel(const int64_t* __restrict__ ocount, int nbuckets,
+                                                             unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s_w[16];
+  unsigned long long v = 0;
+  for (int b = threadIdx.x; b < nbuckets; b += blockDim.x) v += (unsigned long long)ocount[b];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w];
+    *out = t;
   }
 }
 
@@ -1739,7 +1957,9 @@ struct fpdb_group_table {
   unsigned int* head_count = nullptr;   // nbuckets
   int* nhead = nullptr;              // head keys of the current batch
   uint64_t* hpart = nullptr;         // max_blocks x (1 + accumulators) x max_heads head partials
-  uint64_t* hmerge = nullptr;        // kMergeSplits x (1 + accumulators) x max_heads: the first merge's output
+  int64_t* bcount = nullptr;         // nbuckets: records of a bucket the compact finished in one piece
+  int64_t* seg_first = nullptr;      // max_blocks: records of a piece's first bucket, when split
+  int64_t* seg_last = nullptr;       // max_blocks: records of a piece's last bucket, when split
   uint64_t* tail_total = nullptr;    // nbuckets: tail rows of the batch per bucket
   uint64_t* tail_start = nullptr;    // nbuckets: pool position of the bucket's first tail row
   unsigned long long* totals = nullptr;   // [0] records of the new pool, [1] groups
@@ -1893,8 +2113,9 @@ static fpdb_status groupby_create_impl(fpdb_context* ctx, const int32_t* key_typ
   if (s == FPDB_OK) s = gb_alloc(t, 8, reinterpret_cast<void**>(&t->nhead));
   if (s == FPDB_OK)
     s = gb_alloc(t, (uint64_t)t->max_blocks * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hpart));
-  if (s == FPDB_OK)
-    s = gb_alloc(t, (uint64_t)kMergeSplits * (1 + nacc) * heads * 8, reinterpret_cast<void**>(&t->hmerge));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->bcount));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)t->max_blocks * 8, reinterpret_cast<void**>(&t->seg_first));
+  if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)t->max_blocks * 8, reinterpret_cast<void**>(&t->seg_last));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_total));
   if (s == FPDB_OK) s = gb_alloc(t, (uint64_t)nb * 8, reinterpret_cast<void**>(&t->tail_start));
   if (s == FPDB_OK) s = gb_alloc(t, 32, reinterpret_cast<void**>(&t->totals));
@@ -1966,10 +2187,22 @@ static void launch_main(Context* c, fpdb_group_table* t, int blocks, const GKeyS
 }
 template <int NACC>
 static void launch_compact(Context* c, fpdb_group_table* t, int rk, int rr, int64_t row_base) {
-  hipLaunchKernelGGL(gb_compact_kernel<NACC>, dim3(t->nbuckets), dim3(kGroupThreads), t->compact_lds, c->stream,
-                     t->acc, t->compact_slots, t->rec_words, t->pool, t->ostart, t->ocount, t->spare, t->nstart,
-                     t->newcount, NACC > 0 ? t->tpool : nullptr, t->tail_start, t->tail_total, rk, rr, row_base,
-                     t->totals + 1, t->flags);
+  if constexpr (NACC > 0) {
+    // raw tail rows: balanced pieces, then the merge of buckets split over pieces
+    const int pieces = t->max_blocks;
+    hipLaunchKernelGGL(gb_compact_piece_kernel<NACC>, dim3(pieces), dim3(kGroupThreads), t->compact_lds, c->stream,
+                       t->acc, t->compact_slots, t->rec_words, t->nbuckets, t->totals, t->pool, t->ostart, t->ocount,
+                       t->spare, t->nstart, t->newcount, t->tpool, t->tail_start, t->tail_total, rk, rr, row_base,
+                       t->bcount, t->seg_first, t->seg_last, t->flags);
+    hipLaunchKernelGGL(gb_compact_merge_kernel<NACC>, dim3(t->nbuckets), dim3(kGroupThreads), t->compact_lds,
+                       c->stream, t->acc, t->compact_slots, t->rec_words, pieces, t->totals, t->spare, t->nstart,
+                       t->ostart, t->ocount, t->newcount, t->tail_total, t->bcount, t->seg_first, t->seg_last,
+                       t->flags);
+  } else {
+    hipLaunchKernelGGL(gb_compact_kernel<NACC>, dim3(t->nbuckets), dim3(kGroupThreads), t->compact_lds, c->stream,
+                       t->acc, t->compact_slots, t->rec_words, t->pool, t->ostart, t->ocount, t->spare, t->nstart,
+                       t->newcount, nullptr, t->tail_start, t->tail_total, rk, rr, row_base, t->flags);
+  }
 }
 
 #define FPDB_GB_DISPATCH(FN, ...)                 \
@@ -2077,15 +2310,9 @@ static fpdb_status groupby_consume_batch(Context* c, fpdb_group_table* t, const 
   }
   {
     ProfScope ps(c, "groupby_compact");
-    const unsigned hgroups = (unsigned)ceil_div(t->max_heads, 64);
-    const size_t mlds = (size_t)kMergeWaves * (1 + t->acc.n) * 64 * 8;
-    const int splits = std::min(kMergeSplits, blocks);
-    hipLaunchKernelGGL(gb_head_merge_kernel, dim3(hgroups, (unsigned)splits), dim3(64 * kMergeWaves), mlds,
-                       c->stream, t->acc, G, blocks, t->head_keys, t->head_rank, t->nhead, t->hpart, t->hmerge,
-                       t->nstart, t->ocount, t->spare);
-    hipLaunchKernelGGL(gb_head_merge_kernel, dim3(hgroups, 1), dim3(64 * kMergeWaves), mlds, c->stream, t->acc, G,
-                       splits, t->head_keys, t->head_rank, t->nhead, t->hmerge, nullptr, t->nstart, t->ocount,
-                       t->spare);
+    hipLaunchKernelGGL(gb_head_merge_kernel, dim3((unsigned)ceil_div(t->max_heads, 64), (unsigned)(1 + t->acc.n)),
+                       dim3(64 * kMergeWaves), 0, c->stream, t->acc, G, blocks, t->head_keys, t->head_rank,
+                       t->nhead, t->hpart, t->nstart, t->ocount, t->spare);
     FPDB_GB_DISPATCH(launch_compact, c, t, rk, rr, G.row_base);
     FPDB_HIP(hipGetLastError());
   }
@@ -2175,8 +2402,8 @@ fpdb_status fpdb_groupby_destroy(fpdb_group_table* t) {
   if (!t) return FPDB_OK;
   void* bufs[] = {t->pool, t->spare, t->tpool, t->sample_keys, t->sample_cnt, t->list_keys, t->list_cnt,
                   t->list_hist, t->ostart, t->ocount, t->nstart, t->newcount, t->base, t->hist,
-                  t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart, t->hmerge,
-                  t->tail_total, t->tail_start, t->totals, t->flags};
+                  t->head_index, t->head_keys, t->head_rank, t->head_count, t->nhead, t->hpart,
+                  t->bcount, t->seg_first, t->seg_last, t->tail_total, t->tail_start, t->totals, t->flags};
   for (void* b : bufs)
     if (b) ctx_free(t->ctx, b);
   delete t;
@@ -2244,6 +2471,8 @@ fpdb_status fpdb_groupby_num_groups(fpdb_context* ctx, fpdb_group_table* t, int6
   Context* c = as_ctx(ctx);
   unsigned long long groups[2] = {0, 0};
   int flags[2] = {0, 0};
+  hipLaunchKernelGGL(gb_sum_groups_kernel, dim3(1), dim3(1024), 0, c->stream, t->ocount, t->nbuckets, t->totals + 1);
+  FPDB_HIP(hipGetLastError());
   FPDB_HIP(hipMemcpyAsync(groups, t->totals, 16, hipMemcpyDeviceToHost, c->stream));
   FPDB_HIP(hipMemcpyAsync(flags, t->flags, 8, hipMemcpyDeviceToHost, c->stream));
   FPDB_HIP(hipStreamSynchronize(c->stream));

@@ -47,8 +47,9 @@ GROUPS = {
                            "blocked bloom find on the 2^16-block filter: sliced find + combine"),
     "groupby_consume": ("groupby", ["gb_sample_count_kernel", "gb_sample_gather_kernel", "gb_sample_select_kernel",
                                     "gb_count_kernel", "gb_bucket_scan_kernel", "gb_bucket_offsets_kernel",
-                                    "gb_main_staged_kernel", "gb_head_merge_kernel", "gb_compact_kernel"],
-                        "group-by consume: sample + count + offsets + main + head merge + compact"),
+                                    "gb_main_staged_kernel", "gb_head_merge_kernel", "gb_compact_kernel",
+                                    "gb_compact_piece_kernel", "gb_compact_merge_kernel"],
+                        "group-by consume: sample + count + offsets + main + head merge + compact (pieces + merge)"),
     "shuffle_scatter": ("shuffle", ["shuffle_scatter_kernel"], "hash shuffle scatter pass"),
 }
 
