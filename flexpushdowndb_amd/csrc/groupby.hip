@@ -562,7 +562,9 @@ __global__ __launch_bounds__(1024) void gb_sample_gather_kernel(unsigned long lo
                                                                 uint32_t* __restrict__ l_cnt,
                                                                 uint32_t* __restrict__ l_hist) {
   __shared__ uint32_t s_hist[kCountBins];
+  __shared__ uint32_t s_wcnt[16], s_base;
   const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   for (int b = threadIdx.x; b < kCountBins; b += blockDim.x) s_hist[b] = 0;
   __syncthreads();
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;   // grid covers the slots, plus one for the reserved
@@ -574,18 +576,23 @@ __global__ __launch_bounds__(1024) void gb_sample_gather_kernel(unsigned long lo
     if (s < (uint32_t)kSampleTable) g_keys[s] = kEmptyKey;
   }
   if (c) atomicAdd(&s_hist[min(c, (uint32_t)(kCountBins - 1))], 1u);
+  // list positions: one atomic per workgroup on the list length (kept after
+  // the bins), the waves' entries in wave order after it
   const uint64_t m = __ballot(c != 0);
-  if (m) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&l_hist[kCountBins], (uint32_t)__popcll(m));   // list length after the bins
-    base = __builtin_amdgcn_readfirstlane(base);
-    if (c) {
-      const uint32_t at = base + rank_below(m);
-      l_keys[at] = key;
-      l_cnt[at] = c;
-    }
+  if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_wcnt[w];
+    s_base = t ? atomicAdd(&l_hist[kCountBins], t) : 0u;
   }
   __syncthreads();
+  if (c) {
+    uint32_t at = s_base + rank_below(m);
+    for (int w = 0; w < wave; ++w) at += s_wcnt[w];
+    l_keys[at] = key;
+    l_cnt[at] = c;
+  }
   for (int b = threadIdx.x; b < kCountBins; b += blockDim.x)
     if (s_hist[b]) atomicAdd(&l_hist[b], s_hist[b]);
 }
