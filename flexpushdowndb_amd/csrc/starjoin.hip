@@ -139,28 +139,37 @@ __device__ __forceinline__ bool bf_find(const uint64_t* blocks, int lnb, const u
 // word.  Once the error word is set every workgroup stops at its next barrier
 // (returns false, and the kernel returns), so a missed barrier ends the launch
 // within about one bound, with FPDB_ERR_INTERNAL reported by the result call.
-// Only thread 0 fences: __syncthreads orders the workgroup's stores before its
-// agent-scope release (an L2 write-back on this part), and its acquire before
-// the workgroup's later loads.
+// Only thread 0 fences, once each way: __syncthreads orders the workgroup's
+// stores before its agent-scope release fence (an L2 write-back on this part)
+// and its acquire fence (an L2 invalidate) before the workgroup's later loads.
+// The barrier words themselves are relaxed atomics (they go past the L2s, and
+// a fence per access would write back or invalidate the L2 each time: with
+// acquire/release on every access a barrier of 512 workgroups took ~39 us,
+// with one fence each way ~13 us).  Each arrival is a relaxed increment after
+// the release fence; the last arriver acquires (so every workgroup's stores
+// happen before what follows), resets the count and bumps the generation with
+// a release; the others poll the generation and then acquire.  The count and
+// the generation sit in different 128-byte lines.
 __device__ __forceinline__ bool star_barrier(const StarDev& P) {
   unsigned long long* count = P.state + 16;
-  unsigned long long* gen = P.state + 17;
+  unsigned long long* gen = P.state + 48;
   unsigned long long* err = P.state + 32;
   __shared__ int s_ok;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();    // the workgroup's writes of this phase become visible device-wide
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     int ok = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     if (ok) {
-      const unsigned long long g = __hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       const unsigned long long arrived =
-          __hip_atomic_fetch_add(count, 1ULL, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1;
+          __hip_atomic_fetch_add(count, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
       if (arrived == gridDim.x) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __hip_atomic_store(count, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_add(gen, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       } else {
         const uint64_t t0 = wall_clock64();
-        while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
           __builtin_amdgcn_s_sleep(2);
           if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
             ok = 0;   // another workgroup gave up
@@ -174,7 +183,7 @@ __device__ __forceinline__ bool star_barrier(const StarDev& P) {
         }
       }
     }
-    __threadfence();    // and the workgroup observes the other workgroups' writes
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // the workgroup observes the other workgroups' writes
     s_ok = ok;
   }
   __syncthreads();
