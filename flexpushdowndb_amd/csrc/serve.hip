@@ -75,12 +75,6 @@ struct ServeBatch {
   int32_t ndims;
   int32_t active;                  // slots in this tick
   int32_t total_groups;
-  // the last workgroup of the tick stores tick_seq into *tick_flag (pinned
-  // host memory): done_counter counts finished workgroups over all ticks
-  unsigned long long* done_counter;
-  uint64_t done_target;            // tick_seq * workgroups
-  uint64_t* tick_flag;
-  uint64_t tick_seq;
 };
 
 // The finalize of one query, writing into pinned host memory.
@@ -224,16 +218,15 @@ __global__ __launch_bounds__(kServeThreads) void serve_scan_kernel(ServeBatch B)
     const uint64_t f = (uint64_t)begin + lfirst[e] + 1;
     if (p[2] == 0 || f < p[2]) p[2] = f;
   }
-  // the last workgroup to finish tells the host the tick has run (pacing only)
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned long long done = atomicAdd(B.done_counter, 1ULL) + 1;
-    if (done == B.done_target) {
-      __threadfence_system();
-      __hip_atomic_store(B.tick_flag, B.tick_seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-  }
+}
+
+// Tells the host that the scan stream has run tick `seq` (pacing only: the
+// queries' results travel through their own ready words).  A one-thread
+// launch after the scan: the scan's workgroups then need no shared
+// completion counter, whose increments queue on one address as they finish.
+__global__ void serve_tick_kernel(uint64_t* tick_flag, uint64_t seq) {
+  __threadfence_system();
+  __hip_atomic_store(tick_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // A query's partials added up per group: 64 groups per workgroup, each wave
@@ -390,7 +383,6 @@ struct fpdb_serve {
   std::vector<fpdb::ServeHostBlock> host_free, host_all;
   uint64_t* tick_flag = nullptr;          // pinned: last tick the scan stream has run
   uint64_t* tick_flag_dev = nullptr;
-  unsigned long long* done_counter = nullptr;   // workgroups of all ticks so far
   int32_t* fk32[fpdb::kServeMaxDims] = {};      // int32 copies of the fact foreign keys
   uint64_t ticks = 0;
   uint64_t workspace_used = 0;
@@ -406,7 +398,6 @@ static void serve_free(fpdb_serve* s) {
     if (s->dims[d].qmask) ctx_free(c, s->dims[d].qmask);
     if (s->fk32[d]) ctx_free(c, s->fk32[d]);
   }
-  if (s->done_counter) ctx_free(c, s->done_counter);
   for (auto& R : s->queries)
     if (R.partial) ctx_free(c, R.partial);
   for (auto& h : s->host_all) (void)hipHostFree(h.host);
@@ -577,8 +568,6 @@ static fpdb_status serve_step_impl(fpdb_serve* s, int* completed) {
   if (s->ticks >= (uint64_t)kServeAheadTicks) FPDB_TRY(wait_tick(s, s->ticks + 1 - kServeAheadTicks));
   ServeBatch B;
   std::memset(&B, 0, sizeof(B));
-  B.done_counter = s->done_counter;
-  B.tick_flag = s->tick_flag_dev;
   bool scan = false, drain = false;
   std::vector<int64_t> finishing;
   {
@@ -622,13 +611,12 @@ static fpdb_status serve_step_impl(fpdb_serve* s, int* completed) {
   }
   if (scan) {
     ++s->ticks;
-    B.tick_seq = s->ticks;
-    B.done_target = s->ticks * (uint64_t)s->blocks;
     {
       ProfScope ps(c, "serve_scan");
       hipLaunchKernelGGL(serve_scan_kernel, dim3(s->blocks), dim3(kServeThreads), (size_t)B.total_groups * 16,
                          c->stream, B);
     }
+    hipLaunchKernelGGL(serve_tick_kernel, dim3(1), dim3(1), 0, c->stream, s->tick_flag_dev, s->ticks);
     FPDB_HIP(hipGetLastError());
 
     if (!finishing.empty()) {
@@ -729,8 +717,6 @@ fpdb_status fpdb_serve_create(fpdb_context* ctx, const fpdb_serve_schema* schema
   int* d_bad = nullptr;
   if (st == FPDB_OK) st = ctx_alloc(c, 8, reinterpret_cast<void**>(&d_bad));
   if (st == FPDB_OK) st = launch_fill(c, d_bad, 0, 8);
-  if (st == FPDB_OK) st = ctx_alloc(c, 8, reinterpret_cast<void**>(&s->done_counter));
-  if (st == FPDB_OK) st = launch_fill(c, s->done_counter, 0, 8);
   for (int d = 0; d < s->ndims && st == FPDB_OK; ++d) {
     const fpdb_serve_dimension& D = schema->dims[d];
     ServeDim& X = s->dims[d];
