@@ -50,6 +50,7 @@ struct StarDimDev {
   int32_t ht_shift;          // hash-table capacity >= kept rows << ht_shift
   uint8_t* kept;             // rows bytes
   uint64_t* fbf;             // forward filter blocks (capacity for `rows` keys)
+  int32_t fbf_lnb;           // log2 of that capacity in blocks (the init kernel zeroes all of it)
   uint64_t* bbf;             // backward filter blocks (capacity for all fact rows)
   StarSlot* ht;              // hash table storage (ht_mask + 1 slots); a run uses the
   uint64_t ht_mask;          // first star_ht_mask(...) + 1 of them
@@ -308,23 +309,17 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     __syncthreads();
   }
   if (!star_barrier(P)) return;
-  // A2: zero the forward filters at their sizes; clear the used part of the
-  // hash tables
-  for (int d = 0; d < P.ndims; ++d) {
-    const StarDimDev& D = P.dim[d];
-    int lnb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    for (int64_t b = gtid; b < (int64_t(1) << lnb); b += gthreads) D.fbf[b] = 0;
-    const uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-    const int64_t hm = (int64_t)star_ht_mask(D, P.state, d);
-    for (int64_t s = gtid; s <= hm; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
-  }
-  if (!star_barrier(P)) return;
-  // A3: insert the kept keys into the forward filters
+  // A2: insert the kept keys into the forward filters (the init kernel zeroed
+  // them at capacity, so no barrier separates a clear from the inserts), and
+  // clear the used part of the hash tables (first used in phase C)
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
     int lnb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     for (int64_t r = gtid; r < D.rows; r += gthreads)
       if (D.kept[r]) bf_insert(D.fbf, lnb, s_masks, D.key[r]);
+    const uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    const int64_t hm = (int64_t)star_ht_mask(D, P.state, d);
+    for (int64_t s = gtid; s <= hm; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
   }
   if (!star_barrier(P)) return;
   // B: forward pass over every fact row (all partitions), filters in `order`.
@@ -452,6 +447,12 @@ __global__ void star_init_kernel(StarDev P) {
     P.g_cnt[s] = 0;
     P.g_sum[s] = 0;
     P.g_first[s] = LLONG_MAX;
+  }
+  // the forward filters, whole (a run uses a prefix sized by its kept rows)
+  for (int d = 0; d < P.ndims; ++d) {
+    uint64_t* f = P.dim[d].fbf;
+    const int64_t words = int64_t(1) << P.dim[d].fbf_lnb;
+    for (int64_t b = gtid; b < words; b += gthreads) f[b] = 0;
   }
 }
 
@@ -828,7 +829,7 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
   // register use
   FPDB_CHECK(s->grid >= 1 && s->grid <= s->resident_per_cu * std::max(1, c->num_cus), FPDB_ERR_INTERNAL,
              "star join: persistent grid exceeds the resident workgroups");
-  hipLaunchKernelGGL(star_init_kernel, dim3(64), dim3(256), 0, c->stream, s->dev);
+  hipLaunchKernelGGL(star_init_kernel, dim3(256), dim3(256), 0, c->stream, s->dev);
   FPDB_HIP(hipGetLastError());
   {
     ProfScope ps(c, "star_persistent");
@@ -966,7 +967,8 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
     X.ht_mask = (uint64_t)cap - 1;
     X.ht_shift = s->tune.ht_load_shift;
     st = star_alloc(s, (uint64_t)std::max<int64_t>(X.rows, 1), reinterpret_cast<void**>(&X.kept));
-    if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << host_log_blocks(X.rows), reinterpret_cast<void**>(&X.fbf));
+    X.fbf_lnb = host_log_blocks(X.rows);
+    if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << X.fbf_lnb, reinterpret_cast<void**>(&X.fbf));
     if (st == FPDB_OK) st = star_alloc(s, (uint64_t)8 << blnb, reinterpret_cast<void**>(&X.bbf));
     if (st == FPDB_OK) st = star_alloc(s, (uint64_t)cap * sizeof(StarSlot), reinterpret_cast<void**>(&X.ht));
   }
