@@ -440,6 +440,7 @@ struct DenseMeta {
 };
 
 constexpr int kMinMaxThreads = 1024;
+constexpr int kBuildUnroll = 4;   // rows a thread handles per trip in the dense build passes
 
 // Per-workgroup {min, max, count} of the non-null build keys into partials
 // (3 words per workgroup), and every slot of the capacity cleared (build row
@@ -460,12 +461,23 @@ __global__ __launch_bounds__(kMinMaxThreads) void hj_minmax_kernel(KeySpec k, in
   }
   if (gtid == 0) meta->dup = 0;
   int64_t lo = INT64_MAX, hi = INT64_MIN, cnt = 0;
-  for (int64_t r = (int64_t)gtid; r < n; r += (int64_t)gthreads) {
-    int64_t key;
-    if (!get_key<KM>(k, r, &key)) continue;
-    lo = key < lo ? key : lo;
-    hi = key > hi ? key : hi;
-    ++cnt;
+  // kBuildUnroll rows a thread per trip, their keys loaded together
+  for (int64_t r0 = (int64_t)gtid; r0 < n; r0 += (int64_t)gthreads * kBuildUnroll) {
+    int64_t key[kBuildUnroll];
+    bool ok[kBuildUnroll];
+#pragma unroll
+    for (int j = 0; j < kBuildUnroll; ++j) {
+      const int64_t r = r0 + (int64_t)j * (int64_t)gthreads;
+      key[j] = 0;
+      ok[j] = get_key<KM>(k, r < n ? r : n - 1, &key[j]) && r < n;
+    }
+#pragma unroll
+    for (int j = 0; j < kBuildUnroll; ++j) {
+      if (!ok[j]) continue;
+      lo = key[j] < lo ? key[j] : lo;
+      hi = key[j] > hi ? key[j] : hi;
+      ++cnt;
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     int64_t l2 = __shfl_xor(lo, off), h2 = __shfl_xor(hi, off);
@@ -542,28 +554,44 @@ __global__ __launch_bounds__(256) void hj_dense_scatter_kernel(uint32_t* __restr
   if (dr.span == 0) return;
   const uint64_t kmin = (uint64_t)dr.kmin;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
-    int64_t key;
-    if (!get_key<KM>(k, r, &key)) continue;
-    const uint64_t slot = (uint64_t)key - kmin;
-    if (pairs)
-      pairs[slot] = make_uint2((uint32_t)r, reinterpret_cast<const uint32_t*>(payload)[r + payload_off]);
-    else
-      rows[slot] = (uint32_t)r;
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r0 < n; r0 += stride * kBuildUnroll) {
+    int64_t key[kBuildUnroll];
+    bool ok[kBuildUnroll];
+    uint32_t pv[kBuildUnroll];
+#pragma unroll
+    for (int j = 0; j < kBuildUnroll; ++j) {
+      const int64_t r = r0 + (int64_t)j * stride;
+      const int64_t rc = r < n ? r : n - 1;
+      key[j] = 0;
+      ok[j] = get_key<KM>(k, rc, &key[j]) && r < n;
+      pv[j] = pairs ? reinterpret_cast<const uint32_t*>(payload)[rc + payload_off] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kBuildUnroll; ++j) {
+      if (!ok[j]) continue;
+      const int64_t r = r0 + (int64_t)j * stride;
+      const uint64_t slot = (uint64_t)key[j] - kmin;
+      if (pairs)
+        pairs[slot] = make_uint2((uint32_t)r, pv[j]);
+      else
+        rows[slot] = (uint32_t)r;
+    }
   }
 }
 
 // After the scatter, in one launch: the slots split into the row and payload
 // arrays (when the scatter wrote pairs), the presence bitmap (bit s of word
-// s / 64 set iff slot s holds a key), and the duplicate check (a duplicate
-// key leaves one of its rows not finding itself in its slot).  Workgroup 0
-// stores the range into meta for the host's one read-back.
+// s / 64 set iff slot s holds a key), and per workgroup the number of filled
+// slots, into filled[blockIdx.x].  Keys are distinct iff the filled slots
+// number the keys (two rows of one key land in one slot), which
+// hj_dense_decide_kernel checks; this needs no random read per build row.
+// Workgroup 0 stores the range into meta for the host's one read-back.
 template <int KM>
 __global__ __launch_bounds__(256) void hj_dense_finish_kernel(const uint2* __restrict__ pairs, uint64_t cap,
                                                               const int64_t* __restrict__ partials, int parts,
                                                               DenseMeta* meta, uint32_t* __restrict__ rows,
                                                               uint32_t* __restrict__ pay, uint64_t* __restrict__ bits,
-                                                              KeySpec k, int64_t n) {
+                                                              int64_t* __restrict__ filled) {
   const DenseRange dr = dense_range(partials, parts, cap);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     meta->kmin = dr.kmin;
@@ -571,39 +599,62 @@ __global__ __launch_bounds__(256) void hj_dense_finish_kernel(const uint2* __res
     meta->count = dr.count;
   }
   const uint64_t span = dr.span;
-  if (span == 0) return;
+  __shared__ unsigned long long s_fill;
+  if (threadIdx.x == 0) s_fill = 0;
+  __syncthreads();
+  if (span == 0) {
+    if (threadIdx.x == 0) filled[blockIdx.x] = 0;
+    return;
+  }
   const int lane = threadIdx.x & 63;
   const uint64_t gthreads = (uint64_t)gridDim.x * blockDim.x;
   const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // slots, a wave per 64 consecutive slots
   const uint64_t words = (span + 63) >> 6;
-  for (uint64_t w = gtid >> 6; w < words; w += gthreads >> 6) {
-    const uint64_t s = w * 64 + lane;
-    const uint64_t sc = s < span ? s : 0;
-    uint32_t row;
-    if (pairs) {
-      const uint2 e = pairs[sc];
-      row = e.x;
-      if (s < span) {
-        rows[s] = e.x;
-        pay[s] = e.y;
-      }
-    } else {
-      row = rows[sc];
+  const uint64_t wstride = gthreads >> 6;
+  for (uint64_t w0 = gtid >> 6; w0 < words; w0 += wstride * kBuildUnroll) {
+    uint2 e[kBuildUnroll];
+#pragma unroll
+    for (int j = 0; j < kBuildUnroll; ++j) {
+      const uint64_t s = (w0 + j * wstride) * 64 + lane;
+      const uint64_t sc = s < span ? s : 0;
+      if (pairs) e[j] = pairs[sc];
+      else e[j] = make_uint2(rows[sc], 0u);
     }
-    const uint64_t word = __ballot(s < span && row != kEmptyRow);
-    if (lane == 0) bits[w] = word;
+#pragma unroll
+    for (int j = 0; j < kBuildUnroll; ++j) {
+      const uint64_t w = w0 + j * wstride;
+      const uint64_t s = w * 64 + lane;
+      if (pairs && s < span) {
+        rows[s] = e[j].x;
+        pay[s] = e[j].y;
+      }
+      const uint64_t word = __ballot(s < span && e[j].x != kEmptyRow);
+      if (lane == 0 && w < words) {
+        bits[w] = word;
+        atomicAdd(&s_fill, (unsigned long long)__popcll(word));
+      }
+    }
   }
-  // build rows
-  const uint64_t kmin = (uint64_t)dr.kmin;
-  bool dup = false;
-  for (int64_t r = (int64_t)gtid; r < n; r += (int64_t)gthreads) {
-    int64_t key;
-    if (!get_key<KM>(k, r, &key)) continue;
-    const uint64_t slot = (uint64_t)key - kmin;
-    dup |= (pairs ? pairs[slot].x : rows[slot]) != (uint32_t)r;
+  __syncthreads();
+  if (threadIdx.x == 0) filled[blockIdx.x] = (int64_t)s_fill;
+}
+
+// One workgroup: the filled slots over the finish workgroups against the key
+// count; meta->dup = 1 when they differ (a key occurs twice).
+__global__ __launch_bounds__(1024) void hj_dense_decide_kernel(const int64_t* __restrict__ filled, int parts,
+                                                               DenseMeta* meta) {
+  __shared__ int64_t s_w[16];
+  int64_t v = 0;
+  for (int b = threadIdx.x; b < parts; b += blockDim.x) v += filled[b];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w];
+    meta->dup = t != meta->count ? 1 : 0;
   }
-  if (__ballot(dup) && lane == 0) meta->dup = 1;   // every writer stores 1
 }
 
 // Dense probe, pass 1: per probe row whether its key is present (the hit
@@ -1215,8 +1266,10 @@ static fpdb_status try_dense_build(Context* c, fpdb_hash_table* t, const KeySpec
   const int gm = grid_for(c, std::max<int64_t>(n, (int64_t)cap), kMinMaxThreads, 1);
   const int g = grid_for(c, n, 256, 4);
   const int gf = grid_for(c, (int64_t)cap, 256, 4);
-  int64_t* parts = reinterpret_cast<int64_t*>(c->d_partials);   // 3 words per workgroup
-  FPDB_CHECK(3 * gm <= kMaxPartials, FPDB_ERR_INTERNAL, "hash join dense build: too many minmax workgroups");
+  const int gfin = std::max(g, gf);
+  // 3 words per minmax workgroup, then one filled-slot count per finish workgroup
+  int64_t* parts = reinterpret_cast<int64_t*>(c->d_partials);
+  FPDB_CHECK(3 * gm + gfin <= kMaxPartials, FPDB_ERR_INTERNAL, "hash join dense build: too many workgroups");
   {
     ProfScope ps(c, "hj_build");
     switch (key_mode_of(k)) {
@@ -1225,8 +1278,9 @@ static fpdb_status try_dense_build(Context* c, fpdb_hash_table* t, const KeySpec
                      pairs, cap, meta);                                                                        \
   hipLaunchKernelGGL(hj_dense_scatter_kernel<KM>, dim3(g), dim3(256), 0, c->stream, arr, pairs, cap, parts, gm, \
                      k, n, pv, poff);                                                                          \
-  hipLaunchKernelGGL(hj_dense_finish_kernel<KM>, dim3(std::max(g, gf)), dim3(256), 0, c->stream, pairs, cap,     \
-                     parts, gm, meta, arr, pay, bits, k, n)
+  hipLaunchKernelGGL(hj_dense_finish_kernel<KM>, dim3(gfin), dim3(256), 0, c->stream, pairs, cap, parts, gm,   \
+                     meta, arr, pay, bits, parts + 3 * gm);                                                    \
+  hipLaunchKernelGGL(hj_dense_decide_kernel, dim3(1), dim3(1024), 0, c->stream, parts + 3 * gm, gfin, meta)
       case 1: FPDB_DENSE_BUILD(1); break;
       case 2: FPDB_DENSE_BUILD(2); break;
       default: FPDB_DENSE_BUILD(0); break;
