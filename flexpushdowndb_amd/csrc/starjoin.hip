@@ -357,13 +357,29 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     if (S.rows == 0) continue;
     const int64_t rounded = (S.rows + step - 1) / step * step;   // whole waves iterate together
     const int64_t last = S.rows - 1;
+    // The first filter's keys of a step are loaded during the step before,
+    // issued after that step's second-filter loads: the wait for those then
+    // leaves the next keys in flight (loads complete in issue order), so a
+    // step waits for memory about once.
+    const int d1 = P.ndims > 1 ? P.order[1] : d0;
+    const int64_t* col0 = S.fk[d0];
+    const int64_t* col1 = S.fk[d1];
+    int64_t nk[ITEMS];
+    {
+      const int64_t q0 = (int64_t)blockIdx.x * blockDim.x * ITEMS + threadIdx.x;
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const int64_t r = q0 + (int64_t)j * blockDim.x;
+        nk[j] = __builtin_nontemporal_load(col0 + (r <= last ? r : last));   // streamed once
+      }
+    }
     for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * ITEMS + threadIdx.x; r0 < rounded; r0 += step) {
       int64_t rc[ITEMS], fk[ITEMS];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         int64_t r = r0 + (int64_t)j * blockDim.x;
         rc[j] = r <= last ? r : last;
-        fk[j] = __builtin_nontemporal_load(S.fk[d0] + rc[j]);   // streamed once
+        fk[j] = nk[j];
       }
       bool pass[ITEMS];
 #pragma unroll
@@ -373,10 +389,23 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
                                  : bf_find(P.dim[d0].fbf, lnb[d0], s_masks, fk[j]);
         pass[j] = r <= last && hit;
       }
-      // later filters: rows still passing read their key, the others all read
-      // row 0 (one line), so the loads of all items are issued together with
-      // no branch; a wave with no row left stops
-      for (int i = 1; i < P.ndims; ++i) {
+      // second filter, for every step (rows no longer passing read row 0:
+      // one line, no branch); with one dimension it repeats the first
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) fk[j] = col1[pass[j] ? rc[j] : 0];
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const int64_t r = r0 + step + (int64_t)j * blockDim.x;
+        nk[j] = __builtin_nontemporal_load(col0 + (r <= last ? r : last));
+      }
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const bool hit = loff[d1] >= 0 ? bf_find(s_filt + loff[d1], lnb[d1], s_masks, fk[j])
+                                       : bf_find(P.dim[d1].fbf, lnb[d1], s_masks, fk[j]);
+        pass[j] = pass[j] & hit;
+      }
+      // later filters: as the second, and a wave with no row left stops
+      for (int i = 2; i < P.ndims; ++i) {
         bool any = false;
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) any |= pass[j];
