@@ -1572,15 +1572,16 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_piece_kernel(
         if (i0 + (int64_t)j * blockDim.x < r1) apply(w[j]);
     }
     // raw tail rows (words loaded by their runtime position, not picked from
-    // a register array, which would put the array in scratch memory)
+    // a register array, which would put the array in scratch memory), RT rows
+    // a thread per trip, the next trip's rows loaded before this trip's are
+    // aggregated (two trips' rows in registers)
     const int64_t j0 = s0 - bs - nrec > 0 ? s0 - bs - nrec : 0, j1 = s1 - bs - nrec;
     const uint64_t* tails = tpool + (size_t)tail_start[b] * rr;
-    for (int64_t t0 = j0; t0 < j1; t0 += (int64_t)R * blockDim.x) {
-      const int64_t i0 = t0 + threadIdx.x;
-      uint64_t w[R][RW];
+    constexpr int RT = 2;
+    auto load_trip = [&](int64_t t0, uint64_t (&w)[RT][RW]) {
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        int64_t i = i0 + (int64_t)j * blockDim.x;
+      for (int j = 0; j < RT; ++j) {
+        int64_t i = t0 + threadIdx.x + (int64_t)j * blockDim.x;
         i = i < j1 ? i : j1 - 1;
         const uint64_t* raw = tails + (size_t)i * rr;
         w[j][0] = raw[0];
@@ -1588,8 +1589,20 @@ __global__ __launch_bounds__(kGroupThreads) void gb_compact_piece_kernel(
 #pragma unroll
         for (int a = 0; a < NACC; ++a) w[j][2 + a] = vcol[a] < 0 ? 1ULL : raw[vcol[a]];
       }
+    };
+    uint64_t nw[RT][RW];
+    if (j0 < j1) load_trip(j0, nw);
+    for (int64_t t0 = j0; t0 < j1; t0 += (int64_t)RT * blockDim.x) {
+      const int64_t i0 = t0 + threadIdx.x;
+      uint64_t w[RT][RW];
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
+      for (int j = 0; j < RT; ++j)
+#pragma unroll
+        for (int x = 0; x < RW; ++x) w[j][x] = nw[j][x];
+      // past the last trip this reloads the current one (unused)
+      load_trip(t0 + (int64_t)RT * blockDim.x < j1 ? t0 + (int64_t)RT * blockDim.x : t0, nw);
+#pragma unroll
+      for (int j = 0; j < RT; ++j) {
         if (i0 + (int64_t)j * blockDim.x >= j1) continue;
         if (rk == 2) {
           w[j][1] = (uint64_t)(row_base + (int64_t)(uint32_t)w[j][1]);
