@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     // segment by segment (one for a contiguous fact table, else one per
     // partition), rows numbered within the segment
     for (int sg = 0; sg < P.nsegs; ++sg) {
-    const StarSeg S = P.segs[sg];
+    const StarSeg& S = P.segs[sg];   // read in place: a copy indexed by runtime dimension would live in scratch
     if (S.rows == 0) continue;
     const int64_t rounded = (S.rows + step - 1) / step * step;   // whole waves iterate together
     const int64_t last = S.rows - 1;
