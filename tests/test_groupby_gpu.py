@@ -75,6 +75,38 @@ def test_skewed_layouts_vs_oracle(ctx, layout):
     _check(key, {"sum_i": [a[1], a[2]], "sum_f": [a[3]]}, k, a[0], [qty, price], [disc])
 
 
+def test_distinct_wide_keys_and_empty_batches(ctx):
+    """Every key distinct and spread over the whole int64 range (the sample
+    finds no head key worth its slot, so nearly every row takes the tail
+    path), consumed as an empty batch, two batches and another empty batch;
+    then a second table whose only batch is empty has no groups."""
+    rng = np.random.default_rng(31)
+    n = 1_000_003
+    keys = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n, dtype=np.int64)
+    assert len(np.unique(keys)) == n
+    qty = rng.integers(-10 ** 6, 10 ** 6, n).astype(np.int64)
+    price = rng.integers(0, 10 ** 9, n).astype(np.int64)
+    disc = rng.standard_normal(n)
+    g = ops.GroupBy(ctx, [C.TYPE_INT64],
+                    [(C.AGG_COUNT_ALL, None, False), (C.AGG_SUM, C.TYPE_INT64, False),
+                     (C.AGG_SUM, C.TYPE_INT64, False), (C.AGG_SUM, C.TYPE_FLOAT64, False)], n + 16)
+    for lo, hi in [(0, 0), (0, 400_000), (400_000, n), (n, n)]:
+        g.consume([DeviceColumn.from_numpy(ctx, keys[lo:hi])],
+                  [None, DeviceColumn.from_numpy(ctx, qty[lo:hi]), DeviceColumn.from_numpy(ctx, price[lo:hi]),
+                   DeviceColumn.from_numpy(ctx, disc[lo:hi])], hi - lo)
+    k, a = g.finalize()
+    k, a = k[0].numpy(), [x.numpy() for x in a]
+    assert np.array_equal(k, keys)          # all distinct: first appearance is row order
+    assert (a[0] == 1).all()
+    assert np.array_equal(a[1], qty) and np.array_equal(a[2], price) and np.array_equal(a[3], disc)
+    _check(keys.view(np.uint64), {"sum_i": [a[1], a[2]], "sum_f": [a[3]]}, k.view(np.uint64), a[0], [qty, price],
+           [disc])
+    e = ops.GroupBy(ctx, [C.TYPE_INT64], [(C.AGG_COUNT_ALL, None, False)], 8)
+    e.consume([DeviceColumn.from_numpy(ctx, keys[:0])], [None], 0)
+    ek, ea = e.finalize()
+    assert len(ek[0].numpy()) == 0 and len(ea[0].numpy()) == 0
+
+
 def test_attic_group_vector(ctx):
     # fpdb-executor/ATTIC/test/GroupTest.cpp:137-150: sum(ab) by aa -> 10: 27, 12: 15
     g = ops.GroupBy(ctx, [C.TYPE_INT64], [(C.AGG_SUM, C.TYPE_INT64, False)], 16)
