@@ -260,6 +260,15 @@ __device__ __forceinline__ void group_add(const StarDev& P, uint64_t gkey, int64
 
 // ---- phases shared by the persistent kernel and the plain path ---------------
 
+// The fact segments' column pointers are read from device memory (the
+// segment table), so the compiler cannot tell that they point to global
+// memory and would load through them with flat instructions.  Flat loads
+// also count against the LDS wait counter: each wait for a filter word or mask
+// in LDS would then wait for every key loaded ahead as well.  Loads through
+// this view are global loads.
+typedef const int64_t __attribute__((address_space(1))) GlobalI64;
+__device__ __forceinline__ GlobalI64* global_view(const int64_t* p) { return (GlobalI64*)p; }
+
 // Forward-pass survivors are appended with one atomic per wavefront.
 __device__ __forceinline__ void append_survivor(const StarDev& P, const StarSeg& S, bool pass, int64_t lr) {
   uint64_t mask = __ballot(pass);
@@ -275,8 +284,8 @@ __device__ __forceinline__ void append_survivor(const StarDev& P, const StarSeg&
     uint64_t below = lane ? (mask & (~0ULL >> (64 - lane))) : 0ULL;
     int64_t pos = (int64_t)(base + __popcll(below));
     P.s_idx[pos] = S.base + lr;
-    for (int d = 0; d < P.ndims; ++d) P.s_fk[(int64_t)d * P.fact_rows + pos] = S.fk[d][lr];
-    P.s_meas[pos] = S.meas[lr];
+    for (int d = 0; d < P.ndims; ++d) P.s_fk[(int64_t)d * P.fact_rows + pos] = global_view(S.fk[d])[lr];
+    P.s_meas[pos] = global_view(S.meas)[lr];
   }
 }
 
@@ -362,8 +371,8 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     // leaves the next keys in flight (loads complete in issue order), so a
     // step waits for memory about once.
     const int d1 = P.ndims > 1 ? P.order[1] : d0;
-    const int64_t* col0 = S.fk[d0];
-    const int64_t* col1 = S.fk[d1];
+    GlobalI64* col0 = global_view(S.fk[d0]);
+    GlobalI64* col1 = global_view(S.fk[d1]);
     int64_t nk[ITEMS];
     {
       const int64_t q0 = (int64_t)blockIdx.x * blockDim.x * ITEMS + threadIdx.x;
@@ -411,7 +420,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
         for (int j = 0; j < ITEMS; ++j) any |= pass[j];
         if (!__ballot(any)) break;
         const int d = P.order[i];
-        const int64_t* col = S.fk[d];
+        GlobalI64* col = global_view(S.fk[d]);
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) fk[j] = col[pass[j] ? rc[j] : 0];
 #pragma unroll
