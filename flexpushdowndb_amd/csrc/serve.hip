@@ -135,6 +135,12 @@ __global__ void serve_clear_kernel(uint64_t* p, int64_t words) {
     p[i] = 0;
 }
 
+// Group attribute columns are read through pointers kept in LDS; as such the
+// compiler cannot tell that they point to global memory and would load with
+// flat instructions, whose waits also hold up LDS accesses.  This view makes
+// them global loads.
+typedef const int32_t __attribute__((address_space(1))) GlobalI32;
+
 __global__ __launch_bounds__(kServeThreads) void serve_scan_kernel(ServeBatch B) {
   extern __shared__ uint64_t lds[];
   // per-query descriptors indexed by a per-lane slot number live in LDS
@@ -188,13 +194,13 @@ __global__ __launch_bounds__(kServeThreads) void serve_scan_kernel(ServeBatch B)
         const int d0 = s_gdim0[q];
         int32_t f0 = 0;
         for (int d = 0; d < kServeMaxDims; ++d) if (d == d0) f0 = fkv[d];   // fkv stays in registers
-        g = s_gattr[d0][f0];
+        g = ((GlobalI32*)s_gattr[d0])[f0];
       }
       if (ng == 2) {
         const int d1 = s_gdim1[q];
         int32_t f1 = 0;
         for (int d = 0; d < kServeMaxDims; ++d) if (d == d1) f1 = fkv[d];
-        g = g * s_dom1[q] + s_gattr[d1][f1];
+        g = g * s_dom1[q] + ((GlobalI32*)s_gattr[d1])[f1];
       }
       const int e = s_off[q] + g;
       atomicAdd(&lsum[e], meas);
