@@ -269,6 +269,14 @@ __device__ __forceinline__ void group_add(const StarDev& P, uint64_t gkey, int64
 typedef const int64_t __attribute__((address_space(1))) GlobalI64;
 __device__ __forceinline__ GlobalI64* global_view(const int64_t* p) { return (GlobalI64*)p; }
 
+// Orders one wave's LDS writes before its other lanes' reads (and reads
+// before later writes) of the same words.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Forward-pass survivors are appended with one atomic per wavefront.
 __device__ __forceinline__ void append_survivor(const StarDev& P, const StarSeg& S, bool pass, int64_t lr) {
   uint64_t mask = __ballot(pass);
@@ -292,6 +300,10 @@ __device__ __forceinline__ void append_survivor(const StarDev& P, const StarSeg&
 template <int ITEMS>
 __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P) {
   __shared__ uint64_t s_masks[kMaskNum];
+  // per wave: the rows of one step that pass the first filter, on top of
+  // fewer than 64 left from the steps before
+  constexpr int kStackCap = 64 * (ITEMS + 1);
+  __shared__ uint32_t s_stack[(kStarThreads / 64) * kStackCap];
   for (int i = threadIdx.x; i < kMaskNum; i += blockDim.x) s_masks[i] = c_masks.m[i];
   __syncthreads();
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,7 +346,8 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
   // B: forward pass over every fact row (all partitions), filters in `order`.
   // The filters that fit (in that order) are copied into LDS; each thread
   // takes ITEMS rows per step with the first filter's keys loaded
-  // together, and only rows still passing read the next filter's keys.
+  // together, and only rows passing the first filter read the next filters'
+  // keys, a wave of such rows at a time.
   {
     // per-dimension filter size and LDS offset (-1: read from global memory),
     // in LDS: they are indexed by runtime dimension numbers
@@ -358,7 +371,41 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     }
     __syncthreads();
     const int d0 = P.order[0];
+    const int d1 = P.ndims > 1 ? P.order[1] : d0;
+    // the first two filters' geometry, wave-uniform, in registers
+    const int lnb0 = __builtin_amdgcn_readfirstlane(lnb[d0]);
+    const int lnb1 = __builtin_amdgcn_readfirstlane(lnb[d1]);
+    const bool lds0 = loff[d0] >= 0, lds1 = loff[d1] >= 0;
+    const uint64_t* lf0 = s_filt + (lds0 ? loff[d0] : 0);
+    const uint64_t* lf1 = s_filt + (lds1 ? loff[d1] : 0);
+    const uint64_t* gf0 = P.dim[d0].fbf;
+    const uint64_t* gf1 = P.dim[d1].fbf;
     const int64_t step = gthreads * ITEMS;
+    // Rows that pass the first filter are pushed onto a per-wave stack in
+    // LDS; whole waves of them (64 rows) are popped and run through the other
+    // filters, so those filters cost one test per passing row instead of one
+    // per row.  Survivor order does not matter: a group's first row is the
+    // minimum of its rows' global numbers.
+    const int lane = threadIdx.x & 63;
+    uint32_t* stack = s_stack + (threadIdx.x >> 6) * kStackCap;
+    // the filters after the first, for one row a lane (a wave at a time)
+    auto rest = [&](const StarSeg& S, bool live, uint32_t lr) {
+      bool pass = live;
+      if (P.ndims > 1) {
+        const int64_t k = global_view(S.fk[d1])[live ? lr : 0u];
+        const bool hit = lds1 ? bf_find(lf1, lnb1, s_masks, k) : bf_find(gf1, lnb1, s_masks, k);
+        pass = pass && hit;
+      }
+      for (int i = 2; i < P.ndims; ++i) {
+        if (!__ballot(pass)) break;   // no row left in the wave
+        const int d = P.order[i];
+        const int64_t k = global_view(S.fk[d])[pass ? lr : 0u];
+        const bool hit = loff[d] >= 0 ? bf_find(s_filt + loff[d], lnb[d], s_masks, k)
+                                      : bf_find(P.dim[d].fbf, lnb[d], s_masks, k);
+        pass = pass && hit;
+      }
+      append_survivor(P, S, pass, (int64_t)lr);
+    };
     // segment by segment (one for a contiguous fact table, else one per
     // partition), rows numbered within the segment
     for (int sg = 0; sg < P.nsegs; ++sg) {
@@ -366,13 +413,8 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     if (S.rows == 0) continue;
     const int64_t rounded = (S.rows + step - 1) / step * step;   // whole waves iterate together
     const int64_t last = S.rows - 1;
-    // The first filter's keys of a step are loaded during the step before,
-    // issued after that step's second-filter loads: the wait for those then
-    // leaves the next keys in flight (loads complete in issue order), so a
-    // step waits for memory about once.
-    const int d1 = P.ndims > 1 ? P.order[1] : d0;
     GlobalI64* col0 = global_view(S.fk[d0]);
-    GlobalI64* col1 = global_view(S.fk[d1]);
+    // each step's first-filter keys are loaded during the step before
     int64_t nk[ITEMS];
     {
       const int64_t q0 = (int64_t)blockIdx.x * blockDim.x * ITEMS + threadIdx.x;
@@ -382,26 +424,11 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
         nk[j] = __builtin_nontemporal_load(col0 + (r <= last ? r : last));   // streamed once
       }
     }
+    int qn = 0;   // rows on this wave's stack (wave-uniform, < 64 between steps)
     for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x * ITEMS + threadIdx.x; r0 < rounded; r0 += step) {
-      int64_t rc[ITEMS], fk[ITEMS];
+      int64_t fk[ITEMS];
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        int64_t r = r0 + (int64_t)j * blockDim.x;
-        rc[j] = r <= last ? r : last;
-        fk[j] = nk[j];
-      }
-      bool pass[ITEMS];
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        int64_t r = r0 + (int64_t)j * blockDim.x;
-        bool hit = loff[d0] >= 0 ? bf_find(s_filt + loff[d0], lnb[d0], s_masks, fk[j])
-                                 : bf_find(P.dim[d0].fbf, lnb[d0], s_masks, fk[j]);
-        pass[j] = r <= last && hit;
-      }
-      // second filter, for every step (rows no longer passing read row 0:
-      // one line, no branch); with one dimension it repeats the first
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) fk[j] = col1[pass[j] ? rc[j] : 0];
+      for (int j = 0; j < ITEMS; ++j) fk[j] = nk[j];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         const int64_t r = r0 + step + (int64_t)j * blockDim.x;
@@ -409,29 +436,29 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
       }
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
-        const bool hit = loff[d1] >= 0 ? bf_find(s_filt + loff[d1], lnb[d1], s_masks, fk[j])
-                                       : bf_find(P.dim[d1].fbf, lnb[d1], s_masks, fk[j]);
-        pass[j] = pass[j] & hit;
+        const int64_t r = r0 + (int64_t)j * blockDim.x;
+        const bool hit = lds0 ? bf_find(lf0, lnb0, s_masks, fk[j]) : bf_find(gf0, lnb0, s_masks, fk[j]);
+        const bool pass = r <= last && hit;
+        const uint64_t m = __ballot(pass);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (pass) stack[qn + (int)below] = (uint32_t)r;
+        qn += __popcll(m);
       }
-      // later filters: as the second, and a wave with no row left stops
-      for (int i = 2; i < P.ndims; ++i) {
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) any |= pass[j];
-        if (!__ballot(any)) break;
-        const int d = P.order[i];
-        GlobalI64* col = global_view(S.fk[d]);
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) fk[j] = col[pass[j] ? rc[j] : 0];
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-          const bool hit = loff[d] >= 0 ? bf_find(s_filt + loff[d], lnb[d], s_masks, fk[j])
-                                        : bf_find(P.dim[d].fbf, lnb[d], s_masks, fk[j]);
-          pass[j] = pass[j] & hit;
-        }
+      while (qn >= 64) {
+        wave_lds_sync();
+        const uint32_t lr = stack[qn - 64 + lane];
+        qn -= 64;
+        rest(S, true, lr);
+        wave_lds_sync();   // the next pushes overwrite the popped entries
       }
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) append_survivor(P, S, pass[j], r0 + (int64_t)j * blockDim.x);
+    }
+    if (qn > 0) {   // the segment's last passing rows
+      wave_lds_sync();
+      const bool live = lane < qn;
+      const uint32_t lr = stack[live ? lane : 0];
+      qn = 0;
+      rest(S, live, lr);
+      wave_lds_sync();
     }
     }
   }
@@ -892,6 +919,8 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   *out = nullptr;
   FPDB_CHECK(q->num_dims >= 1 && q->num_dims <= kStarMaxDims, FPDB_ERR_INVALID, "star join: 1..8 dimensions");
   FPDB_CHECK(q->num_partitions >= 1 && q->partition_offsets, FPDB_ERR_INVALID, "star join: partitions needed");
+  FPDB_CHECK(q->fact_rows >= 0 && q->fact_rows < (int64_t(1) << 32), FPDB_ERR_INVALID,
+             "star join: at most 2^32 - 1 fact rows (rows within a segment are 32-bit)");
   FPDB_CHECK(q->partition_offsets[0] == 0 && q->partition_offsets[q->num_partitions] == q->fact_rows,
              FPDB_ERR_INVALID, "star join: partition offsets must cover the fact table");
   for (int p = 0; p < q->num_partitions; ++p)
