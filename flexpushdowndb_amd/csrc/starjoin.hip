@@ -30,6 +30,7 @@
 namespace fpdb {
 
 constexpr int kStarMaxDims = 8;
+constexpr int kStarStateWords = 64 + kStarMaxDims;   // the state words of a star join (see StarDev::state)
 constexpr uint32_t kStarEmptyRow = 0xFFFFFFFFu;
 constexpr uint64_t kStarEmptyKey = ~0ULL;
 
@@ -78,7 +79,9 @@ struct StarDev {
   int64_t* s_fk;                   // ndims x fact_rows
   int64_t* s_meas;
   // state words: [0] survivor count, [1..ndims] kept count per dim,
-  // [16..31] barrier, [32] error flag
+  // [16..31] barrier, [32] error flag (all reset by the init kernel);
+  // [64 + d] log2 of the backward filter blocks of dim d that may be set,
+  // kept across runs so that the next init kernel clears them
   unsigned long long* state;
   // group table
   uint64_t* g_keys;
@@ -88,6 +91,7 @@ struct StarDev {
   uint64_t g_mask;
   int32_t tile_rows;
   uint64_t barrier_ticks;          // grid barrier wait bound, device wall-clock ticks
+  int32_t bbf_cap_lnb;             // backward filter capacity, log2 blocks
 };
 
 __device__ __forceinline__ int log_blocks_for(int64_t n) {
@@ -193,8 +197,8 @@ __device__ __forceinline__ bool star_barrier(const StarDev& P) {
 
 // Hash-table mask of a dimension for this run: capacity the smallest power of
 // two >= its kept rows << ht_shift (at least 64), within the storage.
-__device__ __forceinline__ uint64_t star_ht_mask(const StarDimDev& D, const unsigned long long* state, int d) {
-  const uint64_t kept = __hip_atomic_load(state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Slots of a dimension's hash table in use for `kept` filtered rows.
+__device__ __forceinline__ uint64_t star_ht_mask(const StarDimDev& D, uint64_t kept) {
   uint64_t cap = 64;
   while (cap < (kept << D.ht_shift)) cap <<= 1;
   return cap - 1 < D.ht_mask ? cap - 1 : D.ht_mask;
@@ -330,16 +334,23 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     __syncthreads();
   }
   if (!star_barrier(P)) return;
+  // every dimension's kept-row count, read once: the later phases size the
+  // filters and hash tables from them (device-scope loads go to memory, and
+  // one per dimension per phase in sequence added up to microseconds)
+  __shared__ uint64_t s_kept_rows[kStarMaxDims];
+  if (threadIdx.x < P.ndims)
+    s_kept_rows[threadIdx.x] = __hip_atomic_load(P.state + 1 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
   // A2: insert the kept keys into the forward filters (the init kernel zeroed
   // them at capacity, so no barrier separates a clear from the inserts), and
   // clear the used part of the hash tables (first used in phase C)
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
-    int lnb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    int lnb = log_blocks_for((int64_t)s_kept_rows[d]);
     for (int64_t r = gtid; r < D.rows; r += gthreads)
       if (D.kept[r]) bf_insert(D.fbf, lnb, s_masks, D.key[r]);
     const uint4 empty = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-    const int64_t hm = (int64_t)star_ht_mask(D, P.state, d);
+    const int64_t hm = (int64_t)star_ht_mask(D, s_kept_rows[d]);
     for (int64_t s = gtid; s <= hm; s += gthreads) reinterpret_cast<uint4*>(D.ht)[s] = empty;
   }
   if (!star_barrier(P)) return;
@@ -357,7 +368,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
     int64_t used = 0;
     for (int i = 0; i < P.ndims; ++i) {
       int d = P.order[i];
-      const int lb = log_blocks_for((int64_t)__hip_atomic_load(P.state + 1 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      const int lb = log_blocks_for((int64_t)s_kept_rows[d]);
       int64_t words = int64_t(1) << lb;
       const bool fits = used + words <= kStarFilterLdsWords;
       if (threadIdx.x == 0) {
@@ -466,9 +477,8 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
   // B2: backward filters over the survivors
   const int64_t nsurv = (int64_t)__hip_atomic_load(P.state + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int blnb = log_blocks_for(nsurv);
-  for (int d = 0; d < P.ndims; ++d)
-    for (int64_t b = gtid; b < (int64_t(1) << blnb); b += gthreads) P.dim[d].bbf[b] = 0;
-  if (!star_barrier(P)) return;
+  // the init kernel cleared the filters; the next one clears what this run sets
+  if (gtid < P.ndims) P.state[64 + gtid] = (unsigned long long)blnb;
   for (int d = 0; d < P.ndims; ++d)
     for (int64_t i = gtid; i < nsurv; i += gthreads)
       bf_insert(P.dim[d].bbf, blnb, s_masks, P.s_fk[(int64_t)d * P.fact_rows + i]);
@@ -476,7 +486,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
   // C: kept dimension rows that pass the backward filter go into the hash tables
   for (int d = 0; d < P.ndims; ++d) {
     const StarDimDev& D = P.dim[d];
-    const uint64_t hm = star_ht_mask(D, P.state, d);
+    const uint64_t hm = star_ht_mask(D, s_kept_rows[d]);
     for (int64_t r = gtid; r < D.rows; r += gthreads) {
       if (!D.kept[r]) continue;
       int64_t k = D.key[r];
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
   if (!star_barrier(P)) return;
   // D: join the survivors with every dimension and aggregate
   __shared__ uint64_t hms[kStarMaxDims];
-  if (threadIdx.x < P.ndims) hms[threadIdx.x] = star_ht_mask(P.dim[threadIdx.x], P.state, threadIdx.x);
+  if (threadIdx.x < P.ndims) hms[threadIdx.x] = star_ht_mask(P.dim[threadIdx.x], s_kept_rows[threadIdx.x]);
   __syncthreads();
   for (int64_t i = gtid; i < nsurv; i += gthreads) {
     uint32_t g[2] = {0, 0};
@@ -507,6 +517,14 @@ __global__ void star_init_kernel(StarDev P) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gthreads = (int64_t)gridDim.x * blockDim.x;
   if (gtid < 64) P.state[gtid] = 0;
+  // the backward filters, as far as the run before may have set them (the
+  // persistent kernel inserts into them without clearing first)
+  for (int d = 0; d < P.ndims; ++d) {
+    uint64_t* f = P.dim[d].bbf;
+    const unsigned long long set = P.state[64 + d];
+    const int64_t words = int64_t(1) << (set < (unsigned long long)P.bbf_cap_lnb ? (int)set : P.bbf_cap_lnb);
+    for (int64_t b = gtid; b < words; b += gthreads) f[b] = 0;
+  }
   for (int64_t s = gtid; s <= (int64_t)P.g_mask + 1; s += gthreads) {
     P.g_keys[s] = kStarEmptyKey;
     P.g_cnt[s] = 0;
@@ -622,6 +640,8 @@ struct fpdb_star {
   int64_t kept[fpdb::kStarMaxDims] = {0};
   int fwd_lnb[fpdb::kStarMaxDims] = {0};
   int bwd_lnb = 0;
+  int bwd_cap_lnb = 0;                 // backward filter capacity (log2 blocks)
+  unsigned long long bwd_mark[fpdb::kStarMaxDims] = {};   // source of mark_backward_filters' copy
   int64_t survivors = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -664,6 +684,17 @@ static fpdb_column col_i32(const int32_t* p, int64_t n) {
 }
 
 // Per-partition launches of the operator entry points (the drop-in path).
+// Records every backward filter as set over its whole capacity, so that the
+// next persistent run's init kernel clears all of it: after preparation (the
+// allocations hold anything) and after a per-partition run (which uses the
+// filters too).
+static fpdb_status mark_backward_filters(fpdb_star* s) {
+  for (int d = 0; d < fpdb::kStarMaxDims; ++d) s->bwd_mark[d] = (unsigned long long)s->bwd_cap_lnb;
+  FPDB_HIP(hipMemcpyAsync(s->dev.state + 64, s->bwd_mark, sizeof(s->bwd_mark), hipMemcpyHostToDevice,
+                          s->ctx->stream));
+  return FPDB_OK;
+}
+
 static fpdb_status star_plain(fpdb_star* s) {
   Context* c = s->ctx;
   fpdb_context* ctx = reinterpret_cast<fpdb_context*>(c);
@@ -757,6 +788,7 @@ static fpdb_status star_plain(fpdb_star* s) {
   s->survivors = total;
   for (int d = 0; d < nd; ++d) {
     STAR_TRY(launch_fill(c, P.dim[d].bbf, 0, (uint64_t)8 << s->bwd_lnb));
+    STAR_TRY(mark_backward_filters(s));   // for the next persistent run
     for (int p = 0; p < np; ++p) {
       if (!surv_n[p]) continue;
       fpdb_column k = col_i64(surv[p][d], surv_n[p]);
@@ -1043,7 +1075,7 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8, reinterpret_cast<void**>(&P.s_idx));
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8 * q->num_dims, reinterpret_cast<void**>(&P.s_fk));
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)N * 8, reinterpret_cast<void**>(&P.s_meas));
-  if (st == FPDB_OK) st = star_alloc(s, 64 * 8, reinterpret_cast<void**>(&P.state));
+  if (st == FPDB_OK) st = star_alloc(s, kStarStateWords * 8, reinterpret_cast<void**>(&P.state));
   if (st == FPDB_OK) st = star_alloc(s, s->segs.size() * sizeof(StarSeg), reinterpret_cast<void**>(const_cast<StarSeg**>(&P.segs)));
   if (st == FPDB_OK) st = hipMemcpy(const_cast<StarSeg*>(P.segs), s->segs.data(), s->segs.size() * sizeof(StarSeg),
                                     hipMemcpyHostToDevice) == hipSuccess ? FPDB_OK : fail(FPDB_ERR_HIP, "star join: segment copy");
@@ -1060,6 +1092,9 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   if (st == FPDB_OK) st = star_alloc(s, (uint64_t)q->max_groups * 8, reinterpret_cast<void**>(&s->out_sum));
   if (st == FPDB_OK) st = star_alloc(s, 8, reinterpret_cast<void**>(&s->out_n));
   if (st == FPDB_OK) st = launch_fill(c, P.state, 0, 64 * 8);   // no error recorded before the first run
+  s->bwd_cap_lnb = blnb;
+  P.bbf_cap_lnb = blnb;
+  if (st == FPDB_OK) st = mark_backward_filters(s);   // the allocations hold anything
   if (st != FPDB_OK) {
     fpdb_star_destroy(s);
     return st;
