@@ -544,6 +544,7 @@ __global__ void star_init_kernel(StarDev P) {
 // slot); beyond kStarSortGroups groups each group counts the groups that came
 // first instead.
 constexpr int kStarSortGroups = 8192;
+constexpr int kStarEmitScan = 8;   // group slots a thread per trip in the emit's scan
 
 __device__ __forceinline__ void star_write_group(const StarDev& P, int64_t s, int64_t cap, int64_t rank,
                                                  int32_t* out_g0, int32_t* out_g1, int64_t* out_cnt,
@@ -567,15 +568,42 @@ __global__ __launch_bounds__(1024) void star_emit_kernel(StarDev P, int32_t* out
   const int64_t cap = (int64_t)P.g_mask + 2;
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
-  for (int64_t s = threadIdx.x; s < cap; s += blockDim.x) {
-    long long f = P.g_first[s];
-    if (f == LLONG_MAX) continue;
-    unsigned i = atomicAdd(&s_n, 1u);
-    if (i < kStarSortGroups) skey[i] = ((unsigned long long)f << 22) | (unsigned long long)s;
+  // the used slots, kStarEmitScan a thread per trip with their loads issued
+  // together; one LDS atomic per wave and trip item
+  const int lane = threadIdx.x & 63;
+  for (int64_t s0 = threadIdx.x; s0 < cap; s0 += (int64_t)blockDim.x * kStarEmitScan) {
+    long long f[kStarEmitScan];
+#pragma unroll
+    for (int j = 0; j < kStarEmitScan; ++j) {
+      const int64_t s = s0 + (int64_t)j * blockDim.x;
+      f[j] = s < cap ? P.g_first[s] : LLONG_MAX;
+    }
+#pragma unroll
+    for (int j = 0; j < kStarEmitScan; ++j) {
+      const bool used = f[j] != LLONG_MAX;
+      const uint64_t m = __ballot(used);
+      if (!m) continue;
+      unsigned base = 0;
+      if (lane == __builtin_ctzll(m)) base = atomicAdd(&s_n, (unsigned)__popcll(m));
+      base = __shfl(base, __builtin_ctzll(m));
+      const unsigned i = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      if (used && i < kStarSortGroups)
+        skey[i] = ((unsigned long long)f[j] << 22) | (unsigned long long)(s0 + (int64_t)j * blockDim.x);
+    }
   }
   __syncthreads();
   const unsigned n = s_n;
-  if (n > kStarSortGroups) {
+  if (n <= blockDim.x) {
+    // one group a thread: its rank is the number of smaller keys (keys are
+    // distinct: the slot is in the low bits); every thread reads the same
+    // entry at a time, an LDS broadcast
+    const unsigned long long mine = threadIdx.x < n ? skey[threadIdx.x] : 0ULL;
+    unsigned rank = 0;
+    for (unsigned j = 0; j < n; ++j) rank += skey[j] < mine ? 1u : 0u;
+    if (threadIdx.x < n)
+      star_write_group(P, (int64_t)(mine & ((1ULL << 22) - 1)), cap, rank, out_g0, out_g1, out_cnt, out_sum,
+                       max_groups);
+  } else if (n > kStarSortGroups) {
     for (int64_t s = threadIdx.x; s < cap; s += blockDim.x) {
       long long f = P.g_first[s];
       if (f == LLONG_MAX) continue;

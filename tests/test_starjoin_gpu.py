@@ -67,6 +67,22 @@ def test_star_join_plain_batched_graph(ctx, n_fact, n_parts):
         _same(sj.result(), exp)
 
 
+@pytest.mark.parametrize("domain0", [300, 60])
+def test_star_join_many_groups(ctx, domain0):
+    """Group counts on both sides of one workgroup's worth (1,024), where the
+    emit ranks groups by counting smaller first rows or by a bitonic sort."""
+    w = datagen.star_workload(n_fact=2_000_000, n_parts=16, seed=13, keep_below=(60, 50, 40, 80))
+    rng = np.random.default_rng(14)
+    w["dims"][0]["group"] = rng.integers(0, domain0, len(w["dims"][0]["key"])).astype(np.int32)
+    exp = datagen.star_expected(w)
+    assert (len(exp["count"]) > 1024) == (domain0 == 300)
+    sj = StarJoin(ctx, w)
+    sj.run_batched()
+    _same(sj.result(), exp)
+    sj.run_plain()
+    _same(sj.result(), exp)
+
+
 def test_star_join_empty_dimension_and_partitions(ctx):
     w = datagen.star_workload(n_fact=100_000, n_parts=16, seed=3, keep_below=(30, 0, 10, 40))
     w["offsets"] = np.array([0, 0, 50_000, 50_000, 100_000], np.int64)
