@@ -31,6 +31,10 @@ namespace fpdb {
 
 constexpr int kStarMaxDims = 8;
 constexpr int kStarStateWords = 64 + kStarMaxDims;   // the state words of a star join (see StarDev::state)
+constexpr int kStarInitThreads = 256;
+constexpr int kStarInitRows = 8;          // dimension rows a thread per trip in the init kernel
+constexpr int kStarInitMaxBlocks = 2048;  // init workgroups at most (its partial counts: 2 a persistent thread)
+static_assert(kStarInitMaxBlocks % 1024 == 0, "whole loads of partial counts per persistent thread");
 constexpr uint32_t kStarEmptyRow = 0xFFFFFFFFu;
 constexpr uint64_t kStarEmptyKey = ~0ULL;
 
@@ -92,6 +96,10 @@ struct StarDev {
   int32_t tile_rows;
   uint64_t barrier_ticks;          // grid barrier wait bound, device wall-clock ticks
   int32_t bbf_cap_lnb;             // backward filter capacity, log2 blocks
+  // kept rows per dimension per init workgroup ([kStarMaxDims][init_blocks]),
+  // written by the init kernel and summed by every persistent workgroup
+  unsigned long long* kept_part;
+  int32_t init_blocks;
 };
 
 __device__ __forceinline__ int log_blocks_for(int64_t n) {
@@ -313,34 +321,37 @@ __global__ __launch_bounds__(kStarThreads) void star_persistent_kernel(StarDev P
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gthreads = (int64_t)gridDim.x * blockDim.x;
 
-  // A1: filter the dimensions, count kept rows, prepare the hash tables
-  for (int d = 0; d < P.ndims; ++d) {
-    const StarDimDev& D = P.dim[d];
-    unsigned long long kept = 0;
-    for (int64_t r = gtid; r < D.rows; r += gthreads) {
-      bool k = star_cmp(D.attr[r], D.op, D.cst);
-      D.kept[r] = k ? 1 : 0;
-      kept += k ? 1 : 0;
-    }
-    // one device atomic per workgroup (same-address atomics serialize)
-    __shared__ unsigned long long s_kept;
-    if (threadIdx.x == 0) s_kept = 0;
-    __syncthreads();
-    for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
-    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_kept, kept);
-    __syncthreads();
-    if (threadIdx.x == 0 && s_kept)
-      __hip_atomic_fetch_add(P.state + 1 + d, s_kept, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-  }
-  if (!star_barrier(P)) return;
-  // every dimension's kept-row count, read once: the later phases size the
-  // filters and hash tables from them (device-scope loads go to memory, and
-  // one per dimension per phase in sequence added up to microseconds)
-  __shared__ uint64_t s_kept_rows[kStarMaxDims];
-  if (threadIdx.x < P.ndims)
-    s_kept_rows[threadIdx.x] = __hip_atomic_load(P.state + 1 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  // A1 ran in the init kernel (kept flags and per-workgroup counts: a launch
+  // boundary instead of a grid barrier).  Every workgroup sums the counts
+  // into LDS once: the later phases size the filters and hash tables from
+  // them (device-scope loads go to memory, and one per dimension per phase
+  // in sequence added up to microseconds).
+  __shared__ unsigned long long s_kept_rows[kStarMaxDims];
+  if (threadIdx.x < kStarMaxDims) s_kept_rows[threadIdx.x] = 0;
   __syncthreads();
+  {
+    const int G = P.init_blocks;
+    unsigned long long v[kStarMaxDims];
+#pragma unroll
+    for (int d = 0; d < kStarMaxDims; ++d) {
+      v[d] = 0;
+#pragma unroll
+      for (int q = 0; q < kStarInitMaxBlocks / kStarThreads; ++q) {
+        const int b = threadIdx.x + q * kStarThreads;
+        if (d < P.ndims && b < G) v[d] += P.kept_part[(int64_t)d * G + b];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < kStarMaxDims; ++d) {
+      if (d >= P.ndims) break;
+      unsigned long long x = v[d];
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+      if ((threadIdx.x & 63) == 0 && x) atomicAdd(&s_kept_rows[d], x);
+    }
+  }
+  __syncthreads();
+  // the counts also go where the filter inspection reads them
+  if (blockIdx.x == 0 && threadIdx.x < P.ndims) P.state[1 + threadIdx.x] = s_kept_rows[threadIdx.x];
   // A2: insert the kept keys into the forward filters (the init kernel zeroed
   // them at capacity, so no barrier separates a clear from the inserts), and
   // clear the used part of the hash tables (first used in phase C)
@@ -524,6 +535,36 @@ __global__ void star_init_kernel(StarDev P) {
     const unsigned long long set = P.state[64 + d];
     const int64_t words = int64_t(1) << (set < (unsigned long long)P.bbf_cap_lnb ? (int)set : P.bbf_cap_lnb);
     for (int64_t b = gtid; b < words; b += gthreads) f[b] = 0;
+  }
+  // A1: filter the dimensions: kept flags, and kept rows per dimension per
+  // workgroup (no device atomics: same-address atomics serialize)
+  __shared__ unsigned long long s_cnt;
+  for (int d = 0; d < P.ndims; ++d) {
+    const StarDimDev& D = P.dim[d];
+    unsigned long long kept = 0;
+    for (int64_t r0 = gtid; r0 < D.rows; r0 += gthreads * kStarInitRows) {
+      int32_t av[kStarInitRows];
+#pragma unroll
+      for (int j = 0; j < kStarInitRows; ++j) {
+        const int64_t r = r0 + (int64_t)j * gthreads;
+        av[j] = r < D.rows ? D.attr[r] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < kStarInitRows; ++j) {
+        const int64_t r = r0 + (int64_t)j * gthreads;
+        if (r >= D.rows) continue;
+        const bool k = star_cmp(av[j], D.op, D.cst);
+        D.kept[r] = k ? 1 : 0;
+        kept += k ? 1 : 0;
+      }
+    }
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) kept += __shfl_xor(kept, off);
+    if ((threadIdx.x & 63) == 0 && kept) atomicAdd(&s_cnt, kept);
+    __syncthreads();
+    if (threadIdx.x == 0) P.kept_part[(int64_t)d * gridDim.x + blockIdx.x] = s_cnt;
+    __syncthreads();   // s_cnt is reused for the next dimension
   }
   for (int64_t s = gtid; s <= (int64_t)P.g_mask + 1; s += gthreads) {
     P.g_keys[s] = kStarEmptyKey;
@@ -954,7 +995,7 @@ static fpdb_status star_batched_enqueue(fpdb_star* s) {
   // register use
   FPDB_CHECK(s->grid >= 1 && s->grid <= s->resident_per_cu * std::max(1, c->num_cus), FPDB_ERR_INTERNAL,
              "star join: persistent grid exceeds the resident workgroups");
-  hipLaunchKernelGGL(star_init_kernel, dim3(256), dim3(256), 0, c->stream, s->dev);
+  hipLaunchKernelGGL(star_init_kernel, dim3(s->dev.init_blocks), dim3(kStarInitThreads), 0, c->stream, s->dev);
   FPDB_HIP(hipGetLastError());
   {
     ProfScope ps(c, "star_persistent");
@@ -1122,6 +1163,15 @@ fpdb_status fpdb_star_prepare(fpdb_context* ctx, const fpdb_star_query* q, const
   if (st == FPDB_OK) st = launch_fill(c, P.state, 0, 64 * 8);   // no error recorded before the first run
   s->bwd_cap_lnb = blnb;
   P.bbf_cap_lnb = blnb;
+  {
+    // init workgroups: kStarInitRows rows a thread of the largest dimension
+    int64_t max_rows = 0;
+    for (int d = 0; d < q->num_dims; ++d) max_rows = std::max<int64_t>(max_rows, P.dim[d].rows);
+    const int64_t per = (int64_t)kStarInitThreads * kStarInitRows;
+    P.init_blocks = (int32_t)std::min<int64_t>(std::max<int64_t>((max_rows + per - 1) / per, 256), kStarInitMaxBlocks);
+  }
+  if (st == FPDB_OK)
+    st = star_alloc(s, (uint64_t)kStarMaxDims * P.init_blocks * 8, reinterpret_cast<void**>(&P.kept_part));
   if (st == FPDB_OK) st = mark_backward_filters(s);   // the allocations hold anything
   if (st != FPDB_OK) {
     fpdb_star_destroy(s);
